@@ -1,0 +1,112 @@
+"""ctypes binding of libhodlr3d.so (the C ABI declared in include/hodlr3d.h).
+
+The library is built in-tree by ``hodlr3d_amd/csrc/Makefile``; importing this
+module fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhodlr3d.so")
+
+H3D_OK = 0
+H3D_ERR_INVALID = 1
+H3D_ERR_OOM = 2
+H3D_ERR_NONFINITE = 3
+H3D_ERR_HIP = 4
+H3D_ERR_STATE = 5
+H3D_ERR_NOT_CONVERGED = 6
+H3D_ERR_INTERNAL = 7
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_BF16 = 2
+HOST = 0
+DEVICE = 1
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [("qchoice", _i32), ("scale", _d), ("diag", _d), ("diag_vec", _p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("center", _d * 3), ("nlevels", _i32), ("workspace_bytes", _i64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("N", _i64), ("nlevels", _i32), ("max_rank", _i32), ("n_blocks", _i64),
+                ("rank_sum", _i64), ("factor_elems", _i64), ("find_memory", _d),
+                ("near_pairs", _i64), ("t_tree_ms", _d), ("t_assemble_ms", _d),
+                ("factor_bytes", _i64 * 3)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("outer_iterations", _i32), ("inner_iterations", _i32), ("rel_residual", _d),
+                ("history", _d * 64)]
+
+
+# name -> (restype, argtypes); the list is also the export contract tested on CPU
+SIGNATURES = {
+    "h3d_nlevels": (_i32, [_i64, _i32]),
+    "h3d_create": (_i32, [_p, _i64, _i32, _d, _i32, ctypes.POINTER(Kernel),
+                          ctypes.POINTER(Options), ctypes.POINTER(_p)]),
+    "h3d_assemble": (_i32, [_p]),
+    "h3d_prepare_precision": (_i32, [_p, _i32]),
+    "h3d_release_precision": (_i32, [_p, _i32]),
+    "h3d_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
+    "h3d_matvec_transpose": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
+    "h3d_set_diag": (_i32, [_p, _p, _i32]),
+    "h3d_gmres": (_i32, [_p, _p, _p, _i32, _d, ctypes.POINTER(_d), ctypes.POINTER(_i32), _p,
+                         _i32]),
+    "h3d_solve_ir": (_i32, [_p, _p, _p, _i32, _i32, _d, _i32, _d, _i32, _i32,
+                            ctypes.POINTER(SolveInfo), _i32]),
+    "h3d_export_perm": (_i32, [_p, _p]),
+    "h3d_export_boxes": (_i32, [_p, _i32, _p, _p]),
+    "h3d_export_neighbors": (_i32, [_p, _i32, _p]),
+    "h3d_il_size": (_i64, [_p, _i32]),
+    "h3d_export_il": (_i32, [_p, _i32, _p, _p]),
+    "h3d_export_ranks": (_i32, [_p, _i32, _p, _p]),
+    "h3d_export_block": (_i32, [_p, _i32, _i64, _p, _p, _p, _p]),
+    "h3d_get_stats": (_i32, [_p, ctypes.POINTER(Stats)]),
+    "h3d_set_stream": (_i32, [_p, _p]),
+    "h3d_synchronize": (_i32, [_p]),
+    "h3d_destroy": (None, [_p]),
+    "h3d_status_string": (ctypes.c_char_p, [_i32]),
+    "h3d_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class H3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hodlr3d: {msg} (status {code})")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `make -C hodlr3d_amd/csrc` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != H3D_OK:
+        L = lib()
+        raise H3DError(code, (L.h3d_last_error() or b"").decode() or
+                       L.h3d_status_string(code).decode())

@@ -1,0 +1,487 @@
+// The C ABI (include/hodlr3d.h): argument checks, handle lifetime, host/device
+// staging and error translation.  Every entry point returns a status code;
+// exceptions never cross the boundary.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+struct h3d_handle {
+  h3d::Engine e;
+};
+
+namespace h3d {
+
+Engine::~Engine() { free_all(); }
+
+void Engine::free_all() {
+  dfree(d_px);
+  dfree(d_py);
+  dfree(d_pz);
+  dfree(d_lx);
+  dfree(d_ly);
+  dfree(d_lz);
+  dfree(d_perm);
+  dfree(d_iperm);
+  dfree(d_diag);
+  dfree(d_leafc);
+  dfree(d_bstart);
+  dfree(d_bcount);
+  for (Level& lv : lev) {
+    dfree(lv.d_iloff);
+    dfree(lv.d_ilidx);
+    dfree(lv.d_nbr);
+    dfree(lv.d_pivI);
+    dfree(lv.d_pivJ);
+  }
+  lev.clear();
+  for (Factors& f : fac) {
+    for (int j = 0; j < 8; ++j) {
+      if (f.U[j]) (void)hipFree(f.U[j]);
+      if (f.V[j]) (void)hipFree(f.V[j]);
+      f.U[j] = f.V[j] = nullptr;
+    }
+    f.bytes = 0;
+  }
+  dfree(d_tdst);
+  dfree(d_p1);
+  dfree(d_p1r);
+  dfree(d_p2);
+  dfree(d_near);
+  dfree(d_tp1x);
+  dfree(d_tp1r);
+  dfree(d_tp2);
+  dfree(d_xp);
+  dfree(d_xf);
+  dfree(d_t);
+  dfree(d_part);
+  dfree(d_ylev);
+  dfree(d_io);
+  if (own_stream) (void)hipStreamDestroy(own_stream);
+  own_stream = nullptr;
+}
+
+// Gauss-Legendre rule on [-1,1] by Newton's iteration on P_n from the
+// Chebyshev-like initial guesses (textbook; see Abramowitz & Stegun 25.4.29).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0);
+  w.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    long double z = std::cos(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+    long double dp = 0;
+    for (int it = 0; it < 100; ++it) {
+      long double p0 = 1, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1);
+      long double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs((double)dz) < 1e-20) break;
+    }
+    x[i] = -(double)z;
+    w[i] = (double)(2 / ((1 - z * z) * dp * dp));
+  }
+}
+
+// IE_CUBE_3D self term (include/kernel.hpp:45-60, src/Integral3D.cpp:9-43):
+// 1 + 8 * integral over [0,h/2]^3 of 1/(4 pi r), h = N^(-1/3), 16^3 points.
+static double ie_cube_kii(int64_t N) {
+  std::vector<double> x, w;
+  gauss_legendre(16, x, w);
+  const double h = 1.0 / std::cbrt(double(N));
+  const double c = 0.5 * (h * 0.5), l = 0.5 * (h * 0.5);
+  double acc = 0.0;
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j)
+      for (int k = 0; k < 16; ++k) {
+        double tx = c * x[i] + l, ty = c * x[j] + l, tz = c * x[k] + l;
+        acc += w[i] * w[j] * w[k] / (4 * 3.1415926535897932384 * std::sqrt(tx * tx + ty * ty + tz * tz));
+      }
+  return acc * (c * c * c) * 8 + 1;
+}
+
+}  // namespace h3d
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return H3D_OK;
+  } catch (const h3d::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return H3D_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return H3D_ERR_INTERNAL;
+  }
+}
+
+int nlevels_formula(int64_t N, int leaf1d) {
+  // src/HODLR3D.cpp:6-7
+  double cubeRootN = std::pow((double)N, 1.0 / 3.0);
+  return (int)std::ceil(3 * std::log(double(cubeRootN) / leaf1d) / std::log(8));
+}
+
+void check_level(h3d_handle* h, int level) {
+  if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+  if (level < 1 || level > h->e.nL) throw h3d::Error(H3D_ERR_INVALID, "level out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int h3d_nlevels(int64_t N, int leaf1d) {
+  if (N < 1 || leaf1d < 1) return -1;
+  return nlevels_formula(N, leaf1d);
+}
+
+int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
+               const h3d_kernel* kernel, const h3d_options* opt, h3d_handle** out) {
+  if (out) *out = nullptr;
+  return guarded([&] {
+    if (!xyz || !out) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (N < 1 || N > INT32_MAX) throw h3d::Error(H3D_ERR_INVALID, "N out of range");
+    if (leaf1d < 1) throw h3d::Error(H3D_ERR_INVALID, "leaf1d must be >= 1");
+    if (!(L > 0) || !std::isfinite(L)) throw h3d::Error(H3D_ERR_INVALID, "L must be positive");
+    for (int64_t i = 0; i < 3 * N; ++i)
+      if (!std::isfinite(xyz[i])) throw h3d::Error(H3D_ERR_NONFINITE, "non-finite coordinate");
+    int nL = (opt && opt->nlevels >= 0) ? opt->nlevels : nlevels_formula(N, leaf1d);
+    if (nL < 1 || nL > 7)
+      throw h3d::Error(H3D_ERR_INVALID, "nLevels must be in [1,7] (got " + std::to_string(nL) + ")");
+    h3d_kernel k{7, 1.0, 0.0, nullptr};
+    if (kernel) k = *kernel;
+    if (k.qchoice < 0 || k.qchoice > 16) throw h3d::Error(H3D_ERR_INVALID, "qchoice out of range");
+    auto* h = new h3d_handle();
+    try {
+      h3d::Engine& e = h->e;
+      e.N = N;
+      e.nL = nL;
+      e.leaf1d = leaf1d;
+      e.L = L;
+      if (opt) {
+        e.ctr[0] = opt->center[0];
+        e.ctr[1] = opt->center[1];
+        e.ctr[2] = opt->center[2];
+        e.ws_budget = opt->workspace_bytes;
+      }
+      e.tol_pow = tol_pow;
+      e.tol = std::pow(10, -1.0 * tol_pow);  // src/ACA.cpp:5
+      e.kern.q = k.qchoice;
+      e.kern.scale = k.scale;
+      e.kern.diag = k.diag;
+      if (k.qchoice == 16 && k.scale == 0.0 && k.diag == 0.0) {
+        e.kern.scale = 1.0 / double(N);  // include/kernel.hpp:59
+        e.kern.diag = h3d::ie_cube_kii(N);
+      }
+      H3D_HIP(hipStreamCreateWithFlags(&e.own_stream, hipStreamNonBlocking));
+      e.stream = e.own_stream;
+      if (k.diag_vec) {
+        e.d_diag = h3d::dalloc<double>(N);
+        H3D_HIP(hipMemcpyAsync(e.d_diag, k.diag_vec, sizeof(double) * N, hipMemcpyHostToDevice,
+                               e.stream));
+      }
+      h3d::build_tree(e, xyz);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int h3d_assemble(h3d_handle* h) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h3d::Engine& e = h->e;
+    if (e.assembled) throw h3d::Error(H3D_ERR_STATE, "already assembled");
+    h3d::assemble_all(e);
+    h3d::build_plan(e);
+    h3d::ensure_scratch(e, 1);
+  });
+}
+
+int h3d_prepare_precision(h3d_handle* h, int prec) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (prec < 0 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    h3d::prepare_precision(h->e, prec);
+  });
+}
+
+int h3d_release_precision(h3d_handle* h, int prec) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (prec < 1 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "only fp32/bf16 copies can be released");
+    h3d::Factors& f = h->e.fac[prec];
+    H3D_HIP(hipStreamSynchronize(h->e.stream));
+    for (int j = 0; j < 8; ++j) {
+      if (f.U[j]) (void)hipFree(f.U[j]);
+      if (f.V[j]) (void)hipFree(f.V[j]);
+      f.U[j] = f.V[j] = nullptr;
+    }
+    f.bytes = 0;
+  });
+}
+
+static int do_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int prec, int where,
+                     bool transpose) {
+  return guarded([&] {
+    if (!h || !b || !y) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (nrhs < 0) throw h3d::Error(H3D_ERR_INVALID, "nrhs < 0");
+    if (where != H3D_HOST && where != H3D_DEVICE) throw h3d::Error(H3D_ERR_INVALID, "bad where");
+    h3d::Engine& e = h->e;
+    if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "matvec before assemble");
+    if (nrhs == 0) return;
+    const int64_t N = e.N;
+    if (where == H3D_DEVICE) {
+      h3d::matvec_device(e, b, y, nrhs, prec, transpose);
+      return;
+    }
+    h3d::ensure_scratch(e, 1);
+    for (int r = 0; r < nrhs; ++r) {
+      H3D_HIP(hipMemcpyAsync(e.d_io, b + (int64_t)r * N, sizeof(double) * N, hipMemcpyHostToDevice,
+                             e.stream));
+      h3d::matvec_device(e, e.d_io, e.d_io + N, 1, prec, transpose);
+      H3D_HIP(hipMemcpyAsync(y + (int64_t)r * N, e.d_io + N, sizeof(double) * N,
+                             hipMemcpyDeviceToHost, e.stream));
+    }
+    H3D_HIP(hipStreamSynchronize(e.stream));
+  });
+}
+
+int h3d_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int prec, int where) {
+  return do_matvec(h, b, y, nrhs, prec, where, false);
+}
+
+int h3d_matvec_transpose(h3d_handle* h, const double* b, double* y, int nrhs, int prec,
+                         int where) {
+  return do_matvec(h, b, y, nrhs, prec, where, true);
+}
+
+int h3d_set_diag(h3d_handle* h, const double* diag_vec, int where) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h3d::Engine& e = h->e;
+    if (!diag_vec) {
+      h3d::dfree(e.d_diag);
+      return;
+    }
+    if (!e.d_diag) e.d_diag = h3d::dalloc<double>(e.N);
+    H3D_HIP(hipMemcpyAsync(e.d_diag, diag_vec, sizeof(double) * e.N,
+                           where == H3D_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           e.stream));
+    H3D_HIP(hipStreamSynchronize(e.stream));
+  });
+}
+
+int h3d_gmres(h3d_handle* h, const double* b, double* x, int maxit, double threshold,
+              double* error, int* iterations, double* hist, int where) {
+  return guarded([&] {
+    if (!h || !b || !x || !error || !iterations || !hist)
+      throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (maxit < 1) throw h3d::Error(H3D_ERR_INVALID, "maxit must be >= 1");
+    h3d::Engine& e = h->e;
+    if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "gmres before assemble");
+    const int64_t N = e.N;
+    if (where == H3D_DEVICE) {
+      h3d::gmres_reference(e, b, x, maxit, threshold, error, iterations, hist);
+      return;
+    }
+    double* db = h3d::dalloc<double>(2 * N);
+    try {
+      H3D_HIP(hipMemcpyAsync(db, b, sizeof(double) * N, hipMemcpyHostToDevice, e.stream));
+      h3d::gmres_reference(e, db, db + N, maxit, threshold, error, iterations, hist);
+      H3D_HIP(hipMemcpyAsync(x, db + N, sizeof(double) * N, hipMemcpyDeviceToHost, e.stream));
+      H3D_HIP(hipStreamSynchronize(e.stream));
+    } catch (...) {
+      h3d::dfree(db);
+      throw;
+    }
+    h3d::dfree(db);
+  });
+}
+
+int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max_inner,
+                 double inner_tol, int inner_prec, double tol, int max_outer, int transpose,
+                 h3d_solve_info* info, int where) {
+  int st = H3D_OK;
+  int rc = guarded([&] {
+    if (!h || !b || !x) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (inner_prec < 0 || inner_prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    h3d::Engine& e = h->e;
+    if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "solve before assemble");
+    const int64_t N = e.N;
+    if (where == H3D_DEVICE) {
+      st = h3d::solve_ir(e, b, x, restart, max_inner, inner_tol, inner_prec, tol, max_outer,
+                         transpose != 0, info);
+      return;
+    }
+    double* db = h3d::dalloc<double>(2 * N);
+    try {
+      H3D_HIP(hipMemcpyAsync(db, b, sizeof(double) * N, hipMemcpyHostToDevice, e.stream));
+      st = h3d::solve_ir(e, db, db + N, restart, max_inner, inner_tol, inner_prec, tol, max_outer,
+                         transpose != 0, info);
+      H3D_HIP(hipMemcpyAsync(x, db + N, sizeof(double) * N, hipMemcpyDeviceToHost, e.stream));
+      H3D_HIP(hipStreamSynchronize(e.stream));
+    } catch (...) {
+      h3d::dfree(db);
+      throw;
+    }
+    h3d::dfree(db);
+  });
+  if (rc == H3D_OK && st != H3D_OK) g_last_error = "refinement did not reach the tolerance";
+  return rc != H3D_OK ? rc : st;
+}
+
+int h3d_export_perm(h3d_handle* h, int32_t* perm) {
+  return guarded([&] {
+    if (!h || !perm) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    std::memcpy(perm, h->e.perm.data(), sizeof(int32_t) * h->e.N);
+  });
+}
+
+int h3d_export_boxes(h3d_handle* h, int level, int32_t* start, int32_t* count) {
+  return guarded([&] {
+    if (!h || !start || !count) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (level < 0 || level > h->e.nL) throw h3d::Error(H3D_ERR_INVALID, "level out of range");
+    const int64_t lo = h3d::level_offset(level), nb = int64_t(1) << (3 * level);
+    std::memcpy(start, h->e.bstart.data() + lo, sizeof(int32_t) * nb);
+    std::memcpy(count, h->e.bcount.data() + lo, sizeof(int32_t) * nb);
+  });
+}
+
+int h3d_export_neighbors(h3d_handle* h, int level, int32_t* nbr26) {
+  return guarded([&] {
+    check_level(h, level);
+    if (!nbr26) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    const h3d::Level& lv = h->e.lev[level];
+    H3D_HIP(hipMemcpy(nbr26, lv.d_nbr, sizeof(int32_t) * lv.nboxes * 26, hipMemcpyDeviceToHost));
+  });
+}
+
+int64_t h3d_il_size(h3d_handle* h, int level) {
+  if (!h || level < 1 || level > h->e.nL) return -1;
+  return (int64_t)h->e.lev[level].ilidx.size();
+}
+
+int h3d_export_il(h3d_handle* h, int level, int32_t* offsets, int32_t* boxes) {
+  return guarded([&] {
+    check_level(h, level);
+    if (!offsets || !boxes) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    const h3d::Level& lv = h->e.lev[level];
+    std::memcpy(offsets, lv.iloff.data(), sizeof(int32_t) * lv.iloff.size());
+    std::memcpy(boxes, lv.ilidx.data(), sizeof(int32_t) * lv.ilidx.size());
+  });
+}
+
+int h3d_export_ranks(h3d_handle* h, int level, int32_t* rank, int32_t* status) {
+  return guarded([&] {
+    check_level(h, level);
+    if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    const h3d::Level& lv = h->e.lev[level];
+    if (rank) std::memcpy(rank, lv.rank.data(), sizeof(int32_t) * lv.rank.size());
+    if (status) std::memcpy(status, lv.status.data(), sizeof(int32_t) * lv.status.size());
+  });
+}
+
+int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J, double* U,
+                     double* V) {
+  return guarded([&] {
+    check_level(h, level);
+    h3d::Engine& e = h->e;
+    if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    const h3d::Level& lv = e.lev[level];
+    if (b < 0 || b >= (int64_t)lv.rank.size()) throw h3d::Error(H3D_ERR_INVALID, "block out of range");
+    const int r = lv.rank[b];
+    if (r == 0) return;
+    if (I) H3D_HIP(hipMemcpy(I, lv.d_pivI + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
+    if (J) H3D_HIP(hipMemcpy(J, lv.d_pivJ + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
+    // locate the target box of block b
+    int64_t k = std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) - lv.iloff.begin() - 1;
+    const int64_t lo = h3d::level_offset(level);
+    const int32_t ki = lv.ilidx[b];
+    const int n = e.bcount[lo + k], m = e.bcount[lo + ki];
+    if (U) {
+      const double* src = (const double*)e.fac[0].U[level] + lv.uoff[k] + (int64_t)lv.bcol_t[b] * n;
+      H3D_HIP(hipMemcpy(U, src, sizeof(double) * (int64_t)r * n, hipMemcpyDeviceToHost));
+    }
+    if (V) {
+      std::vector<double> rows((size_t)m * lv.Rs[ki]);
+      const double* src = (const double*)e.fac[0].V[level] + lv.voff[ki];
+      H3D_HIP(hipMemcpy(rows.data(), src, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+      for (int l = 0; l < r; ++l)
+        for (int s = 0; s < m; ++s) V[(int64_t)l * m + s] = rows[(size_t)s * lv.Rs[ki] + lv.bcol_s[b] + l];
+    }
+  });
+}
+
+int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
+  return guarded([&] {
+    if (!h || !s) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    const h3d::Engine& e = h->e;
+    std::memset(s, 0, sizeof(*s));
+    s->N = e.N;
+    s->nlevels = e.nL;
+    s->max_rank = e.max_rank;
+    s->n_blocks = e.n_blocks;
+    s->rank_sum = e.rank_sum;
+    s->factor_elems = e.factor_elems;
+    s->find_memory = e.find_memory;
+    s->near_pairs = e.near_pairs;
+    s->t_tree_ms = e.t_tree_ms;
+    s->t_assemble_ms = e.t_assemble_ms;
+    for (int p = 0; p < 3; ++p) s->factor_bytes[p] = e.fac[p].bytes;
+  });
+}
+
+int h3d_set_stream(h3d_handle* h, void* stream) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h->e.stream = stream ? (hipStream_t)stream : h->e.own_stream;
+  });
+}
+
+int h3d_synchronize(h3d_handle* h) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    H3D_HIP(hipStreamSynchronize(h->e.stream));
+  });
+}
+
+void h3d_destroy(h3d_handle* h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->e.stream);
+  delete h;
+}
+
+const char* h3d_status_string(int status) {
+  switch (status) {
+    case H3D_OK: return "ok";
+    case H3D_ERR_INVALID: return "invalid argument";
+    case H3D_ERR_OOM: return "out of memory";
+    case H3D_ERR_NONFINITE: return "non-finite kernel entry";
+    case H3D_ERR_HIP: return "HIP runtime error";
+    case H3D_ERR_STATE: return "call out of order";
+    case H3D_ERR_NOT_CONVERGED: return "not converged";
+    case H3D_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char* h3d_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+"""Host-side mirror of the reference's facade, kernel plugin and GMRES.
+
+Names and argument meaning follow SAFRAN-LAB/HODLR3D:
+  * ``HODLR3D(nParticlesInLeafAlong1D, L, TOL_POW, Qchoice, particles)``,
+    ``assemble()``, ``MatVecProduct(b)``  -- include/HODLR3D.hpp:12-22,
+    src/HODLR3D.cpp:3-27;
+  * ``userkernel`` (Qchoice catalogue + diagonal) -- include/kernel.hpp:37-115,
+    examples/testHODLR3Dsolve.cpp:32-76;
+  * ``classGMRES().gmres(H, b, maxIterations, threshold)`` -- include/gmres.hpp:13-27,
+    src/gmres.cpp:40-94, returning what the reference writes to its reference
+    arguments: ``(x, error, noOfIterations, e)``;
+  * ``findMemory()`` / ``getMaxRank()`` / ``assTime`` -- src/HODLR3DTree.cpp:1757-1818.
+Every call goes through libhodlr3d.so (the C ABI of include/hodlr3d.h) and runs
+on the GPU; invalid input raises ``H3DError`` instead of undefined behaviour.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native as nat
+
+PRECISIONS = {"fp64": nat.PREC_FP64, "fp32": nat.PREC_FP32, "bf16": nat.PREC_BF16}
+
+
+def _prec(p):
+    if isinstance(p, str):
+        return PRECISIONS[p]
+    return int(p)
+
+
+def nlevels(N, leaf1d):
+    """nLevels = ceil(3 log(N^(1/3)/leaf1d)/log 8) (src/HODLR3D.cpp:6-7)."""
+    return nat.lib().h3d_nlevels(int(N), int(leaf1d))
+
+
+class pts3D:
+    """A point (include/definitions.hpp:7-9)."""
+    __slots__ = ("x", "y", "z")
+
+    def __init__(self, x=0.0, y=0.0, z=0.0):
+        self.x, self.y, self.z = float(x), float(y), float(z)
+
+
+def as_xyz(particles):
+    if isinstance(particles, np.ndarray):
+        a = np.ascontiguousarray(particles, dtype=np.float64)
+    elif hasattr(particles, "detach"):
+        a = np.ascontiguousarray(particles.detach().cpu().numpy(), dtype=np.float64)
+    else:
+        a = np.array([[p.x, p.y, p.z] for p in particles], dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError("particles must be N x 3")
+    return a
+
+
+class userkernel:
+    """Kernel plugin: entry(i,j) = scale * phi_Qchoice(r_ij) for i != j, diag for i == j.
+
+    ``scale=1, diag=0`` is the reference for Qchoice 0..15.  Qchoice 16
+    (IE_CUBE_3D) with ``scale=diag=0`` uses h3 = 1/N and Kii = 1 + the
+    self-cell integral, as include/kernel.hpp:45-60 does.  ``diag_vec`` sets a
+    per-point diagonal (second-kind operators diag(c) + scale*K).
+    """
+
+    def __init__(self, Qchoice=7, scale=None, diag=0.0, diag_vec=None):
+        self.Qchoice = int(Qchoice)
+        if scale is None:
+            scale = 0.0 if self.Qchoice == 16 else 1.0
+        self.scale = float(scale)
+        self.diag = float(diag)
+        self.diag_vec = None if diag_vec is None else np.ascontiguousarray(diag_vec, np.float64)
+
+
+def _is_torch_cuda(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+class HODLR3D:
+    """The reference facade (src/HODLR3D.cpp:3-27) on the MI355X engine."""
+
+    def __init__(self, nParticlesInLeafAlong1D, L, TOL_POW, Qchoice=7, particles=None, *,
+                 kernel=None, center=None, nlevels=None, workspace_bytes=0):
+        if particles is None:
+            raise ValueError("particles are required")
+        xyz = as_xyz(particles)
+        self.N = len(xyz)
+        self.kernel = kernel if kernel is not None else userkernel(Qchoice)
+        k = nat.Kernel(self.kernel.Qchoice, self.kernel.scale, self.kernel.diag,
+                       None if self.kernel.diag_vec is None else
+                       self.kernel.diag_vec.ctypes.data_as(ctypes.c_void_p))
+        opt = nat.Options()
+        c = center if center is not None else (0.0, 0.0, 0.0)
+        for i in range(3):
+            opt.center[i] = float(c[i])
+        opt.nlevels = -1 if nlevels is None else int(nlevels)
+        opt.workspace_bytes = int(workspace_bytes)
+        h = ctypes.c_void_p()
+        self._h = None
+        nat.check(nat.lib().h3d_create(xyz.ctypes.data_as(ctypes.c_void_p), self.N,
+                                       int(nParticlesInLeafAlong1D), float(L), int(TOL_POW),
+                                       ctypes.byref(k), ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self.nParticlesInLeafAlong1D = int(nParticlesInLeafAlong1D)
+        self.L = float(L)
+        self.TOL_POW = int(TOL_POW)
+        self.nLevels = self.stats()["nlevels"]
+        self.assembled = False
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            nat.lib().h3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference API
+    def assemble(self):
+        nat.check(nat.lib().h3d_assemble(self._h))
+        self.assembled = True
+
+    def MatVecProduct(self, b, prec="fp64"):
+        """Returns A b (original ordering), like HODLR3D::MatVecProduct."""
+        return self.matvec(b, prec=prec)
+
+    def findMemory(self):
+        return self.stats()["find_memory"]
+
+    def getMaxRank(self):
+        return self.stats()["max_rank"]
+
+    @property
+    def assTime(self):
+        return self.stats()["t_assemble_ms"] * 1e-3
+
+    # -- engine API
+    def _bind_stream(self):
+        import torch
+        nat.check(nat.lib().h3d_set_stream(self._h,
+                                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def matvec(self, b, prec="fp64", transpose=False):
+        """y = A b (or A^T b).  b: numpy (host) or torch CUDA tensor, shape (N,) or (N, nrhs)."""
+        fn = nat.lib().h3d_matvec_transpose if transpose else nat.lib().h3d_matvec
+        if _is_torch_cuda(b):
+            import torch
+            bt = b.detach().to(torch.float64)
+            cols = bt.reshape(self.N, -1).t().contiguous()  # nrhs x N, column-major RHS
+            out = torch.empty_like(cols)
+            self._bind_stream()
+            try:
+                nat.check(fn(self._h, ctypes.c_void_p(cols.data_ptr()),
+                             ctypes.c_void_p(out.data_ptr()), cols.shape[0], _prec(prec), nat.DEVICE))
+            finally:
+                nat.lib().h3d_set_stream(self._h, None)
+            return out.t().reshape(b.shape)
+        a = np.asarray(b, dtype=np.float64)
+        if a.shape[0] != self.N:
+            raise ValueError("b has the wrong length")
+        cols = np.ascontiguousarray(a.reshape(self.N, -1).T)
+        out = np.empty_like(cols)
+        nat.check(fn(self._h, cols.ctypes.data_as(ctypes.c_void_p),
+                     out.ctypes.data_as(ctypes.c_void_p), cols.shape[0], _prec(prec), nat.HOST))
+        return out.T.reshape(a.shape)
+
+    def matvec_transpose(self, b, prec="fp64"):
+        return self.matvec(b, prec=prec, transpose=True)
+
+    def prepare_precision(self, prec):
+        nat.check(nat.lib().h3d_prepare_precision(self._h, _prec(prec)))
+
+    def release_precision(self, prec):
+        nat.check(nat.lib().h3d_release_precision(self._h, _prec(prec)))
+
+    def set_diag(self, c):
+        if _is_torch_cuda(c):
+            import torch
+            ct = c.detach().to(torch.float64).contiguous()
+            self._bind_stream()
+            try:
+                nat.check(nat.lib().h3d_set_diag(self._h, ctypes.c_void_p(ct.data_ptr()), nat.DEVICE))
+            finally:
+                nat.lib().h3d_set_stream(self._h, None)
+            return
+        if c is None:
+            nat.check(nat.lib().h3d_set_diag(self._h, None, nat.HOST))
+            return
+        a = np.ascontiguousarray(c, dtype=np.float64)
+        nat.check(nat.lib().h3d_set_diag(self._h, a.ctypes.data_as(ctypes.c_void_p), nat.HOST))
+
+    def solve(self, b, tol=1e-10, restart=30, max_inner=300, inner_tol=1e-5, inner_prec="fp32",
+              max_outer=20, transpose=False):
+        """GMRES(restart) in `inner_prec` inside fp64 iterative refinement.
+
+        Returns (x, info dict).  Raises on non-convergence."""
+        info = nat.SolveInfo()
+        if _is_torch_cuda(b):
+            import torch
+            bt = b.detach().to(torch.float64).contiguous()
+            x = torch.empty_like(bt)
+            self._bind_stream()
+            try:
+                rc = nat.lib().h3d_solve_ir(self._h, ctypes.c_void_p(bt.data_ptr()),
+                                            ctypes.c_void_p(x.data_ptr()), restart, max_inner,
+                                            inner_tol, _prec(inner_prec), tol, max_outer,
+                                            int(transpose), ctypes.byref(info), nat.DEVICE)
+            finally:
+                nat.lib().h3d_set_stream(self._h, None)
+        else:
+            a = np.ascontiguousarray(b, dtype=np.float64)
+            x = np.empty_like(a)
+            rc = nat.lib().h3d_solve_ir(self._h, a.ctypes.data_as(ctypes.c_void_p),
+                                        x.ctypes.data_as(ctypes.c_void_p), restart, max_inner,
+                                        inner_tol, _prec(inner_prec), tol, max_outer,
+                                        int(transpose), ctypes.byref(info), nat.HOST)
+        d = {"outer_iterations": info.outer_iterations, "inner_iterations": info.inner_iterations,
+             "rel_residual": info.rel_residual,
+             "history": [info.history[i] for i in range(min(info.outer_iterations, 64))]}
+        nat.check(rc)
+        return x, d
+
+    def stats(self):
+        s = nat.Stats()
+        nat.check(nat.lib().h3d_get_stats(self._h, ctypes.byref(s)))
+        return {"N": s.N, "nlevels": s.nlevels, "max_rank": s.max_rank, "n_blocks": s.n_blocks,
+                "rank_sum": s.rank_sum, "factor_elems": s.factor_elems,
+                "find_memory": s.find_memory, "near_pairs": s.near_pairs,
+                "t_tree_ms": s.t_tree_ms, "t_assemble_ms": s.t_assemble_ms,
+                "factor_bytes": list(s.factor_bytes)}
+
+    # -- structure export (bit-exact checks)
+    def perm(self):
+        p = np.zeros(self.N, dtype=np.int32)
+        nat.check(nat.lib().h3d_export_perm(self._h, p.ctypes.data_as(ctypes.c_void_p)))
+        return p
+
+    def boxes(self, level):
+        nb = 8 ** level
+        s = np.zeros(nb, dtype=np.int32)
+        c = np.zeros(nb, dtype=np.int32)
+        nat.check(nat.lib().h3d_export_boxes(self._h, level, s.ctypes.data_as(ctypes.c_void_p),
+                                             c.ctypes.data_as(ctypes.c_void_p)))
+        return s, c
+
+    def neighbors(self, level):
+        n = np.zeros((8 ** level, 26), dtype=np.int32)
+        nat.check(nat.lib().h3d_export_neighbors(self._h, level, n.ctypes.data_as(ctypes.c_void_p)))
+        return n
+
+    def il(self, level):
+        sz = nat.lib().h3d_il_size(self._h, level)
+        if sz < 0:
+            raise ValueError("level out of range")
+        off = np.zeros(8 ** level + 1, dtype=np.int32)
+        idx = np.zeros(max(sz, 1), dtype=np.int32)
+        nat.check(nat.lib().h3d_export_il(self._h, level, off.ctypes.data_as(ctypes.c_void_p),
+                                          idx.ctypes.data_as(ctypes.c_void_p)))
+        return off, idx[:sz]
+
+    def ranks(self, level):
+        sz = nat.lib().h3d_il_size(self._h, level)
+        r = np.zeros(max(sz, 1), dtype=np.int32)
+        s = np.zeros(max(sz, 1), dtype=np.int32)
+        nat.check(nat.lib().h3d_export_ranks(self._h, level, r.ctypes.data_as(ctypes.c_void_p),
+                                             s.ctypes.data_as(ctypes.c_void_p)))
+        return r[:sz], s[:sz]
+
+    def block(self, level, b, rank, n=0, m=0, uv=False):
+        I = np.zeros(max(rank, 1), dtype=np.int32)
+        J = np.zeros(max(rank, 1), dtype=np.int32)
+        U = np.zeros((rank, n)) if uv else None
+        V = np.zeros((rank, m)) if uv else None
+        vp = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        nat.check(nat.lib().h3d_export_block(self._h, level, int(b), vp(I), vp(J), vp(U), vp(V)))
+        return I[:rank], J[:rank], U, V
+
+
+class classGMRES:
+    """src/gmres.cpp:40-94 on the GPU: unrestarted GMRES, x0 = 0, MGS + Givens."""
+
+    def gmres(self, M, b, maxIterations, threshold):
+        a = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(a)
+        err = ctypes.c_double()
+        its = ctypes.c_int()
+        hist = np.zeros(maxIterations + 1)
+        nat.check(nat.lib().h3d_gmres(M._h, a.ctypes.data_as(ctypes.c_void_p),
+                                      x.ctypes.data_as(ctypes.c_void_p), int(maxIterations),
+                                      float(threshold), ctypes.byref(err), ctypes.byref(its),
+                                      hist.ctypes.data_as(ctypes.c_void_p), nat.HOST))
+        n = its.value + 1
+        return x, err.value, its.value, hist[:n].tolist()

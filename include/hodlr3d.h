@@ -1,0 +1,156 @@
+/*
+ * hodlr3d.h -- C ABI of the MI355X HODLR3D engine (libhodlr3d.so).
+ *
+ * The reference (SAFRAN-LAB/HODLR3D) exposes C++ classes only; each entry
+ * point below names the reference interface it stands in for.  All functions
+ * return an int status (H3D_OK == 0).  Pointers are host memory unless a
+ * `where` argument says H3D_DEVICE.  Calls on one handle are ordered on the
+ * handle's HIP stream and are not re-entrant; distinct handles are independent.
+ * Vectors are in the caller's (original) point order, like the reference's
+ * MatVecProduct after `reorder` (src/HODLR3DTree.cpp:1795-1805).
+ */
+#ifndef HODLR3D_H_
+#define HODLR3D_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference has none: void APIs, UB on bad input) ---- */
+#define H3D_OK 0
+#define H3D_ERR_INVALID 1      /* bad argument, N < 1, nLevels outside [1,7], ... */
+#define H3D_ERR_OOM 2          /* device or host allocation failed */
+#define H3D_ERR_NONFINITE 3    /* non-finite kernel entry (duplicate points, NaN) */
+#define H3D_ERR_HIP 4          /* HIP runtime error */
+#define H3D_ERR_STATE 5        /* call out of order, e.g. matvec before assemble */
+#define H3D_ERR_NOT_CONVERGED 6
+#define H3D_ERR_INTERNAL 7
+
+/* ---- stored-factor precisions ---- */
+#define H3D_PREC_FP64 0  /* fp64 factors, fp64 accumulation, fp64 near field */
+#define H3D_PREC_FP32 1  /* fp32 factors, fp32 accumulation, fp32 near field */
+#define H3D_PREC_BF16 2  /* bf16 factors, fp32 accumulation, fp32 near field */
+
+/* ---- pointer locations ---- */
+#define H3D_HOST 0
+#define H3D_DEVICE 1
+
+typedef struct h3d_handle h3d_handle;
+
+/* Kernel plugin.  Replaces `userkernel::getMatrixEntry` (include/kernel.hpp:112,
+ * dispatch examples/testHODLR3Dsolve.cpp:32-76): entry(i,j) = scale * phi_q(r_ij)
+ * for i != j and the diagonal value for i == j.  qchoice follows the
+ * reference's Qchoice numbering 0..16; scale = 1, diag = 0 reproduces the
+ * reference for Qchoice 0..15.  For Qchoice 16 (IE_CUBE_3D,
+ * src/kernel.cpp:65-73) pass scale = diag = 0 to get h3 = 1/N and
+ * Kii = 1 + the self-cell integral (src/Integral3D.cpp:9-43). */
+typedef struct h3d_kernel {
+  int qchoice;
+  double scale;
+  double diag;
+  const double* diag_vec; /* optional per-point diagonal (host, original order) */
+} h3d_kernel;
+
+typedef struct h3d_options {
+  double center[3];        /* root cube centre; the reference uses the origin */
+  int nlevels;             /* < 0: the reference formula (src/HODLR3D.cpp:6-7) */
+  int64_t workspace_bytes; /* device workspace budget for assembly, 0 = automatic */
+} h3d_options;
+
+typedef struct h3d_stats {
+  int64_t N;
+  int nlevels;
+  int max_rank;            /* HODLR3DTree::getMaxRank (src/HODLR3DTree.cpp:1807-1818) */
+  int64_t n_blocks;        /* low-rank blocks (all levels) */
+  int64_t rank_sum;        /* sum of block ranks */
+  int64_t factor_elems;    /* sum over blocks of r*(n+m): U and V entries */
+  double find_memory;      /* HODLR3DTree::findMemory entries (src/HODLR3DTree.cpp:1757-1778) */
+  int64_t near_pairs;      /* sum over leaves of n_k * (sum of near-box sizes) */
+  double t_tree_ms;        /* octree, bucketing, lists (device, wall) */
+  double t_assemble_ms;    /* ACA + factor layout (device, wall) */
+  int64_t factor_bytes[3]; /* resident factor bytes per precision (0 if absent) */
+} h3d_stats;
+
+typedef struct h3d_solve_info {
+  int outer_iterations;    /* refinement steps */
+  int inner_iterations;    /* total inner GMRES iterations */
+  double rel_residual;     /* final ||b - A x|| / ||b|| with the fp64 operator */
+  double history[64];      /* relative residual after each refinement step */
+} h3d_solve_info;
+
+/* nLevels = ceil(3 log(N^(1/3)/leaf1d) / log 8): src/HODLR3D.cpp:6-7. */
+int h3d_nlevels(int64_t N, int leaf1d);
+
+/* Replaces HODLR3D::HODLR3D (src/HODLR3D.cpp:3-15): copies the points
+ * (N x 3, pts3D layout, include/definitions.hpp:7-9), builds the octree,
+ * interaction lists and point buckets on the GPU.  `opt` may be NULL. */
+int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
+               const h3d_kernel* kernel, const h3d_options* opt, h3d_handle** out);
+
+/* Replaces HODLR3D::assemble (src/HODLR3D.cpp:17-19 -> getUVtTree,
+ * src/HODLR3DTree.cpp:1621-1645): pivot-exact ACA of every block on the GPU,
+ * factors stored in fp64. */
+int h3d_assemble(h3d_handle* h);
+
+/* Builds (or frees) the fp32 / bf16 copies of the fp64 factors. */
+int h3d_prepare_precision(h3d_handle* h, int prec);
+int h3d_release_precision(h3d_handle* h, int prec);
+
+/* Replaces HODLR3D::MatVecProduct (src/HODLR3D.cpp:21-27): y = A b for nrhs
+ * column-major right-hand sides of length N. */
+int h3d_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int prec, int where);
+
+/* y = A^T b with the same stored factors (no reference counterpart; the
+ * adjoint of MatVecProduct). */
+int h3d_matvec_transpose(h3d_handle* h, const double* b, double* y, int nrhs, int prec,
+                         int where);
+
+/* Replaces the per-point diagonal after creation (the self-block entries). */
+int h3d_set_diag(h3d_handle* h, const double* diag_vec, int where);
+
+/* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,
+ * modified Gram-Schmidt, Givens rotations.  hist receives error values (its+1
+ * entries, at most maxit+1). */
+int h3d_gmres(h3d_handle* h, const double* b, double* x, int maxit, double threshold,
+              double* error, int* iterations, double* hist, int where);
+
+/* GMRES(restart) with the `inner_prec` operator inside fp64 iterative
+ * refinement: r = b - A x with fp64 factors, inner solve to `inner_tol`,
+ * x += d, until ||r||/||b|| <= tol.  transpose != 0 solves A^T x = b. */
+int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max_inner,
+                 double inner_tol, int inner_prec, double tol, int max_outer, int transpose,
+                 h3d_solve_info* info, int where);
+
+/* Structure export for bit-exact checks (the reference keeps these in
+ * HODLR3DBox, include/HODLR3DTree.hpp:11-24). */
+int h3d_export_perm(h3d_handle* h, int32_t* perm);                 /* leaf-DFS order */
+int h3d_export_boxes(h3d_handle* h, int level, int32_t* start, int32_t* count);
+int h3d_export_neighbors(h3d_handle* h, int level, int32_t* nbr26);
+int64_t h3d_il_size(h3d_handle* h, int level);
+int h3d_export_il(h3d_handle* h, int level, int32_t* offsets, int32_t* boxes);
+int h3d_export_ranks(h3d_handle* h, int level, int32_t* rank, int32_t* status);
+/* Local pivots of block b (row within the target box, column within the
+ * source box) and, if U/V are non-NULL, the ACA vectors (rank x n, rank x m). */
+int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J, double* U,
+                     double* V);
+
+int h3d_get_stats(h3d_handle* h, h3d_stats* s);
+
+/* Use a caller-owned hipStream_t (passed as void*); NULL restores the handle's own. */
+int h3d_set_stream(h3d_handle* h, void* stream);
+int h3d_synchronize(h3d_handle* h);
+
+/* Replaces HODLR3D::~HODLR3D (src/HODLR3D.cpp:29-32). */
+void h3d_destroy(h3d_handle* h);
+
+const char* h3d_status_string(int status);
+/* Text of the last error raised on this thread. */
+const char* h3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HODLR3D_H_ */
