@@ -21,6 +21,7 @@ H3D_ERR_INTERNAL = 7
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
+ACC_FP64 = 0x100
 HOST = 0
 DEVICE = 1
 
@@ -35,18 +36,20 @@ class Kernel(ctypes.Structure):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("center", _d * 3), ("nlevels", _i32), ("workspace_bytes", _i64)]
+    _fields_ = [("center", _d * 3), ("nlevels", _i32), ("workspace_bytes", _i64),
+                ("store_mask", _i32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("N", _i64), ("nlevels", _i32), ("max_rank", _i32), ("n_blocks", _i64),
                 ("rank_sum", _i64), ("factor_elems", _i64), ("find_memory", _d),
                 ("near_pairs", _i64), ("t_tree_ms", _d), ("t_assemble_ms", _d),
-                ("factor_bytes", _i64 * 3)]
+                ("factor_bytes", _i64 * 3), ("u_elems", _i64), ("v_elems", _i64)]
 
 
 class SolveInfo(ctypes.Structure):
-    _fields_ = [("outer_iterations", _i32), ("inner_iterations", _i32), ("rel_residual", _d),
+    _fields_ = [("outer_iterations", _i32), ("inner_iterations", _i32), ("outer_prec", _i32),
+                ("rel_residual", _d),
                 ("history", _d * 64)]
 
 
@@ -58,6 +61,9 @@ SIGNATURES = {
     "h3d_assemble": (_i32, [_p]),
     "h3d_prepare_precision": (_i32, [_p, _i32]),
     "h3d_release_precision": (_i32, [_p, _i32]),
+    "h3d_move_precision": (_i32, [_p, _i32]),
+    "h3d_set_operator": (_i32, [_p, _d, _d]),
+    "h3d_kernel_timing": (_i32, [_p, _i32, _p, ctypes.POINTER(_i32)]),
     "h3d_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_matvec_transpose": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_set_diag": (_i32, [_p, _p, _i32]),
@@ -65,6 +71,7 @@ SIGNATURES = {
                          _i32]),
     "h3d_solve_ir": (_i32, [_p, _p, _p, _i32, _i32, _d, _i32, _d, _i32, _i32,
                             ctypes.POINTER(SolveInfo), _i32]),
+    "h3d_direct": (_i32, [_p, _p, _p, _i64, _p, _i32]),
     "h3d_export_perm": (_i32, [_p, _p]),
     "h3d_export_boxes": (_i32, [_p, _i32, _p, _p]),
     "h3d_export_neighbors": (_i32, [_p, _i32, _p]),

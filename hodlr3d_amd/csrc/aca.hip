@@ -406,50 +406,44 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
   }
 }
 
-// Copies the first r vectors and pivots of each block into compact staging.
-struct ExtractDesc {
-  int64_t uw, vw;     // workspace offsets
-  int64_t su, sv;     // staging offsets (doubles)
-  int32_t n, m, r, pw, rcap, sp;  // sp: staging pivot offset (ints)
+// Moves one finished block from the ACA workspace into its chunk: u-vectors
+// as consecutive columns of the target box's U (column-major, contiguous),
+// v-vectors transposed to row-major m x r, pivots I then J.  Every stored
+// precision is written from the fp64 values.
+struct WriteDesc {
+  int64_t uw, vw;  // workspace offsets (doubles)
+  int64_t uo, vo;  // chunk offsets (elements) of the block's U columns / V
+  int64_t po;      // pivot offset
+  int32_t n, m, r, pw, rcap, pad;
 };
 
-__global__ void extract_kernel(const ExtractDesc* __restrict__ ds, const double* __restrict__ Uws,
-                               const double* __restrict__ Vws, const int32_t* __restrict__ pivws,
-                               double* __restrict__ stU, double* __restrict__ stV,
-                               int32_t* __restrict__ stP) {
-  const ExtractDesc d = ds[blockIdx.x];
-  const int64_t nu = (int64_t)d.r * d.n, nv = (int64_t)d.r * d.m;
-  for (int64_t q = threadIdx.x; q < nu; q += blockDim.x) stU[d.su + q] = Uws[d.uw + q];
-  for (int64_t q = threadIdx.x; q < nv; q += blockDim.x) stV[d.sv + q] = Vws[d.vw + q];
-  for (int q = threadIdx.x; q < d.r; q += blockDim.x) {
-    stP[d.sp + q] = pivws[d.pw + q];
-    stP[d.sp + d.r + q] = pivws[d.pw + d.rcap + q];
-  }
-}
-
-// Moves one block's vectors from staging into the per-level matvec layout.
-struct LayoutDesc {
-  const double* su;  // staging u_l (r x n)
-  const double* sv;  // staging v_l (r x m)
-  int64_t uo;        // U: element offset of the block's first column
-  int64_t vo;        // V: element offset of row 0, first column of the block
-  int32_t n, m, r, Rp;
-  int32_t tdo;       // tdst offset of the block's first column
-  int32_t tval;      // t index of the block's first rank
-};
-
-__global__ void layout_kernel(const LayoutDesc* __restrict__ ds, double* __restrict__ Ulev,
-                              double* __restrict__ Vlev, int32_t* __restrict__ tdst) {
-  const LayoutDesc d = ds[blockIdx.x];
+__global__ void write_kernel(const WriteDesc* __restrict__ ds, const double* __restrict__ Uws,
+                             const double* __restrict__ Vws, const int32_t* __restrict__ pivws,
+                             double* __restrict__ U64, double* __restrict__ V64,
+                             float* __restrict__ U32, float* __restrict__ V32,
+                             uint16_t* __restrict__ U16, uint16_t* __restrict__ V16,
+                             int32_t* __restrict__ piv) {
+  const WriteDesc d = ds[blockIdx.x];
   const int64_t nu = (int64_t)d.r * d.n;
-  for (int64_t q = threadIdx.x; q < nu; q += blockDim.x) Ulev[d.uo + q] = d.su[q];
+  for (int64_t q = threadIdx.x; q < nu; q += blockDim.x) {
+    const double v = Uws[d.uw + q];
+    if (U64) U64[d.uo + q] = v;
+    if (U32) U32[d.uo + q] = (float)v;
+    if (U16) U16[d.uo + q] = f_to_bf16((float)v);
+  }
   const int64_t nv = (int64_t)d.r * d.m;
   for (int64_t q = threadIdx.x; q < nv; q += blockDim.x) {
-    int l = (int)(q % d.r);
-    int64_t s = q / d.r;
-    Vlev[d.vo + s * d.Rp + l] = d.sv[(int64_t)l * d.m + s];
+    const int64_t s = q / d.r;
+    const int l = (int)(q - s * d.r);
+    const double v = Vws[d.vw + (int64_t)l * d.m + s];
+    if (V64) V64[d.vo + q] = v;
+    if (V32) V32[d.vo + q] = (float)v;
+    if (V16) V16[d.vo + q] = f_to_bf16((float)v);
   }
-  for (int l = threadIdx.x; l < d.r; l += blockDim.x) tdst[d.tdo + l] = d.tval + l;
+  for (int l = threadIdx.x; l < d.r; l += blockDim.x) {
+    piv[d.po + l] = pivws[d.pw + l];
+    piv[d.po + d.r + l] = pivws[d.pw + d.rcap + l];
+  }
 }
 
 size_t aca_smem(int rcap, int n, int m) {
@@ -457,11 +451,95 @@ size_t aca_smem(int rcap, int n, int m) {
          sizeof(uint32_t) * (size_t)(((n + 31) >> 5) + ((m + 31) >> 5)) + 16;
 }
 
-struct Staged {
-  double* su = nullptr;
-  double* sv = nullptr;
-  int32_t* sp = nullptr;  // I then J
+// Workspace of one ACA launch group.
+struct Pass {
+  double* U = nullptr;
+  double* V = nullptr;
+  int32_t* P = nullptr;
+  std::vector<AcaDesc> ds;
+  std::vector<int64_t> ids;  // CSR block index within the level
+  std::vector<int32_t> rank, status;
+  void release() {
+    dfree(U);
+    dfree(V);
+    dfree(P);
+  }
 };
+
+int cap_for(int pass, int n, int m) {
+  const int mn = std::min(n, m);
+  const bool big = std::max(n, m) > 256;
+  int c = pass == 0 ? (big ? 128 : 96) : (pass == 1 ? 512 : mn);
+  return std::min(c, mn);
+}
+
+void run_pass(Engine& e, Pass& P, int cls_filter_unused) {
+  (void)cls_filter_unused;
+  hipStream_t st = e.stream;
+  const int64_t nb = (int64_t)P.ds.size();
+  P.rank.assign(nb, 0);
+  P.status.assign(nb, 0);
+  if (nb == 0) return;
+  int64_t wsU = 0, wsV = 0, pw = 0;
+  for (auto& d : P.ds) {
+    wsU = std::max<int64_t>(wsU, d.uw + (int64_t)d.rcap * d.n);
+    wsV = std::max<int64_t>(wsV, d.vw + (int64_t)d.rcap * d.m);
+    pw = std::max<int64_t>(pw, d.pw + 2 * (int64_t)d.rcap);
+  }
+  P.U = dalloc<double>(wsU);
+  P.V = dalloc<double>(wsV);
+  P.P = dalloc<int32_t>(pw);
+  int32_t* d_rank = dalloc<int32_t>(nb);
+  int32_t* d_stat = dalloc<int32_t>(nb);
+  const bool fast = e.kern.q == 7 || e.kern.q == 16;
+  // two launches: small blocks on one wave, large blocks on four
+  for (int cls = 0; cls < 2; ++cls) {
+    std::vector<AcaDesc> sub;
+    std::vector<int64_t> where;
+    size_t smax = 0;
+    for (int64_t q = 0; q < nb; ++q) {
+      const AcaDesc& d = P.ds[q];
+      const int c = std::max(d.n, d.m) <= 256 ? 0 : 1;
+      if (c != cls) continue;
+      sub.push_back(d);
+      where.push_back(q);
+      smax = std::max(smax, aca_smem(d.rcap, d.n, d.m));
+    }
+    if (sub.empty()) continue;
+    if (smax > 160 * 1024) throw Error(H3D_ERR_INVALID, "ACA block too large for LDS");
+    const void* kfun = cls == 0 ? (fast ? (const void*)aca_kernel<64, InvR>
+                                        : (const void*)aca_kernel<64, AnyQ>)
+                                : (fast ? (const void*)aca_kernel<256, InvR>
+                                        : (const void*)aca_kernel<256, AnyQ>);
+    if (smax > 64 * 1024)
+      H3D_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smax));
+    AcaDesc* d_ds = dalloc<AcaDesc>((int64_t)sub.size());
+    int32_t* d_r = dalloc<int32_t>((int64_t)sub.size());
+    int32_t* d_s = dalloc<int32_t>((int64_t)sub.size());
+    H3D_HIP(hipMemcpyAsync(d_ds, sub.data(), sizeof(AcaDesc) * sub.size(), hipMemcpyHostToDevice, st));
+    for (size_t off = 0; off < sub.size(); off += 1000000) {
+      const unsigned g = (unsigned)std::min<size_t>(1000000, sub.size() - off);
+      AcaDesc* dd = d_ds + off;
+      int32_t* rr = d_r + off;
+      int32_t* ss = d_s + off;
+      void* args[] = {&dd, &e.d_px, &e.d_py, &e.d_pz, &e.kern, &e.tol, &P.U, &P.V, &P.P, &rr, &ss};
+      H3D_HIP(hipLaunchKernel(kfun, dim3(g), dim3(cls == 0 ? 64 : 256), args, smax, st));
+    }
+    std::vector<int32_t> rk(sub.size()), sv(sub.size());
+    H3D_HIP(hipMemcpyAsync(rk.data(), d_r, sizeof(int32_t) * sub.size(), hipMemcpyDeviceToHost, st));
+    H3D_HIP(hipMemcpyAsync(sv.data(), d_s, sizeof(int32_t) * sub.size(), hipMemcpyDeviceToHost, st));
+    H3D_HIP(hipStreamSynchronize(st));
+    for (size_t q = 0; q < sub.size(); ++q) {
+      P.rank[where[q]] = rk[q];
+      P.status[where[q]] = sv[q];
+    }
+    dfree(d_ds);
+    dfree(d_r);
+    dfree(d_s);
+  }
+  dfree(d_rank);
+  dfree(d_stat);
+}
 
 }  // namespace
 
@@ -472,286 +550,219 @@ void assemble_all(Engine& e) {
   size_t freeb = 0, totb = 0;
   H3D_HIP(hipMemGetInfo(&freeb, &totb));
   int64_t budget = e.ws_budget > 0 ? e.ws_budget
-                                   : std::min<int64_t>((int64_t)(freeb / 4), int64_t(32) << 30);
+                                   : std::min<int64_t>((int64_t)(freeb / 8), int64_t(24) << 30);
   budget = std::max<int64_t>(budget, int64_t(64) << 20);
 
   e.T = 0;
-  e.nU = e.nV = 0;
   e.max_rank = 0;
-  e.n_blocks = e.rank_sum = e.factor_elems = 0;
-  std::vector<double*> Ulev(nL + 1, nullptr), Vlev(nL + 1, nullptr);
-  std::vector<int64_t> nUlev(nL + 1, 0), nVlev(nL + 1, 0);
-  std::vector<int32_t*> tdlev(nL + 1, nullptr);
-  std::vector<int64_t> ntd(nL + 1, 0);
+  e.n_blocks = e.rank_sum = e.factor_elems = e.u_elems = e.v_elems = 0;
+  e.find_memory = 0;
+  e.nblk_all = 0;
+  for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
 
   for (int j = 1; j <= nL; ++j) {
     Level& lv = e.lev[j];
     const int64_t nb = lv.nboxes;
     const int64_t nblk = (int64_t)lv.ilidx.size();
     const int64_t lo = level_offset(j);
+    lv.bbase = e.nblk_all;
+    e.nblk_all += nblk;
     lv.rank.assign(nblk, 0);
     lv.status.assign(nblk, 0);
-    std::vector<int32_t> tgt(nblk);
-    for (int64_t k = 0; k < nb; ++k)
-      for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) tgt[b] = (int32_t)k;
-    std::vector<Staged> stg(nblk);
-    std::vector<std::pair<void*, int>> chunks;  // allocations to free after layout
-
-    // pending blocks with their rank capacity
-    std::vector<int64_t> todo;
-    for (int64_t b = 0; b < nblk; ++b) {
-      int n = e.bcount[lo + tgt[b]], m = e.bcount[lo + lv.ilidx[b]];
-      if (n > 0 && m > 0) todo.push_back(b);
-    }
-    for (int pass = 0; pass < 3 && !todo.empty(); ++pass) {
-      std::vector<int64_t> retry;
-      for (int cls = 0; cls < 2; ++cls) {
-        std::vector<int64_t> list;
-        for (int64_t b : todo) {
-          int n = e.bcount[lo + tgt[b]], m = e.bcount[lo + lv.ilidx[b]];
-          int c = std::max(n, m) <= 256 ? 0 : 1;
-          if (c == cls) list.push_back(b);
-        }
-        size_t pos = 0;
-        while (pos < list.size()) {
-          // gather a batch within the workspace budget
-          std::vector<AcaDesc> ds;
-          std::vector<int64_t> ids;
-          int64_t wsU = 0, wsV = 0;
-          int32_t pw = 0;
-          size_t smax = 0;
-          while (pos < list.size()) {
-            int64_t b = list[pos];
-            int32_t k = tgt[b], ki = lv.ilidx[b];
-            int n = e.bcount[lo + k], m = e.bcount[lo + ki];
-            int cap = pass == 0 ? 64 : (pass == 1 ? 512 : std::min(n, m));
-            cap = std::min(cap, std::min(n, m));
-            int64_t need = (int64_t)cap * (n + m) * 8;
-            if (!ds.empty() && (wsU + wsV) * 8 + need > budget) break;
-            if (ds.size() >= 1000000) break;
-            AcaDesc d;
-            d.rs = e.bstart[lo + k];
-            d.n = n;
-            d.cs = e.bstart[lo + ki];
-            d.m = m;
-            d.uw = wsU;
-            d.vw = wsV;
-            d.rcap = cap;
-            d.pw = pw;
-            wsU += (int64_t)cap * n;
-            wsV += (int64_t)cap * m;
-            pw += 2 * cap;
-            smax = std::max(smax, aca_smem(cap, n, m));
-            ds.push_back(d);
-            ids.push_back(b);
-            ++pos;
-          }
-          const int64_t nbat = (int64_t)ds.size();
-          AcaDesc* d_ds = dalloc<AcaDesc>(nbat);
-          double* d_U = dalloc<double>(wsU);
-          double* d_V = dalloc<double>(wsV);
-          int32_t* d_pw = dalloc<int32_t>(pw);
-          int32_t* d_rank = dalloc<int32_t>(nbat);
-          int32_t* d_stat = dalloc<int32_t>(nbat);
-          H3D_HIP(hipMemcpyAsync(d_ds, ds.data(), sizeof(AcaDesc) * nbat, hipMemcpyHostToDevice, st));
-          if (smax > 160 * 1024)
-            throw Error(H3D_ERR_INVALID, "ACA block too large for LDS");
-          const bool fast = e.kern.q == 7 || e.kern.q == 16;
-          const void* kfun =
-              cls == 0 ? (fast ? (const void*)aca_kernel<64, InvR> : (const void*)aca_kernel<64, AnyQ>)
-                       : (fast ? (const void*)aca_kernel<256, InvR> : (const void*)aca_kernel<256, AnyQ>);
-          if (smax > 64 * 1024)
-            H3D_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smax));
-          void* args[] = {&d_ds, &e.d_px, &e.d_py, &e.d_pz, &e.kern, &e.tol, &d_U, &d_V, &d_pw,
-                          &d_rank, &d_stat};
-          H3D_HIP(hipLaunchKernel(kfun, dim3((unsigned)nbat), dim3(cls == 0 ? 64 : 256), args, smax, st));
-          H3D_LAUNCH_CHECK();
-          std::vector<int32_t> rk(nbat), ss(nbat);
-          H3D_HIP(hipMemcpyAsync(rk.data(), d_rank, sizeof(int32_t) * nbat, hipMemcpyDeviceToHost, st));
-          H3D_HIP(hipMemcpyAsync(ss.data(), d_stat, sizeof(int32_t) * nbat, hipMemcpyDeviceToHost, st));
-          H3D_HIP(hipStreamSynchronize(st));
-          // stage accepted blocks
-          std::vector<ExtractDesc> xs;
-          int64_t su = 0, sv = 0;
-          int32_t sp = 0;
-          std::vector<std::array<int64_t, 3>> soff;
-          for (int64_t q = 0; q < nbat; ++q) {
-            int64_t b = ids[q];
-            if (ss[q] & 2) {
-              retry.push_back(b);
-              continue;
-            }
-            if (ss[q] & 1) throw Error(H3D_ERR_NONFINITE, "non-finite kernel entry in ACA");
-            lv.rank[b] = rk[q];
-            lv.status[b] = ss[q];
-            if (rk[q] == 0) continue;
-            ExtractDesc x;
-            x.uw = ds[q].uw;
-            x.vw = ds[q].vw;
-            x.su = su;
-            x.sv = sv;
-            x.n = ds[q].n;
-            x.m = ds[q].m;
-            x.r = rk[q];
-            x.pw = ds[q].pw;
-            x.rcap = ds[q].rcap;
-            x.sp = sp;
-            soff.push_back({su, sv, sp});
-            su += (int64_t)x.r * x.n;
-            sv += (int64_t)x.r * x.m;
-            sp += 2 * x.r;
-            xs.push_back(x);
-          }
-          if (!xs.empty()) {
-            double* cu = dalloc<double>(su);
-            double* cv = dalloc<double>(sv);
-            int32_t* cp = dalloc<int32_t>(sp);
-            chunks.push_back({cu, 0});
-            chunks.push_back({cv, 0});
-            chunks.push_back({cp, 0});
-            ExtractDesc* d_x = dalloc<ExtractDesc>((int64_t)xs.size());
-            H3D_HIP(hipMemcpyAsync(d_x, xs.data(), sizeof(ExtractDesc) * xs.size(),
-                                   hipMemcpyHostToDevice, st));
-            extract_kernel<<<(unsigned)xs.size(), 256, 0, st>>>(d_x, d_U, d_V, d_pw, cu, cv, cp);
-            H3D_LAUNCH_CHECK();
-            size_t qq = 0;
-            for (int64_t q = 0; q < nbat; ++q) {
-              int64_t b = ids[q];
-              if ((ss[q] & 2) || rk[q] == 0) continue;
-              stg[b].su = cu + soff[qq][0];
-              stg[b].sv = cv + soff[qq][1];
-              stg[b].sp = cp + soff[qq][2];
-              ++qq;
-            }
-            H3D_HIP(hipStreamSynchronize(st));
-            dfree(d_x);
-          }
-          dfree(d_ds);
-          dfree(d_U);
-          dfree(d_V);
-          dfree(d_pw);
-          dfree(d_rank);
-          dfree(d_stat);
-        }
-      }
-      todo.swap(retry);
-    }
-    if (!todo.empty()) throw Error(H3D_ERR_INTERNAL, "ACA did not finish within capacity");
-
-    // ---- layout of this level
+    lv.col.assign(nblk, 0);
+    lv.voff.assign(nblk, 0);
+    lv.pivoff.assign(nblk, 0);
     lv.Rt.assign(nb, 0);
-    lv.Rs.assign(nb, 0);
-    lv.bcol_t.assign(nblk, 0);
-    lv.bcol_s.assign(nblk, 0);
-    lv.pivoff.assign(nblk + 1, 0);
-    for (int64_t b = 0; b < nblk; ++b) {
-      int r = lv.rank[b];
-      lv.bcol_t[b] = lv.Rt[tgt[b]];
-      lv.Rt[tgt[b]] += r;
-      lv.bcol_s[b] = lv.Rs[lv.ilidx[b]];
-      lv.Rs[lv.ilidx[b]] += r;
-      lv.pivoff[b + 1] = lv.pivoff[b] + r;
-      e.max_rank = std::max(e.max_rank, r);
-      int n = e.bcount[lo + tgt[b]], m = e.bcount[lo + lv.ilidx[b]];
-      e.factor_elems += (int64_t)r * (n + m);
-      e.rank_sum += r;
-      e.find_memory += (double)r * n + (double)r * r + (double)m * r;
+    lv.toff.assign(nb, 0);
+    lv.chunk.assign(nb, -1);
+    lv.uoff.assign(nb, 0);
+    int64_t k0 = 0;
+    while (k0 < nb) {
+      // batch of complete target boxes within the workspace budget
+      int64_t k1 = k0, bytes = 0;
+      while (k1 < nb) {
+        int64_t need = 0;
+        const int n = e.bcount[lo + k1];
+        for (int32_t b = lv.iloff[k1]; b < lv.iloff[k1 + 1]; ++b) {
+          const int m = e.bcount[lo + lv.ilidx[b]];
+          need += (int64_t)cap_for(0, n, m) * (n + m) * 8;
+        }
+        if (k1 > k0 && (bytes + need > budget || lv.iloff[k1 + 1] - lv.iloff[k0] > 2000000)) break;
+        bytes += need;
+        ++k1;
+      }
+      std::vector<int64_t> todo;
+      for (int64_t k = k0; k < k1; ++k) {
+        const int n = e.bcount[lo + k];
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b)
+          if (n > 0 && e.bcount[lo + lv.ilidx[b]] > 0) todo.push_back(b);
+      }
+      std::vector<Pass> passes;
+      for (int pass = 0; pass < 3 && !todo.empty(); ++pass) {
+        passes.emplace_back();
+        Pass& P = passes.back();
+        int64_t wsU = 0, wsV = 0;
+        int32_t pw = 0;
+        for (int64_t b : todo) {
+          int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
+                                lv.iloff.begin() - 1);
+          const int32_t ki = lv.ilidx[b];
+          AcaDesc d;
+          d.rs = e.bstart[lo + k];
+          d.n = e.bcount[lo + k];
+          d.cs = e.bstart[lo + ki];
+          d.m = e.bcount[lo + ki];
+          d.rcap = cap_for(pass, d.n, d.m);
+          d.uw = wsU;
+          d.vw = wsV;
+          d.pw = pw;
+          wsU += (int64_t)d.rcap * d.n;
+          wsV += (int64_t)d.rcap * d.m;
+          pw += 2 * d.rcap;
+          P.ds.push_back(d);
+          P.ids.push_back(b);
+        }
+        run_pass(e, P, 0);
+        std::vector<int64_t> retry;
+        for (size_t q = 0; q < P.ids.size(); ++q) {
+          const int64_t b = P.ids[q];
+          if (P.status[q] & 2) {
+            retry.push_back(b);
+            P.rank[q] = -1;  // not final
+            continue;
+          }
+          if (P.status[q] & 1) throw Error(H3D_ERR_NONFINITE, "non-finite kernel entry in ACA");
+          lv.rank[b] = P.rank[q];
+          lv.status[b] = P.status[q];
+        }
+        todo.swap(retry);
+      }
+      if (!todo.empty()) throw Error(H3D_ERR_INTERNAL, "ACA did not finish within capacity");
+      // chunk layout for boxes [k0, k1)
+      Chunk ch;
+      for (int64_t k = k0; k < k1; ++k) {
+        const int n = e.bcount[lo + k];
+        int32_t R = 0;
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int r = lv.rank[b];
+          const int m = e.bcount[lo + lv.ilidx[b]];
+          lv.col[b] = R;
+          R += r;
+          lv.voff[b] = ch.nV;
+          ch.nV += (int64_t)r * m;
+          lv.pivoff[b] = ch.npiv;
+          ch.npiv += 2 * (int64_t)r;
+          e.max_rank = std::max(e.max_rank, r);
+          e.factor_elems += (int64_t)r * (n + m);
+          e.u_elems += (int64_t)r * n;
+          e.v_elems += (int64_t)r * m;
+          e.rank_sum += r;
+          e.find_memory += (double)r * n + (double)r * r + (double)m * r;
+        }
+        lv.Rt[k] = R;
+        lv.uoff[k] = ch.nU;
+        ch.nU += (int64_t)n * R;
+        lv.chunk[k] = (int32_t)e.chunks.size();
+      }
+      for (int p = 0; p < 3; ++p) {
+        if (!(e.store_mask & (1 << p))) continue;
+        const int64_t el = std::max<int64_t>(ch.nU + ch.nV, 1);
+        H3D_HIP(hipMalloc(&ch.data[p], (size_t)el * elem_size(p)));
+        e.fac_bytes[p] += (ch.nU + ch.nV) * elem_size(p);
+      }
+      ch.piv = dalloc<int32_t>(std::max<int64_t>(ch.npiv, 1));
+      for (Pass& P : passes) {
+        std::vector<WriteDesc> wd;
+        for (size_t q = 0; q < P.ids.size(); ++q) {
+          if (P.rank[q] <= 0) continue;
+          const int64_t b = P.ids[q];
+          int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
+                                lv.iloff.begin() - 1);
+          WriteDesc w;
+          w.uw = P.ds[q].uw;
+          w.vw = P.ds[q].vw;
+          w.n = P.ds[q].n;
+          w.m = P.ds[q].m;
+          w.r = P.rank[q];
+          w.pw = P.ds[q].pw;
+          w.rcap = P.ds[q].rcap;
+          w.pad = 0;
+          w.uo = lv.uoff[k] + (int64_t)lv.col[b] * w.n;
+          w.vo = lv.voff[b];
+          w.po = lv.pivoff[b];
+          wd.push_back(w);
+        }
+        if (!wd.empty()) {
+          WriteDesc* d_wd = dalloc<WriteDesc>((int64_t)wd.size());
+          H3D_HIP(hipMemcpyAsync(d_wd, wd.data(), sizeof(WriteDesc) * wd.size(),
+                                 hipMemcpyHostToDevice, st));
+          double* U64 = (double*)ch.data[0];
+          double* V64 = U64 ? U64 + ch.nU : nullptr;
+          float* U32 = (float*)ch.data[1];
+          float* V32 = U32 ? U32 + ch.nU : nullptr;
+          uint16_t* U16 = (uint16_t*)ch.data[2];
+          uint16_t* V16 = U16 ? U16 + ch.nU : nullptr;
+          for (size_t off = 0; off < wd.size(); off += 1000000) {
+            const unsigned g = (unsigned)std::min<size_t>(1000000, wd.size() - off);
+            write_kernel<<<g, 256, 0, st>>>(d_wd + off, P.U, P.V, P.P, U64, V64, U32, V32, U16,
+                                            V16, ch.piv);
+            H3D_LAUNCH_CHECK();
+          }
+          H3D_HIP(hipStreamSynchronize(st));
+          dfree(d_wd);
+        }
+        P.release();
+      }
+      e.chunks.push_back(ch);
+      k0 = k1;
+    }
+    for (int64_t k = 0; k < nb; ++k) {
+      lv.toff[k] = e.T;
+      if ((int64_t)e.T + lv.Rt[k] > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+      e.T += lv.Rt[k];
     }
     e.n_blocks += nblk;
-    for (int64_t k = 0; k < nb; ++k) lv.Rs[k] = (lv.Rs[k] + 3) & ~3;  // pad to 4 columns
-    lv.uoff.assign(nb, 0);
-    lv.voff.assign(nb, 0);
-    lv.toff.assign(nb, 0);
-    lv.tcol.assign(nb, 0);
-    int64_t uo = 0, vo = 0;
-    int64_t tc = 0;
-    for (int64_t k = 0; k < nb; ++k) {
-      int n = e.bcount[lo + k];
-      lv.uoff[k] = uo;
-      uo += (int64_t)n * lv.Rt[k];
-      lv.voff[k] = vo;
-      vo += (int64_t)n * lv.Rs[k];
-      lv.toff[k] = e.T;
-      e.T += lv.Rt[k];
-      lv.tcol[k] = (int32_t)tc;
-      tc += lv.Rs[k];
-    }
-    nUlev[j] = uo;
-    nVlev[j] = vo;
-    ntd[j] = tc;
-    if (e.T < 0 || tc > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large (t index)");
-    Ulev[j] = dalloc<double>(std::max<int64_t>(uo, 1));
-    Vlev[j] = dalloc<double>(std::max<int64_t>(vo, 1));
-    tdlev[j] = dalloc<int32_t>(std::max<int64_t>(tc, 1));
-    H3D_HIP(hipMemsetAsync(Vlev[j], 0, sizeof(double) * std::max<int64_t>(vo, 1), st));
-    H3D_HIP(hipMemsetAsync(tdlev[j], 0xff, sizeof(int32_t) * std::max<int64_t>(tc, 1), st));
-    std::vector<LayoutDesc> ld;
-    ld.reserve(nblk);
-    std::vector<int32_t> hpI(lv.pivoff[nblk]), hpJ(lv.pivoff[nblk]);
-    for (int64_t b = 0; b < nblk; ++b) {
-      int r = lv.rank[b];
-      if (r == 0) continue;
-      int32_t k = tgt[b], ki = lv.ilidx[b];
-      LayoutDesc x;
-      x.su = stg[b].su;
-      x.sv = stg[b].sv;
-      x.n = e.bcount[lo + k];
-      x.m = e.bcount[lo + ki];
-      x.r = r;
-      x.Rp = lv.Rs[ki];
-      x.uo = lv.uoff[k] + (int64_t)lv.bcol_t[b] * x.n;
-      x.vo = lv.voff[ki] + lv.bcol_s[b];
-      x.tdo = lv.tcol[ki] + lv.bcol_s[b];
-      x.tval = lv.toff[k] + lv.bcol_t[b];
-      ld.push_back(x);
-    }
-    if (!ld.empty()) {
-      LayoutDesc* d_ld = dalloc<LayoutDesc>((int64_t)ld.size());
-      H3D_HIP(hipMemcpyAsync(d_ld, ld.data(), sizeof(LayoutDesc) * ld.size(), hipMemcpyHostToDevice, st));
-      layout_kernel<<<(unsigned)ld.size(), 256, 0, st>>>(d_ld, Ulev[j], Vlev[j], tdlev[j]);
-      H3D_LAUNCH_CHECK();
-      // pivots: copy each block's staged pivots to the compact per-level arrays
-      lv.d_pivI = dalloc<int32_t>(std::max<int64_t>(lv.pivoff[nblk], 1));
-      lv.d_pivJ = dalloc<int32_t>(std::max<int64_t>(lv.pivoff[nblk], 1));
-      for (int64_t b = 0; b < nblk; ++b) {
-        int r = lv.rank[b];
-        if (r == 0) continue;
-        H3D_HIP(hipMemcpyAsync(lv.d_pivI + lv.pivoff[b], stg[b].sp, sizeof(int32_t) * r,
-                               hipMemcpyDeviceToDevice, st));
-        H3D_HIP(hipMemcpyAsync(lv.d_pivJ + lv.pivoff[b], stg[b].sp + r, sizeof(int32_t) * r,
-                               hipMemcpyDeviceToDevice, st));
-      }
-      H3D_HIP(hipStreamSynchronize(st));
-      dfree(d_ld);
-    }
-    for (auto& c : chunks) (void)hipFree(c.first);
   }
-  // The factors stay in one allocation per level; tdst (one int per V
-  // column) is merged into a single array with level offsets folded in.
-  int64_t ntdall = 0;
-  e.nU = e.nV = 0;
-  for (int j = 1; j <= nL; ++j) {
-    ntdall += ntd[j];
-    e.nUlev[j] = nUlev[j];
-    e.nVlev[j] = nVlev[j];
-    e.nU += nUlev[j];
-    e.nV += nVlev[j];
-    e.fac[0].U[j] = Ulev[j];
-    e.fac[0].V[j] = Vlev[j];
-  }
-  e.d_tdst = dalloc<int32_t>(std::max<int64_t>(ntdall, 1));
-  int64_t ot = 0;
-  for (int j = 1; j <= nL; ++j) {
-    Level& lv = e.lev[j];
-    if (ntd[j])
-      H3D_HIP(hipMemcpyAsync(e.d_tdst + ot, tdlev[j], sizeof(int32_t) * ntd[j],
-                             hipMemcpyDeviceToDevice, st));
+  // per-block device table, blocks grouped by source box, t index per block
+  {
+    std::vector<BlockDev> blk(e.nblk_all);
+    std::vector<int32_t> btoff(e.nblk_all);
+    e.by_src.clear();
+    e.by_src.reserve(e.nblk_all);
+    for (int j = 1; j <= nL; ++j) {
+      const Level& lv = e.lev[j];
+      const int64_t lo = level_offset(j);
+      std::vector<std::vector<int32_t>> bys(lv.nboxes);
+      for (int64_t k = 0; k < lv.nboxes; ++k)
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int32_t ki = lv.ilidx[b];
+          BlockDev d;
+          d.voff = lv.voff[b];
+          d.col = lv.col[b];
+          d.r = lv.rank[b];
+          d.m = e.bcount[lo + ki];
+          d.src = e.bstart[lo + ki];
+          d.chunk = lv.chunk[k];
+          d.pad = 0;
+          blk[lv.bbase + b] = d;
+          btoff[lv.bbase + b] = lv.toff[k] + lv.col[b];
+          bys[ki].push_back((int32_t)(lv.bbase + b));
+        }
+      for (auto& v : bys) e.by_src.insert(e.by_src.end(), v.begin(), v.end());
+    }
+    dfree(e.d_blk);
+    dfree(e.d_btoff);
+    dfree(e.d_by_src);
+    e.d_blk = dalloc<BlockDev>(std::max<int64_t>(e.nblk_all, 1));
+    e.d_btoff = dalloc<int32_t>(std::max<int64_t>(e.nblk_all, 1));
+    e.d_by_src = dalloc<int32_t>(std::max<int64_t>(e.nblk_all, 1));
+    if (e.nblk_all) {
+      H3D_HIP(hipMemcpyAsync(e.d_blk, blk.data(), sizeof(BlockDev) * e.nblk_all, hipMemcpyHostToDevice, st));
+      H3D_HIP(hipMemcpyAsync(e.d_btoff, btoff.data(), sizeof(int32_t) * e.nblk_all, hipMemcpyHostToDevice, st));
+      H3D_HIP(hipMemcpyAsync(e.d_by_src, e.by_src.data(), sizeof(int32_t) * e.nblk_all, hipMemcpyHostToDevice, st));
+    }
     H3D_HIP(hipStreamSynchronize(st));
-    dfree(tdlev[j]);
-    for (auto& x : lv.tcol) x += (int32_t)ot;
-    ot += ntd[j];
   }
-  e.fac[0].bytes = (e.nU + e.nV) * 8;
   // near-field pair count and findMemory's dense part (src/HODLR3DTree.cpp:1770-1777)
   {
     const int64_t lo = level_offset(nL);
@@ -769,6 +780,7 @@ void assemble_all(Engine& e) {
     }
     e.find_memory += (double)e.near_pairs;
   }
+  upload_chunk_table(e);
   e.assembled = true;
   e.t_assemble_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -4,13 +4,14 @@
 // y = A b is computed as (src/HODLR3DTree.cpp:1647-1805 restated for stored
 // factors U = K(:,J) R^-1, V^T = L^-1 K(I,:), which equal the ACA vectors):
 //   gather    x[s] = b[perm[s]]                               (assignCharges)
-//   phase 1   t = V^T x, per source box, one wave per 64 columns
-//   phase 2   y_j = U t, per target box, one wave per <= 64 rows
+//   phase 1   t = V^T x: one wave per 64 consecutive columns of a target box;
+//             each lane owns one column (of whichever block it falls in) and
+//             walks that column's rows, so no cross-lane reduction is needed
+//   phase 2   y_j = U t: one wave per <= 64 rows of a target box; lanes own
+//             rows and walk the box's columns (contiguous column-major reads)
 //   near      y_L += sum over self + face/edge leaves of K(i,j) x_j
 //   combine   y[i] = sum_j y_j[iperm[i]]                (collect + reorder)
-// Phase 1 lanes own columns and phase 2 lanes own rows, so neither needs a
-// cross-lane reduction and every wave instruction reads a contiguous run of
-// factors.  Each factor element is read exactly once per matvec.
+// Every factor element is read exactly once per matvec.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -24,10 +25,6 @@ namespace {
 
 constexpr int kP1Rows = 1024;  // rows of one phase-1 item before splitting
 constexpr int kWavesPerBlock = 4;
-
-struct LevPtr {
-  const void* p[8];
-};
 
 template <class S>
 struct Load;
@@ -45,13 +42,6 @@ struct Load<uint16_t> {
     return bf16_to_f(__builtin_nontemporal_load(p));
   }
 };
-
-template <class A>
-__device__ inline A rl(A v, int l);
-template <>
-__device__ inline double rl<double>(double v, int l) { return readlane_d(v, l); }
-template <>
-__device__ inline float rl<float>(float v, int l) { return readlane_f(v, l); }
 
 template <class A>
 __device__ inline A wave_sum_xor(A v) {
@@ -72,46 +62,51 @@ __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __res
 // ---------------------------------------------------------------- phase 1
 template <class S, class A>
 __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ items, int nitems,
-                                                     LevPtr Vl, const A* __restrict__ x,
-                                                     const int32_t* __restrict__ tdst,
-                                                     A* __restrict__ t, A* __restrict__ part) {
+                                                     void* const* __restrict__ vbase,
+                                                     const BlockDev* __restrict__ blk,
+                                                     const A* __restrict__ x, A* __restrict__ t,
+                                                     A* __restrict__ part) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const P1Item it = items[w];
-  const S* V = (const S*)Vl.p[it.level] + it.voff;
   const bool active = lane < it.nc;
   const int c = it.c0 + (active ? lane : 0);
-  const int R = it.R;
+  // last block of the box whose first column is <= c
+  int lo = it.bfirst, hi = it.bfirst + it.bcount - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk[mid].col <= c) lo = mid; else hi = mid - 1;
+  }
+  const BlockDev bd = blk[lo];
+  const int r = bd.r;
+  const S* vp = (const S*)vbase[it.chunk] + bd.voff + (c - bd.col);
+  const A* xp = x + bd.src;
+  const int se = min(it.s1, bd.m);
   A acc = 0;
-  for (int s = it.s0; s < it.s1; s += 64) {
-    const int cnt = min(64, it.s1 - s);
-    const A xv = (lane < cnt) ? x[it.xoff + s + lane] : (A)0;
-    const S* row = V + (int64_t)s * R + c;
-    int u = 0;
-    for (; u + 8 <= cnt; u += 8) {
-      A v[8];
+  int s = it.s0;
+  for (; s + 8 <= se; s += 8) {
+    A v[8], xv[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = (A)Load<S>::f(row + (int64_t)(u + q) * R);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) acc += v[q] * rl<A>(xv, u + q);
+    for (int q = 0; q < 8; ++q) {
+      v[q] = (A)Load<S>::f(vp + (int64_t)(s + q) * r);
+      xv[q] = xp[s + q];
     }
-    for (; u < cnt; ++u) acc += (A)Load<S>::f(row + (int64_t)u * R) * rl<A>(xv, u);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += v[q] * xv[q];
   }
+  for (; s < se; ++s) acc += (A)Load<S>::f(vp + (int64_t)s * r) * xp[s];
   if (!active) return;
-  if (it.pslot < 0) {
-    const int dst = tdst[it.tcol + c];
-    if (dst >= 0) t[dst] = acc;
-  } else {
+  if (it.pslot < 0)
+    t[it.toff + c] = acc;
+  else
     part[(int64_t)it.pslot * 64 + lane] = acc;
-  }
 }
 
-// Sums the row-split partials of a column chunk in row order.  direct != 0:
-// the destination is tcol + c0 + lane; otherwise tdst[tcol + c0 + lane].
+// Sums the row-split partials of a column chunk in row order into t[toff + lane].
 template <class A>
 __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ part,
-                              const int32_t* __restrict__ tdst, A* __restrict__ t, int direct) {
+                              A* __restrict__ t) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
@@ -119,15 +114,14 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
   if (lane >= r.nc) return;
   A acc = 0;
   for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)(r.pbase + q) * 64 + lane];
-  const int idx = r.tcol + r.c0 + lane;
-  const int dst = direct ? idx : tdst[idx];
-  if (dst >= 0) t[dst] = acc;
+  t[r.toff + lane] = acc;
 }
 
 // ---------------------------------------------------------------- phase 2
 template <class S, class A>
 __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ items, int nitems,
-                                                     LevPtr Ul, const A* __restrict__ t,
+                                                     void* const* __restrict__ ubase,
+                                                     const A* __restrict__ t,
                                                      double* __restrict__ ylev, int64_t N) {
   __shared__ A red[kWavesPerBlock][64];
   const int wv = threadIdx.x >> 6;
@@ -143,7 +137,7 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
     G = 64 / RB;
     const int row = lane % RB, g = lane / RB;
     if (g < G) {
-      const S* U = (const S*)Ul.p[it.level] + it.uoff + it.i0 + row;
+      const S* U = (const S*)ubase[it.chunk] + it.uoff + it.i0 + row;
       const A* tt = t + it.toff;
       const int64_t ld = it.ld;
       int c = g;
@@ -171,24 +165,23 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
 }
 
 // --------------------------------------------------------- transpose phases
-// Transpose phase 1 (items reuse P1Item: voff = U offset of the target box,
-// R = its leading dimension, xoff = its first position, tcol = its t offset):
-// t[toff + c] = sum_i U[c*ld + i] x[i], lanes over rows, one wave reduction
-// per column.
+// t[toff + c] = sum_i U[c*ld + i] x[yoff + i] for the item's columns: lanes
+// over rows, one wave reduction per column.
 template <class S, class A>
-__global__ __launch_bounds__(256) void tphase1_kernel(const P1Item* __restrict__ items, int nitems,
-                                                      LevPtr Ul, const A* __restrict__ x,
-                                                      A* __restrict__ t, A* __restrict__ part) {
+__global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict__ items, int nitems,
+                                                      void* const* __restrict__ ubase,
+                                                      const A* __restrict__ x, A* __restrict__ t,
+                                                      A* __restrict__ part) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
-  const P1Item it = items[w];
-  const S* U = (const S*)Ul.p[it.level] + it.voff;
-  const int64_t ld = it.R;
+  const TP1Item it = items[w];
+  const S* U = (const S*)ubase[it.chunk] + it.uoff;
+  const int64_t ld = it.ld;
   A mine = 0;
   for (int s = it.s0; s < it.s1; s += 64) {
     const bool rv = s + lane < it.s1;
-    const A xv = rv ? x[it.xoff + s + lane] : (A)0;
+    const A xv = rv ? x[it.yoff + s + lane] : (A)0;
     const int rr = rv ? s + lane : it.s0;
     for (int cc = 0; cc < it.nc; ++cc) {
       A p = (A)Load<S>::f(U + (int64_t)(it.c0 + cc) * ld + rr) * xv;
@@ -198,42 +191,42 @@ __global__ __launch_bounds__(256) void tphase1_kernel(const P1Item* __restrict__
   }
   if (lane >= it.nc) return;
   if (it.pslot < 0)
-    t[it.tcol + it.c0 + lane] = mine;
+    t[it.toff + it.c0 + lane] = mine;
   else
     part[(int64_t)it.pslot * 64 + lane] = mine;
 }
 
-// Transpose phase 2 (items reuse P2Item: uoff = V offset of the source box,
-// ld = R = its padded column count, toff = its tdst offset): y[s] = sum_c
-// V[s*Rp + c] t[tdst[c]], lanes over columns, one wave reduction per row.
+// y[s] = sum over blocks b with this source box of sum_l V_b[s, l] t[btoff_b + l]:
+// lanes over rows, every block's row s read as r contiguous values.
 template <class S, class A>
-__global__ __launch_bounds__(256) void tphase2_kernel(const P2Item* __restrict__ items, int nitems,
-                                                      LevPtr Vl, const A* __restrict__ t,
-                                                      const int32_t* __restrict__ tdst,
+__global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict__ items, int nitems,
+                                                      void* const* __restrict__ vbase,
+                                                      const BlockDev* __restrict__ blk,
+                                                      const int32_t* __restrict__ by_src,
+                                                      const int32_t* __restrict__ btoff,
+                                                      const A* __restrict__ t,
                                                       double* __restrict__ ylev, int64_t N) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
-  const P2Item it = items[w];
-  const S* V = (const S*)Vl.p[it.level] + it.uoff;
-  const int Rp = it.ld;
-  A mine = 0;
-  for (int c0 = 0; c0 < it.R; c0 += 64) {
-    const int c = c0 + lane;
-    A tv = 0;
-    if (c < it.R) {
-      const int d = tdst[it.toff + c];
-      tv = d >= 0 ? t[d] : (A)0;
-    }
-    const int cc = c < it.R ? c : c0;
-    for (int r = 0; r < it.nr; ++r) {
-      A p = (A)Load<S>::f(V + (int64_t)(it.i0 + r) * Rp + cc) * tv;
-      p = wave_sum_xor(p);
-      if (lane == r) mine += p;
-    }
+  const TP2Item it = items[w];
+  const bool act = lane < it.nr;
+  const int s = it.i0 + (act ? lane : 0);
+  A acc = 0;
+  for (int q = 0; q < it.scount; ++q) {
+    const int b = by_src[it.sfirst + q];
+    const BlockDev bd = blk[b];
+    if (bd.r == 0) continue;
+    const S* row = (const S*)vbase[bd.chunk] + bd.voff + (int64_t)s * bd.r;
+    const A* tt = t + btoff[b];
+    for (int l = 0; l < bd.r; ++l) acc += (A)Load<S>::f(row + l) * tt[l];
   }
-  if (lane < it.nr) ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + lane] = (double)mine;
+  if (act) ylev[(int64_t)(it.level - 1) * N + it.yoff + s] = (double)acc;
 }
+
+}  // namespace
+
+namespace {
 
 // ---------------------------------------------------------------- near field
 // fp64: exact entries (the reference's arithmetic).  The i == j term uses the
@@ -274,8 +267,6 @@ __global__ __launch_bounds__(256) void near64_kernel(
     }
   }
   if (!act) return;
-  const double dg = kd.diag_vec ? kd.diag_vec[perm[ip]] : kd.diag;
-  acc += dg * x[ip];
   yL[ip] += acc;
 }
 
@@ -325,35 +316,48 @@ __global__ __launch_bounds__(256) void near32_kernel(
     acc += part;
   }
   if (!act) return;
-  const double dg = kd.diag_vec ? kd.diag_vec[perm[ip]] : kd.diag;
-  yL[ip] += (double)(sc * acc) + dg * (double)x[ip];
+  yL[ip] += (double)(sc * acc);
 }
 
+// y[i] = alpha * (sum over levels of y_j at i's position) + d_i * b[i], where
+// d_i is the diagonal entry K(i,i) (per point when diag_vec is set): the
+// collect and reorder steps of src/HODLR3DTree.cpp:1781-1805 plus the i == j
+// term of the self block.
 __global__ void combine_kernel(const double* __restrict__ ylev, int nL, int64_t N,
-                               const int32_t* __restrict__ iperm, double* __restrict__ y) {
+                               const int32_t* __restrict__ iperm, const double* __restrict__ b,
+                               double alpha, double diag, const double* __restrict__ diag_vec,
+                               double* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t s = iperm[i];
   double acc = 0.0;
   for (int j = 0; j < nL; ++j) acc = acc + ylev[(int64_t)j * N + s];
-  y[i] = acc;
+  const double d = diag_vec ? diag_vec[i] : diag;
+  y[i] = alpha * acc + d * b[i];
 }
 
 template <class S>
-__global__ void convert_kernel(const double* __restrict__ src, int64_t n, S* __restrict__ dst);
+__global__ void convert_kernel(const void* __restrict__ src, int sprec, int64_t n, S* __restrict__ dst);
 template <>
-__global__ void convert_kernel<float>(const double* __restrict__ src, int64_t n,
+__global__ void convert_kernel<float>(const void* __restrict__ src, int sprec, int64_t n,
                                       float* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = (float)src[i];
+    dst[i] = sprec == 0 ? (float)((const double*)src)[i] : ((const float*)src)[i];
 }
 template <>
-__global__ void convert_kernel<uint16_t>(const double* __restrict__ src, int64_t n,
+__global__ void convert_kernel<uint16_t>(const void* __restrict__ src, int sprec, int64_t n,
                                          uint16_t* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = f_to_bf16((float)src[i]);
+    dst[i] = f_to_bf16(sprec == 0 ? (float)((const double*)src)[i] : ((const float*)src)[i]);
+}
+template <>
+__global__ void convert_kernel<double>(const void* __restrict__ src, int sprec, int64_t n,
+                                       double* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (double)((const float*)src)[i];
 }
 
 inline unsigned wave_grid(int64_t items) {
@@ -368,13 +372,93 @@ T* upload(const std::vector<T>& v, hipStream_t st) {
   return d;
 }
 
-LevPtr levptr(void* const* a) {
-  LevPtr p{};
-  for (int j = 0; j < 8; ++j) p.p[j] = a[j];
-  return p;
+// Direct FP64 O(N) sum for one row per workgroup: y_i = alpha * sum_{j != i}
+// K(i,j) b_j + d_i b_i with Neumaier-compensated partial sums per thread and a
+// compensated tree over the workgroup.  The reference's drivers form exact
+// products the same way, from kernel columns (examples/testHODLR3D.cpp:139-143).
+template <class KF>
+__global__ __launch_bounds__(256) void direct_kernel(
+    const int64_t* __restrict__ rows, int64_t nrows, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ pz, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ iperm, int64_t N, KernelDev kd, double alpha,
+    const double* __restrict__ diag_vec, const double* __restrict__ b, double* __restrict__ out) {
+  __shared__ double rs[256], rc[256];
+  const int64_t row = blockIdx.x;
+  if (row >= nrows) return;
+  const int64_t i = rows[row];
+  const int64_t ip = iperm[i];
+  const double xi = px[ip], yi = py[ip], zi = pz[ip];
+  double s = 0.0, c = 0.0;
+  for (int64_t q = threadIdx.x; q < N; q += 256) {
+    if (q == ip) continue;
+    const double dx = xi - px[q], dy = yi - py[q], dz = zi - pz[q];
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    const double v = kd.scale * KF::f64(kd.q, R2) * b[perm[q]];
+    const double t = s + v;
+    c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  rs[threadIdx.x] = s;
+  rc[threadIdx.x] = c;
+  __syncthreads();
+  for (int d = 128; d >= 1; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      const double a = rs[threadIdx.x], bb = rs[threadIdx.x + d];
+      const double t = a + bb;
+      rc[threadIdx.x] += rc[threadIdx.x + d] + (fabs(a) >= fabs(bb) ? (a - t) + bb : (bb - t) + a);
+      rs[threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double dg = diag_vec ? diag_vec[i] : kd.diag;
+    out[row] = alpha * (rs[0] + rc[0]) + dg * b[i];
+  }
 }
 
 }  // namespace
+
+void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out) {
+  if (nrows <= 0) return;
+  for (int64_t r = 0; r < nrows; ++r)
+    if (h_rows[r] < 0 || h_rows[r] >= e.N) throw Error(H3D_ERR_INVALID, "row out of range");
+  hipStream_t st = e.stream;
+  int64_t* d_rows = dalloc<int64_t>(nrows);
+  H3D_HIP(hipMemcpyAsync(d_rows, h_rows, sizeof(int64_t) * nrows, hipMemcpyHostToDevice, st));
+  for (int64_t off = 0; off < nrows; off += 65535) {
+    const unsigned g = (unsigned)std::min<int64_t>(65535, nrows - off);
+    if (e.kern.q == 7 || e.kern.q == 16)
+      direct_kernel<InvR><<<g, 256, 0, st>>>(d_rows + off, nrows - off, e.d_px, e.d_py, e.d_pz,
+                                             e.d_perm, e.d_iperm, e.N, e.kern, e.alpha, e.d_diag,
+                                             d_b, d_out + off);
+    else
+      direct_kernel<AnyQ><<<g, 256, 0, st>>>(d_rows + off, nrows - off, e.d_px, e.d_py, e.d_pz,
+                                             e.d_perm, e.d_iperm, e.N, e.kern, e.alpha, e.d_diag,
+                                             d_b, d_out + off);
+    H3D_LAUNCH_CHECK();
+  }
+  H3D_HIP(hipStreamSynchronize(st));
+  dfree(d_rows);
+}
+
+void upload_chunk_table(Engine& e) {
+  const int64_t nc = (int64_t)e.chunks.size();
+  for (int p = 0; p < 3; ++p) {
+    dfree(e.d_ubase[p]);
+    dfree(e.d_vbase[p]);
+    bool have = nc > 0;
+    for (const Chunk& c : e.chunks) have = have && c.data[p] != nullptr;
+    if (!have) continue;
+    std::vector<void*> u(nc), v(nc);
+    for (int64_t i = 0; i < nc; ++i) {
+      u[i] = e.chunks[i].data[p];
+      v[i] = (char*)e.chunks[i].data[p] + e.chunks[i].nU * elem_size(p);
+    }
+    e.d_ubase[p] = upload(u, e.stream);
+    e.d_vbase[p] = upload(v, e.stream);
+  }
+  H3D_HIP(hipStreamSynchronize(e.stream));
+}
 
 void build_plan(Engine& e) {
   const int nL = e.nL;
@@ -383,96 +467,101 @@ void build_plan(Engine& e) {
   e.p1r.clear();
   e.p2.clear();
   e.nitems.clear();
+  e.tp1.clear();
+  e.tp1r.clear();
   e.tp2.clear();
-  int32_t pslot = 0;
-  std::vector<P1Item> tp1;
-  std::vector<P1Reduce> tp1r;
-  int32_t tslot = 0;
+  int32_t pslot = 0, tslot = 0;
+  int64_t src_cursor = 0;
   for (int j = 1; j <= nL; ++j) {
     const Level& lv = e.lev[j];
     const int64_t lo = level_offset(j);
+    std::vector<int32_t> nsrc(lv.nboxes, 0);
+    for (int32_t ki : lv.ilidx) nsrc[ki]++;
     for (int64_t k = 0; k < lv.nboxes; ++k) {
       const int n = e.bcount[lo + k];
-      if (n == 0) continue;
-      // forward phase 1 over source box k
-      if (lv.Rs[k] > 0) {
-        const int nsplit = (n + kP1Rows - 1) / kP1Rows;
-        for (int c0 = 0; c0 < lv.Rs[k]; c0 += 64) {
-          const int nc = std::min(64, lv.Rs[k] - c0);
-          if (nsplit > 1) e.p1r.push_back({pslot, nsplit, lv.tcol[k], c0, nc, 0});
+      const int R = lv.Rt[k];
+      if (n > 0 && R > 0) {
+        const int32_t bf = (int32_t)(lv.bbase + lv.iloff[k]);
+        const int32_t bc = lv.iloff[k + 1] - lv.iloff[k];
+        // phase 1: 64 columns at a time; rows up to the largest source box touched
+        for (int c0 = 0; c0 < R; c0 += 64) {
+          const int nc = std::min(64, R - c0);
+          int smax = 0;
+          for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+            const int r = lv.rank[b], cb = lv.col[b];
+            if (r > 0 && cb < c0 + nc && cb + r > c0)
+              smax = std::max(smax, (int)e.bcount[lo + lv.ilidx[b]]);
+          }
+          const int nsplit = std::max(1, (smax + kP1Rows - 1) / kP1Rows);
+          if (nsplit > 1) e.p1r.push_back({pslot, nsplit, lv.toff[k] + c0, nc});
           for (int q = 0; q < nsplit; ++q) {
             P1Item it{};
-            it.voff = lv.voff[k];
-            it.level = j;
-            it.R = lv.Rs[k];
+            it.chunk = lv.chunk[k];
+            it.bfirst = bf;
+            it.bcount = bc;
             it.c0 = c0;
             it.nc = nc;
-            it.xoff = e.bstart[lo + k];
             it.s0 = q * kP1Rows;
-            it.s1 = std::min(n, (q + 1) * kP1Rows);
-            it.tcol = lv.tcol[k];
+            it.s1 = std::min(smax, (q + 1) * kP1Rows);
+            it.toff = lv.toff[k];
             it.pslot = nsplit > 1 ? pslot++ : -1;
             e.p1.push_back(it);
           }
-          // transpose phase 2 over source box k
         }
-        for (int i0 = 0; i0 < n; i0 += 64) {
-          P2Item it{};
-          it.uoff = lv.voff[k];
-          it.ld = lv.Rs[k];
-          it.i0 = i0;
-          it.nr = std::min(64, n - i0);
-          it.R = lv.Rs[k];
-          it.toff = lv.tcol[k];
-          it.yoff = e.bstart[lo + k];
-          it.level = j;
-          e.tp2.push_back(it);
-        }
-      }
-      // forward phase 2 and transpose phase 1 over target box k
-      if (lv.Rt[k] > 0) {
+        // phase 2: rows in chunks of 64
         for (int i0 = 0; i0 < n; i0 += 64) {
           P2Item it{};
           it.uoff = lv.uoff[k];
+          it.chunk = lv.chunk[k];
           it.ld = n;
           it.i0 = i0;
           it.nr = std::min(64, n - i0);
-          it.R = lv.Rt[k];
+          it.R = R;
           it.toff = lv.toff[k];
           it.yoff = e.bstart[lo + k];
           it.level = j;
           e.p2.push_back(it);
         }
+        // transpose phase 1
         const int nsplit = (n + kP1Rows - 1) / kP1Rows;
-        for (int c0 = 0; c0 < lv.Rt[k]; c0 += 64) {
-          const int nc = std::min(64, lv.Rt[k] - c0);
-          if (nsplit > 1) tp1r.push_back({tslot, nsplit, lv.toff[k], c0, nc, 0});
+        for (int c0 = 0; c0 < R; c0 += 64) {
+          const int nc = std::min(64, R - c0);
+          if (nsplit > 1) e.tp1r.push_back({tslot, nsplit, lv.toff[k] + c0, nc});
           for (int q = 0; q < nsplit; ++q) {
-            P1Item it{};
-            it.voff = lv.uoff[k];
-            it.level = j;
-            it.R = n;
+            TP1Item it{};
+            it.uoff = lv.uoff[k];
+            it.chunk = lv.chunk[k];
+            it.ld = n;
             it.c0 = c0;
             it.nc = nc;
-            it.xoff = e.bstart[lo + k];
             it.s0 = q * kP1Rows;
             it.s1 = std::min(n, (q + 1) * kP1Rows);
-            it.tcol = lv.toff[k];
+            it.yoff = e.bstart[lo + k];
+            it.toff = lv.toff[k];
             it.pslot = nsplit > 1 ? tslot++ : -1;
-            tp1.push_back(it);
+            e.tp1.push_back(it);
           }
         }
       }
+    }
+    // transpose phase 2 over source boxes (by_src is grouped by level, then box)
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int m = e.bcount[lo + k];
+      if (m > 0 && nsrc[k] > 0)
+        for (int i0 = 0; i0 < m; i0 += 64)
+          e.tp2.push_back({(int32_t)src_cursor, nsrc[k], e.bstart[lo + k], i0, std::min(64, m - i0), j});
+      src_cursor += nsrc[k];
     }
   }
   {
     const int64_t lo = level_offset(nL);
     for (int64_t k = 0; k < e.lev[nL].nboxes; ++k) {
       const int n = e.bcount[lo + k];
-      for (int i0 = 0; i0 < n; i0 += 64) e.nitems.push_back({(int32_t)k, i0, std::min(64, n - i0), 0});
+      for (int i0 = 0; i0 < n; i0 += 64)
+        e.nitems.push_back({(int32_t)k, i0, std::min(64, n - i0), 0});
     }
   }
-  // Larger items first: less tail at the end of the launch.
+  // Larger items first: less tail at the end of a launch.
   std::stable_sort(e.p2.begin(), e.p2.end(), [](const P2Item& a, const P2Item& b) {
     return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
   });
@@ -481,17 +570,15 @@ void build_plan(Engine& e) {
   dfree(e.d_p1r);
   dfree(e.d_p2);
   dfree(e.d_near);
-  dfree(e.d_tp1x);
+  dfree(e.d_tp1);
   dfree(e.d_tp1r);
   dfree(e.d_tp2);
   e.d_p1 = upload(e.p1, st);
   e.d_p1r = upload(e.p1r, st);
   e.d_p2 = upload(e.p2, st);
   e.d_near = upload(e.nitems, st);
-  e.tp1_items = tp1;
-  e.tp1_red = tp1r;
-  e.d_tp1x = upload(tp1, st);
-  e.d_tp1r = upload(tp1r, st);
+  e.d_tp1 = upload(e.tp1, st);
+  e.d_tp1r = upload(e.tp1r, st);
   e.d_tp2 = upload(e.tp2, st);
   H3D_HIP(hipStreamSynchronize(st));
 }
@@ -513,123 +600,158 @@ void ensure_scratch(Engine& e, int nrhs) {
   e.max_nrhs = nrhs;
 }
 
-void prepare_precision(Engine& e, int prec) {
-  if (prec == H3D_PREC_FP64 || e.fac[prec].bytes > 0) return;
+// Builds the precision-`prec` copy of every chunk from the fp64 copy (or the
+// fp32 one when fp64 is not resident).
+void prepare_precision(Engine& e, int prec, bool move) {
+  if (e.fac_bytes[prec] > 0 || e.chunks.empty()) return;
+  int src = -1;
+  for (int p : {0, 1})
+    if (p != prec && e.fac_bytes[p] > 0) {
+      src = p;
+      break;
+    }
+  if (src < 0) throw Error(H3D_ERR_STATE, "no factor copy to convert from");
   hipStream_t st = e.stream;
   int64_t bytes = 0;
-  for (int j = 1; j <= e.nL; ++j) {
-    for (int side = 0; side < 2; ++side) {
-      const int64_t n = side == 0 ? e.nUlev[j] : e.nVlev[j];
-      const double* src = (const double*)(side == 0 ? e.fac[0].U[j] : e.fac[0].V[j]);
-      void* dst = nullptr;
-      if (n > 0) {
-        if (prec == H3D_PREC_FP32) {
-          float* d = dalloc<float>(n);
-          convert_kernel<float><<<4096, 256, 0, st>>>(src, n, d);
-          dst = d;
-          bytes += n * 4;
-        } else {
-          uint16_t* d = dalloc<uint16_t>(n);
-          convert_kernel<uint16_t><<<4096, 256, 0, st>>>(src, n, d);
-          dst = d;
-          bytes += n * 2;
+  for (Chunk& c : e.chunks) {
+    const int64_t n = c.nU + c.nV;
+    H3D_HIP(hipMalloc(&c.data[prec], (size_t)std::max<int64_t>(n, 1) * elem_size(prec)));
+    if (n > 0) {
+      if (prec == H3D_PREC_FP32)
+        convert_kernel<float><<<2048, 256, 0, st>>>(c.data[src], src, n, (float*)c.data[prec]);
+      else if (prec == H3D_PREC_BF16)
+        convert_kernel<uint16_t><<<2048, 256, 0, st>>>(c.data[src], src, n, (uint16_t*)c.data[prec]);
+      else
+        convert_kernel<double><<<2048, 256, 0, st>>>(c.data[src], src, n, (double*)c.data[prec]);
+      H3D_LAUNCH_CHECK();
+    }
+    bytes += n * elem_size(prec);
+    if (move) {  // free the other copies of this chunk right away (peak = one chunk)
+      H3D_HIP(hipStreamSynchronize(st));
+      for (int p = 0; p < 3; ++p)
+        if (p != prec && c.data[p]) {
+          (void)hipFree(c.data[p]);
+          c.data[p] = nullptr;
         }
-        H3D_LAUNCH_CHECK();
-      }
-      (side == 0 ? e.fac[prec].U[j] : e.fac[prec].V[j]) = dst;
     }
   }
   H3D_HIP(hipStreamSynchronize(st));
-  e.fac[prec].bytes = std::max<int64_t>(bytes, 1);
+  if (move)
+    for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
+  e.fac_bytes[prec] = std::max<int64_t>(bytes, 1);
+  upload_chunk_table(e);
+}
+
+void release_precision(Engine& e, int prec) {
+  int others = 0;
+  for (int p = 0; p < 3; ++p)
+    if (p != prec && e.fac_bytes[p] > 0 && p != 2) ++others;
+  if (e.fac_bytes[prec] > 0 && others == 0 && prec != 2)
+    throw Error(H3D_ERR_STATE, "cannot release the last fp64/fp32 factor copy");
+  H3D_HIP(hipStreamSynchronize(e.stream));
+  for (Chunk& c : e.chunks) {
+    if (c.data[prec]) (void)hipFree(c.data[prec]);
+    c.data[prec] = nullptr;
+  }
+  e.fac_bytes[prec] = 0;
+  upload_chunk_table(e);
 }
 
 namespace {
 
 template <class S, class A>
-void run_forward(Engine& e, const A* x, A* t, A* part, int nitems1, int nitems2) {
+void run_forward(Engine& e, int prec, const A* x, A* t, A* part) {
   hipStream_t st = e.stream;
-  int prec = std::is_same<S, double>::value ? 0 : (std::is_same<S, float>::value ? 1 : 2);
-  LevPtr Vl = levptr(e.fac[prec].V), Ul = levptr(e.fac[prec].U);
-  if (nitems1)
-    phase1_kernel<S, A><<<wave_grid(nitems1), 256, 0, st>>>(e.d_p1, nitems1, Vl, x, e.d_tdst, t,
-                                                            part);
+  const int n1 = (int)e.p1.size(), n2 = (int)e.p2.size();
+  if (e.timing) H3D_HIP(hipEventRecord(e.ev[1], st));
+  if (n1)
+    phase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec], e.d_blk, x, t,
+                                                       part);
   H3D_LAUNCH_CHECK();
   if (!e.p1r.empty())
-    reduce_kernel<A><<<wave_grid(e.p1r.size()), 256, 0, st>>>(e.d_p1r, (int)e.p1r.size(), part,
-                                                              e.d_tdst, t, 0);
+    reduce_kernel<A><<<wave_grid(e.p1r.size()), 256, 0, st>>>(e.d_p1r, (int)e.p1r.size(), part, t);
   H3D_LAUNCH_CHECK();
-  if (nitems2)
-    phase2_kernel<S, A><<<wave_grid(nitems2), 256, 0, st>>>(e.d_p2, nitems2, Ul, t, e.d_ylev, e.N);
+  if (e.timing) H3D_HIP(hipEventRecord(e.ev[2], st));
+  if (n2)
+    phase2_kernel<S, A><<<wave_grid(n2), 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec], t, e.d_ylev,
+                                                       e.N);
   H3D_LAUNCH_CHECK();
 }
 
 template <class S, class A>
-void run_transpose(Engine& e, const A* x, A* t, A* part) {
+void run_transpose(Engine& e, int prec, const A* x, A* t, A* part) {
   hipStream_t st = e.stream;
-  int prec = std::is_same<S, double>::value ? 0 : (std::is_same<S, float>::value ? 1 : 2);
-  LevPtr Vl = levptr(e.fac[prec].V), Ul = levptr(e.fac[prec].U);
-  const int n1 = (int)e.tp1_items.size(), n2 = (int)e.tp2.size();
+  const int n1 = (int)e.tp1.size(), n2 = (int)e.tp2.size();
+  if (e.timing) H3D_HIP(hipEventRecord(e.ev[1], st));
   if (n1)
-    tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(e.d_tp1x, n1, Ul, x, t, part);
+    tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(e.d_tp1, n1, e.d_ubase[prec], x, t, part);
   H3D_LAUNCH_CHECK();
-  if (!e.tp1_red.empty())
-    reduce_kernel<A><<<wave_grid(e.tp1_red.size()), 256, 0, st>>>(
-        e.d_tp1r, (int)e.tp1_red.size(), part, e.d_tdst, t, 1);
+  if (!e.tp1r.empty())
+    reduce_kernel<A><<<wave_grid(e.tp1r.size()), 256, 0, st>>>(e.d_tp1r, (int)e.tp1r.size(), part, t);
   H3D_LAUNCH_CHECK();
+  if (e.timing) H3D_HIP(hipEventRecord(e.ev[2], st));
   if (n2)
-    tphase2_kernel<S, A><<<wave_grid(n2), 256, 0, st>>>(e.d_tp2, n2, Vl, t, e.d_tdst, e.d_ylev, e.N);
+    tphase2_kernel<S, A><<<wave_grid(n2), 256, 0, st>>>(e.d_tp2, n2, e.d_vbase[prec], e.d_blk,
+                                                        e.d_by_src, e.d_btoff, t, e.d_ylev, e.N);
   H3D_LAUNCH_CHECK();
 }
 
 }  // namespace
 
-void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool transpose) {
+void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode, bool transpose) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");
-  if (prec < 0 || prec > 2) throw Error(H3D_ERR_INVALID, "unknown precision");
-  if (prec != H3D_PREC_FP64 && e.fac[prec].bytes == 0) prepare_precision(e, prec);
+  const int prec = mode & 0xff;
+  if (prec < 0 || prec > 2 || (mode & ~(0xff | H3D_ACC_FP64)))
+    throw Error(H3D_ERR_INVALID, "unknown precision");
+  const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
+  if (e.fac_bytes[prec] == 0) prepare_precision(e, prec);
   ensure_scratch(e, 1);
   hipStream_t st = e.stream;
   const int64_t N = e.N;
   const unsigned gN = (unsigned)((N + 255) / 256);
+  if (e.timing && !e.ev[0])
+    for (auto& ev : e.ev) H3D_HIP(hipEventCreate(&ev));
   for (int r = 0; r < nrhs; ++r) {
     const double* b = d_b + (int64_t)r * N;
     double* y = d_y + (int64_t)r * N;
-    gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, prec == 0 ? nullptr : e.d_xf);
+    if (e.timing) H3D_HIP(hipEventRecord(e.ev[0], st));
+    gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, acc64 ? nullptr : e.d_xf);
     H3D_LAUNCH_CHECK();
     H3D_HIP(hipMemsetAsync(e.d_ylev, 0, sizeof(double) * N * e.nL, st));
-    const int n1 = (int)e.p1.size(), n2 = (int)e.p2.size();
-    if (prec == H3D_PREC_FP64) {
-      if (!transpose)
-        run_forward<double, double>(e, e.d_xp, e.d_t, e.d_part, n1, n2);
-      else
-        run_transpose<double, double>(e, e.d_xp, e.d_t, e.d_part);
-    } else if (prec == H3D_PREC_FP32) {
-      float* t = (float*)e.d_t;
-      float* part = (float*)e.d_part;
-      if (!transpose)
-        run_forward<float, float>(e, e.d_xf, t, part, n1, n2);
-      else
-        run_transpose<float, float>(e, e.d_xf, t, part);
+    if (acc64) {
+      if (prec == H3D_PREC_FP64) {
+        if (!transpose) run_forward<double, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+        else run_transpose<double, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+      } else if (prec == H3D_PREC_FP32) {
+        if (!transpose) run_forward<float, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+        else run_transpose<float, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+      } else {
+        if (!transpose) run_forward<uint16_t, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+        else run_transpose<uint16_t, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+      }
     } else {
       float* t = (float*)e.d_t;
       float* part = (float*)e.d_part;
-      if (!transpose)
-        run_forward<uint16_t, float>(e, e.d_xf, t, part, n1, n2);
-      else
-        run_transpose<uint16_t, float>(e, e.d_xf, t, part);
+      if (prec == H3D_PREC_FP32) {
+        if (!transpose) run_forward<float, float>(e, prec, e.d_xf, t, part);
+        else run_transpose<float, float>(e, prec, e.d_xf, t, part);
+      } else {
+        if (!transpose) run_forward<uint16_t, float>(e, prec, e.d_xf, t, part);
+        else run_transpose<uint16_t, float>(e, prec, e.d_xf, t, part);
+      }
     }
+    if (e.timing) H3D_HIP(hipEventRecord(e.ev[3], st));
     // near field into the leaf level (symmetric kernels: the same for A^T)
     const int nn = (int)e.nitems.size();
     double* yL = e.d_ylev + (int64_t)(e.nL - 1) * N;
     const int64_t lo = level_offset(e.nL);
     KernelDev kd = e.kern;
-    kd.diag_vec = e.d_diag;
     if (nn) {
-      if (prec == H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
+      if (acc64 && (kd.q == 7 || kd.q == 16))
         near64_kernel<InvR><<<wave_grid(nn), 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
                                                            e.d_bstart + lo, e.d_bcount + lo, e.d_px,
                                                            e.d_py, e.d_pz, e.d_perm, kd, e.d_xp, yL);
-      else if (prec == H3D_PREC_FP64)
+      else if (acc64)
         near64_kernel<AnyQ><<<wave_grid(nn), 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
                                                            e.d_bstart + lo, e.d_bcount + lo, e.d_px,
                                                            e.d_py, e.d_pz, e.d_perm, kd, e.d_xp, yL);
@@ -640,8 +762,22 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec
                                                      e.d_xf, yL);
       H3D_LAUNCH_CHECK();
     }
-    combine_kernel<<<gN, 256, 0, st>>>(e.d_ylev, e.nL, N, e.d_iperm, y);
+    if (e.timing) H3D_HIP(hipEventRecord(e.ev[4], st));
+    combine_kernel<<<gN, 256, 0, st>>>(e.d_ylev, e.nL, N, e.d_iperm, b, e.alpha, e.kern.diag,
+                                       e.d_diag, y);
     H3D_LAUNCH_CHECK();
+    if (e.timing) {
+      H3D_HIP(hipEventRecord(e.ev[5], st));
+      H3D_HIP(hipEventSynchronize(e.ev[5]));
+      float ms[5];
+      H3D_HIP(hipEventElapsedTime(&ms[0], e.ev[1], e.ev[2]));  // phase 1 (+ partial reduce)
+      H3D_HIP(hipEventElapsedTime(&ms[1], e.ev[2], e.ev[3]));  // phase 2
+      H3D_HIP(hipEventElapsedTime(&ms[2], e.ev[3], e.ev[4]));  // near field
+      H3D_HIP(hipEventElapsedTime(&ms[3], e.ev[4], e.ev[5]));  // combine
+      H3D_HIP(hipEventElapsedTime(&ms[4], e.ev[0], e.ev[5]));  // whole matvec
+      for (int q = 0; q < 5; ++q) e.ktime[q] += ms[q];
+      e.kcount++;
+    }
   }
 }
 

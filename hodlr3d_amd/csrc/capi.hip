@@ -33,24 +33,27 @@ void Engine::free_all() {
     dfree(lv.d_iloff);
     dfree(lv.d_ilidx);
     dfree(lv.d_nbr);
-    dfree(lv.d_pivI);
-    dfree(lv.d_pivJ);
   }
   lev.clear();
-  for (Factors& f : fac) {
-    for (int j = 0; j < 8; ++j) {
-      if (f.U[j]) (void)hipFree(f.U[j]);
-      if (f.V[j]) (void)hipFree(f.V[j]);
-      f.U[j] = f.V[j] = nullptr;
-    }
-    f.bytes = 0;
+  for (Chunk& c : chunks) {
+    for (int p = 0; p < 3; ++p)
+      if (c.data[p]) (void)hipFree(c.data[p]);
+    dfree(c.piv);
   }
-  dfree(d_tdst);
+  chunks.clear();
+  for (int p = 0; p < 3; ++p) {
+    dfree(d_ubase[p]);
+    dfree(d_vbase[p]);
+    fac_bytes[p] = 0;
+  }
+  dfree(d_blk);
+  dfree(d_by_src);
+  dfree(d_btoff);
   dfree(d_p1);
   dfree(d_p1r);
   dfree(d_p2);
   dfree(d_near);
-  dfree(d_tp1x);
+  dfree(d_tp1);
   dfree(d_tp1r);
   dfree(d_tp2);
   dfree(d_xp);
@@ -59,6 +62,8 @@ void Engine::free_all() {
   dfree(d_part);
   dfree(d_ylev);
   dfree(d_io);
+  for (auto& ev : this->ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (own_stream) (void)hipStreamDestroy(own_stream);
   own_stream = nullptr;
 }
@@ -176,6 +181,8 @@ int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
         e.ctr[1] = opt->center[1];
         e.ctr[2] = opt->center[2];
         e.ws_budget = opt->workspace_bytes;
+        if (opt->store_mask & ~7) throw h3d::Error(H3D_ERR_INVALID, "bad store_mask");
+        e.store_mask = opt->store_mask ? opt->store_mask : 1;
       }
       e.tol_pow = tol_pow;
       e.tol = std::pow(10, -1.0 * tol_pow);  // src/ACA.cpp:5
@@ -222,18 +229,42 @@ int h3d_prepare_precision(h3d_handle* h, int prec) {
   });
 }
 
+int h3d_move_precision(h3d_handle* h, int prec) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (prec < 0 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    h3d::prepare_precision(h->e, prec, true);
+  });
+}
+
+int h3d_set_operator(h3d_handle* h, double alpha, double diag) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (!std::isfinite(alpha) || !std::isfinite(diag)) throw h3d::Error(H3D_ERR_INVALID, "non-finite");
+    h->e.alpha = alpha;
+    h->e.kern.diag = diag;
+  });
+}
+
+int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h3d::Engine& e = h->e;
+    if (ms)
+      for (int q = 0; q < 5; ++q) ms[q] = e.ktime[q];
+    if (count) *count = e.kcount;
+    for (int q = 0; q < 5; ++q) e.ktime[q] = 0;
+    e.kcount = 0;
+    e.timing = enable != 0;
+  });
+}
+
 int h3d_release_precision(h3d_handle* h, int prec) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    if (prec < 1 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "only fp32/bf16 copies can be released");
-    h3d::Factors& f = h->e.fac[prec];
-    H3D_HIP(hipStreamSynchronize(h->e.stream));
-    for (int j = 0; j < 8; ++j) {
-      if (f.U[j]) (void)hipFree(f.U[j]);
-      if (f.V[j]) (void)hipFree(f.V[j]);
-      f.U[j] = f.V[j] = nullptr;
-    }
-    f.bytes = 0;
+    if (prec < 0 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    h3d::release_precision(h->e, prec);
   });
 }
 
@@ -321,7 +352,8 @@ int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max
   int st = H3D_OK;
   int rc = guarded([&] {
     if (!h || !b || !x) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
-    if (inner_prec < 0 || inner_prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if ((inner_prec & 0xff) > 2 || (inner_prec & ~(0xff | H3D_ACC_FP64)))
+      throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
     h3d::Engine& e = h->e;
     if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "solve before assemble");
     const int64_t N = e.N;
@@ -345,6 +377,32 @@ int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max
   });
   if (rc == H3D_OK && st != H3D_OK) g_last_error = "refinement did not reach the tolerance";
   return rc != H3D_OK ? rc : st;
+}
+
+int h3d_direct(h3d_handle* h, const double* b, const int64_t* rows, int64_t nrows, double* out,
+               int where) {
+  return guarded([&] {
+    if (!h || !b || (!rows && nrows > 0) || (!out && nrows > 0))
+      throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (nrows < 0) throw h3d::Error(H3D_ERR_INVALID, "nrows < 0");
+    h3d::Engine& e = h->e;
+    if (nrows == 0) return;
+    if (where == H3D_DEVICE) {
+      h3d::direct_rows(e, b, rows, nrows, out);
+      return;
+    }
+    double* db = h3d::dalloc<double>(e.N + nrows);
+    try {
+      H3D_HIP(hipMemcpyAsync(db, b, sizeof(double) * e.N, hipMemcpyHostToDevice, e.stream));
+      h3d::direct_rows(e, db, rows, nrows, db + e.N);
+      H3D_HIP(hipMemcpyAsync(out, db + e.N, sizeof(double) * nrows, hipMemcpyDeviceToHost, e.stream));
+      H3D_HIP(hipStreamSynchronize(e.stream));
+    } catch (...) {
+      h3d::dfree(db);
+      throw;
+    }
+    h3d::dfree(db);
+  });
 }
 
 int h3d_export_perm(h3d_handle* h, int32_t* perm) {
@@ -408,23 +466,25 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
     if (b < 0 || b >= (int64_t)lv.rank.size()) throw h3d::Error(H3D_ERR_INVALID, "block out of range");
     const int r = lv.rank[b];
     if (r == 0) return;
-    if (I) H3D_HIP(hipMemcpy(I, lv.d_pivI + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
-    if (J) H3D_HIP(hipMemcpy(J, lv.d_pivJ + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
     // locate the target box of block b
     int64_t k = std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) - lv.iloff.begin() - 1;
     const int64_t lo = h3d::level_offset(level);
     const int32_t ki = lv.ilidx[b];
     const int n = e.bcount[lo + k], m = e.bcount[lo + ki];
+    const h3d::Chunk& ch = e.chunks[lv.chunk[k]];
+    if (I) H3D_HIP(hipMemcpy(I, ch.piv + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
+    if (J) H3D_HIP(hipMemcpy(J, ch.piv + lv.pivoff[b] + r, sizeof(int32_t) * r, hipMemcpyDeviceToHost));
+    if ((U || V) && !ch.data[0]) throw h3d::Error(H3D_ERR_STATE, "fp64 factors not resident");
     if (U) {
-      const double* src = (const double*)e.fac[0].U[level] + lv.uoff[k] + (int64_t)lv.bcol_t[b] * n;
+      const double* src = (const double*)ch.data[0] + lv.uoff[k] + (int64_t)lv.col[b] * n;
       H3D_HIP(hipMemcpy(U, src, sizeof(double) * (int64_t)r * n, hipMemcpyDeviceToHost));
     }
     if (V) {
-      std::vector<double> rows((size_t)m * lv.Rs[ki]);
-      const double* src = (const double*)e.fac[0].V[level] + lv.voff[ki];
+      std::vector<double> rows((size_t)m * r);
+      const double* src = (const double*)ch.data[0] + ch.nU + lv.voff[b];
       H3D_HIP(hipMemcpy(rows.data(), src, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
       for (int l = 0; l < r; ++l)
-        for (int s = 0; s < m; ++s) V[(int64_t)l * m + s] = rows[(size_t)s * lv.Rs[ki] + lv.bcol_s[b] + l];
+        for (int s = 0; s < m; ++s) V[(int64_t)l * m + s] = rows[(size_t)s * r + l];
     }
   });
 }
@@ -444,7 +504,9 @@ int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
     s->near_pairs = e.near_pairs;
     s->t_tree_ms = e.t_tree_ms;
     s->t_assemble_ms = e.t_assemble_ms;
-    for (int p = 0; p < 3; ++p) s->factor_bytes[p] = e.fac[p].bytes;
+    for (int p = 0; p < 3; ++p) s->factor_bytes[p] = e.fac_bytes[p];
+    s->u_elems = e.u_elems;
+    s->v_elems = e.v_elems;
   });
 }
 

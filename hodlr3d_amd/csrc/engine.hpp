@@ -6,12 +6,15 @@
 //   perm      position -> original index; iperm its inverse
 //   bstart,bcount per level (level j at offset (8^j-1)/7): each box is the
 //             contiguous position segment [bstart, bstart+bcount)
-//   U (per level, per target box k): column-major n_k x R_k with ld = n_k,
-//             the u-vectors of the box's blocks in interaction-list order
-//   V (per level, per source box s): row-major m_s x Rp_s (Rp_s = R_s padded
-//             to a multiple of 4), the v-vectors of every block whose source
-//             is s; tdst maps each of these columns to its slot in t
-//   t         one value per (block, rank) in target order
+//   factors   stored in chunks; one chunk holds every block of a run of
+//             consecutive target boxes of one level:
+//               U region: per target box k, column-major n_k x R_k (ld = n_k),
+//                         the u-vectors of its blocks in interaction-list order
+//               V region: per block, row-major m_b x r_b (the v-vectors)
+//             Each stored precision has its own copy of every chunk with the
+//             same element offsets.
+//   t         one value per (block, rank) in target order: box k's blocks
+//             occupy t[toff_k, toff_k + R_k)
 #pragma once
 
 #include <vector>
@@ -27,36 +30,68 @@ struct KernelDev {
   const double* diag_vec = nullptr;  // device, original order (or null)
 };
 
-// Far-field phase 1 item: t = V^T x over one column chunk of one source box.
+// Per-block data read by the phase-1 lanes (global block id = level base + CSR index).
+struct BlockDev {
+  int64_t voff;   // element offset of V_b in its chunk's V region
+  int32_t col;    // first column of the block in its target box's concatenation
+  int32_t r;      // rank
+  int32_t m;      // rows of V_b (source box size)
+  int32_t src;    // position of the source box's first point
+  int32_t chunk;  // chunk holding the block
+  int32_t pad;
+};
+
+// Far-field phase 1 item: t[toff + c0 + lane] = V^T x for 64 consecutive
+// columns of one target box (may span several blocks).
 struct P1Item {
-  int64_t voff;   // element offset (within the level's V) of row 0, column 0
-  int32_t level;  // 1..nLevels
-  int32_t R;      // row stride (padded columns)
-  int32_t c0;     // first column of this chunk
-  int32_t nc;     // columns in this chunk (<= 64)
-  int32_t xoff;   // position of the box's row 0
-  int32_t s0, s1; // row range handled by this item
-  int32_t tcol;   // tdst offset of the box's column 0
-  int32_t pslot;  // -1: write t; otherwise partial-sum slot (64 values)
+  int32_t chunk;
+  int32_t bfirst, bcount;  // global block ids of the box
+  int32_t c0, nc;          // column range within the box
+  int32_t s0, s1;          // row range
+  int32_t toff;            // t offset of the box
+  int32_t pslot;           // -1: write t; otherwise partial-sum slot (64 values)
   int32_t pad;
 };
 
 // Partial-sum reduction for a column chunk split over several row ranges.
 struct P1Reduce {
-  int32_t pbase, nsplit, tcol, c0, nc, pad;
+  int32_t pbase, nsplit, toff, nc;
 };
 
 // Far-field phase 2 item: y_level[rows] = U t over one row chunk of a target box.
 struct P2Item {
-  int64_t uoff;  // element offset (within the level's U) of row 0, column 0
-  int32_t ld;    // column stride (= box size)
-  int32_t i0;    // first row of this chunk
-  int32_t nr;    // rows in this chunk (<= 64)
-  int32_t R;     // columns
-  int32_t toff;  // offset of the box's t values
-  int32_t yoff;  // position of the box's row 0
-  int32_t level; // 1..nLevels
+  int64_t uoff;   // element offset of the box's U in its chunk's U region
+  int32_t chunk;
+  int32_t ld;     // column stride (= box size)
+  int32_t i0;     // first row of this chunk
+  int32_t nr;     // rows in this chunk (<= 64)
+  int32_t R;      // columns
+  int32_t toff;   // offset of the box's t values
+  int32_t yoff;   // position of the box's row 0
+  int32_t level;  // 1..nLevels
+};
+
+// Transpose phase 1 item: columns [c0, c0+nc) of a target box's U against
+// rows [s0, s1) of x.
+struct TP1Item {
+  int64_t uoff;
+  int32_t chunk;
+  int32_t ld;
+  int32_t c0, nc;
+  int32_t s0, s1;
+  int32_t yoff;
+  int32_t toff;
+  int32_t pslot;
   int32_t pad;
+};
+
+// Transpose phase 2 item: rows [i0, i0+nr) of one source box gather
+// V_b t_b from every block whose source is that box.
+struct TP2Item {
+  int32_t sfirst, scount;  // range in the by-source block list
+  int32_t yoff;            // position of the source box's row 0
+  int32_t i0, nr;
+  int32_t level;
 };
 
 // Near-field item: one row chunk of a leaf box against itself and its
@@ -70,29 +105,27 @@ struct NearItem {
 
 struct Level {
   int64_t nboxes = 0;
+  int64_t bbase = 0;                  // global id of the level's first block
   std::vector<int32_t> iloff, ilidx;  // host copies of the interaction lists
   int32_t* d_iloff = nullptr;
   int32_t* d_ilidx = nullptr;
   int32_t* d_nbr = nullptr;           // 26 slots per box
-  // ACA results, one entry per block (CSR order)
-  std::vector<int32_t> rank, status;
-  std::vector<int64_t> pivoff;        // offset of the block's pivots
-  int32_t* d_pivI = nullptr;
-  int32_t* d_pivJ = nullptr;
-  // factor layout
-  std::vector<int32_t> Rt;            // columns per target box
-  std::vector<int32_t> Rs;            // padded columns per source box
-  std::vector<int64_t> uoff, voff;    // element offsets per box (global arrays)
-  std::vector<int32_t> toff;          // t offset per target box (global)
-  std::vector<int32_t> tcol;          // tdst offset per source box (global)
-  std::vector<int32_t> bcol_t, bcol_s;  // per block: first column in target / source concat
+  std::vector<int32_t> rank, status;  // per block (CSR order)
+  std::vector<int32_t> col;           // per block: first column in its target box
+  std::vector<int64_t> voff;          // per block: V offset in its chunk
+  std::vector<int64_t> pivoff;        // per block: pivot offset in its chunk
+  std::vector<int32_t> Rt;            // per target box: columns
+  std::vector<int32_t> toff;          // per target box: t offset (global)
+  std::vector<int32_t> chunk;         // per target box: chunk id
+  std::vector<int64_t> uoff;          // per target box: U offset in its chunk
 };
 
-// Factors of one precision: one U and one V array per level (index 1..nL).
-struct Factors {
-  void* U[8] = {};
-  void* V[8] = {};
-  int64_t bytes = 0;
+// One chunk of factors: per precision a U region followed by a V region.
+struct Chunk {
+  void* data[3] = {nullptr, nullptr, nullptr};
+  int64_t nU = 0, nV = 0;  // elements
+  int32_t* piv = nullptr;  // pivots: for each block I then J (r each)
+  int64_t npiv = 0;
 };
 
 struct Engine {
@@ -104,6 +137,7 @@ struct Engine {
   int tol_pow = 7;
   double tol = 1e-7;
   int64_t ws_budget = 0;
+  int store_mask = 1;  // precisions written at assembly (bit p = precision p)
   KernelDev kern;
   std::vector<double> radius;  // boxRadius[j]
 
@@ -124,42 +158,54 @@ struct Engine {
   std::vector<Level> lev;
 
   bool assembled = false;
-  int64_t nU = 0, nV = 0;        // elements of U / V (all levels)
-  int64_t nUlev[8] = {}, nVlev[8] = {};
+  int64_t nblk_all = 0;
   int32_t T = 0;                 // total t length
-  int32_t* d_tdst = nullptr;     // per V column (all levels, padded): t index or -1
-  Factors fac[3];
+  std::vector<Chunk> chunks;
+  void** d_ubase[3] = {nullptr, nullptr, nullptr};  // per precision: U base of every chunk
+  void** d_vbase[3] = {nullptr, nullptr, nullptr};  // per precision: V base of every chunk
+  int64_t fac_bytes[3] = {0, 0, 0};
+  BlockDev* d_blk = nullptr;     // all blocks
+  std::vector<int32_t> by_src;   // global block ids grouped by source box
+  int32_t* d_by_src = nullptr;
+  int32_t* d_btoff = nullptr;    // per block: global t index of its first column
 
   // matvec plan
   std::vector<P1Item> p1;
   std::vector<P1Reduce> p1r;
   std::vector<P2Item> p2;
   std::vector<NearItem> nitems;
+  std::vector<TP1Item> tp1;
+  std::vector<P1Reduce> tp1r;
+  std::vector<TP2Item> tp2;
   int32_t npartial = 0;
   P1Item* d_p1 = nullptr;
   P1Reduce* d_p1r = nullptr;
   P2Item* d_p2 = nullptr;
   NearItem* d_near = nullptr;
-  // transpose plan (phase 1 over target boxes, phase 2 over source boxes)
-  std::vector<P1Item> tp1_items;
-  std::vector<P1Reduce> tp1_red;
-  std::vector<P2Item> tp2;
-  P1Item* d_tp1x = nullptr;
+  TP1Item* d_tp1 = nullptr;
   P1Reduce* d_tp1r = nullptr;
-  P2Item* d_tp2 = nullptr;
+  TP2Item* d_tp2 = nullptr;
 
   // scratch for one matvec
   int max_nrhs = 0;
-  double* d_xp = nullptr;     // x by position (nrhs)
+  double* d_xp = nullptr;     // x by position
   float* d_xf = nullptr;      // fp32 copy of d_xp
-  double* d_t = nullptr;      // t (nrhs), fp64 or reinterpreted as fp32
+  double* d_t = nullptr;      // t, fp64 or reinterpreted as fp32
   double* d_part = nullptr;   // phase-1 partial sums
-  double* d_ylev = nullptr;   // per-level y by position: nL * N (* nrhs)
+  double* d_ylev = nullptr;   // per-level y by position: nL * N
   double* d_io = nullptr;     // staging for host vectors
+
+  // apply-time operator: y = alpha * (off-diagonal part) + diag * b
+  double alpha = 1.0;
+  // per-kernel event timing of matvecs
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double ktime[5] = {};
+  int kcount = 0;
 
   // stats
   int max_rank = 0;
-  int64_t n_blocks = 0, rank_sum = 0, factor_elems = 0;
+  int64_t n_blocks = 0, rank_sum = 0, factor_elems = 0, u_elems = 0, v_elems = 0;
   double find_memory = 0;
   int64_t near_pairs = 0;
   double t_tree_ms = 0, t_assemble_ms = 0;
@@ -169,6 +215,7 @@ struct Engine {
 };
 
 inline int64_t level_offset(int j) { return ((int64_t(1) << (3 * j)) - 1) / 7; }
+inline int elem_size(int prec) { return prec == 0 ? 8 : (prec == 1 ? 4 : 2); }
 
 // tree.hip
 void build_tree(Engine& e, const double* h_xyz);
@@ -177,8 +224,11 @@ void assemble_all(Engine& e);
 // apply.hip
 void build_plan(Engine& e);
 void ensure_scratch(Engine& e, int nrhs);
-void prepare_precision(Engine& e, int prec);
+void prepare_precision(Engine& e, int prec, bool move = false);
+void release_precision(Engine& e, int prec);
+void upload_chunk_table(Engine& e);
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool transpose);
+void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);

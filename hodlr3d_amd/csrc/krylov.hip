@@ -240,10 +240,13 @@ int solve_ir(Engine& e, const double* d_b, double* d_x, int restart, int max_inn
   const double bnorm = v.get(0);
   if (!(bnorm > 0)) {
     H3D_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, e.stream));
-    if (info) *info = h3d_solve_info{0, 0, 0.0, {}};
+    if (info) *info = h3d_solve_info{};
     dfree(Q), dfree(Ax), dfree(r), dfree(rr), dfree(d);
     return 0;
   }
+  // Outer residual: fp64 factors when resident, otherwise fp64 arithmetic over
+  // the fp32-stored factors.
+  const int outer_prec = e.fac_bytes[0] > 0 ? H3D_PREC_FP64 : (H3D_PREC_FP32 | H3D_ACC_FP64);
   H3D_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, e.stream));
   H3D_HIP(hipMemcpyAsync(r, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, e.stream));
   double rel = 1.0;
@@ -281,7 +284,7 @@ int solve_ir(Engine& e, const double* d_b, double* d_x, int restart, int max_inn
       H3D_HIP(hipStreamSynchronize(e.stream));
       dfree(d_one);
     }
-    matvec_device(e, d_x, Ax, 1, H3D_PREC_FP64, transpose);
+    matvec_device(e, d_x, Ax, 1, outer_prec, transpose);
     v.sub(r, d_b, Ax);
     v.dot(r, r, 0, 1);
     rel = v.get(0) / bnorm;
@@ -289,6 +292,7 @@ int solve_ir(Engine& e, const double* d_b, double* d_x, int restart, int max_inn
     ++outer;
   }
   loc.outer_iterations = outer;
+  loc.outer_prec = outer_prec;
   loc.inner_iterations = inner_total;
   loc.rel_residual = rel;
   if (info) *info = loc;

@@ -24,8 +24,15 @@ PRECISIONS = {"fp64": nat.PREC_FP64, "fp32": nat.PREC_FP32, "bf16": nat.PREC_BF1
 
 
 def _prec(p):
+    """'fp64' | 'fp32' | 'bf16', optionally with '/acc64' (fp64 accumulation)."""
     if isinstance(p, str):
-        return PRECISIONS[p]
+        base, _, acc = p.partition("/")
+        v = PRECISIONS[base]
+        if acc:
+            if acc != "acc64":
+                raise ValueError(f"unknown accumulation {acc!r}")
+            v |= nat.ACC_FP64
+        return v
     return int(p)
 
 
@@ -80,7 +87,7 @@ class HODLR3D:
     """The reference facade (src/HODLR3D.cpp:3-27) on the MI355X engine."""
 
     def __init__(self, nParticlesInLeafAlong1D, L, TOL_POW, Qchoice=7, particles=None, *,
-                 kernel=None, center=None, nlevels=None, workspace_bytes=0):
+                 kernel=None, center=None, nlevels=None, workspace_bytes=0, store=("fp64",)):
         if particles is None:
             raise ValueError("particles are required")
         xyz = as_xyz(particles)
@@ -95,6 +102,7 @@ class HODLR3D:
             opt.center[i] = float(c[i])
         opt.nlevels = -1 if nlevels is None else int(nlevels)
         opt.workspace_bytes = int(workspace_bytes)
+        opt.store_mask = sum(1 << PRECISIONS[p] for p in store)
         h = ctypes.c_void_p()
         self._h = None
         nat.check(nat.lib().h3d_create(xyz.ctypes.data_as(ctypes.c_void_p), self.N,
@@ -177,6 +185,22 @@ class HODLR3D:
     def release_precision(self, prec):
         nat.check(nat.lib().h3d_release_precision(self._h, _prec(prec)))
 
+    def move_precision(self, prec):
+        """Convert the factors to `prec` chunk by chunk, freeing the other copies."""
+        nat.check(nat.lib().h3d_move_precision(self._h, _prec(prec)))
+
+    def set_operator(self, alpha=1.0, diag=0.0):
+        """Apply-time operator y = alpha * K~ b + diag * b (per-point diagonal if set)."""
+        nat.check(nat.lib().h3d_set_operator(self._h, float(alpha), float(diag)))
+
+    def kernel_timing(self, enable=True):
+        """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
+        ms = (ctypes.c_double * 5)()
+        cnt = ctypes.c_int()
+        nat.check(nat.lib().h3d_kernel_timing(self._h, int(enable), ms, ctypes.byref(cnt)))
+        return {"phase1": ms[0], "phase2": ms[1], "near": ms[2], "combine": ms[3],
+                "total": ms[4], "count": cnt.value}
+
     def set_diag(self, c):
         if _is_torch_cuda(c):
             import torch
@@ -219,10 +243,33 @@ class HODLR3D:
                                         inner_tol, _prec(inner_prec), tol, max_outer,
                                         int(transpose), ctypes.byref(info), nat.HOST)
         d = {"outer_iterations": info.outer_iterations, "inner_iterations": info.inner_iterations,
+             "outer_prec": info.outer_prec,
              "rel_residual": info.rel_residual,
              "history": [info.history[i] for i in range(min(info.outer_iterations, 64))]}
         nat.check(rc)
         return x, d
+
+    def direct(self, b, rows):
+        """Direct FP64 sum (A b)[rows] from kernel entries (compensated)."""
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        if _is_torch_cuda(b):
+            import torch
+            bt = b.detach().to(torch.float64).contiguous()
+            out = torch.empty(len(r), dtype=torch.float64, device=bt.device)
+            self._bind_stream()
+            try:
+                nat.check(nat.lib().h3d_direct(self._h, ctypes.c_void_p(bt.data_ptr()),
+                                               r.ctypes.data_as(ctypes.c_void_p), len(r),
+                                               ctypes.c_void_p(out.data_ptr()), nat.DEVICE))
+            finally:
+                nat.lib().h3d_set_stream(self._h, None)
+            return out
+        a = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty(len(r))
+        nat.check(nat.lib().h3d_direct(self._h, a.ctypes.data_as(ctypes.c_void_p),
+                                       r.ctypes.data_as(ctypes.c_void_p), len(r),
+                                       out.ctypes.data_as(ctypes.c_void_p), nat.HOST))
+        return out
 
     def stats(self):
         s = nat.Stats()
@@ -231,7 +278,12 @@ class HODLR3D:
                 "rank_sum": s.rank_sum, "factor_elems": s.factor_elems,
                 "find_memory": s.find_memory, "near_pairs": s.near_pairs,
                 "t_tree_ms": s.t_tree_ms, "t_assemble_ms": s.t_assemble_ms,
-                "factor_bytes": list(s.factor_bytes)}
+                "factor_bytes": list(s.factor_bytes), "u_elems": s.u_elems,
+                "v_elems": s.v_elems}
+
+    def geometry_stats(self):
+        """Sizes that set the algorithmic bytes of a matvec."""
+        return self.stats()
 
     # -- structure export (bit-exact checks)
     def perm(self):

@@ -32,6 +32,9 @@ extern "C" {
 #define H3D_PREC_FP64 0  /* fp64 factors, fp64 accumulation, fp64 near field */
 #define H3D_PREC_FP32 1  /* fp32 factors, fp32 accumulation, fp32 near field */
 #define H3D_PREC_BF16 2  /* bf16 factors, fp32 accumulation, fp32 near field */
+/* Flag or-ed into a precision: accumulate in fp64 (fp64 near field) over
+ * fp32/bf16-stored factors. */
+#define H3D_ACC_FP64 0x100
 
 /* ---- pointer locations ---- */
 #define H3D_HOST 0
@@ -57,6 +60,8 @@ typedef struct h3d_options {
   double center[3];        /* root cube centre; the reference uses the origin */
   int nlevels;             /* < 0: the reference formula (src/HODLR3D.cpp:6-7) */
   int64_t workspace_bytes; /* device workspace budget for assembly, 0 = automatic */
+  int store_mask;          /* precisions written by h3d_assemble, bit p = H3D_PREC_p;
+                              0 means fp64 only */
 } h3d_options;
 
 typedef struct h3d_stats {
@@ -71,11 +76,14 @@ typedef struct h3d_stats {
   double t_tree_ms;        /* octree, bucketing, lists (device, wall) */
   double t_assemble_ms;    /* ACA + factor layout (device, wall) */
   int64_t factor_bytes[3]; /* resident factor bytes per precision (0 if absent) */
+  int64_t u_elems;         /* sum over blocks of r*n (U entries, read by phase 2) */
+  int64_t v_elems;         /* sum over blocks of r*m (V entries, read by phase 1) */
 } h3d_stats;
 
 typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
   int inner_iterations;    /* total inner GMRES iterations */
+  int outer_prec;          /* precision mode of the outer residual operator */
   double rel_residual;     /* final ||b - A x|| / ||b|| with the fp64 operator */
   double history[64];      /* relative residual after each refinement step */
 } h3d_solve_info;
@@ -94,8 +102,12 @@ int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
  * factors stored in fp64. */
 int h3d_assemble(h3d_handle* h);
 
-/* Builds (or frees) the fp32 / bf16 copies of the fp64 factors. */
+/* Builds a copy of the factors in `prec` (from the fp64 copy, else the fp32
+ * one), or frees one; the last fp64/fp32 copy cannot be freed. */
 int h3d_prepare_precision(h3d_handle* h, int prec);
+/* Like h3d_prepare_precision, but frees every other copy chunk by chunk, so the
+ * peak is one chunk above the larger of the two copies. */
+int h3d_move_precision(h3d_handle* h, int prec);
 int h3d_release_precision(h3d_handle* h, int prec);
 
 /* Replaces HODLR3D::MatVecProduct (src/HODLR3D.cpp:21-27): y = A b for nrhs
@@ -107,8 +119,20 @@ int h3d_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int prec, in
 int h3d_matvec_transpose(h3d_handle* h, const double* b, double* y, int nrhs, int prec,
                          int where);
 
-/* Replaces the per-point diagonal after creation (the self-block entries). */
+/* Replaces the per-point diagonal K(i,i) after creation (NULL: back to the
+ * kernel's scalar diagonal). */
 int h3d_set_diag(h3d_handle* h, const double* diag_vec, int where);
+
+/* Apply-time operator: y = alpha * (assembled off-diagonal operator) b + d b,
+ * d = the per-point diagonal if set, else `diag`.  Defaults: alpha = 1 and the
+ * kernel's diagonal.  Lets one assembly of K serve A = diag + alpha K. */
+int h3d_set_operator(h3d_handle* h, double alpha, double diag);
+
+/* Per-kernel timing of subsequent matvecs with HIP events on the handle's
+ * stream (enable != 0).  ms[5] receives the accumulated milliseconds of
+ * phase 1 (V^T x), phase 2 (U t), near field, combine and whole matvecs since
+ * the last read, *count the number of matvecs; both are then reset. */
+int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count);
 
 /* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,
  * modified Gram-Schmidt, Givens rotations.  hist receives error values (its+1
@@ -117,11 +141,19 @@ int h3d_gmres(h3d_handle* h, const double* b, double* x, int maxit, double thres
               double* error, int* iterations, double* hist, int where);
 
 /* GMRES(restart) with the `inner_prec` operator inside fp64 iterative
- * refinement: r = b - A x with fp64 factors, inner solve to `inner_tol`,
- * x += d, until ||r||/||b|| <= tol.  transpose != 0 solves A^T x = b. */
+ * refinement: r = b - A x in fp64 (fp64 factors if resident, else fp64
+ * arithmetic over the fp32 factors), inner solve to `inner_tol`, x += d, until
+ * ||r||/||b|| <= tol.  transpose != 0 solves A^T x = b. */
 int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max_inner,
                  double inner_tol, int inner_prec, double tol, int max_outer, int transpose,
                  h3d_solve_info* info, int where);
+
+/* Direct O(N) FP64 evaluation of rows of the same operator (every entry from
+ * the kernel, compensated summation): out[r] = (A b)[rows[r]].  rows are
+ * original indices in host memory; b and out follow `where`.  The reference's
+ * drivers form exact products from kernel columns (examples/testHODLR3D.cpp:139-143). */
+int h3d_direct(h3d_handle* h, const double* b, const int64_t* rows, int64_t nrows, double* out,
+               int where);
 
 /* Structure export for bit-exact checks (the reference keeps these in
  * HODLR3DBox, include/HODLR3DTree.hpp:11-24). */

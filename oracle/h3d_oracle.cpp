@@ -1032,4 +1032,36 @@ double orc_entry(int qchoice, double scale, double diag, const double* a, const 
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
+// Wall seconds of the reference-semantics far-field apply (entry regeneration
+// + BLAS-2, src/HODLR3DTree.cpp:1660-1706) over the assembled target boxes of
+// one level; q in original order.  Used to time a bounded sample.
+double orc_time_far_level(void* p, int level, const double* q) {
+  Handle* h = (Handle*)p;
+  std::vector<double> qp(h->N), pot(h->N, 0.0);
+  for (int64_t s = 0; s < h->N; ++s) qp[s] = q[h->perm[s]];
+  std::vector<int32_t> boxes;
+  for (size_t k = 0; k < h->assembled[level].size(); ++k)
+    if (h->assembled[level][k]) boxes.push_back((int32_t)k);
+  double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t t = 0; t < (int64_t)boxes.size(); ++t) {
+    double tg = 0, tm = 0;
+    int32_t k = boxes[t];
+    for (int32_t b = h->iloff[level][k]; b < h->iloff[level][k + 1]; ++b)
+      far_block_apply(*h, level, k, h->blocks[level][b], qp.data(), pot.data(), tg, tm);
+  }
+  return omp_get_wtime() - t0;
+}
+
+// Wall seconds of the near field (:1708-1755) over the leaves in leafmask.
+double orc_time_near(void* p, const double* q, const uint8_t* leafmask) {
+  Handle* h = (Handle*)p;
+  std::vector<double> qp(h->N), pot(h->N, 0.0);
+  for (int64_t s = 0; s < h->N; ++s) qp[s] = q[h->perm[s]];
+  double tg = 0, tm = 0;
+  double t0 = omp_get_wtime();
+  near_apply(*h, qp.data(), pot.data(), leafmask, tg, tm);
+  return omp_get_wtime() - t0;
+}
+
 }  // extern "C"

@@ -58,6 +58,8 @@ def lib():
             "orc_direct_rows": (None, [_p, _i64, ctypes.c_int, _d, _d, _p, _p, _i64, _p, _p]),
             "orc_entry": (_d, [ctypes.c_int, _d, _d, _p, _p, _i64, _i64]),
             "orc_max_threads": (ctypes.c_int, []),
+            "orc_time_far_level": (_d, [_p, ctypes.c_int, _p]),
+            "orc_time_near": (_d, [_p, _p, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -69,6 +71,10 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def max_threads():
+    return lib().orc_max_threads()
 
 
 def nlevels(N, leaf1d):
@@ -198,6 +204,15 @@ class Oracle:
         m = ctypes.c_double()
         lib().orc_timers(self.h, ctypes.byref(a), ctypes.byref(m))
         return a.value, m.value
+
+    def time_far_level(self, level, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        return lib().orc_time_far_level(self.h, level, _ptr(q))
+
+    def time_near(self, q, leafmask):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lm = np.ascontiguousarray(leafmask, dtype=np.uint8)
+        return lib().orc_time_near(self.h, _ptr(q), _ptr(lm))
 
     def find_memory(self):
         return lib().orc_find_memory(self.h)

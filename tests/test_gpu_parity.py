@@ -1,0 +1,127 @@
+"""HIP path vs the CPU oracle: structure bit-exact, ACA ranks/pivots/factors
+bit-exact, fp64 matvec within 1e-12, transposes, precisions (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import lattice, uniform
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (points, leaf1d, tol_pow)
+    "C1-lattice16": (lambda: lattice(16), 4, 10),
+    "uniform-32k": (lambda: uniform(32768, 11), 4, 8),
+    "octant-6k": (lambda: uniform(6000, 12, 0.02, 0.98), 4, 9),  # many empty boxes
+}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def pair(request):
+    import hodlr3d_amd as H3
+    mk, leaf, tol = CASES[request.param]
+    x = mk()
+    print(f"[case {request.param}] N={len(x)}", flush=True)
+    h = H3.HODLR3D(leaf, 1.0, tol, 7, x)
+    h.assemble()
+    o = O.Oracle(x, leaf, tol_pow=tol, keep_uv=len(x) <= 8000)
+    o.assemble()
+    yield request.param, x, h, o
+    h.close()
+
+
+def test_tree_bit_exact(pair):
+    name, x, h, o = pair
+    assert h.nLevels == o.nLevels
+    assert np.array_equal(h.perm(), o.perm())
+    for j in range(1, h.nLevels + 1):
+        s, c = h.boxes(j)
+        so, co, _ = o.boxes(j)
+        assert np.array_equal(s, so) and np.array_equal(c, co)
+        a, b = h.il(j), o.il(j)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(h.neighbors(j), o.neighbors(j))
+
+
+def test_ranks_and_pivots_bit_exact(pair):
+    name, x, h, o = pair
+    for j in range(1, h.nLevels + 1):
+        rg, _ = h.ranks(j)
+        ro, _ = o.ranks(j)
+        assert np.array_equal(rg, ro), (name, j, int((rg != ro).sum()))
+        rng = np.random.default_rng(j)
+        for b in rng.choice(len(rg), min(len(rg), 200), replace=False):
+            if rg[b] == 0:
+                continue
+            Ig, Jg, _, _ = h.block(j, b, rg[b])
+            Io, Jo, _, _, _, _ = o.block(j, b, ro[b])
+            assert np.array_equal(Ig, Io) and np.array_equal(Jg, Jo), (name, j, b)
+
+
+def test_factors_bit_exact(pair):
+    name, x, h, o = pair
+    if len(x) > 8000:
+        pytest.skip("oracle keeps ACA vectors only for small cases")
+    s, c, _ = o.boxes(1)
+    for j in range(1, h.nLevels + 1):
+        off, idx = o.il(j)
+        s, c, _ = o.boxes(j)
+        rg, _ = h.ranks(j)
+        for k in np.random.default_rng(j).choice(8 ** j, min(8 ** j, 20), replace=False):
+            for b in range(off[k], off[k + 1], 5):
+                r = int(rg[b])
+                if r == 0:
+                    continue
+                n, m = int(c[k]), int(c[idx[b]])
+                _, _, Ug, Vg = h.block(j, b, r, n, m, uv=True)
+                _, _, _, _, Uo, Vo = o.block(j, b, r, n, m, uv=True)
+                assert np.array_equal(Ug, Uo) and np.array_equal(Vg, Vo), (name, j, b)
+
+
+def test_fp64_matvec_matches_oracle(pair):
+    name, x, h, o = pair
+    q = np.random.default_rng(5).standard_normal(len(x))
+    yg = h.MatVecProduct(q)
+    yo = o.matvec(q, mode=0)
+    assert np.linalg.norm(yg - yo) / np.linalg.norm(yo) < 1e-12
+
+
+def test_direct_sum_matches_oracle(pair):
+    name, x, h, o = pair
+    q = np.random.default_rng(6).standard_normal(len(x))
+    rows = np.random.default_rng(7).choice(len(x), 64, replace=False)
+    assert np.allclose(h.direct(q, rows), O.direct_rows(x, q, rows), rtol=1e-13, atol=1e-300)
+
+
+def test_transpose_is_the_adjoint(pair):
+    name, x, h, o = pair
+    rng = np.random.default_rng(8)
+    a, b = rng.standard_normal(len(x)), rng.standard_normal(len(x))
+    for prec, tol in (("fp64", 1e-12), ("fp32/acc64", 1e-12), ("fp32", 1e-5)):
+        lhs = b @ h.matvec(a, prec=prec)
+        rhs = a @ h.matvec_transpose(b, prec=prec)
+        assert abs(lhs - rhs) <= tol * np.linalg.norm(a) * np.linalg.norm(b) * 10
+
+
+def test_reduced_precision_errors(pair):
+    name, x, h, o = pair
+    q = np.random.default_rng(9).standard_normal(len(x))
+    y = h.matvec(q)
+    ny = np.linalg.norm(y)
+    e32 = np.linalg.norm(h.matvec(q, prec="fp32") - y) / ny
+    e32a = np.linalg.norm(h.matvec(q, prec="fp32/acc64") - y) / ny
+    e16 = np.linalg.norm(h.matvec(q, prec="bf16") - y) / ny
+    print(f"[{name}] fp32 {e32:.2e} fp32/acc64 {e32a:.2e} bf16 {e16:.2e}")
+    assert e32a < 2e-7 and e32 < 3e-6 and e16 < 3e-2
+
+
+def test_nrhs_and_device_tensors(pair):
+    import torch
+    name, x, h, o = pair
+    rng = np.random.default_rng(10)
+    B = rng.standard_normal((len(x), 3))
+    Y = h.matvec(B)
+    for i in range(3):
+        assert np.array_equal(Y[:, i], h.matvec(B[:, i]))
+    Yd = h.matvec(torch.from_numpy(B).cuda()).cpu().numpy()
+    assert np.array_equal(Yd, Y)
