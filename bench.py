@@ -162,6 +162,8 @@ def main():
 
     def measure(prec):
         out = {}
+        out["autotune"] = H.autotune(prec, reps=3)
+        log(f"[bench] autotune {prec}: {out['autotune']}")
         y = H.matvec(q, prec=prec)
         out["rel_err"] = rel_err(y)
         H.kernel_timing(True)

@@ -64,6 +64,8 @@ SIGNATURES = {
     "h3d_move_precision": (_i32, [_p, _i32]),
     "h3d_set_operator": (_i32, [_p, _d, _d]),
     "h3d_kernel_timing": (_i32, [_p, _i32, _p, ctypes.POINTER(_i32)]),
+    "h3d_autotune": (_i32, [_p, _i32, _i32, _p]),
+    "h3d_set_tuning": (_i32, [_p, _i32, _i32, _i32]),
     "h3d_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_matvec_transpose": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_set_diag": (_i32, [_p, _p, _i32]),
@@ -81,6 +83,7 @@ SIGNATURES = {
     "h3d_export_block": (_i32, [_p, _i32, _i64, _p, _p, _p, _p]),
     "h3d_get_stats": (_i32, [_p, ctypes.POINTER(Stats)]),
     "h3d_set_stream": (_i32, [_p, _p]),
+    "h3d_use_own_stream": (_i32, [_p]),
     "h3d_synchronize": (_i32, [_p]),
     "h3d_destroy": (None, [_p]),
     "h3d_status_string": (ctypes.c_char_p, [_i32]),
@@ -96,9 +99,20 @@ class H3DError(RuntimeError):
         self.code = code
 
 
+def _prefer_torch_runtime():
+    """PyTorch-ROCm ships its own libamdhip64.so.7.  When torch is present it must
+    be loaded first, so that this library binds to the same HIP runtime (one
+    runtime per process); otherwise the system runtime from /opt/rocm is used."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
+        _prefer_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `make -C hodlr3d_amd/csrc` "

@@ -424,21 +424,24 @@ __global__ void write_kernel(const WriteDesc* __restrict__ ds, const double* __r
                              uint16_t* __restrict__ U16, uint16_t* __restrict__ V16,
                              int32_t* __restrict__ piv) {
   const WriteDesc d = ds[blockIdx.x];
+  const int ld = d.n + (d.n & 1), ldv = d.m + (d.m & 1);
   const int64_t nu = (int64_t)d.r * d.n;
   for (int64_t q = threadIdx.x; q < nu; q += blockDim.x) {
+    const int64_t l = q / d.n;
+    const int64_t o = d.uo + l * ld + (q - l * d.n);
     const double v = Uws[d.uw + q];
-    if (U64) U64[d.uo + q] = v;
-    if (U32) U32[d.uo + q] = (float)v;
-    if (U16) U16[d.uo + q] = f_to_bf16((float)v);
+    if (U64) U64[o] = v;
+    if (U32) U32[o] = (float)v;
+    if (U16) U16[o] = f_to_bf16((float)v);
   }
   const int64_t nv = (int64_t)d.r * d.m;
   for (int64_t q = threadIdx.x; q < nv; q += blockDim.x) {
-    const int64_t s = q / d.r;
-    const int l = (int)(q - s * d.r);
-    const double v = Vws[d.vw + (int64_t)l * d.m + s];
-    if (V64) V64[d.vo + q] = v;
-    if (V32) V32[d.vo + q] = (float)v;
-    if (V16) V16[d.vo + q] = f_to_bf16((float)v);
+    const int64_t l = q / d.m;
+    const int64_t o = d.vo + l * ldv + (q - l * d.m);
+    const double v = Vws[d.vw + q];
+    if (V64) V64[o] = v;
+    if (V32) V32[o] = (float)v;
+    if (V16) V16[o] = f_to_bf16((float)v);
   }
   for (int l = threadIdx.x; l < d.r; l += blockDim.x) {
     piv[d.po + l] = pivws[d.pw + l];
@@ -466,10 +469,13 @@ struct Pass {
   }
 };
 
+// Rank capacity of the ACA workspace: a block that would need more stops
+// with status 2 and is run again from scratch with the next capacity.
 int cap_for(int pass, int n, int m) {
   const int mn = std::min(n, m);
-  const bool big = std::max(n, m) > 256;
-  int c = pass == 0 ? (big ? 128 : 96) : (pass == 1 ? 512 : mn);
+  const int big = std::max(n, m);
+  const int first = big > 32768 ? 256 : (big > 256 ? 128 : 96);
+  int c = pass == 0 ? first : (pass == 1 ? 512 : mn);
   return std::min(c, mn);
 }
 
@@ -649,7 +655,7 @@ void assemble_all(Engine& e) {
           lv.col[b] = R;
           R += r;
           lv.voff[b] = ch.nV;
-          ch.nV += (int64_t)r * m;
+          ch.nV += (int64_t)r * (m + (m & 1));
           lv.pivoff[b] = ch.npiv;
           ch.npiv += 2 * (int64_t)r;
           e.max_rank = std::max(e.max_rank, r);
@@ -661,13 +667,14 @@ void assemble_all(Engine& e) {
         }
         lv.Rt[k] = R;
         lv.uoff[k] = ch.nU;
-        ch.nU += (int64_t)n * R;
+        ch.nU += (int64_t)(n + (n & 1)) * R;
         lv.chunk[k] = (int32_t)e.chunks.size();
       }
       for (int p = 0; p < 3; ++p) {
         if (!(e.store_mask & (1 << p))) continue;
-        const int64_t el = std::max<int64_t>(ch.nU + ch.nV, 1);
-        H3D_HIP(hipMalloc(&ch.data[p], (size_t)el * elem_size(p)));
+        const size_t bytes = (size_t)(ch.nU + ch.nV + kChunkTail) * elem_size(p);
+        H3D_HIP(hipMalloc(&ch.data[p], bytes));
+        H3D_HIP(hipMemsetAsync(ch.data[p], 0, bytes, st));
         e.fac_bytes[p] += (ch.nU + ch.nV) * elem_size(p);
       }
       ch.piv = dalloc<int32_t>(std::max<int64_t>(ch.npiv, 1));
@@ -687,7 +694,7 @@ void assemble_all(Engine& e) {
           w.pw = P.ds[q].pw;
           w.rcap = P.ds[q].rcap;
           w.pad = 0;
-          w.uo = lv.uoff[k] + (int64_t)lv.col[b] * w.n;
+          w.uo = lv.uoff[k] + (int64_t)lv.col[b] * (w.n + (w.n & 1));
           w.vo = lv.voff[b];
           w.po = lv.pivoff[b];
           wd.push_back(w);

@@ -23,7 +23,9 @@ namespace h3d {
 
 namespace {
 
-constexpr int kP1Rows = 1024;  // rows of one phase-1 item before splitting
+constexpr int kP1Rows = 512;          // rows of a phase-1 item (64 lanes x 8 rows)
+constexpr int64_t kP1Elems = 8192;    // factor entries per multi-block phase-1 item
+constexpr int kP2Rows = 128;          // rows of one phase-2 item
 constexpr int kWavesPerBlock = 4;
 
 template <class S>
@@ -59,51 +61,115 @@ __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __res
   if (xf) xf[s] = (float)v;
 }
 
-// ---------------------------------------------------------------- phase 1
+// Vector loads of 2..16 bytes (4-byte aligned from 4 bytes up), converted to
+// the accumulation type.  Factor streams carry the non-temporal hint.
+typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x2a __attribute__((ext_vector_type(2), aligned(4)));
+
 template <class S, class A>
+__device__ inline A cvt_elem(uint32_t bits, int half) {
+  if constexpr (sizeof(S) == 2) return (A)__uint_as_float(half ? (bits & 0xffff0000u) : (bits << 16));
+  else return (A)__uint_as_float(bits);
+}
+
+// NV consecutive stored elements at p (NV * sizeof(S) in {2,4,8,16} bytes)
+template <class S, class A, int NV>
+__device__ inline void load_factor(const S* p, A (&o)[NV]) {
+  constexpr int B = NV * (int)sizeof(S);
+  if constexpr (sizeof(S) == 8) {
+    if constexpr (NV == 2) {
+      u32x4a v = __builtin_nontemporal_load((const u32x4a*)p);
+      o[0] = (A)__hiloint2double((int)v.y, (int)v.x);
+      o[1] = (A)__hiloint2double((int)v.w, (int)v.z);
+    } else {
+      o[0] = (A)__builtin_nontemporal_load(p);
+    }
+  } else if constexpr (B == 16) {
+    u32x4a v = __builtin_nontemporal_load((const u32x4a*)p);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
+  } else if constexpr (B == 8) {
+    u32x2a v = __builtin_nontemporal_load((const u32x2a*)p);
+    const uint32_t w[2] = {v.x, v.y};
+#pragma unroll
+    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
+  } else if constexpr (B == 4) {
+    const uint32_t v = __builtin_nontemporal_load((const uint32_t*)p);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(v, q & 1);
+  } else {
+    o[0] = (A)bf16_to_f(__builtin_nontemporal_load((const uint16_t*)p));
+  }
+}
+
+// ---------------------------------------------------------------- phase 1
+// One wave walks the blocks [b0, b1) of a target box (consecutive in memory).
+// For each block V_b (column-major, ldv = m + (m & 1)) lpc lanes share a
+// column: each lane holds up to 8 rows of x in registers (loaded once per
+// block) and reads VEC consecutive rows of the column per load; the lanes'
+// partial sums are combined with an xor butterfly inside the group.  Rows past
+// the item's range see x = 0, so loads may run into padding or the next column.
+template <class S, class A, int VEC>
 __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ items, int nitems,
                                                      void* const* __restrict__ vbase,
                                                      const BlockDev* __restrict__ blk,
                                                      const A* __restrict__ x, A* __restrict__ t,
                                                      A* __restrict__ part) {
+  constexpr int KMAX = 8 / VEC > 0 ? 8 / VEC : 1;  // rows per lane <= 8 (items have <= 512 rows)
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const P1Item it = items[w];
-  const bool active = lane < it.nc;
-  const int c = it.c0 + (active ? lane : 0);
-  // last block of the box whose first column is <= c
-  int lo = it.bfirst, hi = it.bfirst + it.bcount - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (blk[mid].col <= c) lo = mid; else hi = mid - 1;
-  }
-  const BlockDev bd = blk[lo];
-  const int r = bd.r;
-  const S* vp = (const S*)vbase[it.chunk] + bd.voff + (c - bd.col);
-  const A* xp = x + bd.src;
-  const int se = min(it.s1, bd.m);
-  A acc = 0;
-  int s = it.s0;
-  for (; s + 8 <= se; s += 8) {
-    A v[8], xv[8];
+  const S* V = (const S*)vbase[it.chunk];
+  for (int b = it.b0; b < it.b1; ++b) {
+    const BlockDev bd = blk[b];
+    const int s1 = min(it.s1, bd.m);
+    const int rows = s1 - it.s0;
+    if (bd.r == 0 || rows <= 0) continue;
+    const int ldv = bd.m + (bd.m & 1);
+    const int need = (rows + VEC - 1) / VEC;
+    int lpc = 1;
+    while (lpc < need && lpc < 64) lpc <<= 1;
+    const int K = (need + lpc - 1) / lpc;
+    const int cpp = 64 / lpc, slot = lane / lpc, sub = lane % lpc;
+    A xv[KMAX][VEC];
+    const A* xs = x + bd.src + it.s0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      v[q] = (A)Load<S>::f(vp + (int64_t)(s + q) * r);
-      xv[q] = xp[s + q];
+    for (int k = 0; k < KMAX; ++k) {
+      const int s = (sub + k * lpc) * VEC;
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) xv[k][q] = (k < K && s + q < rows) ? xs[s + q] : (A)0;
     }
+    const S* base = V + bd.voff + it.s0;
+    for (int c0 = 0; c0 < bd.r; c0 += cpp) {
+      const int l = c0 + slot;
+      A acc = 0;
+      if (l < bd.r) {
+        const S* col = base + (int64_t)l * ldv;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc += v[q] * xv[q];
+        for (int k = 0; k < KMAX; ++k) {
+          if (k < K) {
+            A v[VEC];
+            load_factor<S, A, VEC>(col + (sub + k * lpc) * VEC, v);
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) acc += v[q] * xv[k][q];
+          }
+        }
+      }
+      for (int d = lpc >> 1; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+      if (l < bd.r && sub == 0) {
+        if (it.pslot < 0)
+          t[it.toff + bd.col + l] = acc;
+        else
+          part[(int64_t)it.pslot + l] = acc;
+      }
+    }
   }
-  for (; s < se; ++s) acc += (A)Load<S>::f(vp + (int64_t)s * r) * xp[s];
-  if (!active) return;
-  if (it.pslot < 0)
-    t[it.toff + c] = acc;
-  else
-    part[(int64_t)it.pslot * 64 + lane] = acc;
 }
 
-// Sums the row-split partials of a column chunk in row order into t[toff + lane].
+// Sums the row-split partials of one block (nsplit x nc values, row ranges in
+// order) into t[toff + l].
 template <class A>
 __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ part,
                               A* __restrict__ t) {
@@ -111,58 +177,123 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
   const P1Reduce r = rs[w];
-  if (lane >= r.nc) return;
-  A acc = 0;
-  for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)(r.pbase + q) * 64 + lane];
-  t[r.toff + lane] = acc;
+  for (int l = lane; l < r.nc; l += 64) {
+    A acc = 0;
+    for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
+    t[r.toff + l] = acc;
+  }
 }
 
 // ---------------------------------------------------------------- phase 2
-template <class S, class A>
+// U of a target box is column-major with ld = n + (n & 1).  A lane owns VR
+// consecutive rows (one vector load per column); the wave's lanes are split
+// into G groups that walk different columns, so one wave instruction reads G
+// consecutive columns of the box (contiguous when the item spans the box).
+template <class S, class A, int VR>
 __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ items, int nitems,
                                                      void* const* __restrict__ ubase,
                                                      const A* __restrict__ t,
                                                      double* __restrict__ ylev, int64_t N) {
-  __shared__ A red[kWavesPerBlock][64];
+  __shared__ A red[kWavesPerBlock][64 * VR];
   const int wv = threadIdx.x >> 6;
   const int w = blockIdx.x * kWavesPerBlock + wv;
   const int lane = threadIdx.x & 63;
   const bool valid = w < nitems;
   P2Item it;
   if (valid) it = items[w];
-  int G = 1, RB = 1;
-  A acc = 0;
-  if (valid) {
-    RB = it.nr;
-    G = 64 / RB;
-    const int row = lane % RB, g = lane / RB;
+  int G = 1, rcn = 1;
+  A acc[VR];
+#pragma unroll
+  for (int q = 0; q < VR; ++q) acc[q] = 0;
+  if (valid) rcn = (it.nr + VR - 1) / VR;
+  // more row chunks than lanes (narrow loads): lanes take 64-chunk passes, one column at a time
+  const bool multi = valid && rcn > 64;
+  if (multi) {
+    const S* U0 = (const S*)ubase[it.chunk] + it.uoff + it.i0;
+    const A* tt = t + it.toff;
+    const int64_t ld = it.ld;
+    for (int p0 = 0; p0 < rcn; p0 += 64) {
+      const int rho = p0 + lane;
+      if (rho >= rcn) break;
+      const S* U = U0 + rho * VR;
+      A a[VR];
+#pragma unroll
+      for (int q = 0; q < VR; ++q) a[q] = 0;
+      int c = 0;
+      for (; c + 3 < it.R; c += 4) {
+        A v0[VR], v1[VR], v2[VR], v3[VR];
+        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
+        load_factor<S, A, VR>(U + (int64_t)(c + 1) * ld, v1);
+        load_factor<S, A, VR>(U + (int64_t)(c + 2) * ld, v2);
+        load_factor<S, A, VR>(U + (int64_t)(c + 3) * ld, v3);
+#pragma unroll
+        for (int q = 0; q < VR; ++q) {
+          a[q] += v0[q] * tt[c];
+          a[q] += v1[q] * tt[c + 1];
+          a[q] += v2[q] * tt[c + 2];
+          a[q] += v3[q] * tt[c + 3];
+        }
+      }
+      for (; c < it.R; ++c) {
+        A v0[VR];
+        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
+#pragma unroll
+        for (int q = 0; q < VR; ++q) a[q] += v0[q] * tt[c];
+      }
+#pragma unroll
+      for (int q = 0; q < VR; ++q) {
+        const int row = rho * VR + q;
+        if (row < it.nr) ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + row] = (double)a[q];
+      }
+    }
+  } else if (valid) {
+    G = 64 / rcn;
+    const int rho = lane % rcn, g = lane / rcn;
     if (g < G) {
-      const S* U = (const S*)ubase[it.chunk] + it.uoff + it.i0 + row;
+      const S* U = (const S*)ubase[it.chunk] + it.uoff + it.i0 + rho * VR;
       const A* tt = t + it.toff;
       const int64_t ld = it.ld;
       int c = g;
       for (; c + 3 * G < it.R; c += 4 * G) {
-        A v0 = (A)Load<S>::f(U + (int64_t)c * ld);
-        A v1 = (A)Load<S>::f(U + (int64_t)(c + G) * ld);
-        A v2 = (A)Load<S>::f(U + (int64_t)(c + 2 * G) * ld);
-        A v3 = (A)Load<S>::f(U + (int64_t)(c + 3 * G) * ld);
-        acc += v0 * tt[c];
-        acc += v1 * tt[c + G];
-        acc += v2 * tt[c + 2 * G];
-        acc += v3 * tt[c + 3 * G];
+        A v0[VR], v1[VR], v2[VR], v3[VR];
+        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
+        load_factor<S, A, VR>(U + (int64_t)(c + G) * ld, v1);
+        load_factor<S, A, VR>(U + (int64_t)(c + 2 * G) * ld, v2);
+        load_factor<S, A, VR>(U + (int64_t)(c + 3 * G) * ld, v3);
+        const A t0 = tt[c], t1 = tt[c + G], t2 = tt[c + 2 * G], t3 = tt[c + 3 * G];
+#pragma unroll
+        for (int q = 0; q < VR; ++q) {
+          acc[q] += v0[q] * t0;
+          acc[q] += v1[q] * t1;
+          acc[q] += v2[q] * t2;
+          acc[q] += v3[q] * t3;
+        }
       }
-      for (; c < it.R; c += G) acc += (A)Load<S>::f(U + (int64_t)c * ld) * tt[c];
+      for (; c < it.R; c += G) {
+        A v0[VR];
+        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
+        const A t0 = tt[c];
+#pragma unroll
+        for (int q = 0; q < VR; ++q) acc[q] += v0[q] * t0;
+      }
     }
   }
-  red[wv][lane] = acc;
+  if (G > 1) {
+#pragma unroll
+    for (int q = 0; q < VR; ++q) red[wv][lane * VR + q] = acc[q];
+  }
   __syncthreads();
-  if (!valid) return;
-  if (lane < RB) {
-    A s = 0;
-    for (int g = 0; g < G; ++g) s += red[wv][g * RB + lane];
-    ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + lane] = (double)s;
+  if (!valid || multi || lane >= rcn) return;
+#pragma unroll
+  for (int q = 0; q < VR; ++q) {
+    const int row = lane * VR + q;
+    if (row >= it.nr) break;
+    A s = acc[q];
+    for (int g = 1; g < G; ++g) s += red[wv][(g * rcn + lane) * VR + q];
+    ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + row] = (double)s;
   }
 }
+
 
 // --------------------------------------------------------- transpose phases
 // t[toff + c] = sum_i U[c*ld + i] x[yoff + i] for the item's columns: lanes
@@ -193,7 +324,7 @@ __global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict_
   if (it.pslot < 0)
     t[it.toff + it.c0 + lane] = mine;
   else
-    part[(int64_t)it.pslot * 64 + lane] = mine;
+    part[(int64_t)it.pslot + lane] = mine;
 }
 
 // y[s] = sum over blocks b with this source box of sum_l V_b[s, l] t[btoff_b + l]:
@@ -217,9 +348,10 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
     const int b = by_src[it.sfirst + q];
     const BlockDev bd = blk[b];
     if (bd.r == 0) continue;
-    const S* row = (const S*)vbase[bd.chunk] + bd.voff + (int64_t)s * bd.r;
+    const int64_t ldv = bd.m + (bd.m & 1);
+    const S* col = (const S*)vbase[bd.chunk] + bd.voff + s;
     const A* tt = t + btoff[b];
-    for (int l = 0; l < bd.r; ++l) acc += (A)Load<S>::f(row + l) * tt[l];
+    for (int l = 0; l < bd.r; ++l) acc += (A)Load<S>::f(col + l * ldv) * tt[l];
   }
   if (act) ylev[(int64_t)(it.level - 1) * N + it.yoff + s] = (double)acc;
 }
@@ -481,41 +613,63 @@ void build_plan(Engine& e) {
       const int n = e.bcount[lo + k];
       const int R = lv.Rt[k];
       if (n > 0 && R > 0) {
-        const int32_t bf = (int32_t)(lv.bbase + lv.iloff[k]);
-        const int32_t bc = lv.iloff[k + 1] - lv.iloff[k];
-        // phase 1: 64 columns at a time; rows up to the largest source box touched
-        for (int c0 = 0; c0 < R; c0 += 64) {
-          const int nc = std::min(64, R - c0);
-          int smax = 0;
+        // phase 1: runs of consecutive blocks up to kP1Elems factor entries;
+        // a block with more than kP1Rows rows gets one item per row range
+        // (partial sums reduced afterwards in range order)
+        {
+          int32_t g0 = -1;
+          int64_t gel = 0;
+          auto flush = [&](int32_t end) {
+            if (g0 >= 0) {
+              P1Item it{};
+              it.chunk = lv.chunk[k];
+              it.b0 = (int32_t)lv.bbase + g0;
+              it.b1 = (int32_t)lv.bbase + end;
+              it.s0 = 0;
+              it.s1 = kP1Rows;
+              it.toff = lv.toff[k];
+              it.pslot = -1;
+              e.p1.push_back(it);
+            }
+            g0 = -1;
+            gel = 0;
+          };
           for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
-            const int r = lv.rank[b], cb = lv.col[b];
-            if (r > 0 && cb < c0 + nc && cb + r > c0)
-              smax = std::max(smax, (int)e.bcount[lo + lv.ilidx[b]]);
+            const int r = lv.rank[b];
+            const int m = e.bcount[lo + lv.ilidx[b]];
+            if (r == 0) continue;
+            if (m > kP1Rows) {
+              flush(b);
+              const int nsplit = (m + kP1Rows - 1) / kP1Rows;
+              e.p1r.push_back({pslot, nsplit, lv.toff[k] + lv.col[b], r});
+              for (int q = 0; q < nsplit; ++q) {
+                P1Item it{};
+                it.chunk = lv.chunk[k];
+                it.b0 = (int32_t)lv.bbase + b;
+                it.b1 = it.b0 + 1;
+                it.s0 = q * kP1Rows;
+                it.s1 = std::min(m, (q + 1) * kP1Rows);
+                it.toff = lv.toff[k];
+                it.pslot = pslot;
+                pslot += r;
+                e.p1.push_back(it);
+              }
+              continue;
+            }
+            if (g0 >= 0 && gel + (int64_t)r * m > kP1Elems) flush(b);
+            if (g0 < 0) g0 = b;
+            gel += (int64_t)r * m;
           }
-          const int nsplit = std::max(1, (smax + kP1Rows - 1) / kP1Rows);
-          if (nsplit > 1) e.p1r.push_back({pslot, nsplit, lv.toff[k] + c0, nc});
-          for (int q = 0; q < nsplit; ++q) {
-            P1Item it{};
-            it.chunk = lv.chunk[k];
-            it.bfirst = bf;
-            it.bcount = bc;
-            it.c0 = c0;
-            it.nc = nc;
-            it.s0 = q * kP1Rows;
-            it.s1 = std::min(smax, (q + 1) * kP1Rows);
-            it.toff = lv.toff[k];
-            it.pslot = nsplit > 1 ? pslot++ : -1;
-            e.p1.push_back(it);
-          }
+          flush(lv.iloff[k + 1]);
         }
-        // phase 2: rows in chunks of 64
-        for (int i0 = 0; i0 < n; i0 += 64) {
+        // phase 2: rows in chunks of 128 (64 lanes x 2 fp64 rows at most)
+        for (int i0 = 0; i0 < n; i0 += kP2Rows) {
           P2Item it{};
           it.uoff = lv.uoff[k];
           it.chunk = lv.chunk[k];
-          it.ld = n;
+          it.ld = n + (n & 1);
           it.i0 = i0;
-          it.nr = std::min(64, n - i0);
+          it.nr = std::min(kP2Rows, n - i0);
           it.R = R;
           it.toff = lv.toff[k];
           it.yoff = e.bstart[lo + k];
@@ -531,14 +685,15 @@ void build_plan(Engine& e) {
             TP1Item it{};
             it.uoff = lv.uoff[k];
             it.chunk = lv.chunk[k];
-            it.ld = n;
+            it.ld = n + (n & 1);
             it.c0 = c0;
             it.nc = nc;
             it.s0 = q * kP1Rows;
             it.s1 = std::min(n, (q + 1) * kP1Rows);
             it.yoff = e.bstart[lo + k];
             it.toff = lv.toff[k];
-            it.pslot = nsplit > 1 ? tslot++ : -1;
+            it.pslot = nsplit > 1 ? tslot : -1;
+            if (nsplit > 1) tslot += nc;
             e.tp1.push_back(it);
           }
         }
@@ -591,10 +746,13 @@ void ensure_scratch(Engine& e, int nrhs) {
   dfree(e.d_ylev);
   dfree(e.d_io);
   dfree(e.d_xf);
-  e.d_xp = dalloc<double>(e.N * nrhs);
-  e.d_xf = dalloc<float>(e.N * nrhs);
+  // x is read in 16-byte vectors that may run up to 7 elements past a box
+  e.d_xp = dalloc<double>(e.N * nrhs + 64);
+  e.d_xf = dalloc<float>(e.N * nrhs + 64);
+  H3D_HIP(hipMemsetAsync(e.d_xp, 0, sizeof(double) * (e.N * nrhs + 64), e.stream));
+  H3D_HIP(hipMemsetAsync(e.d_xf, 0, sizeof(float) * (e.N * nrhs + 64), e.stream));
   e.d_t = dalloc<double>(std::max<int64_t>((int64_t)e.T * nrhs, 1));
-  e.d_part = dalloc<double>(std::max<int64_t>((int64_t)e.npartial * 64 * nrhs, 1));
+  e.d_part = dalloc<double>(std::max<int64_t>((int64_t)e.npartial * nrhs, 1));
   e.d_ylev = dalloc<double>(e.N * e.nL * nrhs);
   e.d_io = dalloc<double>(2 * e.N * nrhs);
   e.max_nrhs = nrhs;
@@ -614,8 +772,8 @@ void prepare_precision(Engine& e, int prec, bool move) {
   hipStream_t st = e.stream;
   int64_t bytes = 0;
   for (Chunk& c : e.chunks) {
-    const int64_t n = c.nU + c.nV;
-    H3D_HIP(hipMalloc(&c.data[prec], (size_t)std::max<int64_t>(n, 1) * elem_size(prec)));
+    const int64_t n = c.nU + c.nV + kChunkTail;
+    H3D_HIP(hipMalloc(&c.data[prec], (size_t)n * elem_size(prec)));
     if (n > 0) {
       if (prec == H3D_PREC_FP32)
         convert_kernel<float><<<2048, 256, 0, st>>>(c.data[src], src, n, (float*)c.data[prec]);
@@ -625,7 +783,7 @@ void prepare_precision(Engine& e, int prec, bool move) {
         convert_kernel<double><<<2048, 256, 0, st>>>(c.data[src], src, n, (double*)c.data[prec]);
       H3D_LAUNCH_CHECK();
     }
-    bytes += n * elem_size(prec);
+    bytes += (c.nU + c.nV) * elem_size(prec);
     if (move) {  // free the other copies of this chunk right away (peak = one chunk)
       H3D_HIP(hipStreamSynchronize(st));
       for (int p = 0; p < 3; ++p)
@@ -659,23 +817,108 @@ void release_precision(Engine& e, int prec) {
 
 namespace {
 
+// Phase-1 / phase-2 launch with a load width of `bytes` per lane.
+template <class S, class A>
+void launch_p1(Engine& e, int prec, int bytes, const A* x, A* t, A* part) {
+  const int n1 = (int)e.p1.size();
+  if (!n1) return;
+  const unsigned g = wave_grid(n1);
+  hipStream_t st = e.stream;
+  constexpr int es = (int)sizeof(S);
+  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
+    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec], e.d_blk, x, t, part);
+  else if (bytes == 8 && 8 / es >= 1)
+    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
+                                                                      e.d_blk, x, t, part);
+  else if (bytes == 4 && 4 / es >= 1)
+    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
+                                                                      e.d_blk, x, t, part);
+  else
+    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
+                                                                      e.d_blk, x, t, part);
+  H3D_LAUNCH_CHECK();
+}
+
+template <class S, class A>
+void launch_p2(Engine& e, int prec, int bytes, const A* t) {
+  const int n2 = (int)e.p2.size();
+  if (!n2) return;
+  const unsigned g = wave_grid(n2);
+  hipStream_t st = e.stream;
+  constexpr int es = (int)sizeof(S);
+  if (bytes >= 16)
+    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec], t, e.d_ylev, e.N);
+  else if (bytes == 8)
+    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
+                                                                      t, e.d_ylev, e.N);
+  else if (bytes == 4)
+    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
+                                                                      t, e.d_ylev, e.N);
+  else
+    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
+                                                                      t, e.d_ylev, e.N);
+  H3D_LAUNCH_CHECK();
+}
+
+inline int tune_key(int prec, bool acc64) { return prec + (acc64 && prec != 0 ? 3 : 0); }
+
 template <class S, class A>
 void run_forward(Engine& e, int prec, const A* x, A* t, A* part) {
   hipStream_t st = e.stream;
-  const int n1 = (int)e.p1.size(), n2 = (int)e.p2.size();
+  const int key = tune_key(prec, sizeof(A) == 8);
   if (e.timing) H3D_HIP(hipEventRecord(e.ev[1], st));
-  if (n1)
-    phase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec], e.d_blk, x, t,
-                                                       part);
-  H3D_LAUNCH_CHECK();
+  launch_p1<S, A>(e, prec, e.tune_p1[key], x, t, part);
   if (!e.p1r.empty())
     reduce_kernel<A><<<wave_grid(e.p1r.size()), 256, 0, st>>>(e.d_p1r, (int)e.p1r.size(), part, t);
   H3D_LAUNCH_CHECK();
   if (e.timing) H3D_HIP(hipEventRecord(e.ev[2], st));
-  if (n2)
-    phase2_kernel<S, A><<<wave_grid(n2), 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec], t, e.d_ylev,
-                                                       e.N);
-  H3D_LAUNCH_CHECK();
+  launch_p2<S, A>(e, prec, e.tune_p2[key], t);
+}
+
+// Average ms of `reps` launches of phase 1 (which = 1) or phase 2 (which = 2).
+template <class S, class A>
+double time_phase(Engine& e, int prec, int which, int bytes, int reps, const A* x, A* t, A* part) {
+  hipEvent_t a, b;
+  H3D_HIP(hipEventCreate(&a));
+  H3D_HIP(hipEventCreate(&b));
+  // one untimed launch first
+  if (which == 1) launch_p1<S, A>(e, prec, bytes, x, t, part);
+  else launch_p2<S, A>(e, prec, bytes, t);
+  H3D_HIP(hipEventRecord(a, e.stream));
+  for (int r = 0; r < reps; ++r) {
+    if (which == 1) launch_p1<S, A>(e, prec, bytes, x, t, part);
+    else launch_p2<S, A>(e, prec, bytes, t);
+  }
+  H3D_HIP(hipEventRecord(b, e.stream));
+  H3D_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  H3D_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+template <class S, class A>
+void tune_mode(Engine& e, int prec, int reps, const A* x, A* t, A* part, double* ms) {
+  const int key = tune_key(prec, sizeof(A) == 8);
+  const int widths[4] = {16, 8, 4, 2};
+  for (int which = 1; which <= 2; ++which) {
+    double best = 1e300;
+    int bw = 16;
+    for (int q = 0; q < 4; ++q) {
+      const int w = widths[q];
+      double v = -1;
+      if (w >= (int)sizeof(S)) {
+        v = time_phase<S, A>(e, prec, which, w, reps, x, t, part);
+        if (v < best) {
+          best = v;
+          bw = w;
+        }
+      }
+      if (ms) ms[(which - 1) * 4 + q] = v;
+    }
+    (which == 1 ? e.tune_p1 : e.tune_p2)[key] = bw;
+  }
 }
 
 template <class S, class A>
@@ -697,6 +940,29 @@ void run_transpose(Engine& e, int prec, const A* x, A* t, A* part) {
 }
 
 }  // namespace
+
+void autotune(Engine& e, int mode, int reps, double* ms) {
+  if (!e.assembled) throw Error(H3D_ERR_STATE, "autotune before assemble");
+  const int prec = mode & 0xff;
+  if (prec < 0 || prec > 2 || (mode & ~(0xff | H3D_ACC_FP64)))
+    throw Error(H3D_ERR_INVALID, "unknown precision");
+  if (reps < 1) throw Error(H3D_ERR_INVALID, "reps must be >= 1");
+  const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
+  if (e.fac_bytes[prec] == 0) prepare_precision(e, prec);
+  ensure_scratch(e, 1);
+  // timing does not depend on the values: x is whatever the last matvec left
+  if (acc64) {
+    if (prec == H3D_PREC_FP64) tune_mode<double, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
+    else if (prec == H3D_PREC_FP32) tune_mode<float, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
+    else tune_mode<uint16_t, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
+  } else {
+    float* t = (float*)e.d_t;
+    float* part = (float*)e.d_part;
+    if (prec == H3D_PREC_FP32) tune_mode<float, float>(e, prec, reps, e.d_xf, t, part, ms);
+    else tune_mode<uint16_t, float>(e, prec, reps, e.d_xf, t, part, ms);
+  }
+  H3D_HIP(hipStreamSynchronize(e.stream));
+}
 
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode, bool transpose) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");

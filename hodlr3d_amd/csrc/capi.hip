@@ -247,6 +247,27 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag) {
   });
 }
 
+int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    const int p = prec & 0xff;
+    if (p < 0 || p > 2 || (prec & ~(0xff | H3D_ACC_FP64)))
+      throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    auto ok = [&](int b) { return (b == 16 || b == 8 || b == 4 || b == 2) && b >= h3d::elem_size(p); };
+    if (!ok(p1_bytes) || !ok(p2_bytes)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
+    const int key = p + ((prec & H3D_ACC_FP64) && p != 0 ? 3 : 0);
+    h->e.tune_p1[key] = p1_bytes;
+    h->e.tune_p2[key] = p2_bytes;
+  });
+}
+
+int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h3d::autotune(h->e, prec, reps, ms);
+  });
+}
+
 int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
@@ -475,16 +496,16 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
     if (I) H3D_HIP(hipMemcpy(I, ch.piv + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
     if (J) H3D_HIP(hipMemcpy(J, ch.piv + lv.pivoff[b] + r, sizeof(int32_t) * r, hipMemcpyDeviceToHost));
     if ((U || V) && !ch.data[0]) throw h3d::Error(H3D_ERR_STATE, "fp64 factors not resident");
+    const int ld = n + (n & 1), ldv = m + (m & 1);
     if (U) {
-      const double* src = (const double*)ch.data[0] + lv.uoff[k] + (int64_t)lv.col[b] * n;
-      H3D_HIP(hipMemcpy(U, src, sizeof(double) * (int64_t)r * n, hipMemcpyDeviceToHost));
+      const double* src = (const double*)ch.data[0] + lv.uoff[k] + (int64_t)lv.col[b] * ld;
+      H3D_HIP(hipMemcpy2D(U, sizeof(double) * n, src, sizeof(double) * ld, sizeof(double) * n, r,
+                          hipMemcpyDeviceToHost));
     }
     if (V) {
-      std::vector<double> rows((size_t)m * r);
       const double* src = (const double*)ch.data[0] + ch.nU + lv.voff[b];
-      H3D_HIP(hipMemcpy(rows.data(), src, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
-      for (int l = 0; l < r; ++l)
-        for (int s = 0; s < m; ++s) V[(int64_t)l * m + s] = rows[(size_t)s * r + l];
+      H3D_HIP(hipMemcpy2D(V, sizeof(double) * m, src, sizeof(double) * ldv, sizeof(double) * m, r,
+                          hipMemcpyDeviceToHost));
     }
   });
 }
@@ -513,7 +534,14 @@ int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
 int h3d_set_stream(h3d_handle* h, void* stream) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    h->e.stream = stream ? (hipStream_t)stream : h->e.own_stream;
+    h->e.stream = (hipStream_t)stream;
+  });
+}
+
+int h3d_use_own_stream(h3d_handle* h) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    h->e.stream = h->e.own_stream;
   });
 }
 

@@ -8,9 +8,13 @@
 //             contiguous position segment [bstart, bstart+bcount)
 //   factors   stored in chunks; one chunk holds every block of a run of
 //             consecutive target boxes of one level:
-//               U region: per target box k, column-major n_k x R_k (ld = n_k),
-//                         the u-vectors of its blocks in interaction-list order
-//               V region: per block, row-major m_b x r_b (the v-vectors)
+//               U region: per target box k, column-major n_k x R_k with
+//                         ld = n_k rounded up to even, the u-vectors of its
+//                         blocks in interaction-list order
+//               V region: per block, column-major m_b x r_b with ldv = m_b
+//                         rounded up to even (the v-vectors)
+//             Padding is zero and every chunk ends with 64 zero elements, so
+//             16-byte vector loads of any column stay inside the allocation.
 //             Each stored precision has its own copy of every chunk with the
 //             same element offsets.
 //   t         one value per (block, rank) in target order: box k's blocks
@@ -41,19 +45,19 @@ struct BlockDev {
   int32_t pad;
 };
 
-// Far-field phase 1 item: t[toff + c0 + lane] = V^T x for 64 consecutive
-// columns of one target box (may span several blocks).
+// Far-field phase 1 item: t = V_b^T x for the consecutive blocks [b0, b1) of
+// one target box, over rows [s0, s1) of each block (at most 512 rows).
 struct P1Item {
   int32_t chunk;
-  int32_t bfirst, bcount;  // global block ids of the box
-  int32_t c0, nc;          // column range within the box
-  int32_t s0, s1;          // row range
-  int32_t toff;            // t offset of the box
-  int32_t pslot;           // -1: write t; otherwise partial-sum slot (64 values)
+  int32_t b0, b1;  // global block ids
+  int32_t s0, s1;  // row range
+  int32_t toff;    // t offset of the target box
+  int32_t pslot;   // -1: write t; otherwise element offset of r partial sums
   int32_t pad;
 };
 
-// Partial-sum reduction for a column chunk split over several row ranges.
+// Partial-sum reduction: nsplit row ranges x nc columns at part[pbase + q*nc + l]
+// summed in range order into t[toff + l].
 struct P1Reduce {
   int32_t pbase, nsplit, toff, nc;
 };
@@ -197,6 +201,10 @@ struct Engine {
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
+  // autotuned load width in bytes per lane for phase 1 / phase 2, indexed by
+  // storage precision + 3 * (fp64 accumulation over fp32/bf16)
+  int tune_p1[6] = {16, 16, 16, 16, 16, 16};
+  int tune_p2[6] = {16, 16, 16, 16, 16, 16};
   // per-kernel event timing of matvecs
   bool timing = false;
   hipEvent_t ev[6] = {};
@@ -215,6 +223,7 @@ struct Engine {
 };
 
 inline int64_t level_offset(int j) { return ((int64_t(1) << (3 * j)) - 1) / 7; }
+constexpr int64_t kChunkTail = 64;  // zero elements after every factor chunk
 inline int elem_size(int prec) { return prec == 0 ? 8 : (prec == 1 ? 4 : 2); }
 
 // tree.hip
@@ -229,6 +238,10 @@ void release_precision(Engine& e, int prec);
 void upload_chunk_table(Engine& e);
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool transpose);
 void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out);
+// Times every load width of phase 1 and phase 2 for one precision mode on the
+// assembled problem and keeps the fastest; ms[8] = phase 1 at 16/8/4/2 bytes,
+// then phase 2 at 16/8/4/2 bytes (negative when the width does not apply).
+void autotune(Engine& e, int mode, int reps, double* ms);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);

@@ -165,7 +165,7 @@ class HODLR3D:
                 nat.check(fn(self._h, ctypes.c_void_p(cols.data_ptr()),
                              ctypes.c_void_p(out.data_ptr()), cols.shape[0], _prec(prec), nat.DEVICE))
             finally:
-                nat.lib().h3d_set_stream(self._h, None)
+                nat.lib().h3d_use_own_stream(self._h)
             return out.t().reshape(b.shape)
         a = np.asarray(b, dtype=np.float64)
         if a.shape[0] != self.N:
@@ -193,6 +193,19 @@ class HODLR3D:
         """Apply-time operator y = alpha * K~ b + diag * b (per-point diagonal if set)."""
         nat.check(nat.lib().h3d_set_operator(self._h, float(alpha), float(diag)))
 
+    def autotune(self, prec="fp64", reps=3):
+        """Pick the fastest per-lane load width of both far-field kernels for `prec`
+        on this problem; returns the measured ms per width."""
+        ms = (ctypes.c_double * 8)()
+        nat.check(nat.lib().h3d_autotune(self._h, _prec(prec), int(reps), ms))
+        w = [16, 8, 4, 2]
+        return {"phase1": {w[i]: ms[i] for i in range(4) if ms[i] >= 0},
+                "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0}}
+
+    def set_tuning(self, prec, p1_bytes, p2_bytes):
+        """Pin the per-lane load widths of the two far-field kernels for `prec`."""
+        nat.check(nat.lib().h3d_set_tuning(self._h, _prec(prec), int(p1_bytes), int(p2_bytes)))
+
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
         ms = (ctypes.c_double * 5)()
@@ -209,7 +222,7 @@ class HODLR3D:
             try:
                 nat.check(nat.lib().h3d_set_diag(self._h, ctypes.c_void_p(ct.data_ptr()), nat.DEVICE))
             finally:
-                nat.lib().h3d_set_stream(self._h, None)
+                nat.lib().h3d_use_own_stream(self._h)
             return
         if c is None:
             nat.check(nat.lib().h3d_set_diag(self._h, None, nat.HOST))
@@ -234,7 +247,7 @@ class HODLR3D:
                                             inner_tol, _prec(inner_prec), tol, max_outer,
                                             int(transpose), ctypes.byref(info), nat.DEVICE)
             finally:
-                nat.lib().h3d_set_stream(self._h, None)
+                nat.lib().h3d_use_own_stream(self._h)
         else:
             a = np.ascontiguousarray(b, dtype=np.float64)
             x = np.empty_like(a)
@@ -262,7 +275,7 @@ class HODLR3D:
                                                r.ctypes.data_as(ctypes.c_void_p), len(r),
                                                ctypes.c_void_p(out.data_ptr()), nat.DEVICE))
             finally:
-                nat.lib().h3d_set_stream(self._h, None)
+                nat.lib().h3d_use_own_stream(self._h)
             return out
         a = np.ascontiguousarray(b, dtype=np.float64)
         out = np.empty(len(r))

@@ -128,6 +128,15 @@ int h3d_set_diag(h3d_handle* h, const double* diag_vec, int where);
  * kernel's diagonal.  Lets one assembly of K serve A = diag + alpha K. */
 int h3d_set_operator(h3d_handle* h, double alpha, double diag);
 
+/* Autotuner: times the far-field kernels of precision mode `prec` with every
+ * per-lane load width (16/8/4/2 bytes) on the assembled problem, `reps`
+ * launches each, and keeps the fastest for later matvecs.  ms[8] (may be
+ * NULL) receives phase-1 then phase-2 averages per width, -1 where a width
+ * does not apply. */
+int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
+/* Pins the per-lane load widths (bytes, >= the element size) for one mode. */
+int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
+
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
  * stream (enable != 0).  ms[5] receives the accumulated milliseconds of
  * phase 1 (V^T x), phase 2 (U t), near field, combine and whole matvecs since
@@ -170,8 +179,10 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
 
 int h3d_get_stats(h3d_handle* h, h3d_stats* s);
 
-/* Use a caller-owned hipStream_t (passed as void*); NULL restores the handle's own. */
+/* Run subsequent calls on a caller-owned hipStream_t (passed as void*; NULL is
+ * the legacy default stream); h3d_use_own_stream returns to the handle's own. */
 int h3d_set_stream(h3d_handle* h, void* stream);
+int h3d_use_own_stream(h3d_handle* h);
 int h3d_synchronize(h3d_handle* h);
 
 /* Replaces HODLR3D::~HODLR3D (src/HODLR3D.cpp:29-32). */

@@ -125,3 +125,29 @@ def test_nrhs_and_device_tensors(pair):
         assert np.array_equal(Y[:, i], h.matvec(B[:, i]))
     Yd = h.matvec(torch.from_numpy(B).cuda()).cpu().numpy()
     assert np.array_equal(Yd, Y)
+
+
+def test_every_load_width_gives_the_same_product(pair):
+    """The autotuner may pick any width: each must compute the same matvec."""
+    name, x, h, o = pair
+    q = np.random.default_rng(11).standard_normal(len(x))
+    for prec, es, tol in (("fp64", 8, 1e-14), ("fp32", 4, 2e-6), ("fp32/acc64", 4, 1e-12),
+                          ("bf16", 2, 2e-2), ("bf16/acc64", 2, 1e-12)):
+        ref = None
+        for w1 in (16, 8, 4, 2):
+            for w2 in (16, 8, 4, 2):
+                if w1 < es or w2 < es:
+                    continue
+                h.set_tuning(prec, w1, w2)
+                y = h.matvec(q, prec=prec)
+                yt = h.matvec_transpose(q, prec=prec)
+                if ref is None:
+                    ref = (y, yt)
+                    continue
+                assert np.linalg.norm(y - ref[0]) <= tol * np.linalg.norm(ref[0]), (prec, w1, w2)
+                assert np.linalg.norm(yt - ref[1]) <= tol * np.linalg.norm(ref[1]), (prec, w1, w2)
+        h.set_tuning(prec, 16, 16)
+    # autotune keeps a correct choice
+    h.autotune("fp64", reps=1)
+    yo = o.matvec(q, mode=0)
+    assert np.linalg.norm(h.matvec(q) - yo) / np.linalg.norm(yo) < 1e-12
