@@ -40,6 +40,9 @@ MODE_DESC = {
     "fp32/acc64": "fp32 factors, fp64 accumulation, fp64 on-the-fly near field",
     "bf16": "bf16 factors, fp32 accumulation, fp32 on-the-fly near field",
     "bf16/acc64": "bf16 factors, fp64 accumulation, fp64 on-the-fly near field",
+    "mixed": "recompressed factors (eigendecomposition of each block's Gram matrices), "
+             "columns dropped / fp16 / fp32 under one global error budget, fp64 "
+             "accumulation, fp32 near-field entries summed in fp64",
 }
 
 
@@ -72,6 +75,27 @@ def timed(torch, fn, steps, warmup):
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def mixed_kernels(H, km):
+    """(name, ms, algorithmic bytes) of the four far-field kernels of the mixed
+    copy: phase 1 / phase 2 of its fp32 class and of its fp16 class.  Phase 1
+    reads the class's V entries, x (fp64) once per level and writes t; phase 2
+    reads the U entries and t and writes (fp32 class) or updates (fp16 class)
+    y once per level."""
+    ms_ = H.mixed_stats()
+    st = H.geometry_stats()
+    xn = st["nlevels"] * st["N"] * 8
+    out = []
+    for c, (es, name) in enumerate(((4, "fp32"), (2, "fp16"))):
+        rank = ms_["rank_fp32"] if c == 0 else ms_["rank_fp16"]
+        p1 = es * ms_["v_elems"][c] + xn + 8 * rank
+        p2 = es * ms_["u_elems"][c] + 8 * rank + xn * (1 if c == 0 else 2)
+        t1 = km["phase1"] - km["phase1_second"] if c == 0 else km["phase1_second"]
+        t2 = km["phase2"] - km["phase2_second"] if c == 0 else km["phase2_second"]
+        out.append((f"phase1 {name} class", t1, p1))
+        out.append((f"phase2 {name} class", t2, p2))
+    return out
 
 
 def alg_bytes(H, elem, acc64=False):
@@ -130,8 +154,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="C3", choices=["C1", "C2", "C3"])
-    ap.add_argument("--modes", default="fp32/acc64,fp32,bf16",
+    ap.add_argument("--modes", default="mixed,fp32/acc64,fp32,bf16",
                     help="reduced-precision modes measured after fp64 (comma separated)")
+    ap.add_argument("--mixed-tol", type=float, default=5e-7,
+                    help="added relative error budget of the recompressed mixed copy")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (for scaling "
                          "HBM counters of a profiler run)")
@@ -185,7 +211,8 @@ def main():
         kt = H.kernel_timing(False)
         c = max(kt["count"], 1)
         out["ms"] = ms
-        out["kernel_ms"] = {k: kt[k] / c for k in ("phase1", "phase2", "near", "combine", "total")}
+        out["kernel_ms"] = {k: kt[k] / c for k in ("phase1", "phase2", "near", "combine", "total",
+                                                   "phase1_second", "phase2_second")}
         log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e}, kernels "
             f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }")
         return out
@@ -202,6 +229,15 @@ def main():
 
     res["fp64"] = measure("fp64")
     modes = [m for m in args.modes.split(",") if m]
+    mixed_info = None
+    if "mixed" in modes:
+        # built from the fp64 factors, so before they are converted
+        modes = ["mixed"] + [m for m in modes if m != "mixed"]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mixed_info = H.compress(args.mixed_tol)
+        mixed_info["wall_s"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] mixed copy: {mixed_info}")
     for mode in modes:
         base = mode.split("/")[0]
         if base == "fp32" and H.stats()["factor_bytes"][1] == 0:
@@ -212,13 +248,16 @@ def main():
     passing = [k for k in modes if res[k]["rel_err"] <= REL_ERR_BAR]
     mixed = min(passing, key=lambda k: res[k]["ms"]) if passing else "fp64"
     m = res[mixed]
-    elem = {"fp64": 8, "fp32": 4, "bf16": 2}[mixed.split("/")[0]]
-    p1b, p2b = alg_bytes(H, elem, acc64="acc64" in mixed)
     km = m["kernel_ms"]
-    dom = "phase2" if km["phase2"] >= km["phase1"] else "phase1"
-    dom_bytes = p2b if dom == "phase2" else p1b
-    achieved = dom_bytes / (km[dom] * 1e-3) / 1e9
-    far_gbs = (p1b + p2b) / ((km["phase1"] + km["phase2"]) * 1e-3) / 1e9
+    if mixed == "mixed":
+        kern = mixed_kernels(H, km)
+    else:
+        elem = {"fp64": 8, "fp32": 4, "bf16": 2}[mixed.split("/")[0]]
+        p1b, p2b = alg_bytes(H, elem, acc64="acc64" in mixed)
+        kern = [("phase1", km["phase1"], p1b), ("phase2", km["phase2"], p2b)]
+    dom, dom_ms, dom_bytes = max(kern, key=lambda k: k[1])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    far_gbs = sum(k[2] for k in kern) / (sum(k[1] for k in kern) * 1e-3) / 1e9
     p1_64, p2_64 = alg_bytes(H, 8)
     fp64_far_gbs = (p1_64 + p2_64) / ((res["fp64"]["kernel_ms"]["phase1"] +
                                        res["fp64"]["kernel_ms"]["phase2"]) * 1e-3) / 1e9
@@ -231,8 +270,9 @@ def main():
         b = torch.from_numpy(np.random.default_rng(3).standard_normal(N)).to(dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        inner = "mixed" if mixed_info else "fp32"
         x, info = H.solve(b, tol=1e-10, restart=30, max_inner=600, inner_tol=1e-5,
-                          inner_prec="fp32", max_outer=12)
+                          inner_prec=inner, max_outer=12)
         torch.cuda.synchronize()
         t_solve = (time.perf_counter() - t0) * 1e3
         solve = {"ms": round(t_solve, 1), "rel_residual": info["rel_residual"],
@@ -240,7 +280,7 @@ def main():
                  "inner_iterations": info["inner_iterations"],
                  "outer_operator": "fp64 arithmetic over fp32-stored factors"
                  if info["outer_prec"] != 0 else "fp64 factors",
-                 "inner": "GMRES(30), fp32 factors, fp32 accumulation, fp64 Arnoldi/Givens",
+                 "inner": "GMRES(30), " + MODE_DESC[inner] + ", fp64 Arnoldi/Givens",
                  "operator": "A = I + h^3/(4 pi) * (1/r), h = 2/128", "history": info["history"]}
         log(f"[bench] solve {solve}")
         H.set_operator(alpha=1.0, diag=0.0)
@@ -262,10 +302,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "fp32/acc64": "f32 storage, f64 accumulation",
-                  "bf16": "bf16", "bf16/acc64": "bf16 storage, f64 accumulation"}.get(mixed, mixed),
+                  "bf16": "bf16", "bf16/acc64": "bf16 storage, f64 accumulation",
+                  "mixed": "f32+f16 storage, f64 accumulation"}.get(mixed, mixed),
         "data": "synthetic",
         "config": {"workload": wl["desc"], "N": N, "nlevels": st["nlevels"],
                    "mixed": MODE_DESC.get(mixed, mixed), "err_bar": REL_ERR_BAR,
+                   "mixed_tol": args.mixed_tol if mixed == "mixed" else None,
                    "rhs": "iid N(0,1)", "parallelism": "single GPU"},
         "fp64_ms": round(res["fp64"]["ms"], 4),
         "speedup_vs_fp64": round(res["fp64"]["ms"] / m["ms"], 3),
@@ -284,7 +326,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "alg_bytes": int(dom_bytes), "kernel_ms": round(km[dom], 4)},
+                     "alg_bytes": int(dom_bytes), "kernel_ms": round(dom_ms, 4),
+                     "far_kernels": [{"kernel": k[0], "ms": round(k[1], 4), "alg_bytes": int(k[2]),
+                                      "GB/s": round(k[2] / (k[1] * 1e-3) / 1e9, 1) if k[1] > 0 else None}
+                                     for k in kern]},
+        "mixed_copy": mixed_info,
         "cpu_baseline": cpu,
         "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
     }

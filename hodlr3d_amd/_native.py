@@ -21,6 +21,7 @@ H3D_ERR_INTERNAL = 7
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
+PREC_MIXED = 3
 ACC_FP64 = 0x100
 HOST = 0
 DEVICE = 1
@@ -47,6 +48,14 @@ class Stats(ctypes.Structure):
                 ("factor_bytes", _i64 * 3), ("u_elems", _i64), ("v_elems", _i64)]
 
 
+class MixedStats(ctypes.Structure):
+    _fields_ = [("tol", _d), ("predicted_rel_err", _d), ("err_drop", _d), ("err_fp16", _d),
+                ("err_fp32", _d), ("fro_far", _d), ("fro_near", _d),
+                ("aca_rank_sum", _i64), ("rank_fp32", _i64), ("rank_fp16", _i64),
+                ("dropped", _i64), ("raw_blocks", _i64), ("raw_rank", _i64), ("bytes", _i64),
+                ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d)]
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [("outer_iterations", _i32), ("inner_iterations", _i32), ("outer_prec", _i32),
                 ("rel_residual", _d),
@@ -66,6 +75,8 @@ SIGNATURES = {
     "h3d_kernel_timing": (_i32, [_p, _i32, _p, ctypes.POINTER(_i32)]),
     "h3d_autotune": (_i32, [_p, _i32, _i32, _p]),
     "h3d_set_tuning": (_i32, [_p, _i32, _i32, _i32]),
+    "h3d_compress": (_i32, [_p, _d, ctypes.POINTER(MixedStats)]),
+    "h3d_get_mixed_stats": (_i32, [_p, ctypes.POINTER(MixedStats)]),
     "h3d_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_matvec_transpose": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_set_diag": (_i32, [_p, _p, _i32]),

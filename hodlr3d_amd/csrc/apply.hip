@@ -27,21 +27,34 @@ constexpr int kP1Rows = 512;          // rows of a phase-1 item (64 lanes x 8 ro
 constexpr int64_t kP1Elems = 8192;    // factor entries per multi-block phase-1 item
 constexpr int kP2Rows = 128;          // rows of one phase-2 item
 constexpr int kWavesPerBlock = 4;
+constexpr int kMixMaxCols = 1024;     // columns of a mixed phase-1 item (LDS t-index map)
+
+// Factor pointers come from per-chunk tables, so the compiler cannot prove
+// they are global memory; loading through an address-space-1 pointer makes
+// them global (not flat) loads, which do not share the LDS wait counter.
+template <class T>
+__device__ inline T ntload(const void* p) {
+  return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)p);
+}
 
 template <class S>
 struct Load;
 template <>
 struct Load<double> {
-  __device__ static inline double f(const double* p) { return __builtin_nontemporal_load(p); }
+  __device__ static inline double f(const double* p) { return ntload<double>(p); }
 };
 template <>
 struct Load<float> {
-  __device__ static inline float f(const float* p) { return __builtin_nontemporal_load(p); }
+  __device__ static inline float f(const float* p) { return ntload<float>(p); }
 };
 template <>
 struct Load<uint16_t> {
-  __device__ static inline float f(const uint16_t* p) {
-    return bf16_to_f(__builtin_nontemporal_load(p));
+  __device__ static inline float f(const uint16_t* p) { return bf16_to_f(ntload<uint16_t>(p)); }
+};
+template <>
+struct Load<_Float16> {
+  __device__ static inline float f(const _Float16* p) {
+    return (float)__builtin_bit_cast(_Float16, ntload<uint16_t>(p));
   }
 };
 
@@ -66,10 +79,18 @@ __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __res
 typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u32x2a __attribute__((ext_vector_type(2), aligned(4)));
 
+// Element `half` (0: low, 1: high 16 bits) of a 32-bit word of stored S values:
+// uint16_t is bf16, _Float16 is IEEE half.
 template <class S, class A>
 __device__ inline A cvt_elem(uint32_t bits, int half) {
-  if constexpr (sizeof(S) == 2) return (A)__uint_as_float(half ? (bits & 0xffff0000u) : (bits << 16));
-  else return (A)__uint_as_float(bits);
+  if constexpr (std::is_same<S, _Float16>::value) {
+    const uint16_t h = half ? (uint16_t)(bits >> 16) : (uint16_t)(bits & 0xffffu);
+    return (A)(float)__builtin_bit_cast(_Float16, h);
+  } else if constexpr (sizeof(S) == 2) {
+    return (A)__uint_as_float(half ? (bits & 0xffff0000u) : (bits << 16));
+  } else {
+    return (A)__uint_as_float(bits);
+  }
 }
 
 // NV consecutive stored elements at p (NV * sizeof(S) in {2,4,8,16} bytes)
@@ -78,28 +99,28 @@ __device__ inline void load_factor(const S* p, A (&o)[NV]) {
   constexpr int B = NV * (int)sizeof(S);
   if constexpr (sizeof(S) == 8) {
     if constexpr (NV == 2) {
-      u32x4a v = __builtin_nontemporal_load((const u32x4a*)p);
+      u32x4a v = ntload<u32x4a>(p);
       o[0] = (A)__hiloint2double((int)v.y, (int)v.x);
       o[1] = (A)__hiloint2double((int)v.w, (int)v.z);
     } else {
-      o[0] = (A)__builtin_nontemporal_load(p);
+      o[0] = (A)ntload<double>(p);
     }
   } else if constexpr (B == 16) {
-    u32x4a v = __builtin_nontemporal_load((const u32x4a*)p);
+    u32x4a v = ntload<u32x4a>(p);
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
   } else if constexpr (B == 8) {
-    u32x2a v = __builtin_nontemporal_load((const u32x2a*)p);
+    u32x2a v = ntload<u32x2a>(p);
     const uint32_t w[2] = {v.x, v.y};
 #pragma unroll
     for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
   } else if constexpr (B == 4) {
-    const uint32_t v = __builtin_nontemporal_load((const uint32_t*)p);
+    const uint32_t v = ntload<uint32_t>(p);
 #pragma unroll
     for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(v, q & 1);
   } else {
-    o[0] = (A)bf16_to_f(__builtin_nontemporal_load((const uint16_t*)p));
+    o[0] = cvt_elem<S, A>((uint32_t)ntload<uint16_t>(p), 0);
   }
 }
 
@@ -110,12 +131,14 @@ __device__ inline void load_factor(const S* p, A (&o)[NV]) {
 // block) and reads VEC consecutive rows of the column per load; the lanes'
 // partial sums are combined with an xor butterfly inside the group.  Rows past
 // the item's range see x = 0, so loads may run into padding or the next column.
+// With `scale` (indexed like t) the finished t_k is multiplied by scale_k.
 template <class S, class A, int VEC>
 __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ items, int nitems,
                                                      void* const* __restrict__ vbase,
                                                      const BlockDev* __restrict__ blk,
                                                      const A* __restrict__ x, A* __restrict__ t,
-                                                     A* __restrict__ part) {
+                                                     A* __restrict__ part,
+                                                     const double* __restrict__ scale) {
   constexpr int KMAX = 8 / VEC > 0 ? 8 / VEC : 1;  // rows per lane <= 8 (items have <= 512 rows)
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
@@ -149,7 +172,8 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
         const S* col = base + (int64_t)l * ldv;
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-          if (k < K) {
+          // a load starts inside the rows, so it ends < VEC elements past them
+          if (k < K && (sub + k * lpc) * VEC < rows) {
             A v[VEC];
             load_factor<S, A, VEC>(col + (sub + k * lpc) * VEC, v);
 #pragma unroll
@@ -160,7 +184,7 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
       for (int d = lpc >> 1; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
       if (l < bd.r && sub == 0) {
         if (it.pslot < 0)
-          t[it.toff + bd.col + l] = acc;
+          t[it.toff + bd.col + l] = scale ? acc * (A)scale[it.toff + bd.col + l] : acc;
         else
           part[(int64_t)it.pslot + l] = acc;
       }
@@ -169,10 +193,10 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
 }
 
 // Sums the row-split partials of one block (nsplit x nc values, row ranges in
-// order) into t[toff + l].
+// order) into t[toff + l] (times scale[toff + l] when scaled).
 template <class A>
 __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ part,
-                              A* __restrict__ t) {
+                              A* __restrict__ t, const double* __restrict__ scale) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
@@ -180,7 +204,114 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
   for (int l = lane; l < r.nc; l += 64) {
     A acc = 0;
     for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
-    t[r.toff + l] = acc;
+    t[r.toff + l] = scale ? acc * (A)scale[r.toff + l] : acc;
+  }
+}
+
+// ---------------------------------------------------------- mixed phase 1
+// t = V^T x for one class of the recompressed copy, grouped by source box: an
+// item is a run of columns of V_s (the columns of every block with source s,
+// concatenated) over at most 512 rows, so x is loaded once per item.  lpc lanes
+// share a column, each holding K <= kmax load vectors of VEC rows (x in
+// registers), and reduce with an xor butterfly.  The wave first writes the t
+// index of each of its (at most 1024) columns into its LDS slice from the
+// column map.  A load starts inside the rows, so it ends less than VEC
+// elements past them (next column, padding or the chunk's zero tail).
+template <class S, class A, int VEC>
+__global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restrict__ items,
+                                                      int nitems, void* const* __restrict__ vbase,
+                                                      const ColMap* __restrict__ cmap,
+                                                      const A* __restrict__ x, A* __restrict__ t,
+                                                      A* __restrict__ part,
+                                                      const double* __restrict__ scale, int kmax) {
+  constexpr int KMAX = VEC >= 2 ? 4 : 8;  // 64 lanes x KMAX x VEC >= 512 rows
+  __shared__ int32_t tix_s[kWavesPerBlock][kMixMaxCols];
+  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const MixP1Item it = items[w];
+  int32_t* tix = tix_s[threadIdx.x >> 6];
+  if (it.pslot < 0) {
+    for (int q = lane; q < it.bcnt; q += 64) {
+      const ColMap cm = cmap[it.bpos0 + q];
+      const int a = max(cm.beg, it.c0), z = min(cm.end, it.c1);
+      for (int c = a; c < z; ++c) tix[c - it.c0] = cm.tmc + c;
+    }
+    // the wave's own LDS writes, read back by other lanes of the wave
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int rows = it.s1 - it.s0;
+  const int need = (rows + VEC - 1) / VEC;
+  const int kk = min(max(kmax, 1), KMAX);
+  int lpc = 1;
+  while (lpc * kk < need && lpc < 64) lpc <<= 1;
+  const int K = (need + lpc - 1) / lpc;
+  const int cpp = 64 / lpc, slot = lane / lpc, sub = lane % lpc;
+  A xv[KMAX][VEC];
+  const A* xs = x + it.xoff + it.s0;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int s = (sub + k * lpc) * VEC;
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) xv[k][q] = (k < K && s + q < rows) ? xs[s + q] : (A)0;
+  }
+  const S* base = (const S*)vbase[it.vchunk] + it.voff + it.s0;
+  // UP passes at once when a lane holds one load vector per column, so that
+  // four loads per lane are in flight either way
+  constexpr int UP = 4;
+  const int step = K == 1 ? UP * cpp : cpp;
+  for (int c0 = it.c0; c0 < it.c1; c0 += step) {
+    A acc[UP];
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+      acc[u] = 0;
+      const int c = c0 + u * cpp + slot;
+      if ((u == 0 || K == 1) && c < it.c1) {
+        const S* col = base + (int64_t)c * it.ld;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+          if (k < K && (sub + k * lpc) * VEC < rows) {
+            A v[VEC];
+            load_factor<S, A, VEC>(col + (sub + k * lpc) * VEC, v);
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) acc[u] += v[q] * xv[k][q];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+      if (u > 0 && K != 1) break;
+      A a = acc[u];
+      for (int d = lpc >> 1; d >= 1; d >>= 1) a += __shfl_xor(a, d);
+      const int c = c0 + u * cpp + slot;
+      if (c < it.c1 && sub == 0) {
+        if (it.pslot < 0) {
+          const int ti = tix[c - it.c0];
+          t[ti] = scale ? a * (A)scale[ti] : a;
+        } else {
+          part[(int64_t)it.pslot + (c - it.c0)] = a;
+        }
+      }
+    }
+  }
+}
+
+// Row-split partials of the mixed phase 1 summed in range order into t.
+template <class A>
+__global__ void mreduce_kernel(const MixReduce* __restrict__ rs, int n, const A* __restrict__ part,
+                               const int32_t* __restrict__ tidx, A* __restrict__ t,
+                               const double* __restrict__ scale) {
+  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (w >= n) return;
+  const int lane = threadIdx.x & 63;
+  const MixReduce r = rs[w];
+  for (int l = lane; l < r.nc; l += 64) {
+    A acc = 0;
+    for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
+    const int ti = tidx[r.ioff + l];
+    t[ti] = scale ? acc * (A)scale[ti] : acc;
   }
 }
 
@@ -189,11 +320,13 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
 // consecutive rows (one vector load per column); the wave's lanes are split
 // into G groups that walk different columns, so one wave instruction reads G
 // consecutive columns of the box (contiguous when the item spans the box).
+// accumulate != 0 adds to y_level (a second factor copy) instead of storing.
 template <class S, class A, int VR>
 __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ items, int nitems,
                                                      void* const* __restrict__ ubase,
                                                      const A* __restrict__ t,
-                                                     double* __restrict__ ylev, int64_t N) {
+                                                     double* __restrict__ ylev, int64_t N,
+                                                     int accumulate) {
   __shared__ A red[kWavesPerBlock][64 * VR];
   const int wv = threadIdx.x >> 6;
   const int w = blockIdx.x * kWavesPerBlock + wv;
@@ -243,7 +376,10 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
 #pragma unroll
       for (int q = 0; q < VR; ++q) {
         const int row = rho * VR + q;
-        if (row < it.nr) ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + row] = (double)a[q];
+        if (row < it.nr) {
+          double* yp = ylev + (int64_t)(it.level - 1) * N + it.yoff + it.i0 + row;
+          *yp = accumulate ? *yp + (double)a[q] : (double)a[q];
+        }
       }
     }
   } else if (valid) {
@@ -290,7 +426,8 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
     if (row >= it.nr) break;
     A s = acc[q];
     for (int g = 1; g < G; ++g) s += red[wv][(g * rcn + lane) * VR + q];
-    ylev[(int64_t)(it.level - 1) * N + it.yoff + it.i0 + row] = (double)s;
+    double* yp = ylev + (int64_t)(it.level - 1) * N + it.yoff + it.i0 + row;
+    *yp = accumulate ? *yp + (double)s : (double)s;
   }
 }
 
@@ -302,7 +439,8 @@ template <class S, class A>
 __global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict__ items, int nitems,
                                                       void* const* __restrict__ ubase,
                                                       const A* __restrict__ x, A* __restrict__ t,
-                                                      A* __restrict__ part) {
+                                                      A* __restrict__ part,
+                                                      const double* __restrict__ scale) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -322,7 +460,7 @@ __global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict_
   }
   if (lane >= it.nc) return;
   if (it.pslot < 0)
-    t[it.toff + it.c0 + lane] = mine;
+    t[it.toff + it.c0 + lane] = scale ? mine * (A)scale[it.toff + it.c0 + lane] : mine;
   else
     part[(int64_t)it.pslot + lane] = mine;
 }
@@ -336,7 +474,8 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
                                                       const int32_t* __restrict__ by_src,
                                                       const int32_t* __restrict__ btoff,
                                                       const A* __restrict__ t,
-                                                      double* __restrict__ ylev, int64_t N) {
+                                                      double* __restrict__ ylev, int64_t N,
+                                                      int accumulate) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -353,7 +492,9 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
     const A* tt = t + btoff[b];
     for (int l = 0; l < bd.r; ++l) acc += (A)Load<S>::f(col + l * ldv) * tt[l];
   }
-  if (act) ylev[(int64_t)(it.level - 1) * N + it.yoff + s] = (double)acc;
+  if (!act) return;
+  double* yp = ylev + (int64_t)(it.level - 1) * N + it.yoff + s;
+  *yp = accumulate ? *yp + (double)acc : (double)acc;
 }
 
 }  // namespace
@@ -402,15 +543,17 @@ __global__ __launch_bounds__(256) void near64_kernel(
   yL[ip] += acc;
 }
 
-// fp32: coordinates as fp32 offsets from the leaf centre plus the fp32
+// fp32 entries: coordinates as fp32 offsets from the leaf centre plus the fp32
 // difference of the two leaf centres, so near pairs keep ~1e-7 relative
-// accuracy in r.
+// accuracy in r.  A is the type of x and of the sums (float, or double for
+// fp64 accumulation of the fp32 entries).
+template <class A>
 __global__ __launch_bounds__(256) void near32_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
     const double* __restrict__ leafc, const int32_t* __restrict__ perm, KernelDev kd,
-    const float* __restrict__ x, double* __restrict__ yL) {
+    const A* __restrict__ x, double* __restrict__ yL) {
   const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -421,8 +564,7 @@ __global__ __launch_bounds__(256) void near32_kernel(
   const float xi = lx[ip], yi = ly[ip], zi = lz[ip];
   const double cxk = leafc[3 * (int64_t)it.leaf], cyk = leafc[3 * (int64_t)it.leaf + 1],
                czk = leafc[3 * (int64_t)it.leaf + 2];
-  const float sc = (float)kd.scale;
-  float acc = 0.0f;
+  A acc = 0;
   for (int sl = 0; sl <= 26; ++sl) {
     const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
     if (nn < 0) continue;
@@ -431,24 +573,27 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const float dcy = (float)(cyk - leafc[3 * (int64_t)nn + 1]);
     const float dcz = (float)(czk - leafc[3 * (int64_t)nn + 2]);
     const float ax = xi + dcx, ay = yi + dcy, az = zi + dcz;
-    float part = 0.0f;
+    A part = 0;
     for (int s0 = 0; s0 < m; s0 += 64) {
       const int cnt = min(64, m - s0);
       const int jp = cs + s0 + (lane < cnt ? lane : 0);
-      const float sx = lx[jp], sy = ly[jp], sz = lz[jp], sq = x[jp];
+      const float sx = lx[jp], sy = ly[jp], sz = lz[jp];
+      const A sq = x[jp];
       for (int u = 0; u < cnt; ++u) {
-        const float X = readlane_f(sx, u), Y = readlane_f(sy, u), Z = readlane_f(sz, u),
-                    Q = readlane_f(sq, u);
+        const float X = readlane_f(sx, u), Y = readlane_f(sy, u), Z = readlane_f(sz, u);
+        A Q;
+        if constexpr (sizeof(A) == 8) Q = readlane_d(sq, u);
+        else Q = readlane_f(sq, u);
         float dx = ax - X, dy = ay - Y, dz = az - Z;
         float R2 = dx * dx + dy * dy + dz * dz;
         float v = (cs + s0 + u == ip) ? 0.0f : phi32(kd.q, R2);
-        part += v * Q;
+        part += (A)v * Q;
       }
     }
     acc += part;
   }
   if (!act) return;
-  yL[ip] += (double)(sc * acc);
+  yL[ip] += kd.scale * (double)acc;
 }
 
 // y[i] = alpha * (sum over levels of y_j at i's position) + d_i * b[i], where
@@ -494,14 +639,6 @@ __global__ void convert_kernel<double>(const void* __restrict__ src, int sprec, 
 
 inline unsigned wave_grid(int64_t items) {
   return (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
-}
-
-template <class T>
-T* upload(const std::vector<T>& v, hipStream_t st) {
-  T* d = dalloc<T>(std::max<int64_t>((int64_t)v.size(), 1));
-  if (!v.empty())
-    H3D_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
-  return d;
 }
 
 // Direct FP64 O(N) sum for one row per workgroup: y_i = alpha * sum_{j != i}
@@ -592,26 +729,35 @@ void upload_chunk_table(Engine& e) {
   H3D_HIP(hipStreamSynchronize(e.stream));
 }
 
-void build_plan(Engine& e) {
+void Plan::free_device() {
+  dfree(d_p1);
+  dfree(d_p1r);
+  dfree(d_p2);
+  dfree(d_tp1);
+  dfree(d_tp1r);
+  dfree(d_tp2);
+}
+
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out) {
   const int nL = e.nL;
   hipStream_t st = e.stream;
-  e.p1.clear();
-  e.p1r.clear();
-  e.p2.clear();
-  e.nitems.clear();
-  e.tp1.clear();
-  e.tp1r.clear();
-  e.tp2.clear();
+  out.p1.clear();
+  out.p1r.clear();
+  out.p2.clear();
+  out.tp1.clear();
+  out.tp1r.clear();
+  out.tp2.clear();
   int32_t pslot = 0, tslot = 0;
   int64_t src_cursor = 0;
   for (int j = 1; j <= nL; ++j) {
     const Level& lv = e.lev[j];
+    const LevelFactors& lf = *lfs[j];
     const int64_t lo = level_offset(j);
     std::vector<int32_t> nsrc(lv.nboxes, 0);
     for (int32_t ki : lv.ilidx) nsrc[ki]++;
     for (int64_t k = 0; k < lv.nboxes; ++k) {
       const int n = e.bcount[lo + k];
-      const int R = lv.Rt[k];
+      const int R = lf.Rt[k];
       if (n > 0 && R > 0) {
         // phase 1: runs of consecutive blocks up to kP1Elems factor entries;
         // a block with more than kP1Rows rows gets one item per row range
@@ -627,21 +773,21 @@ void build_plan(Engine& e) {
               it.b1 = (int32_t)lv.bbase + end;
               it.s0 = 0;
               it.s1 = kP1Rows;
-              it.toff = lv.toff[k];
+              it.toff = lf.toff[k];
               it.pslot = -1;
-              e.p1.push_back(it);
+              out.p1.push_back(it);
             }
             g0 = -1;
             gel = 0;
           };
           for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
-            const int r = lv.rank[b];
+            const int r = lf.rank[b];
             const int m = e.bcount[lo + lv.ilidx[b]];
             if (r == 0) continue;
             if (m > kP1Rows) {
               flush(b);
               const int nsplit = (m + kP1Rows - 1) / kP1Rows;
-              e.p1r.push_back({pslot, nsplit, lv.toff[k] + lv.col[b], r});
+              out.p1r.push_back({pslot, nsplit, lf.toff[k] + lf.col[b], r});
               for (int q = 0; q < nsplit; ++q) {
                 P1Item it{};
                 it.chunk = lv.chunk[k];
@@ -649,10 +795,10 @@ void build_plan(Engine& e) {
                 it.b1 = it.b0 + 1;
                 it.s0 = q * kP1Rows;
                 it.s1 = std::min(m, (q + 1) * kP1Rows);
-                it.toff = lv.toff[k];
+                it.toff = lf.toff[k];
                 it.pslot = pslot;
                 pslot += r;
-                e.p1.push_back(it);
+                out.p1.push_back(it);
               }
               continue;
             }
@@ -665,25 +811,25 @@ void build_plan(Engine& e) {
         // phase 2: rows in chunks of 128 (64 lanes x 2 fp64 rows at most)
         for (int i0 = 0; i0 < n; i0 += kP2Rows) {
           P2Item it{};
-          it.uoff = lv.uoff[k];
+          it.uoff = lf.uoff[k];
           it.chunk = lv.chunk[k];
           it.ld = n + (n & 1);
           it.i0 = i0;
           it.nr = std::min(kP2Rows, n - i0);
           it.R = R;
-          it.toff = lv.toff[k];
+          it.toff = lf.toff[k];
           it.yoff = e.bstart[lo + k];
           it.level = j;
-          e.p2.push_back(it);
+          out.p2.push_back(it);
         }
         // transpose phase 1
         const int nsplit = (n + kP1Rows - 1) / kP1Rows;
         for (int c0 = 0; c0 < R; c0 += 64) {
           const int nc = std::min(64, R - c0);
-          if (nsplit > 1) e.tp1r.push_back({tslot, nsplit, lv.toff[k] + c0, nc});
+          if (nsplit > 1) out.tp1r.push_back({tslot, nsplit, lf.toff[k] + c0, nc});
           for (int q = 0; q < nsplit; ++q) {
             TP1Item it{};
-            it.uoff = lv.uoff[k];
+            it.uoff = lf.uoff[k];
             it.chunk = lv.chunk[k];
             it.ld = n + (n & 1);
             it.c0 = c0;
@@ -691,10 +837,10 @@ void build_plan(Engine& e) {
             it.s0 = q * kP1Rows;
             it.s1 = std::min(n, (q + 1) * kP1Rows);
             it.yoff = e.bstart[lo + k];
-            it.toff = lv.toff[k];
+            it.toff = lf.toff[k];
             it.pslot = nsplit > 1 ? tslot : -1;
             if (nsplit > 1) tslot += nc;
-            e.tp1.push_back(it);
+            out.tp1.push_back(it);
           }
         }
       }
@@ -704,42 +850,51 @@ void build_plan(Engine& e) {
       const int m = e.bcount[lo + k];
       if (m > 0 && nsrc[k] > 0)
         for (int i0 = 0; i0 < m; i0 += 64)
-          e.tp2.push_back({(int32_t)src_cursor, nsrc[k], e.bstart[lo + k], i0, std::min(64, m - i0), j});
+          out.tp2.push_back({(int32_t)src_cursor, nsrc[k], e.bstart[lo + k], i0, std::min(64, m - i0), j});
       src_cursor += nsrc[k];
     }
   }
-  {
-    const int64_t lo = level_offset(nL);
-    for (int64_t k = 0; k < e.lev[nL].nboxes; ++k) {
-      const int n = e.bcount[lo + k];
-      for (int i0 = 0; i0 < n; i0 += 64)
-        e.nitems.push_back({(int32_t)k, i0, std::min(64, n - i0), 0});
-    }
-  }
   // Larger items first: less tail at the end of a launch.
-  std::stable_sort(e.p2.begin(), e.p2.end(), [](const P2Item& a, const P2Item& b) {
+  std::stable_sort(out.p2.begin(), out.p2.end(), [](const P2Item& a, const P2Item& b) {
     return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
   });
-  e.npartial = std::max(pslot, tslot);
-  dfree(e.d_p1);
-  dfree(e.d_p1r);
-  dfree(e.d_p2);
-  dfree(e.d_near);
-  dfree(e.d_tp1);
-  dfree(e.d_tp1r);
-  dfree(e.d_tp2);
-  e.d_p1 = upload(e.p1, st);
-  e.d_p1r = upload(e.p1r, st);
-  e.d_p2 = upload(e.p2, st);
-  e.d_near = upload(e.nitems, st);
-  e.d_tp1 = upload(e.tp1, st);
-  e.d_tp1r = upload(e.tp1r, st);
-  e.d_tp2 = upload(e.tp2, st);
+  out.npartial = std::max(pslot, tslot);
+  out.free_device();
+  out.d_p1 = upload(out.p1, st);
+  out.d_p1r = upload(out.p1r, st);
+  out.d_p2 = upload(out.p2, st);
+  out.d_tp1 = upload(out.tp1, st);
+  out.d_tp1r = upload(out.tp1r, st);
+  out.d_tp2 = upload(out.tp2, st);
   H3D_HIP(hipStreamSynchronize(st));
 }
 
+void build_plan(Engine& e) {
+  std::vector<const LevelFactors*> lfs(e.nL + 1, nullptr);
+  for (int j = 1; j <= e.nL; ++j) lfs[j] = &e.lev[j];
+  build_plan_for(e, lfs, e.plan);
+  e.nitems.clear();
+  const int64_t lo = level_offset(e.nL);
+  for (int64_t k = 0; k < e.lev[e.nL].nboxes; ++k) {
+    const int n = e.bcount[lo + k];
+    for (int i0 = 0; i0 < n; i0 += 64) e.nitems.push_back({(int32_t)k, i0, std::min(64, n - i0), 0});
+  }
+  dfree(e.d_near);
+  e.d_near = upload(e.nitems, e.stream);
+  H3D_HIP(hipStreamSynchronize(e.stream));
+}
+
 void ensure_scratch(Engine& e, int nrhs) {
-  if (e.max_nrhs >= nrhs && e.d_xp) return;
+  int64_t T = e.T, np = e.plan.npartial;
+  if (e.mix.ready) {
+    T = std::max<int64_t>(T, (int64_t)e.mix.cls[0].T + e.mix.cls[1].T);
+    for (const FactorClass& c : e.mix.cls)
+      np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
+  }
+  T = std::max<int64_t>(T, 1);
+  np = std::max<int64_t>(np, 1);
+  if (e.max_nrhs >= nrhs && e.d_xp && e.t_cap >= T && e.part_cap >= np) return;
+  nrhs = std::max(nrhs, e.max_nrhs);
   dfree(e.d_xp);
   dfree(e.d_t);
   dfree(e.d_part);
@@ -751,11 +906,13 @@ void ensure_scratch(Engine& e, int nrhs) {
   e.d_xf = dalloc<float>(e.N * nrhs + 64);
   H3D_HIP(hipMemsetAsync(e.d_xp, 0, sizeof(double) * (e.N * nrhs + 64), e.stream));
   H3D_HIP(hipMemsetAsync(e.d_xf, 0, sizeof(float) * (e.N * nrhs + 64), e.stream));
-  e.d_t = dalloc<double>(std::max<int64_t>((int64_t)e.T * nrhs, 1));
-  e.d_part = dalloc<double>(std::max<int64_t>((int64_t)e.npartial * nrhs, 1));
+  e.d_t = dalloc<double>(T * nrhs);
+  e.d_part = dalloc<double>(np * nrhs);
   e.d_ylev = dalloc<double>(e.N * e.nL * nrhs);
   e.d_io = dalloc<double>(2 * e.N * nrhs);
   e.max_nrhs = nrhs;
+  e.t_cap = T;
+  e.part_cap = np;
 }
 
 // Builds the precision-`prec` copy of every chunk from the fp64 copy (or the
@@ -817,77 +974,153 @@ void release_precision(Engine& e, int prec) {
 
 namespace {
 
-// Phase-1 / phase-2 launch with a load width of `bytes` per lane.
+// One factor copy as the far-field kernels see it.
+struct FactorView {
+  const Plan* plan;
+  const BlockDev* blk;
+  const int32_t* btoff;
+  void* const* ubase;
+  void* const* vbase;
+  const double* scale;  // per t entry of this copy, or null
+  int accumulate;       // phase 2 adds into y_level (not the first copy)
+  int key;              // autotuning slot
+  const FactorClass* fc;  // recompressed class (source-grouped phase 1), or null
+};
+
+FactorView aca_view(Engine& e, int prec, bool acc64) {
+  return {&e.plan, e.d_blk, e.d_btoff, e.d_ubase[prec], e.d_vbase[prec], nullptr, 0,
+          prec + (acc64 && prec != 0 ? 3 : 0), nullptr};
+}
+
+// Class c of the mixed copy: class 0 (fp32 storage) accumulates in fp64 with t
+// at d_t[0, T0); class 1 (fp16 storage, a small part of the operator) works
+// in fp32 with its t as floats right after class 0's.
+FactorView class_view(Engine& e, int c) {
+  const FactorClass& fc = e.mix.cls[c];
+  return {&fc.plan, fc.d_blk, fc.d_btoff, fc.d_ubase, fc.d_vbase, e.mix.d_scale + fc.tbase, c,
+          6 + c, &fc};
+}
+
+inline double* class_t0(Engine& e) { return e.d_t; }
+inline float* class_t1(Engine& e) { return (float*)(e.d_t + e.mix.cls[0].T); }
+
+// Source-grouped phase 1 of a mixed class with a load width of `bytes` per lane.
 template <class S, class A>
-void launch_p1(Engine& e, int prec, int bytes, const A* x, A* t, A* part) {
-  const int n1 = (int)e.p1.size();
+void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x, A* t, A* part) {
+  const FactorClass& fc = *f.fc;
+  const int n1 = (int)fc.mp1.size();
   if (!n1) return;
   const unsigned g = wave_grid(n1);
   hipStream_t st = e.stream;
   constexpr int es = (int)sizeof(S);
-  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
-    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec], e.d_blk, x, t, part);
-  else if (bytes == 8 && 8 / es >= 1)
-    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
-                                                                      e.d_blk, x, t, part);
-  else if (bytes == 4 && 4 / es >= 1)
-    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
-                                                                      e.d_blk, x, t, part);
+  const MixP1Item* it = fc.d_mp1;
+  if (bytes >= 16)
+    mphase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x, t, part, f.scale,
+                                                     kmax);
+  else if (bytes == 8)
+    mphase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
+                                                                       t, part, f.scale, kmax);
+  else if (bytes == 4)
+    mphase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
+                                                                       t, part, f.scale, kmax);
   else
-    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(e.d_p1, n1, e.d_vbase[prec],
-                                                                      e.d_blk, x, t, part);
+    mphase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
+                                                                       t, part, f.scale, kmax);
+  H3D_LAUNCH_CHECK();
+}
+
+// Phase-1 / phase-2 launch with a load width of `bytes` per lane.
+template <class S, class A>
+void launch_p1(Engine& e, const FactorView& f, int bytes, const A* x, A* t, A* part) {
+  if (f.fc) {
+    launch_mp1<S, A>(e, f, bytes, e.tune_k[f.key], x, t, part);
+    return;
+  }
+  const int n1 = (int)f.plan->p1.size();
+  if (!n1) return;
+  const unsigned g = wave_grid(n1);
+  hipStream_t st = e.stream;
+  constexpr int es = (int)sizeof(S);
+  const P1Item* it = f.plan->d_p1;
+  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
+    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t, part, f.scale);
+  else if (bytes == 8 && 8 / es >= 1)
+    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
+                                                                      part, f.scale);
+  else if (bytes == 4 && 4 / es >= 1)
+    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
+                                                                      part, f.scale);
+  else
+    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
+                                                                      part, f.scale);
+  H3D_LAUNCH_CHECK();
+}
+
+template <class A>
+void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
+  if (f.fc) {
+    const FactorClass& fc = *f.fc;
+    if (fc.mp1r.empty()) return;
+    mreduce_kernel<A><<<wave_grid(fc.mp1r.size()), 256, 0, e.stream>>>(
+        fc.d_mp1r, (int)fc.mp1r.size(), part, fc.d_mtidx, t, f.scale);
+    H3D_LAUNCH_CHECK();
+    return;
+  }
+  const Plan& p = *f.plan;
+  if (p.p1r.empty()) return;
+  reduce_kernel<A><<<wave_grid(p.p1r.size()), 256, 0, e.stream>>>(p.d_p1r, (int)p.p1r.size(), part,
+                                                                   t, f.scale);
   H3D_LAUNCH_CHECK();
 }
 
 template <class S, class A>
-void launch_p2(Engine& e, int prec, int bytes, const A* t) {
-  const int n2 = (int)e.p2.size();
+void launch_p2(Engine& e, const FactorView& f, int bytes, const A* t) {
+  const int n2 = (int)f.plan->p2.size();
   if (!n2) return;
   const unsigned g = wave_grid(n2);
   hipStream_t st = e.stream;
   constexpr int es = (int)sizeof(S);
+  const P2Item* it = f.plan->d_p2;
+  const int acc = f.accumulate;
   if (bytes >= 16)
-    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec], t, e.d_ylev, e.N);
+    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev, e.N, acc);
   else if (bytes == 8)
-    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
-                                                                      t, e.d_ylev, e.N);
+    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
+                                                                      e.N, acc);
   else if (bytes == 4)
-    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
-                                                                      t, e.d_ylev, e.N);
+    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
+                                                                      e.N, acc);
   else
-    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(e.d_p2, n2, e.d_ubase[prec],
-                                                                      t, e.d_ylev, e.N);
+    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
+                                                                      e.N, acc);
   H3D_LAUNCH_CHECK();
 }
 
-inline int tune_key(int prec, bool acc64) { return prec + (acc64 && prec != 0 ? 3 : 0); }
+template <class S, class A>
+void run_p1(Engine& e, const FactorView& f, const A* x, A* t, A* part) {
+  launch_p1<S, A>(e, f, e.tune_p1[f.key], x, t, part);
+  launch_p1_reduce<A>(e, f, part, t);
+}
 
 template <class S, class A>
-void run_forward(Engine& e, int prec, const A* x, A* t, A* part) {
-  hipStream_t st = e.stream;
-  const int key = tune_key(prec, sizeof(A) == 8);
-  if (e.timing) H3D_HIP(hipEventRecord(e.ev[1], st));
-  launch_p1<S, A>(e, prec, e.tune_p1[key], x, t, part);
-  if (!e.p1r.empty())
-    reduce_kernel<A><<<wave_grid(e.p1r.size()), 256, 0, st>>>(e.d_p1r, (int)e.p1r.size(), part, t);
-  H3D_LAUNCH_CHECK();
-  if (e.timing) H3D_HIP(hipEventRecord(e.ev[2], st));
-  launch_p2<S, A>(e, prec, e.tune_p2[key], t);
+void run_p2(Engine& e, const FactorView& f, const A* t) {
+  launch_p2<S, A>(e, f, e.tune_p2[f.key], t);
 }
 
 // Average ms of `reps` launches of phase 1 (which = 1) or phase 2 (which = 2).
 template <class S, class A>
-double time_phase(Engine& e, int prec, int which, int bytes, int reps, const A* x, A* t, A* part) {
+double time_phase(Engine& e, const FactorView& f, int which, int bytes, int reps, const A* x, A* t,
+                  A* part) {
   hipEvent_t a, b;
   H3D_HIP(hipEventCreate(&a));
   H3D_HIP(hipEventCreate(&b));
   // one untimed launch first
-  if (which == 1) launch_p1<S, A>(e, prec, bytes, x, t, part);
-  else launch_p2<S, A>(e, prec, bytes, t);
+  if (which == 1) launch_p1<S, A>(e, f, bytes, x, t, part);
+  else launch_p2<S, A>(e, f, bytes, t);
   H3D_HIP(hipEventRecord(a, e.stream));
   for (int r = 0; r < reps; ++r) {
-    if (which == 1) launch_p1<S, A>(e, prec, bytes, x, t, part);
-    else launch_p2<S, A>(e, prec, bytes, t);
+    if (which == 1) launch_p1<S, A>(e, f, bytes, x, t, part);
+    else launch_p2<S, A>(e, f, bytes, t);
   }
   H3D_HIP(hipEventRecord(b, e.stream));
   H3D_HIP(hipEventSynchronize(b));
@@ -898,112 +1131,165 @@ double time_phase(Engine& e, int prec, int which, int bytes, int reps, const A* 
   return ms / reps;
 }
 
+// Times every load width (and, for a mixed class's phase 1, rows per lane of
+// 4 or 1 load vectors) and keeps the fastest; ms gets the best time per width.
 template <class S, class A>
-void tune_mode(Engine& e, int prec, int reps, const A* x, A* t, A* part, double* ms) {
-  const int key = tune_key(prec, sizeof(A) == 8);
+void tune_mode(Engine& e, const FactorView& f, int reps, const A* x, A* t, A* part, double* ms) {
   const int widths[4] = {16, 8, 4, 2};
   for (int which = 1; which <= 2; ++which) {
     double best = 1e300;
-    int bw = 16;
+    int bw = 16, bk = e.tune_k[f.key];
     for (int q = 0; q < 4; ++q) {
       const int w = widths[q];
       double v = -1;
       if (w >= (int)sizeof(S)) {
-        v = time_phase<S, A>(e, prec, which, w, reps, x, t, part);
-        if (v < best) {
-          best = v;
-          bw = w;
+        const int nk = (which == 1 && f.fc) ? 2 : 1;
+        for (int ki = 0; ki < nk; ++ki) {
+          if (f.fc) e.tune_k[f.key] = ki == 0 ? 4 : 1;
+          const double tv = time_phase<S, A>(e, f, which, w, reps, x, t, part);
+          if (v < 0 || tv < v) v = tv;
+          if (tv < best) {
+            best = tv;
+            bw = w;
+            if (which == 1 && f.fc) bk = e.tune_k[f.key];
+          }
         }
       }
       if (ms) ms[(which - 1) * 4 + q] = v;
     }
-    (which == 1 ? e.tune_p1 : e.tune_p2)[key] = bw;
+    (which == 1 ? e.tune_p1 : e.tune_p2)[f.key] = bw;
+    if (which == 1 && f.fc) e.tune_k[f.key] = bk;
   }
 }
 
 template <class S, class A>
-void run_transpose(Engine& e, int prec, const A* x, A* t, A* part) {
+void run_tp1(Engine& e, const FactorView& f, const A* x, A* t, A* part) {
+  const Plan& p = *f.plan;
+  const int n1 = (int)p.tp1.size();
   hipStream_t st = e.stream;
-  const int n1 = (int)e.tp1.size(), n2 = (int)e.tp2.size();
-  if (e.timing) H3D_HIP(hipEventRecord(e.ev[1], st));
   if (n1)
-    tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(e.d_tp1, n1, e.d_ubase[prec], x, t, part);
+    tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(p.d_tp1, n1, f.ubase, x, t, part, f.scale);
   H3D_LAUNCH_CHECK();
-  if (!e.tp1r.empty())
-    reduce_kernel<A><<<wave_grid(e.tp1r.size()), 256, 0, st>>>(e.d_tp1r, (int)e.tp1r.size(), part, t);
+  if (!p.tp1r.empty())
+    reduce_kernel<A><<<wave_grid(p.tp1r.size()), 256, 0, st>>>(p.d_tp1r, (int)p.tp1r.size(), part,
+                                                               t, f.scale);
   H3D_LAUNCH_CHECK();
-  if (e.timing) H3D_HIP(hipEventRecord(e.ev[2], st));
+}
+
+template <class S, class A>
+void run_tp2(Engine& e, const FactorView& f, const A* t) {
+  const Plan& p = *f.plan;
+  const int n2 = (int)p.tp2.size();
   if (n2)
-    tphase2_kernel<S, A><<<wave_grid(n2), 256, 0, st>>>(e.d_tp2, n2, e.d_vbase[prec], e.d_blk,
-                                                        e.d_by_src, e.d_btoff, t, e.d_ylev, e.N);
+    tphase2_kernel<S, A><<<wave_grid(n2), 256, 0, e.stream>>>(p.d_tp2, n2, f.vbase, f.blk,
+                                                              e.d_by_src, f.btoff, t, e.d_ylev,
+                                                              e.N, f.accumulate);
   H3D_LAUNCH_CHECK();
+}
+
+// Phase 1 (which = 1) or phase 2 (which = 2) of one copy, forward or transpose.
+template <class S, class A>
+void run_phase(Engine& e, const FactorView& f, int which, bool transpose, const A* x, A* t, A* part) {
+  if (!transpose) {
+    if (which == 1) run_p1<S, A>(e, f, x, t, part);
+    else run_p2<S, A>(e, f, t);
+  } else {
+    if (which == 1) run_tp1<S, A>(e, f, x, t, part);
+    else run_tp2<S, A>(e, f, t);
+  }
+}
+
+void check_mode(const Engine& e, int mode, int& prec, bool& acc64) {
+  prec = mode & 0xff;
+  if (prec < 0 || prec > kPrecMixed || (mode & ~(0xff | H3D_ACC_FP64)))
+    throw Error(H3D_ERR_INVALID, "unknown precision");
+  acc64 = prec == H3D_PREC_FP64 || prec == kPrecMixed || (mode & H3D_ACC_FP64);
+}
+
+void ensure_copy(Engine& e, int prec) {
+  if (prec == kPrecMixed) {
+    if (!e.mix.ready) build_mixed(e, 5.0 * e.tol);
+  } else if (e.fac_bytes[prec] == 0) {
+    prepare_precision(e, prec);
+  }
 }
 
 }  // namespace
 
 void autotune(Engine& e, int mode, int reps, double* ms) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "autotune before assemble");
-  const int prec = mode & 0xff;
-  if (prec < 0 || prec > 2 || (mode & ~(0xff | H3D_ACC_FP64)))
-    throw Error(H3D_ERR_INVALID, "unknown precision");
+  int prec;
+  bool acc64;
+  check_mode(e, mode, prec, acc64);
   if (reps < 1) throw Error(H3D_ERR_INVALID, "reps must be >= 1");
-  const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
-  if (e.fac_bytes[prec] == 0) prepare_precision(e, prec);
+  ensure_copy(e, prec);
   ensure_scratch(e, 1);
   // timing does not depend on the values: x is whatever the last matvec left
-  if (acc64) {
-    if (prec == H3D_PREC_FP64) tune_mode<double, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
-    else if (prec == H3D_PREC_FP32) tune_mode<float, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
-    else tune_mode<uint16_t, double>(e, prec, reps, e.d_xp, e.d_t, e.d_part, ms);
+  if (prec == kPrecMixed) {
+    tune_mode<float, double>(e, class_view(e, 0), reps, e.d_xp, class_t0(e), e.d_part, ms);
+    tune_mode<_Float16, float>(e, class_view(e, 1), reps, e.d_xf, class_t1(e), (float*)e.d_part,
+                               ms ? ms + 8 : nullptr);
+  } else if (acc64) {
+    const FactorView f = aca_view(e, prec, true);
+    if (prec == H3D_PREC_FP64) tune_mode<double, double>(e, f, reps, e.d_xp, e.d_t, e.d_part, ms);
+    else if (prec == H3D_PREC_FP32) tune_mode<float, double>(e, f, reps, e.d_xp, e.d_t, e.d_part, ms);
+    else tune_mode<uint16_t, double>(e, f, reps, e.d_xp, e.d_t, e.d_part, ms);
   } else {
+    const FactorView f = aca_view(e, prec, false);
     float* t = (float*)e.d_t;
     float* part = (float*)e.d_part;
-    if (prec == H3D_PREC_FP32) tune_mode<float, float>(e, prec, reps, e.d_xf, t, part, ms);
-    else tune_mode<uint16_t, float>(e, prec, reps, e.d_xf, t, part, ms);
+    if (prec == H3D_PREC_FP32) tune_mode<float, float>(e, f, reps, e.d_xf, t, part, ms);
+    else tune_mode<uint16_t, float>(e, f, reps, e.d_xf, t, part, ms);
   }
   H3D_HIP(hipStreamSynchronize(e.stream));
 }
 
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode, bool transpose) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");
-  const int prec = mode & 0xff;
-  if (prec < 0 || prec > 2 || (mode & ~(0xff | H3D_ACC_FP64)))
-    throw Error(H3D_ERR_INVALID, "unknown precision");
-  const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
-  if (e.fac_bytes[prec] == 0) prepare_precision(e, prec);
+  int prec;
+  bool acc64;
+  check_mode(e, mode, prec, acc64);
+  ensure_copy(e, prec);
   ensure_scratch(e, 1);
   hipStream_t st = e.stream;
   const int64_t N = e.N;
   const unsigned gN = (unsigned)((N + 255) / 256);
   if (e.timing && !e.ev[0])
     for (auto& ev : e.ev) H3D_HIP(hipEventCreate(&ev));
+  const bool mixed = prec == kPrecMixed;
   for (int r = 0; r < nrhs; ++r) {
     const double* b = d_b + (int64_t)r * N;
     double* y = d_y + (int64_t)r * N;
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[0], st));
-    gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, acc64 ? nullptr : e.d_xf);
+    gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, (acc64 && !mixed) ? nullptr : e.d_xf);
     H3D_LAUNCH_CHECK();
     H3D_HIP(hipMemsetAsync(e.d_ylev, 0, sizeof(double) * N * e.nL, st));
-    if (acc64) {
-      if (prec == H3D_PREC_FP64) {
-        if (!transpose) run_forward<double, double>(e, prec, e.d_xp, e.d_t, e.d_part);
-        else run_transpose<double, double>(e, prec, e.d_xp, e.d_t, e.d_part);
-      } else if (prec == H3D_PREC_FP32) {
-        if (!transpose) run_forward<float, double>(e, prec, e.d_xp, e.d_t, e.d_part);
-        else run_transpose<float, double>(e, prec, e.d_xp, e.d_t, e.d_part);
+    // event slots: 1 phase 1 start, 6 after the first copy's phase 1, 2 phase 2
+    // start, 7 after the first copy's phase 2, 3 near start, 4 combine start, 5 end
+    for (int which = 1; which <= 2; ++which) {
+      if (e.timing) H3D_HIP(hipEventRecord(e.ev[which], st));
+      if (mixed) {
+        run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_t0(e),
+                                 e.d_part);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
+        run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_t1(e),
+                                   (float*)e.d_part);
+      } else if (acc64) {
+        const FactorView f = aca_view(e, prec, true);
+        if (prec == H3D_PREC_FP64)
+          run_phase<double, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+        else if (prec == H3D_PREC_FP32)
+          run_phase<float, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+        else
+          run_phase<uint16_t, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
       } else {
-        if (!transpose) run_forward<uint16_t, double>(e, prec, e.d_xp, e.d_t, e.d_part);
-        else run_transpose<uint16_t, double>(e, prec, e.d_xp, e.d_t, e.d_part);
-      }
-    } else {
-      float* t = (float*)e.d_t;
-      float* part = (float*)e.d_part;
-      if (prec == H3D_PREC_FP32) {
-        if (!transpose) run_forward<float, float>(e, prec, e.d_xf, t, part);
-        else run_transpose<float, float>(e, prec, e.d_xf, t, part);
-      } else {
-        if (!transpose) run_forward<uint16_t, float>(e, prec, e.d_xf, t, part);
-        else run_transpose<uint16_t, float>(e, prec, e.d_xf, t, part);
+        const FactorView f = aca_view(e, prec, false);
+        float* t = (float*)e.d_t;
+        float* part = (float*)e.d_part;
+        if (prec == H3D_PREC_FP32) run_phase<float, float>(e, f, which, transpose, e.d_xf, t, part);
+        else run_phase<uint16_t, float>(e, f, which, transpose, e.d_xf, t, part);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
       }
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[3], st));
@@ -1013,19 +1299,23 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     const int64_t lo = level_offset(e.nL);
     KernelDev kd = e.kern;
     if (nn) {
-      if (acc64 && (kd.q == 7 || kd.q == 16))
-        near64_kernel<InvR><<<wave_grid(nn), 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
-                                                           e.d_bstart + lo, e.d_bcount + lo, e.d_px,
-                                                           e.d_py, e.d_pz, e.d_perm, kd, e.d_xp, yL);
+      const unsigned g = wave_grid(nn);
+      if (mixed)
+        near32_kernel<double><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                                 e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                 e.d_leafc, e.d_perm, kd, e.d_xp, yL);
+      else if (acc64 && (kd.q == 7 || kd.q == 16))
+        near64_kernel<InvR><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
+                                               kd, e.d_xp, yL);
       else if (acc64)
-        near64_kernel<AnyQ><<<wave_grid(nn), 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
-                                                           e.d_bstart + lo, e.d_bcount + lo, e.d_px,
-                                                           e.d_py, e.d_pz, e.d_perm, kd, e.d_xp, yL);
+        near64_kernel<AnyQ><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
+                                               kd, e.d_xp, yL);
       else
-        near32_kernel<<<wave_grid(nn), 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
-                                                     e.d_bstart + lo, e.d_bcount + lo, e.d_lx,
-                                                     e.d_ly, e.d_lz, e.d_leafc, e.d_perm, kd,
-                                                     e.d_xf, yL);
+        near32_kernel<float><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                                e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz, e.d_leafc,
+                                                e.d_perm, kd, e.d_xf, yL);
       H3D_LAUNCH_CHECK();
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[4], st));
@@ -1035,13 +1325,15 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     if (e.timing) {
       H3D_HIP(hipEventRecord(e.ev[5], st));
       H3D_HIP(hipEventSynchronize(e.ev[5]));
-      float ms[5];
+      float ms[7];
       H3D_HIP(hipEventElapsedTime(&ms[0], e.ev[1], e.ev[2]));  // phase 1 (+ partial reduce)
       H3D_HIP(hipEventElapsedTime(&ms[1], e.ev[2], e.ev[3]));  // phase 2
       H3D_HIP(hipEventElapsedTime(&ms[2], e.ev[3], e.ev[4]));  // near field
       H3D_HIP(hipEventElapsedTime(&ms[3], e.ev[4], e.ev[5]));  // combine
       H3D_HIP(hipEventElapsedTime(&ms[4], e.ev[0], e.ev[5]));  // whole matvec
-      for (int q = 0; q < 5; ++q) e.ktime[q] += ms[q];
+      H3D_HIP(hipEventElapsedTime(&ms[5], e.ev[6], e.ev[2]));  // phase 1 of the second copy
+      H3D_HIP(hipEventElapsedTime(&ms[6], e.ev[7], e.ev[3]));  // phase 2 of the second copy
+      for (int q = 0; q < 7; ++q) e.ktime[q] += ms[q];
       e.kcount++;
     }
   }

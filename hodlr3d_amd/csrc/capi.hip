@@ -46,16 +46,12 @@ void Engine::free_all() {
     dfree(d_vbase[p]);
     fac_bytes[p] = 0;
   }
+  mix.free_all();
   dfree(d_blk);
   dfree(d_by_src);
   dfree(d_btoff);
-  dfree(d_p1);
-  dfree(d_p1r);
-  dfree(d_p2);
+  plan.free_device();
   dfree(d_near);
-  dfree(d_tp1);
-  dfree(d_tp1r);
-  dfree(d_tp2);
   dfree(d_xp);
   dfree(d_xf);
   dfree(d_t);
@@ -223,8 +219,12 @@ int h3d_assemble(h3d_handle* h) {
 int h3d_prepare_precision(h3d_handle* h, int prec) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    if (prec < 0 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if (prec < 0 || prec > H3D_PREC_MIXED) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
     if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    if (prec == H3D_PREC_MIXED) {
+      if (!h->e.mix.ready) h3d::build_mixed(h->e, 5.0 * h->e.tol);
+      return;
+    }
     h3d::prepare_precision(h->e, prec);
   });
 }
@@ -251,13 +251,60 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
     const int p = prec & 0xff;
-    if (p < 0 || p > 2 || (prec & ~(0xff | H3D_ACC_FP64)))
+    if (p < 0 || p > H3D_PREC_MIXED || (prec & ~(0xff | H3D_ACC_FP64)))
       throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
-    auto ok = [&](int b) { return (b == 16 || b == 8 || b == 4 || b == 2) && b >= h3d::elem_size(p); };
+    // the mixed copy's fp32 class bounds its widths from below
+    const int es = p == H3D_PREC_MIXED ? 4 : h3d::elem_size(p);
+    auto ok = [&](int b) { return (b == 16 || b == 8 || b == 4 || b == 2) && b >= es; };
     if (!ok(p1_bytes) || !ok(p2_bytes)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
+    if (p == H3D_PREC_MIXED) {  // both classes: fp32 (slot 4) and fp16 (slot 6)
+      for (int key : {4, 6}) {
+        h->e.tune_p1[key] = p1_bytes;
+        h->e.tune_p2[key] = p2_bytes;
+      }
+      return;
+    }
     const int key = p + ((prec & H3D_ACC_FP64) && p != 0 ? 3 : 0);
     h->e.tune_p1[key] = p1_bytes;
     h->e.tune_p2[key] = p2_bytes;
+  });
+}
+
+int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* out) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "not assembled");
+    h3d::build_mixed(h->e, tol);
+    if (out) h3d_get_mixed_stats(h, out);
+  });
+}
+
+int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
+  return guarded([&] {
+    if (!h || !s) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    const h3d::MixedCopy& m = h->e.mix;
+    std::memset(s, 0, sizeof(*s));
+    if (!m.ready) return;
+    s->tol = m.tol;
+    const double a2 = m.fro2_far + m.fro2_near;
+    s->predicted_rel_err = a2 > 0 ? std::sqrt((m.err2_drop + m.err2_fp16 + m.err2_fp32) / a2) : 0;
+    s->err_drop = a2 > 0 ? std::sqrt(m.err2_drop / a2) : 0;
+    s->err_fp16 = a2 > 0 ? std::sqrt(m.err2_fp16 / a2) : 0;
+    s->err_fp32 = a2 > 0 ? std::sqrt(m.err2_fp32 / a2) : 0;
+    s->fro_far = std::sqrt(m.fro2_far);
+    s->fro_near = std::sqrt(m.fro2_near);
+    s->aca_rank_sum = m.aca_rank_sum;
+    s->rank_fp32 = m.cls[0].rank_sum;
+    s->rank_fp16 = m.cls[1].rank_sum;
+    s->dropped = m.dropped;
+    s->raw_blocks = m.raw_blocks;
+    s->raw_rank = m.raw_rank;
+    s->bytes = m.cls[0].bytes + m.cls[1].bytes;
+    s->u_elems[0] = m.cls[0].u_elems;
+    s->u_elems[1] = m.cls[1].u_elems;
+    s->v_elems[0] = m.cls[0].v_elems;
+    s->v_elems[1] = m.cls[1].v_elems;
+    s->build_ms = m.ms;
   });
 }
 
@@ -273,9 +320,9 @@ int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
     h3d::Engine& e = h->e;
     if (ms)
-      for (int q = 0; q < 5; ++q) ms[q] = e.ktime[q];
+      for (int q = 0; q < 7; ++q) ms[q] = e.ktime[q];
     if (count) *count = e.kcount;
-    for (int q = 0; q < 5; ++q) e.ktime[q] = 0;
+    for (int q = 0; q < 7; ++q) e.ktime[q] = 0;
     e.kcount = 0;
     e.timing = enable != 0;
   });
@@ -284,7 +331,11 @@ int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
 int h3d_release_precision(h3d_handle* h, int prec) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    if (prec < 0 || prec > 2) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if (prec < 0 || prec > H3D_PREC_MIXED) throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+    if (prec == H3D_PREC_MIXED) {
+      h3d::release_mixed(h->e);
+      return;
+    }
     h3d::release_precision(h->e, prec);
   });
 }
@@ -373,7 +424,7 @@ int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max
   int st = H3D_OK;
   int rc = guarded([&] {
     if (!h || !b || !x) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
-    if ((inner_prec & 0xff) > 2 || (inner_prec & ~(0xff | H3D_ACC_FP64)))
+    if ((inner_prec & 0xff) > H3D_PREC_MIXED || (inner_prec & ~(0xff | H3D_ACC_FP64)))
       throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
     h3d::Engine& e = h->e;
     if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "solve before assemble");

@@ -1,0 +1,1044 @@
+// Recompressed mixed-precision copy of the far field.
+//
+// For every low-rank block B = U V^T (ACA vectors, fp64 copy):
+//   gram_kernel     Gu = U^T U, Gv = V^T V
+//   eig_kernel      Gv = Lv Lv^T (Cholesky), S = Lv^T Gu Lv = Z diag(lam) Z^T
+//                   (cyclic Jacobi), eigenpairs sorted by lam descending;
+//                   Tu = Lv Z, Tv = Lv^-T Z
+//   project_kernel  U' = U Tu, V' = V Tv, so B = U' V'^T exactly (Z orthogonal),
+//                   V' has orthonormal columns and ||U'_k||^2 = lam_k
+// A column k of a block then costs lam_k in ||.||_F^2 if dropped and about
+// lam_k * eps_c^2 if stored in a format of relative precision eps_c.  One
+// global budget tol^2 ||A||_F^2 (||A||_F^2 = sum of every block's lam plus the
+// near field's sum of squares) is spent greedily on the columns with the
+// smallest lam_k / (n + m) (error per stored element): the cheapest ones are
+// dropped, the next ones stored in fp16, the rest in fp32.  For iid right-hand
+// sides E||(A - A~) x|| / ||A x|| = ||A - A~||_F / ||A||_F, so `tol` is the
+// expected relative matvec error added to the ACA error.
+//
+// Blocks whose Gv is numerically singular (or of rank above kMaxEigRank) are
+// kept "raw": every ACA column in fp32, unscaled.
+//
+// The eigen step runs twice (once to collect every lam for the thresholds,
+// once to write the copy) so that only O(rank) numbers per block are kept
+// between the passes; both runs are deterministic and give the same bits.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace h3d {
+
+namespace {
+
+constexpr int kMaxEigRank = 142;          // larger blocks stay raw
+// Expected relative Frobenius error of a stored column pair u v^T (both in
+// the format): round-to-nearest leaves an error uniform in +-ulp/2, and over
+// a binade E[ulp(x)^2 / x^2] = 2^-2p * 0.54, so ||du|| / ||u|| ~ 0.425 * 2^-p
+// per side and sqrt(2) * 0.425 * 2^-p ~ 0.6 * 2^-p for the pair (p = 11 for
+// fp16, 24 for fp32); the constants carry a 15% margin.
+constexpr double kFp16Err = 3.4e-4;
+constexpr double kFp32Err = 4.2e-8;
+constexpr double kCholFail = 1e-12;       // pivot / original diagonal below this: raw block
+constexpr int kMaxSweeps = 24;
+constexpr int kBinsPerOctave = 8;
+constexpr int kMixRows = 512;                       // rows of a mixed phase-1 item
+constexpr int kMixMaxCols = 1024;                   // columns of a mixed phase-1 item (LDS map)
+constexpr int64_t kMixItemBytes = 32768;            // factor bytes of a mixed phase-1 item
+constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (whole source boxes)
+
+// Per-block work description (pass 1 and pass 2).
+struct CDesc {
+  int64_t xu;     // fp64 element offset of U_b's first column in its chunk
+  int64_t xv;     // fp64 element offset of V_b in its chunk
+  int64_t w;      // workspace offset: Gu/Tu (r*r), Gv/Tv (r*r), lam (r), status (1)
+  int32_t n, m, r, ldu, ldv, chunk;
+};
+
+// Per-block output description (pass 2).
+struct PDesc {
+  int64_t u_out[2];  // class element offset of the block's first U column (in its U chunk)
+  int64_t v_out[2];  // class element offset of the block's V (in its V chunk)
+  int32_t vch[2];    // V chunk of the block per class
+  int32_t r32, r16;  // columns kept per class (prefix of the sorted columns)
+  int32_t raw;
+  int32_t pad;
+  double su, sv;     // power-of-two scales of the stored columns
+};
+
+// One project work item: rows [row0, row0 + 256) of one side of one block.
+struct PItem {
+  int32_t blk;   // index into the batch's CDesc / PDesc
+  int32_t side;  // 0: U, 1: V
+  int32_t row0;
+  int32_t pad;
+};
+
+// ------------------------------------------------------------------ Gram
+// One workgroup per (block, side): G = X^T X for the n x r (or m x r) column
+// block X, row tiles of 32 staged in LDS, each thread owning up to EMAX of
+// the r(r+1)/2 entries (l <= l').
+template <int EMAX>
+__global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
+                                                   void* const* __restrict__ base64,
+                                                   const int32_t* __restrict__ list, int nlist,
+                                                   double* __restrict__ ws) {
+  constexpr int TR = 32;
+  extern __shared__ double tile[];  // r x (TR + 1)
+  const int item = blockIdx.x >> 1;
+  if (item >= nlist) return;
+  const int side = blockIdx.x & 1;
+  const CDesc d = ds[list[item]];
+  const int r = d.r;
+  const int rows = side ? d.m : d.n;
+  const int ld = side ? d.ldv : d.ldu;
+  const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu);
+  const int E = r * (r + 1) / 2;
+  int el[EMAX], er[EMAX];
+  double acc[EMAX];
+#pragma unroll
+  for (int q = 0; q < EMAX; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    int b = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (b * (b + 1) / 2 > e) --b;
+    while ((b + 1) * (b + 2) / 2 <= e) ++b;
+    el[q] = e - b * (b + 1) / 2;  // l <= l' = b
+    er[q] = b;
+    acc[q] = 0.0;
+  }
+  for (int r0 = 0; r0 < rows; r0 += TR) {
+    const int nr = min(TR, rows - r0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TR * r; idx += 256) {
+      const int l = idx / TR, i = idx - l * TR;
+      tile[l * (TR + 1) + i] = i < nr ? X[(int64_t)l * ld + r0 + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < EMAX; ++q) {
+      if (threadIdx.x + 256 * q < E) {
+        const double* a = tile + el[q] * (TR + 1);
+        const double* b = tile + er[q] * (TR + 1);
+        double s = 0.0;
+        for (int i = 0; i < TR; ++i) s += a[i] * b[i];
+        acc[q] += s;
+      }
+    }
+  }
+  double* G = ws + d.w + (side ? (int64_t)r * r : 0);
+#pragma unroll
+  for (int q = 0; q < EMAX; ++q) {
+    if (threadIdx.x + 256 * q < E) {
+      G[el[q] + er[q] * r] = acc[q];
+      G[er[q] + el[q] * r] = acc[q];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ eigen
+// One workgroup per block.  L (r x r, Cholesky factor of Gv), S (r x r) and
+// Z (r x r), row-major with stride r, live in LDS, or in global scratch
+// (GLOBAL, 3 * gstride doubles per work item) for ranks whose matrices do not
+// fit; per-pair rotation data is always in LDS.
+__device__ inline void wg_sync() { __syncthreads(); }
+
+template <bool GLOBAL>
+__global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
+                                                  const int32_t* __restrict__ list, int nlist,
+                                                  double* __restrict__ ws,
+                                                  double* __restrict__ gscratch,
+                                                  int64_t gstride) {
+  extern __shared__ double sm[];
+  const int item = blockIdx.x;
+  if (item >= nlist) return;
+  const CDesc d = ds[list[item]];
+  const int r = d.r;
+  const int rr = r + (r & 1);
+  double* L = GLOBAL ? gscratch + 3 * gstride * item : sm;
+  double* S = L + r * r;
+  double* Z = S + r * r;
+  double* cs = GLOBAL ? sm : Z + r * r;  // rr/2
+  double* sn = cs + rr / 2;      // rr/2
+  double* dg = sn + rr / 2;      // r: original diagonal of Gv
+  double* lam = dg + r;          // r
+  int* pp = (int*)(lam + r);     // rr/2
+  int* pq = pp + rr / 2;         // rr/2
+  int* perm = pq + rr / 2;       // r
+  __shared__ int flag;
+  double* W = ws + d.w;
+  const int tid = threadIdx.x, nt = blockDim.x;
+
+  for (int idx = tid; idx < r * r; idx += nt) {
+    L[idx] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
+    S[idx] = W[idx];                   // Gu
+  }
+  if (tid == 0) flag = 0;
+  wg_sync();
+  for (int i = tid; i < r; i += nt) dg[i] = L[i * r + i];
+  wg_sync();
+  // Cholesky Gv = L L^T (lower), right-looking
+  for (int k = 0; k < r; ++k) {
+    const double dkk = L[k * r + k];
+    if (!(dkk > kCholFail * dg[k])) {  // also catches NaN
+      if (tid == 0) flag = 1;
+      break;
+    }
+    const double lkk = sqrt(dkk);
+    wg_sync();
+    for (int i = k + 1 + tid; i < r; i += nt) L[i * r + k] /= lkk;
+    if (tid == 0) L[k * r + k] = lkk;
+    wg_sync();
+    const int m = r - k - 1;
+    for (int idx = tid; idx < m * m; idx += nt) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      if (j <= i) L[i * r + j] -= L[i * r + k] * L[j * r + k];
+    }
+    wg_sync();
+  }
+  wg_sync();
+  double* out_lam = W + 2 * (int64_t)r * r;
+  if (flag) {
+    if (tid == 0) W[2 * (int64_t)r * r + r] = 1.0;  // raw
+    return;
+  }
+  // W1 = Gu L (into Z), then S = L^T W1
+  for (int idx = tid; idx < r * r; idx += nt) {
+    const int i = idx / r, j = idx % r;
+    double s = 0.0;
+    for (int k = j; k < r; ++k) s += S[i * r + k] * L[k * r + j];
+    Z[idx] = s;
+  }
+  wg_sync();
+  for (int idx = tid; idx < r * r; idx += nt) {
+    const int i = idx / r, j = idx % r;
+    double s = 0.0;
+    for (int k = i; k < r; ++k) s += L[k * r + i] * Z[k * r + j];
+    S[idx] = s;
+  }
+  wg_sync();
+  for (int idx = tid; idx < r * r; idx += nt) {
+    const int i = idx / r, j = idx % r;
+    if (i < j) {
+      const double a = 0.5 * (S[i * r + j] + S[j * r + i]);
+      S[i * r + j] = a;
+      S[j * r + i] = a;
+    }
+  }
+  for (int idx = tid; idx < r * r; idx += nt) Z[idx] = (idx / r == idx % r) ? 1.0 : 0.0;
+  wg_sync();
+  // cyclic Jacobi, round-robin pairing (index 0 fixed, the others rotate)
+  const int np = rr / 2;
+  for (int sweep = 0; sweep < kMaxSweeps && r > 1; ++sweep) {
+    if (tid == 0) flag = 0;
+    wg_sync();
+    for (int round = 0; round < rr - 1; ++round) {
+      for (int t = tid; t < np; t += nt) {
+        auto pos = [&](int i) { return i == 0 ? 0 : 1 + (i - 1 + round) % (rr - 1); };
+        int p = pos(t), q = pos(rr - 1 - t);
+        if (p > q) {
+          const int x = p;
+          p = q;
+          q = x;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < r) {
+          const double app = S[p * r + p], aqq = S[q * r + q], apq = S[p * r + q];
+          if (apq != 0.0 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = tt * c;
+            flag = 1;
+          }
+        }
+        cs[t] = c;
+        sn[t] = s;
+        pp[t] = p;
+        pq[t] = q < r ? q : -1;
+      }
+      wg_sync();
+      // rows: S <- J^T S
+      for (int idx = tid; idx < np * r; idx += nt) {
+        const int t = idx / r, j = idx % r;
+        const int p = pp[t], q = pq[t];
+        if (q < 0 || sn[t] == 0.0) continue;
+        const double c = cs[t], s = sn[t];
+        const double a = S[p * r + j], b = S[q * r + j];
+        S[p * r + j] = c * a - s * b;
+        S[q * r + j] = s * a + c * b;
+      }
+      wg_sync();
+      // columns: S <- S J, Z <- Z J
+      for (int idx = tid; idx < np * r; idx += nt) {
+        const int t = idx / r, i = idx % r;
+        const int p = pp[t], q = pq[t];
+        if (q < 0 || sn[t] == 0.0) continue;
+        const double c = cs[t], s = sn[t];
+        const double a = S[i * r + p], b = S[i * r + q];
+        S[i * r + p] = c * a - s * b;
+        S[i * r + q] = s * a + c * b;
+        const double za = Z[i * r + p], zb = Z[i * r + q];
+        Z[i * r + p] = c * za - s * zb;
+        Z[i * r + q] = s * za + c * zb;
+      }
+      wg_sync();
+    }
+    const int any = flag;
+    wg_sync();
+    if (!any) break;
+  }
+  // sort eigenvalues descending (ties by index)
+  for (int i = tid; i < r; i += nt) lam[i] = S[i * r + i];
+  wg_sync();
+  for (int i = tid; i < r; i += nt) {
+    int rank = 0;
+    for (int j = 0; j < r; ++j) rank += (lam[j] > lam[i]) || (lam[j] == lam[i] && j < i);
+    perm[rank] = i;
+  }
+  wg_sync();
+  double tr = 0.0;
+  for (int i = 0; i < r; ++i) tr += fmax(lam[i], 0.0);
+  // safety margin: the Gram matrices carry rounding of order eps * r * trace
+  const double margin = 32.0 * r * 2.220446049250313e-16 * tr;
+  // Tu[:, k] = L Z[:, perm k], Tv[:, k] = L^-T Z[:, perm k] (column-major, ld r)
+  for (int k = tid; k < r; k += nt) {
+    const int c = perm[k];
+    double* tu = W + (int64_t)k * r;
+    double* tv = W + (int64_t)r * r + (int64_t)k * r;
+    for (int i = 0; i < r; ++i) {
+      double s = 0.0;
+      for (int j = 0; j <= i; ++j) s += L[i * r + j] * Z[j * r + c];
+      tu[i] = s;
+    }
+    for (int i = r - 1; i >= 0; --i) {
+      double s = Z[i * r + c];
+      for (int j = i + 1; j < r; ++j) s -= L[j * r + i] * tv[j];
+      tv[i] = s / L[i * r + i];
+    }
+    out_lam[k] = fmax(lam[c], 0.0) + margin;
+  }
+  if (tid == 0) W[2 * (int64_t)r * r + r] = 0.0;
+}
+
+// ------------------------------------------------------------------ project
+// Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
+// and stored in the column's class (k < r32: fp32, else fp16).  Raw blocks
+// copy their columns unscaled into fp32.
+__global__ __launch_bounds__(256) void project_kernel(
+    const PItem* __restrict__ items, const CDesc* __restrict__ ds, const PDesc* __restrict__ ps,
+    void* const* __restrict__ base64, const double* __restrict__ ws,
+    void* const* __restrict__ ubase32, void* const* __restrict__ vbase32,
+    void* const* __restrict__ ubase16, void* const* __restrict__ vbase16) {
+  constexpr int KT = 8;
+  const PItem it = items[blockIdx.x];
+  const CDesc d = ds[it.blk];
+  const PDesc p = ps[it.blk];
+  const int side = it.side;
+  const int rows = side ? d.m : d.n;
+  const int row = it.row0 + threadIdx.x;
+  if (row >= rows) return;
+  const int r = d.r;
+  const int ld = side ? d.ldv : d.ldu;  // fp64 copy and class layouts share ld
+  const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu) + row;
+  const double* W = ws + d.w;
+  const double* T = W + (side ? (int64_t)r * r : 0);
+  const double* lam = W + 2 * (int64_t)r * r;
+  float* o32 = (float*)(side ? vbase32[p.vch[0]] : ubase32[d.chunk]) +
+               (side ? p.v_out[0] : p.u_out[0]) + row;
+  _Float16* o16 = p.r16 > 0 ? (_Float16*)(side ? vbase16[p.vch[1]] : ubase16[d.chunk]) +
+                                  (side ? p.v_out[1] : p.u_out[1]) + row
+                            : nullptr;
+  if (p.raw) {
+    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld] = (float)X[(int64_t)k * ld];
+    return;
+  }
+  const int keep = p.r32 + p.r16;
+  const double sc = side ? p.sv : p.su;
+  for (int k0 = 0; k0 < keep; k0 += KT) {
+    double acc[KT];
+#pragma unroll
+    for (int q = 0; q < KT; ++q) acc[q] = 0.0;
+    for (int l = 0; l < r; ++l) {
+      const double x = X[(int64_t)l * ld];
+#pragma unroll
+      for (int q = 0; q < KT; ++q)
+        if (k0 + q < keep) acc[q] += x * T[l + (int64_t)(k0 + q) * r];
+    }
+#pragma unroll
+    for (int q = 0; q < KT; ++q) {
+      const int k = k0 + q;
+      if (k >= keep) break;
+      // U side: unit-norm column times su; V side: V' columns are orthonormal
+      const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(lam[k]));
+      if (k < p.r32) o32[(int64_t)k * ld] = (float)v;
+      else o16[(int64_t)(k - p.r32) * ld] = (_Float16)v;
+    }
+  }
+}
+
+// Sum over near pairs (i != j) of K(i, j)^2, one partial per near item.
+template <class KF>
+__global__ __launch_bounds__(256) void near_fro2_kernel(
+    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    KernelDev kd, double* __restrict__ out) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const NearItem it = items[w];
+  const int rs = bstart[it.leaf];
+  const bool act = lane < it.nr;
+  const int ip = rs + it.i0 + (act ? lane : 0);
+  double acc = 0.0;
+  if (act) {
+    for (int sl = 0; sl <= 26; ++sl) {
+      const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
+      if (nn < 0) continue;
+      const int cs = bstart[nn], m = bcount[nn];
+      for (int jp = cs; jp < cs + m; ++jp) {
+        if (jp == ip) continue;
+        const double dx = px[ip] - px[jp], dy = py[ip] - py[jp], dz = pz[ip] - pz[jp];
+        const double v = kd.scale * KF::f64(kd.q, dx * dx + dy * dy + dz * dz);
+        acc += v * v;
+      }
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) out[w] = acc;
+}
+
+// Packs every block's lam (r values) and status into a compact array:
+// out[off[i] + k] = lam_k, status[i] = raw flag.
+__global__ void pack_lam_kernel(const CDesc* __restrict__ ds, int nblk,
+                                const int64_t* __restrict__ off, const double* __restrict__ ws,
+                                double* __restrict__ out, uint8_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblk) return;
+  const CDesc d = ds[i];
+  const double* W = ws + d.w + 2 * (int64_t)d.r * d.r;
+  for (int k = 0; k < d.r; ++k) out[off[i] + k] = W[k];
+  status[i] = W[d.r] != 0.0;
+}
+
+inline double pow2_near_sqrt(int n) {
+  return std::ldexp(1.0, (int)std::lround(0.5 * std::log2((double)std::max(n, 1))));
+}
+
+struct BlockRef {
+  int j;       // level
+  int32_t b;   // block (CSR index within the level)
+  int32_t k;   // target box
+};
+
+}  // namespace
+
+void FactorClass::free_all() {
+  for (void*& p : udata)
+    if (p) (void)hipFree(p);
+  for (void*& p : vdata)
+    if (p) (void)hipFree(p);
+  udata.clear();
+  vdata.clear();
+  nU.clear();
+  nV.clear();
+  dfree(d_ubase);
+  dfree(d_vbase);
+  dfree(d_blk);
+  dfree(d_btoff);
+  dfree(d_mp1);
+  dfree(d_mp1r);
+  dfree(d_mtidx);
+  dfree(d_cmap);
+  plan.free_device();
+  plan = Plan();
+  mp1.clear();
+  mp1r.clear();
+  mtidx.clear();
+  cmap.clear();
+  lf.clear();
+  T = tbase = mpartial = 0;
+  u_elems = v_elems = rank_sum = bytes = 0;
+}
+
+void MixedCopy::free_all() {
+  for (FactorClass& c : cls) c.free_all();
+  dfree(d_scale);
+  ready = false;
+}
+
+void release_mixed(Engine& e) {
+  H3D_HIP(hipStreamSynchronize(e.stream));
+  e.mix.free_all();
+}
+
+void build_mixed(Engine& e, double tol) {
+  if (!e.assembled) throw Error(H3D_ERR_STATE, "compress before assemble");
+  if (!(tol > 0.0) || !std::isfinite(tol)) throw Error(H3D_ERR_INVALID, "tol must be > 0");
+  if (e.fac_bytes[0] == 0)
+    throw Error(H3D_ERR_STATE, "the recompressed copy is built from the fp64 factors");
+  auto t0 = std::chrono::steady_clock::now();
+  release_mixed(e);
+  MixedCopy& mx = e.mix;
+  mx.tol = tol;
+  hipStream_t st = e.stream;
+  const int nL = e.nL;
+
+  int dev = 0, lds_max = 65536;
+  H3D_HIP(hipGetDevice(&dev));
+  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  lds_max = std::min(lds_max, 65536);
+  // LDS of the eigen step: matrices (unless global) + rotation data
+  auto eig_lds = [](int r, bool global) {
+    const int rr = r + (r & 1);
+    return (size_t)((global ? 0 : 3 * r * r) + rr + 2 * r) * sizeof(double) +
+           (size_t)(rr + r) * sizeof(int);
+  };
+  int rmax = 1;  // largest rank whose matrices fit in LDS
+  while (rmax + 1 <= kMaxEigRank && eig_lds(rmax + 1, false) <= (size_t)lds_max) ++rmax;
+
+  // every block in chunk order, with its work description
+  std::vector<BlockRef> refs;
+  std::vector<CDesc> cd;
+  std::vector<int32_t> chunk_first(e.chunks.size() + 1, 0);
+  {
+    int cur = -1;
+    for (int j = 1; j <= nL; ++j) {
+      const Level& lv = e.lev[j];
+      const int64_t lo = level_offset(j);
+      for (int64_t k = 0; k < lv.nboxes; ++k) {
+        const int n = e.bcount[lo + k];
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int r = lv.rank[b];
+          if (r == 0) continue;
+          const int m = e.bcount[lo + lv.ilidx[b]];
+          CDesc c{};
+          c.n = n;
+          c.m = m;
+          c.r = r;
+          c.ldu = n + (n & 1);
+          c.ldv = m + (m & 1);
+          c.chunk = lv.chunk[k];
+          c.xu = lv.uoff[k] + (int64_t)lv.col[b] * c.ldu;
+          c.xv = e.chunks[c.chunk].nU + lv.voff[b];
+          while (cur < c.chunk) chunk_first[++cur] = (int32_t)cd.size();
+          refs.push_back({j, b, (int32_t)k});
+          cd.push_back(c);
+        }
+      }
+    }
+    while (cur < (int)e.chunks.size()) chunk_first[++cur] = (int32_t)cd.size();
+  }
+  const int64_t nb = (int64_t)cd.size();
+  // base pointers of the fp64 chunks (whole chunk: U region then V region)
+  std::vector<void*> b64(e.chunks.size());
+  for (size_t c = 0; c < e.chunks.size(); ++c) b64[c] = e.chunks[c].data[0];
+  void** d_b64 = upload(b64, st);
+
+  // per block workspace size (doubles) and batches bounded by a budget
+  auto wsize = [](int r) { return 2 * (int64_t)r * r + r + 1; };
+  size_t freeb = 0, totb = 0;
+  H3D_HIP(hipMemGetInfo(&freeb, &totb));
+  const int64_t ws_budget =
+      std::max<int64_t>(int64_t(64) << 20, std::min<int64_t>((int64_t)(freeb / 4), int64_t(8) << 30)) / 8;
+
+  std::vector<double> lam_all;      // every column's lam (sorted per block), pass 1
+  std::vector<int64_t> lam_off(nb + 1, 0);
+  for (int64_t i = 0; i < nb; ++i) lam_off[i + 1] = lam_off[i] + cd[i].r;
+  lam_all.assign(lam_off[nb], 0.0);
+  std::vector<uint8_t> raw(nb, 0);
+
+  // Runs gram + eig on blocks [i0, i1) (ws offsets assigned here); pass 1
+  // copies lam/status out, pass 2 keeps them on the device for projection.
+  double* d_ws = nullptr;
+  int64_t ws_cap = 0;
+  CDesc* d_cd = nullptr;
+  int32_t* d_list = nullptr;
+  auto run_eig = [&](int64_t i0, int64_t i1) {
+    int64_t need = 0;
+    for (int64_t i = i0; i < i1; ++i) {
+      cd[i].w = need;
+      need += wsize(cd[i].r);
+    }
+    if (need > ws_cap) {
+      dfree(d_ws);
+      d_ws = dalloc<double>(need);
+      ws_cap = need;
+    }
+    dfree(d_cd);
+    std::vector<CDesc> sub(cd.begin() + i0, cd.begin() + i1);
+    d_cd = upload(sub, st);
+    // groups by rank class: gram EMAX; ranks above kMaxEigRank stay raw
+    std::vector<int32_t> g4, g16, g40, eg;
+    for (int64_t i = i0; i < i1; ++i) {
+      const int r = cd[i].r;
+      const int32_t li = (int32_t)(i - i0);
+      if (r > kMaxEigRank) continue;
+      const int E = r * (r + 1) / 2;
+      (E <= 4 * 256 ? g4 : (E <= 16 * 256 ? g16 : g40)).push_back(li);
+      eg.push_back(li);
+    }
+    std::vector<int32_t> all;
+    all.insert(all.end(), g4.begin(), g4.end());
+    all.insert(all.end(), g16.begin(), g16.end());
+    all.insert(all.end(), g40.begin(), g40.end());
+    dfree(d_list);
+    d_list = upload(all, st);
+    int64_t off = 0;
+    auto launch_g = [&](std::vector<int32_t>& g, int which) {
+      if (g.empty()) return;
+      int rm = 1;
+      for (int32_t li : g) rm = std::max(rm, cd[i0 + li].r);
+      const size_t lds = sizeof(double) * 33 * (size_t)rm;
+      for (int64_t s = 0; s < (int64_t)g.size(); s += 1000000) {
+        const int cnt = (int)std::min<int64_t>(1000000, (int64_t)g.size() - s);
+        const unsigned grid = 2u * (unsigned)cnt;
+        if (which == 0)
+          gram_kernel<4><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+        else if (which == 1)
+          gram_kernel<16><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+        else
+          gram_kernel<40><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+        H3D_LAUNCH_CHECK();
+      }
+      off += (int64_t)g.size();
+    };
+    launch_g(g4, 0);
+    launch_g(g16, 1);
+    launch_g(g40, 2);
+    // eig: small ranks one wave per block, larger ranks four waves, ranks
+    // beyond rmax with their matrices in global scratch
+    std::vector<int32_t> grp[3];
+    for (int32_t li : eg) {
+      const int r = cd[i0 + li].r;
+      grp[r <= 24 ? 0 : (r <= rmax ? 1 : 2)].push_back(li);
+    }
+    for (int q = 0; q < 3; ++q) {
+      std::vector<int32_t>& g = grp[q];
+      if (g.empty()) continue;
+      int rm = 1;
+      for (int32_t li : g) rm = std::max(rm, cd[i0 + li].r);
+      int32_t* d_g = upload(g, st);
+      const int64_t gstride = (int64_t)rm * rm;
+      const int64_t per = q == 2 ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (3 * gstride)))
+                                 : 1000000;
+      double* gs = q == 2 ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
+      for (int64_t s = 0; s < (int64_t)g.size(); s += per) {
+        const int cnt = (int)std::min<int64_t>(per, (int64_t)g.size() - s);
+        if (q < 2)
+          eig_kernel<false><<<cnt, q ? 256 : 64, eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws,
+                                                                           nullptr, 0);
+        else
+          eig_kernel<true><<<cnt, 256, eig_lds(rm, true), st>>>(d_cd, d_g + s, cnt, d_ws, gs,
+                                                                gstride);
+        H3D_LAUNCH_CHECK();
+      }
+      H3D_HIP(hipStreamSynchronize(st));
+      dfree(gs);
+      dfree(d_g);
+    }
+  };
+
+  // ---------------- pass 1: eigenvalues of every block
+  {
+    int64_t i0 = 0;
+    while (i0 < nb) {
+      int64_t i1 = i0, need = 0;
+      while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
+      run_eig(i0, i1);
+      const int cnt = (int)(i1 - i0);
+      std::vector<int64_t> off(cnt);
+      for (int64_t i = i0; i < i1; ++i) off[i - i0] = lam_off[i] - lam_off[i0];
+      const int64_t nlam = lam_off[i1] - lam_off[i0];
+      int64_t* d_off = upload(off, st);
+      double* d_lam = dalloc<double>(std::max<int64_t>(nlam, 1));
+      uint8_t* d_stat = dalloc<uint8_t>(cnt);
+      pack_lam_kernel<<<(cnt + 255) / 256, 256, 0, st>>>(d_cd, cnt, d_off, d_ws, d_lam, d_stat);
+      H3D_LAUNCH_CHECK();
+      std::vector<uint8_t> stat(cnt);
+      H3D_HIP(hipMemcpyAsync(lam_all.data() + lam_off[i0], d_lam, sizeof(double) * nlam,
+                             hipMemcpyDeviceToHost, st));
+      H3D_HIP(hipMemcpyAsync(stat.data(), d_stat, cnt, hipMemcpyDeviceToHost, st));
+      H3D_HIP(hipStreamSynchronize(st));
+      dfree(d_off);
+      dfree(d_lam);
+      dfree(d_stat);
+      for (int64_t i = i0; i < i1; ++i)
+        raw[i] = cd[i].r > kMaxEigRank ? 1 : stat[i - i0];
+      i0 = i1;
+    }
+  }
+  // ||A||_F^2: far blocks (exact for the recompressed ones) + near field
+  {
+    mx.fro2_far = 0;
+    for (int64_t i = 0; i < nb; ++i)
+      if (!raw[i])
+        for (int k = 0; k < cd[i].r; ++k) mx.fro2_far += lam_all[lam_off[i] + k];
+    const int nn = (int)e.nitems.size();
+    mx.fro2_near = 0;
+    if (nn) {
+      double* d_out = dalloc<double>(nn);
+      const int64_t lo = level_offset(nL);
+      const unsigned g = (unsigned)((nn + 3) / 4);
+      if (e.kern.q == 7 || e.kern.q == 16)
+        near_fro2_kernel<InvR><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[nL].d_nbr, e.d_bstart + lo,
+                                                  e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.kern,
+                                                  d_out);
+      else
+        near_fro2_kernel<AnyQ><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[nL].d_nbr, e.d_bstart + lo,
+                                                  e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.kern,
+                                                  d_out);
+      H3D_LAUNCH_CHECK();
+      std::vector<double> h(nn);
+      H3D_HIP(hipMemcpyAsync(h.data(), d_out, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+      H3D_HIP(hipStreamSynchronize(st));
+      dfree(d_out);
+      for (double v : h) mx.fro2_near += v;
+    }
+  }
+  // thresholds: bins of log2(lam / (n + m)) at 1/8 octave
+  const int kBinLo = -1100 * kBinsPerOctave, kBins = 2200 * kBinsPerOctave;
+  auto bin_of = [&](double lam, int nm) {
+    if (!(lam > 0)) return 0;
+    const double v = std::log2(lam / nm) * kBinsPerOctave;
+    return (int)std::min<double>(kBins - 1, std::max<double>(0, std::floor(v) - kBinLo));
+  };
+  const int delta = (int)std::lround(std::log2(1.0 / (kFp16Err * kFp16Err)) * kBinsPerOctave);
+  int bd = 0;
+  {
+    std::vector<double> D(kBins + 1, 0.0);
+    for (int64_t i = 0; i < nb; ++i) {
+      if (raw[i]) continue;
+      const int nm = cd[i].n + cd[i].m;
+      for (int k = 0; k < cd[i].r; ++k) D[bin_of(lam_all[lam_off[i] + k], nm)] += lam_all[lam_off[i] + k];
+    }
+    std::vector<double> P(kBins + 1, 0.0);  // P[b] = sum of D[0..b)
+    for (int b = 0; b < kBins; ++b) P[b + 1] = P[b] + D[b];
+    const double budget = tol * tol * (mx.fro2_far + mx.fro2_near);
+    auto err = [&](int b) {
+      const int h = std::min(kBins, b + delta);
+      return P[b] + kFp16Err * kFp16Err * (P[h] - P[b]) + kFp32Err * kFp32Err * (P[kBins] - P[h]);
+    };
+    // largest b with err(b) <= budget (err is non-decreasing in b)
+    int lo = 0, hi = kBins;
+    if (err(0) > budget) {
+      bd = 0;
+    } else {
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (err(mid) <= budget) lo = mid;
+        else hi = mid - 1;
+      }
+      bd = lo;
+    }
+    mx.theta_drop = std::ldexp(1.0, 0) * std::pow(2.0, (double)(bd + kBinLo) / kBinsPerOctave);
+    mx.err2_drop = P[bd];
+    const int h = std::min(kBins, bd + delta);
+    mx.err2_fp16 = kFp16Err * kFp16Err * (P[h] - P[bd]);
+    mx.err2_fp32 = kFp32Err * kFp32Err * (P[kBins] - P[h]);
+  }
+  // per block class ranks
+  std::vector<int32_t> r32(nb), r16(nb);
+  mx.aca_rank_sum = 0;
+  mx.dropped = 0;
+  mx.raw_blocks = 0;
+  mx.raw_rank = 0;
+  for (int64_t i = 0; i < nb; ++i) {
+    const int r = cd[i].r;
+    mx.aca_rank_sum += r;
+    if (raw[i]) {
+      r32[i] = r;
+      r16[i] = 0;
+      mx.raw_blocks++;
+      mx.raw_rank += r;
+      continue;
+    }
+    const int nm = cd[i].n + cd[i].m;
+    int keep = 0, k32 = 0;
+    for (int k = 0; k < r; ++k) {
+      const int b = bin_of(lam_all[lam_off[i] + k], nm);
+      keep += b >= bd;
+      k32 += b >= bd + delta;
+    }
+    r32[i] = k32;
+    r16[i] = keep - k32;
+    mx.dropped += r - keep;
+  }
+
+  // ---------------- class layouts
+  // U: per target box as in the ACA copy (same chunk ids), the class's columns
+  // of its blocks in interaction-list order.  V: per source box, the class's
+  // columns of every block with that source in by-source order (V_s), in V
+  // chunks of whole source boxes.
+  std::vector<int64_t> ci(e.nblk_all, -1);  // global block id -> index in cd
+  for (int64_t i = 0; i < nb; ++i) ci[e.lev[refs[i].j].bbase + refs[i].b] = i;
+  for (int c = 0; c < 2; ++c) {
+    FactorClass& fc = mx.cls[c];
+    fc.elem = c == 0 ? 4 : 2;
+    fc.lf.assign(nL + 1, LevelFactors());
+    fc.nU.assign(e.chunks.size(), 0);
+    fc.udata.assign(e.chunks.size(), nullptr);
+    fc.nV.assign(1, 0);
+    fc.cmap.assign(e.nblk_all, ColMap{0, 0, 0, 0});
+  }
+  std::vector<PDesc> pd(nb);
+  std::vector<int32_t> vch[2];
+  for (int c = 0; c < 2; ++c) vch[c].assign(e.nblk_all, 0);
+  {
+    int32_t T[2] = {0, 0};
+    for (int j = 1; j <= nL; ++j) {
+      const Level& lv = e.lev[j];
+      const int64_t lo = level_offset(j);
+      const int64_t nblk = (int64_t)lv.ilidx.size();
+      for (int c = 0; c < 2; ++c) {
+        LevelFactors& f = mx.cls[c].lf[j];
+        f.rank.assign(nblk, 0);
+        f.col.assign(nblk, 0);
+        f.voff.assign(nblk, 0);
+        f.Rt.assign(lv.nboxes, 0);
+        f.toff.assign(lv.nboxes, 0);
+        f.uoff.assign(lv.nboxes, 0);
+      }
+      for (int64_t k = 0; k < lv.nboxes; ++k) {
+        const int n = e.bcount[lo + k];
+        const int ch = lv.chunk[k];
+        int32_t R[2] = {0, 0};
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int64_t i = ci[lv.bbase + b];
+          if (i < 0) continue;
+          const int m = e.bcount[lo + lv.ilidx[b]];
+          const int rc[2] = {r32[i], r16[i]};
+          for (int c = 0; c < 2; ++c) {
+            FactorClass& fc = mx.cls[c];
+            LevelFactors& f = fc.lf[j];
+            f.rank[b] = rc[c];
+            f.col[b] = R[c];
+            R[c] += rc[c];
+            fc.rank_sum += rc[c];
+            fc.u_elems += (int64_t)rc[c] * n;
+            fc.v_elems += (int64_t)rc[c] * m;
+          }
+          pd[i].r32 = rc[0];
+          pd[i].r16 = rc[1];
+          pd[i].raw = raw[i];
+          pd[i].su = raw[i] ? 1.0 : pow2_near_sqrt(n);
+          pd[i].sv = raw[i] ? 1.0 : pow2_near_sqrt(m);
+        }
+        for (int c = 0; c < 2; ++c) {
+          FactorClass& fc = mx.cls[c];
+          LevelFactors& f = fc.lf[j];
+          f.Rt[k] = R[c];
+          f.uoff[k] = fc.nU[ch];
+          fc.nU[ch] += (int64_t)(n + (n & 1)) * R[c];
+          f.toff[k] = T[c];
+          T[c] += R[c];
+        }
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int64_t i = ci[lv.bbase + b];
+          if (i < 0) continue;
+          for (int c = 0; c < 2; ++c) {
+            const LevelFactors& f = mx.cls[c].lf[j];
+            pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * (n + (n & 1));
+          }
+        }
+      }
+      // V by source box; the by-source list of level j occupies
+      // e.by_src[bbase, bbase + nblk), grouped by source box in order
+      std::vector<int32_t> nsrc(lv.nboxes, 0), tgt(nblk, 0);
+      for (int32_t ki : lv.ilidx) nsrc[ki]++;
+      for (int64_t k = 0; k < lv.nboxes; ++k)
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) tgt[b] = (int32_t)k;
+      int64_t pos = lv.bbase;
+      for (int64_t sb = 0; sb < lv.nboxes; ++sb) {
+        const int m = e.bcount[lo + sb];
+        const int ld = m + (m & 1);
+        for (int c = 0; c < 2; ++c) {
+          FactorClass& fc = mx.cls[c];
+          LevelFactors& f = fc.lf[j];
+          int64_t C = 0;
+          for (int32_t q = 0; q < nsrc[sb]; ++q) C += f.rank[e.by_src[pos + q] - lv.bbase];
+          if (fc.nV.back() > 0 && fc.nV.back() + C * ld > kVChunkElems) fc.nV.push_back(0);
+          const int32_t vc = (int32_t)fc.nV.size() - 1;
+          const int64_t vs = fc.nV.back();  // V_s column 0
+          int32_t col = 0;
+          for (int32_t q = 0; q < nsrc[sb]; ++q) {
+            const int32_t gid = e.by_src[pos + q];
+            const int32_t b = gid - (int32_t)lv.bbase;
+            const int r = f.rank[b];
+            f.voff[b] = vs + (int64_t)col * ld;
+            vch[c][gid] = vc;
+            fc.cmap[pos + q] = ColMap{col, col + r, f.toff[tgt[b]] + f.col[b] - col, 0};
+            const int64_t i = ci[gid];
+            if (i >= 0) {
+              pd[i].v_out[c] = f.voff[b];
+              pd[i].vch[c] = vc;
+            }
+            col += r;
+          }
+          fc.nV.back() += C * ld;
+          // phase-1 items: rows in ranges of <= 512, columns in runs of
+          // about kMixItemBytes of factors
+          if (m == 0 || C == 0) continue;
+          const int nsplit = (m + kMixRows - 1) / kMixRows;
+          const int rows0 = std::min(m, kMixRows);
+          const int ncol = (int)std::max<int64_t>(
+              16, std::min<int64_t>(kMixMaxCols, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
+          int32_t qb = 0;  // block cursor for bpos0
+          for (int32_t c0 = 0; c0 < C; c0 += ncol) {
+            const int32_t c1 = (int32_t)std::min<int64_t>(C, (int64_t)c0 + ncol);
+            while (fc.cmap[pos + qb].end <= c0) ++qb;
+            int32_t pbase = -1;
+            if (nsplit > 1) {
+              pbase = fc.mpartial;
+              fc.mpartial += nsplit * (c1 - c0);
+              fc.mp1r.push_back({pbase, nsplit, c1 - c0, (int32_t)fc.mtidx.size()});
+              int32_t qq = qb;
+              for (int32_t cc = c0; cc < c1; ++cc) {
+                while (fc.cmap[pos + qq].end <= cc) ++qq;
+                fc.mtidx.push_back(fc.cmap[pos + qq].tmc + cc);
+              }
+            }
+            for (int q = 0; q < nsplit; ++q) {
+              MixP1Item it{};
+              it.voff = vs;
+              it.vchunk = vc;
+              it.ld = ld;
+              it.s0 = q * kMixRows;
+              it.s1 = std::min(m, (q + 1) * kMixRows);
+              it.c0 = c0;
+              it.c1 = c1;
+              it.bpos0 = (int32_t)(pos + qb);
+              it.xoff = e.bstart[lo + sb];
+              it.pslot = nsplit > 1 ? pbase + q * (c1 - c0) : -1;
+              int32_t qe = qb;
+              while (fc.cmap[pos + qe].end < c1) ++qe;
+              it.bcnt = qe - qb + 1;
+              fc.mp1.push_back(it);
+            }
+          }
+        }
+        pos += nsrc[sb];
+      }
+    }
+    mx.cls[0].T = T[0];
+    mx.cls[1].T = T[1];
+    mx.cls[0].tbase = 0;
+    mx.cls[1].tbase = T[0];
+  }
+  // allocate and zero the class chunks
+  for (int c = 0; c < 2; ++c) {
+    FactorClass& fc = mx.cls[c];
+    fc.vdata.assign(fc.nV.size(), nullptr);
+    auto alloc = [&](void*& p, int64_t n) {
+      const size_t bytes = (size_t)(n + kChunkTail) * fc.elem;
+      H3D_HIP(hipMalloc(&p, bytes));
+      H3D_HIP(hipMemsetAsync(p, 0, bytes, st));
+      fc.bytes += n * fc.elem;
+    };
+    for (size_t ch = 0; ch < fc.nU.size(); ++ch) alloc(fc.udata[ch], fc.nU[ch]);
+    for (size_t vc = 0; vc < fc.nV.size(); ++vc) alloc(fc.vdata[vc], fc.nV[vc]);
+    fc.d_ubase = upload(fc.udata, st);
+    fc.d_vbase = upload(fc.vdata, st);
+  }
+  // ---------------- pass 2: eigen again, then project into the classes
+  {
+    int64_t i0 = 0;
+    while (i0 < nb) {
+      int64_t i1 = i0, need = 0;
+      while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
+      run_eig(i0, i1);
+      std::vector<PDesc> sub(pd.begin() + i0, pd.begin() + i1);
+      // blocks of rank > kMaxEigRank have no eigen step: raw in both passes
+      PDesc* d_pd = upload(sub, st);
+      std::vector<PItem> items;
+      for (int64_t i = i0; i < i1; ++i) {
+        if (pd[i].r32 + pd[i].r16 == 0) continue;
+        for (int side = 0; side < 2; ++side) {
+          const int rows = side ? cd[i].m : cd[i].n;
+          for (int r0 = 0; r0 < rows; r0 += 256) items.push_back({(int32_t)(i - i0), side, r0, 0});
+        }
+      }
+      PItem* d_it = upload(items, st);
+      for (int64_t s = 0; s < (int64_t)items.size(); s += 1 << 30) {
+        const unsigned g = (unsigned)std::min<int64_t>(1 << 30, (int64_t)items.size() - s);
+        project_kernel<<<g, 256, 0, st>>>(d_it + s, d_cd, d_pd, d_b64, d_ws, mx.cls[0].d_ubase,
+                                          mx.cls[0].d_vbase, mx.cls[1].d_ubase, mx.cls[1].d_vbase);
+        H3D_LAUNCH_CHECK();
+      }
+      H3D_HIP(hipStreamSynchronize(st));
+      dfree(d_pd);
+      dfree(d_it);
+      i0 = i1;
+    }
+  }
+  dfree(d_ws);
+  dfree(d_cd);
+  dfree(d_list);
+  dfree(d_b64);
+  // per column factor c_k = nu_k nv_k / (su sv) = sqrt(lam_k) / (su sv) in t
+  // order (raw columns: 1), block tables and plans of both classes
+  {
+    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T, 1.0);
+    std::vector<BlockDev> blk[2];
+    std::vector<int32_t> btoff[2];
+    for (int c = 0; c < 2; ++c) {
+      blk[c].assign(e.nblk_all, BlockDev{});
+      btoff[c].assign(e.nblk_all, 0);
+    }
+    for (int j = 1; j <= nL; ++j) {
+      const Level& lv = e.lev[j];
+      const int64_t lo = level_offset(j);
+      for (int64_t k = 0; k < lv.nboxes; ++k)
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int32_t ki = lv.ilidx[b];
+          for (int c = 0; c < 2; ++c) {
+            const FactorClass& fc = mx.cls[c];
+            const LevelFactors& f = fc.lf[j];
+            BlockDev d{};
+            d.voff = f.voff[b];
+            d.col = f.col[b];
+            d.r = f.rank[b];
+            d.m = e.bcount[lo + ki];
+            d.src = e.bstart[lo + ki];
+            d.chunk = vch[c][lv.bbase + b];
+            blk[c][lv.bbase + b] = d;
+            btoff[c][lv.bbase + b] = f.toff[k] + f.col[b];
+          }
+          const int64_t i = ci[lv.bbase + b];
+          if (i < 0) continue;
+          if (!raw[i]) {
+            const double inv = 1.0 / (pd[i].su * pd[i].sv);
+            const LevelFactors& f0 = mx.cls[0].lf[j];
+            const LevelFactors& f1 = mx.cls[1].lf[j];
+            for (int kk = 0; kk < r32[i]; ++kk)
+              scale[(size_t)f0.toff[k] + f0.col[b] + kk] = std::sqrt(lam_all[lam_off[i] + kk]) * inv;
+            for (int kk = 0; kk < r16[i]; ++kk)
+              scale[(size_t)mx.cls[1].tbase + f1.toff[k] + f1.col[b] + kk] =
+                  std::sqrt(lam_all[lam_off[i] + r32[i] + kk]) * inv;
+          }
+        }
+    }
+    mx.d_scale = upload(scale, st);
+    for (int c = 0; c < 2; ++c) {
+      FactorClass& fc = mx.cls[c];
+      fc.d_blk = upload(blk[c], st);
+      fc.d_btoff = upload(btoff[c], st);
+      std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
+      for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
+      build_plan_for(e, lfs, fc.plan);
+      fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
+      fc.plan.p1r.clear();
+      fc.d_mp1 = upload(fc.mp1, st);
+      fc.d_mp1r = upload(fc.mp1r, st);
+      fc.d_mtidx = upload(fc.mtidx, st);
+      fc.d_cmap = upload(fc.cmap, st);
+    }
+    H3D_HIP(hipStreamSynchronize(st));
+  }
+  mx.ready = true;
+  mx.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace h3d

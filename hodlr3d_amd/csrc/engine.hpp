@@ -21,6 +21,7 @@
 //             occupy t[toff_k, toff_k + R_k)
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -107,21 +108,57 @@ struct NearItem {
   int32_t pad;
 };
 
-struct Level {
+// Phase-1 item of a recompressed class: rows [s0, s1) of columns [c0, c1) of
+// a source box's V_s (the V columns of every block with that source,
+// concatenated in by-source order, column-major with ld = box size rounded
+// up to even).
+struct MixP1Item {
+  int64_t voff;    // element offset of V_s column 0 in its V chunk
+  int32_t vchunk;
+  int32_t ld;
+  int32_t s0, s1;  // row range
+  int32_t c0, c1;  // column range
+  int32_t bpos0;   // column-map entry of the block holding column c0
+  int32_t xoff;    // position of the source box's row 0
+  int32_t pslot;   // -1: write t; otherwise element offset of (c1 - c0) partial sums
+  int32_t bcnt;    // column-map entries from bpos0 that cover [c0, c1)
+};
+
+// Column map of a class, one entry per block in by-source order: the block
+// owns columns [beg, end) of V_s and the t index of its column c is tmc + c.
+struct ColMap {
+  int32_t beg, end, tmc, pad;
+};
+
+// Row-split partial sums of a mixed phase-1 column range: nsplit x nc values at
+// part[pbase + q*nc + l] summed in range order into t[tidx[ioff + l]].
+struct MixReduce {
+  int32_t pbase, nsplit, nc, ioff;
+};
+
+// Per-level factor bookkeeping of one copy, indexed by the level's blocks (CSR order)
+// and target boxes.
+struct LevelFactors {
+  std::vector<int32_t> rank, col;  // per block
+  std::vector<int64_t> voff;       // per block
+  std::vector<int32_t> Rt, toff;   // per target box
+  std::vector<int64_t> uoff;       // per target box
+};
+
+// One level of the tree: structure plus the ACA copy's factor bookkeeping
+// (rank: block rank; col: first column of the block in its target box; voff:
+// V offset in the chunk; Rt: columns of a target box; toff: its global t
+// offset; uoff: its U offset in the chunk).
+struct Level : LevelFactors {
   int64_t nboxes = 0;
   int64_t bbase = 0;                  // global id of the level's first block
   std::vector<int32_t> iloff, ilidx;  // host copies of the interaction lists
   int32_t* d_iloff = nullptr;
   int32_t* d_ilidx = nullptr;
   int32_t* d_nbr = nullptr;           // 26 slots per box
-  std::vector<int32_t> rank, status;  // per block (CSR order)
-  std::vector<int32_t> col;           // per block: first column in its target box
-  std::vector<int64_t> voff;          // per block: V offset in its chunk
+  std::vector<int32_t> status;        // per block (CSR order)
   std::vector<int64_t> pivoff;        // per block: pivot offset in its chunk
-  std::vector<int32_t> Rt;            // per target box: columns
-  std::vector<int32_t> toff;          // per target box: t offset (global)
   std::vector<int32_t> chunk;         // per target box: chunk id
-  std::vector<int64_t> uoff;          // per target box: U offset in its chunk
 };
 
 // One chunk of factors: per precision a U region followed by a V region.
@@ -130,6 +167,70 @@ struct Chunk {
   int64_t nU = 0, nV = 0;  // elements
   int32_t* piv = nullptr;  // pivots: for each block I then J (r each)
   int64_t npiv = 0;
+};
+
+// Work lists of one factor copy's matvec (built on the host, mirrored on the
+// device): phase-1 items and their partial-sum reductions, phase-2 items, and
+// the transpose's items.
+struct Plan {
+  std::vector<P1Item> p1;
+  std::vector<P1Reduce> p1r;
+  std::vector<P2Item> p2;
+  std::vector<TP1Item> tp1;
+  std::vector<P1Reduce> tp1r;
+  std::vector<TP2Item> tp2;
+  int32_t npartial = 0;
+  P1Item* d_p1 = nullptr;
+  P1Reduce* d_p1r = nullptr;
+  P2Item* d_p2 = nullptr;
+  TP1Item* d_tp1 = nullptr;
+  P1Reduce* d_tp1r = nullptr;
+  TP2Item* d_tp2 = nullptr;
+  void free_device();
+};
+
+// One storage class of the recompressed copy (fp32 or fp16 columns).  Columns
+// are stored scaled: the class's U and V columns hold u_k * su / nu_k and
+// v_k * sv / nv_k (su, sv powers of two near sqrt(n), sqrt(m)), and the
+// factor c_k = nu_k nv_k / (su sv) multiplies t_k after phase 1.  Chunk ids
+// and the block/box structure are those of the ACA copy.
+struct FactorClass {
+  int elem = 4;                      // bytes per stored element (4: fp32, 2: fp16)
+  std::vector<LevelFactors> lf;      // [level]; voff is relative to the block's V chunk
+  std::vector<void*> udata, vdata;   // U chunks (the ACA copy's chunk ids), V chunks
+  std::vector<int64_t> nU, nV;       // elements per U chunk / V chunk
+  void** d_ubase = nullptr;          // per U chunk
+  void** d_vbase = nullptr;          // per V chunk
+  BlockDev* d_blk = nullptr;         // chunk = V chunk id
+  int32_t* d_btoff = nullptr;
+  Plan plan;                         // phase 2 and transpose items (phase 1 below)
+  std::vector<MixP1Item> mp1;
+  std::vector<MixReduce> mp1r;
+  std::vector<int32_t> mtidx;
+  std::vector<ColMap> cmap;
+  MixP1Item* d_mp1 = nullptr;
+  MixReduce* d_mp1r = nullptr;
+  int32_t* d_mtidx = nullptr;
+  ColMap* d_cmap = nullptr;
+  int32_t mpartial = 0;              // partial sums of the phase-1 items
+  int32_t T = 0;                     // t entries of this class
+  int32_t tbase = 0;                 // offset of this class in the shared t / scale arrays
+  int64_t u_elems = 0, v_elems = 0, rank_sum = 0, bytes = 0;
+  void free_all();
+};
+
+// Recompressed mixed-precision copy: SVD-truncated factors of every block with
+// columns in fp32 or fp16 chosen against a global Frobenius error budget.
+struct MixedCopy {
+  bool ready = false;
+  double tol = 0;                    // requested relative Frobenius error
+  FactorClass cls[2];                // 0: fp32 columns, 1: fp16 columns
+  double* d_scale = nullptr;         // c_k per t entry (class 0 then class 1)
+  // statistics of the last build
+  double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0;
+  double theta_drop = 0, ms = 0;
+  int64_t aca_rank_sum = 0, dropped = 0, raw_blocks = 0, raw_rank = 0;
+  void free_all();
 };
 
 struct Engine {
@@ -173,22 +274,11 @@ struct Engine {
   int32_t* d_by_src = nullptr;
   int32_t* d_btoff = nullptr;    // per block: global t index of its first column
 
-  // matvec plan
-  std::vector<P1Item> p1;
-  std::vector<P1Reduce> p1r;
-  std::vector<P2Item> p2;
+  // matvec plan of the ACA copy; near-field items are shared by every copy
+  Plan plan;
   std::vector<NearItem> nitems;
-  std::vector<TP1Item> tp1;
-  std::vector<P1Reduce> tp1r;
-  std::vector<TP2Item> tp2;
-  int32_t npartial = 0;
-  P1Item* d_p1 = nullptr;
-  P1Reduce* d_p1r = nullptr;
-  P2Item* d_p2 = nullptr;
   NearItem* d_near = nullptr;
-  TP1Item* d_tp1 = nullptr;
-  P1Reduce* d_tp1r = nullptr;
-  TP2Item* d_tp2 = nullptr;
+  MixedCopy mix;
 
   // scratch for one matvec
   int max_nrhs = 0;
@@ -198,17 +288,21 @@ struct Engine {
   double* d_part = nullptr;   // phase-1 partial sums
   double* d_ylev = nullptr;   // per-level y by position: nL * N
   double* d_io = nullptr;     // staging for host vectors
+  int64_t t_cap = 0, part_cap = 0;  // allocated entries of d_t / d_part per right-hand side
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
   // autotuned load width in bytes per lane for phase 1 / phase 2, indexed by
-  // storage precision + 3 * (fp64 accumulation over fp32/bf16)
-  int tune_p1[6] = {16, 16, 16, 16, 16, 16};
-  int tune_p2[6] = {16, 16, 16, 16, 16, 16};
+  // storage precision + 3 * (fp64 accumulation over fp32/bf16); 6 and 7 are
+  // the fp32 and fp16 classes of the mixed copy
+  int tune_p1[8] = {16, 16, 16, 16, 16, 16, 16, 16};
+  int tune_p2[8] = {16, 16, 16, 16, 16, 16, 16, 16};
+  // rows per lane of the mixed phase-1 kernel, in load vectors (1..4), slots 6, 7
+  int tune_k[8] = {4, 4, 4, 4, 4, 4, 4, 4};
   // per-kernel event timing of matvecs
   bool timing = false;
-  hipEvent_t ev[6] = {};
-  double ktime[5] = {};
+  hipEvent_t ev[8] = {};
+  double ktime[7] = {};
   int kcount = 0;
 
   // stats
@@ -225,6 +319,8 @@ struct Engine {
 inline int64_t level_offset(int j) { return ((int64_t(1) << (3 * j)) - 1) / 7; }
 constexpr int64_t kChunkTail = 64;  // zero elements after every factor chunk
 inline int elem_size(int prec) { return prec == 0 ? 8 : (prec == 1 ? 4 : 2); }
+// Precision code of the recompressed mixed copy (fp32 + fp16 columns, fp64 accumulation).
+constexpr int kPrecMixed = 3;
 
 // tree.hip
 void build_tree(Engine& e, const double* h_xyz);
@@ -242,6 +338,12 @@ void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nr
 // assembled problem and keeps the fastest; ms[8] = phase 1 at 16/8/4/2 bytes,
 // then phase 2 at 16/8/4/2 bytes (negative when the width does not apply).
 void autotune(Engine& e, int mode, int reps, double* ms);
+// compress.hip: builds e.mix with relative Frobenius error budget `tol`
+void build_mixed(Engine& e, double tol);
+void release_mixed(Engine& e);
+// Plan of one copy from its per-level factor bookkeeping (apply.hip).
+// lf[j] describes level j (1..nLevels).
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);
@@ -260,6 +362,14 @@ template <class T>
 void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
   p = nullptr;
+}
+// Device copy of a host vector (at least one element), queued on `st`.
+template <class T>
+T* upload(const std::vector<T>& v, hipStream_t st) {
+  T* d = dalloc<T>(std::max<int64_t>((int64_t)v.size(), 1));
+  if (!v.empty())
+    H3D_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+  return d;
 }
 
 }  // namespace h3d

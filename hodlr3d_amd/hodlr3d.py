@@ -20,11 +20,13 @@ import numpy as np
 
 from . import _native as nat
 
-PRECISIONS = {"fp64": nat.PREC_FP64, "fp32": nat.PREC_FP32, "bf16": nat.PREC_BF16}
+PRECISIONS = {"fp64": nat.PREC_FP64, "fp32": nat.PREC_FP32, "bf16": nat.PREC_BF16,
+              "mixed": nat.PREC_MIXED}
 
 
 def _prec(p):
-    """'fp64' | 'fp32' | 'bf16', optionally with '/acc64' (fp64 accumulation)."""
+    """'fp64' | 'fp32' | 'bf16', optionally with '/acc64' (fp64 accumulation), or
+    'mixed' (the recompressed fp32/fp16 copy built by ``compress``)."""
     if isinstance(p, str):
         base, _, acc = p.partition("/")
         v = PRECISIONS[base]
@@ -195,12 +197,34 @@ class HODLR3D:
 
     def autotune(self, prec="fp64", reps=3):
         """Pick the fastest per-lane load width of both far-field kernels for `prec`
-        on this problem; returns the measured ms per width."""
-        ms = (ctypes.c_double * 8)()
+        on this problem; returns the measured ms per width (for 'mixed' also the
+        fp16 class under 'phase1_fp16' / 'phase2_fp16')."""
+        ms = (ctypes.c_double * 16)(*([-1.0] * 16))
         nat.check(nat.lib().h3d_autotune(self._h, _prec(prec), int(reps), ms))
         w = [16, 8, 4, 2]
-        return {"phase1": {w[i]: ms[i] for i in range(4) if ms[i] >= 0},
-                "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0}}
+        out = {"phase1": {w[i]: ms[i] for i in range(4) if ms[i] >= 0},
+               "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0}}
+        if _prec(prec) == nat.PREC_MIXED:
+            out["phase1_fp16"] = {w[i]: ms[8 + i] for i in range(4) if ms[8 + i] >= 0}
+            out["phase2_fp16"] = {w[i]: ms[12 + i] for i in range(4) if ms[12 + i] >= 0}
+        return out
+
+    def compress(self, tol=None):
+        """Build the recompressed mixed-precision copy ('mixed') with an expected
+        added relative error of `tol` (default 5 * 10^-TOL_POW); returns its stats."""
+        if tol is None:
+            tol = 5.0 * 10.0 ** (-self.TOL_POW)
+        s = nat.MixedStats()
+        nat.check(nat.lib().h3d_compress(self._h, float(tol), ctypes.byref(s)))
+        return self.mixed_stats()
+
+    def mixed_stats(self):
+        s = nat.MixedStats()
+        nat.check(nat.lib().h3d_get_mixed_stats(self._h, ctypes.byref(s)))
+        d = {k: getattr(s, k) for k, _ in nat.MixedStats._fields_}
+        d["u_elems"] = list(s.u_elems)
+        d["v_elems"] = list(s.v_elems)
+        return d
 
     def set_tuning(self, prec, p1_bytes, p2_bytes):
         """Pin the per-lane load widths of the two far-field kernels for `prec`."""
@@ -208,11 +232,12 @@ class HODLR3D:
 
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
-        ms = (ctypes.c_double * 5)()
+        ms = (ctypes.c_double * 7)()
         cnt = ctypes.c_int()
         nat.check(nat.lib().h3d_kernel_timing(self._h, int(enable), ms, ctypes.byref(cnt)))
         return {"phase1": ms[0], "phase2": ms[1], "near": ms[2], "combine": ms[3],
-                "total": ms[4], "count": cnt.value}
+                "total": ms[4], "phase1_second": ms[5], "phase2_second": ms[6],
+                "count": cnt.value}
 
     def set_diag(self, c):
         if _is_torch_cuda(c):

@@ -32,6 +32,9 @@ extern "C" {
 #define H3D_PREC_FP64 0  /* fp64 factors, fp64 accumulation, fp64 near field */
 #define H3D_PREC_FP32 1  /* fp32 factors, fp32 accumulation, fp32 near field */
 #define H3D_PREC_BF16 2  /* bf16 factors, fp32 accumulation, fp32 near field */
+/* Recompressed copy (see h3d_compress): SVD-truncated factors with fp32 and
+ * fp16 columns, fp64 accumulation, fp32 near-field entries summed in fp64. */
+#define H3D_PREC_MIXED 3
 /* Flag or-ed into a precision: accumulate in fp64 (fp64 near field) over
  * fp32/bf16-stored factors. */
 #define H3D_ACC_FP64 0x100
@@ -80,6 +83,19 @@ typedef struct h3d_stats {
   int64_t v_elems;         /* sum over blocks of r*m (V entries, read by phase 1) */
 } h3d_stats;
 
+typedef struct h3d_mixed_stats {
+  double tol;               /* requested relative Frobenius error */
+  double predicted_rel_err; /* estimate of ||A_mixed - A_aca||_F / ||A||_F actually spent */
+  double err_drop, err_fp16, err_fp32;   /* its parts: dropped columns, fp16 and fp32 rounding */
+  double fro_far, fro_near; /* ||.||_F of the far (low-rank) and near (dense) parts */
+  int64_t aca_rank_sum;     /* columns before recompression */
+  int64_t rank_fp32, rank_fp16, dropped; /* columns per class and dropped */
+  int64_t raw_blocks, raw_rank;          /* blocks kept as ACA columns in fp32 */
+  int64_t bytes;            /* resident bytes of the copy */
+  int64_t u_elems[2], v_elems[2];        /* per class (0: fp32, 1: fp16) */
+  double build_ms;
+} h3d_mixed_stats;
+
 typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
   int inner_iterations;    /* total inner GMRES iterations */
@@ -103,12 +119,25 @@ int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
 int h3d_assemble(h3d_handle* h);
 
 /* Builds a copy of the factors in `prec` (from the fp64 copy, else the fp32
- * one), or frees one; the last fp64/fp32 copy cannot be freed. */
+ * one), or frees one (H3D_PREC_MIXED too); the last fp64/fp32 copy cannot be
+ * freed. */
 int h3d_prepare_precision(h3d_handle* h, int prec);
 /* Like h3d_prepare_precision, but frees every other copy chunk by chunk, so the
  * peak is one chunk above the larger of the two copies. */
 int h3d_move_precision(h3d_handle* h, int prec);
 int h3d_release_precision(h3d_handle* h, int prec);
+
+/* Builds the recompressed mixed-precision copy (no reference counterpart; an
+ * extension of the reference's HODLR3D::assemble, src/HODLR3D.cpp:17-19) from
+ * the fp64 factors: every block B = U V^T is recompressed through the
+ * eigendecomposition of its Gram matrices (B = U' V'^T, V' orthonormal), and
+ * its columns are dropped, stored in fp16 or in fp32 so that the expected
+ * added relative error ||A_mixed - A||_F / ||A||_F stays within `tol` (one
+ * budget over all blocks, spent on the columns with the least error per stored
+ * element).  Needs the fp64 copy resident; stats may be NULL.  A matvec with
+ * H3D_PREC_MIXED builds the copy with tol = 5 * 10^-TOL_POW if absent. */
+int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* stats);
+int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* stats);
 
 /* Replaces HODLR3D::MatVecProduct (src/HODLR3D.cpp:21-27): y = A b for nrhs
  * column-major right-hand sides of length N. */
@@ -130,17 +159,20 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag);
 
 /* Autotuner: times the far-field kernels of precision mode `prec` with every
  * per-lane load width (16/8/4/2 bytes) on the assembled problem, `reps`
- * launches each, and keeps the fastest for later matvecs.  ms[8] (may be
- * NULL) receives phase-1 then phase-2 averages per width, -1 where a width
- * does not apply. */
+ * launches each, and keeps the fastest for later matvecs.  ms (may be NULL)
+ * receives phase-1 then phase-2 averages per width, -1 where a width does not
+ * apply: 8 values, or 16 for H3D_PREC_MIXED (fp32 class, then fp16 class). */
 int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
-/* Pins the per-lane load widths (bytes, >= the element size) for one mode. */
+/* Pins the per-lane load widths (bytes, >= the element size; >= 4 and for
+ * both classes with H3D_PREC_MIXED) for one mode. */
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
- * stream (enable != 0).  ms[5] receives the accumulated milliseconds of
+ * stream (enable != 0).  ms[7] receives the accumulated milliseconds of
  * phase 1 (V^T x), phase 2 (U t), near field, combine and whole matvecs since
- * the last read, *count the number of matvecs; both are then reset. */
+ * the last read, then the parts of phase 1 and phase 2 spent on the second
+ * factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); *count the
+ * number of matvecs; all are then reset. */
 int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count);
 
 /* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,

@@ -1,0 +1,154 @@
+"""Recompressed mixed-precision copy (fp32 + fp16 columns, fp64 accumulation):
+error against the direct FP64 product within the requested budget, adjoint,
+determinism, every load width, and refined solves on it (GPU).
+
+The copy has no counterpart in the reference: its bar is the error budget it
+is built for, measured against the exact O(N^2) product of the same operator."""
+import numpy as np
+import pytest
+
+from tests.conftest import uniform
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (points, leaf1d, tol_pow, tol of the mixed copy)
+    "uniform-32k-t7": (lambda: uniform(32768, 21), 4, 7, 5e-7),
+    "uniform-20k-t9": (lambda: uniform(20000, 22), 4, 9, 2e-7),
+}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def built(request):
+    import hodlr3d_amd as H3
+    mk, leaf, tol_pow, tol = CASES[request.param]
+    x = mk()
+    print(f"[case {request.param}] N={len(x)}", flush=True)
+    h = H3.HODLR3D(leaf, 1.0, tol_pow, 7, x)
+    h.assemble()
+    st = h.compress(tol)
+    print(f"[case {request.param}] mixed stats {st}", flush=True)
+    yield request.param, x, h, tol, st
+    h.close()
+
+
+def _rows(N, k=2048, seed=3):
+    return np.sort(np.random.default_rng(seed).choice(N, size=min(N, k), replace=False))
+
+
+def test_stats_are_consistent(built):
+    name, x, h, tol, st = built
+    assert st["tol"] == tol
+    assert st["rank_fp32"] + st["rank_fp16"] + st["dropped"] == st["aca_rank_sum"]
+    assert 0 <= st["predicted_rel_err"] <= tol * (1 + 1e-9)
+    parts = np.sqrt(st["err_drop"] ** 2 + st["err_fp16"] ** 2 + st["err_fp32"] ** 2)
+    assert abs(parts - st["predicted_rel_err"]) <= 1e-9 * tol
+    full = h.stats()
+    assert st["aca_rank_sum"] == full["rank_sum"]
+    # fewer bytes than the fp32 copy of the ACA factors
+    assert st["bytes"] < 4 * full["factor_elems"]
+    assert st["fro_far"] > 0 and st["fro_near"] > 0
+
+
+def test_error_within_budget(built):
+    name, x, h, tol, st = built
+    N = len(x)
+    q = np.random.default_rng(5).standard_normal(N)
+    rows = _rows(N)
+    yd = h.direct(q, rows)
+    y64 = h.matvec(q, prec="fp64")
+    ym = h.matvec(q, prec="mixed")
+    e64 = np.linalg.norm(y64[rows] - yd) / np.linalg.norm(yd)
+    em = np.linalg.norm(ym[rows] - yd) / np.linalg.norm(yd)
+    dm = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
+    print(f"[{name}] fp64 err {e64:.3e} mixed err {em:.3e} mixed-fp64 {dm:.3e} "
+          f"predicted {st['predicted_rel_err']:.3e}", flush=True)
+    # the added error is an expectation over iid right-hand sides: allow 2x
+    assert dm <= 2.0 * tol
+    assert em <= np.hypot(e64, 2.0 * tol)
+
+
+def test_transpose_is_the_adjoint(built):
+    """The far field's transpose reads the same stored factors (exact adjoint);
+    the near field's fp32 entries are evaluated from each row's own leaf, so
+    K~(i,j) and K~(j,i) differ in the last fp32 bits: adjoint to ~1e-7."""
+    name, x, h, tol, st = built
+    N = len(x)
+    rng = np.random.default_rng(6)
+    a, b = rng.standard_normal(N), rng.standard_normal(N)
+    ya = h.matvec(a, prec="mixed")
+    lhs = np.dot(ya, b)
+    rhs = np.dot(a, h.matvec(b, prec="mixed", transpose=True))
+    assert abs(lhs - rhs) <= 1e-6 * np.linalg.norm(ya) * np.linalg.norm(b)
+    # the mixed transpose agrees with the fp64 adjoint within the copy's error
+    yt = h.matvec(b, prec="mixed", transpose=True)
+    yt64 = h.matvec(b, prec="fp64", transpose=True)
+    assert np.linalg.norm(yt - yt64) <= 2.0 * tol * np.linalg.norm(yt64)
+
+
+def test_rebuild_is_bit_identical(built):
+    name, x, h, tol, st = built
+    q = np.random.default_rng(7).standard_normal(len(x))
+    y1 = h.matvec(q, prec="mixed")
+    st2 = h.compress(tol)
+    y2 = h.matvec(q, prec="mixed")
+    assert st2["bytes"] == st["bytes"] and st2["dropped"] == st["dropped"]
+    assert np.array_equal(y1, y2)
+
+
+def test_every_load_width_gives_the_same_product(built):
+    """Load widths and rows per lane change the summation order; the fp16
+    class (about 1e-3 of the operator) sums in fp32, so results agree to
+    ~1e-10 rather than to fp64 rounding."""
+    name, x, h, tol, st = built
+    q = np.random.default_rng(8).standard_normal(len(x))
+    h.set_tuning("mixed", 16, 16)
+    ref = h.matvec(q, prec="mixed")
+    reft = h.matvec(q, prec="mixed", transpose=True)
+    for w1 in (16, 8, 4):
+        for w2 in (16, 8, 4):
+            h.set_tuning("mixed", w1, w2)
+            y = h.matvec(q, prec="mixed")
+            yt = h.matvec(q, prec="mixed", transpose=True)
+            assert np.linalg.norm(y - ref) <= 1e-9 * np.linalg.norm(ref), (w1, w2)
+            assert np.linalg.norm(yt - reft) <= 1e-9 * np.linalg.norm(reft), (w1, w2)
+    tuned = h.autotune("mixed", reps=1)
+    assert set(tuned) == {"phase1", "phase2", "phase1_fp16", "phase2_fp16"}
+    y = h.matvec(q, prec="mixed")
+    assert np.linalg.norm(y - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+def test_tiny_budget_keeps_every_column_in_fp32():
+    """Below the fp32 rounding floor (~4e-8) every column stays fp32 and the
+    predicted error is that floor."""
+    import hodlr3d_amd as H3
+    x = uniform(6000, 23)
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    st = h.compress(1e-15)
+    assert st["dropped"] == 0 and st["rank_fp16"] == 0
+    assert st["err_drop"] == 0 and st["err_fp16"] == 0 and st["err_fp32"] < 1e-7
+    q = np.random.default_rng(9).standard_normal(len(x))
+    y64 = h.matvec(q, prec="fp64")
+    ym = h.matvec(q, prec="mixed")
+    # fp32 storage of every factor, fp32 near-field entries, fp64 sums
+    assert np.linalg.norm(ym - y64) <= 1e-6 * np.linalg.norm(y64)
+    h.release_precision("mixed")
+    assert h.mixed_stats()["bytes"] == 0
+    h.close()
+
+
+def test_refined_solve_with_the_mixed_inner_operator():
+    import hodlr3d_amd as H3
+    from hodlr3d_amd import configs
+    x = uniform(20000, 24)
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    h.compress(5e-8)
+    h.set_operator(alpha=configs.second_kind_scale(27), diag=1.0)
+    b = np.random.default_rng(10).standard_normal(len(x))
+    sol, info = h.solve(b, tol=1e-10, restart=30, max_inner=300, inner_tol=1e-5,
+                        inner_prec="mixed", max_outer=10)
+    r = b - h.matvec(sol, prec="fp64")
+    assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(b), info
+    h.close()
