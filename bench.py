@@ -156,8 +156,12 @@ def main():
     ap.add_argument("--config", default="C3", choices=["C1", "C2", "C3"])
     ap.add_argument("--modes", default="mixed,fp32/acc64,fp32,bf16",
                     help="reduced-precision modes measured after fp64 (comma separated)")
-    ap.add_argument("--mixed-tol", type=float, default=5e-7,
-                    help="added relative error budget of the recompressed mixed copy")
+    # The copy's budget is relative to ||A||_F (the expected ||A x|| for iid x);
+    # for this all-positive 1/r matrix ||A x|| of one right-hand side is often
+    # well below that (0.6..1.3 ||A||_F in draws of N(0,1) vectors), so the
+    # budget keeps a margin under the 1e-6 bar on ||A x||.
+    ap.add_argument("--mixed-tol", type=float, default=4e-7,
+                    help="added relative error budget (vs ||A||_F) of the recompressed mixed copy")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (for scaling "
                          "HBM counters of a profiler run)")
@@ -199,6 +203,7 @@ def main():
         return float(np.linalg.norm(y.cpu().numpy()[rows] - y_direct) / ynorm)
 
     res = {}
+    ref = {}
 
     def measure(prec):
         out = {}
@@ -206,6 +211,11 @@ def main():
         log(f"[bench] autotune {prec}: {out['autotune']}")
         y = H.matvec(q, prec=prec)
         out["rel_err"] = rel_err(y)
+        if prec == "fp64":
+            ref["y64"] = y.clone()
+        elif "y64" in ref and H.stats()["factor_bytes"][0] > 0:
+            # difference from the fp64 product of the same ACA operator (all N rows)
+            out["vs_fp64"] = float(torch.linalg.norm(y - ref["y64"]) / torch.linalg.norm(ref["y64"]))
         H.kernel_timing(True)
         ms = timed(torch, lambda: H.matvec(q, prec=prec), args.steps, args.warmup)
         kt = H.kernel_timing(False)
@@ -314,6 +324,7 @@ def main():
         "hbm_gbs": round(far_gbs, 1),
         "fp64_hbm_gbs": round(fp64_far_gbs, 1),
         "rel_err": {k: v["rel_err"] for k, v in res.items()},
+        "rel_diff_vs_fp64": {k: v["vs_fp64"] for k, v in res.items() if "vs_fp64" in v},
         "matvec_ms": {k: round(v["ms"], 4) for k, v in res.items()},
         "err_rows": len(rows),
         "solve_ms": solve["ms"] if solve else None,

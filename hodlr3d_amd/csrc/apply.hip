@@ -27,7 +27,6 @@ constexpr int kP1Rows = 512;          // rows of a phase-1 item (64 lanes x 8 ro
 constexpr int64_t kP1Elems = 8192;    // factor entries per multi-block phase-1 item
 constexpr int kP2Rows = 128;          // rows of one phase-2 item
 constexpr int kWavesPerBlock = 4;
-constexpr int kMixMaxCols = 1024;     // columns of a mixed phase-1 item (LDS t-index map)
 
 // Factor pointers come from per-chunk tables, so the compiler cannot prove
 // they are global memory; loading through an address-space-1 pointer makes
@@ -57,6 +56,13 @@ struct Load<_Float16> {
     return (float)__builtin_bit_cast(_Float16, ntload<uint16_t>(p));
   }
 };
+
+// Global index of the calling wave (one work item per wave), made visibly
+// wave-uniform so that the compiler loads the item and its tables with scalar
+// loads.
+__device__ inline int wave_id() {
+  return __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+}
 
 template <class A>
 __device__ inline A wave_sum_xor(A v) {
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
                                                      A* __restrict__ part,
                                                      const double* __restrict__ scale) {
   constexpr int KMAX = 8 / VEC > 0 ? 8 / VEC : 1;  // rows per lane <= 8 (items have <= 512 rows)
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const P1Item it = items[w];
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
 template <class A>
 __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ part,
                               A* __restrict__ t, const double* __restrict__ scale) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
   const P1Reduce r = rs[w];
@@ -213,34 +219,21 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
 // item is a run of columns of V_s (the columns of every block with source s,
 // concatenated) over at most 512 rows, so x is loaded once per item.  lpc lanes
 // share a column, each holding K <= kmax load vectors of VEC rows (x in
-// registers), and reduce with an xor butterfly.  The wave first writes the t
-// index of each of its (at most 1024) columns into its LDS slice from the
-// column map.  A load starts inside the rows, so it ends less than VEC
-// elements past them (next column, padding or the chunk's zero tail).
+// registers), and reduce with an xor butterfly.  t is written in source order
+// (t_src[tsb + c], contiguous stores); a gather puts it in target order.  A
+// load starts inside the rows, so it ends less than VEC elements past them
+// (next column, padding or the chunk's zero tail).
 template <class S, class A, int VEC>
 __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restrict__ items,
                                                       int nitems, void* const* __restrict__ vbase,
-                                                      const ColMap* __restrict__ cmap,
-                                                      const A* __restrict__ x, A* __restrict__ t,
-                                                      A* __restrict__ part,
-                                                      const double* __restrict__ scale, int kmax) {
+                                                      const A* __restrict__ x,
+                                                      A* __restrict__ tsrc, A* __restrict__ part,
+                                                      int kmax) {
   constexpr int KMAX = VEC >= 2 ? 4 : 8;  // 64 lanes x KMAX x VEC >= 512 rows
-  __shared__ int32_t tix_s[kWavesPerBlock][kMixMaxCols];
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const MixP1Item it = items[w];
-  int32_t* tix = tix_s[threadIdx.x >> 6];
-  if (it.pslot < 0) {
-    for (int q = lane; q < it.bcnt; q += 64) {
-      const ColMap cm = cmap[it.bpos0 + q];
-      const int a = max(cm.beg, it.c0), z = min(cm.end, it.c1);
-      for (int c = a; c < z; ++c) tix[c - it.c0] = cm.tmc + c;
-    }
-    // the wave's own LDS writes, read back by other lanes of the wave
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  }
   const int rows = it.s1 - it.s0;
   const int need = (rows + VEC - 1) / VEC;
   const int kk = min(max(kmax, 1), KMAX);
@@ -287,32 +280,36 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
       for (int d = lpc >> 1; d >= 1; d >>= 1) a += __shfl_xor(a, d);
       const int c = c0 + u * cpp + slot;
       if (c < it.c1 && sub == 0) {
-        if (it.pslot < 0) {
-          const int ti = tix[c - it.c0];
-          t[ti] = scale ? a * (A)scale[ti] : a;
-        } else {
-          part[(int64_t)it.pslot + (c - it.c0)] = a;
-        }
+        if (it.pslot < 0) tsrc[it.tsb + c] = a;
+        else part[(int64_t)it.pslot + (c - it.c0)] = a;
       }
     }
   }
 }
 
-// Row-split partials of the mixed phase 1 summed in range order into t.
+// Row-split partials of the mixed phase 1 summed in range order into t_src.
 template <class A>
 __global__ void mreduce_kernel(const MixReduce* __restrict__ rs, int n, const A* __restrict__ part,
-                               const int32_t* __restrict__ tidx, A* __restrict__ t,
-                               const double* __restrict__ scale) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+                               A* __restrict__ tsrc) {
+  const int w = wave_id();
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
   const MixReduce r = rs[w];
   for (int l = lane; l < r.nc; l += 64) {
     A acc = 0;
     for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
-    const int ti = tidx[r.ioff + l];
-    t[ti] = scale ? acc * (A)scale[ti] : acc;
+    tsrc[r.tsb + l] = acc;
   }
+}
+
+// t[i] = scale[i] * t_src[map[i]]: source-order phase-1 results into target
+// order (contiguous map reads and t writes, gathered t_src reads).
+template <class A>
+__global__ void tgather_kernel(const int32_t* __restrict__ map, int n, const A* __restrict__ tsrc,
+                               const double* __restrict__ scale, A* __restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  t[i] = tsrc[map[i]] * (A)scale[i];
 }
 
 // ---------------------------------------------------------------- phase 2
@@ -328,8 +325,8 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
                                                      double* __restrict__ ylev, int64_t N,
                                                      int accumulate) {
   __shared__ A red[kWavesPerBlock][64 * VR];
-  const int wv = threadIdx.x >> 6;
-  const int w = blockIdx.x * kWavesPerBlock + wv;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = wave_id();
   const int lane = threadIdx.x & 63;
   const bool valid = w < nitems;
   P2Item it;
@@ -441,7 +438,7 @@ __global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict_
                                                       const A* __restrict__ x, A* __restrict__ t,
                                                       A* __restrict__ part,
                                                       const double* __restrict__ scale) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const TP1Item it = items[w];
@@ -476,7 +473,7 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
                                                       const A* __restrict__ t,
                                                       double* __restrict__ ylev, int64_t N,
                                                       int accumulate) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const TP2Item it = items[w];
@@ -511,7 +508,7 @@ __global__ __launch_bounds__(256) void near64_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const int32_t* __restrict__ perm, KernelDev kd, const double* __restrict__ x,
     double* __restrict__ yL) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const NearItem it = items[w];
@@ -554,7 +551,7 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
     const double* __restrict__ leafc, const int32_t* __restrict__ perm, KernelDev kd,
     const A* __restrict__ x, double* __restrict__ yL) {
-  const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const NearItem it = items[w];
@@ -594,6 +591,45 @@ __global__ __launch_bounds__(256) void near32_kernel(
   }
   if (!act) return;
   yL[ip] += kd.scale * (double)acc;
+}
+
+// 1/r near field with fp64 coordinates and sums: 1/r is the fp32 reciprocal
+// square root of r^2 refined by one Newton step in fp64,
+//   y1 = y0 (3 - r^2 y0^2) / 2,
+// which squares its relative error (~1e-7 -> ~2e-14) at the cost of three
+// fp64 operations instead of an fp64 square root and division.  Source points
+// are read through wave-uniform addresses (scalar loads).
+__global__ __launch_bounds__(256) void near_nr_kernel(
+    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    KernelDev kd, const double* __restrict__ x, double* __restrict__ yL) {
+  // the wave index is uniform: item, neighbour and source-point loads become scalar
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const NearItem it = items[w];
+  const int rs = bstart[it.leaf];
+  const bool act = lane < it.nr;
+  const int ip = rs + it.i0 + (act ? lane : 0);
+  const double xi = px[ip], yi = py[ip], zi = pz[ip];
+  double acc = 0.0;
+  for (int sl = 0; sl <= 26; ++sl) {
+    const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
+    if (nn < 0) continue;
+    const int cs = bstart[nn], m = bcount[nn];
+#pragma unroll 4
+    for (int j = cs; j < cs + m; ++j) {
+      const double dx = xi - px[j], dy = yi - py[j], dz = zi - pz[j];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      // raw v_rsq_f32 on a normal fp32 argument (r^2 below FLT_MIN is clamped)
+      const double y0 = (double)__builtin_amdgcn_rsqf(fmaxf((float)r2, 1.17549435e-38f));
+      const double y1 = y0 * (1.5 - 0.5 * r2 * y0 * y0);
+      acc += (j == ip ? 0.0 : y1) * x[j];
+    }
+  }
+  if (!act) return;
+  yL[ip] += kd.scale * acc;
 }
 
 // y[i] = alpha * (sum over levels of y_j at i's position) + d_i * b[i], where
@@ -887,7 +923,8 @@ void build_plan(Engine& e) {
 void ensure_scratch(Engine& e, int nrhs) {
   int64_t T = e.T, np = e.plan.npartial;
   if (e.mix.ready) {
-    T = std::max<int64_t>(T, (int64_t)e.mix.cls[0].T + e.mix.cls[1].T);
+    // target- and source-order t of both classes
+    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T));
     for (const FactorClass& c : e.mix.cls)
       np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
   }
@@ -985,24 +1022,31 @@ struct FactorView {
   int accumulate;       // phase 2 adds into y_level (not the first copy)
   int key;              // autotuning slot
   const FactorClass* fc;  // recompressed class (source-grouped phase 1), or null
+  void* tsrc;           // that class's source-order t (phase-1 output)
 };
 
 FactorView aca_view(Engine& e, int prec, bool acc64) {
   return {&e.plan, e.d_blk, e.d_btoff, e.d_ubase[prec], e.d_vbase[prec], nullptr, 0,
-          prec + (acc64 && prec != 0 ? 3 : 0), nullptr};
+          prec + (acc64 && prec != 0 ? 3 : 0), nullptr, nullptr};
 }
 
 // Class c of the mixed copy: class 0 (fp32 storage) accumulates in fp64 with t
 // at d_t[0, T0); class 1 (fp16 storage, a small part of the operator) works
 // in fp32 with its t as floats right after class 0's.
+// Scratch t layout of the mixed copy (in doubles): class 0 target order [0, T0),
+// class 1 target order (floats) [T0, T0 + T1), then the source-order copies.
+inline double* class_t0(Engine& e) { return e.d_t; }
+inline float* class_t1(Engine& e) { return (float*)(e.d_t + e.mix.cls[0].T); }
+inline void* class_tsrc(Engine& e, int c) {
+  const int64_t T0 = e.mix.cls[0].T, T1 = e.mix.cls[1].T;
+  return c == 0 ? (void*)(e.d_t + T0 + T1) : (void*)(e.d_t + 2 * T0 + T1);
+}
+
 FactorView class_view(Engine& e, int c) {
   const FactorClass& fc = e.mix.cls[c];
   return {&fc.plan, fc.d_blk, fc.d_btoff, fc.d_ubase, fc.d_vbase, e.mix.d_scale + fc.tbase, c,
-          6 + c, &fc};
+          6 + c, &fc, class_tsrc(e, c)};
 }
-
-inline double* class_t0(Engine& e) { return e.d_t; }
-inline float* class_t1(Engine& e) { return (float*)(e.d_t + e.mix.cls[0].T); }
 
 // Source-grouped phase 1 of a mixed class with a load width of `bytes` per lane.
 template <class S, class A>
@@ -1015,17 +1059,16 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   constexpr int es = (int)sizeof(S);
   const MixP1Item* it = fc.d_mp1;
   if (bytes >= 16)
-    mphase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x, t, part, f.scale,
-                                                     kmax);
+    mphase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, x, (A*)f.tsrc, part, kmax);
   else if (bytes == 8)
-    mphase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
-                                                                       t, part, f.scale, kmax);
+    mphase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
+                                                                       (A*)f.tsrc, part, kmax);
   else if (bytes == 4)
-    mphase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
-                                                                       t, part, f.scale, kmax);
+    mphase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
+                                                                       (A*)f.tsrc, part, kmax);
   else
-    mphase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, fc.d_cmap, x,
-                                                                       t, part, f.scale, kmax);
+    mphase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
+                                                                       (A*)f.tsrc, part, kmax);
   H3D_LAUNCH_CHECK();
 }
 
@@ -1060,9 +1103,13 @@ template <class A>
 void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
   if (f.fc) {
     const FactorClass& fc = *f.fc;
-    if (fc.mp1r.empty()) return;
-    mreduce_kernel<A><<<wave_grid(fc.mp1r.size()), 256, 0, e.stream>>>(
-        fc.d_mp1r, (int)fc.mp1r.size(), part, fc.d_mtidx, t, f.scale);
+    if (!fc.mp1r.empty())
+      mreduce_kernel<A><<<wave_grid(fc.mp1r.size()), 256, 0, e.stream>>>(
+          fc.d_mp1r, (int)fc.mp1r.size(), part, (A*)f.tsrc);
+    H3D_LAUNCH_CHECK();
+    if (fc.T > 0)
+      tgather_kernel<A><<<(fc.T + 255) / 256, 256, 0, e.stream>>>(fc.d_tmap, fc.T, (const A*)f.tsrc,
+                                                                  f.scale, t);
     H3D_LAUNCH_CHECK();
     return;
   }
@@ -1145,7 +1192,7 @@ void tune_mode(Engine& e, const FactorView& f, int reps, const A* x, A* t, A* pa
       if (w >= (int)sizeof(S)) {
         const int nk = (which == 1 && f.fc) ? 2 : 1;
         for (int ki = 0; ki < nk; ++ki) {
-          if (f.fc) e.tune_k[f.key] = ki == 0 ? 4 : 1;
+          if (nk == 2) e.tune_k[f.key] = ki == 0 ? 4 : 1;
           const double tv = time_phase<S, A>(e, f, which, w, reps, x, t, part);
           if (v < 0 || tv < v) v = tv;
           if (tv < best) {
@@ -1300,10 +1347,17 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     KernelDev kd = e.kern;
     if (nn) {
       const unsigned g = wave_grid(nn);
+      // mixed: fp32 entries summed in fp64 (their error, ~2e-8 of ||A x|| at
+      // 2^20 points, sits below the copy's fp32 rounding floor); fp32/bf16
+      // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
+      // fp64: the exact entries
       if (mixed)
         near32_kernel<double><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
                                                  e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
                                                  e.d_leafc, e.d_perm, kd, e.d_xp, yL);
+      else if (acc64 && prec != H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
+        near_nr_kernel<<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                          e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd, e.d_xp, yL);
       else if (acc64 && (kd.q == 7 || kd.q == 16))
         near64_kernel<InvR><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
                                                e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,

@@ -1,11 +1,11 @@
 // Recompressed mixed-precision copy of the far field.
 //
 // For every low-rank block B = U V^T (ACA vectors, fp64 copy):
-//   gram_kernel     Gu = U^T U, Gv = V^T V
-//   eig_kernel      Gv = Lv Lv^T (Cholesky), S = Lv^T Gu Lv = Z diag(lam) Z^T
-//                   (cyclic Jacobi), eigenpairs sorted by lam descending;
-//                   Tu = Lv Z, Tv = Lv^-T Z
-//   project_kernel  U' = U Tu, V' = V Tv, so B = U' V'^T exactly (Z orthogonal),
+//   gram_kernel     Gu = U^T U, Gv = V^T V (compensated sums)
+//   eig_kernel      Gu = Lu Lu^T, Gv = Lv Lv^T (Cholesky), one-sided Jacobi on
+//                   M = Lu^T Lv: M J = Y with orthogonal columns, sorted by
+//                   lam_k = ||Y_k||^2 descending; Tu = Lv J, Tv = Lv^-T J
+//   project_kernel  U' = U Tu, V' = V Tv, so B = U' V'^T exactly (J orthogonal),
 //                   V' has orthonormal columns and ||U'_k||^2 = lam_k
 // A column k of a block then costs lam_k in ||.||_F^2 if dropped and about
 // lam_k * eps_c^2 if stored in a format of relative precision eps_c.  One
@@ -16,10 +16,10 @@
 // sides E||(A - A~) x|| / ||A x|| = ||A - A~||_F / ||A||_F, so `tol` is the
 // expected relative matvec error added to the ACA error.
 //
-// Blocks whose Gv is numerically singular (or of rank above kMaxEigRank) are
-// kept "raw": every ACA column in fp32, unscaled.
+// Blocks whose Gu or Gv is numerically singular (or of rank above kMaxEigRank)
+// are kept "raw": every ACA column in fp32, unscaled.
 //
-// The eigen step runs twice (once to collect every lam for the thresholds,
+// The SVD step runs twice (once to collect every lam for the thresholds,
 // once to write the copy) so that only O(rank) numbers per block are kept
 // between the passes; both runs are deterministic and give the same bits.
 #include <algorithm>
@@ -44,8 +44,8 @@ constexpr double kFp32Err = 4.2e-8;
 constexpr double kCholFail = 1e-12;       // pivot / original diagonal below this: raw block
 constexpr int kMaxSweeps = 24;
 constexpr int kBinsPerOctave = 8;
+constexpr double kColShare = 1.0 / 4096;  // largest share of the error budget one column may take
 constexpr int kMixRows = 512;                       // rows of a mixed phase-1 item
-constexpr int kMixMaxCols = 1024;                   // columns of a mixed phase-1 item (LDS map)
 constexpr int64_t kMixItemBytes = 32768;            // factor bytes of a mixed phase-1 item
 constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (whole source boxes)
 
@@ -53,7 +53,7 @@ constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (wh
 struct CDesc {
   int64_t xu;     // fp64 element offset of U_b's first column in its chunk
   int64_t xv;     // fp64 element offset of V_b in its chunk
-  int64_t w;      // workspace offset: Gu/Tu (r*r), Gv/Tv (r*r), lam (r), status (1)
+  int64_t w;      // workspace offset: Gu/Tu (r*r), Gv/Tv (r*r), lam (r), status, floor
   int32_t n, m, r, ldu, ldv, chunk;
 };
 
@@ -79,7 +79,8 @@ struct PItem {
 // ------------------------------------------------------------------ Gram
 // One workgroup per (block, side): G = X^T X for the n x r (or m x r) column
 // block X, row tiles of 32 staged in LDS, each thread owning up to EMAX of
-// the r(r+1)/2 entries (l <= l').
+// the r(r+1)/2 entries (l <= l'); the tile sums are accumulated with Kahan
+// compensation, so long columns add no rounding beyond a 32-term sum.
 template <int EMAX>
 __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
                                                    void* const* __restrict__ base64,
@@ -97,9 +98,10 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
   const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu);
   const int E = r * (r + 1) / 2;
   int el[EMAX], er[EMAX];
-  double acc[EMAX];
+  double acc[EMAX], cmp[EMAX];  // sums with Kahan compensation over the row tiles
 #pragma unroll
   for (int q = 0; q < EMAX; ++q) {
+    cmp[q] = 0.0;
     const int e = threadIdx.x + 256 * q;
     int b = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
     while (b * (b + 1) / 2 > e) --b;
@@ -123,7 +125,10 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
         const double* b = tile + er[q] * (TR + 1);
         double s = 0.0;
         for (int i = 0; i < TR; ++i) s += a[i] * b[i];
-        acc[q] += s;
+        const double y = s - cmp[q];
+        const double t = acc[q] + y;
+        cmp[q] = (t - acc[q]) - y;
+        acc[q] = t;
       }
     }
   }
@@ -137,12 +142,46 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
   }
 }
 
-// ------------------------------------------------------------------ eigen
-// One workgroup per block.  L (r x r, Cholesky factor of Gv), S (r x r) and
-// Z (r x r), row-major with stride r, live in LDS, or in global scratch
-// (GLOBAL, 3 * gstride doubles per work item) for ranks whose matrices do not
-// fit; per-pair rotation data is always in LDS.
+// ------------------------------------------------------------------ SVD
+// One workgroup per block.  With Gv = Lv Lv^T and Gu = Lu Lu^T (Cholesky),
+// U = Qu Lu^T and V = Qv Lv^T for orthonormal Qu, Qv, so B = Qu M Qv^T with the
+// r x r matrix M = Lu^T Lv.  One-sided (Hestenes) Jacobi rotates M's columns
+// until they are orthogonal, M J = Y; then B = (U Lv J)(V Lv^-T J)^T with the
+// second factor orthonormal and ||U Lv J_k|| = ||Y_k||.  Working on M rather
+// than on S = M^T M keeps the small singular values to relative accuracy
+// (M's columns are graded like the ACA vectors), which a two-sided eigen
+// solver on S would only resolve down to about sqrt(r eps) of the largest.
+// L (= Lv), A (Lu, then J) and M live in LDS, or in global scratch (GLOBAL,
+// 3 * gstride doubles per work item) for ranks whose matrices do not fit;
+// per-pair rotation data is always in LDS.  All matrices row-major, stride r.
 __device__ inline void wg_sync() { __syncthreads(); }
+
+// In-place right-looking Cholesky of the r x r matrix A (lower triangle used);
+// sets *flag when a pivot falls below kCholFail times the original diagonal.
+// Called by every thread of the workgroup.
+__device__ inline void chol_inplace(double* A, int r, double* dg, int* flag, int tid, int nt) {
+  for (int i = tid; i < r; i += nt) dg[i] = A[i * r + i];
+  wg_sync();
+  for (int k = 0; k < r; ++k) {
+    const double dkk = A[k * r + k];
+    if (!(dkk > kCholFail * dg[k])) {  // also catches NaN
+      if (tid == 0) *flag = 1;
+      break;
+    }
+    const double lkk = sqrt(dkk);
+    wg_sync();
+    for (int i = k + 1 + tid; i < r; i += nt) A[i * r + k] /= lkk;
+    if (tid == 0) A[k * r + k] = lkk;
+    wg_sync();
+    const int m = r - k - 1;
+    for (int idx = tid; idx < m * m; idx += nt) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      if (j <= i) A[i * r + j] -= A[i * r + k] * A[j * r + k];
+    }
+    wg_sync();
+  }
+  wg_sync();
+}
 
 template <bool GLOBAL>
 __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
@@ -157,11 +196,11 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   const int r = d.r;
   const int rr = r + (r & 1);
   double* L = GLOBAL ? gscratch + 3 * gstride * item : sm;
-  double* S = L + r * r;
-  double* Z = S + r * r;
-  double* cs = GLOBAL ? sm : Z + r * r;  // rr/2
+  double* A = L + r * r;
+  double* M = A + r * r;
+  double* cs = GLOBAL ? sm : M + r * r;  // rr/2
   double* sn = cs + rr / 2;      // rr/2
-  double* dg = sn + rr / 2;      // r: original diagonal of Gv
+  double* dg = sn + rr / 2;      // r: original diagonal of the matrix being factored
   double* lam = dg + r;          // r
   int* pp = (int*)(lam + r);     // rr/2
   int* pq = pp + rr / 2;         // rr/2
@@ -172,63 +211,27 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
 
   for (int idx = tid; idx < r * r; idx += nt) {
     L[idx] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
-    S[idx] = W[idx];                   // Gu
+    A[idx] = W[idx];                   // Gu
   }
   if (tid == 0) flag = 0;
   wg_sync();
-  for (int i = tid; i < r; i += nt) dg[i] = L[i * r + i];
-  wg_sync();
-  // Cholesky Gv = L L^T (lower), right-looking
-  for (int k = 0; k < r; ++k) {
-    const double dkk = L[k * r + k];
-    if (!(dkk > kCholFail * dg[k])) {  // also catches NaN
-      if (tid == 0) flag = 1;
-      break;
-    }
-    const double lkk = sqrt(dkk);
-    wg_sync();
-    for (int i = k + 1 + tid; i < r; i += nt) L[i * r + k] /= lkk;
-    if (tid == 0) L[k * r + k] = lkk;
-    wg_sync();
-    const int m = r - k - 1;
-    for (int idx = tid; idx < m * m; idx += nt) {
-      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
-      if (j <= i) L[i * r + j] -= L[i * r + k] * L[j * r + k];
-    }
-    wg_sync();
-  }
-  wg_sync();
-  double* out_lam = W + 2 * (int64_t)r * r;
+  chol_inplace(L, r, dg, &flag, tid, nt);
+  if (!flag) chol_inplace(A, r, dg, &flag, tid, nt);
   if (flag) {
     if (tid == 0) W[2 * (int64_t)r * r + r] = 1.0;  // raw
     return;
   }
-  // W1 = Gu L (into Z), then S = L^T W1
+  // M = Lu^T Lv: M[i][j] = sum_{k >= max(i,j)} Lu[k][i] Lv[k][j]
   for (int idx = tid; idx < r * r; idx += nt) {
     const int i = idx / r, j = idx % r;
     double s = 0.0;
-    for (int k = j; k < r; ++k) s += S[i * r + k] * L[k * r + j];
-    Z[idx] = s;
+    for (int k = max(i, j); k < r; ++k) s += A[k * r + i] * L[k * r + j];
+    M[idx] = s;
   }
   wg_sync();
-  for (int idx = tid; idx < r * r; idx += nt) {
-    const int i = idx / r, j = idx % r;
-    double s = 0.0;
-    for (int k = i; k < r; ++k) s += L[k * r + i] * Z[k * r + j];
-    S[idx] = s;
-  }
+  for (int idx = tid; idx < r * r; idx += nt) A[idx] = (idx / r == idx % r) ? 1.0 : 0.0;  // J
   wg_sync();
-  for (int idx = tid; idx < r * r; idx += nt) {
-    const int i = idx / r, j = idx % r;
-    if (i < j) {
-      const double a = 0.5 * (S[i * r + j] + S[j * r + i]);
-      S[i * r + j] = a;
-      S[j * r + i] = a;
-    }
-  }
-  for (int idx = tid; idx < r * r; idx += nt) Z[idx] = (idx / r == idx % r) ? 1.0 : 0.0;
-  wg_sync();
-  // cyclic Jacobi, round-robin pairing (index 0 fixed, the others rotate)
+  // one-sided Jacobi on M's columns, round-robin pairing (index 0 fixed)
   const int np = rr / 2;
   for (int sweep = 0; sweep < kMaxSweeps && r > 1; ++sweep) {
     if (tid == 0) flag = 0;
@@ -244,9 +247,15 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
         }
         double c = 1.0, s = 0.0;
         if (q < r) {
-          const double app = S[p * r + p], aqq = S[q * r + q], apq = S[p * r + q];
-          if (apq != 0.0 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
-            const double th = (aqq - app) / (2.0 * apq);
+          double a = 0.0, b = 0.0, g = 0.0;
+          for (int i = 0; i < r; ++i) {
+            const double mp = M[i * r + p], mq = M[i * r + q];
+            a += mp * mp;
+            b += mq * mq;
+            g += mp * mq;
+          }
+          if (g != 0.0 && fabs(g) > 1e-15 * sqrt(a * b)) {
+            const double th = (b - a) / (2.0 * g);
             const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
             c = 1.0 / sqrt(1.0 + tt * tt);
             s = tt * c;
@@ -259,29 +268,18 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
         pq[t] = q < r ? q : -1;
       }
       wg_sync();
-      // rows: S <- J^T S
-      for (int idx = tid; idx < np * r; idx += nt) {
-        const int t = idx / r, j = idx % r;
-        const int p = pp[t], q = pq[t];
-        if (q < 0 || sn[t] == 0.0) continue;
-        const double c = cs[t], s = sn[t];
-        const double a = S[p * r + j], b = S[q * r + j];
-        S[p * r + j] = c * a - s * b;
-        S[q * r + j] = s * a + c * b;
-      }
-      wg_sync();
-      // columns: S <- S J, Z <- Z J
+      // columns p, q of M and J: (x_p, x_q) <- (c x_p - s x_q, s x_p + c x_q)
       for (int idx = tid; idx < np * r; idx += nt) {
         const int t = idx / r, i = idx % r;
         const int p = pp[t], q = pq[t];
         if (q < 0 || sn[t] == 0.0) continue;
         const double c = cs[t], s = sn[t];
-        const double a = S[i * r + p], b = S[i * r + q];
-        S[i * r + p] = c * a - s * b;
-        S[i * r + q] = s * a + c * b;
-        const double za = Z[i * r + p], zb = Z[i * r + q];
-        Z[i * r + p] = c * za - s * zb;
-        Z[i * r + q] = s * za + c * zb;
+        const double a = M[i * r + p], b = M[i * r + q];
+        M[i * r + p] = c * a - s * b;
+        M[i * r + q] = s * a + c * b;
+        const double ja = A[i * r + p], jb = A[i * r + q];
+        A[i * r + p] = c * ja - s * jb;
+        A[i * r + q] = s * ja + c * jb;
       }
       wg_sync();
     }
@@ -289,8 +287,12 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
     wg_sync();
     if (!any) break;
   }
-  // sort eigenvalues descending (ties by index)
-  for (int i = tid; i < r; i += nt) lam[i] = S[i * r + i];
+  // squared singular values = squared column norms of M J, sorted descending
+  for (int k = tid; k < r; k += nt) {
+    double s = 0.0;
+    for (int i = 0; i < r; ++i) s += M[i * r + k] * M[i * r + k];
+    lam[k] = s;
+  }
   wg_sync();
   for (int i = tid; i < r; i += nt) {
     int rank = 0;
@@ -299,27 +301,33 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   }
   wg_sync();
   double tr = 0.0;
-  for (int i = 0; i < r; ++i) tr += fmax(lam[i], 0.0);
-  // safety margin: the Gram matrices carry rounding of order eps * r * trace
-  const double margin = 32.0 * r * 2.220446049250313e-16 * tr;
-  // Tu[:, k] = L Z[:, perm k], Tv[:, k] = L^-T Z[:, perm k] (column-major, ld r)
+  for (int i = 0; i < r; ++i) tr += lam[i];
+  // rounding floor of the computed values (a few eps * r of the total): the
+  // stored columns are scaled by sqrt(max(lam, floor)), so that a value below
+  // it cannot blow up a stored fp16 column
+  const double noise = 4.0 * r * 2.220446049250313e-16 * tr;
+  // Tu[:, k] = Lv J[:, perm k], Tv[:, k] = Lv^-T J[:, perm k] (column-major, ld r)
+  double* out_lam = W + 2 * (int64_t)r * r;
   for (int k = tid; k < r; k += nt) {
     const int c = perm[k];
     double* tu = W + (int64_t)k * r;
     double* tv = W + (int64_t)r * r + (int64_t)k * r;
     for (int i = 0; i < r; ++i) {
       double s = 0.0;
-      for (int j = 0; j <= i; ++j) s += L[i * r + j] * Z[j * r + c];
+      for (int j = 0; j <= i; ++j) s += L[i * r + j] * A[j * r + c];
       tu[i] = s;
     }
     for (int i = r - 1; i >= 0; --i) {
-      double s = Z[i * r + c];
+      double s = A[i * r + c];
       for (int j = i + 1; j < r; ++j) s -= L[j * r + i] * tv[j];
       tv[i] = s / L[i * r + i];
     }
-    out_lam[k] = fmax(lam[c], 0.0) + margin;
+    out_lam[k] = lam[c];
   }
-  if (tid == 0) W[2 * (int64_t)r * r + r] = 0.0;
+  if (tid == 0) {
+    W[2 * (int64_t)r * r + r] = 0.0;
+    W[2 * (int64_t)r * r + r + 1] = noise;
+  }
 }
 
 // ------------------------------------------------------------------ project
@@ -345,6 +353,7 @@ __global__ __launch_bounds__(256) void project_kernel(
   const double* W = ws + d.w;
   const double* T = W + (side ? (int64_t)r * r : 0);
   const double* lam = W + 2 * (int64_t)r * r;
+  const double floor_ = lam[r + 1];
   float* o32 = (float*)(side ? vbase32[p.vch[0]] : ubase32[d.chunk]) +
                (side ? p.v_out[0] : p.u_out[0]) + row;
   _Float16* o16 = p.r16 > 0 ? (_Float16*)(side ? vbase16[p.vch[1]] : ubase16[d.chunk]) +
@@ -371,7 +380,7 @@ __global__ __launch_bounds__(256) void project_kernel(
       const int k = k0 + q;
       if (k >= keep) break;
       // U side: unit-norm column times su; V side: V' columns are orthonormal
-      const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(lam[k]));
+      const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(fmax(lam[k], floor_)));
       if (k < p.r32) o32[(int64_t)k * ld] = (float)v;
       else o16[(int64_t)(k - p.r32) * ld] = (_Float16)v;
     }
@@ -410,17 +419,19 @@ __global__ __launch_bounds__(256) void near_fro2_kernel(
   if (lane == 0) out[w] = acc;
 }
 
-// Packs every block's lam (r values) and status into a compact array:
-// out[off[i] + k] = lam_k, status[i] = raw flag.
+// Packs every block's lam (r values), status and noise floor:
+// out[off[i] + k] = lam_k, status[i] = raw flag, flo[i] = floor.
 __global__ void pack_lam_kernel(const CDesc* __restrict__ ds, int nblk,
                                 const int64_t* __restrict__ off, const double* __restrict__ ws,
-                                double* __restrict__ out, uint8_t* __restrict__ status) {
+                                double* __restrict__ out, uint8_t* __restrict__ status,
+                                double* __restrict__ flo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nblk) return;
   const CDesc d = ds[i];
   const double* W = ws + d.w + 2 * (int64_t)d.r * d.r;
   for (int k = 0; k < d.r; ++k) out[off[i] + k] = W[k];
   status[i] = W[d.r] != 0.0;
+  flo[i] = W[d.r + 1];
 }
 
 inline double pow2_near_sqrt(int n) {
@@ -450,14 +461,12 @@ void FactorClass::free_all() {
   dfree(d_btoff);
   dfree(d_mp1);
   dfree(d_mp1r);
-  dfree(d_mtidx);
-  dfree(d_cmap);
+  dfree(d_tmap);
   plan.free_device();
   plan = Plan();
   mp1.clear();
   mp1r.clear();
-  mtidx.clear();
-  cmap.clear();
+  tmap.clear();
   lf.clear();
   T = tbase = mpartial = 0;
   u_elems = v_elems = rank_sum = bytes = 0;
@@ -538,7 +547,7 @@ void build_mixed(Engine& e, double tol) {
   void** d_b64 = upload(b64, st);
 
   // per block workspace size (doubles) and batches bounded by a budget
-  auto wsize = [](int r) { return 2 * (int64_t)r * r + r + 1; };
+  auto wsize = [](int r) { return 2 * (int64_t)r * r + r + 2; };
   size_t freeb = 0, totb = 0;
   H3D_HIP(hipMemGetInfo(&freeb, &totb));
   const int64_t ws_budget =
@@ -549,6 +558,7 @@ void build_mixed(Engine& e, double tol) {
   for (int64_t i = 0; i < nb; ++i) lam_off[i + 1] = lam_off[i] + cd[i].r;
   lam_all.assign(lam_off[nb], 0.0);
   std::vector<uint8_t> raw(nb, 0);
+  std::vector<double> lam_floor(nb, 0.0);
 
   // Runs gram + eig on blocks [i0, i1) (ws offsets assigned here); pass 1
   // copies lam/status out, pass 2 keeps them on the device for projection.
@@ -655,9 +665,13 @@ void build_mixed(Engine& e, double tol) {
       int64_t* d_off = upload(off, st);
       double* d_lam = dalloc<double>(std::max<int64_t>(nlam, 1));
       uint8_t* d_stat = dalloc<uint8_t>(cnt);
-      pack_lam_kernel<<<(cnt + 255) / 256, 256, 0, st>>>(d_cd, cnt, d_off, d_ws, d_lam, d_stat);
+      double* d_flo = dalloc<double>(cnt);
+      pack_lam_kernel<<<(cnt + 255) / 256, 256, 0, st>>>(d_cd, cnt, d_off, d_ws, d_lam, d_stat,
+                                                         d_flo);
       H3D_LAUNCH_CHECK();
       std::vector<uint8_t> stat(cnt);
+      H3D_HIP(hipMemcpyAsync(lam_floor.data() + i0, d_flo, sizeof(double) * cnt,
+                             hipMemcpyDeviceToHost, st));
       H3D_HIP(hipMemcpyAsync(lam_all.data() + lam_off[i0], d_lam, sizeof(double) * nlam,
                              hipMemcpyDeviceToHost, st));
       H3D_HIP(hipMemcpyAsync(stat.data(), d_stat, cnt, hipMemcpyDeviceToHost, st));
@@ -665,6 +679,7 @@ void build_mixed(Engine& e, double tol) {
       dfree(d_off);
       dfree(d_lam);
       dfree(d_stat);
+      dfree(d_flo);
       for (int64_t i = i0; i < i1; ++i)
         raw[i] = cd[i].r > kMaxEigRank ? 1 : stat[i - i0];
       i0 = i1;
@@ -706,20 +721,45 @@ void build_mixed(Engine& e, double tol) {
     return (int)std::min<double>(kBins - 1, std::max<double>(0, std::floor(v) - kBinLo));
   };
   const int delta = (int)std::lround(std::log2(1.0 / (kFp16Err * kFp16Err)) * kBinsPerOctave);
+  // No single column may carry more than kColShare of the budget (dropped, or
+  // as fp16 rounding): the added error is then a sum over thousands of
+  // independent terms and the error for one right-hand side stays close to
+  // its expectation.  Columns above a cap get the next higher precision.
+  const double budget = tol * tol * (mx.fro2_far + mx.fro2_near);
+  const double cap_drop = kColShare * budget;
+  const double cap_fp16 = kColShare * budget / (kFp16Err * kFp16Err);
   int bd = 0;
   {
-    std::vector<double> D(kBins + 1, 0.0);
+    // Dd: droppable columns; Dh: columns that may be fp16 but not dropped;
+    // Dp: columns that stay fp32
+    std::vector<double> Dd(kBins + 1, 0.0), Dh(kBins + 1, 0.0);
+    double Dp = 0.0;
     for (int64_t i = 0; i < nb; ++i) {
       if (raw[i]) continue;
       const int nm = cd[i].n + cd[i].m;
-      for (int k = 0; k < cd[i].r; ++k) D[bin_of(lam_all[lam_off[i] + k], nm)] += lam_all[lam_off[i] + k];
+      for (int k = 0; k < cd[i].r; ++k) {
+        const double l = lam_all[lam_off[i] + k];
+        if (l > cap_fp16) Dp += l;
+        else if (l > cap_drop) Dh[bin_of(l, nm)] += l;
+        else Dd[bin_of(l, nm)] += l;
+      }
     }
-    std::vector<double> P(kBins + 1, 0.0);  // P[b] = sum of D[0..b)
-    for (int b = 0; b < kBins; ++b) P[b + 1] = P[b] + D[b];
-    const double budget = tol * tol * (mx.fro2_far + mx.fro2_near);
-    auto err = [&](int b) {
+    std::vector<double> Pd(kBins + 1, 0.0), Ph(kBins + 1, 0.0);  // prefix sums
+    for (int b = 0; b < kBins; ++b) {
+      Pd[b + 1] = Pd[b] + Dd[b];
+      Ph[b + 1] = Ph[b] + Dh[b];
+    }
+    const double e16 = kFp16Err * kFp16Err, e32 = kFp32Err * kFp32Err;
+    auto parts = [&](int b, double* drop, double* f16, double* f32) {
       const int h = std::min(kBins, b + delta);
-      return P[b] + kFp16Err * kFp16Err * (P[h] - P[b]) + kFp32Err * kFp32Err * (P[kBins] - P[h]);
+      *drop = Pd[b];
+      *f16 = e16 * (Pd[h] - Pd[b] + Ph[h]);
+      *f32 = e32 * (Pd[kBins] - Pd[h] + Ph[kBins] - Ph[h] + Dp);
+    };
+    auto err = [&](int b) {
+      double a, c, d;
+      parts(b, &a, &c, &d);
+      return a + c + d;
     };
     // largest b with err(b) <= budget (err is non-decreasing in b)
     int lo = 0, hi = kBins;
@@ -733,13 +773,11 @@ void build_mixed(Engine& e, double tol) {
       }
       bd = lo;
     }
-    mx.theta_drop = std::ldexp(1.0, 0) * std::pow(2.0, (double)(bd + kBinLo) / kBinsPerOctave);
-    mx.err2_drop = P[bd];
-    const int h = std::min(kBins, bd + delta);
-    mx.err2_fp16 = kFp16Err * kFp16Err * (P[h] - P[bd]);
-    mx.err2_fp32 = kFp32Err * kFp32Err * (P[kBins] - P[h]);
+    mx.theta_drop = std::pow(2.0, (double)(bd + kBinLo) / kBinsPerOctave);
+    parts(bd, &mx.err2_drop, &mx.err2_fp16, &mx.err2_fp32);
   }
-  // per block class ranks
+  // per block class ranks: the class only rises with lam, so each class is a
+  // prefix of the block's sorted columns
   std::vector<int32_t> r32(nb), r16(nb);
   mx.aca_rank_sum = 0;
   mx.dropped = 0;
@@ -758,9 +796,10 @@ void build_mixed(Engine& e, double tol) {
     const int nm = cd[i].n + cd[i].m;
     int keep = 0, k32 = 0;
     for (int k = 0; k < r; ++k) {
-      const int b = bin_of(lam_all[lam_off[i] + k], nm);
-      keep += b >= bd;
-      k32 += b >= bd + delta;
+      const double l = lam_all[lam_off[i] + k];
+      const int b = bin_of(l, nm);
+      keep += (b >= bd || l > cap_drop);
+      k32 += (b >= bd + delta || l > cap_fp16);
     }
     r32[i] = k32;
     r16[i] = keep - k32;
@@ -781,9 +820,10 @@ void build_mixed(Engine& e, double tol) {
     fc.nU.assign(e.chunks.size(), 0);
     fc.udata.assign(e.chunks.size(), nullptr);
     fc.nV.assign(1, 0);
-    fc.cmap.assign(e.nblk_all, ColMap{0, 0, 0, 0});
+    fc.tmap.clear();
   }
   std::vector<PDesc> pd(nb);
+  int32_t tsrc_cur[2] = {0, 0};
   std::vector<int32_t> vch[2];
   for (int c = 0; c < 2; ++c) vch[c].assign(e.nblk_all, 0);
   {
@@ -850,6 +890,7 @@ void build_mixed(Engine& e, double tol) {
       for (int32_t ki : lv.ilidx) nsrc[ki]++;
       for (int64_t k = 0; k < lv.nboxes; ++k)
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) tgt[b] = (int32_t)k;
+      for (int c = 0; c < 2; ++c) mx.cls[c].tmap.resize(T[c], 0);
       int64_t pos = lv.bbase;
       for (int64_t sb = 0; sb < lv.nboxes; ++sb) {
         const int m = e.bcount[lo + sb];
@@ -862,6 +903,8 @@ void build_mixed(Engine& e, double tol) {
           if (fc.nV.back() > 0 && fc.nV.back() + C * ld > kVChunkElems) fc.nV.push_back(0);
           const int32_t vc = (int32_t)fc.nV.size() - 1;
           const int64_t vs = fc.nV.back();  // V_s column 0
+          const int32_t tsb = tsrc_cur[c];  // source-order t of V_s column 0
+          tsrc_cur[c] += (int32_t)C;
           int32_t col = 0;
           for (int32_t q = 0; q < nsrc[sb]; ++q) {
             const int32_t gid = e.by_src[pos + q];
@@ -869,7 +912,8 @@ void build_mixed(Engine& e, double tol) {
             const int r = f.rank[b];
             f.voff[b] = vs + (int64_t)col * ld;
             vch[c][gid] = vc;
-            fc.cmap[pos + q] = ColMap{col, col + r, f.toff[tgt[b]] + f.col[b] - col, 0};
+            const int32_t tt = f.toff[tgt[b]] + f.col[b];
+            for (int l = 0; l < r; ++l) fc.tmap[tt + l] = tsb + col + l;
             const int64_t i = ci[gid];
             if (i >= 0) {
               pd[i].v_out[c] = f.voff[b];
@@ -884,21 +928,14 @@ void build_mixed(Engine& e, double tol) {
           const int nsplit = (m + kMixRows - 1) / kMixRows;
           const int rows0 = std::min(m, kMixRows);
           const int ncol = (int)std::max<int64_t>(
-              16, std::min<int64_t>(kMixMaxCols, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
-          int32_t qb = 0;  // block cursor for bpos0
+              16, std::min<int64_t>(1024, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
           for (int32_t c0 = 0; c0 < C; c0 += ncol) {
             const int32_t c1 = (int32_t)std::min<int64_t>(C, (int64_t)c0 + ncol);
-            while (fc.cmap[pos + qb].end <= c0) ++qb;
             int32_t pbase = -1;
             if (nsplit > 1) {
               pbase = fc.mpartial;
               fc.mpartial += nsplit * (c1 - c0);
-              fc.mp1r.push_back({pbase, nsplit, c1 - c0, (int32_t)fc.mtidx.size()});
-              int32_t qq = qb;
-              for (int32_t cc = c0; cc < c1; ++cc) {
-                while (fc.cmap[pos + qq].end <= cc) ++qq;
-                fc.mtidx.push_back(fc.cmap[pos + qq].tmc + cc);
-              }
+              fc.mp1r.push_back({pbase, nsplit, c1 - c0, tsb + c0});
             }
             for (int q = 0; q < nsplit; ++q) {
               MixP1Item it{};
@@ -909,12 +946,9 @@ void build_mixed(Engine& e, double tol) {
               it.s1 = std::min(m, (q + 1) * kMixRows);
               it.c0 = c0;
               it.c1 = c1;
-              it.bpos0 = (int32_t)(pos + qb);
+              it.tsb = tsb;
               it.xoff = e.bstart[lo + sb];
               it.pslot = nsplit > 1 ? pbase + q * (c1 - c0) : -1;
-              int32_t qe = qb;
-              while (fc.cmap[pos + qe].end < c1) ++qe;
-              it.bcnt = qe - qb + 1;
               fc.mp1.push_back(it);
             }
           }
@@ -1012,11 +1046,13 @@ void build_mixed(Engine& e, double tol) {
             const double inv = 1.0 / (pd[i].su * pd[i].sv);
             const LevelFactors& f0 = mx.cls[0].lf[j];
             const LevelFactors& f1 = mx.cls[1].lf[j];
+            const double fl = lam_floor[i];
             for (int kk = 0; kk < r32[i]; ++kk)
-              scale[(size_t)f0.toff[k] + f0.col[b] + kk] = std::sqrt(lam_all[lam_off[i] + kk]) * inv;
+              scale[(size_t)f0.toff[k] + f0.col[b] + kk] =
+                  std::sqrt(std::max(lam_all[lam_off[i] + kk], fl)) * inv;
             for (int kk = 0; kk < r16[i]; ++kk)
               scale[(size_t)mx.cls[1].tbase + f1.toff[k] + f1.col[b] + kk] =
-                  std::sqrt(lam_all[lam_off[i] + r32[i] + kk]) * inv;
+                  std::sqrt(std::max(lam_all[lam_off[i] + r32[i] + kk], fl)) * inv;
           }
         }
     }
@@ -1032,8 +1068,7 @@ void build_mixed(Engine& e, double tol) {
       fc.plan.p1r.clear();
       fc.d_mp1 = upload(fc.mp1, st);
       fc.d_mp1r = upload(fc.mp1r, st);
-      fc.d_mtidx = upload(fc.mtidx, st);
-      fc.d_cmap = upload(fc.cmap, st);
+      fc.d_tmap = upload(fc.tmap, st);
     }
     H3D_HIP(hipStreamSynchronize(st));
   }

@@ -118,22 +118,16 @@ struct MixP1Item {
   int32_t ld;
   int32_t s0, s1;  // row range
   int32_t c0, c1;  // column range
-  int32_t bpos0;   // column-map entry of the block holding column c0
+  int32_t tsb;     // source-order t index of V_s column 0
   int32_t xoff;    // position of the source box's row 0
   int32_t pslot;   // -1: write t; otherwise element offset of (c1 - c0) partial sums
-  int32_t bcnt;    // column-map entries from bpos0 that cover [c0, c1)
-};
-
-// Column map of a class, one entry per block in by-source order: the block
-// owns columns [beg, end) of V_s and the t index of its column c is tmc + c.
-struct ColMap {
-  int32_t beg, end, tmc, pad;
+  int32_t pad;
 };
 
 // Row-split partial sums of a mixed phase-1 column range: nsplit x nc values at
-// part[pbase + q*nc + l] summed in range order into t[tidx[ioff + l]].
+// part[pbase + q*nc + l] summed in range order into t_src[tsb + l].
 struct MixReduce {
-  int32_t pbase, nsplit, nc, ioff;
+  int32_t pbase, nsplit, nc, tsb;
 };
 
 // Per-level factor bookkeeping of one copy, indexed by the level's blocks (CSR order)
@@ -204,14 +198,15 @@ struct FactorClass {
   BlockDev* d_blk = nullptr;         // chunk = V chunk id
   int32_t* d_btoff = nullptr;
   Plan plan;                         // phase 2 and transpose items (phase 1 below)
+  // Phase 1 runs by source box and writes t in source order (t_src, every
+  // store contiguous); tmap[i] is the t_src index of target-order t entry i,
+  // gathered (and scaled) into target order before phase 2.
   std::vector<MixP1Item> mp1;
   std::vector<MixReduce> mp1r;
-  std::vector<int32_t> mtidx;
-  std::vector<ColMap> cmap;
+  std::vector<int32_t> tmap;
   MixP1Item* d_mp1 = nullptr;
   MixReduce* d_mp1r = nullptr;
-  int32_t* d_mtidx = nullptr;
-  ColMap* d_cmap = nullptr;
+  int32_t* d_tmap = nullptr;
   int32_t mpartial = 0;              // partial sums of the phase-1 items
   int32_t T = 0;                     // t entries of this class
   int32_t tbase = 0;                 // offset of this class in the shared t / scale arrays
