@@ -77,24 +77,50 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def mixed_kernels(H, km):
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "matvec_c3_mixed_hbm_counters.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy calibration) from the committed rocprofv3 counter summary of the C3
+    mixed workload, or None when the kernel is not listed there."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 5 and cells[0] == f"`{kernel}`":
+                    return int((float(cells[3]) + float(cells[4])) * 1e9)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def best_width(tuned):
+    return min(tuned, key=tuned.get)
+
+
+def mixed_kernels(H, km, tuned):
     """(name, ms, algorithmic bytes) of the four far-field kernels of the mixed
-    copy: phase 1 / phase 2 of its fp32 class and of its fp16 class.  Phase 1
-    reads the class's V entries, x (fp64) once per level and writes t; phase 2
-    reads the U entries and t and writes (fp32 class) or updates (fp16 class)
-    y once per level."""
+    copy: phase 1 / phase 2 of its fp32 class (fp64 x and t) and of its fp16
+    class (fp32 x and t).  Phase 1 (the kernel alone, event-timed) reads the
+    class's V entries and x once per level and writes t; phase 2 reads the U
+    entries and t and writes (fp32 class) or updates (fp16 class) y once per
+    level."""
     ms_ = H.mixed_stats()
     st = H.geometry_stats()
-    xn = st["nlevels"] * st["N"] * 8
+    lvn = st["nlevels"] * st["N"]
     out = []
-    for c, (es, name) in enumerate(((4, "fp32"), (2, "fp16"))):
+    for c, (es, sx, name, tp) in enumerate(((4, 8, "fp32", "float, double"),
+                                           (2, 4, "fp16", "_Float16, float"))):
         rank = ms_["rank_fp32"] if c == 0 else ms_["rank_fp16"]
-        p1 = es * ms_["v_elems"][c] + xn + 8 * rank
-        p2 = es * ms_["u_elems"][c] + 8 * rank + xn * (1 if c == 0 else 2)
-        t1 = km["phase1"] - km["phase1_second"] if c == 0 else km["phase1_second"]
+        p1 = es * ms_["v_elems"][c] + sx * lvn + sx * rank
+        p2 = es * ms_["u_elems"][c] + sx * rank + 8 * lvn * (1 if c == 0 else 2)
+        t1 = km["phase1_kernel"] if c == 0 else km["phase1_kernel_second"]
         t2 = km["phase2"] - km["phase2_second"] if c == 0 else km["phase2_second"]
-        out.append((f"phase1 {name} class", t1, p1))
-        out.append((f"phase2 {name} class", t2, p2))
+        w1 = best_width(tuned["phase1" if c == 0 else "phase1_fp16"]) // es
+        w2 = best_width(tuned["phase2" if c == 0 else "phase2_fp16"]) // es
+        out.append((f"mphase1_kernel<{tp}, {w1}>", t1, p1))
+        out.append((f"phase2_kernel<{tp}, {w2}>", t2, p2))
     return out
 
 
@@ -222,7 +248,8 @@ def main():
         c = max(kt["count"], 1)
         out["ms"] = ms
         out["kernel_ms"] = {k: kt[k] / c for k in ("phase1", "phase2", "near", "combine", "total",
-                                                   "phase1_second", "phase2_second")}
+                                                   "phase1_second", "phase2_second",
+                                                   "phase1_kernel", "phase1_kernel_second")}
         log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e}, kernels "
             f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }")
         return out
@@ -260,11 +287,11 @@ def main():
     m = res[mixed]
     km = m["kernel_ms"]
     if mixed == "mixed":
-        kern = mixed_kernels(H, km)
+        kern = mixed_kernels(H, km, m["autotune"])
     else:
         elem = {"fp64": 8, "fp32": 4, "bf16": 2}[mixed.split("/")[0]]
         p1b, p2b = alg_bytes(H, elem, acc64="acc64" in mixed)
-        kern = [("phase1", km["phase1"], p1b), ("phase2", km["phase2"], p2b)]
+        kern = [("phase1_kernel", km["phase1_kernel"], p1b), ("phase2_kernel", km["phase2"], p2b)]
     dom, dom_ms, dom_bytes = max(kern, key=lambda k: k[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     far_gbs = sum(k[2] for k in kern) / (sum(k[1] for k in kern) * 1e-3) / 1e9
@@ -336,7 +363,10 @@ def main():
                     "fp64_GB": round(st["factor_elems"] * 8 / 1e9, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom) if args.config == "C3" else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+                     if args.config == "C3" else None,
                      "alg_bytes": int(dom_bytes), "kernel_ms": round(dom_ms, 4),
                      "far_kernels": [{"kernel": k[0], "ms": round(k[1], 4), "alg_bytes": int(k[2]),
                                       "GB/s": round(k[2] / (k[1] * 1e-3) / 1e9, 1) if k[1] > 0 else None}

@@ -1143,9 +1143,11 @@ void launch_p2(Engine& e, const FactorView& f, int bytes, const A* t) {
   H3D_LAUNCH_CHECK();
 }
 
+// `mark` (if set) is recorded right after the phase-1 kernel itself.
 template <class S, class A>
-void run_p1(Engine& e, const FactorView& f, const A* x, A* t, A* part) {
+void run_p1(Engine& e, const FactorView& f, const A* x, A* t, A* part, hipEvent_t mark) {
   launch_p1<S, A>(e, f, e.tune_p1[f.key], x, t, part);
+  if (mark) H3D_HIP(hipEventRecord(mark, e.stream));
   launch_p1_reduce<A>(e, f, part, t);
 }
 
@@ -1210,13 +1212,14 @@ void tune_mode(Engine& e, const FactorView& f, int reps, const A* x, A* t, A* pa
 }
 
 template <class S, class A>
-void run_tp1(Engine& e, const FactorView& f, const A* x, A* t, A* part) {
+void run_tp1(Engine& e, const FactorView& f, const A* x, A* t, A* part, hipEvent_t mark) {
   const Plan& p = *f.plan;
   const int n1 = (int)p.tp1.size();
   hipStream_t st = e.stream;
   if (n1)
     tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(p.d_tp1, n1, f.ubase, x, t, part, f.scale);
   H3D_LAUNCH_CHECK();
+  if (mark) H3D_HIP(hipEventRecord(mark, st));
   if (!p.tp1r.empty())
     reduce_kernel<A><<<wave_grid(p.tp1r.size()), 256, 0, st>>>(p.d_tp1r, (int)p.tp1r.size(), part,
                                                                t, f.scale);
@@ -1236,12 +1239,13 @@ void run_tp2(Engine& e, const FactorView& f, const A* t) {
 
 // Phase 1 (which = 1) or phase 2 (which = 2) of one copy, forward or transpose.
 template <class S, class A>
-void run_phase(Engine& e, const FactorView& f, int which, bool transpose, const A* x, A* t, A* part) {
+void run_phase(Engine& e, const FactorView& f, int which, bool transpose, const A* x, A* t, A* part,
+               hipEvent_t mark = nullptr) {
   if (!transpose) {
-    if (which == 1) run_p1<S, A>(e, f, x, t, part);
+    if (which == 1) run_p1<S, A>(e, f, x, t, part, mark);
     else run_p2<S, A>(e, f, t);
   } else {
-    if (which == 1) run_tp1<S, A>(e, f, x, t, part);
+    if (which == 1) run_tp1<S, A>(e, f, x, t, part, mark);
     else run_tp2<S, A>(e, f, t);
   }
 }
@@ -1311,32 +1315,40 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, (acc64 && !mixed) ? nullptr : e.d_xf);
     H3D_LAUNCH_CHECK();
     H3D_HIP(hipMemsetAsync(e.d_ylev, 0, sizeof(double) * N * e.nL, st));
-    // event slots: 1 phase 1 start, 6 after the first copy's phase 1, 2 phase 2
-    // start, 7 after the first copy's phase 2, 3 near start, 4 combine start, 5 end
+    // event slots: 1 phase 1 start, 8 after the first copy's phase-1 kernel,
+    // 6 after its phase 1 (reduce, gather), 9 after the second copy's phase-1
+    // kernel, 2 phase 2 start, 7 after the first copy's phase 2, 3 near start,
+    // 4 combine start, 5 end
     for (int which = 1; which <= 2; ++which) {
       if (e.timing) H3D_HIP(hipEventRecord(e.ev[which], st));
+      hipEvent_t m1 = (e.timing && which == 1) ? e.ev[8] : nullptr;
+      hipEvent_t m2 = (e.timing && which == 1) ? e.ev[9] : nullptr;
       if (mixed) {
         run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_t0(e),
-                                 e.d_part);
+                                 e.d_part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_t1(e),
-                                   (float*)e.d_part);
+                                   (float*)e.d_part, m2);
       } else if (acc64) {
         const FactorView f = aca_view(e, prec, true);
         if (prec == H3D_PREC_FP64)
-          run_phase<double, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+          run_phase<double, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
         else if (prec == H3D_PREC_FP32)
-          run_phase<float, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+          run_phase<float, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
         else
-          run_phase<uint16_t, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part);
+          run_phase<uint16_t, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
+        if (m2) H3D_HIP(hipEventRecord(m2, st));
       } else {
         const FactorView f = aca_view(e, prec, false);
         float* t = (float*)e.d_t;
         float* part = (float*)e.d_part;
-        if (prec == H3D_PREC_FP32) run_phase<float, float>(e, f, which, transpose, e.d_xf, t, part);
-        else run_phase<uint16_t, float>(e, f, which, transpose, e.d_xf, t, part);
+        if (prec == H3D_PREC_FP32)
+          run_phase<float, float>(e, f, which, transpose, e.d_xf, t, part, m1);
+        else
+          run_phase<uint16_t, float>(e, f, which, transpose, e.d_xf, t, part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
+        if (m2) H3D_HIP(hipEventRecord(m2, st));
       }
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[3], st));
@@ -1379,7 +1391,7 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     if (e.timing) {
       H3D_HIP(hipEventRecord(e.ev[5], st));
       H3D_HIP(hipEventSynchronize(e.ev[5]));
-      float ms[7];
+      float ms[9];
       H3D_HIP(hipEventElapsedTime(&ms[0], e.ev[1], e.ev[2]));  // phase 1 (+ partial reduce)
       H3D_HIP(hipEventElapsedTime(&ms[1], e.ev[2], e.ev[3]));  // phase 2
       H3D_HIP(hipEventElapsedTime(&ms[2], e.ev[3], e.ev[4]));  // near field
@@ -1387,7 +1399,9 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       H3D_HIP(hipEventElapsedTime(&ms[4], e.ev[0], e.ev[5]));  // whole matvec
       H3D_HIP(hipEventElapsedTime(&ms[5], e.ev[6], e.ev[2]));  // phase 1 of the second copy
       H3D_HIP(hipEventElapsedTime(&ms[6], e.ev[7], e.ev[3]));  // phase 2 of the second copy
-      for (int q = 0; q < 7; ++q) e.ktime[q] += ms[q];
+      H3D_HIP(hipEventElapsedTime(&ms[7], e.ev[1], e.ev[8]));  // first copy's phase-1 kernel
+      H3D_HIP(hipEventElapsedTime(&ms[8], e.ev[6], e.ev[9]));  // second copy's phase-1 kernel
+      for (int q = 0; q < 9; ++q) e.ktime[q] += ms[q];
       e.kcount++;
     }
   }

@@ -320,9 +320,9 @@ int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
     h3d::Engine& e = h->e;
     if (ms)
-      for (int q = 0; q < 7; ++q) ms[q] = e.ktime[q];
+      for (int q = 0; q < 9; ++q) ms[q] = e.ktime[q];
     if (count) *count = e.kcount;
-    for (int q = 0; q < 7; ++q) e.ktime[q] = 0;
+    for (int q = 0; q < 9; ++q) e.ktime[q] = 0;
     e.kcount = 0;
     e.timing = enable != 0;
   });

@@ -296,8 +296,8 @@ struct Engine {
   int tune_k[8] = {4, 4, 4, 4, 4, 4, 4, 4};
   // per-kernel event timing of matvecs
   bool timing = false;
-  hipEvent_t ev[8] = {};
-  double ktime[7] = {};
+  hipEvent_t ev[10] = {};
+  double ktime[9] = {};
   int kcount = 0;
 
   // stats

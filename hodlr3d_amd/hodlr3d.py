@@ -232,12 +232,12 @@ class HODLR3D:
 
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
-        ms = (ctypes.c_double * 7)()
+        ms = (ctypes.c_double * 9)()
         cnt = ctypes.c_int()
         nat.check(nat.lib().h3d_kernel_timing(self._h, int(enable), ms, ctypes.byref(cnt)))
         return {"phase1": ms[0], "phase2": ms[1], "near": ms[2], "combine": ms[3],
                 "total": ms[4], "phase1_second": ms[5], "phase2_second": ms[6],
-                "count": cnt.value}
+                "phase1_kernel": ms[7], "phase1_kernel_second": ms[8], "count": cnt.value}
 
     def set_diag(self, c):
         if _is_torch_cuda(c):

@@ -168,11 +168,13 @@ int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
- * stream (enable != 0).  ms[7] receives the accumulated milliseconds of
- * phase 1 (V^T x), phase 2 (U t), near field, combine and whole matvecs since
- * the last read, then the parts of phase 1 and phase 2 spent on the second
- * factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); *count the
- * number of matvecs; all are then reset. */
+ * stream (enable != 0).  ms[9] receives the accumulated milliseconds of
+ * phase 1 (V^T x, with its partial-sum reduction and, for H3D_PREC_MIXED,
+ * the gather into target order), phase 2 (U t), near field, combine and whole
+ * matvecs since the last read; then the parts of phase 1 and phase 2 spent on
+ * the second factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); then
+ * the phase-1 kernel alone of the first and of the second copy.  *count gets
+ * the number of matvecs; all are then reset. */
 int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count);
 
 /* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,
