@@ -30,6 +30,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("matvec ms + HBM GB/s at N=2^20, mixed vs FP64 @ rel err ≤1e-6; solve ms")
 HBM_PEAK_GBS = 8000.0
+# VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (one lane-instruction per clock)
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# VALU instructions per near-field pair in the inner loop of the near kernel each
+# mode uses, counted in the gfx950 assembly of this build (hipcc -S of apply.hip)
+NEAR_VALU_PER_PAIR = {"near32_kernel<double, InvR32>": 14, "near32_kernel<float, InvR32>": 13,
+                      "near_nr_kernel": 18, "near64_kernel<InvR>": 41}
+NEAR_KERNEL = {"mixed": "near32_kernel<double, InvR32>", "fp32": "near32_kernel<float, InvR32>",
+               "bf16": "near32_kernel<float, InvR32>", "fp32/acc64": "near_nr_kernel",
+               "bf16/acc64": "near_nr_kernel", "fp64": "near64_kernel<InvR>"}
 NUM_ERR_ROWS = 2048
 REL_ERR_BAR = 1e-6
 
@@ -93,6 +102,23 @@ def profiled_traffic(kernel):
     except (OSError, ValueError):
         pass
     return None
+
+
+def near_metrics(st, res, mode):
+    """On-the-fly near field: pair evaluations per second against the VALU peak
+    divided by the kernel's VALU instructions per pair."""
+    k = NEAR_KERNEL[mode]
+    ms = res[mode]["kernel_ms"]["near"]
+    per = NEAR_VALU_PER_PAIR[k]
+    evals = st["near_pairs"] / (ms * 1e-3)
+    peak = VALU_LANE_OPS / per
+    return {"kernel": k, "pairs": st["near_pairs"], "ms": round(ms, 4),
+            "evals_per_s": round(evals / 1e9, 1), "unit": "G pair evaluations/s",
+            "valu_per_pair": per, "peak": round(peak / 1e9, 1), "frac": round(evals / peak, 4),
+            "fp64_mode": {"kernel": NEAR_KERNEL["fp64"], "ms": round(res["fp64"]["kernel_ms"]["near"], 4),
+                          "evals_per_s": round(st["near_pairs"] / (res["fp64"]["kernel_ms"]["near"] * 1e-3) / 1e9, 1),
+                          "frac": round(st["near_pairs"] / (res["fp64"]["kernel_ms"]["near"] * 1e-3)
+                                        / (VALU_LANE_OPS / NEAR_VALU_PER_PAIR[NEAR_KERNEL["fp64"]]), 4)}}
 
 
 def best_width(tuned):
@@ -372,6 +398,7 @@ def main():
                                       "GB/s": round(k[2] / (k[1] * 1e-3) / 1e9, 1) if k[1] > 0 else None}
                                      for k in kern]},
         "mixed_copy": mixed_info,
+        "near_field": near_metrics(st, res, mixed),
         "cpu_baseline": cpu,
         "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
     }

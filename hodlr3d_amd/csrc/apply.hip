@@ -499,7 +499,13 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
 namespace {
 
 // ---------------------------------------------------------------- near field
-// fp64: exact entries (the reference's arithmetic).  The i == j term uses the
+// One wave per <= 64 rows of a leaf; lanes own target rows and walk every
+// source point of the leaf and its face/edge neighbours in order; the i == j
+// term is removed by a select.
+//
+// fp64: exact entries (the reference's arithmetic).  The source points come in
+// as scalar loads (the wave index is uniform): with ~40 VALU per pair their
+// latency is hidden, and no per-pair broadcast is needed.  The i == j term uses the
 // diagonal (reference: 0).
 template <class KF>
 __global__ __launch_bounds__(256) void near64_kernel(
@@ -521,30 +527,34 @@ __global__ __launch_bounds__(256) void near64_kernel(
     const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
     if (nn < 0) continue;
     const int cs = bstart[nn], m = bcount[nn];
-    for (int s0 = 0; s0 < m; s0 += 64) {
-      const int cnt = min(64, m - s0);
-      const int jp = cs + s0 + (lane < cnt ? lane : 0);
-      const double sx = px[jp], sy = py[jp], sz = pz[jp], sq = x[jp];
-      for (int u = 0; u < cnt; ++u) {
-        const double X = readlane_d(sx, u), Y = readlane_d(sy, u), Z = readlane_d(sz, u),
-                     Q = readlane_d(sq, u);
-        const int jpos = cs + s0 + u;
-        if (jpos == ip) continue;
-        double dx = xi - X, dy = yi - Y, dz = zi - Z;
-        double R2 = dx * dx + dy * dy + dz * dz;
-        acc += kd.scale * KF::f64(kd.q, R2) * Q;
-      }
+#pragma unroll 2
+    for (int j = cs; j < cs + m; ++j) {
+      const double dx = xi - px[j], dy = yi - py[j], dz = zi - pz[j];
+      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double v = kd.scale * KF::f64(kd.q, R2) * x[j];
+      acc += j == ip ? 0.0 : v;
     }
   }
   if (!act) return;
   yL[ip] += acc;
 }
 
+// fp32 kernel functors for the on-the-fly fp32 near field: 1/r as the raw
+// fp32 reciprocal square root (r^2 below FLT_MIN clamped), or the catalogue.
+struct InvR32 {
+  __device__ static inline float f(int, float R2) {
+    return __builtin_amdgcn_rsqf(fmaxf(R2, 1.17549435e-38f));
+  }
+};
+struct AnyQ32 {
+  __device__ static inline float f(int q, float R2) { return phi32(q, R2); }
+};
+
 // fp32 entries: coordinates as fp32 offsets from the leaf centre plus the fp32
 // difference of the two leaf centres, so near pairs keep ~1e-7 relative
 // accuracy in r.  A is the type of x and of the sums (float, or double for
 // fp64 accumulation of the fp32 entries).
-template <class A>
+template <class A, class KF>
 __global__ __launch_bounds__(256) void near32_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
@@ -571,19 +581,21 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const float dcz = (float)(czk - leafc[3 * (int64_t)nn + 2]);
     const float ax = xi + dcx, ay = yi + dcy, az = zi + dcz;
     A part = 0;
+    // sources in runs of 64: one coalesced load per lane, then broadcast by
+    // readlane (few VALU per pair here, so scalar loads would expose latency)
     for (int s0 = 0; s0 < m; s0 += 64) {
       const int cnt = min(64, m - s0);
       const int jp = cs + s0 + (lane < cnt ? lane : 0);
       const float sx = lx[jp], sy = ly[jp], sz = lz[jp];
       const A sq = x[jp];
       for (int u = 0; u < cnt; ++u) {
-        const float X = readlane_f(sx, u), Y = readlane_f(sy, u), Z = readlane_f(sz, u);
+        const float dx = ax - readlane_f(sx, u), dy = ay - readlane_f(sy, u),
+                    dz = az - readlane_f(sz, u);
+        const float R2 = dx * dx + dy * dy + dz * dz;
+        const float v = cs + s0 + u == ip ? 0.0f : KF::f(kd.q, R2);
         A Q;
         if constexpr (sizeof(A) == 8) Q = readlane_d(sq, u);
         else Q = readlane_f(sq, u);
-        float dx = ax - X, dy = ay - Y, dz = az - Z;
-        float R2 = dx * dx + dy * dy + dz * dz;
-        float v = (cs + s0 + u == ip) ? 0.0f : phi32(kd.q, R2);
         part += (A)v * Q;
       }
     }
@@ -1363,10 +1375,15 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       // 2^20 points, sits below the copy's fp32 rounding floor); fp32/bf16
       // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
       // fp64: the exact entries
-      if (mixed)
-        near32_kernel<double><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                                 e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                 e.d_leafc, e.d_perm, kd, e.d_xp, yL);
+      const bool invr = kd.q == 7 || kd.q == 16;
+      if (mixed && invr)
+        near32_kernel<double, InvR32><<<g, 256, 0, st>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, yL);
+      else if (mixed)
+        near32_kernel<double, AnyQ32><<<g, 256, 0, st>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, yL);
       else if (acc64 && prec != H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
         near_nr_kernel<<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
                                           e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd, e.d_xp, yL);
@@ -1378,10 +1395,14 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
         near64_kernel<AnyQ><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
                                                e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
                                                kd, e.d_xp, yL);
+      else if (invr)
+        near32_kernel<float, InvR32><<<g, 256, 0, st>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, yL);
       else
-        near32_kernel<float><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                                e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz, e.d_leafc,
-                                                e.d_perm, kd, e.d_xf, yL);
+        near32_kernel<float, AnyQ32><<<g, 256, 0, st>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, yL);
       H3D_LAUNCH_CHECK();
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[4], st));
