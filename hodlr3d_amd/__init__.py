@@ -3,7 +3,9 @@
 The compute path is libhodlr3d.so (hand-written HIP for gfx950) behind the C ABI of
 include/hodlr3d.h; this package is its host-side mirror of the reference API.
 """
-from .hodlr3d import HODLR3D, classGMRES, userkernel, pts3D, nlevels, PRECISIONS  # noqa: F401
+from .hodlr3d import (HODLR3D, MatvecGraph, classGMRES, userkernel, pts3D, nlevels,  # noqa: F401
+                      PRECISIONS)
 from ._native import H3DError  # noqa: F401
 
-__all__ = ["HODLR3D", "classGMRES", "userkernel", "pts3D", "nlevels", "PRECISIONS", "H3DError"]
+__all__ = ["HODLR3D", "MatvecGraph", "classGMRES", "userkernel", "pts3D", "nlevels", "PRECISIONS",
+           "H3DError"]

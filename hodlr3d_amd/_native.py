@@ -93,6 +93,10 @@ SIGNATURES = {
     "h3d_export_ranks": (_i32, [_p, _i32, _p, _p]),
     "h3d_export_block": (_i32, [_p, _i32, _i64, _p, _p, _p, _p]),
     "h3d_get_stats": (_i32, [_p, ctypes.POINTER(Stats)]),
+    "h3d_graph_create": (_i32, [ctypes.POINTER(_p), _i32, ctypes.POINTER(_p), ctypes.POINTER(_p),
+                                _i32, _i32, ctypes.POINTER(_p)]),
+    "h3d_graph_launch": (_i32, [_p, _p]),
+    "h3d_graph_destroy": (None, [_p]),
     "h3d_set_stream": (_i32, [_p, _p]),
     "h3d_use_own_stream": (_i32, [_p]),
     "h3d_synchronize": (_i32, [_p]),

@@ -71,6 +71,14 @@ __device__ inline A wave_sum_xor(A v) {
   return v;
 }
 
+// Zero fill (used instead of a memset node, so that a matvec may be captured
+// into a HIP graph).
+__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __restrict__ perm,
                               int64_t N, double* __restrict__ xp, float* __restrict__ xf) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1307,6 +1315,17 @@ void autotune(Engine& e, int mode, int reps, double* ms) {
   H3D_HIP(hipStreamSynchronize(e.stream));
 }
 
+void prepare_mode(Engine& e, int mode) {
+  if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");
+  int prec;
+  bool acc64;
+  check_mode(e, mode, prec, acc64);
+  ensure_copy(e, prec);
+  ensure_scratch(e, 1);
+  if (e.timing && !e.ev[0])
+    for (auto& ev : e.ev) H3D_HIP(hipEventCreate(&ev));
+}
+
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode, bool transpose) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");
   int prec;
@@ -1326,7 +1345,11 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[0], st));
     gather_kernel<<<gN, 256, 0, st>>>(b, e.d_perm, N, e.d_xp, (acc64 && !mixed) ? nullptr : e.d_xf);
     H3D_LAUNCH_CHECK();
-    H3D_HIP(hipMemsetAsync(e.d_ylev, 0, sizeof(double) * N * e.nL, st));
+    {
+      const int64_t n = N * e.nL;
+      zero_kernel<<<(unsigned)std::min<int64_t>(4096, (n + 255) / 256), 256, 0, st>>>(e.d_ylev, n);
+      H3D_LAUNCH_CHECK();
+    }
     // event slots: 1 phase 1 start, 8 after the first copy's phase-1 kernel,
     // 6 after its phase 1 (reduce, gather), 9 after the second copy's phase-1
     // kernel, 2 phase 2 start, 7 after the first copy's phase 2, 3 near start,

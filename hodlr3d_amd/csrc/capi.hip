@@ -582,6 +582,81 @@ int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
   });
 }
 
+struct h3d_graph {
+  std::vector<h3d_handle*> hs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, double* const* y,
+                     int prec, int transpose, h3d_graph** out) {
+  return guarded([&] {
+    if (!hs || !b || !y || !out || count < 1) throw h3d::Error(H3D_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i) {
+      if (!hs[i] || !b[i] || !y[i]) throw h3d::Error(H3D_ERR_INVALID, "null handle or vector");
+      h3d::prepare_mode(hs[i]->e, prec);  // copies and scratch exist before the capture
+      H3D_HIP(hipStreamSynchronize(hs[i]->e.stream));
+    }
+    hipStream_t cap = nullptr;
+    H3D_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    std::vector<hipStream_t> saved(count);
+    std::vector<bool> timing(count);
+    for (int i = 0; i < count; ++i) {
+      saved[i] = hs[i]->e.stream;
+      timing[i] = hs[i]->e.timing;
+      hs[i]->e.stream = cap;
+      hs[i]->e.timing = false;
+    }
+    auto restore = [&] {
+      for (int i = 0; i < count; ++i) {
+        hs[i]->e.stream = saved[i];
+        hs[i]->e.timing = timing[i];
+      }
+    };
+    auto* g = new h3d_graph;
+    try {
+      H3D_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int i = 0; i < count; ++i)
+          h3d::matvec_device(hs[i]->e, b[i], y[i], 1, prec, transpose != 0);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cap, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      H3D_HIP(hipStreamEndCapture(cap, &g->graph));
+      H3D_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      restore();
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      (void)hipStreamDestroy(cap);
+      throw;
+    }
+    restore();
+    (void)hipStreamDestroy(cap);
+    g->hs.assign(hs, hs + count);
+    *out = g;
+  });
+}
+
+int h3d_graph_launch(h3d_graph* g, void* stream) {
+  return guarded([&] {
+    if (!g || !g->exec) throw h3d::Error(H3D_ERR_INVALID, "null graph");
+    H3D_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  });
+}
+
+void h3d_graph_destroy(h3d_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
 int h3d_set_stream(h3d_handle* h, void* stream) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");

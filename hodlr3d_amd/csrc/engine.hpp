@@ -328,6 +328,9 @@ void prepare_precision(Engine& e, int prec, bool move = false);
 void release_precision(Engine& e, int prec);
 void upload_chunk_table(Engine& e);
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool transpose);
+// Builds whatever a matvec in `mode` needs (factor copy, scratch) so that the
+// matvec itself allocates nothing and can be captured into a graph.
+void prepare_mode(Engine& e, int mode);
 void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out);
 // Times every load width of phase 1 and phase 2 for one precision mode on the
 // assembled problem and keeps the fastest; ms[8] = phase 1 at 16/8/4/2 bytes,

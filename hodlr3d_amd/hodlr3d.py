@@ -370,6 +370,48 @@ class HODLR3D:
         return I[:rank], J[:rank], U, V
 
 
+class MatvecGraph:
+    """One matvec (or transpose) of each handle, b[i] -> y[i], captured into a
+    single HIP graph (h3d_graph_create); ``launch()`` replays all of them with
+    one launch on torch's current stream.  b and y are CUDA float64 tensors of
+    each handle's N, kept alive here; rewrite b in place between launches."""
+
+    def __init__(self, handles, b, y, prec="fp64", transpose=False):
+        import torch
+        if not (len(handles) == len(b) == len(y)) or not handles:
+            raise ValueError("handles, b and y must have the same non-zero length")
+        for h, bi, yi in zip(handles, b, y):
+            for t in (bi, yi):
+                if not (_is_torch_cuda(t) and t.dtype == torch.float64 and t.is_contiguous()
+                        and t.numel() == h.N):
+                    raise ValueError("b and y must be contiguous CUDA float64 tensors of length N")
+        self.handles, self.b, self.y = list(handles), list(b), list(y)
+        n = len(handles)
+        hs = (ctypes.c_void_p * n)(*[h._h.value for h in handles])
+        bp = (ctypes.c_void_p * n)(*[t.data_ptr() for t in b])
+        yp = (ctypes.c_void_p * n)(*[t.data_ptr() for t in y])
+        g = ctypes.c_void_p()
+        nat.check(nat.lib().h3d_graph_create(hs, n, bp, yp, _prec(prec), int(transpose),
+                                             ctypes.byref(g)))
+        self._g = g
+
+    def launch(self):
+        import torch
+        nat.check(nat.lib().h3d_graph_launch(
+            self._g, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def close(self):
+        if getattr(self, "_g", None):
+            nat.lib().h3d_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class classGMRES:
     """src/gmres.cpp:40-94 on the GPU: unrestarted GMRES, x0 = 0, MGS + Givens."""
 

@@ -213,6 +213,21 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
 
 int h3d_get_stats(h3d_handle* h, h3d_stats* s);
 
+/* Batched products in one HIP graph (no reference counterpart; the caller side
+ * of MatVecProduct for many instances).  h3d_graph_create runs and captures one
+ * matvec (or transpose) of each handle, hs[i]: b[i] -> y[i] (device pointers,
+ * baked into the graph), in mode `prec`; h3d_graph_launch replays all of them
+ * with one launch on `stream` (NULL: the legacy default stream).  The caller
+ * rewrites the contents of b[i] between launches.  The graph stays valid while
+ * every handle keeps the factor copies and scratch it had at capture: no
+ * prepare/move/release of precisions, compress or larger nrhs in between.  The
+ * handles' own streams are restored after the capture. */
+typedef struct h3d_graph h3d_graph;
+int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, double* const* y,
+                     int prec, int transpose, h3d_graph** out);
+int h3d_graph_launch(h3d_graph* g, void* stream);
+void h3d_graph_destroy(h3d_graph* g);
+
 /* Run subsequent calls on a caller-owned hipStream_t (passed as void*; NULL is
  * the legacy default stream); h3d_use_own_stream returns to the handle's own. */
 int h3d_set_stream(h3d_handle* h, void* stream);
