@@ -17,6 +17,7 @@ H3D_ERR_HIP = 4
 H3D_ERR_STATE = 5
 H3D_ERR_NOT_CONVERGED = 6
 H3D_ERR_INTERNAL = 7
+H3D_ERR_IO = 8
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -56,6 +57,12 @@ class MixedStats(ctypes.Structure):
                 ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d)]
 
 
+class Config(ctypes.Structure):
+    _fields_ = [("N", _i64), ("nlevels", _i32), ("leaf1d", _i32), ("L", _d), ("center", _d * 3),
+                ("tol_pow", _i32), ("qchoice", _i32), ("kernel_scale", _d), ("kernel_diag", _d),
+                ("alpha", _d), ("has_diag_vec", _i32)]
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [("outer_iterations", _i32), ("inner_iterations", _i32), ("outer_prec", _i32),
                 ("rel_residual", _d),
@@ -68,6 +75,9 @@ SIGNATURES = {
     "h3d_create": (_i32, [_p, _i64, _i32, _d, _i32, ctypes.POINTER(Kernel),
                           ctypes.POINTER(Options), ctypes.POINTER(_p)]),
     "h3d_assemble": (_i32, [_p]),
+    "h3d_save": (_i32, [_p, ctypes.c_char_p]),
+    "h3d_load": (_i32, [ctypes.c_char_p, ctypes.POINTER(Options), ctypes.POINTER(_p)]),
+    "h3d_get_config": (_i32, [_p, ctypes.POINTER(Config)]),
     "h3d_prepare_precision": (_i32, [_p, _i32]),
     "h3d_release_precision": (_i32, [_p, _i32]),
     "h3d_move_precision": (_i32, [_p, _i32]),

@@ -658,12 +658,6 @@ void assemble_all(Engine& e) {
           ch.nV += (int64_t)r * (m + (m & 1));
           lv.pivoff[b] = ch.npiv;
           ch.npiv += 2 * (int64_t)r;
-          e.max_rank = std::max(e.max_rank, r);
-          e.factor_elems += (int64_t)r * (n + m);
-          e.u_elems += (int64_t)r * n;
-          e.v_elems += (int64_t)r * m;
-          e.rank_sum += r;
-          e.find_memory += (double)r * n + (double)r * r + (double)m * r;
         }
         lv.Rt[k] = R;
         lv.uoff[k] = ch.nU;
@@ -728,7 +722,37 @@ void assemble_all(Engine& e) {
       if ((int64_t)e.T + lv.Rt[k] > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
       e.T += lv.Rt[k];
     }
-    e.n_blocks += nblk;
+  }
+  finish_assembly(e);
+  e.t_assemble_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Statistics, the per-block device table, blocks grouped by source box, the t
+// index per block and the chunk tables, from the per-level factor bookkeeping
+// (after assembly, or after loading a saved representation).
+void finish_assembly(Engine& e) {
+  hipStream_t st = e.stream;
+  const int nL = e.nL;
+  e.max_rank = 0;
+  e.n_blocks = e.rank_sum = e.factor_elems = e.u_elems = e.v_elems = 0;
+  e.find_memory = 0;
+  for (int j = 1; j <= nL; ++j) {
+    const Level& lv = e.lev[j];
+    const int64_t lo = level_offset(j);
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int64_t n = e.bcount[lo + k];
+      for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+        const int64_t r = lv.rank[b], m = e.bcount[lo + lv.ilidx[b]];
+        e.max_rank = std::max<int>(e.max_rank, (int)r);
+        e.factor_elems += r * (n + m);
+        e.u_elems += r * n;
+        e.v_elems += r * m;
+        e.rank_sum += r;
+        e.find_memory += (double)r * n + (double)r * r + (double)m * r;
+      }
+    }
+    e.n_blocks += (int64_t)lv.ilidx.size();
   }
   // per-block device table, blocks grouped by source box, t index per block
   {
@@ -789,8 +813,6 @@ void assemble_all(Engine& e) {
   }
   upload_chunk_table(e);
   e.assembled = true;
-  e.t_assemble_ms =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // namespace h3d

@@ -216,6 +216,30 @@ int h3d_assemble(h3d_handle* h) {
   });
 }
 
+int h3d_save(h3d_handle* h, const char* path) {
+  return guarded([&] {
+    if (!h || !path) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    h3d::save_engine(h->e, path);
+  });
+}
+
+int h3d_load(const char* path, const h3d_options* opt, h3d_handle** out) {
+  if (out) *out = nullptr;
+  return guarded([&] {
+    if (!path || !out) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    auto* h = new h3d_handle();
+    try {
+      h3d::load_engine(h->e, path, opt ? opt->workspace_bytes : 0);
+      h3d::build_plan(h->e);
+      h3d::ensure_scratch(h->e, 1);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
 int h3d_prepare_precision(h3d_handle* h, int prec) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
@@ -257,8 +281,8 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
     const int es = p == H3D_PREC_MIXED ? 4 : h3d::elem_size(p);
     auto ok = [&](int b) { return (b == 16 || b == 8 || b == 4 || b == 2) && b >= es; };
     if (!ok(p1_bytes) || !ok(p2_bytes)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
-    if (p == H3D_PREC_MIXED) {  // both classes: fp32 (slot 4) and fp16 (slot 6)
-      for (int key : {4, 6}) {
+    if (p == H3D_PREC_MIXED) {  // both classes: fp32 (slot 6) and fp16 (slot 7)
+      for (int key : {6, 7}) {
         h->e.tune_p1[key] = p1_bytes;
         h->e.tune_p2[key] = p2_bytes;
       }
@@ -582,6 +606,25 @@ int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
   });
 }
 
+int h3d_get_config(h3d_handle* h, h3d_config* c) {
+  return guarded([&] {
+    if (!h || !c) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    const h3d::Engine& e = h->e;
+    std::memset(c, 0, sizeof(*c));
+    c->N = e.N;
+    c->nlevels = e.nL;
+    c->leaf1d = e.leaf1d;
+    c->L = e.L;
+    for (int i = 0; i < 3; ++i) c->center[i] = e.ctr[i];
+    c->tol_pow = e.tol_pow;
+    c->qchoice = e.kern.q;
+    c->kernel_scale = e.kern.scale;
+    c->kernel_diag = e.kern.diag;
+    c->alpha = e.alpha;
+    c->has_diag_vec = e.d_diag ? 1 : 0;
+  });
+}
+
 struct h3d_graph {
   std::vector<h3d_handle*> hs;
   hipGraph_t graph = nullptr;
@@ -694,6 +737,7 @@ const char* h3d_status_string(int status) {
     case H3D_ERR_STATE: return "call out of order";
     case H3D_ERR_NOT_CONVERGED: return "not converged";
     case H3D_ERR_INTERNAL: return "internal error";
+    case H3D_ERR_IO: return "file error";
     default: return "unknown status";
   }
 }

@@ -321,6 +321,11 @@ constexpr int kPrecMixed = 3;
 void build_tree(Engine& e, const double* h_xyz);
 // aca.hip
 void assemble_all(Engine& e);
+// Derived tables and statistics from the per-level factor bookkeeping and chunks.
+void finish_assembly(Engine& e);
+// store.hip: the saved assembled representation (load fills a fresh engine)
+void save_engine(Engine& e, const char* path);
+void load_engine(Engine& e, const char* path, int64_t ws_budget);
 // apply.hip
 void build_plan(Engine& e);
 void ensure_scratch(Engine& e, int nrhs);

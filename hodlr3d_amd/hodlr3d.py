@@ -15,6 +15,7 @@ on the GPU; invalid input raises ``H3DError`` instead of undefined behaviour.
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -116,6 +117,42 @@ class HODLR3D:
         self.TOL_POW = int(TOL_POW)
         self.nLevels = self.stats()["nlevels"]
         self.assembled = False
+
+    # -- saved representation (no reference counterpart: the reference keeps nothing on disk)
+    def save(self, path):
+        """Write the assembled representation (points, kernel, operator, tree
+        permutation, ranks, factors, pivots; checksummed) to ``path``."""
+        nat.check(nat.lib().h3d_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, *, workspace_bytes=0):
+        """An assembled handle from a file written by ``save``: its products
+        equal the saved handle's bit for bit.  Raises H3DError (status
+        H3D_ERR_IO) for a missing, truncated or corrupt file."""
+        opt = nat.Options()
+        opt.nlevels = -1
+        opt.workspace_bytes = int(workspace_bytes)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().h3d_load(os.fsencode(path), ctypes.byref(opt), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        c = self.config()
+        self.N = c["N"]
+        self.kernel = userkernel(c["qchoice"], c["kernel_scale"], c["kernel_diag"])
+        self.nParticlesInLeafAlong1D = c["leaf1d"]
+        self.L = c["L"]
+        self.TOL_POW = c["tol_pow"]
+        self.nLevels = c["nlevels"]
+        self.assembled = True
+        return self
+
+    def config(self):
+        c = nat.Config()
+        nat.check(nat.lib().h3d_get_config(self._h, ctypes.byref(c)))
+        return {"N": c.N, "nlevels": c.nlevels, "leaf1d": c.leaf1d, "L": c.L,
+                "center": tuple(c.center), "tol_pow": c.tol_pow, "qchoice": c.qchoice,
+                "kernel_scale": c.kernel_scale, "kernel_diag": c.kernel_diag, "alpha": c.alpha,
+                "has_diag_vec": bool(c.has_diag_vec)}
 
     # -- lifetime
     def close(self):

@@ -27,6 +27,7 @@ extern "C" {
 #define H3D_ERR_STATE 5        /* call out of order, e.g. matvec before assemble */
 #define H3D_ERR_NOT_CONVERGED 6
 #define H3D_ERR_INTERNAL 7
+#define H3D_ERR_IO 8           /* file missing, unreadable, truncated or corrupt */
 
 /* ---- stored-factor precisions ---- */
 #define H3D_PREC_FP64 0  /* fp64 factors, fp64 accumulation, fp64 near field */
@@ -117,6 +118,20 @@ int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
  * src/HODLR3DTree.cpp:1621-1645): pivot-exact ACA of every block on the GPU,
  * factors stored in fp64. */
 int h3d_assemble(h3d_handle* h);
+
+/* Saved assembled representation (no reference counterpart: the reference
+ * keeps nothing on disk; the caller side of HODLR3D::assemble for repeated
+ * runs).  h3d_save writes points, kernel, operator scaling, diagonal, tree
+ * permutation, per-level ranks and offsets, factor chunks (the fp64 copy, else
+ * the fp32 one) and ACA pivots to `path`, with a checksum.  h3d_load rebuilds
+ * the tree from the saved points, checks it against the saved permutation and
+ * every offset against its chunk, and returns an assembled handle whose
+ * products equal the saved handle's bit for bit.  Only opt->workspace_bytes is
+ * read from `opt` (may be NULL).  The mixed copy is not saved: h3d_compress
+ * rebuilds it deterministically.  H3D_ERR_IO: missing, truncated, foreign or
+ * corrupt file. */
+int h3d_save(h3d_handle* h, const char* path);
+int h3d_load(const char* path, const h3d_options* opt, h3d_handle** out);
 
 /* Builds a copy of the factors in `prec` (from the fp64 copy, else the fp32
  * one), or frees one (H3D_PREC_MIXED too); the last fp64/fp32 copy cannot be
@@ -212,6 +227,25 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
                      double* V);
 
 int h3d_get_stats(h3d_handle* h, h3d_stats* s);
+
+/* The construction parameters of a handle (those of h3d_create, or read back
+ * from a saved file by h3d_load); kernel_scale/kernel_diag are the effective
+ * values (Qchoice 16 with scale = diag = 0 resolves to h^3 = 1/N and K_ii).
+ * alpha and the scalar diagonal are the current operator (h3d_set_operator). */
+typedef struct h3d_config {
+  int64_t N;
+  int nlevels;
+  int leaf1d;
+  double L;
+  double center[3];
+  int tol_pow;
+  int qchoice;
+  double kernel_scale;
+  double kernel_diag;
+  double alpha;
+  int has_diag_vec;
+} h3d_config;
+int h3d_get_config(h3d_handle* h, h3d_config* c);
 
 /* Batched products in one HIP graph (no reference counterpart; the caller side
  * of MatVecProduct for many instances).  h3d_graph_create runs and captures one

@@ -65,6 +65,24 @@ def test_invalid_arguments_are_rejected_before_the_gpu():
     assert L.h3d_status_string(nat.H3D_ERR_STATE) == b"call out of order"
 
 
+def test_load_refuses_missing_and_foreign_files_before_the_gpu(tmp_path):
+    from hodlr3d_amd import _native as nat
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    missing = str(tmp_path / "absent.h3d").encode()
+    assert L.h3d_load(missing, None, ctypes.byref(h)) == nat.H3D_ERR_IO
+    for blob in (b"", b"H3DF", b"not a saved operator at all, just text" * 4,
+                 b"H3DFACT1" + b"\x02\x00\x00\x00" + b"\0" * 200):
+        p = tmp_path / "foreign.h3d"
+        p.write_bytes(blob)
+        assert L.h3d_load(str(p).encode(), None, ctypes.byref(h)) == nat.H3D_ERR_IO, blob[:12]
+    assert not h.value
+    assert L.h3d_load(None, None, ctypes.byref(h)) == nat.H3D_ERR_INVALID
+    assert L.h3d_save(None, missing) == nat.H3D_ERR_INVALID
+    assert L.h3d_get_config(None, None) == nat.H3D_ERR_INVALID
+    assert L.h3d_status_string(nat.H3D_ERR_IO) == b"file error"
+
+
 def test_python_mirror_rejects_bad_shapes():
     import hodlr3d_amd as H3
     with pytest.raises(ValueError):
