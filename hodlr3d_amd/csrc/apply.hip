@@ -18,6 +18,7 @@
 
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "stream.hpp"
 
 namespace h3d {
 
@@ -26,43 +27,6 @@ namespace {
 constexpr int kP1Rows = 512;          // rows of a phase-1 item (64 lanes x 8 rows)
 constexpr int64_t kP1Elems = 8192;    // factor entries per multi-block phase-1 item
 constexpr int kP2Rows = 128;          // rows of one phase-2 item
-constexpr int kWavesPerBlock = 4;
-
-// Factor pointers come from per-chunk tables, so the compiler cannot prove
-// they are global memory; loading through an address-space-1 pointer makes
-// them global (not flat) loads, which do not share the LDS wait counter.
-template <class T>
-__device__ inline T ntload(const void* p) {
-  return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)p);
-}
-
-template <class S>
-struct Load;
-template <>
-struct Load<double> {
-  __device__ static inline double f(const double* p) { return ntload<double>(p); }
-};
-template <>
-struct Load<float> {
-  __device__ static inline float f(const float* p) { return ntload<float>(p); }
-};
-template <>
-struct Load<uint16_t> {
-  __device__ static inline float f(const uint16_t* p) { return bf16_to_f(ntload<uint16_t>(p)); }
-};
-template <>
-struct Load<_Float16> {
-  __device__ static inline float f(const _Float16* p) {
-    return (float)__builtin_bit_cast(_Float16, ntload<uint16_t>(p));
-  }
-};
-
-// Global index of the calling wave (one work item per wave), made visibly
-// wave-uniform so that the compiler loads the item and its tables with scalar
-// loads.
-__device__ inline int wave_id() {
-  return __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-}
 
 template <class A>
 __device__ inline A wave_sum_xor(A v) {
@@ -71,13 +35,6 @@ __device__ inline A wave_sum_xor(A v) {
   return v;
 }
 
-// Zero fill (used instead of a memset node, so that a matvec may be captured
-// into a HIP graph).
-__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0;
-}
 
 __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __restrict__ perm,
                               int64_t N, double* __restrict__ xp, float* __restrict__ xf) {
@@ -90,8 +47,6 @@ __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __res
 
 // Vector loads of 2..16 bytes (4-byte aligned from 4 bytes up), converted to
 // the accumulation type.  Factor streams carry the non-temporal hint.
-typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
-typedef uint32_t u32x2a __attribute__((ext_vector_type(2), aligned(4)));
 
 // Element `half` (0: low, 1: high 16 bits) of a 32-bit word of stored S values:
 // uint16_t is bf16, _Float16 is IEEE half.
@@ -547,17 +502,6 @@ __global__ __launch_bounds__(256) void near64_kernel(
   yL[ip] += acc;
 }
 
-// fp32 kernel functors for the on-the-fly fp32 near field: 1/r as the raw
-// fp32 reciprocal square root (r^2 below FLT_MIN clamped), or the catalogue.
-struct InvR32 {
-  __device__ static inline float f(int, float R2) {
-    return __builtin_amdgcn_rsqf(fmaxf(R2, 1.17549435e-38f));
-  }
-};
-struct AnyQ32 {
-  __device__ static inline float f(int q, float R2) { return phi32(q, R2); }
-};
-
 // fp32 entries: coordinates as fp32 offsets from the leaf centre plus the fp32
 // difference of the two leaf centres, so near pairs keep ~1e-7 relative
 // accuracy in r.  A is the type of x and of the sums (float, or double for
@@ -693,9 +637,6 @@ __global__ void convert_kernel<double>(const void* __restrict__ src, int sprec, 
     dst[i] = (double)((const float*)src)[i];
 }
 
-inline unsigned wave_grid(int64_t items) {
-  return (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
-}
 
 // Direct FP64 O(N) sum for one row per workgroup: y_i = alpha * sum_{j != i}
 // K(i,j) b_j + d_i b_i with Neumaier-compensated partial sums per thread and a
@@ -1333,6 +1274,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
   check_mode(e, mode, prec, acc64);
   ensure_copy(e, prec);
   ensure_scratch(e, 1);
+  if (block_width(e, nrhs, prec, transpose) > 1) {
+    matvec_block(e, d_b, d_y, nrhs, prec, acc64);
+    return;
+  }
   hipStream_t st = e.stream;
   const int64_t N = e.N;
   const unsigned gN = (unsigned)((N + 255) / 256);

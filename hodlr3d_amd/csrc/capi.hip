@@ -58,6 +58,12 @@ void Engine::free_all() {
   dfree(d_part);
   dfree(d_ylev);
   dfree(d_io);
+  dfree(d_xw);
+  dfree(d_tw);
+  dfree(d_partw);
+  dfree(d_ylevw);
+  dfree(d_iow);
+  w_cap = io_w = 0;
   for (auto& ev : this->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -271,6 +277,15 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag) {
   });
 }
 
+int h3d_set_rhs_block(h3d_handle* h, int width) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (width != 0 && width != 16 && width != 32)
+      throw h3d::Error(H3D_ERR_INVALID, "width must be 0, 16 or 32");
+    h->e.rhs_block = width;
+  });
+}
+
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
@@ -379,6 +394,25 @@ static int do_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int pr
       return;
     }
     h3d::ensure_scratch(e, 1);
+    const int G = h3d::block_width(e, nrhs, prec & 0xff, transpose);
+    if (G > 1) {  // stage whole groups of columns
+      const int Wg = e.rhs_block == 16 ? 16 : 32;
+      if (e.io_w < Wg) {
+        h3d::dfree(e.d_iow);
+        e.d_iow = h3d::dalloc<double>(2 * N * Wg);
+        e.io_w = Wg;
+      }
+      for (int r0 = 0; r0 < nrhs; r0 += Wg) {
+        const int cnt = std::min(Wg, nrhs - r0);
+        H3D_HIP(hipMemcpyAsync(e.d_iow, b + (int64_t)r0 * N, sizeof(double) * N * cnt,
+                               hipMemcpyHostToDevice, e.stream));
+        h3d::matvec_device(e, e.d_iow, e.d_iow + N * Wg, cnt, prec, transpose);
+        H3D_HIP(hipMemcpyAsync(y + (int64_t)r0 * N, e.d_iow + N * Wg, sizeof(double) * N * cnt,
+                               hipMemcpyDeviceToHost, e.stream));
+      }
+      H3D_HIP(hipStreamSynchronize(e.stream));
+      return;
+    }
     for (int r = 0; r < nrhs; ++r) {
       H3D_HIP(hipMemcpyAsync(e.d_io, b + (int64_t)r * N, sizeof(double) * N, hipMemcpyHostToDevice,
                              e.stream));

@@ -284,6 +284,17 @@ struct Engine {
   double* d_ylev = nullptr;   // per-level y by position: nL * N
   double* d_io = nullptr;     // staging for host vectors
   int64_t t_cap = 0, part_cap = 0;  // allocated entries of d_t / d_part per right-hand side
+  // products with several right-hand sides: groups of up to rhs_block (16 or
+  // 32; 0 = one at a time) applied together (multi.hip); scratch for W per entry
+  int rhs_block = 32;
+  int w_cap = 0;                    // W of the allocated group scratch
+  int64_t w_t = 0, w_np = 0;        // t and partial entries it holds per RHS
+  double* d_xw = nullptr;           // x by position, quad and row layouts
+  double* d_tw = nullptr;           // t, W per column
+  double* d_partw = nullptr;        // phase-1 partial sums, W per entry
+  double* d_ylevw = nullptr;        // per-level y by position, W per point
+  double* d_iow = nullptr;          // host staging of one group (2 N W)
+  int io_w = 0;
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
@@ -326,6 +337,10 @@ void finish_assembly(Engine& e);
 // store.hip: the saved assembled representation (load fills a fresh engine)
 void save_engine(Engine& e, const char* path);
 void load_engine(Engine& e, const char* path, int64_t ws_budget);
+// multi.hip: group width for nrhs right-hand sides (1: one at a time), and
+// the grouped product (forward, ACA factor copies)
+int block_width(const Engine& e, int nrhs, int prec, bool transpose);
+void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64);
 // apply.hip
 void build_plan(Engine& e);
 void ensure_scratch(Engine& e, int nrhs);

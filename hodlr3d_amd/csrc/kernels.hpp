@@ -81,4 +81,24 @@ static __device__ __noinline__ float phi32_any(int q, float R2) {
   }
 }
 
+// fp32 kernel functors for the on-the-fly fp32 near field: 1/r as the raw
+// fp32 reciprocal square root (r^2 below FLT_MIN clamped), or the catalogue.
+struct InvR32 {
+  __device__ static inline float f(int, float R2) {
+    return __builtin_amdgcn_rsqf(fmaxf(R2, 1.17549435e-38f));
+  }
+};
+struct AnyQ32 {
+  __device__ static inline float f(int q, float R2) { return phi32(q, R2); }
+};
+
+// 1/r in fp64 from the fp32 reciprocal square root refined by one Newton step,
+// y1 = y0 (3 - r^2 y0^2) / 2 (relative error ~1e-7 -> ~2e-14).
+struct InvRNewton {
+  __device__ static inline double f64(int, double R2) {
+    const double y0 = (double)__builtin_amdgcn_rsqf(fmaxf((float)R2, 1.17549435e-38f));
+    return y0 * (1.5 - 0.5 * R2 * y0 * y0);
+  }
+};
+
 }  // namespace h3d

@@ -1,0 +1,519 @@
+// Products with many right-hand sides: the RHS are taken W (16 or 32) at a
+// time and every factor element is read once per group and applied to all W
+// of them on the matrix cores (the caller side of MatVecProduct,
+// src/HODLR3D.cpp:21-27, called once per column in the reference).
+//
+//   gather    x by position in two layouts: quads (x[p..p+3] of one RHS
+//             contiguous, the MFMA B operand of phase 1) and rows (the W
+//             values of one point contiguous, read by the near field)
+//   phase 1   T_b = V_b^T X_s   per block, 16x16x4 MFMA: A = 16 columns x 4
+//             rows of V (one 4-row vector per lane), B = 4 rows x 16 RHS
+//   phase 2   Y_k = U_k T_k     per <= 64 rows of a target box, 16x16x4 MFMA:
+//             A = 16 row quads x 4 columns of U, B = 4 columns x 16 RHS of T
+//   near      y_L += K_near X   lanes own target rows, W sums each, source
+//             points and their W values through wave-uniform (scalar) loads
+//   combine   y[:, j] = alpha * sum_levels + d * b[:, j] (an LDS transpose)
+//
+// The work lists are the single-RHS plan's (phase-1 items, partial sums and
+// phase-2 items), so the order of the block loop is the same; fp64 factors
+// and the fp64-accumulation modes use the fp64 MFMA, fp32/bf16 factors with
+// fp32 accumulation the fp32 MFMA.
+#include <algorithm>
+#include <type_traits>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "stream.hpp"
+
+namespace h3d {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16) on one wave: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16], and holds D[row(l / 16, a)][l % 16] in element a:
+// row = 4 (l / 16) + a for the fp32 instruction, 4 a + l / 16 for the fp64
+// one (checked element by element on gfx950).
+template <class A>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using acc = f32x4;
+  __device__ static inline acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static constexpr int row(int q, int a) { return 4 * q + a; }
+};
+template <>
+struct Mfma<double> {
+  using acc = f64x4;
+  __device__ static inline acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static constexpr int row(int q, int a) { return 4 * a + q; }
+};
+
+// Four consecutive stored elements (4-byte aligned) converted to A.
+template <class S, class A>
+__device__ inline void load4(const S* p, A (&o)[4]) {
+  if constexpr (sizeof(S) == 8) {
+    const u32x4a v0 = ntload<u32x4a>(p), v1 = ntload<u32x4a>(p + 2);
+    o[0] = (A)__hiloint2double((int)v0.y, (int)v0.x);
+    o[1] = (A)__hiloint2double((int)v0.w, (int)v0.z);
+    o[2] = (A)__hiloint2double((int)v1.y, (int)v1.x);
+    o[3] = (A)__hiloint2double((int)v1.w, (int)v1.z);
+  } else if constexpr (sizeof(S) == 4) {
+    const u32x4a v = ntload<u32x4a>(p);
+    o[0] = (A)__uint_as_float(v.x);
+    o[1] = (A)__uint_as_float(v.y);
+    o[2] = (A)__uint_as_float(v.z);
+    o[3] = (A)__uint_as_float(v.w);
+  } else {  // bf16: element 0 in the low half of each word
+    const u32x2a v = ntload<u32x2a>(p);
+    o[0] = (A)__uint_as_float(v.x << 16);
+    o[1] = (A)__uint_as_float(v.x & 0xffff0000u);
+    o[2] = (A)__uint_as_float(v.y << 16);
+    o[3] = (A)__uint_as_float(v.y & 0xffff0000u);
+  }
+}
+
+// Four consecutive x values of one RHS in the quad layout (16 or 32 bytes).
+template <class A>
+__device__ inline void load_quad(const A* p, A (&o)[4]) {
+  if constexpr (sizeof(A) == 8) {
+    const f64x4 v = *(const f64x4*)p;
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  } else {
+    const f32x4 v = *(const f32x4*)p;
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+}
+
+// Quad layout: x(p, j) at xq[((p / 4) W + j) 4 + p % 4], Npad positions (the
+// rows past N are zero); row layout: x(p, j) at xr[p W + j].  Columns j >= cnt
+// are zero.
+template <class A, int W>
+__global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __restrict__ perm,
+                                int64_t N, int64_t Npad, int cnt, A* __restrict__ xq,
+                                A* __restrict__ xr) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= Npad * W) return;
+  const int j = (int)((id >> 2) % W);
+  const int64_t p = 4 * (id / (4 * W)) + (id & 3);
+  const A v = (p < N && j < cnt) ? (A)b[(int64_t)j * N + perm[p]] : (A)0;
+  xq[id] = v;
+  xr[p * W + j] = v;
+}
+
+// Phase 1: for each block of the item, T_b(c, j) = sum over rows p in
+// [s0, s1) of V_b(c, p) x(src + p, j), 64 columns (four 16-column tiles) per
+// pass.  Rows are walked in steps of 16 aligned to absolute position: lane
+// (c = l % 16, q = l / 16) takes rows P + 4q .. P + 4q + 3 of column c; rows
+// outside the item's range contribute 0 on both operands.
+template <class S, class A, int W>
+__global__ __launch_bounds__(256) void p1w_kernel(const P1Item* __restrict__ items, int nitems,
+                                                  void* const* __restrict__ vbase,
+                                                  const BlockDev* __restrict__ blk,
+                                                  const A* __restrict__ xq, A* __restrict__ tw,
+                                                  A* __restrict__ partw) {
+  using M = Mfma<A>;
+  using Acc = typename M::acc;
+  constexpr int H = W / 16;
+  constexpr int CT = 4;
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const int lc = lane & 15, kq = lane >> 4;
+  const P1Item it = items[w];
+  const S* V = (const S*)vbase[it.chunk];
+  for (int b = it.b0; b < it.b1; ++b) {
+    const BlockDev bd = blk[b];
+    const int s1 = min(it.s1, bd.m);
+    if (bd.r == 0 || s1 <= it.s0) continue;
+    const int64_t ldv = bd.m + (bd.m & 1);
+    const int p0 = bd.src + it.s0, pe = bd.src + s1;
+    for (int cb = 0; cb < bd.r; cb += 16 * CT) {
+      Acc acc[CT][H];
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
+      for (int P = p0 & ~3; P < pe; P += 16) {
+        const int pr = P + 4 * kq;
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ok[u] = pr + u >= p0 && pr + u < pe;
+        const bool full = ok[0] && ok[3];
+        A xb[H][4];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          load_quad<A>(xq + ((int64_t)(pr >> 2) * W + 16 * h + lc) * 4, xb[h]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xb[h][u] = ok[u] ? xb[h][u] : (A)0;
+        }
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          if (cb + 16 * ct >= bd.r) break;  // wave-uniform
+          const int c = cb + 16 * ct + lc;
+          A av[4] = {0, 0, 0, 0};
+          if (c < bd.r) {
+            const S* col = V + bd.voff + c * ldv + (pr - bd.src);
+            if (full) {
+              load4<S, A>(col, av);
+            } else {
+#pragma unroll
+              for (int u = 0; u < 4; ++u)
+                if (ok[u]) av[u] = (A)Load<S>::f(col + u);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[u], xb[h][u], acc[ct][h]);
+        }
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        if (cb + 16 * ct >= bd.r) break;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int c = cb + 16 * ct + M::row(kq, a);
+          if (c >= bd.r) continue;
+          A* out = it.pslot < 0 ? tw + (int64_t)(it.toff + bd.col + c) * W
+                                : partw + ((int64_t)it.pslot + c) * W;
+#pragma unroll
+          for (int h = 0; h < H; ++h) out[16 * h + lc] = acc[ct][h][a];
+        }
+      }
+    }
+  }
+}
+
+// Row-split partial sums of one block, summed in range order.
+template <class A, int W>
+__global__ void reduce_w_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ partw,
+                                A* __restrict__ tw) {
+  const int w = wave_id();
+  if (w >= n) return;
+  const int lane = threadIdx.x & 63;
+  const P1Reduce r = rs[w];
+  for (int e = lane; e < r.nc * W; e += 64) {
+    const int l = e / W, j = e % W;
+    A acc = 0;
+    for (int q = 0; q < r.nsplit; ++q)
+      acc += partw[((int64_t)r.pbase + (int64_t)q * r.nc + l) * W + j];
+    tw[(int64_t)(r.toff + l) * W + j] = acc;
+  }
+}
+
+// Phase 2: y(i, j) = sum_c U_k(i, c) T_k(c, j) for <= 64 rows at a time; lane
+// (g = l % 16, q = l / 16) takes rows i + 4g .. i + 4g + 3 of column c + q.
+// The u-th MFMA of a step produces rows i + 4g' + u (g' the MFMA output row).
+template <class S, class A, int W>
+__global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
+                                                  void* const* __restrict__ ubase,
+                                                  const A* __restrict__ tw,
+                                                  double* __restrict__ ylevw, int64_t N) {
+  using M = Mfma<A>;
+  using Acc = typename M::acc;
+  constexpr int H = W / 16;
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const int lc = lane & 15, kq = lane >> 4;
+  const P2Item it = items[w];
+  const S* U = (const S*)ubase[it.chunk] + it.uoff;
+  double* yb = ylevw + ((int64_t)(it.level - 1) * N + it.yoff) * W;
+  for (int r0 = 0; r0 < it.nr; r0 += 64) {
+    const int lim = it.i0 + min(it.nr, r0 + 64);  // rows < lim (box-relative)
+    const int i = it.i0 + r0 + 4 * lc;
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ok[u] = i + u < lim;
+    Acc acc[4][H];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int h = 0; h < H; ++h) acc[u][h] = Acc{};
+    for (int c0 = 0; c0 < it.R; c0 += 4) {
+      const int c = c0 + kq;
+      A tb[H];
+      A av[4] = {0, 0, 0, 0};
+      if (c < it.R) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) tb[h] = tw[(int64_t)(it.toff + c) * W + 16 * h + lc];
+        const S* col = U + (int64_t)c * it.ld + i;
+        if (ok[3]) {
+          load4<S, A>(col, av);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (ok[u]) av[u] = (A)Load<S>::f(col + u);
+        }
+      } else {
+#pragma unroll
+        for (int h = 0; h < H; ++h) tb[h] = 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int h = 0; h < H; ++h) acc[u][h] = M::mac(av[u], tb[h], acc[u][h]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int row = it.i0 + r0 + 4 * M::row(kq, a) + u;
+        if (row < lim) {
+#pragma unroll
+          for (int h = 0; h < H; ++h) yb[(int64_t)row * W + 16 * h + lc] = (double)acc[u][h][a];
+        }
+      }
+  }
+}
+
+// Near field, fp64 coordinates and sums (exact entries, or 1/r refined from
+// the fp32 reciprocal square root): lanes own target rows; the source point
+// and its W values are wave-uniform.
+template <class KF, int W>
+__global__ __launch_bounds__(256) void near_w64_kernel(
+    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    KernelDev kd, const double* __restrict__ xr, double* __restrict__ yLw) {
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const NearItem it = items[w];
+  const int rs = bstart[it.leaf];
+  const bool act = lane < it.nr;
+  const int ip = rs + it.i0 + (act ? lane : 0);
+  const double xi = px[ip], yi = py[ip], zi = pz[ip];
+  double acc[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) acc[j] = 0.0;
+  for (int sl = 0; sl <= 26; ++sl) {
+    const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
+    if (nn < 0) continue;
+    const int cs = bstart[nn], m = bcount[nn];
+    for (int s = cs; s < cs + m; ++s) {
+      const double dx = xi - px[s], dy = yi - py[s], dz = zi - pz[s];
+      const double R2 = dx * dx + dy * dy + dz * dz;
+      const double k = s == ip ? 0.0 : KF::f64(kd.q, R2);
+      const double* xs = xr + (int64_t)s * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] += k * xs[j];
+    }
+  }
+  if (!act) return;
+  double* yp = yLw + (int64_t)ip * W;
+#pragma unroll
+  for (int j = 0; j < W; ++j) yp[j] += kd.scale * acc[j];
+}
+
+// Near field with fp32 entries and sums: coordinates as fp32 offsets from the
+// leaf centre plus the fp32 difference of the two centres (as the single-RHS
+// fp32 near field).
+template <class KF, int W>
+__global__ __launch_bounds__(256) void near_w32_kernel(
+    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
+    const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
+    const double* __restrict__ leafc, KernelDev kd, const float* __restrict__ xr,
+    double* __restrict__ yLw) {
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const NearItem it = items[w];
+  const int rs = bstart[it.leaf];
+  const bool act = lane < it.nr;
+  const int ip = rs + it.i0 + (act ? lane : 0);
+  const float xi = lx[ip], yi = ly[ip], zi = lz[ip];
+  const double cxk = leafc[3 * (int64_t)it.leaf], cyk = leafc[3 * (int64_t)it.leaf + 1],
+               czk = leafc[3 * (int64_t)it.leaf + 2];
+  float acc[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) acc[j] = 0.0f;
+  for (int sl = 0; sl <= 26; ++sl) {
+    const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
+    if (nn < 0) continue;
+    const int cs = bstart[nn], m = bcount[nn];
+    const float ax = xi + (float)(cxk - leafc[3 * (int64_t)nn]);
+    const float ay = yi + (float)(cyk - leafc[3 * (int64_t)nn + 1]);
+    const float az = zi + (float)(czk - leafc[3 * (int64_t)nn + 2]);
+    for (int s = cs; s < cs + m; ++s) {
+      const float dx = ax - lx[s], dy = ay - ly[s], dz = az - lz[s];
+      const float R2 = dx * dx + dy * dy + dz * dz;
+      const float k = s == ip ? 0.0f : KF::f(kd.q, R2);
+      const float* xs = xr + (int64_t)s * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[j] += k * xs[j];
+    }
+  }
+  if (!act) return;
+  double* yp = yLw + (int64_t)ip * W;
+#pragma unroll
+  for (int j = 0; j < W; ++j) yp[j] += kd.scale * (double)acc[j];
+}
+
+// y[j N + i] = alpha * sum over levels of y_level(iperm[i], j) + d_i b[j N + i]
+// for j < cnt: 64 points per block, transposed through LDS so that both the
+// level sums (W per point) and the outputs (64 points per RHS) are contiguous.
+template <int W>
+__global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict__ ylevw, int nL,
+                                                        int64_t N, const int32_t* __restrict__ iperm,
+                                                        const double* __restrict__ b, int cnt,
+                                                        double alpha, double diag,
+                                                        const double* __restrict__ diag_vec,
+                                                        double* __restrict__ y) {
+  __shared__ double tile[W][65];
+  const int64_t i0 = (int64_t)blockIdx.x * 64;
+  for (int q = threadIdx.x; q < 64 * W; q += 256) {
+    const int ii = q / W, j = q % W;
+    const int64_t i = i0 + ii;
+    double acc = 0.0;
+    if (i < N && j < cnt) {
+      const int64_t s = iperm[i];
+      for (int lv = 0; lv < nL; ++lv) acc = acc + ylevw[((int64_t)lv * N + s) * W + j];
+    }
+    tile[j][ii] = acc;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 64 * W; q += 256) {
+    const int j = q / 64, ii = q % 64;
+    const int64_t i = i0 + ii;
+    if (i < N && j < cnt) {
+      const double d = diag_vec ? diag_vec[i] : diag;
+      y[(int64_t)j * N + i] = alpha * tile[j][ii] + d * b[(int64_t)j * N + i];
+    }
+  }
+}
+
+inline int64_t padded_points(int64_t N) { return ((N + 3) & ~int64_t(3)) + 16; }
+
+void ensure_block_scratch(Engine& e, int W) {
+  const int64_t T = std::max<int64_t>(e.T, 1), np = std::max<int64_t>(e.plan.npartial, 1);
+  if (e.w_cap >= W && e.w_t >= T && e.w_np >= np) return;
+  W = std::max(W, e.w_cap);
+  dfree(e.d_xw);
+  dfree(e.d_tw);
+  dfree(e.d_partw);
+  dfree(e.d_ylevw);
+  e.w_cap = 0;
+  e.d_xw = dalloc<double>(2 * padded_points(e.N) * W);
+  e.d_tw = dalloc<double>(T * W);
+  e.d_partw = dalloc<double>(np * W);
+  e.d_ylevw = dalloc<double>(e.N * e.nL * W);
+  e.w_cap = W;
+  e.w_t = T;
+  e.w_np = np;
+}
+
+template <class S, class A, int W>
+void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int cnt) {
+  hipStream_t st = e.stream;
+  const int64_t N = e.N, Npad = padded_points(N);
+  A* xq = (A*)e.d_xw;
+  A* xr = xq + Npad * W;
+  A* tw = (A*)e.d_tw;
+  A* partw = (A*)e.d_partw;
+  gather_w_kernel<A, W><<<(unsigned)((Npad * W + 255) / 256), 256, 0, st>>>(b, e.d_perm, N, Npad,
+                                                                           cnt, xq, xr);
+  H3D_LAUNCH_CHECK();
+  {
+    const int64_t n = N * e.nL * W;
+    zero_kernel<<<(unsigned)std::min<int64_t>(8192, (n + 255) / 256), 256, 0, st>>>(e.d_ylevw, n);
+    H3D_LAUNCH_CHECK();
+  }
+  const Plan& p = e.plan;
+  if (!p.p1.empty()) {
+    p1w_kernel<S, A, W><<<wave_grid(p.p1.size()), 256, 0, st>>>(
+        p.d_p1, (int)p.p1.size(), e.d_vbase[prec], e.d_blk, xq, tw, partw);
+    H3D_LAUNCH_CHECK();
+  }
+  if (!p.p1r.empty()) {
+    reduce_w_kernel<A, W><<<wave_grid(p.p1r.size()), 256, 0, st>>>(p.d_p1r, (int)p.p1r.size(),
+                                                                   partw, tw);
+    H3D_LAUNCH_CHECK();
+  }
+  if (!p.p2.empty()) {
+    p2w_kernel<S, A, W><<<wave_grid(p.p2.size()), 256, 0, st>>>(p.d_p2, (int)p.p2.size(),
+                                                                e.d_ubase[prec], tw, e.d_ylevw, N);
+    H3D_LAUNCH_CHECK();
+  }
+  const int nn = (int)e.nitems.size();
+  if (nn) {
+    double* yLw = e.d_ylevw + (int64_t)(e.nL - 1) * N * W;
+    const int64_t lo = level_offset(e.nL);
+    const KernelDev kd = e.kern;
+    const bool invr = kd.q == 7 || kd.q == 16;
+    const unsigned g = wave_grid(nn);
+    const int32_t* nbr = e.lev[e.nL].d_nbr;
+    if constexpr (std::is_same<A, float>::value) {
+      if (invr)
+        near_w32_kernel<InvR32, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, yLw);
+      else
+        near_w32_kernel<AnyQ32, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, yLw);
+    } else {
+      if (invr && prec != H3D_PREC_FP64)
+        near_w64_kernel<InvRNewton, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                          e.d_bcount + lo, e.d_px, e.d_py,
+                                                          e.d_pz, kd, xr, yLw);
+      else if (invr)
+        near_w64_kernel<InvR, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                    xr, yLw);
+      else
+        near_w64_kernel<AnyQ, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                    xr, yLw);
+    }
+    H3D_LAUNCH_CHECK();
+  }
+  (void)acc64;
+  combine_w_kernel<W><<<(unsigned)((N + 63) / 64), 256, 0, st>>>(
+      e.d_ylevw, e.nL, N, e.d_iperm, b, cnt, e.alpha, e.kern.diag, e.d_diag, y);
+  H3D_LAUNCH_CHECK();
+}
+
+template <int W>
+void run_group_w(Engine& e, int prec, bool acc64, const double* b, double* y, int cnt) {
+  if (prec == H3D_PREC_FP64) run_group<double, double, W>(e, prec, acc64, b, y, cnt);
+  else if (prec == H3D_PREC_FP32 && acc64) run_group<float, double, W>(e, prec, acc64, b, y, cnt);
+  else if (prec == H3D_PREC_FP32) run_group<float, float, W>(e, prec, acc64, b, y, cnt);
+  else if (acc64) run_group<uint16_t, double, W>(e, prec, acc64, b, y, cnt);
+  else run_group<uint16_t, float, W>(e, prec, acc64, b, y, cnt);
+}
+
+}  // namespace
+
+int block_width(const Engine& e, int nrhs, int prec, bool transpose) {
+  if (transpose || prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
+  return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
+}
+
+void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64) {
+  const int Wmax = e.rhs_block == 16 ? 16 : 32;
+  ensure_block_scratch(e, nrhs <= 16 ? 16 : Wmax);
+  const int64_t N = e.N;
+  for (int r0 = 0; r0 < nrhs; r0 += Wmax) {
+    const int cnt = std::min(Wmax, nrhs - r0);
+    const double* b = d_b + (int64_t)r0 * N;
+    double* y = d_y + (int64_t)r0 * N;
+    if (cnt <= 16) run_group_w<16>(e, prec, acc64, b, y, cnt);
+    else run_group_w<32>(e, prec, acc64, b, y, cnt);
+  }
+}
+
+}  // namespace h3d

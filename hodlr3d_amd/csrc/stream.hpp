@@ -1,0 +1,66 @@
+// Device helpers shared by the factor-streaming kernels: one work item per
+// wave, global (non-flat) non-temporal loads of factor streams.
+#pragma once
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace h3d {
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+
+inline unsigned wave_grid(int64_t items) {
+  return (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
+}
+
+// Global index of the calling wave (one work item per wave), made visibly
+// wave-uniform so that the compiler loads the item and its tables with scalar
+// loads.
+__device__ inline int wave_id() {
+  return __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+}
+
+// Factor pointers come from per-chunk tables, so the compiler cannot prove
+// they are global memory; loading through an address-space-1 pointer makes
+// them global (not flat) loads, which do not share the LDS wait counter.
+template <class T>
+__device__ inline T ntload(const void* p) {
+  return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)p);
+}
+
+// Zero fill (used instead of a memset node, so that a matvec may be captured
+// into a HIP graph).
+__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// 8- and 16-byte vectors that need only 4-byte alignment
+typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x2a __attribute__((ext_vector_type(2), aligned(4)));
+
+template <class S>
+struct Load;
+template <>
+struct Load<double> {
+  __device__ static inline double f(const double* p) { return ntload<double>(p); }
+};
+template <>
+struct Load<float> {
+  __device__ static inline float f(const float* p) { return ntload<float>(p); }
+};
+template <>
+struct Load<uint16_t> {
+  __device__ static inline float f(const uint16_t* p) { return bf16_to_f(ntload<uint16_t>(p)); }
+};
+template <>
+struct Load<_Float16> {
+  __device__ static inline float f(const _Float16* p) {
+    return (float)__builtin_bit_cast(_Float16, ntload<uint16_t>(p));
+  }
+};
+
+}  // namespace
+}  // namespace h3d

@@ -267,6 +267,11 @@ class HODLR3D:
         """Pin the per-lane load widths of the two far-field kernels for `prec`."""
         nat.check(nat.lib().h3d_set_tuning(self._h, _prec(prec), int(p1_bytes), int(p2_bytes)))
 
+    def set_rhs_block(self, width):
+        """Group width (16 or 32; 0: one at a time) of products with several
+        right-hand sides on the ACA factor copies (matrix-core path)."""
+        nat.check(nat.lib().h3d_set_rhs_block(self._h, int(width)))
+
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
         ms = (ctypes.c_double * 9)()

@@ -178,6 +178,14 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag);
  * receives phase-1 then phase-2 averages per width, -1 where a width does not
  * apply: 8 values, or 16 for H3D_PREC_MIXED (fp32 class, then fp16 class). */
 int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
+
+/* Products with several right-hand sides (h3d_matvec with nrhs >= 2 on an
+ * ACA factor copy) take them `width` at a time (16 or 32; 0: one at a time)
+ * and read each factor element once per group, on the matrix cores (default
+ * 32).  No reference counterpart: MatVecProduct (src/HODLR3D.cpp:21-27) takes
+ * one vector.  Transposes and the mixed copy always go one RHS at a time. */
+int h3d_set_rhs_block(h3d_handle* h, int width);
+
 /* Pins the per-lane load widths (bytes, >= the element size; >= 4 and for
  * both classes with H3D_PREC_MIXED) for one mode. */
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);

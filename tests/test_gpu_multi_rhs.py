@@ -1,0 +1,107 @@
+"""Products with several right-hand sides (the grouped matrix-core path):
+every column equals the single-RHS product of the same mode within that
+mode's rounding, for group widths 16 and 32, ragged counts, host and device
+buffers; a non-finite column stays in its own column (GPU).
+
+fp64 and the fp64-accumulation modes sum in fp64 on both paths (different
+order): 1e-13.  fp32/bf16 with fp32 sums: 1e-5 relative (fp32 rounding of
+~1e3-term sums in another order)."""
+import numpy as np
+import pytest
+
+from tests.conftest import uniform
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"fp64": 1e-13, "fp32/acc64": 1e-13, "bf16/acc64": 1e-13, "fp32": 1e-5, "bf16": 1e-5}
+
+
+@pytest.fixture(scope="module")
+def handle():
+    import hodlr3d_amd as H3
+    x = uniform(24000, 51)
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    print(f"[multi-rhs] N={len(x)} levels={h.nLevels}", flush=True)
+    yield x, h
+    h.close()
+
+
+def _single(h, Q, prec):
+    return np.stack([h.matvec(Q[:, j], prec=prec) for j in range(Q.shape[1])], axis=1)
+
+
+def _relcols(A, B):
+    return max(np.linalg.norm(A[:, j] - B[:, j]) / np.linalg.norm(B[:, j]) for j in range(B.shape[1]))
+
+
+@pytest.mark.parametrize("prec", list(MODES))
+def test_grouped_columns_equal_single_products(handle, prec):
+    x, h = handle
+    tol = MODES[prec]
+    Q = np.random.default_rng(1).standard_normal((len(x), 40))
+    ref = _single(h, Q, prec)
+    h.set_rhs_block(32)
+    for n in (2, 5, 16, 17, 40):
+        print(f"[{prec}] nrhs={n}", flush=True)
+        Y = h.matvec(Q[:, :n], prec=prec)
+        assert _relcols(Y, ref[:, :n]) <= tol, (prec, n)
+    h.set_rhs_block(16)
+    Y = h.matvec(Q[:, :40], prec=prec)
+    assert _relcols(Y, ref) <= tol, (prec, "width 16")
+    h.set_rhs_block(32)
+
+
+def test_width_zero_is_the_single_path(handle):
+    x, h = handle
+    Q = np.random.default_rng(2).standard_normal((len(x), 3))
+    h.set_rhs_block(0)
+    Y = h.matvec(Q)
+    h.set_rhs_block(32)
+    assert np.array_equal(Y, _single(h, Q, "fp64"))
+
+
+def test_device_buffers_and_direct_rows(handle):
+    import torch
+    x, h = handle
+    N = len(x)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    Qt = torch.randn(N, 20, dtype=torch.float64, device="cuda", generator=gen)
+    Yt = h.matvec(Qt)
+    Yh = h.matvec(Qt.cpu().numpy())
+    assert np.linalg.norm(Yt.cpu().numpy() - Yh) <= 1e-15 * np.linalg.norm(Yh)
+    rows = np.arange(0, N, 97)
+    for j in (0, 19):
+        d = h.direct(Qt[:, j].cpu().numpy(), rows)
+        assert np.linalg.norm(Yh[rows, j] - d) <= 1e-6 * np.linalg.norm(d), j
+
+
+def test_nonfinite_column_stays_in_its_column(handle):
+    x, h = handle
+    Q = np.random.default_rng(4).standard_normal((len(x), 6))
+    Q[123, 2] = np.nan
+    Y = h.matvec(Q)
+    assert np.isnan(Y[:, 2]).any()
+    good = [0, 1, 3, 4, 5]
+    assert np.isfinite(Y[:, good]).all()
+    assert _relcols(Y[:, good], _single(h, Q[:, good], "fp64")) <= 1e-13
+
+
+def test_transpose_and_mixed_stay_per_column(handle):
+    x, h = handle
+    Q = np.random.default_rng(5).standard_normal((len(x), 3))
+    YT = h.matvec(Q, transpose=True)
+    for j in range(3):
+        assert np.array_equal(YT[:, j], h.matvec(Q[:, j], transpose=True))
+    h.compress(1e-6)
+    YM = h.matvec(Q, prec="mixed")
+    for j in range(3):
+        assert np.array_equal(YM[:, j], h.matvec(Q[:, j], prec="mixed"))
+    h.release_precision("mixed")
+
+
+def test_rejects_bad_width(handle):
+    from hodlr3d_amd import _native as nat
+    x, h = handle
+    with pytest.raises(nat.H3DError):
+        h.set_rhs_block(8)
