@@ -63,6 +63,11 @@ void Engine::free_all() {
   dfree(d_partw);
   dfree(d_ylevw);
   dfree(d_iow);
+  dfree(d_rcols);
+  dfree(d_gitems);
+  dfree(d_pbw);
+  dfree(d_gblk);
+  n_rcols = n_gitems = g_np = 0;
   w_cap = io_w = 0;
   for (auto& ev : this->ev)
     if (ev) (void)hipEventDestroy(ev);

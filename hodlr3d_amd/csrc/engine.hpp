@@ -57,6 +57,15 @@ struct P1Item {
   int32_t pad;
 };
 
+// Phase-1 item of the multi-RHS path: rows [s0, s1) of one source box (at
+// position src) against the blocks by_src[q0 .. q1), which all have it as source.
+struct GItem {
+  int32_t q0, q1;
+  int32_t src;
+  int32_t s0, s1;
+  int32_t pad;
+};
+
 // Partial-sum reduction: nsplit row ranges x nc columns at part[pbase + q*nc + l]
 // summed in range order into t[toff + l].
 struct P1Reduce {
@@ -295,6 +304,13 @@ struct Engine {
   double* d_ylevw = nullptr;        // per-level y by position, W per point
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
+  void* d_rcols = nullptr;          // per column of a row-split block: int4 reduce entry
+  int64_t n_rcols = 0;
+  void* d_gitems = nullptr;         // phase-1 items grouped by source box (GItem)
+  int64_t n_gitems = 0;
+  void* d_pbw = nullptr;            // per block: first partial slot, or -1
+  void* d_gblk = nullptr;           // blocks of rank >= 1 grouped by source box
+  int64_t g_np = 0;                 // partial entries per RHS
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;

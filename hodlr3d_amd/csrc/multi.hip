@@ -28,6 +28,9 @@
 namespace h3d {
 namespace {
 
+constexpr int kSColumns = 256;  // columns of one phase-1 item
+constexpr int kGRows = 256;     // rows of one phase-1 item (x staged in LDS)
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -112,110 +115,194 @@ __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __r
   xr[p * W + j] = v;
 }
 
-// Phase 1: for each block of the item, T_b(c, j) = sum over rows p in
-// [s0, s1) of V_b(c, p) x(src + p, j), 64 columns (four 16-column tiles) per
-// pass.  Rows are walked in steps of 16 aligned to absolute position: lane
-// (c = l % 16, q = l / 16) takes rows P + 4q .. P + 4q + 3 of column c; rows
-// outside the item's range contribute 0 on both operands.
+// Phase 1, grouped by source box: one workgroup per item = a row range
+// [s0, s1) (at most kGRows rows) of one source box and a slice of the blocks
+// with that source (at most kSColumns columns).  x of the range (as quads,
+// rows outside the range zeroed) is staged once in LDS and shared by the four
+// waves, which take the slice's blocks in turn: T_b(c, j) = sum over the range
+// of V_b(c, p) x(p, j), 32 columns (two 16-column tiles) per pass, UF 16-row
+// steps per unit of work (gblk lists the blocks of rank >= 1 by source box).
+// In step s lane (c = l % 16, q = l / 16) takes rows
+// P + 16 s + 4q .. + 3 of column c.  The factor loads of the next unit (next
+// steps, next pass or next block) are issued before the MFMAs of the current
+// one (two register buffers).  A block whose source has more than kGRows rows
+// writes partial sums (slot pbw[b] + range * r + c), summed in range order
+// afterwards.
 template <class S, class A, int W>
-__global__ __launch_bounds__(256) void p1w_kernel(const P1Item* __restrict__ items, int nitems,
-                                                  void* const* __restrict__ vbase,
+__global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ items, int nitems,
+                                                  const int32_t* __restrict__ gblk,
                                                   const BlockDev* __restrict__ blk,
+                                                  const int32_t* __restrict__ btoff,
+                                                  const int32_t* __restrict__ pbw,
+                                                  void* const* __restrict__ vbase,
                                                   const A* __restrict__ xq, A* __restrict__ tw,
                                                   A* __restrict__ partw) {
   using M = Mfma<A>;
   using Acc = typename M::acc;
   constexpr int H = W / 16;
-  constexpr int CT = 4;
-  const int w = wave_id();
-  if (w >= nitems) return;
+  constexpr int CT = 2;
+  constexpr int UF = sizeof(A) == 8 ? 2 : 4;
+  constexpr int QMAX = 4 * ((kGRows + 15) / 16);  // quads of the steps of one range
+  __shared__ A xl[QMAX * W * 4];
+  const int item = blockIdx.x;
+  if (item >= nitems) return;  // whole workgroup
+  const GItem it = items[item];
+  const int a0 = it.src + it.s0;
+  const int nrows = it.s1 - it.s0;
+  const int nsteps = (nrows + 15) >> 4;
+  {
+    // x of range row i (position a0 + i) at xl[((i / 4) W + j) 4 + i % 4]; zero past the range
+    const int ne = 16 * nsteps * W;
+    for (int e = threadIdx.x; e < ne; e += 256) {
+      const int i = (e / (4 * W)) * 4 + (e & 3), j = (e >> 2) % W;
+      const int p = a0 + i;
+      xl[e] = i < nrows ? xq[((int64_t)(p >> 2) * W + j) * 4 + (p & 3)] : (A)0;
+    }
+  }
+  __syncthreads();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int lc = lane & 15, kq = lane >> 4;
-  const P1Item it = items[w];
-  const S* V = (const S*)vbase[it.chunk];
-  for (int b = it.b0; b < it.b1; ++b) {
+  const int range = it.s0 / kGRows;
+
+  // unit of work: block by_src[q], column pass cb, steps [s0, s0 + UF)
+  struct Unit {
+    int q, r, cb, s0;
+    const S* V;
+    int64_t ldv;
+    A* out;
+  };
+  auto open_block = [&](Unit& u) -> bool {  // first unit of block gblk[q] (rank >= 1)
+    if (u.q >= it.q1) return false;
+    const int b = gblk[u.q];
     const BlockDev bd = blk[b];
-    const int s1 = min(it.s1, bd.m);
-    if (bd.r == 0 || s1 <= it.s0) continue;
-    const int64_t ldv = bd.m + (bd.m & 1);
-    const int p0 = bd.src + it.s0, pe = bd.src + s1;
-    for (int cb = 0; cb < bd.r; cb += 16 * CT) {
-      Acc acc[CT][H];
+    u.r = bd.r;
+    u.cb = 0;
+    u.s0 = 0;
+    u.V = (const S*)vbase[bd.chunk] + bd.voff;
+    u.ldv = bd.m + (bd.m & 1);
+    const int pb = pbw[b];
+    u.out = pb < 0 ? tw + (int64_t)btoff[b] * W
+                   : partw + ((int64_t)pb + (int64_t)range * bd.r) * W;
+    return true;
+  };
+  auto advance = [&](Unit& u) -> bool {
+    u.s0 += UF;
+    if (u.s0 < nsteps) return true;
+    u.s0 = 0;
+    u.cb += 16 * CT;
+    if (u.cb < u.r) return true;
+    u.q += kWavesPerBlock;
+    return open_block(u);
+  };
+  // Branch-free loads: one 4-row vector per lane and tile at an in-bounds
+  // address (columns clamped to r - 1; a vector starting inside the range ends
+  // at most 3 rows past it, in the column padding, the next column or the
+  // chunk's zero tail), then masked by selects.
+  auto load = [&](const Unit& u, A (&av)[UF][CT][4]) {
+    const bool two = u.cb + 16 < u.r;
+    (void)two;
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-        for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
-      for (int P = p0 & ~3; P < pe; P += 16) {
-        const int pr = P + 4 * kq;
-        bool ok[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) ok[u] = pr + u >= p0 && pr + u < pe;
-        const bool full = ok[0] && ok[3];
-        A xb[H][4];
-#pragma unroll
-        for (int h = 0; h < H; ++h) {
-          load_quad<A>(xq + ((int64_t)(pr >> 2) * W + 16 * h + lc) * 4, xb[h]);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) xb[h][u] = ok[u] ? xb[h][u] : (A)0;
-        }
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          if (cb + 16 * ct >= bd.r) break;  // wave-uniform
-          const int c = cb + 16 * ct + lc;
-          A av[4] = {0, 0, 0, 0};
-          if (c < bd.r) {
-            const S* col = V + bd.voff + c * ldv + (pr - bd.src);
-            if (full) {
-              load4<S, A>(col, av);
-            } else {
-#pragma unroll
-              for (int u = 0; u < 4; ++u)
-                if (ok[u]) av[u] = (A)Load<S>::f(col + u);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[u], xb[h][u], acc[ct][h]);
-        }
-      }
+    for (int st = 0; st < UF; ++st) {
+      const int i = 16 * (u.s0 + st) + 4 * kq;
+      const int il = i < nrows ? i : 0;
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        if (cb + 16 * ct >= bd.r) break;
+        const int c = u.cb + 16 * ct + lc;
+        const int cl = c < u.r ? c : u.r - 1;
+        A v[4];
+        load4<S, A>(u.V + cl * u.ldv + it.s0 + il, v);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int c = cb + 16 * ct + M::row(kq, a);
-          if (c >= bd.r) continue;
-          A* out = it.pslot < 0 ? tw + (int64_t)(it.toff + bd.col + c) * W
-                                : partw + ((int64_t)it.pslot + c) * W;
-#pragma unroll
-          for (int h = 0; h < H; ++h) out[16 * h + lc] = acc[ct][h][a];
-        }
+        for (int e = 0; e < 4; ++e) av[st][ct][e] = (c < u.r && i + e < nrows) ? v[e] : (A)0;
       }
     }
+  };
+  auto compute = [&](const Unit& u, const A (&av)[UF][CT][4], Acc (&acc)[CT][H]) {
+    const bool two = u.cb + 16 < u.r;
+#pragma unroll
+    for (int st = 0; st < UF; ++st) {
+      if (u.s0 + st >= nsteps) break;
+      A xb[H][4];
+#pragma unroll
+      for (int h = 0; h < H; ++h)
+        load_quad<A>(xl + ((4 * (u.s0 + st) + kq) * W + 16 * h + lc) * 4, xb[h]);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          if (ct == 1 && !two) break;
+#pragma unroll
+          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
+        }
+    }
+  };
+  auto store = [&](const Unit& u, Acc (&acc)[CT][H]) {
+    const bool two = u.cb + 16 < u.r;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      if (ct == 0 || two) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int c = u.cb + 16 * ct + M::row(kq, a);
+          if (c < u.r) {
+#pragma unroll
+            for (int h = 0; h < H; ++h) u.out[(int64_t)c * W + 16 * h + lc] = acc[ct][h][a];
+          }
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
+    }
+  };
+
+  Unit cur;
+  cur.q = it.q0 + wv;
+  if (!open_block(cur)) return;
+  A av0[UF][CT][4], av1[UF][CT][4];
+  Acc acc[CT][H];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
+  load(cur, av0);
+  for (;;) {
+    Unit nxt = cur;
+    bool more = advance(nxt);
+    if (more) load(nxt, av1);
+    compute(cur, av0, acc);
+    if (!more || nxt.q != cur.q || nxt.cb != cur.cb) store(cur, acc);
+    if (!more) break;
+    cur = nxt;
+    nxt = cur;
+    more = advance(nxt);
+    if (more) load(nxt, av0);
+    compute(cur, av1, acc);
+    if (!more || nxt.q != cur.q || nxt.cb != cur.cb) store(cur, acc);
+    if (!more) break;
+    cur = nxt;
   }
 }
 
-// Row-split partial sums of one block, summed in range order.
+// Row-split partial sums, one thread per (column, RHS): column q of the table
+// holds {first partial entry, number of ranges, entries per range, t index};
+// the ranges are summed in order.
 template <class A, int W>
-__global__ void reduce_w_kernel(const P1Reduce* __restrict__ rs, int n, const A* __restrict__ partw,
-                                A* __restrict__ tw) {
-  const int w = wave_id();
-  if (w >= n) return;
-  const int lane = threadIdx.x & 63;
-  const P1Reduce r = rs[w];
-  for (int e = lane; e < r.nc * W; e += 64) {
-    const int l = e / W, j = e % W;
-    A acc = 0;
-    for (int q = 0; q < r.nsplit; ++q)
-      acc += partw[((int64_t)r.pbase + (int64_t)q * r.nc + l) * W + j];
-    tw[(int64_t)(r.toff + l) * W + j] = acc;
-  }
+__global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
+                                const A* __restrict__ partw, A* __restrict__ tw) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= ncols * W) return;
+  const int j = (int)(id % W);
+  const int4 c = cols[id / W];
+  A acc = 0;
+  for (int q = 0; q < c.y; ++q) acc += partw[((int64_t)c.x + (int64_t)q * c.z) * W + j];
+  tw[(int64_t)c.w * W + j] = acc;
 }
 
 // Phase 2: y(i, j) = sum_c U_k(i, c) T_k(c, j) for <= 64 rows at a time; lane
-// (g = l % 16, q = l / 16) takes rows i + 4g .. i + 4g + 3 of column c + q.
-// The u-th MFMA of a step produces rows i + 4g' + u (g' the MFMA output row).
+// (g = l % 16, q = l / 16) takes rows i + 4g .. i + 4g + 3 of column c + q,
+// UF 4-column steps per unit, the next unit's loads issued before the MFMAs
+// of the current one.  The u-th MFMA of a step produces rows i + 4g' + u (g'
+// the MFMA output row).
 template <class S, class A, int W>
 __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
                                                   void* const* __restrict__ ubase,
@@ -224,6 +311,7 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
   using M = Mfma<A>;
   using Acc = typename M::acc;
   constexpr int H = W / 16;
+  constexpr int UF = sizeof(A) == 8 ? 2 : 4;
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -237,34 +325,50 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
     bool ok[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) ok[u] = i + u < lim;
+    // branch-free loads at in-bounds addresses (column clamped to R - 1, rows
+    // of lanes past the chunk moved to its first row; a vector starting inside
+    // the box ends at most 3 rows past it), masked by selects
+    const int il = ok[0] ? i : it.i0 + r0;
+    auto load = [&](int c0, A (&av)[UF][4], A (&tb)[UF][H]) {
+#pragma unroll
+      for (int st = 0; st < UF; ++st) {
+        const int c = c0 + 4 * st + kq;
+        const int cl = c < it.R ? c : it.R - 1;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          const A v = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
+          tb[st][h] = c < it.R ? v : (A)0;
+        }
+        A v[4];
+        load4<S, A>(U + (int64_t)cl * it.ld + il, v);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[st][u] = (c < it.R && ok[u]) ? v[u] : (A)0;
+      }
+    };
     Acc acc[4][H];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int h = 0; h < H; ++h) acc[u][h] = Acc{};
-    for (int c0 = 0; c0 < it.R; c0 += 4) {
-      const int c = c0 + kq;
-      A tb[H];
-      A av[4] = {0, 0, 0, 0};
-      if (c < it.R) {
+    auto compute = [&](int c0, const A (&av)[UF][4], const A (&tb)[UF][H]) {
 #pragma unroll
-        for (int h = 0; h < H; ++h) tb[h] = tw[(int64_t)(it.toff + c) * W + 16 * h + lc];
-        const S* col = U + (int64_t)c * it.ld + i;
-        if (ok[3]) {
-          load4<S, A>(col, av);
-        } else {
+      for (int st = 0; st < UF; ++st) {
+        if (c0 + 4 * st >= it.R) break;  // wave-uniform
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (ok[u]) av[u] = (A)Load<S>::f(col + u);
-        }
-      } else {
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int h = 0; h < H; ++h) tb[h] = 0;
+          for (int h = 0; h < H; ++h) acc[u][h] = M::mac(av[st][u], tb[st][h], acc[u][h]);
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int h = 0; h < H; ++h) acc[u][h] = M::mac(av[u], tb[h], acc[u][h]);
+    };
+    constexpr int STEP = 4 * UF;
+    A av0[UF][4], av1[UF][4], tb0[UF][H], tb1[UF][H];
+    if (it.R > 0) load(0, av0, tb0);
+    for (int c0 = 0; c0 < it.R; c0 += 2 * STEP) {
+      if (c0 + STEP < it.R) load(c0 + STEP, av1, tb1);
+      compute(c0, av0, tb0);
+      if (c0 + STEP >= it.R) break;
+      if (c0 + 2 * STEP < it.R) load(c0 + 2 * STEP, av0, tb0);
+      compute(c0 + STEP, av1, tb1);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -398,8 +502,72 @@ __global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict
 
 inline int64_t padded_points(int64_t N) { return ((N + 3) & ~int64_t(3)) + 16; }
 
+// Phase-1 items grouped by source box, the partial-sum slot of every block
+// whose source has more than kGRows rows (-1 otherwise), and one reduce entry
+// per column of those blocks {first slot, ranges, slot stride, t index}.
+void build_group_tables(Engine& e) {
+  std::vector<GItem> items;
+  std::vector<int32_t> gblk;  // blocks of rank >= 1, grouped by source box
+  std::vector<int32_t> pbw(std::max<int64_t>(e.nblk_all, 1), -1);
+  std::vector<int4> cols;
+  std::vector<int32_t> btoff(std::max<int64_t>(e.nblk_all, 1), 0);
+  if (e.nblk_all)
+    H3D_HIP(hipMemcpy(btoff.data(), e.d_btoff, sizeof(int32_t) * e.nblk_all, hipMemcpyDeviceToHost));
+  int64_t np = 0;
+  for (int j = 1; j <= e.nL; ++j) {
+    const Level& lv = e.lev[j];
+    const int64_t lo = level_offset(j);
+    std::vector<int64_t> first(lv.nboxes + 1, 0);  // by-source offsets within the level
+    for (int32_t k : lv.ilidx) ++first[k + 1];
+    for (int64_t k = 0; k < lv.nboxes; ++k) first[k + 1] += first[k];
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int m = e.bcount[lo + k];
+      const int64_t qb = lv.bbase + first[k], qe = lv.bbase + first[k + 1];
+      if (m == 0 || qb == qe) continue;
+      const int nsplit = (m + kGRows - 1) / kGRows;
+      if (nsplit > 1)
+        for (int64_t q = qb; q < qe; ++q) {
+          const int32_t b = e.by_src[q];
+          const int r = lv.rank[b - lv.bbase];
+          pbw[b] = (int32_t)np;
+          for (int c = 0; c < r; ++c) cols.push_back(make_int4((int)np + c, nsplit, r, btoff[b] + c));
+          np += (int64_t)nsplit * r;
+          if (np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+        }
+      const int64_t g0 = (int64_t)gblk.size();
+      for (int64_t q = qb; q < qe; ++q)
+        if (lv.rank[e.by_src[q] - lv.bbase] > 0) gblk.push_back(e.by_src[q]);
+      const int64_t g1 = (int64_t)gblk.size();
+      for (int s0 = 0; s0 < m; s0 += kGRows) {
+        int64_t q = g0;
+        while (q < g1) {
+          GItem it{(int32_t)q, 0, e.bstart[lo + k], s0, std::min(m, s0 + kGRows), 0};
+          int ncol = 0;
+          while (q < g1) {
+            const int r = lv.rank[gblk[q] - lv.bbase];
+            if (ncol > 0 && ncol + r > kSColumns) break;
+            ncol += r;
+            ++q;
+          }
+          it.q1 = (int32_t)q;
+          items.push_back(it);
+        }
+      }
+    }
+  }
+  e.n_gitems = (int64_t)items.size();
+  e.d_gitems = upload(items, e.stream);
+  e.d_gblk = upload(gblk, e.stream);
+  e.d_pbw = upload(pbw, e.stream);
+  e.n_rcols = (int64_t)cols.size();
+  e.d_rcols = upload(cols, e.stream);
+  e.g_np = np;
+  H3D_HIP(hipStreamSynchronize(e.stream));
+}
+
 void ensure_block_scratch(Engine& e, int W) {
-  const int64_t T = std::max<int64_t>(e.T, 1), np = std::max<int64_t>(e.plan.npartial, 1);
+  if (!e.d_gitems) build_group_tables(e);
+  const int64_t T = std::max<int64_t>(e.T, 1), np = std::max<int64_t>(e.g_np, 1);
   if (e.w_cap >= W && e.w_t >= T && e.w_np >= np) return;
   W = std::max(W, e.w_cap);
   dfree(e.d_xw);
@@ -433,14 +601,15 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     H3D_LAUNCH_CHECK();
   }
   const Plan& p = e.plan;
-  if (!p.p1.empty()) {
-    p1w_kernel<S, A, W><<<wave_grid(p.p1.size()), 256, 0, st>>>(
-        p.d_p1, (int)p.p1.size(), e.d_vbase[prec], e.d_blk, xq, tw, partw);
+  if (e.n_gitems) {
+    p1g_kernel<S, A, W><<<(unsigned)e.n_gitems, 256, 0, st>>>(
+        (const GItem*)e.d_gitems, (int)e.n_gitems, (const int32_t*)e.d_gblk, e.d_blk, e.d_btoff,
+        (const int32_t*)e.d_pbw, e.d_vbase[prec], xq, tw, partw);
     H3D_LAUNCH_CHECK();
   }
-  if (!p.p1r.empty()) {
-    reduce_w_kernel<A, W><<<wave_grid(p.p1r.size()), 256, 0, st>>>(p.d_p1r, (int)p.p1r.size(),
-                                                                   partw, tw);
+  if (e.n_rcols) {
+    reduce_w_kernel<A, W><<<(unsigned)((e.n_rcols * W + 255) / 256), 256, 0, st>>>(
+        (const int4*)e.d_rcols, e.n_rcols, partw, tw);
     H3D_LAUNCH_CHECK();
   }
   if (!p.p2.empty()) {

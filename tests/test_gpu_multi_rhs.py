@@ -105,3 +105,21 @@ def test_rejects_bad_width(handle):
     x, h = handle
     with pytest.raises(nat.H3DError):
         h.set_rhs_block(8)
+
+
+def test_clustered_leaves_with_many_rows():
+    """A leaf with ~700 points (several source windows, and row-split items in
+    the single-RHS plan) next to ordinary leaves."""
+    import hodlr3d_amd as H3
+    rng = np.random.default_rng(7)
+    base = rng.uniform(-1, 1, (6000, 3))
+    ball = np.array([0.31, -0.42, 0.55]) + rng.uniform(-0.04, 0.04, (700, 3))
+    x = np.vstack([base, ball])
+    h = H3.HODLR3D(3, 1.0, 9, 7, x)
+    h.assemble()
+    Q = np.random.default_rng(8).standard_normal((len(x), 20))
+    for prec, tol in (("fp64", 1e-13), ("fp32", 1e-5)):
+        ref = _single(h, Q, prec)
+        Y = h.matvec(Q, prec=prec)
+        assert _relcols(Y, ref) <= tol, prec
+    h.close()
