@@ -14,6 +14,12 @@ Workloads (synthetic, seeded):
       iterative refinement, b ~ N(0,1)
   C2  N = 262,144 iid uniform in [-1,1]^3, leaf1d 4 (nLevels 4), tol 1e-8
   C1  16^3 lattice of the reference driver, leaf1d 4 (nLevels 2), tol 1e-10
+
+After the headline (and with --only-multi instead of it) the products with
+several right-hand sides are measured on N = 262,144 iid uniform points,
+leaf1d 4, ACA tol 1e-8, fp32 factors, nrhs = --multi-rhs (default 32) iid
+N(0,1) columns: the grouped matrix-core path against one product per column
+("multi_rhs" in the JSON line).
 """
 import argparse
 import json
@@ -200,6 +206,46 @@ def cpu_baseline(wl, q, budget_s):
             "wall_s": round(time.perf_counter() - t0, 1)}
 
 
+def multi_rhs(torch, H3, nrhs, steps, warmup):
+    """Grouped products of nrhs columns (fp32 factors) vs one product per column."""
+    dev = torch.device("cuda:0")
+    N = 262144
+    pts = np.random.default_rng(11).uniform(-1.0, 1.0, (N, 3))
+    t0 = time.perf_counter()
+    h = H3.HODLR3D(4, 1.0, 8, 7, pts)
+    h.assemble()
+    t_asm = time.perf_counter() - t0
+    h.move_precision("fp32")
+    st = h.stats()
+    gen = torch.Generator(device="cuda").manual_seed(12)
+    Q = torch.randn(N, nrhs, dtype=torch.float64, device=dev, generator=gen)
+    h.set_rhs_block(0)
+    y1 = h.matvec(Q, prec="fp32")
+    single_ms = timed(torch, lambda: h.matvec(Q, prec="fp32"), 1, 0)
+    h.set_rhs_block(32)
+    y = h.matvec(Q, prec="fp32")
+    ms = timed(torch, lambda: h.matvec(Q, prec="fp32"), steps, warmup)
+    diff = float((torch.linalg.norm(y - y1, dim=0) / torch.linalg.norm(y1, dim=0)).max())
+    rows = np.sort(np.random.default_rng(13).choice(N, size=1024, replace=False))
+    errs = []
+    for j in (0, nrhs - 1):
+        d = h.direct(Q[:, j], rows).cpu().numpy()
+        errs.append(float(np.linalg.norm(y[:, j].cpu().numpy()[rows] - d) / np.linalg.norm(d)))
+    fbytes = st["factor_elems"] * 4
+    h.close()
+    out = {"workload": "N=262144 iid uniform in [-1,1]^3, leaf1d 4 (nLevels 4), K=1/r, "
+                       "ACA tol 1e-8, fp32 factors, fp32 accumulation",
+           "nrhs": nrhs, "group_width": 32, "ms": round(ms, 4),
+           "ms_per_rhs": round(ms / nrhs, 4), "one_at_a_time_ms": round(single_ms, 3),
+           "speedup": round(single_ms / ms, 2),
+           "factor_GB": round(fbytes / 1e9, 2),
+           "factor_GB_per_s": round(fbytes * math.ceil(nrhs / 32) / (ms * 1e-3) / 1e9, 1),
+           "rel_diff_vs_one_at_a_time": diff, "rel_err_vs_direct": max(errs),
+           "err_rows": len(rows), "assemble_s": round(t_asm, 2)}
+    log(f"[bench] multi-rhs {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +266,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--multi-rhs", type=int, default=32, metavar="NRHS",
+                    help="also time products with NRHS right-hand sides (0: skip)")
+    ap.add_argument("--only-multi", action="store_true",
+                    help="only the multi-RHS measurement (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -231,6 +281,15 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
+    if args.only_multi:
+        mr = multi_rhs(torch, H3, max(args.multi_rhs, 2), args.steps, args.warmup)
+        print(json.dumps({"metric": f"matvec ms with {mr['nrhs']} right-hand sides",
+                          "value": mr["ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": mr["ms"],
+                          "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": mr["workload"]}, "multi_rhs": mr}), flush=True)
+        return
     wl = workload(args.config)
     N = len(wl["pts"])
     log(f"[bench] {wl['name']}: {wl['desc']}")
@@ -402,6 +461,10 @@ def main():
         "cpu_baseline": cpu,
         "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
     }
+    if args.multi_rhs > 0:
+        H.close()  # frees the headline factors first
+        torch.cuda.empty_cache()
+        line["multi_rhs"] = multi_rhs(torch, H3, args.multi_rhs, min(args.steps, 5), 1)
     print(json.dumps(line), flush=True)
 
 
