@@ -65,7 +65,6 @@ void Engine::free_all() {
   dfree(d_iow);
   dfree(d_rcols);
   dfree(d_gitems);
-  dfree(d_pbw);
   dfree(d_gblk);
   n_rcols = n_gitems = g_np = 0;
   w_cap = io_w = 0;

@@ -66,6 +66,13 @@ struct GItem {
   int32_t pad;
 };
 
+// Block descriptor of the multi-RHS phase 1 (one scalar load): V offset and
+// chunk, rank, rows, t index of column 0, first partial slot (-1: none).
+struct GDesc {
+  int64_t voff;
+  int32_t chunk, r, m, tb, pb, pad;
+};
+
 // Partial-sum reduction: nsplit row ranges x nc columns at part[pbase + q*nc + l]
 // summed in range order into t[toff + l].
 struct P1Reduce {
@@ -308,8 +315,7 @@ struct Engine {
   int64_t n_rcols = 0;
   void* d_gitems = nullptr;         // phase-1 items grouped by source box (GItem)
   int64_t n_gitems = 0;
-  void* d_pbw = nullptr;            // per block: first partial slot, or -1
-  void* d_gblk = nullptr;           // blocks of rank >= 1 grouped by source box
+  void* d_gblk = nullptr;           // GDesc of the blocks of rank >= 1, by source box
   int64_t g_np = 0;                 // partial entries per RHS
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b

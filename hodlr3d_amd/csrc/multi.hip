@@ -28,7 +28,7 @@
 namespace h3d {
 namespace {
 
-constexpr int kSColumns = 256;  // columns of one phase-1 item
+constexpr int kSColumns = 1024;  // columns of one phase-1 item
 constexpr int kGRows = 256;     // rows of one phase-1 item (x staged in LDS)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -121,7 +121,7 @@ __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __r
 // rows outside the range zeroed) is staged once in LDS and shared by the four
 // waves, which take the slice's blocks in turn: T_b(c, j) = sum over the range
 // of V_b(c, p) x(p, j), 32 columns (two 16-column tiles) per pass, UF 16-row
-// steps per unit of work (gblk lists the blocks of rank >= 1 by source box).
+// steps per unit of work (gdesc describes the blocks of rank >= 1 by source box).
 // In step s lane (c = l % 16, q = l / 16) takes rows
 // P + 16 s + 4q .. + 3 of column c.  The factor loads of the next unit (next
 // steps, next pass or next block) are issued before the MFMAs of the current
@@ -130,10 +130,7 @@ __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __r
 // afterwards.
 template <class S, class A, int W>
 __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ items, int nitems,
-                                                  const int32_t* __restrict__ gblk,
-                                                  const BlockDev* __restrict__ blk,
-                                                  const int32_t* __restrict__ btoff,
-                                                  const int32_t* __restrict__ pbw,
+                                                  const GDesc* __restrict__ gdesc,
                                                   void* const* __restrict__ vbase,
                                                   const A* __restrict__ xq, A* __restrict__ tw,
                                                   A* __restrict__ partw) {
@@ -165,25 +162,26 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
   const int lc = lane & 15, kq = lane >> 4;
   const int range = it.s0 / kGRows;
 
-  // unit of work: block by_src[q], column pass cb, steps [s0, s0 + UF)
+  // unit of work: block gdesc[q], column pass cb, steps [s0, s0 + UF)
   struct Unit {
     int q, r, cb, s0;
     const S* V;
     int64_t ldv;
     A* out;
   };
-  auto open_block = [&](Unit& u) -> bool {  // first unit of block gblk[q] (rank >= 1)
+  // the wave's next block descriptor is loaded one block ahead
+  GDesc pre = it.q0 + wv < it.q1 ? gdesc[it.q0 + wv] : GDesc{};
+  auto open_block = [&](Unit& u) -> bool {  // first unit of block q (rank >= 1)
     if (u.q >= it.q1) return false;
-    const int b = gblk[u.q];
-    const BlockDev bd = blk[b];
-    u.r = bd.r;
+    const GDesc d = pre;
+    if (u.q + kWavesPerBlock < it.q1) pre = gdesc[u.q + kWavesPerBlock];
+    u.r = d.r;
     u.cb = 0;
     u.s0 = 0;
-    u.V = (const S*)vbase[bd.chunk] + bd.voff;
-    u.ldv = bd.m + (bd.m & 1);
-    const int pb = pbw[b];
-    u.out = pb < 0 ? tw + (int64_t)btoff[b] * W
-                   : partw + ((int64_t)pb + (int64_t)range * bd.r) * W;
+    u.V = (const S*)vbase[d.chunk] + d.voff;
+    u.ldv = d.m + (d.m & 1);
+    u.out = d.pb < 0 ? tw + (int64_t)d.tb * W
+                     : partw + ((int64_t)d.pb + (int64_t)range * d.r) * W;
     return true;
   };
   auto advance = [&](Unit& u) -> bool {
@@ -199,9 +197,9 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
   // address (columns clamped to r - 1; a vector starting inside the range ends
   // at most 3 rows past it, in the column padding, the next column or the
   // chunk's zero tail), then masked by selects.
+  // The raw vectors are masked only when they are used (compute), so that
+  // the wait for a unit's loads falls after the previous unit's MFMAs.
   auto load = [&](const Unit& u, A (&av)[UF][CT][4]) {
-    const bool two = u.cb + 16 < u.r;
-    (void)two;
 #pragma unroll
     for (int st = 0; st < UF; ++st) {
       const int i = 16 * (u.s0 + st) + 4 * kq;
@@ -210,10 +208,7 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
       for (int ct = 0; ct < CT; ++ct) {
         const int c = u.cb + 16 * ct + lc;
         const int cl = c < u.r ? c : u.r - 1;
-        A v[4];
-        load4<S, A>(u.V + cl * u.ldv + it.s0 + il, v);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) av[st][ct][e] = (c < u.r && i + e < nrows) ? v[e] : (A)0;
+        load4<S, A>(u.V + cl * u.ldv + it.s0 + il, av[st][ct]);
       }
     }
   };
@@ -222,6 +217,7 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
 #pragma unroll
     for (int st = 0; st < UF; ++st) {
       if (u.s0 + st >= nsteps) break;
+      const int i = 16 * (u.s0 + st) + 4 * kq;
       A xb[H][4];
 #pragma unroll
       for (int h = 0; h < H; ++h)
@@ -231,8 +227,10 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           if (ct == 1 && !two) break;
+          const bool ok = u.cb + 16 * ct + lc < u.r && i + e < nrows;
+          const A a = ok ? av[st][ct][e] : (A)0;
 #pragma unroll
-          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
+          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(a, xb[h][e], acc[ct][h]);
         }
     }
   };
@@ -329,20 +327,16 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
     // of lanes past the chunk moved to its first row; a vector starting inside
     // the box ends at most 3 rows past it), masked by selects
     const int il = ok[0] ? i : it.i0 + r0;
+    // raw values; masked where used, so the wait for them follows the
+    // previous unit's MFMAs
     auto load = [&](int c0, A (&av)[UF][4], A (&tb)[UF][H]) {
 #pragma unroll
       for (int st = 0; st < UF; ++st) {
         const int c = c0 + 4 * st + kq;
         const int cl = c < it.R ? c : it.R - 1;
 #pragma unroll
-        for (int h = 0; h < H; ++h) {
-          const A v = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
-          tb[st][h] = c < it.R ? v : (A)0;
-        }
-        A v[4];
-        load4<S, A>(U + (int64_t)cl * it.ld + il, v);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) av[st][u] = (c < it.R && ok[u]) ? v[u] : (A)0;
+        for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
+        load4<S, A>(U + (int64_t)cl * it.ld + il, av[st]);
       }
     };
     Acc acc[4][H];
@@ -354,10 +348,16 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
 #pragma unroll
       for (int st = 0; st < UF; ++st) {
         if (c0 + 4 * st >= it.R) break;  // wave-uniform
+        const bool cok = c0 + 4 * st + kq < it.R;
+        A bm[H];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int h = 0; h < H; ++h) bm[h] = cok ? tb[st][h] : (A)0;
 #pragma unroll
-          for (int h = 0; h < H; ++h) acc[u][h] = M::mac(av[st][u], tb[st][h], acc[u][h]);
+        for (int u = 0; u < 4; ++u) {
+          const A a = (cok && ok[u]) ? av[st][u] : (A)0;
+#pragma unroll
+          for (int h = 0; h < H; ++h) acc[u][h] = M::mac(a, bm[h], acc[u][h]);
+        }
       }
     };
     constexpr int STEP = 4 * UF;
@@ -555,10 +555,17 @@ void build_group_tables(Engine& e) {
       }
     }
   }
+  std::vector<BlockDev> blk(std::max<int64_t>(e.nblk_all, 1));
+  if (e.nblk_all)
+    H3D_HIP(hipMemcpy(blk.data(), e.d_blk, sizeof(BlockDev) * e.nblk_all, hipMemcpyDeviceToHost));
+  std::vector<GDesc> desc(gblk.size());
+  for (size_t q = 0; q < gblk.size(); ++q) {
+    const int32_t b = gblk[q];
+    desc[q] = GDesc{blk[b].voff, blk[b].chunk, blk[b].r, blk[b].m, btoff[b], pbw[b], 0};
+  }
   e.n_gitems = (int64_t)items.size();
   e.d_gitems = upload(items, e.stream);
-  e.d_gblk = upload(gblk, e.stream);
-  e.d_pbw = upload(pbw, e.stream);
+  e.d_gblk = upload(desc, e.stream);
   e.n_rcols = (int64_t)cols.size();
   e.d_rcols = upload(cols, e.stream);
   e.g_np = np;
@@ -603,8 +610,8 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
   const Plan& p = e.plan;
   if (e.n_gitems) {
     p1g_kernel<S, A, W><<<(unsigned)e.n_gitems, 256, 0, st>>>(
-        (const GItem*)e.d_gitems, (int)e.n_gitems, (const int32_t*)e.d_gblk, e.d_blk, e.d_btoff,
-        (const int32_t*)e.d_pbw, e.d_vbase[prec], xq, tw, partw);
+        (const GItem*)e.d_gitems, (int)e.n_gitems, (const GDesc*)e.d_gblk, e.d_vbase[prec], xq,
+        tw, partw);
     H3D_LAUNCH_CHECK();
   }
   if (e.n_rcols) {
