@@ -246,6 +246,55 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
     return out
 
 
+def batch_instances(torch, H3, count, nmin, nmax, steps, warmup):
+    """count independent instances (own random box each, fp32 factors): one
+    product of every instance replayed from one captured graph vs issued one
+    instance at a time."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(21)
+    Ns = np.exp(rng.uniform(math.log(nmin), math.log(nmax), count)).astype(int)
+    hs = []
+    t0 = time.perf_counter()
+    for n in Ns:
+        half = rng.uniform(0.25, 1.0)           # box side in [0.5, 2]
+        c = rng.uniform(-3.0, 3.0, 3)
+        pts = c + rng.uniform(-half, half, (int(n), 3))
+        h = H3.HODLR3D(4, half, 8, 7, pts, center=c, store=("fp32",))
+        h.assemble()
+        hs.append(h)
+    t_asm = time.perf_counter() - t0
+    gen = torch.Generator(device="cuda").manual_seed(22)
+    b = [torch.randn(h.N, dtype=torch.float64, device=dev, generator=gen) for h in hs]
+    y = [torch.empty(h.N, dtype=torch.float64, device=dev) for h in hs]
+
+    def one_at_a_time():
+        for h, bi, yi in zip(hs, b, y):
+            yi.copy_(h.matvec(bi, prec="fp32"))
+
+    one_at_a_time()
+    ref = [yi.clone() for yi in y]
+    eager_ms = timed(torch, one_at_a_time, steps, warmup)
+    g = H3.MatvecGraph(hs, b, y, prec="fp32")
+    g.launch()
+    torch.cuda.synchronize()
+    same = all(torch.equal(yi, ri) for yi, ri in zip(y, ref))
+    graph_ms = timed(torch, g.launch, steps, warmup)
+    fbytes = sum(h.stats()["factor_bytes"][1] for h in hs)
+    g.close()
+    for h in hs:
+        h.close()
+    out = {"workload": f"{count} instances, N log-uniform in [{nmin}, {nmax}] (total "
+                       f"{int(Ns.sum())}), each iid uniform in its own random box (side in "
+                       "[0.5, 2], centre in [-3, 3]^3), leaf1d 4, K=1/r, ACA tol 1e-8, fp32 factors",
+           "instances": count, "points": int(Ns.sum()), "graph_ms": round(graph_ms, 4),
+           "one_at_a_time_ms": round(eager_ms, 4), "speedup": round(eager_ms / graph_ms, 2),
+           "factor_GB": round(fbytes / 1e9, 2),
+           "factor_GB_per_s": round(fbytes / (graph_ms * 1e-3) / 1e9, 1),
+           "graph_equals_eager": same, "assemble_s": round(t_asm, 2)}
+    log(f"[bench] batch {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +319,12 @@ def main():
                     help="also time products with NRHS right-hand sides (0: skip)")
     ap.add_argument("--only-multi", action="store_true",
                     help="only the multi-RHS measurement (its own JSON line)")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="only the many-instance graph measurement (its own JSON line)")
+    ap.add_argument("--batch", type=int, default=64, help="instances of --only-batch")
+    ap.add_argument("--batch-n", default="8192,65536",
+                    help="range of N of --only-batch (log-uniform; the spec's 200000 upper end "
+                         "needs ~270 GB of fp32 factors for 64 instances)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -281,6 +336,16 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
+    if args.only_batch:
+        nmin, nmax = (int(v) for v in args.batch_n.split(","))
+        bt = batch_instances(torch, H3, args.batch, nmin, nmax, args.steps, args.warmup)
+        print(json.dumps({"metric": f"ms for one product of each of {bt['instances']} instances",
+                          "value": bt["graph_ms"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": bt["graph_ms"],
+                          "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": bt["workload"]}, "batch": bt}), flush=True)
+        return
     if args.only_multi:
         mr = multi_rhs(torch, H3, max(args.multi_rhs, 2), args.steps, args.warmup)
         print(json.dumps({"metric": f"matvec ms with {mr['nrhs']} right-hand sides",

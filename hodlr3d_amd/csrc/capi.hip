@@ -679,14 +679,35 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       h3d::prepare_mode(hs[i]->e, prec);  // copies and scratch exist before the capture
       H3D_HIP(hipStreamSynchronize(hs[i]->e.stream));
     }
+    // Instances go to kBranches streams forked from the capture stream and
+    // joined back, so the graph holds independent branches that can run
+    // concurrently (small instances alone leave most of the GPU idle).
+    constexpr int kBranches = 8;
+    const int nb = std::min(count, kBranches);
     hipStream_t cap = nullptr;
-    H3D_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    std::vector<hipStream_t> br(nb, nullptr);
+    std::vector<hipEvent_t> ev(nb + 1, nullptr);
+    auto release_streams = [&] {
+      for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+      for (hipStream_t x : br)
+        if (x) (void)hipStreamDestroy(x);
+      if (cap) (void)hipStreamDestroy(cap);
+    };
+    try {
+      H3D_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      for (int k = 0; k < nb; ++k) H3D_HIP(hipStreamCreateWithFlags(&br[k], hipStreamNonBlocking));
+      for (auto& x : ev) H3D_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    } catch (...) {
+      release_streams();
+      throw;
+    }
     std::vector<hipStream_t> saved(count);
     std::vector<bool> timing(count);
     for (int i = 0; i < count; ++i) {
       saved[i] = hs[i]->e.stream;
       timing[i] = hs[i]->e.timing;
-      hs[i]->e.stream = cap;
+      hs[i]->e.stream = br[i % nb];
       hs[i]->e.timing = false;
     }
     auto restore = [&] {
@@ -699,8 +720,14 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
     try {
       H3D_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       try {
+        H3D_HIP(hipEventRecord(ev[nb], cap));  // fork
+        for (int k = 0; k < nb; ++k) H3D_HIP(hipStreamWaitEvent(br[k], ev[nb], 0));
         for (int i = 0; i < count; ++i)
           h3d::matvec_device(hs[i]->e, b[i], y[i], 1, prec, transpose != 0);
+        for (int k = 0; k < nb; ++k) {  // join
+          H3D_HIP(hipEventRecord(ev[k], br[k]));
+          H3D_HIP(hipStreamWaitEvent(cap, ev[k], 0));
+        }
       } catch (...) {
         hipGraph_t dead = nullptr;
         (void)hipStreamEndCapture(cap, &dead);
@@ -714,11 +741,11 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
       delete g;
-      (void)hipStreamDestroy(cap);
+      release_streams();
       throw;
     }
     restore();
-    (void)hipStreamDestroy(cap);
+    release_streams();
     g->hs.assign(hs, hs + count);
     *out = g;
   });
