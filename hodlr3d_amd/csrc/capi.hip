@@ -702,14 +702,22 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       release_streams();
       throw;
     }
+    // every product of one handle goes to the same branch (a handle's scratch
+    // is shared by its products, so they must stay ordered)
+    std::vector<int> branch(count);
+    for (int i = 0, next = 0; i < count; ++i) {
+      branch[i] = -1;
+      for (int j = 0; j < i; ++j)
+        if (hs[j] == hs[i]) branch[i] = branch[j];
+      if (branch[i] < 0) branch[i] = next++ % nb;
+    }
     std::vector<hipStream_t> saved(count);
     std::vector<bool> timing(count);
     for (int i = 0; i < count; ++i) {
       saved[i] = hs[i]->e.stream;
       timing[i] = hs[i]->e.timing;
-      hs[i]->e.stream = br[i % nb];
-      hs[i]->e.timing = false;
     }
+    for (int i = 0; i < count; ++i) hs[i]->e.timing = false;
     auto restore = [&] {
       for (int i = 0; i < count; ++i) {
         hs[i]->e.stream = saved[i];
@@ -722,8 +730,10 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       try {
         H3D_HIP(hipEventRecord(ev[nb], cap));  // fork
         for (int k = 0; k < nb; ++k) H3D_HIP(hipStreamWaitEvent(br[k], ev[nb], 0));
-        for (int i = 0; i < count; ++i)
+        for (int i = 0; i < count; ++i) {
+          hs[i]->e.stream = br[branch[i]];
           h3d::matvec_device(hs[i]->e, b[i], y[i], 1, prec, transpose != 0);
+        }
         for (int k = 0; k < nb; ++k) {  // join
           H3D_HIP(hipEventRecord(ev[k], br[k]));
           H3D_HIP(hipStreamWaitEvent(cap, ev[k], 0));

@@ -142,7 +142,7 @@ struct Reader {
   }
   template <typename T>
   void vec(std::vector<T>& v, int64_t n) {
-    if (n < 0 || (int64_t)(sizeof(T) * (uint64_t)n) > left)
+    if (n < 0 || n > left / (int64_t)sizeof(T))
       throw Error(H3D_ERR_IO, path + ": truncated or corrupt");
     v.resize(n);
     raw(v.data(), sizeof(T) * n);
@@ -329,7 +329,9 @@ void load_engine(Engine& e, const char* path, int64_t ws_budget) {
     c.nU = r.val<int64_t>();
     c.nV = r.val<int64_t>();
     c.npiv = r.val<int64_t>();
-    if (c.nU < 0 || c.nV < 0 || c.npiv < 0 || (int64_t)((c.nU + c.nV) * es) > r.left)
+    const int64_t lim = r.left / (int64_t)es;
+    if (c.nU < 0 || c.nV < 0 || c.npiv < 0 || c.nU > lim || c.nV > lim - c.nU ||
+        c.npiv > r.left / 4)
       corrupt(r.path, "chunk size");
     e.chunks.push_back(c);  // owned by the engine from here on (freed on any error)
     Chunk& ch = e.chunks.back();

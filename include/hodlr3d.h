@@ -197,7 +197,8 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
  * matvecs since the last read; then the parts of phase 1 and phase 2 spent on
  * the second factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); then
  * the phase-1 kernel alone of the first and of the second copy.  *count gets
- * the number of matvecs; all are then reset. */
+ * the number of matvecs; all are then reset.  Grouped products of several
+ * right-hand sides (h3d_set_rhs_block) are not timed. */
 int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count);
 
 /* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,
@@ -259,8 +260,10 @@ int h3d_get_config(h3d_handle* h, h3d_config* c);
  * of MatVecProduct for many instances).  h3d_graph_create runs and captures one
  * matvec (or transpose) of each handle, hs[i]: b[i] -> y[i] (device pointers,
  * baked into the graph), in mode `prec`; h3d_graph_launch replays all of them
- * with one launch on `stream` (NULL: the legacy default stream).  The caller
- * rewrites the contents of b[i] between launches.  The graph stays valid while
+ * with one launch on `stream` (NULL: the legacy default stream); products of
+ * distinct handles are independent branches of the graph and may run
+ * concurrently (those of one handle stay in order).  The caller rewrites the
+ * contents of b[i] between launches; the y[i] must not overlap.  The graph stays valid while
  * every handle keeps the factor copies and scratch it had at capture: no
  * prepare/move/release of precisions, compress or larger nrhs in between.  The
  * handles' own streams are restored after the capture. */

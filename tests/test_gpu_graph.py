@@ -51,6 +51,27 @@ def test_graph_replays_equal_eager_products(instances, prec, transpose):
     g.close()
 
 
+def test_repeated_handle_keeps_its_products_in_order(instances):
+    """The same handle twice in one graph: both products share its scratch,
+    so they must run in order (one branch) and equal the eager products."""
+    import torch
+    import hodlr3d_amd as H3
+    hs = instances
+    seq = [hs[0], hs[1], hs[0], hs[2], hs[3], hs[1]]
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    b = [torch.randn(h.N, dtype=torch.float64, device="cuda", generator=gen) for h in seq]
+    y = [torch.empty(h.N, dtype=torch.float64, device="cuda") for h in seq]
+    g = H3.MatvecGraph(seq, b, y, prec="fp64")
+    for rep in range(3):
+        ref = [h.matvec(bi) for h, bi in zip(seq, b)]
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, (yi, ri) in enumerate(zip(y, ref)):
+            assert torch.equal(yi, ri), (rep, i)
+    g.close()
+
+
 def test_graph_rejects_bad_input(instances):
     import torch
     import hodlr3d_amd as H3
