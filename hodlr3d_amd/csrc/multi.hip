@@ -196,9 +196,9 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
   // Branch-free loads: one 4-row vector per lane and tile at an in-bounds
   // address (columns clamped to r - 1; a vector starting inside the range ends
   // at most 3 rows past it, in the column padding, the next column or the
-  // chunk's zero tail), then masked by selects.
-  // The raw vectors are masked only when they are used (compute), so that
-  // the wait for a unit's loads falls after the previous unit's MFMAs.
+  // chunk's zero tail); every factor value read is finite.
+  // The loaded vectors are consumed only by the MFMAs (compute), so the wait
+  // for a unit's loads falls after the previous unit's MFMAs.
   auto load = [&](const Unit& u, A (&av)[UF][CT][4]) {
 #pragma unroll
     for (int st = 0; st < UF; ++st) {
@@ -217,20 +217,19 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
 #pragma unroll
     for (int st = 0; st < UF; ++st) {
       if (u.s0 + st >= nsteps) break;
-      const int i = 16 * (u.s0 + st) + 4 * kq;
       A xb[H][4];
 #pragma unroll
       for (int h = 0; h < H; ++h)
         load_quad<A>(xl + ((4 * (u.s0 + st) + kq) * W + 16 * h + lc) * 4, xb[h]);
+      // no masks: rows past the range meet zero x (staged so), and the
+      // output rows of clamped columns (c >= r) are never stored
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           if (ct == 1 && !two) break;
-          const bool ok = u.cb + 16 * ct + lc < u.r && i + e < nrows;
-          const A a = ok ? av[st][ct][e] : (A)0;
 #pragma unroll
-          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(a, xb[h][e], acc[ct][h]);
+          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
         }
     }
   };
@@ -325,10 +324,10 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
     for (int u = 0; u < 4; ++u) ok[u] = i + u < lim;
     // branch-free loads at in-bounds addresses (column clamped to R - 1, rows
     // of lanes past the chunk moved to its first row; a vector starting inside
-    // the box ends at most 3 rows past it), masked by selects
+    // the box ends at most 3 rows past it)
     const int il = ok[0] ? i : it.i0 + r0;
-    // raw values; masked where used, so the wait for them follows the
-    // previous unit's MFMAs
+    // consumed only where used, so the wait for them follows the previous
+    // unit's MFMAs
     auto load = [&](int c0, A (&av)[UF][4], A (&tb)[UF][H]) {
 #pragma unroll
       for (int st = 0; st < UF; ++st) {
@@ -348,16 +347,16 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
 #pragma unroll
       for (int st = 0; st < UF; ++st) {
         if (c0 + 4 * st >= it.R) break;  // wave-uniform
+        // only t is masked (clamped columns); rows past the box give output
+        // rows that are never stored
         const bool cok = c0 + 4 * st + kq < it.R;
         A bm[H];
 #pragma unroll
         for (int h = 0; h < H; ++h) bm[h] = cok ? tb[st][h] : (A)0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const A a = (cok && ok[u]) ? av[st][u] : (A)0;
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-          for (int h = 0; h < H; ++h) acc[u][h] = M::mac(a, bm[h], acc[u][h]);
-        }
+          for (int h = 0; h < H; ++h) acc[u][h] = M::mac(av[st][u], bm[h], acc[u][h]);
       }
     };
     constexpr int STEP = 4 * UF;
