@@ -455,6 +455,42 @@ size_t aca_smem(int rcap, int n, int m) {
 }
 
 // Workspace of one ACA launch group.
+// Grow-only device workspace of one ACA pass index, reused by every batch:
+// allocating tens of GB per pass and batch took seconds of the assembly.
+struct AcaWorkspace {
+  double* U = nullptr;
+  double* V = nullptr;
+  int32_t* P = nullptr;
+  int64_t nU = 0, nV = 0, nP = 0;
+  void reserve(int64_t u, int64_t v, int64_t p) {
+    if (u > nU) {
+      dfree(U);
+      nU = 0;
+      U = dalloc<double>(u);
+      nU = u;
+    }
+    if (v > nV) {
+      dfree(V);
+      nV = 0;
+      V = dalloc<double>(v);
+      nV = v;
+    }
+    if (p > nP) {
+      dfree(P);
+      nP = 0;
+      P = dalloc<int32_t>(p);
+      nP = p;
+    }
+  }
+  ~AcaWorkspace() {
+    dfree(U);
+    dfree(V);
+    dfree(P);
+  }
+};
+
+// One ACA pass over a batch of blocks; U, V, P point into the workspace of
+// its pass index and stay valid until the batch's factors are written.
 struct Pass {
   double* U = nullptr;
   double* V = nullptr;
@@ -462,11 +498,6 @@ struct Pass {
   std::vector<AcaDesc> ds;
   std::vector<int64_t> ids;  // CSR block index within the level
   std::vector<int32_t> rank, status;
-  void release() {
-    dfree(U);
-    dfree(V);
-    dfree(P);
-  }
 };
 
 // Rank capacity of the ACA workspace: a block that would need more stops
@@ -479,8 +510,7 @@ int cap_for(int pass, int n, int m) {
   return std::min(c, mn);
 }
 
-void run_pass(Engine& e, Pass& P, int cls_filter_unused) {
-  (void)cls_filter_unused;
+void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
   hipStream_t st = e.stream;
   const int64_t nb = (int64_t)P.ds.size();
   P.rank.assign(nb, 0);
@@ -492,11 +522,10 @@ void run_pass(Engine& e, Pass& P, int cls_filter_unused) {
     wsV = std::max<int64_t>(wsV, d.vw + (int64_t)d.rcap * d.m);
     pw = std::max<int64_t>(pw, d.pw + 2 * (int64_t)d.rcap);
   }
-  P.U = dalloc<double>(wsU);
-  P.V = dalloc<double>(wsV);
-  P.P = dalloc<int32_t>(pw);
-  int32_t* d_rank = dalloc<int32_t>(nb);
-  int32_t* d_stat = dalloc<int32_t>(nb);
+  ws.reserve(wsU, wsV, pw);
+  P.U = ws.U;
+  P.V = ws.V;
+  P.P = ws.P;
   const bool fast = e.kern.q == 7 || e.kern.q == 16;
   // two launches: small blocks on one wave, large blocks on four
   for (int cls = 0; cls < 2; ++cls) {
@@ -543,8 +572,6 @@ void run_pass(Engine& e, Pass& P, int cls_filter_unused) {
     dfree(d_r);
     dfree(d_s);
   }
-  dfree(d_rank);
-  dfree(d_stat);
 }
 
 }  // namespace
@@ -565,6 +592,7 @@ void assemble_all(Engine& e) {
   e.find_memory = 0;
   e.nblk_all = 0;
   for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
+  AcaWorkspace ws[3];  // per pass index, freed when the assembly ends
 
   for (int j = 1; j <= nL; ++j) {
     Level& lv = e.lev[j];
@@ -628,7 +656,7 @@ void assemble_all(Engine& e) {
           P.ds.push_back(d);
           P.ids.push_back(b);
         }
-        run_pass(e, P, 0);
+        run_pass(e, P, ws[pass]);
         std::vector<int64_t> retry;
         for (size_t q = 0; q < P.ids.size(); ++q) {
           const int64_t b = P.ids[q];
@@ -712,7 +740,6 @@ void assemble_all(Engine& e) {
           H3D_HIP(hipStreamSynchronize(st));
           dfree(d_wd);
         }
-        P.release();
       }
       e.chunks.push_back(ch);
       k0 = k1;
