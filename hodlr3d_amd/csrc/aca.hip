@@ -149,6 +149,23 @@ __device__ inline double kentry(const KernelDev& kd, double xa, double ya, doubl
 
 __device__ inline bool bit_get(const uint32_t* m, int s) { return (m[s >> 5] >> (s & 31)) & 1u; }
 
+// x - sum_{l < r} c[l] * Z[l * ld], subtracted in l order (the reference's
+// order, no FMA), with eight loads of Z issued before their subtractions so
+// that a lane has several memory requests in flight.
+__device__ inline double sub_sum(double x, const double* c, const double* __restrict__ Z,
+                                 int64_t ld, int r) {
+  int l = 0;
+  for (; l + 8 <= r; l += 8) {
+    double z[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) z[q] = Z[(int64_t)(l + q) * ld];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x -= c[l + q] * z[q];
+  }
+  for (; l < r; ++l) x -= c[l] * Z[(int64_t)l * ld];
+  return x;
+}
+
 template <int T, class KF>
 __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ descs,
                                                 const double* __restrict__ px,
@@ -164,8 +181,8 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
   const AcaDesc d = descs[blockIdx.x];
   const int t = threadIdx.x;
   const int n = d.n, m = d.m;
-  double* red = smem;           // 2 x 256
-  double* amx = red + 512;      // 3 x (T/64)
+  double* red = smem;           // 8 x 256 (T = 256; the one-wave kernel reduces in registers)
+  double* amx = red + (T == 64 ? 0 : 2048);  // 3 x (T/64)
   double* uil = amx + 16;       // rcap
   double* vjl = uil + d.rcap;   // rcap
   uint32_t* rmask = (uint32_t*)(vjl + d.rcap);
@@ -307,7 +324,7 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
         const int s = base + t + T * p;
         if (s >= m) continue;
         double val = kentry<KF>(kd, xi, yi, zi, cx[s], cy[s], cz[s]);
-        for (int l = 0; l < r; ++l) val -= uil[l] * V[(int64_t)l * m + s];
+        val = sub_sum(val, uil, V + s, m, r);
         vr[s] = val;
         nonfinite |= !isfinite(val);
         acc[0][p] = acc[0][p] + val * val;
@@ -338,7 +355,7 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
         const int s = base + t + T * p;
         if (s >= n) continue;
         double col = kentry<KF>(kd, rx[s], ry[s], rz[s], xj, yj, zj);
-        for (int l = 0; l < r; ++l) col -= vjl[l] * U[(int64_t)l * n + s];
+        col = sub_sum(col, vjl, U + s, n, r);
         nonfinite |= !isfinite(col);
         double u = col / piv;
         ur[s] = u;
@@ -370,7 +387,47 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
     nv = nv2;
     double ns2 = normS * normS + (nu * nv) * (nu * nv);
     // dots of the previous vectors with the new pair, in l order
-    for (int l = 0; l < r - 1; ++l) {
+    // four previous vectors at a time (each dot still summed per lane in s
+    // order and reduced canonically; ns2 gathers them in l order)
+    int l0 = 0;
+    for (; l0 + 4 <= r - 1; l0 += 4) {
+      double a8[8][VPT];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int p = 0; p < VPT; ++p) a8[q][p] = 0.0;
+      for (int base = 0; base < n; base += 256)
+#pragma unroll
+        for (int p = 0; p < VPT; ++p) {
+          const int s = base + t + T * p;
+          if (s < n) {
+            const double x = ur[s];
+            double w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = U[(int64_t)(l0 + j) * n + s];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a8[2 * j][p] = a8[2 * j][p] + w[j] * x;
+          }
+        }
+      for (int base = 0; base < m; base += 256)
+#pragma unroll
+        for (int p = 0; p < VPT; ++p) {
+          const int s = base + t + T * p;
+          if (s < m) {
+            const double x = vr[s];
+            double w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = V[(int64_t)(l0 + j) * m + s];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a8[2 * j + 1][p] = a8[2 * j + 1][p] + w[j] * x;
+          }
+        }
+      double dd[8];
+      canon_reduce<T, 8>(a8, red, dd);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ns2 += 2 * fabs(dd[2 * j] * dd[2 * j + 1]);
+    }
+    for (int l = l0; l < r - 1; ++l) {
       double a2[2][VPT];
 #pragma unroll
       for (int p = 0; p < VPT; ++p) {
@@ -449,12 +506,11 @@ __global__ void write_kernel(const WriteDesc* __restrict__ ds, const double* __r
   }
 }
 
-size_t aca_smem(int rcap, int n, int m) {
-  return sizeof(double) * (512 + 16 + 2 * (size_t)rcap) +
+size_t aca_smem(int T, int rcap, int n, int m) {
+  return sizeof(double) * ((T == 64 ? 0 : 2048) + 16 + 2 * (size_t)rcap) +
          sizeof(uint32_t) * (size_t)(((n + 31) >> 5) + ((m + 31) >> 5)) + 16;
 }
 
-// Workspace of one ACA launch group.
 // Grow-only device workspace of one ACA pass index, reused by every batch:
 // allocating tens of GB per pass and batch took seconds of the assembly.
 struct AcaWorkspace {
@@ -538,7 +594,7 @@ void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
       if (c != cls) continue;
       sub.push_back(d);
       where.push_back(q);
-      smax = std::max(smax, aca_smem(d.rcap, d.n, d.m));
+      smax = std::max(smax, aca_smem(cls == 0 ? 64 : 256, d.rcap, d.n, d.m));
     }
     if (sub.empty()) continue;
     if (smax > 160 * 1024) throw Error(H3D_ERR_INVALID, "ACA block too large for LDS");
