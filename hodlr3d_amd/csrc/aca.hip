@@ -512,36 +512,52 @@ size_t aca_smem(int T, int rcap, int n, int m) {
 }
 
 // Grow-only device workspace of one ACA pass index, reused by every batch:
-// allocating tens of GB per pass and batch took seconds of the assembly.
+// allocating tens of GB per pass and batch took seconds of the assembly.  It
+// also holds the pass's block descriptors, ranks and statuses and the write
+// descriptors, so that no device memory is freed while the assembly runs
+// (hipFree waits for the whole device, which would stall a pass running on
+// another stream).
 struct AcaWorkspace {
   double* U = nullptr;
   double* V = nullptr;
   int32_t* P = nullptr;
-  int64_t nU = 0, nV = 0, nP = 0;
+  AcaDesc* D = nullptr;
+  int32_t* R = nullptr;
+  int32_t* S = nullptr;
+  WriteDesc* WD = nullptr;
+  int64_t nU = 0, nV = 0, nP = 0, nD = 0, nWD = 0;
+  template <class T>
+  static void grow(T*& p, int64_t& cap, int64_t n) {
+    if (n <= cap) return;
+    dfree(p);
+    cap = 0;
+    p = dalloc<T>(n);
+    cap = n;
+  }
   void reserve(int64_t u, int64_t v, int64_t p) {
-    if (u > nU) {
-      dfree(U);
-      nU = 0;
-      U = dalloc<double>(u);
-      nU = u;
-    }
-    if (v > nV) {
-      dfree(V);
-      nV = 0;
-      V = dalloc<double>(v);
-      nV = v;
-    }
-    if (p > nP) {
-      dfree(P);
-      nP = 0;
-      P = dalloc<int32_t>(p);
-      nP = p;
-    }
+    grow(U, nU, u);
+    grow(V, nV, v);
+    grow(P, nP, p);
+  }
+  void reserve_desc(int64_t n) {
+    if (n <= nD) return;
+    dfree(D);
+    dfree(R);
+    dfree(S);
+    nD = 0;
+    D = dalloc<AcaDesc>(n);
+    R = dalloc<int32_t>(n);
+    S = dalloc<int32_t>(n);
+    nD = n;
   }
   ~AcaWorkspace() {
     dfree(U);
     dfree(V);
     dfree(P);
+    dfree(D);
+    dfree(R);
+    dfree(S);
+    dfree(WD);
   }
 };
 
@@ -554,6 +570,13 @@ struct Pass {
   std::vector<AcaDesc> ds;
   std::vector<int64_t> ids;  // CSR block index within the level
   std::vector<int32_t> rank, status;
+  // launched on `st`; per class (one-wave / four-wave kernel) the position
+  // of its blocks in ds and their range in the workspace's D, R, S
+  hipStream_t st = nullptr;
+  AcaWorkspace* ws = nullptr;  // holds its descriptors, ranks and statuses
+  std::vector<AcaDesc> sub[2];
+  std::vector<int64_t> where[2];
+  int64_t off[2] = {0, 0};
 };
 
 // Rank capacity of the ACA workspace: a block that would need more stops
@@ -566,9 +589,10 @@ int cap_for(int pass, int n, int m) {
   return std::min(c, mn);
 }
 
-void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
-  hipStream_t st = e.stream;
+void launch_pass(Engine& e, Pass& P, AcaWorkspace& ws, hipStream_t st) {
   const int64_t nb = (int64_t)P.ds.size();
+  P.st = st;
+  P.ws = &ws;
   P.rank.assign(nb, 0);
   P.status.assign(nb, 0);
   if (nb == 0) return;
@@ -579,14 +603,18 @@ void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
     pw = std::max<int64_t>(pw, d.pw + 2 * (int64_t)d.rcap);
   }
   ws.reserve(wsU, wsV, pw);
+  ws.reserve_desc(nb);
   P.U = ws.U;
   P.V = ws.V;
   P.P = ws.P;
   const bool fast = e.kern.q == 7 || e.kern.q == 16;
   // two launches: small blocks on one wave, large blocks on four
+  int64_t off = 0;
   for (int cls = 0; cls < 2; ++cls) {
-    std::vector<AcaDesc> sub;
-    std::vector<int64_t> where;
+    std::vector<AcaDesc>& sub = P.sub[cls];
+    std::vector<int64_t>& where = P.where[cls];
+    sub.clear();
+    where.clear();
     size_t smax = 0;
     for (int64_t q = 0; q < nb; ++q) {
       const AcaDesc& d = P.ds[q];
@@ -596,6 +624,7 @@ void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
       where.push_back(q);
       smax = std::max(smax, aca_smem(cls == 0 ? 64 : 256, d.rcap, d.n, d.m));
     }
+    P.off[cls] = off;
     if (sub.empty()) continue;
     if (smax > 160 * 1024) throw Error(H3D_ERR_INVALID, "ACA block too large for LDS");
     const void* kfun = cls == 0 ? (fast ? (const void*)aca_kernel<64, InvR>
@@ -604,37 +633,209 @@ void run_pass(Engine& e, Pass& P, AcaWorkspace& ws) {
                                         : (const void*)aca_kernel<256, AnyQ>);
     if (smax > 64 * 1024)
       H3D_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smax));
-    AcaDesc* d_ds = dalloc<AcaDesc>((int64_t)sub.size());
-    int32_t* d_r = dalloc<int32_t>((int64_t)sub.size());
-    int32_t* d_s = dalloc<int32_t>((int64_t)sub.size());
+    AcaDesc* d_ds = ws.D + off;
     H3D_HIP(hipMemcpyAsync(d_ds, sub.data(), sizeof(AcaDesc) * sub.size(), hipMemcpyHostToDevice, st));
-    for (size_t off = 0; off < sub.size(); off += 1000000) {
-      const unsigned g = (unsigned)std::min<size_t>(1000000, sub.size() - off);
-      AcaDesc* dd = d_ds + off;
-      int32_t* rr = d_r + off;
-      int32_t* ss = d_s + off;
+    for (size_t o = 0; o < sub.size(); o += 1000000) {
+      const unsigned g = (unsigned)std::min<size_t>(1000000, sub.size() - o);
+      AcaDesc* dd = d_ds + o;
+      int32_t* rr = ws.R + off + o;
+      int32_t* ss = ws.S + off + o;
       void* args[] = {&dd, &e.d_px, &e.d_py, &e.d_pz, &e.kern, &e.tol, &P.U, &P.V, &P.P, &rr, &ss};
       H3D_HIP(hipLaunchKernel(kfun, dim3(g), dim3(cls == 0 ? 64 : 256), args, smax, st));
     }
-    std::vector<int32_t> rk(sub.size()), sv(sub.size());
-    H3D_HIP(hipMemcpyAsync(rk.data(), d_r, sizeof(int32_t) * sub.size(), hipMemcpyDeviceToHost, st));
-    H3D_HIP(hipMemcpyAsync(sv.data(), d_s, sizeof(int32_t) * sub.size(), hipMemcpyDeviceToHost, st));
-    H3D_HIP(hipStreamSynchronize(st));
-    for (size_t q = 0; q < sub.size(); ++q) {
-      P.rank[where[q]] = rk[q];
-      P.status[where[q]] = sv[q];
-    }
-    dfree(d_ds);
-    dfree(d_r);
-    dfree(d_s);
+    off += (int64_t)sub.size();
   }
+}
+
+// Waits for a launched pass and reads its ranks and statuses.
+void collect_pass(Pass& P) {
+  if (!P.ws) return;  // no blocks
+  const AcaWorkspace& ws = *P.ws;
+  for (int cls = 0; cls < 2; ++cls) {
+    const size_t n = P.sub[cls].size();
+    if (!n) continue;
+    std::vector<int32_t> rk(n), sv(n);
+    H3D_HIP(hipMemcpyAsync(rk.data(), ws.R + P.off[cls], sizeof(int32_t) * n, hipMemcpyDeviceToHost, P.st));
+    H3D_HIP(hipMemcpyAsync(sv.data(), ws.S + P.off[cls], sizeof(int32_t) * n, hipMemcpyDeviceToHost, P.st));
+    H3D_HIP(hipStreamSynchronize(P.st));
+    for (size_t q = 0; q < n; ++q) {
+      P.rank[P.where[cls][q]] = rk[q];
+      P.status[P.where[cls][q]] = sv[q];
+    }
+  }
+}
+
+// End of the batch of complete target boxes of level j starting at k0 whose
+// first-pass workspace fits the budget.
+int64_t batch_end(const Engine& e, int j, int64_t k0, int64_t budget) {
+  const Level& lv = e.lev[j];
+  const int64_t lo = level_offset(j);
+  int64_t k1 = k0, bytes = 0;
+  while (k1 < lv.nboxes) {
+    int64_t need = 0;
+    const int n = e.bcount[lo + k1];
+    for (int32_t b = lv.iloff[k1]; b < lv.iloff[k1 + 1]; ++b) {
+      const int m = e.bcount[lo + lv.ilidx[b]];
+      need += (int64_t)cap_for(0, n, m) * (n + m) * 8;
+    }
+    if (k1 > k0 && (bytes + need > budget || lv.iloff[k1 + 1] - lv.iloff[k0] > 2000000)) break;
+    bytes += need;
+    ++k1;
+  }
+  return k1;
+}
+
+// Blocks of level j with a nonempty target in boxes [k0, k1), and their
+// descriptors for `pass`.
+std::vector<int64_t> batch_blocks(const Engine& e, int j, int64_t k0, int64_t k1) {
+  const Level& lv = e.lev[j];
+  const int64_t lo = level_offset(j);
+  std::vector<int64_t> todo;
+  for (int64_t k = k0; k < k1; ++k) {
+    const int n = e.bcount[lo + k];
+    for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b)
+      if (n > 0 && e.bcount[lo + lv.ilidx[b]] > 0) todo.push_back(b);
+  }
+  return todo;
+}
+
+void fill_pass(const Engine& e, int j, const std::vector<int64_t>& todo, int pass, Pass& P) {
+  const Level& lv = e.lev[j];
+  const int64_t lo = level_offset(j);
+  int64_t wsU = 0, wsV = 0;
+  int32_t pw = 0;
+  for (int64_t b : todo) {
+    int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
+                          lv.iloff.begin() - 1);
+    const int32_t ki = lv.ilidx[b];
+    AcaDesc d;
+    d.rs = e.bstart[lo + k];
+    d.n = e.bcount[lo + k];
+    d.cs = e.bstart[lo + ki];
+    d.m = e.bcount[lo + ki];
+    d.rcap = cap_for(pass, d.n, d.m);
+    d.uw = wsU;
+    d.vw = wsV;
+    d.pw = pw;
+    wsU += (int64_t)d.rcap * d.n;
+    wsV += (int64_t)d.rcap * d.m;
+    pw += 2 * d.rcap;
+    P.ds.push_back(d);
+    P.ids.push_back(b);
+  }
+}
+
+// ACA of the batch [k0, k1) of level j (pass 0 given when it was launched
+// earlier), chunk layout and factor write-out.
+void assemble_batch(Engine& e, int j, int64_t k0, int64_t k1, AcaWorkspace (&ws)[3],
+                    Pass* early) {
+  hipStream_t st = e.stream;
+  Level& lv = e.lev[j];
+  const int64_t lo = level_offset(j);
+  std::vector<int64_t> todo = batch_blocks(e, j, k0, k1);
+  std::vector<Pass> passes;
+  for (int pass = 0; pass < 3 && !todo.empty(); ++pass) {
+    if (pass == 0 && early) {
+      passes.push_back(std::move(*early));  // launched before, on its own stream and workspace
+      collect_pass(passes.back());
+    } else {
+      passes.emplace_back();
+      fill_pass(e, j, todo, pass, passes.back());
+      launch_pass(e, passes.back(), ws[pass], st);
+      collect_pass(passes.back());
+    }
+    Pass& P = passes.back();
+    std::vector<int64_t> retry;
+    for (size_t q = 0; q < P.ids.size(); ++q) {
+      const int64_t b = P.ids[q];
+      if (P.status[q] & 2) {
+        retry.push_back(b);
+        P.rank[q] = -1;  // not final
+        continue;
+      }
+      if (P.status[q] & 1) throw Error(H3D_ERR_NONFINITE, "non-finite kernel entry in ACA");
+      lv.rank[b] = P.rank[q];
+      lv.status[b] = P.status[q];
+    }
+    todo.swap(retry);
+  }
+  if (!todo.empty()) throw Error(H3D_ERR_INTERNAL, "ACA did not finish within capacity");
+    // chunk layout for boxes [k0, k1)
+    Chunk ch;
+    for (int64_t k = k0; k < k1; ++k) {
+      const int n = e.bcount[lo + k];
+      int32_t R = 0;
+      for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+        const int r = lv.rank[b];
+        const int m = e.bcount[lo + lv.ilidx[b]];
+        lv.col[b] = R;
+        R += r;
+        lv.voff[b] = ch.nV;
+        ch.nV += (int64_t)r * (m + (m & 1));
+        lv.pivoff[b] = ch.npiv;
+        ch.npiv += 2 * (int64_t)r;
+      }
+      lv.Rt[k] = R;
+      lv.uoff[k] = ch.nU;
+      ch.nU += (int64_t)(n + (n & 1)) * R;
+      lv.chunk[k] = (int32_t)e.chunks.size();
+    }
+    for (int p = 0; p < 3; ++p) {
+      if (!(e.store_mask & (1 << p))) continue;
+      const size_t bytes = (size_t)(ch.nU + ch.nV + kChunkTail) * elem_size(p);
+      H3D_HIP(hipMalloc(&ch.data[p], bytes));
+      H3D_HIP(hipMemsetAsync(ch.data[p], 0, bytes, st));
+      e.fac_bytes[p] += (ch.nU + ch.nV) * elem_size(p);
+    }
+    ch.piv = dalloc<int32_t>(std::max<int64_t>(ch.npiv, 1));
+    for (Pass& P : passes) {
+      std::vector<WriteDesc> wd;
+      for (size_t q = 0; q < P.ids.size(); ++q) {
+        if (P.rank[q] <= 0) continue;
+        const int64_t b = P.ids[q];
+        int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
+                              lv.iloff.begin() - 1);
+        WriteDesc w;
+        w.uw = P.ds[q].uw;
+        w.vw = P.ds[q].vw;
+        w.n = P.ds[q].n;
+        w.m = P.ds[q].m;
+        w.r = P.rank[q];
+        w.pw = P.ds[q].pw;
+        w.rcap = P.ds[q].rcap;
+        w.pad = 0;
+        w.uo = lv.uoff[k] + (int64_t)lv.col[b] * (w.n + (w.n & 1));
+        w.vo = lv.voff[b];
+        w.po = lv.pivoff[b];
+        wd.push_back(w);
+      }
+      if (!wd.empty()) {
+        AcaWorkspace::grow(ws[0].WD, ws[0].nWD, (int64_t)wd.size());
+        WriteDesc* d_wd = ws[0].WD;
+        H3D_HIP(hipMemcpyAsync(d_wd, wd.data(), sizeof(WriteDesc) * wd.size(),
+                               hipMemcpyHostToDevice, st));
+        double* U64 = (double*)ch.data[0];
+        double* V64 = U64 ? U64 + ch.nU : nullptr;
+        float* U32 = (float*)ch.data[1];
+        float* V32 = U32 ? U32 + ch.nU : nullptr;
+        uint16_t* U16 = (uint16_t*)ch.data[2];
+        uint16_t* V16 = U16 ? U16 + ch.nU : nullptr;
+        for (size_t off = 0; off < wd.size(); off += 1000000) {
+          const unsigned g = (unsigned)std::min<size_t>(1000000, wd.size() - off);
+          write_kernel<<<g, 256, 0, st>>>(d_wd + off, P.U, P.V, P.P, U64, V64, U32, V32, U16,
+                                          V16, ch.piv);
+          H3D_LAUNCH_CHECK();
+        }
+        H3D_HIP(hipStreamSynchronize(st));
+      }
+    }
+    e.chunks.push_back(ch);
 }
 
 }  // namespace
 
 void assemble_all(Engine& e) {
   auto t0 = std::chrono::steady_clock::now();
-  hipStream_t st = e.stream;
   const int nL = e.nL;
   size_t freeb = 0, totb = 0;
   H3D_HIP(hipMemGetInfo(&freeb, &totb));
@@ -648,13 +849,10 @@ void assemble_all(Engine& e) {
   e.find_memory = 0;
   e.nblk_all = 0;
   for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
-  AcaWorkspace ws[3];  // per pass index, freed when the assembly ends
-
-  for (int j = 1; j <= nL; ++j) {
+  for (int j = 1; j <= nL; ++j) {  // block ids and per-level arrays (levels may finish out of order)
     Level& lv = e.lev[j];
     const int64_t nb = lv.nboxes;
     const int64_t nblk = (int64_t)lv.ilidx.size();
-    const int64_t lo = level_offset(j);
     lv.bbase = e.nblk_all;
     e.nblk_all += nblk;
     lv.rank.assign(nblk, 0);
@@ -666,141 +864,77 @@ void assemble_all(Engine& e) {
     lv.toff.assign(nb, 0);
     lv.chunk.assign(nb, -1);
     lv.uoff.assign(nb, 0);
-    int64_t k0 = 0;
-    while (k0 < nb) {
-      // batch of complete target boxes within the workspace budget
-      int64_t k1 = k0, bytes = 0;
-      while (k1 < nb) {
-        int64_t need = 0;
-        const int n = e.bcount[lo + k1];
-        for (int32_t b = lv.iloff[k1]; b < lv.iloff[k1 + 1]; ++b) {
-          const int m = e.bcount[lo + lv.ilidx[b]];
-          need += (int64_t)cap_for(0, n, m) * (n + m) * 8;
-        }
-        if (k1 > k0 && (bytes + need > budget || lv.iloff[k1 + 1] - lv.iloff[k0] > 2000000)) break;
-        bytes += need;
-        ++k1;
-      }
-      std::vector<int64_t> todo;
-      for (int64_t k = k0; k < k1; ++k) {
-        const int n = e.bcount[lo + k];
-        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b)
-          if (n > 0 && e.bcount[lo + lv.ilidx[b]] > 0) todo.push_back(b);
-      }
-      std::vector<Pass> passes;
-      for (int pass = 0; pass < 3 && !todo.empty(); ++pass) {
-        passes.emplace_back();
-        Pass& P = passes.back();
-        int64_t wsU = 0, wsV = 0;
-        int32_t pw = 0;
-        for (int64_t b : todo) {
-          int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
-                                lv.iloff.begin() - 1);
-          const int32_t ki = lv.ilidx[b];
-          AcaDesc d;
-          d.rs = e.bstart[lo + k];
-          d.n = e.bcount[lo + k];
-          d.cs = e.bstart[lo + ki];
-          d.m = e.bcount[lo + ki];
-          d.rcap = cap_for(pass, d.n, d.m);
-          d.uw = wsU;
-          d.vw = wsV;
-          d.pw = pw;
-          wsU += (int64_t)d.rcap * d.n;
-          wsV += (int64_t)d.rcap * d.m;
-          pw += 2 * d.rcap;
-          P.ds.push_back(d);
-          P.ids.push_back(b);
-        }
-        run_pass(e, P, ws[pass]);
-        std::vector<int64_t> retry;
-        for (size_t q = 0; q < P.ids.size(); ++q) {
-          const int64_t b = P.ids[q];
-          if (P.status[q] & 2) {
-            retry.push_back(b);
-            P.rank[q] = -1;  // not final
-            continue;
+  }
+  AcaWorkspace ws[3];  // per pass index, freed when the assembly ends
+  // The first pass of level 1 (a few huge blocks: one workgroup each would
+  // hold a handful of CUs for seconds) runs on a second stream while the
+  // other levels are assembled; level 1 is finished last.  Its factors land
+  // in a chunk of their own, so the layout does not depend on the order.
+  AcaWorkspace wsE;
+  Pass early;
+  hipStream_t side = nullptr;
+  bool use_early = false;
+  std::vector<int64_t> early_todo;
+  if (nL >= 2 && batch_end(e, 1, 0, budget) == e.lev[1].nboxes) {
+    early_todo = batch_blocks(e, 1, 0, e.lev[1].nboxes);
+    int big = 0;
+    for (int64_t b : early_todo)
+      big = std::max<int>(big, e.bcount[level_offset(1) + e.lev[1].ilidx[b]]);
+    use_early = !early_todo.empty() && big >= 8192;
+  }
+  try {
+    // the first-pass workspace of every later batch is reserved now (no frees
+    // while the early pass runs)
+    {
+      int64_t u = 0, v = 0, pw = 0;
+      for (int j = use_early ? 2 : 1; j <= nL; ++j)
+        for (int64_t k0 = 0; k0 < e.lev[j].nboxes;) {
+          const int64_t k1 = batch_end(e, j, k0, budget);
+          Pass P;
+          fill_pass(e, j, batch_blocks(e, j, k0, k1), 0, P);
+          int64_t wu = 0, wv = 0, wp = 0;
+          for (auto& d : P.ds) {
+            wu = std::max<int64_t>(wu, d.uw + (int64_t)d.rcap * d.n);
+            wv = std::max<int64_t>(wv, d.vw + (int64_t)d.rcap * d.m);
+            wp = std::max<int64_t>(wp, d.pw + 2 * (int64_t)d.rcap);
           }
-          if (P.status[q] & 1) throw Error(H3D_ERR_NONFINITE, "non-finite kernel entry in ACA");
-          lv.rank[b] = P.rank[q];
-          lv.status[b] = P.status[q];
+          u = std::max(u, wu);
+          v = std::max(v, wv);
+          pw = std::max(pw, wp);
+          ws[0].reserve_desc((int64_t)P.ds.size());
+          AcaWorkspace::grow(ws[0].WD, ws[0].nWD, (int64_t)P.ds.size());
+          k0 = k1;
         }
-        todo.swap(retry);
-      }
-      if (!todo.empty()) throw Error(H3D_ERR_INTERNAL, "ACA did not finish within capacity");
-      // chunk layout for boxes [k0, k1)
-      Chunk ch;
-      for (int64_t k = k0; k < k1; ++k) {
-        const int n = e.bcount[lo + k];
-        int32_t R = 0;
-        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
-          const int r = lv.rank[b];
-          const int m = e.bcount[lo + lv.ilidx[b]];
-          lv.col[b] = R;
-          R += r;
-          lv.voff[b] = ch.nV;
-          ch.nV += (int64_t)r * (m + (m & 1));
-          lv.pivoff[b] = ch.npiv;
-          ch.npiv += 2 * (int64_t)r;
-        }
-        lv.Rt[k] = R;
-        lv.uoff[k] = ch.nU;
-        ch.nU += (int64_t)(n + (n & 1)) * R;
-        lv.chunk[k] = (int32_t)e.chunks.size();
-      }
-      for (int p = 0; p < 3; ++p) {
-        if (!(e.store_mask & (1 << p))) continue;
-        const size_t bytes = (size_t)(ch.nU + ch.nV + kChunkTail) * elem_size(p);
-        H3D_HIP(hipMalloc(&ch.data[p], bytes));
-        H3D_HIP(hipMemsetAsync(ch.data[p], 0, bytes, st));
-        e.fac_bytes[p] += (ch.nU + ch.nV) * elem_size(p);
-      }
-      ch.piv = dalloc<int32_t>(std::max<int64_t>(ch.npiv, 1));
-      for (Pass& P : passes) {
-        std::vector<WriteDesc> wd;
-        for (size_t q = 0; q < P.ids.size(); ++q) {
-          if (P.rank[q] <= 0) continue;
-          const int64_t b = P.ids[q];
-          int32_t k = (int32_t)(std::upper_bound(lv.iloff.begin(), lv.iloff.end(), (int32_t)b) -
-                                lv.iloff.begin() - 1);
-          WriteDesc w;
-          w.uw = P.ds[q].uw;
-          w.vw = P.ds[q].vw;
-          w.n = P.ds[q].n;
-          w.m = P.ds[q].m;
-          w.r = P.rank[q];
-          w.pw = P.ds[q].pw;
-          w.rcap = P.ds[q].rcap;
-          w.pad = 0;
-          w.uo = lv.uoff[k] + (int64_t)lv.col[b] * (w.n + (w.n & 1));
-          w.vo = lv.voff[b];
-          w.po = lv.pivoff[b];
-          wd.push_back(w);
-        }
-        if (!wd.empty()) {
-          WriteDesc* d_wd = dalloc<WriteDesc>((int64_t)wd.size());
-          H3D_HIP(hipMemcpyAsync(d_wd, wd.data(), sizeof(WriteDesc) * wd.size(),
-                                 hipMemcpyHostToDevice, st));
-          double* U64 = (double*)ch.data[0];
-          double* V64 = U64 ? U64 + ch.nU : nullptr;
-          float* U32 = (float*)ch.data[1];
-          float* V32 = U32 ? U32 + ch.nU : nullptr;
-          uint16_t* U16 = (uint16_t*)ch.data[2];
-          uint16_t* V16 = U16 ? U16 + ch.nU : nullptr;
-          for (size_t off = 0; off < wd.size(); off += 1000000) {
-            const unsigned g = (unsigned)std::min<size_t>(1000000, wd.size() - off);
-            write_kernel<<<g, 256, 0, st>>>(d_wd + off, P.U, P.V, P.P, U64, V64, U32, V32, U16,
-                                            V16, ch.piv);
-            H3D_LAUNCH_CHECK();
-          }
-          H3D_HIP(hipStreamSynchronize(st));
-          dfree(d_wd);
-        }
-      }
-      e.chunks.push_back(ch);
-      k0 = k1;
+      ws[0].reserve(u, v, pw);
     }
-    for (int64_t k = 0; k < nb; ++k) {
+    if (use_early) {  // launched after every allocation made ahead of time
+      H3D_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      fill_pass(e, 1, early_todo, 0, early);
+      launch_pass(e, early, wsE, side);
+    }
+    for (int jj = 1; jj <= nL; ++jj) {
+      const int j = use_early ? (jj < nL ? jj + 1 : 1) : jj;
+      for (int64_t k0 = 0; k0 < e.lev[j].nboxes;) {
+        const int64_t k1 = batch_end(e, j, k0, budget);
+        const bool first = use_early && j == 1 && k0 == 0;
+        assemble_batch(e, j, k0, k1, ws, first ? &early : nullptr);
+        k0 = k1;
+      }
+    }
+  } catch (...) {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    throw;
+  }
+  if (side) {
+    H3D_HIP(hipStreamSynchronize(side));
+    (void)hipStreamDestroy(side);
+  }
+  for (int j = 1; j <= nL; ++j) {  // t offsets in level order
+    Level& lv = e.lev[j];
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
       lv.toff[k] = e.T;
       if ((int64_t)e.T + lv.Rt[k] > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
       e.T += lv.Rt[k];
