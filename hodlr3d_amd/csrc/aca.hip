@@ -150,17 +150,24 @@ __device__ inline double kentry(const KernelDev& kd, double xa, double ya, doubl
 __device__ inline bool bit_get(const uint32_t* m, int s) { return (m[s >> 5] >> (s & 31)) & 1u; }
 
 // x - sum_{l < r} c[l] * Z[l * ld], subtracted in l order (the reference's
-// order, no FMA), with eight loads of Z issued before their subtractions so
-// that a lane has several memory requests in flight.
+// order, no FMA), with sixteen loads of Z issued before their subtractions so
+// that a lane has many memory requests in flight.
 __device__ inline double sub_sum(double x, const double* c, const double* __restrict__ Z,
                                  int64_t ld, int r) {
   int l = 0;
-  for (; l + 8 <= r; l += 8) {
-    double z[8];
+  for (; l + 16 <= r; l += 16) {
+    double z[16];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) z[q] = Z[(int64_t)(l + q) * ld];
+    for (int q = 0; q < 16; ++q) z[q] = Z[(int64_t)(l + q) * ld];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) x -= c[l + q] * z[q];
+    for (int q = 0; q < 16; ++q) x -= c[l + q] * z[q];
+  }
+  for (; l + 4 <= r; l += 4) {
+    double z[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) z[q] = Z[(int64_t)(l + q) * ld];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x -= c[l + q] * z[q];
   }
   for (; l < r; ++l) x -= c[l] * Z[(int64_t)l * ld];
   return x;
@@ -396,32 +403,36 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
       for (int q = 0; q < 8; ++q)
 #pragma unroll
         for (int p = 0; p < VPT; ++p) a8[q][p] = 0.0;
-      for (int base = 0; base < n; base += 256)
+      // two rows per lane and iteration: all ten loads first, then the sums
+      // in row order
+      auto dots4 = [&](const double* __restrict__ Z, const double* __restrict__ zr, int len,
+                       int q0) {
+        for (int base = 0; base < len; base += 512)
 #pragma unroll
-        for (int p = 0; p < VPT; ++p) {
-          const int s = base + t + T * p;
-          if (s < n) {
-            const double x = ur[s];
-            double w[4];
+          for (int p = 0; p < VPT; ++p) {
+            const int s0 = base + t + T * p, s1 = s0 + 256;
+            double x[2] = {0.0, 0.0}, w[2][4];
+            const bool in0 = s0 < len, in1 = s1 < len;
+            if (in0) {
+              x[0] = zr[s0];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) w[j] = U[(int64_t)(l0 + j) * n + s];
+              for (int j = 0; j < 4; ++j) w[0][j] = Z[(int64_t)(l0 + j) * len + s0];
+            }
+            if (in1) {
+              x[1] = zr[s1];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) a8[2 * j][p] = a8[2 * j][p] + w[j] * x;
+              for (int j = 0; j < 4; ++j) w[1][j] = Z[(int64_t)(l0 + j) * len + s1];
+            }
+            if (in0)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) a8[q0 + 2 * j][p] = a8[q0 + 2 * j][p] + w[0][j] * x[0];
+            if (in1)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) a8[q0 + 2 * j][p] = a8[q0 + 2 * j][p] + w[1][j] * x[1];
           }
-        }
-      for (int base = 0; base < m; base += 256)
-#pragma unroll
-        for (int p = 0; p < VPT; ++p) {
-          const int s = base + t + T * p;
-          if (s < m) {
-            const double x = vr[s];
-            double w[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) w[j] = V[(int64_t)(l0 + j) * m + s];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) a8[2 * j + 1][p] = a8[2 * j + 1][p] + w[j] * x;
-          }
-        }
+      };
+      dots4(U, ur, n, 0);
+      dots4(V, vr, m, 1);
       double dd[8];
       canon_reduce<T, 8>(a8, red, dd);
 #pragma unroll
