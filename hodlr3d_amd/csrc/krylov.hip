@@ -74,6 +74,13 @@ __global__ void sub_kernel(double* __restrict__ dst, const double* __restrict__ 
     dst[i] = a[i] - b[i];
 }
 
+// y += x
+__global__ void add_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] + x[i];
+}
+
 // x += sum_c y[c] Q[:, c]  (Q column-major, ld = n)
 __global__ void combine_basis_kernel(double* __restrict__ x, const double* __restrict__ Q,
                                      const double* __restrict__ y, int K, int64_t n) {
@@ -266,24 +273,19 @@ int solve_ir(Engine& e, const double* d_b, double* d_x, int restart, int max_inn
       int k = gmres_cycle(e, v, Q, rr, cur, r0, m, inner_tol, inner_prec, transpose, d, &est,
                           nullptr);
       used += k;
-      // true inner residual rr = r - A_inner d
+      // converged by the cycle's own estimate: the outer step recomputes the
+      // true residual, so no inner one is formed
+      if (k < m && est <= inner_tol) break;
+      // restart from the true inner residual rr = r - A_inner d
       matvec_device(e, d, Ax, 1, inner_prec, transpose);
       v.sub(rr, r, Ax);
       v.dot(rr, rr, 0, 1);
       cur = v.get(0);
-      if (k < m && est <= inner_tol) break;
     }
     inner_total += used;
     // x += d; r = b - A x with the fp64 operator
-    {
-      double one = -1.0;
-      double* d_one = dalloc<double>(1);
-      H3D_HIP(hipMemcpyAsync(d_one, &one, sizeof(double), hipMemcpyHostToDevice, e.stream));
-      axpy_dev_kernel<<<v.grid, 256, 0, e.stream>>>(d_x, d, d_one, n);
-      H3D_LAUNCH_CHECK();
-      H3D_HIP(hipStreamSynchronize(e.stream));
-      dfree(d_one);
-    }
+    add_kernel<<<v.grid, 256, 0, e.stream>>>(d_x, d, n);
+    H3D_LAUNCH_CHECK();
     matvec_device(e, d_x, Ax, 1, outer_prec, transpose);
     v.sub(r, d_b, Ax);
     v.dot(r, r, 0, 1);
