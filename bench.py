@@ -222,7 +222,7 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
     h.set_rhs_block(0)
     y1 = h.matvec(Q, prec="fp32")
     single_ms = timed(torch, lambda: h.matvec(Q, prec="fp32"), 1, 0)
-    h.set_rhs_block(32)
+    tuned = h.autotune_rhs("fp32", nrhs=nrhs, reps=2)
     y = h.matvec(Q, prec="fp32")
     ms = timed(torch, lambda: h.matvec(Q, prec="fp32"), steps, warmup)
     diff = float((torch.linalg.norm(y - y1, dim=0) / torch.linalg.norm(y1, dim=0)).max())
@@ -235,7 +235,9 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
     h.close()
     out = {"workload": "N=262144 iid uniform in [-1,1]^3, leaf1d 4 (nLevels 4), K=1/r, "
                        "ACA tol 1e-8, fp32 factors, fp32 accumulation",
-           "nrhs": nrhs, "group_width": 32, "ms": round(ms, 4),
+           "nrhs": nrhs, "group_width": tuned["width"], "phase1_columns": tuned["columns"],
+           "autotune_ms": {f"w{w}/c{c}": round(v, 3) for (w, c), v in tuned["ms"].items()},
+           "ms": round(ms, 4),
            "ms_per_rhs": round(ms / nrhs, 4), "one_at_a_time_ms": round(single_ms, 3),
            "speedup": round(single_ms / ms, 2),
            "factor_GB": round(fbytes / 1e9, 2),

@@ -281,6 +281,14 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag) {
   });
 }
 
+int h3d_autotune_rhs(h3d_handle* h, int prec, int nrhs, int reps, double* ms, int* best) {
+  return guarded([&] {
+    if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+    if (!h->e.assembled) throw h3d::Error(H3D_ERR_STATE, "autotune before assemble");
+    h3d::autotune_rhs(h->e, prec, nrhs, reps, ms, best);
+  });
+}
+
 int h3d_set_rhs_block(h3d_handle* h, int width) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");

@@ -303,6 +303,7 @@ struct Engine {
   // products with several right-hand sides: groups of up to rhs_block (16 or
   // 32; 0 = one at a time) applied together (multi.hip); scratch for W per entry
   int rhs_block = 32;
+  int g_cols = 1024;                // columns of one multi-RHS phase-1 item
   int w_cap = 0;                    // W of the allocated group scratch
   int64_t w_t = 0, w_np = 0;        // t and partial entries it holds per RHS
   double* d_xw = nullptr;           // x by position, quad and row layouts
@@ -363,6 +364,9 @@ void load_engine(Engine& e, const char* path, int64_t ws_budget);
 // the grouped product (forward, ACA factor copies)
 int block_width(const Engine& e, int nrhs, int prec, bool transpose);
 void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64);
+// times the grouped product of nrhs columns for every group width and
+// phase-1 item size, keeps the fastest; ms[8] (width-major), best[2]
+void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best);
 // apply.hip
 void build_plan(Engine& e);
 void ensure_scratch(Engine& e, int nrhs);

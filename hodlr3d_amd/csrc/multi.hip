@@ -28,7 +28,6 @@
 namespace h3d {
 namespace {
 
-constexpr int kSColumns = 1024;  // columns of one phase-1 item
 constexpr int kGRows = 256;     // rows of one phase-1 item (x staged in LDS)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -117,7 +116,7 @@ __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __r
 
 // Phase 1, grouped by source box: one workgroup per item = a row range
 // [s0, s1) (at most kGRows rows) of one source box and a slice of the blocks
-// with that source (at most kSColumns columns).  x of the range (as quads,
+// with that source (at most Engine::g_cols columns, tuned).  x of the range (as quads,
 // rows outside the range zeroed) is staged once in LDS and shared by the four
 // waves, which take the slice's blocks in turn: T_b(c, j) = sum over the range
 // of V_b(c, p) x(p, j), 32 columns (two 16-column tiles) per pass, UF 16-row
@@ -212,8 +211,12 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
       }
     }
   };
+  // Both column tiles are multiplied even when the second lies past the rank
+  // (its loads are clamped, its output rows never stored): a wave-uniform
+  // branch between the last MFMA and the accumulator reads of the store let
+  // the compiler place those reads 2 wait states after the MFMA issue (it
+  // needs 10 for the fp32 16x16x4 product), so the store took stale sums.
   auto compute = [&](const Unit& u, const A (&av)[UF][CT][4], Acc (&acc)[CT][H]) {
-    const bool two = u.cb + 16 < u.r;
 #pragma unroll
     for (int st = 0; st < UF; ++st) {
       if (u.s0 + st >= nsteps) break;
@@ -226,11 +229,9 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-          if (ct == 1 && !two) break;
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
           for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
-        }
     }
   };
   auto store = [&](const Unit& u, Acc (&acc)[CT][H]) {
@@ -544,7 +545,7 @@ void build_group_tables(Engine& e) {
           int ncol = 0;
           while (q < g1) {
             const int r = lv.rank[gblk[q] - lv.bbase];
-            if (ncol > 0 && ncol + r > kSColumns) break;
+            if (ncol > 0 && ncol + r > e.g_cols) break;
             ncol += r;
             ++q;
           }
@@ -673,6 +674,13 @@ void run_group_w(Engine& e, int prec, bool acc64, const double* b, double* y, in
 
 }  // namespace
 
+void release_group_tables(Engine& e) {
+  dfree(e.d_gitems);
+  dfree(e.d_gblk);
+  dfree(e.d_rcols);
+  e.n_gitems = e.n_rcols = e.g_np = 0;
+}
+
 int block_width(const Engine& e, int nrhs, int prec, bool transpose) {
   if (transpose || prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
   return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
@@ -688,6 +696,74 @@ void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec,
     double* y = d_y + (int64_t)r0 * N;
     if (cnt <= 16) run_group_w<16>(e, prec, acc64, b, y, cnt);
     else run_group_w<32>(e, prec, acc64, b, y, cnt);
+  }
+}
+
+void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best) {
+  const int prec = mode & 0xff;
+  if (prec < 0 || prec > 2 || (mode & ~(0xff | H3D_ACC_FP64)))
+    throw Error(H3D_ERR_INVALID, "grouped products take fp64, fp32 or bf16 factor copies");
+  if (nrhs < 2) throw Error(H3D_ERR_INVALID, "nrhs must be >= 2");
+  if (reps < 1) throw Error(H3D_ERR_INVALID, "reps must be >= 1");
+  const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
+  prepare_mode(e, mode);
+  const int64_t N = e.N;
+  double* B = dalloc<double>(N * nrhs);
+  double* Y = dalloc<double>(N * nrhs);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  const int saved_w = e.rhs_block, saved_c = e.g_cols;
+  const int widths[2] = {16, 32}, cols[4] = {256, 512, 1024, 2048};
+  int bw = saved_w > 0 ? saved_w : 32, bc = saved_c;
+  try {
+    H3D_HIP(hipMemsetAsync(B, 0, sizeof(double) * N * nrhs, e.stream));
+    for (auto& x : ev) H3D_HIP(hipEventCreate(&x));
+    double bt = 1e300;
+    for (int wi = 0; wi < 2; ++wi)
+      for (int ci = 0; ci < 4; ++ci) {
+        if (ms) ms[wi * 4 + ci] = -1.0;
+        if (wi == 1 && nrhs <= 16) continue;  // groups of <= 16 run at width 16 anyway
+        e.rhs_block = widths[wi];
+        if (e.g_cols != cols[ci]) {
+          H3D_HIP(hipStreamSynchronize(e.stream));
+          release_group_tables(e);
+          e.g_cols = cols[ci];
+        }
+        matvec_block(e, B, Y, nrhs, prec, acc64);  // warm-up (builds the tables)
+        H3D_HIP(hipEventRecord(ev[0], e.stream));
+        for (int q = 0; q < reps; ++q) matvec_block(e, B, Y, nrhs, prec, acc64);
+        H3D_HIP(hipEventRecord(ev[1], e.stream));
+        H3D_HIP(hipEventSynchronize(ev[1]));
+        float t = 0;
+        H3D_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+        const double avg = t / reps;
+        if (ms) ms[wi * 4 + ci] = avg;
+        if (avg < bt) {
+          bt = avg;
+          bw = widths[wi];
+          bc = cols[ci];
+        }
+      }
+  } catch (...) {
+    (void)hipStreamSynchronize(e.stream);
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+    dfree(B);
+    dfree(Y);
+    e.rhs_block = saved_w;
+    throw;
+  }
+  H3D_HIP(hipStreamSynchronize(e.stream));
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  dfree(B);
+  dfree(Y);
+  e.rhs_block = bw;
+  if (e.g_cols != bc) {
+    release_group_tables(e);
+    e.g_cols = bc;
+  }
+  if (best) {
+    best[0] = bw;
+    best[1] = bc;
   }
 }
 

@@ -186,6 +186,14 @@ int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
  * one vector.  Transposes and the mixed copy always go one RHS at a time. */
 int h3d_set_rhs_block(h3d_handle* h, int width);
 
+/* Autotuner of the grouped path: times the product of `nrhs` (>= 2) columns
+ * in mode `prec` (an ACA copy) for group widths 16 and 32 and phase-1 items
+ * of 256, 512, 1024 and 2048 columns, `reps` products each, and keeps the
+ * fastest (the results do not depend on the choice).  ms[8] (may be NULL)
+ * receives the averages, width-major, -1 where a width does not apply;
+ * best[2] (may be NULL) the chosen width and item size. */
+int h3d_autotune_rhs(h3d_handle* h, int prec, int nrhs, int reps, double* ms, int* best);
+
 /* Pins the per-lane load widths (bytes, >= the element size; >= 4 and for
  * both classes with H3D_PREC_MIXED) for one mode. */
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);

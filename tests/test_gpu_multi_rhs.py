@@ -123,3 +123,53 @@ def test_clustered_leaves_with_many_rows():
         Y = h.matvec(Q, prec=prec)
         assert _relcols(Y, ref) <= tol, prec
     h.close()
+
+
+def test_autotune_rhs_keeps_the_product(handle):
+    """The group width and the phase-1 item size change only the schedule:
+    each column's sums keep their order (matrix-core products over the same
+    4-row steps, the same per-column near-field loop), so products are
+    bit-identical whatever the autotuner picks."""
+    x, h = handle
+    Q = np.random.default_rng(12).standard_normal((len(x), 24))
+    refs = {}
+    for w in (16, 32):
+        h.set_rhs_block(w)
+        refs[w] = {p: h.matvec(Q, prec=p) for p in ("fp32", "fp64", "bf16")}
+    for p in ("fp32", "fp64", "bf16"):
+        assert np.array_equal(refs[16][p], refs[32][p]), p
+    res = h.autotune_rhs("fp32", nrhs=24, reps=1)
+    assert res["width"] in (16, 32) and res["columns"] in (256, 512, 1024, 2048)
+    assert len(res["ms"]) == 8 and all(v > 0 for v in res["ms"].values())
+    assert np.array_equal(h.matvec(Q, prec="fp32"), refs[32]["fp32"])
+    for w in (16, 32):
+        h.set_rhs_block(w)
+        for p in ("fp32", "fp64"):
+            assert np.array_equal(h.matvec(Q, prec=p), refs[w][p]), (w, p)
+    h.set_rhs_block(32)
+    ref = h.matvec(Q, prec="fp32")
+    res = h.autotune_rhs("fp32", nrhs=24, reps=1)
+    assert res["width"] in (16, 32) and res["columns"] in (256, 512, 1024, 2048)
+    assert len(res["ms"]) == 8 and all(v > 0 for v in res["ms"].values())
+    assert np.array_equal(h.matvec(Q, prec="fp32"), ref)
+    for w, c in ((16, 256), (32, 2048)):
+        h.set_rhs_block(w)
+        assert np.array_equal(h.matvec(Q, prec="fp32"), ref), (w, c)
+    h.set_rhs_block(32)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp64", "fp32/acc64"])
+def test_grouped_products_repeat_bit_for_bit(handle, prec):
+    """A grouped product is deterministic: the same call gives the same bits.
+    The 16-wide fp32-sum phase-1 kernel once stored matrix-core accumulators
+    before the last product had completed (blocks with at most 16 columns
+    left in a pass), which showed only as run-to-run differences of ~1e-8."""
+    x, h = handle
+    Q = np.random.default_rng(13).standard_normal((len(x), 24))
+    for w in (16, 32):
+        h.set_rhs_block(w)
+        for cnt in (3, 16, 24):
+            first = h.matvec(Q[:, :cnt], prec=prec)
+            for _ in range(3):
+                assert np.array_equal(h.matvec(Q[:, :cnt], prec=prec), first), (prec, w, cnt)
+    h.set_rhs_block(32)
