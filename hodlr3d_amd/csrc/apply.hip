@@ -62,35 +62,75 @@ __device__ inline A cvt_elem(uint32_t bits, int half) {
   }
 }
 
-// NV consecutive stored elements at p (NV * sizeof(S) in {2,4,8,16} bytes)
+// NV consecutive stored elements of S (NV * sizeof(S) in {2,4,8,16} bytes) as
+// raw words.  Kernels that keep their next loads in flight hold raw vectors
+// and convert them where they are multiplied, so that the wait for a load
+// falls at its use and not right after it.
+template <int B>
+struct RawOf;
+template <>
+struct RawOf<16> {
+  using T = u32x4a;
+};
+template <>
+struct RawOf<8> {
+  using T = u32x2a;
+};
+template <>
+struct RawOf<4> {
+  using T = uint32_t;
+};
+template <>
+struct RawOf<2> {
+  using T = uint16_t;
+};
+typedef double f64x2a __attribute__((ext_vector_type(2), aligned(8)));
+template <class S, int NV>
+struct RawT {
+  using T = typename RawOf<NV * (int)sizeof(S)>::T;
+};
+template <>
+struct RawT<double, 2> {
+  using T = f64x2a;
+};
+template <>
+struct RawT<double, 1> {
+  using T = double;
+};
+template <class S, int NV>
+using Raw = typename RawT<S, NV>::T;
+
+template <class S, int NV>
+__device__ inline Raw<S, NV> raw_load(const S* p) {
+  return ntload<Raw<S, NV>>(p);
+}
+
 template <class S, class A, int NV>
-__device__ inline void load_factor(const S* p, A (&o)[NV]) {
+__device__ inline void raw_cvt(const Raw<S, NV>& v, A (&o)[NV]) {
   constexpr int B = NV * (int)sizeof(S);
   if constexpr (sizeof(S) == 8) {
     if constexpr (NV == 2) {
-      u32x4a v = ntload<u32x4a>(p);
-      o[0] = (A)__hiloint2double((int)v.y, (int)v.x);
-      o[1] = (A)__hiloint2double((int)v.w, (int)v.z);
+      o[0] = (A)v.x, o[1] = (A)v.y;
     } else {
-      o[0] = (A)ntload<double>(p);
+      o[0] = (A)v;
     }
-  } else if constexpr (B == 16) {
-    u32x4a v = ntload<u32x4a>(p);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
-  } else if constexpr (B == 8) {
-    u32x2a v = ntload<u32x2a>(p);
-    const uint32_t w[2] = {v.x, v.y};
-#pragma unroll
-    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
-  } else if constexpr (B == 4) {
-    const uint32_t v = ntload<uint32_t>(p);
-#pragma unroll
-    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(v, q & 1);
   } else {
-    o[0] = cvt_elem<S, A>((uint32_t)ntload<uint16_t>(p), 0);
+    uint32_t w[B >= 4 ? B / 4 : 1];
+    if constexpr (B == 16) {
+      w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
+    } else if constexpr (B == 8) {
+      w[0] = v.x, w[1] = v.y;
+    } else {
+      w[0] = (uint32_t)v;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) o[q] = cvt_elem<S, A>(w[q * sizeof(S) / 4], (q * sizeof(S) / 2) & 1);
   }
+}
+
+template <class S, class A, int NV>
+__device__ inline void load_factor(const S* p, A (&o)[NV]) {
+  raw_cvt<S, A, NV>(raw_load<S, NV>(p), o);
 }
 
 // ---------------------------------------------------------------- phase 1
@@ -206,48 +246,102 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
   const int cpp = 64 / lpc, slot = lane / lpc, sub = lane % lpc;
   A xv[KMAX][VEC];
   const A* xs = x + it.xoff + it.s0;
+  // All loads first, at indices inside the item's rows, then a 0/1 factor for
+  // the entries outside (a select of the loaded value would be compiled into
+  // a branch around each load, each followed by a wait for it).
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     const int s = (sub + k * lpc) * VEC;
 #pragma unroll
-    for (int q = 0; q < VEC; ++q) xv[k][q] = (k < K && s + q < rows) ? xs[s + q] : (A)0;
+    for (int q = 0; q < VEC; ++q) xv[k][q] = xs[min(s + q, rows - 1)];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int s = (sub + k * lpc) * VEC;
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) xv[k][q] *= (k < K && s + q < rows) ? (A)1 : (A)0;
   }
   const S* base = (const S*)vbase[it.vchunk] + it.voff + it.s0;
-  // UP passes at once when a lane holds one load vector per column, so that
-  // four loads per lane are in flight either way
+  // The columns are taken in units (UP column passes with one load vector per
+  // lane when K == 1, else one pass with KMAX vectors per lane), two units'
+  // loads in flight: the next unit's loads are issued before the current
+  // unit's multiplies, reduction and stores.  Loads are branch-free at
+  // in-bounds addresses (rows past the item or past K read the column's first
+  // rows and meet x = 0; columns past c1 - 1 read column c1 - 1 and are not
+  // stored); the values are consumed only by the multiplies.  Each column's
+  // sum keeps the order of one pass at a time.
   constexpr int UP = 4;
-  const int step = K == 1 ? UP * cpp : cpp;
-  for (int c0 = it.c0; c0 < it.c1; c0 += step) {
-    A acc[UP];
+  const int cols = K == 1 ? UP * cpp : cpp;  // columns per unit
+  const int nu = (it.c1 - it.c0 + cols - 1) / cols;
+  if (nu <= 0) return;  // no columns (the first prefetch would read column c0 - 1)
+  auto finish = [&](int c, A a) {
+    for (int d = lpc >> 1; d >= 1; d >>= 1) a += __shfl_xor(a, d);
+    if (c < it.c1 && sub == 0) {
+      if (it.pslot < 0) tsrc[it.tsb + c] = a;
+      else part[(int64_t)it.pslot + (c - it.c0)] = a;
+    }
+  };
+  // one straight loop per shape (ONE: K == 1), so that the loop body holds
+  // no branch between a unit's loads and their use
+  auto run = [&](auto ONE) {
+    constexpr bool one = decltype(ONE)::value;
+    constexpr int NL = one ? UP : KMAX;  // load vectors per lane and unit
+    int roff[NL];
 #pragma unroll
-    for (int u = 0; u < UP; ++u) {
-      acc[u] = 0;
-      const int c = c0 + u * cpp + slot;
-      if ((u == 0 || K == 1) && c < it.c1) {
-        const S* col = base + (int64_t)c * it.ld;
+    for (int k = 0; k < NL; ++k) {
+      const int r = (sub + k * lpc) * VEC;
+      roff[k] = one ? (sub * VEC < rows ? sub * VEC : 0) : ((k < K && r < rows) ? r : 0);
+    }
+    auto load = [&](int u, Raw<S, VEC> (&v)[NL]) {
+      const int cu = it.c0 + u * cols;
+#pragma unroll
+      for (int k = 0; k < NL; ++k) {
+        const int c = min(cu + (one ? k * cpp : 0) + slot, it.c1 - 1);
+        v[k] = raw_load<S, VEC>(base + (int64_t)c * it.ld + roff[k]);
+      }
+    };
+    auto compute = [&](int u, const Raw<S, VEC> (&v)[NL]) {
+      const int cu = it.c0 + u * cols;
+      if constexpr (one) {
+#pragma unroll
+        for (int k = 0; k < UP; ++k) {
+          A f[VEC];
+          raw_cvt<S, A, VEC>(v[k], f);
+          A a = 0;
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) a += f[q] * xv[0][q];
+          finish(cu + k * cpp + slot, a);
+        }
+      } else {
+        A a = 0;
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-          if (k < K && (sub + k * lpc) * VEC < rows) {
-            A v[VEC];
-            load_factor<S, A, VEC>(col + (sub + k * lpc) * VEC, v);
+          A f[VEC];
+          raw_cvt<S, A, VEC>(v[k], f);
 #pragma unroll
-            for (int q = 0; q < VEC; ++q) acc[u] += v[q] * xv[k][q];
-          }
+          for (int q = 0; q < VEC; ++q) a += f[q] * xv[k][q];
         }
+        finish(cu + slot, a);
       }
+    };
+    // two units per iteration, one exit at the bottom, barriers against the
+    // scheduler sinking the prefetch (an odd count's last unit lies past c1)
+    Raw<S, VEC> v0[NL], v1[NL];
+    load(0, v0);
+    for (int u = 0; u < nu; u += 2) {
+      load(min(u + 1, nu - 1), v1);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(u, v0);
+      load(min(u + 2, nu - 1), v0);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(u + 1, v1);
     }
-#pragma unroll
-    for (int u = 0; u < UP; ++u) {
-      if (u > 0 && K != 1) break;
-      A a = acc[u];
-      for (int d = lpc >> 1; d >= 1; d >>= 1) a += __shfl_xor(a, d);
-      const int c = c0 + u * cpp + slot;
-      if (c < it.c1 && sub == 0) {
-        if (it.pslot < 0) tsrc[it.tsb + c] = a;
-        else part[(int64_t)it.pslot + (c - it.c0)] = a;
-      }
-    }
-  }
+  };
+  if (K == 1)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
 }
 
 // Row-split partials of the mixed phase 1 summed in range order into t_src.
@@ -343,34 +437,58 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
       }
     }
   } else if (valid) {
+    // Units of 4G columns (lane group g takes columns u*4G + g + kG), two
+    // units' loads in flight: the next unit's loads are issued before the
+    // current unit's multiplies.  Loads are branch-free at in-bounds
+    // addresses (column clamped to R - 1, which meets t = 0; a lane past the
+    // last group repeats group G - 1, whose result is not read); the values
+    // are consumed only by the multiplies.  Each row's sum runs over its
+    // columns in increasing order, as one column at a time would.
     G = 64 / rcn;
-    const int rho = lane % rcn, g = lane / rcn;
-    if (g < G) {
-      const S* U = (const S*)ubase[it.chunk] + it.uoff + it.i0 + rho * VR;
-      const A* tt = t + it.toff;
-      const int64_t ld = it.ld;
-      int c = g;
-      for (; c + 3 * G < it.R; c += 4 * G) {
-        A v0[VR], v1[VR], v2[VR], v3[VR];
-        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
-        load_factor<S, A, VR>(U + (int64_t)(c + G) * ld, v1);
-        load_factor<S, A, VR>(U + (int64_t)(c + 2 * G) * ld, v2);
-        load_factor<S, A, VR>(U + (int64_t)(c + 3 * G) * ld, v3);
-        const A t0 = tt[c], t1 = tt[c + G], t2 = tt[c + 2 * G], t3 = tt[c + 3 * G];
+    const int rho = lane % rcn, g = min(lane / rcn, G - 1);
+    const S* U = (const S*)ubase[it.chunk] + it.uoff + it.i0 + rho * VR;
+    const A* tt = t + it.toff;
+    const int64_t ld = it.ld;
+    const int R = it.R, G4 = 4 * G;
+    const int nu = (R + G4 - 1) / G4;
+    auto load = [&](int u, Raw<S, VR> (&v)[4], A (&tv)[4]) {
 #pragma unroll
-        for (int q = 0; q < VR; ++q) {
-          acc[q] += v0[q] * t0;
-          acc[q] += v1[q] * t1;
-          acc[q] += v2[q] * t2;
-          acc[q] += v3[q] * t3;
-        }
+      for (int k = 0; k < 4; ++k) {
+        const int c = u * G4 + k * G + g;
+        const int cl = c < R ? c : R - 1;
+        v[k] = raw_load<S, VR>(U + (int64_t)cl * ld);
+        tv[k] = tt[cl];
       }
-      for (; c < it.R; c += G) {
-        A v0[VR];
-        load_factor<S, A, VR>(U + (int64_t)c * ld, v0);
-        const A t0 = tt[c];
+    };
+    auto compute = [&](int u, const Raw<S, VR> (&v)[4], const A (&tv)[4]) {
 #pragma unroll
-        for (int q = 0; q < VR; ++q) acc[q] += v0[q] * t0;
+      for (int k = 0; k < 4; ++k) {
+        const A tm = u * G4 + k * G + g < R ? tv[k] : (A)0;
+        A f[VR];
+        raw_cvt<S, A, VR>(v[k], f);
+#pragma unroll
+        for (int q = 0; q < VR; ++q) acc[q] += f[q] * tm;
+      }
+    };
+    // the prefetch is unconditional (a unit past the last one repeats the
+    // last one's addresses), so that every path holds the same loads in
+    // flight and the compiler's waits stay partial
+    // Two units per iteration and one exit at the bottom: a mid-loop exit
+    // would let the compiler move the next unit's loads past it, a buffer
+    // copy at the end would wait for them, and the scheduler would sink them
+    // below the multiplies without the barriers.  With an odd count the last
+    // iteration's second unit lies past column R - 1 and adds zeros.
+    if (nu > 0) {
+      Raw<S, VR> v0[4], v1[4];
+      A t0[4], t1[4];
+      load(0, v0, t0);
+      for (int u = 0; u < nu; u += 2) {
+        load(min(u + 1, nu - 1), v1, t1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the multiplies
+        compute(u, v0, t0);
+        load(min(u + 2, nu - 1), v0, t0);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(u + 1, v1, t1);
       }
     }
   }
@@ -735,7 +853,8 @@ void Plan::free_device() {
   dfree(d_tp2);
 }
 
-void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out) {
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
+                    int p2rows) {
   const int nL = e.nL;
   hipStream_t st = e.stream;
   out.p1.clear();
@@ -805,14 +924,16 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
           }
           flush(lv.iloff[k + 1]);
         }
-        // phase 2: rows in chunks of 128 (64 lanes x 2 fp64 rows at most)
-        for (int i0 = 0; i0 < n; i0 += kP2Rows) {
+        // phase 2: rows in chunks of p2rows (64 lanes x one 16-byte vector:
+        // 128 fp64, 256 fp32 or 512 fp16 rows), so that a wave instruction
+        // reads 1 KB of one column when the item spans that many rows
+        for (int i0 = 0; i0 < n; i0 += p2rows) {
           P2Item it{};
           it.uoff = lf.uoff[k];
           it.chunk = lv.chunk[k];
           it.ld = n + (n & 1);
           it.i0 = i0;
-          it.nr = std::min(kP2Rows, n - i0);
+          it.nr = std::min(p2rows, n - i0);
           it.R = R;
           it.toff = lf.toff[k];
           it.yoff = e.bstart[lo + k];
@@ -869,7 +990,7 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
 void build_plan(Engine& e) {
   std::vector<const LevelFactors*> lfs(e.nL + 1, nullptr);
   for (int j = 1; j <= e.nL; ++j) lfs[j] = &e.lev[j];
-  build_plan_for(e, lfs, e.plan);
+  build_plan_for(e, lfs, e.plan, kP2Rows);
   e.nitems.clear();
   const int64_t lo = level_offset(e.nL);
   for (int64_t k = 0; k < e.lev[e.nL].nboxes; ++k) {

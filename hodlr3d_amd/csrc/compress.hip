@@ -1063,7 +1063,7 @@ void build_mixed(Engine& e, double tol) {
       fc.d_btoff = upload(btoff[c], st);
       std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
       for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
-      build_plan_for(e, lfs, fc.plan);
+      build_plan_for(e, lfs, fc.plan, c == 0 ? 256 : 512);  // fp32 / fp16 storage
       fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
       fc.plan.p1r.clear();
       fc.d_mp1 = upload(fc.mp1, st);

@@ -85,7 +85,7 @@ struct P2Item {
   int32_t chunk;
   int32_t ld;     // column stride (= box size)
   int32_t i0;     // first row of this chunk
-  int32_t nr;     // rows in this chunk (<= 64)
+  int32_t nr;     // rows in this chunk (<= the plan's rows per item)
   int32_t R;      // columns
   int32_t toff;   // offset of the box's t values
   int32_t yoff;   // position of the box's row 0
@@ -387,7 +387,9 @@ void build_mixed(Engine& e, double tol);
 void release_mixed(Engine& e);
 // Plan of one copy from its per-level factor bookkeeping (apply.hip).
 // lf[j] describes level j (1..nLevels).
-void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out);
+// p2rows: rows of one phase-2 item (128 for the ACA copies, whose plan serves
+// every precision; 256 / 512 for the fp32 / fp16 classes of the mixed copy)
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out, int p2rows);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);
