@@ -220,7 +220,8 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
 // ---------------------------------------------------------- mixed phase 1
 // t = V^T x for one class of the recompressed copy, grouped by source box: an
 // item is a run of columns of V_s (the columns of every block with source s,
-// concatenated) over at most 512 rows, so x is loaded once per item.  lpc lanes
+// concatenated) over at most 64 x 4 load vectors of rows (1024 fp32 or 2048
+// fp16 rows with 16-byte loads), so x is loaded once per item.  lpc lanes
 // share a column, each holding K <= kmax load vectors of VEC rows (x in
 // registers), and reduce with an xor butterfly.  t is written in source order
 // (t_src[tsb + c], contiguous stores); a gather puts it in target order.  A
@@ -232,14 +233,16 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
                                                       const A* __restrict__ x,
                                                       A* __restrict__ tsrc, A* __restrict__ part,
                                                       int kmax) {
-  constexpr int KMAX = VEC >= 2 ? 4 : 8;  // 64 lanes x KMAX x VEC >= 512 rows
+  constexpr int KMAX = VEC >= 2 ? 4 : 8;  // 64 lanes x KMAX x VEC >= the item's rows
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const MixP1Item it = items[w];
   const int rows = it.s1 - it.s0;
   const int need = (rows + VEC - 1) / VEC;
-  const int kk = min(max(kmax, 1), KMAX);
+  // kmax 0: by item, lanes across the rows (one vector each) for items of
+  // at most 32 vectors, so that a wave instruction reads whole columns
+  const int kk = kmax > 0 ? min(kmax, KMAX) : (need <= 32 ? 1 : KMAX);
   int lpc = 1;
   while (lpc * kk < need && lpc < 64) lpc <<= 1;
   const int K = (need + lpc - 1) / lpc;
@@ -1136,21 +1139,12 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   const FactorClass& fc = *f.fc;
   const int n1 = (int)fc.mp1.size();
   if (!n1) return;
-  const unsigned g = wave_grid(n1);
-  hipStream_t st = e.stream;
+  // the items span up to 64 lanes x 4 vectors x 16 bytes of rows, which only
+  // the 16-byte loads cover
+  if (bytes != 16) throw Error(H3D_ERR_INVALID, "the mixed phase 1 takes 16-byte loads");
   constexpr int es = (int)sizeof(S);
-  const MixP1Item* it = fc.d_mp1;
-  if (bytes >= 16)
-    mphase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, x, (A*)f.tsrc, part, kmax);
-  else if (bytes == 8)
-    mphase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
-                                                                       (A*)f.tsrc, part, kmax);
-  else if (bytes == 4)
-    mphase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
-                                                                       (A*)f.tsrc, part, kmax);
-  else
-    mphase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, x,
-                                                                       (A*)f.tsrc, part, kmax);
+  mphase1_kernel<S, A, 16 / es><<<wave_grid(n1), 256, 0, e.stream>>>(fc.d_mp1, n1, f.vbase, x,
+                                                                     (A*)f.tsrc, part, kmax);
   H3D_LAUNCH_CHECK();
 }
 
@@ -1273,10 +1267,10 @@ void tune_mode(Engine& e, const FactorView& f, int reps, const A* x, A* t, A* pa
     for (int q = 0; q < 4; ++q) {
       const int w = widths[q];
       double v = -1;
-      if (w >= (int)sizeof(S)) {
-        const int nk = (which == 1 && f.fc) ? 2 : 1;
+      if (w >= (int)sizeof(S) && !(which == 1 && f.fc && w != 16)) {
+        const int nk = (which == 1 && f.fc) ? 3 : 1;
         for (int ki = 0; ki < nk; ++ki) {
-          if (nk == 2) e.tune_k[f.key] = ki == 0 ? 4 : 1;
+          if (nk == 3) e.tune_k[f.key] = ki == 0 ? 4 : ki == 1 ? 1 : 0;
           const double tv = time_phase<S, A>(e, f, which, w, reps, x, t, part);
           if (v < 0 || tv < v) v = tv;
           if (tv < best) {

@@ -309,6 +309,8 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
     auto ok = [&](int b) { return (b == 16 || b == 8 || b == 4 || b == 2) && b >= es; };
     if (!ok(p1_bytes) || !ok(p2_bytes)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
     if (p == H3D_PREC_MIXED) {  // both classes: fp32 (slot 6) and fp16 (slot 7)
+      if (p1_bytes != 16)
+        throw h3d::Error(H3D_ERR_INVALID, "the mixed phase 1 takes 16-byte loads");
       for (int key : {6, 7}) {
         h->e.tune_p1[key] = p1_bytes;
         h->e.tune_p2[key] = p2_bytes;

@@ -45,8 +45,10 @@ constexpr double kCholFail = 1e-12;       // pivot / original diagonal below thi
 constexpr int kMaxSweeps = 24;
 constexpr int kBinsPerOctave = 8;
 constexpr double kColShare = 1.0 / 4096;  // largest share of the error budget one column may take
-constexpr int kMixRows = 512;                       // rows of a mixed phase-1 item
-constexpr int64_t kMixItemBytes = 32768;            // factor bytes of a mixed phase-1 item
+// rows of a mixed phase-1 item: 64 lanes x 4 vectors of 16 bytes (the phase-1
+// kernel's reach with 16-byte loads), 1024 fp32 or 2048 fp16 rows
+constexpr int kMixItemRowBytes = 64 * 4 * 16;
+constexpr int64_t kMixItemBytes = 131072;           // factor bytes of a mixed phase-1 item
 constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (whole source boxes)
 
 // Per-block work description (pass 1 and pass 2).
@@ -922,13 +924,14 @@ void build_mixed(Engine& e, double tol) {
             col += r;
           }
           fc.nV.back() += C * ld;
-          // phase-1 items: rows in ranges of <= 512, columns in runs of
+          // phase-1 items: rows in ranges of <= kMixRows, columns in runs of
           // about kMixItemBytes of factors
           if (m == 0 || C == 0) continue;
+          const int kMixRows = kMixItemRowBytes / fc.elem;
           const int nsplit = (m + kMixRows - 1) / kMixRows;
           const int rows0 = std::min(m, kMixRows);
           const int ncol = (int)std::max<int64_t>(
-              16, std::min<int64_t>(1024, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
+              16, std::min<int64_t>(4096, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
           for (int32_t c0 = 0; c0 < C; c0 += ncol) {
             const int32_t c1 = (int32_t)std::min<int64_t>(C, (int64_t)c0 + ncol);
             int32_t pbase = -1;

@@ -194,8 +194,10 @@ int h3d_set_rhs_block(h3d_handle* h, int width);
  * best[2] (may be NULL) the chosen width and item size. */
 int h3d_autotune_rhs(h3d_handle* h, int prec, int nrhs, int reps, double* ms, int* best);
 
-/* Pins the per-lane load widths (bytes, >= the element size; >= 4 and for
- * both classes with H3D_PREC_MIXED) for one mode. */
+/* Pins the per-lane load widths (bytes, >= the element size) for one mode.
+ * With H3D_PREC_MIXED the widths apply to both classes, phase 1 takes 16
+ * bytes only (its items span 64 lanes x 4 vectors of 16 bytes of rows) and
+ * phase 2 at least 4. */
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
