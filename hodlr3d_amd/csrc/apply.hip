@@ -623,10 +623,17 @@ __global__ __launch_bounds__(256) void near64_kernel(
   yL[ip] += acc;
 }
 
-// fp32 entries: coordinates as fp32 offsets from the leaf centre plus the fp32
-// difference of the two leaf centres, so near pairs keep ~1e-7 relative
-// accuracy in r.  A is the type of x and of the sums (float, or double for
-// fp64 accumulation of the fp32 entries).
+// Near field with fp32 entries: coordinates as fp32 offsets from the leaf
+// centre plus the fp32 difference of the two leaf centres, so near pairs keep
+// ~1e-7 relative accuracy in r.  A is the type of x and of the sums (float, or
+// double for fp64 accumulation of the fp32 entries).  One wave
+// per item (<= 64 target rows of a leaf).  The sources come in chunks of <= 64
+// (one coalesced load per lane) staged in the wave's own LDS slice; an item of
+// at most 32 rows splits each chunk between the half-waves (lanes 0-31 the
+// even sources, lanes 32-63 the odd ones, for the same targets; the halves
+// are added at the end), so that every lane works.  A wave's LDS operations
+// complete in order and no other wave uses its slice, so the staging needs no
+// barrier, only to stay in program order (wave barriers for the compiler).
 template <class A, class KF>
 __global__ __launch_bounds__(256) void near32_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
@@ -634,13 +641,20 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
     const double* __restrict__ leafc, const int32_t* __restrict__ perm, KernelDev kd,
     const A* __restrict__ x, double* __restrict__ yL) {
+  __shared__ float4 ls[kWavesPerBlock][64];
+  __shared__ A lq[kWavesPerBlock][64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const NearItem it = items[w];
+  const bool two = it.nr <= 32;
+  const int tl = two ? (lane & 31) : lane;  // target of this lane
+  const int h = two ? (lane >> 5) : 0;      // first source of this lane in a chunk
+  const int step = two ? 2 : 1;
   const int rs = bstart[it.leaf];
-  const bool act = lane < it.nr;
-  const int ip = rs + it.i0 + (act ? lane : 0);
+  const bool act = tl < it.nr;
+  const int ip = rs + it.i0 + (act ? tl : 0);
   const float xi = lx[ip], yi = ly[ip], zi = lz[ip];
   const double cxk = leafc[3 * (int64_t)it.leaf], cyk = leafc[3 * (int64_t)it.leaf + 1],
                czk = leafc[3 * (int64_t)it.leaf + 2];
@@ -654,27 +668,48 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const float dcz = (float)(czk - leafc[3 * (int64_t)nn + 2]);
     const float ax = xi + dcx, ay = yi + dcy, az = zi + dcz;
     A part = 0;
-    // sources in runs of 64: one coalesced load per lane, then broadcast by
-    // readlane (few VALU per pair here, so scalar loads would expose latency)
-    for (int s0 = 0; s0 < m; s0 += 64) {
-      const int cnt = min(64, m - s0);
-      const int jp = cs + s0 + (lane < cnt ? lane : 0);
-      const float sx = lx[jp], sy = ly[jp], sz = lz[jp];
-      const A sq = x[jp];
-      for (int u = 0; u < cnt; ++u) {
-        const float dx = ax - readlane_f(sx, u), dy = ay - readlane_f(sy, u),
-                    dz = az - readlane_f(sz, u);
-        const float R2 = dx * dx + dy * dy + dz * dz;
-        const float v = cs + s0 + u == ip ? 0.0f : KF::f(kd.q, R2);
-        A Q;
-        if constexpr (sizeof(A) == 8) Q = readlane_d(sq, u);
-        else Q = readlane_f(sq, u);
-        part += (A)v * Q;
+    // the self pair can only lie in the leaf's own box (SELF)
+    auto sweep = [&](auto SELF) {
+      constexpr bool self_box = decltype(SELF)::value;
+      for (int s0 = 0; s0 < m; s0 += 64) {
+        const int cnt = min(64, m - s0);
+        const int jp = cs + s0 + (lane < cnt ? lane : 0);
+        const float4 sp = make_float4(lx[jp], ly[jp], lz[jp], 0.0f);
+        const A sq = x[jp];
+        __builtin_amdgcn_wave_barrier();  // after the previous chunk's reads
+        ls[wv][lane] = sp;
+        lq[wv][lane] = sq;
+        __builtin_amdgcn_wave_barrier();  // before this chunk's reads
+        // four sources per step (their LDS reads issued together); the self
+        // pair is selected away after its entry is formed (no branch)
+        const int self = ip - cs - s0;
+        auto pair = [&](const float4& sv, A q, int u) {
+          const float dx = ax - sv.x, dy = ay - sv.y, dz = az - sv.z;
+          const float R2 = dx * dx + dy * dy + dz * dz;
+          float kv = KF::f(kd.q, R2);
+          if constexpr (self_box) kv = u == self ? 0.0f : kv;
+          part += (A)kv * q;
+        };
+        int u = h;
+        for (; u + 3 * step < cnt; u += 4 * step) {
+          float4 sv[4];
+          A q[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) sv[k] = ls[wv][u + k * step], q[k] = lq[wv][u + k * step];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) pair(sv[k], q[k], u + k * step);
+        }
+        for (; u < cnt; u += step) pair(ls[wv][u], lq[wv][u], u);
       }
-    }
+    };
+    if (nn == it.leaf)
+      sweep(std::true_type{});
+    else
+      sweep(std::false_type{});
     acc += part;
   }
-  if (!act) return;
+  if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
+  if (!act || h != 0) return;
   yL[ip] += kd.scale * (double)acc;
 }
 
