@@ -583,14 +583,18 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
 namespace {
 
 // ---------------------------------------------------------------- near field
-// One wave per <= 64 rows of a leaf; lanes own target rows and walk every
-// source point of the leaf and its face/edge neighbours in order; the i == j
-// term is removed by a select.
+// One wave per <= 64 rows of a leaf; lanes own target rows and walk the source
+// points of the leaf and its face/edge neighbours; the i == j term is removed
+// by a select in the leaf's own box (the diagonal enters in the combine).
 //
-// fp64: exact entries (the reference's arithmetic).  The source points come in
-// as scalar loads (the wave index is uniform): with ~40 VALU per pair their
-// latency is hidden, and no per-pair broadcast is needed.  The i == j term uses the
-// diagonal (reference: 0).
+// fp64: exact entries (the reference's arithmetic, scale * K(r^2) * x_j per
+// pair).  The sources come in chunks of <= 64 (one coalesced load per lane)
+// staged in the wave's own LDS slice; an item of at most 32 rows splits each
+// chunk between the half-waves (lanes 0-31 the even sources, lanes 32-63 the
+// odd ones, for the same targets; the halves are added at the end), so that
+// every lane works.  A wave's LDS operations complete in order and no other
+// wave uses its slice, so the staging needs no barrier, only to stay in
+// program order (wave barriers for the compiler).
 template <class KF>
 __global__ __launch_bounds__(256) void near64_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
@@ -598,28 +602,63 @@ __global__ __launch_bounds__(256) void near64_kernel(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const int32_t* __restrict__ perm, KernelDev kd, const double* __restrict__ x,
     double* __restrict__ yL) {
+  __shared__ double2 lxy[kWavesPerBlock][64];
+  __shared__ double2 lzq[kWavesPerBlock][64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const NearItem it = items[w];
+  const bool two = it.nr <= 32;
+  const int tl = two ? (lane & 31) : lane;  // target of this lane
+  const int h = two ? (lane >> 5) : 0;      // first source of this lane in a chunk
+  const int step = two ? 2 : 1;
   const int rs = bstart[it.leaf];
-  const bool act = lane < it.nr;
-  const int ip = rs + it.i0 + (act ? lane : 0);
+  const bool act = tl < it.nr;
+  const int ip = rs + it.i0 + (act ? tl : 0);
   const double xi = px[ip], yi = py[ip], zi = pz[ip];
   double acc = 0.0;
   for (int sl = 0; sl <= 26; ++sl) {
     const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
     if (nn < 0) continue;
     const int cs = bstart[nn], m = bcount[nn];
-#pragma unroll 2
-    for (int j = cs; j < cs + m; ++j) {
-      const double dx = xi - px[j], dy = yi - py[j], dz = zi - pz[j];
-      const double R2 = dx * dx + dy * dy + dz * dz;
-      const double v = kd.scale * KF::f64(kd.q, R2) * x[j];
-      acc += j == ip ? 0.0 : v;
-    }
+    // the self pair can only lie in the leaf's own box (SELF)
+    auto sweep = [&](auto SELF) {
+      constexpr bool self_box = decltype(SELF)::value;
+      for (int s0 = 0; s0 < m; s0 += 64) {
+        const int cnt = min(64, m - s0);
+        const int jp = cs + s0 + (lane < cnt ? lane : 0);
+        const double2 sxy = make_double2(px[jp], py[jp]), szq = make_double2(pz[jp], x[jp]);
+        __builtin_amdgcn_wave_barrier();  // after the previous chunk's reads
+        lxy[wv][lane] = sxy;
+        lzq[wv][lane] = szq;
+        __builtin_amdgcn_wave_barrier();  // before this chunk's reads
+        const int self = ip - cs - s0;
+        auto pair = [&](const double2& a, const double2& b, int u) {
+          const double dx = xi - a.x, dy = yi - a.y, dz = zi - b.x;
+          const double R2 = dx * dx + dy * dy + dz * dz;
+          const double v = kd.scale * KF::f64(kd.q, R2) * b.y;
+          if constexpr (self_box) acc += u == self ? 0.0 : v;
+          else acc += v;
+        };
+        int u = h;
+        for (; u + 3 * step < cnt; u += 4 * step) {
+          double2 a[4], b[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) a[k] = lxy[wv][u + k * step], b[k] = lzq[wv][u + k * step];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) pair(a[k], b[k], u + k * step);
+        }
+        for (; u < cnt; u += step) pair(lxy[wv][u], lzq[wv][u], u);
+      }
+    };
+    if (nn == it.leaf)
+      sweep(std::true_type{});
+    else
+      sweep(std::false_type{});
   }
-  if (!act) return;
+  if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
+  if (!act || h != 0) return;
   yL[ip] += acc;
 }
 
