@@ -39,9 +39,10 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (one lane-instruction per clock)
 VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # VALU instructions per near-field pair in the inner loop of the near kernel each
-# mode uses, counted in the gfx950 assembly of this build (hipcc -S of apply.hip)
-NEAR_VALU_PER_PAIR = {"near32_kernel<double, InvR32>": 14, "near32_kernel<float, InvR32>": 13,
-                      "near_nr_kernel": 18, "near64_kernel<InvR>": 41}
+# mode uses, counted in the gfx950 assembly of this build (hipcc -S of apply.hip;
+# the four-pair step of the sweep over neighbour boxes other than the leaf's own)
+NEAR_VALU_PER_PAIR = {"near32_kernel<double, InvR32>": 13, "near32_kernel<float, InvR32>": 12,
+                      "near_nr_kernel": 17, "near64_kernel<InvR>": 39}
 NEAR_KERNEL = {"mixed": "near32_kernel<double, InvR32>", "fp32": "near32_kernel<float, InvR32>",
                "bf16": "near32_kernel<float, InvR32>", "fp32/acc64": "near_nr_kernel",
                "bf16/acc64": "near_nr_kernel", "fp64": "near64_kernel<InvR>"}

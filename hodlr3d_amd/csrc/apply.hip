@@ -756,38 +756,72 @@ __global__ __launch_bounds__(256) void near32_kernel(
 // square root of r^2 refined by one Newton step in fp64,
 //   y1 = y0 (3 - r^2 y0^2) / 2,
 // which squares its relative error (~1e-7 -> ~2e-14) at the cost of three
-// fp64 operations instead of an fp64 square root and division.  Source points
-// are read through wave-uniform addresses (scalar loads).
+// fp64 operations instead of an fp64 square root and division.
 __global__ __launch_bounds__(256) void near_nr_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     KernelDev kd, const double* __restrict__ x, double* __restrict__ yL) {
-  // the wave index is uniform: item, neighbour and source-point loads become scalar
+  // sources staged per chunk in the wave's LDS slice and split between the
+  // half-waves for items of <= 32 rows, as in near64_kernel
+  __shared__ double2 lxy[kWavesPerBlock][64];
+  __shared__ double2 lzq[kWavesPerBlock][64];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
   const NearItem it = items[w];
+  const bool two = it.nr <= 32;
+  const int tl = two ? (lane & 31) : lane;
+  const int h = two ? (lane >> 5) : 0;
+  const int step = two ? 2 : 1;
   const int rs = bstart[it.leaf];
-  const bool act = lane < it.nr;
-  const int ip = rs + it.i0 + (act ? lane : 0);
+  const bool act = tl < it.nr;
+  const int ip = rs + it.i0 + (act ? tl : 0);
   const double xi = px[ip], yi = py[ip], zi = pz[ip];
   double acc = 0.0;
   for (int sl = 0; sl <= 26; ++sl) {
     const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
     if (nn < 0) continue;
     const int cs = bstart[nn], m = bcount[nn];
-#pragma unroll 4
-    for (int j = cs; j < cs + m; ++j) {
-      const double dx = xi - px[j], dy = yi - py[j], dz = zi - pz[j];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      // raw v_rsq_f32 on a normal fp32 argument (r^2 below FLT_MIN is clamped)
-      const double y0 = (double)__builtin_amdgcn_rsqf(fmaxf((float)r2, 1.17549435e-38f));
-      const double y1 = y0 * (1.5 - 0.5 * r2 * y0 * y0);
-      acc += (j == ip ? 0.0 : y1) * x[j];
-    }
+    auto sweep = [&](auto SELF) {
+      constexpr bool self_box = decltype(SELF)::value;
+      for (int s0 = 0; s0 < m; s0 += 64) {
+        const int cnt = min(64, m - s0);
+        const int jp = cs + s0 + (lane < cnt ? lane : 0);
+        const double2 sxy = make_double2(px[jp], py[jp]), szq = make_double2(pz[jp], x[jp]);
+        __builtin_amdgcn_wave_barrier();  // after the previous chunk's reads
+        lxy[wv][lane] = sxy;
+        lzq[wv][lane] = szq;
+        __builtin_amdgcn_wave_barrier();  // before this chunk's reads
+        const int self = ip - cs - s0;
+        auto pair = [&](const double2& a, const double2& b, int u) {
+          const double dx = xi - a.x, dy = yi - a.y, dz = zi - b.x;
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          // raw v_rsq_f32 on a normal fp32 argument (r^2 below FLT_MIN is clamped)
+          const double y0 = (double)__builtin_amdgcn_rsqf(fmaxf((float)r2, 1.17549435e-38f));
+          double y1 = y0 * (1.5 - 0.5 * r2 * y0 * y0);
+          if constexpr (self_box) y1 = u == self ? 0.0 : y1;
+          acc += y1 * b.y;
+        };
+        int u = h;
+        for (; u + 3 * step < cnt; u += 4 * step) {
+          double2 a[4], b[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) a[k] = lxy[wv][u + k * step], b[k] = lzq[wv][u + k * step];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) pair(a[k], b[k], u + k * step);
+        }
+        for (; u < cnt; u += step) pair(lxy[wv][u], lzq[wv][u], u);
+      }
+    };
+    if (nn == it.leaf)
+      sweep(std::true_type{});
+    else
+      sweep(std::false_type{});
   }
-  if (!act) return;
+  if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
+  if (!act || h != 0) return;
   yL[ip] += kd.scale * acc;
 }
 
