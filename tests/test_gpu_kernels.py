@@ -41,3 +41,28 @@ def test_catalogue_matches_oracle(q):
         for j in range(1, h.nLevels + 1):
             assert np.array_equal(h.ranks(j)[0], o.ranks(j)[0]), (q, j)
     h.close()
+
+
+@pytest.mark.parametrize("q", list(range(16)))
+def test_catalogue_reduced_precision_near_field(q):
+    """fp32 factors with fp32 sums and the mixed copy use the fp32-entry near
+    field (every catalogue kernel through its fp32 evaluation, the i == j pair
+    selected away in the leaf's own box; ~6 points per leaf here, so the sources
+    of each leaf are split between the two half-waves).  Their products stay
+    finite and agree with the fp64 product to their rounding."""
+    import hodlr3d_amd as H3
+    x = uniform(3000, 60 + q)
+    print(f"[Qchoice {q}] N={len(x)}", flush=True)
+    h = H3.HODLR3D(4, 1.0, TOL_POW, q, x)
+    h.assemble()
+    b = np.random.default_rng(100 + q).standard_normal(len(x))
+    y = h.matvec(b)
+    ny = np.linalg.norm(y)
+    y32 = h.matvec(b, prec="fp32")
+    assert np.all(np.isfinite(y32)), q
+    assert np.linalg.norm(y32 - y) <= 1e-4 * ny, q
+    h.compress(1e-6)
+    ym = h.matvec(b, prec="mixed")
+    assert np.all(np.isfinite(ym)), q
+    assert np.linalg.norm(ym - y) <= 1e-4 * ny, q
+    h.close()
