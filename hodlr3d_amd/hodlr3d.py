@@ -264,7 +264,8 @@ class HODLR3D:
         return d
 
     def set_tuning(self, prec, p1_bytes, p2_bytes):
-        """Pin the per-lane load widths of the two far-field kernels for `prec`."""
+        """Pin the per-lane load widths (bytes) of the two far-field kernels for
+        `prec`; for 'mixed' phase 1 takes 16 only (H3DError otherwise)."""
         nat.check(nat.lib().h3d_set_tuning(self._h, _prec(prec), int(p1_bytes), int(p2_bytes)))
 
     def set_rhs_block(self, width):
