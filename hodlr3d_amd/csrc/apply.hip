@@ -601,7 +601,7 @@ __global__ __launch_bounds__(256) void near64_kernel(
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const int32_t* __restrict__ perm, KernelDev kd, const double* __restrict__ x,
-    double* __restrict__ yL) {
+    double* __restrict__ yN) {
   __shared__ double2 lxy[kWavesPerBlock][64];
   __shared__ double2 lzq[kWavesPerBlock][64];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(256) void near64_kernel(
   }
   if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
   if (!act || h != 0) return;
-  yL[ip] += acc;
+  yN[ip] = acc;
 }
 
 // Near field with fp32 entries: coordinates as fp32 offsets from the leaf
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(256) void near32_kernel(
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
     const double* __restrict__ leafc, const int32_t* __restrict__ perm, KernelDev kd,
-    const A* __restrict__ x, double* __restrict__ yL) {
+    const A* __restrict__ x, double* __restrict__ yN) {
   __shared__ float4 ls[kWavesPerBlock][64];
   __shared__ A lq[kWavesPerBlock][64];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(256) void near32_kernel(
   }
   if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
   if (!act || h != 0) return;
-  yL[ip] += kd.scale * (double)acc;
+  yN[ip] = kd.scale * (double)acc;
 }
 
 // 1/r near field with fp64 coordinates and sums: 1/r is the fp32 reciprocal
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void near_nr_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    KernelDev kd, const double* __restrict__ x, double* __restrict__ yL) {
+    KernelDev kd, const double* __restrict__ x, double* __restrict__ yN) {
   // sources staged per chunk in the wave's LDS slice and split between the
   // half-waves for items of <= 32 rows, as in near64_kernel
   __shared__ double2 lxy[kWavesPerBlock][64];
@@ -822,14 +822,15 @@ __global__ __launch_bounds__(256) void near_nr_kernel(
   }
   if (two) acc += __shfl_xor(acc, 32);  // the other half's sources
   if (!act || h != 0) return;
-  yL[ip] += kd.scale * acc;
+  yN[ip] = kd.scale * acc;
 }
 
-// y[i] = alpha * (sum over levels of y_j at i's position) + d_i * b[i], where
-// d_i is the diagonal entry K(i,i) (per point when diag_vec is set): the
-// collect and reorder steps of src/HODLR3DTree.cpp:1781-1805 plus the i == j
-// term of the self block.
+// y[i] = alpha * (sum over levels of y_j at i's position + the near field
+// there) + d_i * b[i], where d_i is the diagonal entry K(i,i) (per point when
+// diag_vec is set): the collect and reorder steps of
+// src/HODLR3DTree.cpp:1781-1805 plus the i == j term of the self block.
 __global__ void combine_kernel(const double* __restrict__ ylev, int nL, int64_t N,
+                               const double* __restrict__ ynear,
                                const int32_t* __restrict__ iperm, const double* __restrict__ b,
                                double alpha, double diag, const double* __restrict__ diag_vec,
                                double* __restrict__ y) {
@@ -838,6 +839,7 @@ __global__ void combine_kernel(const double* __restrict__ ylev, int nL, int64_t 
   const int64_t s = iperm[i];
   double acc = 0.0;
   for (int j = 0; j < nL; ++j) acc = acc + ylev[(int64_t)j * N + s];
+  if (ynear) acc = acc + ynear[s];
   const double d = diag_vec ? diag_vec[i] : diag;
   y[i] = alpha * acc + d * b[i];
 }
@@ -1140,6 +1142,13 @@ void ensure_scratch(Engine& e, int nrhs) {
   e.d_part = dalloc<double>(np * nrhs);
   e.d_ylev = dalloc<double>(e.N * e.nL * nrhs);
   e.d_io = dalloc<double>(2 * e.N * nrhs);
+  dfree(e.d_ynear);
+  e.d_ynear = dalloc<double>(e.N);
+  if (!e.side) {  // the near field's stream (default priority) and its fork / join events
+    H3D_HIP(hipStreamCreateWithFlags(&e.side, hipStreamNonBlocking));
+    H3D_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    H3D_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+  }
   e.max_nrhs = nrhs;
   e.t_cap = T;
   e.part_cap = np;
@@ -1507,6 +1516,51 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
   if (e.timing && !e.ev[0])
     for (auto& ev : e.ev) H3D_HIP(hipEventCreate(&ev));
   const bool mixed = prec == kPrecMixed;
+  // Near field into its own buffer by position (symmetric kernels: the same
+  // for A^T).  Without per-kernel timing it runs on the side stream, forked
+  // after the gather and joined before the combine, so that this VALU-bound
+  // kernel overlaps the HBM-bound far field; with timing, and while a graph
+  // of products is captured, it runs in line.
+  const int nn = (int)e.nitems.size();
+  const bool overlap = nn && !e.timing && e.overlap_near;
+  auto launch_near = [&](hipStream_t ns) {
+    const int64_t lo = level_offset(e.nL);
+    const KernelDev kd = e.kern;
+    const unsigned g = wave_grid(nn);
+    // mixed: fp32 entries summed in fp64 (their error, ~2e-8 of ||A x|| at
+    // 2^20 points, sits below the copy's fp32 rounding floor); fp32/bf16
+    // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
+    // fp64: the exact entries
+    const bool invr = kd.q == 7 || kd.q == 16;
+    if (mixed && invr)
+      near32_kernel<double, InvR32><<<g, 256, 0, ns>>>(
+          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
+    else if (mixed)
+      near32_kernel<double, AnyQ32><<<g, 256, 0, ns>>>(
+          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
+    else if (acc64 && prec != H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
+      near_nr_kernel<<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                        e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd, e.d_xp, e.d_ynear);
+    else if (acc64 && (kd.q == 7 || kd.q == 16))
+      near64_kernel<InvR><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                             e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
+                                             kd, e.d_xp, e.d_ynear);
+    else if (acc64)
+      near64_kernel<AnyQ><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                             e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
+                                             kd, e.d_xp, e.d_ynear);
+    else if (invr)
+      near32_kernel<float, InvR32><<<g, 256, 0, ns>>>(
+          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, e.d_ynear);
+    else
+      near32_kernel<float, AnyQ32><<<g, 256, 0, ns>>>(
+          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, e.d_ynear);
+    H3D_LAUNCH_CHECK();
+  };
   for (int r = 0; r < nrhs; ++r) {
     const double* b = d_b + (int64_t)r * N;
     double* y = d_y + (int64_t)r * N;
@@ -1517,6 +1571,12 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       const int64_t n = N * e.nL;
       zero_kernel<<<(unsigned)std::min<int64_t>(4096, (n + 255) / 256), 256, 0, st>>>(e.d_ylev, n);
       H3D_LAUNCH_CHECK();
+    }
+    if (overlap) {
+      H3D_HIP(hipEventRecord(e.ev_fork, st));
+      H3D_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));
+      launch_near(e.side);
+      H3D_HIP(hipEventRecord(e.ev_join, e.side));
     }
     // event slots: 1 phase 1 start, 8 after the first copy's phase-1 kernel,
     // 6 after its phase 1 (reduce, gather), 9 after the second copy's phase-1
@@ -1555,50 +1615,11 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       }
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[3], st));
-    // near field into the leaf level (symmetric kernels: the same for A^T)
-    const int nn = (int)e.nitems.size();
-    double* yL = e.d_ylev + (int64_t)(e.nL - 1) * N;
-    const int64_t lo = level_offset(e.nL);
-    KernelDev kd = e.kern;
-    if (nn) {
-      const unsigned g = wave_grid(nn);
-      // mixed: fp32 entries summed in fp64 (their error, ~2e-8 of ||A x|| at
-      // 2^20 points, sits below the copy's fp32 rounding floor); fp32/bf16
-      // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
-      // fp64: the exact entries
-      const bool invr = kd.q == 7 || kd.q == 16;
-      if (mixed && invr)
-        near32_kernel<double, InvR32><<<g, 256, 0, st>>>(
-            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, yL);
-      else if (mixed)
-        near32_kernel<double, AnyQ32><<<g, 256, 0, st>>>(
-            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, yL);
-      else if (acc64 && prec != H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
-        near_nr_kernel<<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                          e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd, e.d_xp, yL);
-      else if (acc64 && (kd.q == 7 || kd.q == 16))
-        near64_kernel<InvR><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
-                                               kd, e.d_xp, yL);
-      else if (acc64)
-        near64_kernel<AnyQ><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
-                                               kd, e.d_xp, yL);
-      else if (invr)
-        near32_kernel<float, InvR32><<<g, 256, 0, st>>>(
-            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, yL);
-      else
-        near32_kernel<float, AnyQ32><<<g, 256, 0, st>>>(
-            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, yL);
-      H3D_LAUNCH_CHECK();
-    }
+    if (nn && !overlap) launch_near(st);
+    if (overlap) H3D_HIP(hipStreamWaitEvent(st, e.ev_join, 0));
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[4], st));
-    combine_kernel<<<gN, 256, 0, st>>>(e.d_ylev, e.nL, N, e.d_iperm, b, e.alpha, e.kern.diag,
-                                       e.d_diag, y);
+    combine_kernel<<<gN, 256, 0, st>>>(e.d_ylev, e.nL, N, nn ? e.d_ynear : nullptr, e.d_iperm, b,
+                                       e.alpha, e.kern.diag, e.d_diag, y);
     H3D_LAUNCH_CHECK();
     if (e.timing) {
       H3D_HIP(hipEventRecord(e.ev[5], st));

@@ -57,6 +57,12 @@ void Engine::free_all() {
   dfree(d_t);
   dfree(d_part);
   dfree(d_ylev);
+  dfree(d_ynear);
+  if (side) (void)hipStreamSynchronize(side);
+  for (hipEvent_t* x : {&ev_fork, &ev_join})
+    if (*x) (void)hipEventDestroy(*x), *x = nullptr;
+  if (side) (void)hipStreamDestroy(side);
+  side = nullptr;
   dfree(d_io);
   dfree(d_xw);
   dfree(d_tw);
@@ -727,11 +733,14 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       saved[i] = hs[i]->e.stream;
       timing[i] = hs[i]->e.timing;
     }
-    for (int i = 0; i < count; ++i) hs[i]->e.timing = false;
+    // captured products keep every kernel on their branch (the near field in
+    // line, not forked to the handle's second stream)
+    for (int i = 0; i < count; ++i) hs[i]->e.timing = false, hs[i]->e.overlap_near = false;
     auto restore = [&] {
       for (int i = 0; i < count; ++i) {
         hs[i]->e.stream = saved[i];
         hs[i]->e.timing = timing[i];
+        hs[i]->e.overlap_near = true;
       }
     };
     auto* g = new h3d_graph;

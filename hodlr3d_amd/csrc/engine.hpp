@@ -298,6 +298,10 @@ struct Engine {
   double* d_t = nullptr;      // t, fp64 or reinterpreted as fp32
   double* d_part = nullptr;   // phase-1 partial sums
   double* d_ylev = nullptr;   // per-level y by position: nL * N
+  double* d_ynear = nullptr;  // near field by position: N
+  hipStream_t side = nullptr;                              // the near field's stream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;         // its fork / join events
+  bool overlap_near = true;  // near field on `side` (off while a graph is captured)
   double* d_io = nullptr;     // staging for host vectors
   int64_t t_cap = 0, part_cap = 0;  // allocated entries of d_t / d_part per right-hand side
   // products with several right-hand sides: groups of up to rhs_block (16 or

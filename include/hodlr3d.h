@@ -208,7 +208,10 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
  * the second factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); then
  * the phase-1 kernel alone of the first and of the second copy.  *count gets
  * the number of matvecs; all are then reset.  Grouped products of several
- * right-hand sides (h3d_set_rhs_block) are not timed. */
+ * right-hand sides (h3d_set_rhs_block) are not timed.  While timing is on,
+ * every kernel of a matvec runs in line on the handle's stream; otherwise the
+ * near field runs on a second stream of the handle, overlapping the far field
+ * (forked after the gather of x, joined before the combine). */
 int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count);
 
 /* Replaces classGMRES::gmres (src/gmres.cpp:40-94): unrestarted GMRES, x0 = 0,
