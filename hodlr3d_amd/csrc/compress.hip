@@ -155,30 +155,32 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
 // solver on S would only resolve down to about sqrt(r eps) of the largest.
 // L (= Lv), A (Lu, then J) and M live in LDS, or in global scratch (GLOBAL,
 // 3 * gstride doubles per work item) for ranks whose matrices do not fit;
-// per-pair rotation data is always in LDS.  All matrices row-major, stride r.
+// per-pair rotation data is always in LDS.  All matrices row-major with the
+// odd row stride r | 1 (a column read by consecutive lanes then hits distinct
+// LDS banks; with an even stride they all fell on a few).
 __device__ inline void wg_sync() { __syncthreads(); }
 
 // In-place right-looking Cholesky of the r x r matrix A (lower triangle used);
 // sets *flag when a pivot falls below kCholFail times the original diagonal.
 // Called by every thread of the workgroup.
-__device__ inline void chol_inplace(double* A, int r, double* dg, int* flag, int tid, int nt) {
-  for (int i = tid; i < r; i += nt) dg[i] = A[i * r + i];
+__device__ inline void chol_inplace(double* A, int r, int ld, double* dg, int* flag, int tid, int nt) {
+  for (int i = tid; i < r; i += nt) dg[i] = A[i * ld + i];
   wg_sync();
   for (int k = 0; k < r; ++k) {
-    const double dkk = A[k * r + k];
+    const double dkk = A[k * ld + k];
     if (!(dkk > kCholFail * dg[k])) {  // also catches NaN
       if (tid == 0) *flag = 1;
       break;
     }
     const double lkk = sqrt(dkk);
     wg_sync();
-    for (int i = k + 1 + tid; i < r; i += nt) A[i * r + k] /= lkk;
-    if (tid == 0) A[k * r + k] = lkk;
+    for (int i = k + 1 + tid; i < r; i += nt) A[i * ld + k] /= lkk;
+    if (tid == 0) A[k * ld + k] = lkk;
     wg_sync();
     const int m = r - k - 1;
     for (int idx = tid; idx < m * m; idx += nt) {
       const int i = k + 1 + idx / m, j = k + 1 + idx % m;
-      if (j <= i) A[i * r + j] -= A[i * r + k] * A[j * r + k];
+      if (j <= i) A[i * ld + j] -= A[i * ld + k] * A[j * ld + k];
     }
     wg_sync();
   }
@@ -197,10 +199,11 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   const CDesc d = ds[list[item]];
   const int r = d.r;
   const int rr = r + (r & 1);
+  const int ld = r | 1;  // odd row stride of L, A, M: a column read hits distinct banks
   double* L = GLOBAL ? gscratch + 3 * gstride * item : sm;
-  double* A = L + r * r;
-  double* M = A + r * r;
-  double* cs = GLOBAL ? sm : M + r * r;  // rr/2
+  double* A = L + r * ld;
+  double* M = A + r * ld;
+  double* cs = GLOBAL ? sm : M + r * ld;  // rr/2
   double* sn = cs + rr / 2;      // rr/2
   double* dg = sn + rr / 2;      // r: original diagonal of the matrix being factored
   double* lam = dg + r;          // r
@@ -212,13 +215,14 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   const int tid = threadIdx.x, nt = blockDim.x;
 
   for (int idx = tid; idx < r * r; idx += nt) {
-    L[idx] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
-    A[idx] = W[idx];                   // Gu
+    const int i = idx / r, j = idx % r;
+    L[i * ld + j] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
+    A[i * ld + j] = W[idx];                   // Gu
   }
   if (tid == 0) flag = 0;
   wg_sync();
-  chol_inplace(L, r, dg, &flag, tid, nt);
-  if (!flag) chol_inplace(A, r, dg, &flag, tid, nt);
+  chol_inplace(L, r, ld, dg, &flag, tid, nt);
+  if (!flag) chol_inplace(A, r, ld, dg, &flag, tid, nt);
   if (flag) {
     if (tid == 0) W[2 * (int64_t)r * r + r] = 1.0;  // raw
     return;
@@ -227,47 +231,64 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   for (int idx = tid; idx < r * r; idx += nt) {
     const int i = idx / r, j = idx % r;
     double s = 0.0;
-    for (int k = max(i, j); k < r; ++k) s += A[k * r + i] * L[k * r + j];
-    M[idx] = s;
+    for (int k = max(i, j); k < r; ++k) s += A[k * ld + i] * L[k * ld + j];
+    M[i * ld + j] = s;
   }
   wg_sync();
-  for (int idx = tid; idx < r * r; idx += nt) A[idx] = (idx / r == idx % r) ? 1.0 : 0.0;  // J
+  for (int idx = tid; idx < r * r; idx += nt) A[(idx / r) * ld + idx % r] = (idx / r == idx % r) ? 1.0 : 0.0;  // J
   wg_sync();
-  // one-sided Jacobi on M's columns, round-robin pairing (index 0 fixed)
+  // one-sided Jacobi on M's columns, round-robin pairing (index 0 fixed).  A
+  // pair's column dot products are split over tpp lanes of one wave (row i to
+  // lane i mod tpp, then an xor butterfly: a fixed order), so that the rounds
+  // do not wait on r / 2 threads each running an r-long loop.
   const int np = rr / 2;
+  int tpp = 1;
+  while (tpp < 64 && 2 * tpp * np <= nt) tpp <<= 1;
   for (int sweep = 0; sweep < kMaxSweeps && r > 1; ++sweep) {
     if (tid == 0) flag = 0;
     wg_sync();
     for (int round = 0; round < rr - 1; ++round) {
-      for (int t = tid; t < np; t += nt) {
+      for (int t0 = 0; t0 < np; t0 += nt / tpp) {
+        const int t = t0 + tid / tpp, sub = tid % tpp;
+        const bool act = t < np;
         auto pos = [&](int i) { return i == 0 ? 0 : 1 + (i - 1 + round) % (rr - 1); };
-        int p = pos(t), q = pos(rr - 1 - t);
-        if (p > q) {
-          const int x = p;
-          p = q;
-          q = x;
+        int p = 0, q = r;
+        if (act) {
+          p = pos(t);
+          q = pos(rr - 1 - t);
+          if (p > q) {
+            const int x = p;
+            p = q;
+            q = x;
+          }
         }
-        double c = 1.0, s = 0.0;
-        if (q < r) {
-          double a = 0.0, b = 0.0, g = 0.0;
-          for (int i = 0; i < r; ++i) {
-            const double mp = M[i * r + p], mq = M[i * r + q];
+        double a = 0.0, b = 0.0, g = 0.0;
+        if (act && q < r)
+          for (int i = sub; i < r; i += tpp) {
+            const double mp = M[i * ld + p], mq = M[i * ld + q];
             a += mp * mp;
             b += mq * mq;
             g += mp * mq;
           }
-          if (g != 0.0 && fabs(g) > 1e-15 * sqrt(a * b)) {
+        for (int dd = tpp >> 1; dd >= 1; dd >>= 1) {  // every lane of the wave takes part
+          a += __shfl_xor(a, dd);
+          b += __shfl_xor(b, dd);
+          g += __shfl_xor(g, dd);
+        }
+        if (act && sub == 0) {
+          double c = 1.0, s = 0.0;
+          if (q < r && g != 0.0 && fabs(g) > 1e-15 * sqrt(a * b)) {
             const double th = (b - a) / (2.0 * g);
             const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
             c = 1.0 / sqrt(1.0 + tt * tt);
             s = tt * c;
             flag = 1;
           }
+          cs[t] = c;
+          sn[t] = s;
+          pp[t] = p;
+          pq[t] = q < r ? q : -1;
         }
-        cs[t] = c;
-        sn[t] = s;
-        pp[t] = p;
-        pq[t] = q < r ? q : -1;
       }
       wg_sync();
       // columns p, q of M and J: (x_p, x_q) <- (c x_p - s x_q, s x_p + c x_q)
@@ -276,12 +297,12 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
         const int p = pp[t], q = pq[t];
         if (q < 0 || sn[t] == 0.0) continue;
         const double c = cs[t], s = sn[t];
-        const double a = M[i * r + p], b = M[i * r + q];
-        M[i * r + p] = c * a - s * b;
-        M[i * r + q] = s * a + c * b;
-        const double ja = A[i * r + p], jb = A[i * r + q];
-        A[i * r + p] = c * ja - s * jb;
-        A[i * r + q] = s * ja + c * jb;
+        const double a = M[i * ld + p], b = M[i * ld + q];
+        M[i * ld + p] = c * a - s * b;
+        M[i * ld + q] = s * a + c * b;
+        const double ja = A[i * ld + p], jb = A[i * ld + q];
+        A[i * ld + p] = c * ja - s * jb;
+        A[i * ld + q] = s * ja + c * jb;
       }
       wg_sync();
     }
@@ -292,7 +313,7 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   // squared singular values = squared column norms of M J, sorted descending
   for (int k = tid; k < r; k += nt) {
     double s = 0.0;
-    for (int i = 0; i < r; ++i) s += M[i * r + k] * M[i * r + k];
+    for (int i = 0; i < r; ++i) s += M[i * ld + k] * M[i * ld + k];
     lam[k] = s;
   }
   wg_sync();
@@ -316,13 +337,13 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
     double* tv = W + (int64_t)r * r + (int64_t)k * r;
     for (int i = 0; i < r; ++i) {
       double s = 0.0;
-      for (int j = 0; j <= i; ++j) s += L[i * r + j] * A[j * r + c];
+      for (int j = 0; j <= i; ++j) s += L[i * ld + j] * A[j * ld + c];
       tu[i] = s;
     }
     for (int i = r - 1; i >= 0; --i) {
-      double s = A[i * r + c];
-      for (int j = i + 1; j < r; ++j) s -= L[j * r + i] * tv[j];
-      tv[i] = s / L[i * r + i];
+      double s = A[i * ld + c];
+      for (int j = i + 1; j < r; ++j) s -= L[j * ld + i] * tv[j];
+      tv[i] = s / L[i * ld + i];
     }
     out_lam[k] = lam[c];
   }
@@ -504,7 +525,7 @@ void build_mixed(Engine& e, double tol) {
   // LDS of the eigen step: matrices (unless global) + rotation data
   auto eig_lds = [](int r, bool global) {
     const int rr = r + (r & 1);
-    return (size_t)((global ? 0 : 3 * r * r) + rr + 2 * r) * sizeof(double) +
+    return (size_t)((global ? 0 : 3 * r * (r | 1)) + rr + 2 * r) * sizeof(double) +
            (size_t)(rr + r) * sizeof(int);
   };
   int rmax = 1;  // largest rank whose matrices fit in LDS
@@ -633,7 +654,7 @@ void build_mixed(Engine& e, double tol) {
       int rm = 1;
       for (int32_t li : g) rm = std::max(rm, cd[i0 + li].r);
       int32_t* d_g = upload(g, st);
-      const int64_t gstride = (int64_t)rm * rm;
+      const int64_t gstride = (int64_t)rm * (rm | 1);
       const int64_t per = q == 2 ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (3 * gstride)))
                                  : 1000000;
       double* gs = q == 2 ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
