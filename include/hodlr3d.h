@@ -6,6 +6,9 @@
  * return an int status (H3D_OK == 0).  Pointers are host memory unless a
  * `where` argument says H3D_DEVICE.  Calls on one handle are ordered on the
  * handle's HIP stream and are not re-entrant; distinct handles are independent.
+ * (A product's near field runs on a second stream of the handle, forked after
+ * the gather of x and joined before the combine with events on the handle's
+ * stream, so each product still completes in order on that stream.)
  * Vectors are in the caller's (original) point order, like the reference's
  * MatVecProduct after `reorder` (src/HODLR3DTree.cpp:1795-1805).
  */
