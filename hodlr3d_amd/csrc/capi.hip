@@ -68,6 +68,7 @@ void Engine::free_all() {
   dfree(d_tw);
   dfree(d_partw);
   dfree(d_ylevw);
+  dfree(d_ynw);
   dfree(d_iow);
   dfree(d_rcols);
   dfree(d_gitems);

@@ -314,6 +314,7 @@ struct Engine {
   double* d_tw = nullptr;           // t, W per column
   double* d_partw = nullptr;        // phase-1 partial sums, W per entry
   double* d_ylevw = nullptr;        // per-level y by position, W per point
+  double* d_ynw = nullptr;          // near field by position, W per point
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
   void* d_rcols = nullptr;          // per column of a row-split block: int4 reduce entry

@@ -10,9 +10,10 @@
 //             rows of V (one 4-row vector per lane), B = 4 rows x 16 RHS
 //   phase 2   Y_k = U_k T_k     per <= 64 rows of a target box, 16x16x4 MFMA:
 //             A = 16 row quads x 4 columns of U, B = 4 columns x 16 RHS of T
-//   near      y_L += K_near X   lanes own target rows, W sums each, source
-//             points and their W values through wave-uniform (scalar) loads
-//   combine   y[:, j] = alpha * sum_levels + d * b[:, j] (an LDS transpose)
+//   near      y_N = K_near X    lanes own target rows, W sums each, source
+//             points and their W values through wave-uniform (scalar) loads;
+//             own buffer, on the handle's second stream under phases 1 and 2
+//   combine   y[:, j] = alpha * (sum_levels + y_N) + d * b[:, j] (an LDS transpose)
 //
 // The work lists are the single-RHS plan's (phase-1 items, partial sums and
 // phase-2 items), so the order of the block loop is the same; fp64 factors
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(256) void near_w64_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    KernelDev kd, const double* __restrict__ xr, double* __restrict__ yLw) {
+    KernelDev kd, const double* __restrict__ xr, double* __restrict__ yNw) {
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -417,9 +418,9 @@ __global__ __launch_bounds__(256) void near_w64_kernel(
     }
   }
   if (!act) return;
-  double* yp = yLw + (int64_t)ip * W;
+  double* yp = yNw + (int64_t)ip * W;
 #pragma unroll
-  for (int j = 0; j < W; ++j) yp[j] += kd.scale * acc[j];
+  for (int j = 0; j < W; ++j) yp[j] = kd.scale * acc[j];
 }
 
 // Near field with fp32 entries and sums: coordinates as fp32 offsets from the
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(256) void near_w32_kernel(
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
     const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
     const double* __restrict__ leafc, KernelDev kd, const float* __restrict__ xr,
-    double* __restrict__ yLw) {
+    double* __restrict__ yNw) {
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -462,17 +463,18 @@ __global__ __launch_bounds__(256) void near_w32_kernel(
     }
   }
   if (!act) return;
-  double* yp = yLw + (int64_t)ip * W;
+  double* yp = yNw + (int64_t)ip * W;
 #pragma unroll
-  for (int j = 0; j < W; ++j) yp[j] += kd.scale * (double)acc[j];
+  for (int j = 0; j < W; ++j) yp[j] = kd.scale * (double)acc[j];
 }
 
-// y[j N + i] = alpha * sum over levels of y_level(iperm[i], j) + d_i b[j N + i]
+// y[j N + i] = alpha * (sum over levels of y_level(iperm[i], j) + y_N(iperm[i], j)) + d_i b[j N + i]
 // for j < cnt: 64 points per block, transposed through LDS so that both the
 // level sums (W per point) and the outputs (64 points per RHS) are contiguous.
 template <int W>
 __global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict__ ylevw, int nL,
-                                                        int64_t N, const int32_t* __restrict__ iperm,
+                                                        int64_t N, const double* __restrict__ ynw,
+                                                        const int32_t* __restrict__ iperm,
                                                         const double* __restrict__ b, int cnt,
                                                         double alpha, double diag,
                                                         const double* __restrict__ diag_vec,
@@ -486,6 +488,7 @@ __global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict
     if (i < N && j < cnt) {
       const int64_t s = iperm[i];
       for (int lv = 0; lv < nL; ++lv) acc = acc + ylevw[((int64_t)lv * N + s) * W + j];
+      if (ynw) acc = acc + ynw[s * W + j];
     }
     tile[j][ii] = acc;
   }
@@ -581,11 +584,13 @@ void ensure_block_scratch(Engine& e, int W) {
   dfree(e.d_tw);
   dfree(e.d_partw);
   dfree(e.d_ylevw);
+  dfree(e.d_ynw);
   e.w_cap = 0;
   e.d_xw = dalloc<double>(2 * padded_points(e.N) * W);
   e.d_tw = dalloc<double>(T * W);
   e.d_partw = dalloc<double>(np * W);
   e.d_ylevw = dalloc<double>(e.N * e.nL * W);
+  e.d_ynw = dalloc<double>(e.N * W);
   e.w_cap = W;
   e.w_t = T;
   e.w_np = np;
@@ -607,6 +612,47 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     zero_kernel<<<(unsigned)std::min<int64_t>(8192, (n + 255) / 256), 256, 0, st>>>(e.d_ylevw, n);
     H3D_LAUNCH_CHECK();
   }
+  // near field into its own buffer (W per point) on the handle's second
+  // stream, overlapping phases 1 and 2 (in line while a graph is captured)
+  const int nn = (int)e.nitems.size();
+  const bool overlap = nn && e.overlap_near && e.side;
+  auto launch_near = [&](hipStream_t ns) {
+    const int64_t lo = level_offset(e.nL);
+    const KernelDev kd = e.kern;
+    const bool invr = kd.q == 7 || kd.q == 16;
+    const unsigned g = wave_grid(nn);
+    const int32_t* nbr = e.lev[e.nL].d_nbr;
+    if constexpr (std::is_same<A, float>::value) {
+      if (invr)
+        near_w32_kernel<InvR32, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, e.d_ynw);
+      else
+        near_w32_kernel<AnyQ32, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, e.d_ynw);
+    } else {
+      if (invr && prec != H3D_PREC_FP64)
+        near_w64_kernel<InvRNewton, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                          e.d_bcount + lo, e.d_px, e.d_py,
+                                                          e.d_pz, kd, xr, e.d_ynw);
+      else if (invr)
+        near_w64_kernel<InvR, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                    xr, e.d_ynw);
+      else
+        near_w64_kernel<AnyQ, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                    xr, e.d_ynw);
+    }
+    H3D_LAUNCH_CHECK();
+  };
+  if (overlap) {
+    H3D_HIP(hipEventRecord(e.ev_fork, st));
+    H3D_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));
+    launch_near(e.side);
+    H3D_HIP(hipEventRecord(e.ev_join, e.side));
+  }
   const Plan& p = e.plan;
   if (e.n_gitems) {
     p1g_kernel<S, A, W><<<(unsigned)e.n_gitems, 256, 0, st>>>(
@@ -624,42 +670,12 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
                                                                 e.d_ubase[prec], tw, e.d_ylevw, N);
     H3D_LAUNCH_CHECK();
   }
-  const int nn = (int)e.nitems.size();
-  if (nn) {
-    double* yLw = e.d_ylevw + (int64_t)(e.nL - 1) * N * W;
-    const int64_t lo = level_offset(e.nL);
-    const KernelDev kd = e.kern;
-    const bool invr = kd.q == 7 || kd.q == 16;
-    const unsigned g = wave_grid(nn);
-    const int32_t* nbr = e.lev[e.nL].d_nbr;
-    if constexpr (std::is_same<A, float>::value) {
-      if (invr)
-        near_w32_kernel<InvR32, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                      e.d_leafc, kd, xr, yLw);
-      else
-        near_w32_kernel<AnyQ32, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                      e.d_leafc, kd, xr, yLw);
-    } else {
-      if (invr && prec != H3D_PREC_FP64)
-        near_w64_kernel<InvRNewton, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                          e.d_bcount + lo, e.d_px, e.d_py,
-                                                          e.d_pz, kd, xr, yLw);
-      else if (invr)
-        near_w64_kernel<InvR, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
-                                                    xr, yLw);
-      else
-        near_w64_kernel<AnyQ, W><<<g, 256, 0, st>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
-                                                    xr, yLw);
-    }
-    H3D_LAUNCH_CHECK();
-  }
+  if (nn && !overlap) launch_near(st);
+  if (overlap) H3D_HIP(hipStreamWaitEvent(st, e.ev_join, 0));
   (void)acc64;
   combine_w_kernel<W><<<(unsigned)((N + 63) / 64), 256, 0, st>>>(
-      e.d_ylevw, e.nL, N, e.d_iperm, b, cnt, e.alpha, e.kern.diag, e.d_diag, y);
+      e.d_ylevw, e.nL, N, nn ? e.d_ynw : nullptr, e.d_iperm, b, cnt, e.alpha, e.kern.diag,
+      e.d_diag, y);
   H3D_LAUNCH_CHECK();
 }
 
