@@ -165,37 +165,84 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
     while (lpc < need && lpc < 64) lpc <<= 1;
     const int K = (need + lpc - 1) / lpc;
     const int cpp = 64 / lpc, slot = lane / lpc, sub = lane % lpc;
-    A xv[KMAX][VEC];
+    // Branch-free loads: rows past the block or past K read row 0 (and meet
+    // x = 0), columns past r - 1 read column r - 1 (and are not stored); x is
+    // loaded at clamped indices and masked by a 0/1 factor afterwards (a
+    // select of a loaded value compiles into a branch and a wait per load).
     const A* xs = x + bd.src + it.s0;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-      const int s = (sub + k * lpc) * VEC;
-#pragma unroll
-      for (int q = 0; q < VEC; ++q) xv[k][q] = (k < K && s + q < rows) ? xs[s + q] : (A)0;
-    }
     const S* base = V + bd.voff + it.s0;
-    for (int c0 = 0; c0 < bd.r; c0 += cpp) {
-      const int l = c0 + slot;
-      A acc = 0;
-      if (l < bd.r) {
-        const S* col = base + (int64_t)l * ldv;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-          // a load starts inside the rows, so it ends < VEC elements past them
-          if (k < K && (sub + k * lpc) * VEC < rows) {
-            A v[VEC];
-            load_factor<S, A, VEC>(col + (sub + k * lpc) * VEC, v);
-#pragma unroll
-            for (int q = 0; q < VEC; ++q) acc += v[q] * xv[k][q];
-          }
-        }
-      }
+    auto finish = [&](int l, A acc) {
       for (int d = lpc >> 1; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
       if (l < bd.r && sub == 0) {
         if (it.pslot < 0)
           t[it.toff + bd.col + l] = scale ? acc * (A)scale[it.toff + bd.col + l] : acc;
         else
           part[(int64_t)it.pslot + l] = acc;
+      }
+    };
+    if (K == 1) {
+      // one load vector per lane and column: x and four passes' column loads
+      // in flight together
+      constexpr int UP = 4;
+      const int r0 = sub * VEC;
+      const int ro = r0 < rows ? r0 : 0;
+      A xv[VEC];
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) xv[q] = xs[min(r0 + q, rows - 1)];
+      for (int c0 = 0; c0 < bd.r; c0 += UP * cpp) {
+        Raw<S, VEC> v[UP];
+#pragma unroll
+        for (int u = 0; u < UP; ++u) {
+          const int l = min(c0 + u * cpp + slot, bd.r - 1);
+          v[u] = raw_load<S, VEC>(base + (int64_t)l * ldv + ro);
+        }
+        if (c0 == 0) {
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) xv[q] *= r0 + q < rows ? (A)1 : (A)0;
+        }
+#pragma unroll
+        for (int u = 0; u < UP; ++u) {
+          A f[VEC];
+          raw_cvt<S, A, VEC>(v[u], f);
+          A acc = 0;
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) acc += f[q] * xv[q];
+          finish(c0 + u * cpp + slot, acc);
+        }
+      }
+    } else {
+      A xv[KMAX][VEC];
+      int roff[KMAX];
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        const int r0 = (sub + k * lpc) * VEC;
+        roff[k] = (k < K && r0 < rows) ? r0 : 0;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) xv[k][q] = xs[min(r0 + q, rows - 1)];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        const int r0 = (sub + k * lpc) * VEC;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) xv[k][q] *= (k < K && r0 + q < rows) ? (A)1 : (A)0;
+      }
+      for (int c0 = 0; c0 < bd.r; c0 += cpp) {
+        const int l = min(c0 + slot, bd.r - 1);
+        const S* col = base + (int64_t)l * ldv;
+        Raw<S, VEC> v[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) v[k] = raw_load<S, VEC>(col + roff[k]);
+        A acc = 0;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+          A f[VEC];
+          raw_cvt<S, A, VEC>(v[k], f);
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) acc += f[q] * xv[k][q];
+        }
+        finish(c0 + slot, acc);
       }
     }
   }
