@@ -1013,6 +1013,41 @@ void Plan::free_device() {
   dfree(d_tp2);
 }
 
+// Phase-2 items of one factor copy: each target box's rows in chunks of p2rows
+// (64 lanes x one 16-byte vector: 128 fp64, 256 fp32 or 512 fp16/bf16 rows), so
+// that a wave instruction reads 1 KB of one column when the item spans that
+// many rows; larger items first (less tail at the end of a launch).
+static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int p2rows,
+                     std::vector<P2Item>& out) {
+  out.clear();
+  for (int j = 1; j <= e.nL; ++j) {
+    const Level& lv = e.lev[j];
+    const LevelFactors& lf = *lfs[j];
+    const int64_t lo = level_offset(j);
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      const int R = lf.Rt[k];
+      if (n <= 0 || R <= 0) continue;
+      for (int i0 = 0; i0 < n; i0 += p2rows) {
+        P2Item it{};
+        it.uoff = lf.uoff[k];
+        it.chunk = lv.chunk[k];
+        it.ld = n + (n & 1);
+        it.i0 = i0;
+        it.nr = std::min(p2rows, n - i0);
+        it.R = R;
+        it.toff = lf.toff[k];
+        it.yoff = e.bstart[lo + k];
+        it.level = j;
+        out.push_back(it);
+      }
+    }
+  }
+  std::stable_sort(out.begin(), out.end(), [](const P2Item& a, const P2Item& b) {
+    return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
+  });
+}
+
 void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
                     int p2rows) {
   const int nL = e.nL;
@@ -1084,22 +1119,6 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
           }
           flush(lv.iloff[k + 1]);
         }
-        // phase 2: rows in chunks of p2rows (64 lanes x one 16-byte vector:
-        // 128 fp64, 256 fp32 or 512 fp16 rows), so that a wave instruction
-        // reads 1 KB of one column when the item spans that many rows
-        for (int i0 = 0; i0 < n; i0 += p2rows) {
-          P2Item it{};
-          it.uoff = lf.uoff[k];
-          it.chunk = lv.chunk[k];
-          it.ld = n + (n & 1);
-          it.i0 = i0;
-          it.nr = std::min(p2rows, n - i0);
-          it.R = R;
-          it.toff = lf.toff[k];
-          it.yoff = e.bstart[lo + k];
-          it.level = j;
-          out.p2.push_back(it);
-        }
         // transpose phase 1
         const int nsplit = (n + kP1Rows - 1) / kP1Rows;
         for (int c0 = 0; c0 < R; c0 += 64) {
@@ -1132,10 +1151,7 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
       src_cursor += nsrc[k];
     }
   }
-  // Larger items first: less tail at the end of a launch.
-  std::stable_sort(out.p2.begin(), out.p2.end(), [](const P2Item& a, const P2Item& b) {
-    return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
-  });
+  p2_items(e, lfs, p2rows, out.p2);
   out.npartial = std::max(pslot, tslot);
   out.free_device();
   out.d_p1 = upload(out.p1, st);
@@ -1151,6 +1167,12 @@ void build_plan(Engine& e) {
   std::vector<const LevelFactors*> lfs(e.nL + 1, nullptr);
   for (int j = 1; j <= e.nL; ++j) lfs[j] = &e.lev[j];
   build_plan_for(e, lfs, e.plan, kP2Rows);
+  for (int q = 0; q < 2; ++q) {
+    p2_items(e, lfs, q == 0 ? 2 * kP2Rows : 4 * kP2Rows, e.p2_narrow[q]);
+    dfree(e.d_p2_narrow[q]);
+    e.d_p2_narrow[q] = upload(e.p2_narrow[q], e.stream);
+  }
+  H3D_HIP(hipStreamSynchronize(e.stream));
   e.nitems.clear();
   const int64_t lo = level_offset(e.nL);
   for (int64_t k = 0; k < e.lev[e.nL].nboxes; ++k) {
@@ -1263,6 +1285,8 @@ namespace {
 // One factor copy as the far-field kernels see it.
 struct FactorView {
   const Plan* plan;
+  const P2Item* p2;     // phase-2 items of this copy (device) and their count
+  int n2;
   const BlockDev* blk;
   const int32_t* btoff;
   void* const* ubase;
@@ -1275,7 +1299,9 @@ struct FactorView {
 };
 
 FactorView aca_view(Engine& e, int prec, bool acc64) {
-  return {&e.plan, e.d_blk, e.d_btoff, e.d_ubase[prec], e.d_vbase[prec], nullptr, 0,
+  const P2Item* p2 = prec == 0 ? e.plan.d_p2 : e.d_p2_narrow[prec - 1];
+  const int n2 = (int)(prec == 0 ? e.plan.p2.size() : e.p2_narrow[prec - 1].size());
+  return {&e.plan, p2, n2, e.d_blk, e.d_btoff, e.d_ubase[prec], e.d_vbase[prec], nullptr, 0,
           prec + (acc64 && prec != 0 ? 3 : 0), nullptr, nullptr};
 }
 
@@ -1293,8 +1319,8 @@ inline void* class_tsrc(Engine& e, int c) {
 
 FactorView class_view(Engine& e, int c) {
   const FactorClass& fc = e.mix.cls[c];
-  return {&fc.plan, fc.d_blk, fc.d_btoff, fc.d_ubase, fc.d_vbase, e.mix.d_scale + fc.tbase, c,
-          6 + c, &fc, class_tsrc(e, c)};
+  return {&fc.plan, fc.plan.d_p2, (int)fc.plan.p2.size(), fc.d_blk, fc.d_btoff, fc.d_ubase,
+          fc.d_vbase, e.mix.d_scale + fc.tbase, c, 6 + c, &fc, class_tsrc(e, c)};
 }
 
 // Source-grouped phase 1 of a mixed class with a load width of `bytes` per lane.
@@ -1362,12 +1388,12 @@ void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
 
 template <class S, class A>
 void launch_p2(Engine& e, const FactorView& f, int bytes, const A* t) {
-  const int n2 = (int)f.plan->p2.size();
+  const int n2 = f.n2;
   if (!n2) return;
   const unsigned g = wave_grid(n2);
   hipStream_t st = e.stream;
   constexpr int es = (int)sizeof(S);
-  const P2Item* it = f.plan->d_p2;
+  const P2Item* it = f.p2;
   const int acc = f.accumulate;
   if (bytes >= 16)
     phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev, e.N, acc);

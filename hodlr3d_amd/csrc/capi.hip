@@ -51,6 +51,10 @@ void Engine::free_all() {
   dfree(d_by_src);
   dfree(d_btoff);
   plan.free_device();
+  for (int q = 0; q < 2; ++q) {
+    dfree(d_p2_narrow[q]);
+    p2_narrow[q].clear();
+  }
   dfree(d_near);
   dfree(d_xp);
   dfree(d_xf);

@@ -287,6 +287,11 @@ struct Engine {
 
   // matvec plan of the ACA copy; near-field items are shared by every copy
   Plan plan;
+  // phase-2 items of the fp32 and bf16 copies (256 / 512 rows, so that a
+  // wave instruction reads 1 KB of a column with 16-byte loads, as the plan's
+  // 128-row items do for fp64)
+  std::vector<P2Item> p2_narrow[2];
+  P2Item* d_p2_narrow[2] = {nullptr, nullptr};
   std::vector<NearItem> nitems;
   NearItem* d_near = nullptr;
   MixedCopy mix;
