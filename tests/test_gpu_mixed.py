@@ -96,6 +96,25 @@ def test_rebuild_is_bit_identical(built):
     assert np.array_equal(y1, y2)
 
 
+def test_overlapped_near_field_equals_the_in_line_product(built):
+    """Without per-kernel timing the near field runs on a second stream,
+    concurrently with the far field, into a buffer of its own; with timing
+    every kernel runs in line on one stream.  The two schedules give the same
+    bits (another mode's product in between leaves its own values in the
+    shared buffers)."""
+    name, x, h, tol, st = built
+    q = np.random.default_rng(11).standard_normal(len(x))
+    h.kernel_timing(False)
+    y_conc = h.matvec(q, prec="mixed")
+    h.matvec(q[::-1].copy(), prec="fp64")
+    h.kernel_timing(True)
+    y_line = h.matvec(q, prec="mixed")
+    h.kernel_timing(False)
+    assert np.array_equal(y_conc, y_line)
+    h.matvec(q[::-1].copy(), prec="fp32")
+    assert np.array_equal(h.matvec(q, prec="mixed"), y_conc)
+
+
 def test_every_load_width_gives_the_same_product(built):
     """Load widths and rows per lane change the summation order; the fp16
     class (about 1e-3 of the operator) sums in fp32, so results agree to
