@@ -420,7 +420,7 @@ __global__ void tgather_kernel(const int32_t* __restrict__ map, int n, const A* 
 }
 
 // ---------------------------------------------------------------- phase 2
-// U of a target box is column-major with ld = n + (n & 1).  A lane owns VR
+// U of a target box is column-major with the item's ld.  A lane owns VR
 // consecutive rows (one vector load per column); the wave's lanes are split
 // into G groups that walk different columns, so one wave instruction reads G
 // consecutive columns of the box (contiguous when the item spans the box).
@@ -1018,7 +1018,7 @@ void Plan::free_device() {
 // that a wave instruction reads 1 KB of one column when the item spans that
 // many rows; larger items first (less tail at the end of a launch).
 static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int p2rows,
-                     std::vector<P2Item>& out) {
+                     int class_elem, std::vector<P2Item>& out) {
   out.clear();
   for (int j = 1; j <= e.nL; ++j) {
     const Level& lv = e.lev[j];
@@ -1032,7 +1032,7 @@ static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int
         P2Item it{};
         it.uoff = lf.uoff[k];
         it.chunk = lv.chunk[k];
-        it.ld = n + (n & 1);
+        it.ld = class_elem ? class_ldu(n, class_elem) : n + (n & 1);
         it.i0 = i0;
         it.nr = std::min(p2rows, n - i0);
         it.R = R;
@@ -1049,7 +1049,7 @@ static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int
 }
 
 void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
-                    int p2rows) {
+                    int p2rows, int class_elem) {
   const int nL = e.nL;
   hipStream_t st = e.stream;
   out.p1.clear();
@@ -1128,7 +1128,7 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
             TP1Item it{};
             it.uoff = lf.uoff[k];
             it.chunk = lv.chunk[k];
-            it.ld = n + (n & 1);
+            it.ld = class_elem ? class_ldu(n, class_elem) : n + (n & 1);
             it.c0 = c0;
             it.nc = nc;
             it.s0 = q * kP1Rows;
@@ -1151,7 +1151,7 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
       src_cursor += nsrc[k];
     }
   }
-  p2_items(e, lfs, p2rows, out.p2);
+  p2_items(e, lfs, p2rows, class_elem, out.p2);
   out.npartial = std::max(pslot, tslot);
   out.free_device();
   out.d_p1 = upload(out.p1, st);
@@ -1168,7 +1168,7 @@ void build_plan(Engine& e) {
   for (int j = 1; j <= e.nL; ++j) lfs[j] = &e.lev[j];
   build_plan_for(e, lfs, e.plan, kP2Rows);
   for (int q = 0; q < 2; ++q) {
-    p2_items(e, lfs, q == 0 ? 2 * kP2Rows : 4 * kP2Rows, e.p2_narrow[q]);
+    p2_items(e, lfs, q == 0 ? 2 * kP2Rows : 4 * kP2Rows, 0, e.p2_narrow[q]);
     dfree(e.d_p2_narrow[q]);
     e.d_p2_narrow[q] = upload(e.p2_narrow[q], e.stream);
   }

@@ -66,7 +66,7 @@ struct PDesc {
   int32_t vch[2];    // V chunk of the block per class
   int32_t r32, r16;  // columns kept per class (prefix of the sorted columns)
   int32_t raw;
-  int32_t pad;
+  int32_t ldu_o[2];  // U column stride per class (the V side keeps the fp64 copy's)
   double su, sv;     // power-of-two scales of the stored columns
 };
 
@@ -371,7 +371,8 @@ __global__ __launch_bounds__(256) void project_kernel(
   const int row = it.row0 + threadIdx.x;
   if (row >= rows) return;
   const int r = d.r;
-  const int ld = side ? d.ldv : d.ldu;  // fp64 copy and class layouts share ld
+  const int ld = side ? d.ldv : d.ldu;  // the fp64 copy's stride (and the classes' V stride)
+  const int ld32 = side ? ld : p.ldu_o[0], ld16 = side ? ld : p.ldu_o[1];
   const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu) + row;
   const double* W = ws + d.w;
   const double* T = W + (side ? (int64_t)r * r : 0);
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(256) void project_kernel(
                                   (side ? p.v_out[1] : p.u_out[1]) + row
                             : nullptr;
   if (p.raw) {
-    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld] = (float)X[(int64_t)k * ld];
+    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld32] = (float)X[(int64_t)k * ld];
     return;
   }
   const int keep = p.r32 + p.r16;
@@ -404,8 +405,8 @@ __global__ __launch_bounds__(256) void project_kernel(
       if (k >= keep) break;
       // U side: unit-norm column times su; V side: V' columns are orthonormal
       const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(fmax(lam[k], floor_)));
-      if (k < p.r32) o32[(int64_t)k * ld] = (float)v;
-      else o16[(int64_t)(k - p.r32) * ld] = (_Float16)v;
+      if (k < p.r32) o32[(int64_t)k * ld32] = (float)v;
+      else o16[(int64_t)(k - p.r32) * ld16] = (_Float16)v;
     }
   }
 }
@@ -893,8 +894,8 @@ void build_mixed(Engine& e, double tol) {
           FactorClass& fc = mx.cls[c];
           LevelFactors& f = fc.lf[j];
           f.Rt[k] = R[c];
-          f.uoff[k] = fc.nU[ch];
-          fc.nU[ch] += (int64_t)(n + (n & 1)) * R[c];
+          f.uoff[k] = class_ustart(fc.nU[ch], n, fc.elem);
+          fc.nU[ch] = f.uoff[k] + (int64_t)class_ldu(n, fc.elem) * R[c];
           f.toff[k] = T[c];
           T[c] += R[c];
         }
@@ -903,7 +904,9 @@ void build_mixed(Engine& e, double tol) {
           if (i < 0) continue;
           for (int c = 0; c < 2; ++c) {
             const LevelFactors& f = mx.cls[c].lf[j];
-            pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * (n + (n & 1));
+            const int ldu = class_ldu(n, mx.cls[c].elem);
+            pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * ldu;
+            pd[i].ldu_o[c] = ldu;
           }
         }
       }
@@ -1087,7 +1090,7 @@ void build_mixed(Engine& e, double tol) {
       fc.d_btoff = upload(btoff[c], st);
       std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
       for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
-      build_plan_for(e, lfs, fc.plan, c == 0 ? 256 : 512);  // fp32 / fp16 storage
+      build_plan_for(e, lfs, fc.plan, c == 0 ? 256 : 512, fc.elem);  // fp32 / fp16 storage
       fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
       fc.plan.p1r.clear();
       fc.d_mp1 = upload(fc.mp1, st);

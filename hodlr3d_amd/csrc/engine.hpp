@@ -356,6 +356,21 @@ struct Engine {
 };
 
 inline int64_t level_offset(int j) { return ((int64_t(1) << (3 * j)) - 1) / 7; }
+
+// Column stride of a recompressed class's U for a target box of n rows (elem
+// bytes per entry).  Columns of more than 1 KB span several phase-2 items
+// (1 KB of rows each), so they start on 128-byte lines: an item's piece of a
+// column then touches 8 lines, not 9.  Shorter columns are packed (n rounded
+// up to even): one item reads the box's columns as one region.
+inline int class_ldu(int n, int elem) {
+  const int line = 128 / elem;
+  return n * elem > 1024 ? (n + line - 1) / line * line : n + (n & 1);
+}
+// Element offset (in its chunk) of the U of such a box placed at `off` or after.
+inline int64_t class_ustart(int64_t off, int n, int elem) {
+  const int line = 128 / elem;
+  return n * elem > 1024 ? (off + line - 1) / line * line : off;
+}
 constexpr int64_t kChunkTail = 64;  // zero elements after every factor chunk
 inline int elem_size(int prec) { return prec == 0 ? 8 : (prec == 1 ? 4 : 2); }
 // Precision code of the recompressed mixed copy (fp32 + fp16 columns, fp64 accumulation).
@@ -398,8 +413,11 @@ void release_mixed(Engine& e);
 // Plan of one copy from its per-level factor bookkeeping (apply.hip).
 // lf[j] describes level j (1..nLevels).
 // p2rows: rows of one phase-2 item (128 for the ACA copies, whose plan serves
-// every precision; 256 / 512 for the fp32 / fp16 classes of the mixed copy)
-void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out, int p2rows);
+// every precision; 256 / 512 for the fp32 / fp16 classes of the mixed copy);
+// class_elem: 0 for the ACA layout (U stride n rounded up to even), else the
+// bytes per entry of a mixed class (U stride class_ldu)
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out, int p2rows,
+                    int class_elem = 0);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);
