@@ -410,13 +410,36 @@ __global__ void mreduce_kernel(const MixReduce* __restrict__ rs, int n, const A*
 }
 
 // t[i] = scale[i] * t_src[map[i]]: source-order phase-1 results into target
-// order (contiguous map reads and t writes, gathered t_src reads).
+// order (contiguous map, scale and t, gathered t_src reads), four consecutive
+// entries per thread with 16-byte map / scale / t accesses (the arrays start
+// 256-byte aligned), the last n % 4 entries one at a time.
 template <class A>
 __global__ void tgather_kernel(const int32_t* __restrict__ map, int n, const A* __restrict__ tsrc,
-                               const double* __restrict__ scale, A* __restrict__ t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  t[i] = tsrc[map[i]] * (A)scale[i];
+                               const A* __restrict__ scale, A* __restrict__ t) {
+  const int i = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 3 < n) {
+    const int4 m = *(const int4*)(map + i);
+    A sc[4], v[4];
+    if constexpr (sizeof(A) == 4) {
+      const float4 q = *(const float4*)(scale + i);
+      sc[0] = q.x, sc[1] = q.y, sc[2] = q.z, sc[3] = q.w;
+    } else {
+      const double2 q0 = *(const double2*)(scale + i), q1 = *(const double2*)(scale + i + 2);
+      sc[0] = q0.x, sc[1] = q0.y, sc[2] = q1.x, sc[3] = q1.y;
+    }
+    v[0] = tsrc[m.x] * sc[0];
+    v[1] = tsrc[m.y] * sc[1];
+    v[2] = tsrc[m.z] * sc[2];
+    v[3] = tsrc[m.w] * sc[3];
+    if constexpr (sizeof(A) == 4) {
+      *(float4*)(t + i) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      *(double2*)(t + i) = make_double2(v[0], v[1]);
+      *(double2*)(t + i + 2) = make_double2(v[2], v[3]);
+    }
+  } else {
+    for (int k = i; k < n; ++k) t[k] = tsrc[map[k]] * scale[k];
+  }
 }
 
 // ---------------------------------------------------------------- phase 2
@@ -1187,8 +1210,8 @@ void build_plan(Engine& e) {
 void ensure_scratch(Engine& e, int nrhs) {
   int64_t T = e.T, np = e.plan.npartial;
   if (e.mix.ready) {
-    // target- and source-order t of both classes
-    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T));
+    // target- and source-order t of both classes (class_t0 .. class_tsrc)
+    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T) + 2);
     for (const FactorClass& c : e.mix.cls)
       np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
   }
@@ -1309,12 +1332,16 @@ FactorView aca_view(Engine& e, int prec, bool acc64) {
 // at d_t[0, T0); class 1 (fp16 storage, a small part of the operator) works
 // in fp32 with its t as floats right after class 0's.
 // Scratch t layout of the mixed copy (in doubles): class 0 target order [0, T0),
-// class 1 target order (floats) [T0, T0 + T1), then the source-order copies.
+// class 1 target order (T1 floats) from T0 rounded up to even (16-byte aligned,
+// for the gather's vector stores), then the source-order copies: class 0's
+// T0 doubles from T0e + T1, class 1's T1 floats after them; at most
+// 2 (T0 + T1) + 1 doubles in all.
+inline int64_t class_t1_off(const Engine& e) { return (e.mix.cls[0].T + 1) & ~int64_t(1); }
 inline double* class_t0(Engine& e) { return e.d_t; }
-inline float* class_t1(Engine& e) { return (float*)(e.d_t + e.mix.cls[0].T); }
+inline float* class_t1(Engine& e) { return (float*)(e.d_t + class_t1_off(e)); }
 inline void* class_tsrc(Engine& e, int c) {
-  const int64_t T0 = e.mix.cls[0].T, T1 = e.mix.cls[1].T;
-  return c == 0 ? (void*)(e.d_t + T0 + T1) : (void*)(e.d_t + 2 * T0 + T1);
+  const int64_t T0 = e.mix.cls[0].T, T1 = e.mix.cls[1].T, o = class_t1_off(e) + T1;
+  return c == 0 ? (void*)(e.d_t + o) : (void*)(e.d_t + o + T0);
 }
 
 FactorView class_view(Engine& e, int c) {
@@ -1374,8 +1401,9 @@ void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
           fc.d_mp1r, (int)fc.mp1r.size(), part, (A*)f.tsrc);
     H3D_LAUNCH_CHECK();
     if (fc.T > 0)
-      tgather_kernel<A><<<(fc.T + 255) / 256, 256, 0, e.stream>>>(fc.d_tmap, fc.T, (const A*)f.tsrc,
-                                                                  f.scale, t);
+      tgather_kernel<A><<<(fc.T / 4 + 256) / 256, 256, 0, e.stream>>>(
+          fc.d_tmap, fc.T, (const A*)f.tsrc,
+          sizeof(A) == 4 ? (const A*)fc.d_scale32 : (const A*)f.scale, t);
     H3D_LAUNCH_CHECK();
     return;
   }

@@ -486,6 +486,7 @@ void FactorClass::free_all() {
   dfree(d_mp1);
   dfree(d_mp1r);
   dfree(d_tmap);
+  dfree(d_scale32);
   plan.free_device();
   plan = Plan();
   mp1.clear();
@@ -1084,6 +1085,11 @@ void build_mixed(Engine& e, double tol) {
         }
     }
     mx.d_scale = upload(scale, st);
+    // class 1 sums in fp32: its factors converted once (the value the gather
+    // would convert to), held until the stream synchronizes
+    std::vector<float> scale32(mx.cls[1].T);
+    for (int32_t q = 0; q < mx.cls[1].T; ++q) scale32[q] = (float)scale[(size_t)mx.cls[1].tbase + q];
+    mx.cls[1].d_scale32 = upload(scale32, st);
     for (int c = 0; c < 2; ++c) {
       FactorClass& fc = mx.cls[c];
       fc.d_blk = upload(blk[c], st);

@@ -223,6 +223,7 @@ struct FactorClass {
   MixP1Item* d_mp1 = nullptr;
   MixReduce* d_mp1r = nullptr;
   int32_t* d_tmap = nullptr;
+  float* d_scale32 = nullptr;        // class 1: c_k in target order as fp32 (its sums' type)
   int32_t mpartial = 0;              // partial sums of the phase-1 items
   int32_t T = 0;                     // t entries of this class
   int32_t tbase = 0;                 // offset of this class in the shared t / scale arrays
