@@ -248,6 +248,23 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
   }
 }
 
+// Sum of the nsplit partials part[base + q * nc] (q = 0, 1, ...) in q order,
+// eight loads issued ahead of their adds (a plain loop waits for each load).
+template <class A>
+__device__ inline A split_sum(const A* __restrict__ part, int64_t base, int nsplit, int nc) {
+  A acc = 0;
+  int q = 0;
+  for (; q + 8 <= nsplit; q += 8) {
+    A v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[base + (int64_t)(q + u) * nc];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; q < nsplit; ++q) acc += part[base + (int64_t)q * nc];
+  return acc;
+}
+
 // Sums the row-split partials of one block (nsplit x nc values, row ranges in
 // order) into t[toff + l] (times scale[toff + l] when scaled).
 template <class A>
@@ -258,8 +275,7 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
   const int lane = threadIdx.x & 63;
   const P1Reduce r = rs[w];
   for (int l = lane; l < r.nc; l += 64) {
-    A acc = 0;
-    for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
+    const A acc = split_sum(part, (int64_t)r.pbase + l, r.nsplit, r.nc);
     t[r.toff + l] = scale ? acc * (A)scale[r.toff + l] : acc;
   }
 }
@@ -402,11 +418,8 @@ __global__ void mreduce_kernel(const MixReduce* __restrict__ rs, int n, const A*
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
   const MixReduce r = rs[w];
-  for (int l = lane; l < r.nc; l += 64) {
-    A acc = 0;
-    for (int q = 0; q < r.nsplit; ++q) acc += part[(int64_t)r.pbase + (int64_t)q * r.nc + l];
-    tsrc[r.tsb + l] = acc;
-  }
+  for (int l = lane; l < r.nc; l += 64)
+    tsrc[r.tsb + l] = split_sum(part, (int64_t)r.pbase + l, r.nsplit, r.nc);
 }
 
 // t[i] = scale[i] * t_src[map[i]]: source-order phase-1 results into target
