@@ -284,7 +284,8 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
 
 // Row-split partial sums, one thread per (column, RHS): column q of the table
 // holds {first partial entry, number of ranges, entries per range, t index};
-// the ranges are summed in order.
+// the ranges are summed in order, eight loads issued ahead of their adds (a
+// plain loop waits for each load).
 template <class A, int W>
 __global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
                                 const A* __restrict__ partw, A* __restrict__ tw) {
@@ -292,8 +293,18 @@ __global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
   if (id >= ncols * W) return;
   const int j = (int)(id % W);
   const int4 c = cols[id / W];
+  const A* p = partw + (int64_t)c.x * W + j;
+  const int64_t stride = (int64_t)c.z * W;
   A acc = 0;
-  for (int q = 0; q < c.y; ++q) acc += partw[((int64_t)c.x + (int64_t)q * c.z) * W + j];
+  int q = 0;
+  for (; q + 8 <= c.y; q += 8) {
+    A v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(q + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; q < c.y; ++q) acc += p[(int64_t)q * stride];
   tw[(int64_t)c.w * W + j] = acc;
 }
 
