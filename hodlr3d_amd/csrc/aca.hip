@@ -492,7 +492,7 @@ __global__ void write_kernel(const WriteDesc* __restrict__ ds, const double* __r
                              uint16_t* __restrict__ U16, uint16_t* __restrict__ V16,
                              int32_t* __restrict__ piv) {
   const WriteDesc d = ds[blockIdx.x];
-  const int ld = d.n + (d.n & 1), ldv = d.m + (d.m & 1);
+  const int ld = aca_ldu(d.n), ldv = d.m + (d.m & 1);
   const int64_t nu = (int64_t)d.r * d.n;
   for (int64_t q = threadIdx.x; q < nu; q += blockDim.x) {
     const int64_t l = q / d.n;
@@ -787,8 +787,8 @@ void assemble_batch(Engine& e, int j, int64_t k0, int64_t k1, AcaWorkspace (&ws)
         ch.npiv += 2 * (int64_t)r;
       }
       lv.Rt[k] = R;
-      lv.uoff[k] = ch.nU;
-      ch.nU += (int64_t)(n + (n & 1)) * R;
+      lv.uoff[k] = aca_ustart(ch.nU, n);
+      ch.nU = lv.uoff[k] + (int64_t)aca_ldu(n) * R;
       lv.chunk[k] = (int32_t)e.chunks.size();
     }
     for (int p = 0; p < 3; ++p) {
@@ -815,7 +815,7 @@ void assemble_batch(Engine& e, int j, int64_t k0, int64_t k1, AcaWorkspace (&ws)
         w.pw = P.ds[q].pw;
         w.rcap = P.ds[q].rcap;
         w.pad = 0;
-        w.uo = lv.uoff[k] + (int64_t)lv.col[b] * (w.n + (w.n & 1));
+        w.uo = lv.uoff[k] + (int64_t)lv.col[b] * aca_ldu(w.n);
         w.vo = lv.voff[b];
         w.po = lv.pivoff[b];
         wd.push_back(w);

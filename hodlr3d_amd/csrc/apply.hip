@@ -1068,7 +1068,7 @@ static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int
         P2Item it{};
         it.uoff = lf.uoff[k];
         it.chunk = lv.chunk[k];
-        it.ld = class_elem ? class_ldu(n, class_elem) : n + (n & 1);
+        it.ld = class_elem ? class_ldu(n, class_elem) : aca_ldu(n);
         it.i0 = i0;
         it.nr = std::min(p2rows, n - i0);
         it.R = R;
@@ -1164,7 +1164,7 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
             TP1Item it{};
             it.uoff = lf.uoff[k];
             it.chunk = lv.chunk[k];
-            it.ld = class_elem ? class_ldu(n, class_elem) : n + (n & 1);
+            it.ld = class_elem ? class_ldu(n, class_elem) : aca_ldu(n);
             it.c0 = c0;
             it.nc = nc;
             it.s0 = q * kP1Rows;

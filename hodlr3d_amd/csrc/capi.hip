@@ -630,7 +630,7 @@ int h3d_export_block(h3d_handle* h, int level, int64_t b, int32_t* I, int32_t* J
     if (I) H3D_HIP(hipMemcpy(I, ch.piv + lv.pivoff[b], sizeof(int32_t) * r, hipMemcpyDeviceToHost));
     if (J) H3D_HIP(hipMemcpy(J, ch.piv + lv.pivoff[b] + r, sizeof(int32_t) * r, hipMemcpyDeviceToHost));
     if ((U || V) && !ch.data[0]) throw h3d::Error(H3D_ERR_STATE, "fp64 factors not resident");
-    const int ld = n + (n & 1), ldv = m + (m & 1);
+    const int ld = h3d::aca_ldu(n), ldv = m + (m & 1);
     if (U) {
       const double* src = (const double*)ch.data[0] + lv.uoff[k] + (int64_t)lv.col[b] * ld;
       H3D_HIP(hipMemcpy2D(U, sizeof(double) * n, src, sizeof(double) * ld, sizeof(double) * n, r,

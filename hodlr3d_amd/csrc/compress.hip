@@ -552,7 +552,7 @@ void build_mixed(Engine& e, double tol) {
           c.n = n;
           c.m = m;
           c.r = r;
-          c.ldu = n + (n & 1);
+          c.ldu = aca_ldu(n);
           c.ldv = m + (m & 1);
           c.chunk = lv.chunk[k];
           c.xu = lv.uoff[k] + (int64_t)lv.col[b] * c.ldu;

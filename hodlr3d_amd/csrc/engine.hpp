@@ -9,7 +9,8 @@
 //   factors   stored in chunks; one chunk holds every block of a run of
 //             consecutive target boxes of one level:
 //               U region: per target box k, column-major n_k x R_k with
-//                         ld = n_k rounded up to even, the u-vectors of its
+//                         ld = aca_ldu(n_k) (n_k rounded up to even, or to a
+//                         multiple of 32 above 256 rows), the u-vectors of its
 //                         blocks in interaction-list order
 //               V region: per block, column-major m_b x r_b with ldv = m_b
 //                         rounded up to even (the v-vectors)
@@ -358,6 +359,15 @@ struct Engine {
 
 inline int64_t level_offset(int j) { return ((int64_t(1) << (3 * j)) - 1) / 7; }
 
+// Column stride of the ACA copies' U for a target box of n rows (element
+// offsets are shared by the fp64, fp32 and bf16 copies).  Columns of more than
+// 256 rows span several phase-2 items of every copy (1 KB of rows each), so
+// they start on multiples of 32 elements (256 / 128 / 64 bytes); shorter
+// columns are packed (n rounded up to even).
+__host__ __device__ inline int aca_ldu(int n) { return n > 256 ? (n + 31) / 32 * 32 : n + (n & 1); }
+// Element offset (in its chunk) of the U of such a box placed at `off` or after.
+inline int64_t aca_ustart(int64_t off, int n) { return n > 256 ? (off + 31) / 32 * 32 : off; }
+
 // Column stride of a recompressed class's U for a target box of n rows (elem
 // bytes per entry).  Columns of more than 1 KB span several phase-2 items
 // (1 KB of rows each), so they start on 128-byte lines: an item's piece of a
@@ -415,7 +425,7 @@ void release_mixed(Engine& e);
 // lf[j] describes level j (1..nLevels).
 // p2rows: rows of one phase-2 item (128 for the ACA copies, whose plan serves
 // every precision; 256 / 512 for the fp32 / fp16 classes of the mixed copy);
-// class_elem: 0 for the ACA layout (U stride n rounded up to even), else the
+// class_elem: 0 for the ACA layout (U stride aca_ldu), else the
 // bytes per entry of a mixed class (U stride class_ldu)
 void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out, int p2rows,
                     int class_elem = 0);

@@ -31,7 +31,7 @@ namespace h3d {
 namespace {
 
 constexpr char kMagic[8] = {'H', '3', 'D', 'F', 'A', 'C', 'T', '1'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: U columns of > 256 rows on 32-element strides
 constexpr uint32_t kByteOrder = 0x01020304u;
 constexpr size_t kStage = size_t(64) << 20;  // host staging for factor transfers
 
@@ -368,7 +368,7 @@ void load_engine(Engine& e, const char* path, int64_t ws_budget) {
       if (lv.iloff[k + 1] == lv.iloff[k] && Rt == 0) continue;
       if (c < 0 || c >= h.nchunks) corrupt(r.path, "chunk id");
       const Chunk& ch = e.chunks[c];
-      if (lv.uoff[k] < 0 || lv.uoff[k] + (n + (n & 1)) * Rt > ch.nU) corrupt(r.path, "U offset");
+      if (lv.uoff[k] < 0 || lv.uoff[k] + (int64_t)aca_ldu(n) * Rt > ch.nU) corrupt(r.path, "U offset");
       int64_t R = 0;
       for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
         const int64_t rk = lv.rank[b], m = e.bcount[lo + lv.ilidx[b]];

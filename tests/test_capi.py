@@ -72,7 +72,7 @@ def test_load_refuses_missing_and_foreign_files_before_the_gpu(tmp_path):
     missing = str(tmp_path / "absent.h3d").encode()
     assert L.h3d_load(missing, None, ctypes.byref(h)) == nat.H3D_ERR_IO
     for blob in (b"", b"H3DF", b"not a saved operator at all, just text" * 4,
-                 b"H3DFACT1" + b"\x02\x00\x00\x00" + b"\0" * 200):
+                 b"H3DFACT1" + b"\x03\x00\x00\x00" + b"\0" * 200):
         p = tmp_path / "foreign.h3d"
         p.write_bytes(blob)
         assert L.h3d_load(str(p).encode(), None, ctypes.byref(h)) == nat.H3D_ERR_IO, blob[:12]
