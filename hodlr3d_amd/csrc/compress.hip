@@ -471,10 +471,8 @@ struct BlockRef {
 }  // namespace
 
 void FactorClass::free_all() {
-  for (void*& p : udata)
-    if (p) (void)hipFree(p);
-  for (void*& p : vdata)
-    if (p) (void)hipFree(p);
+  if (arena) (void)hipFree(arena);
+  arena = nullptr;
   udata.clear();
   vdata.clear();
   nU.clear();
@@ -989,18 +987,25 @@ void build_mixed(Engine& e, double tol) {
     mx.cls[0].tbase = 0;
     mx.cls[1].tbase = T[0];
   }
-  // allocate and zero the class chunks
+  // allocate and zero the class chunks: one allocation per class (a few large
+  // allocations rather than one per chunk), each chunk at a 256-byte boundary
   for (int c = 0; c < 2; ++c) {
     FactorClass& fc = mx.cls[c];
     fc.vdata.assign(fc.nV.size(), nullptr);
-    auto alloc = [&](void*& p, int64_t n) {
-      const size_t bytes = (size_t)(n + kChunkTail) * fc.elem;
-      H3D_HIP(hipMalloc(&p, bytes));
-      H3D_HIP(hipMemsetAsync(p, 0, bytes, st));
+    auto span = [&](int64_t n) { return ((n + kChunkTail) * fc.elem + 255) / 256 * 256; };
+    int64_t total = 0;
+    for (int64_t n : fc.nU) total += span(n);
+    for (int64_t n : fc.nV) total += span(n);
+    H3D_HIP(hipMalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256)));
+    H3D_HIP(hipMemsetAsync(fc.arena, 0, (size_t)std::max<int64_t>(total, 256), st));
+    int64_t off = 0;
+    auto carve = [&](void*& p, int64_t n) {
+      p = (char*)fc.arena + off;
+      off += span(n);
       fc.bytes += n * fc.elem;
     };
-    for (size_t ch = 0; ch < fc.nU.size(); ++ch) alloc(fc.udata[ch], fc.nU[ch]);
-    for (size_t vc = 0; vc < fc.nV.size(); ++vc) alloc(fc.vdata[vc], fc.nV[vc]);
+    for (size_t ch = 0; ch < fc.nU.size(); ++ch) carve(fc.udata[ch], fc.nU[ch]);
+    for (size_t vc = 0; vc < fc.nV.size(); ++vc) carve(fc.vdata[vc], fc.nV[vc]);
     fc.d_ubase = upload(fc.udata, st);
     fc.d_vbase = upload(fc.vdata, st);
   }

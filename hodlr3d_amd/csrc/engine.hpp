@@ -209,6 +209,7 @@ struct FactorClass {
   int elem = 4;                      // bytes per stored element (4: fp32, 2: fp16)
   std::vector<LevelFactors> lf;      // [level]; voff is relative to the block's V chunk
   std::vector<void*> udata, vdata;   // U chunks (the ACA copy's chunk ids), V chunks
+  void* arena = nullptr;             // one allocation holding every chunk of the class
   std::vector<int64_t> nU, nV;       // elements per U chunk / V chunk
   void** d_ubase = nullptr;          // per U chunk
   void** d_vbase = nullptr;          // per V chunk
