@@ -132,7 +132,8 @@ int h3d_assemble(h3d_handle* h);
  * products equal the saved handle's bit for bit.  Only opt->workspace_bytes is
  * read from `opt` (may be NULL).  The mixed copy is not saved: h3d_compress
  * rebuilds it deterministically.  H3D_ERR_IO: missing, truncated, foreign or
- * corrupt file. */
+ * corrupt file, or one of another format version (version 2 stores U columns
+ * of more than 256 rows on 32-element strides; version 1 files are refused). */
 int h3d_save(h3d_handle* h, const char* path);
 int h3d_load(const char* path, const h3d_options* opt, h3d_handle** out);
 
