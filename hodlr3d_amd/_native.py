@@ -85,6 +85,8 @@ SIGNATURES = {
     "h3d_kernel_timing": (_i32, [_p, _i32, _p, ctypes.POINTER(_i32)]),
     "h3d_autotune": (_i32, [_p, _i32, _i32, _p]),
     "h3d_set_tuning": (_i32, [_p, _i32, _i32, _i32]),
+    "h3d_get_tuning": (_i32, [_p, _i32, _p]),
+    "h3d_set_tuning_state": (_i32, [_p, _i32, _p]),
     "h3d_set_rhs_block": (_i32, [_p, _i32]),
     "h3d_autotune_rhs": (_i32, [_p, _i32, _i32, _i32, _p, _p]),
     "h3d_compress": (_i32, [_p, _d, ctypes.POINTER(MixedStats)]),

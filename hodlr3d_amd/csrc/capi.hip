@@ -334,6 +334,66 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes) {
   });
 }
 
+namespace {
+// Tuning slots of a mode: {phase-1/phase-2 slot, second slot or -1}.
+void tuning_slots(int prec, int& a, int& b) {
+  const int p = prec & 0xff;
+  if (p < 0 || p > H3D_PREC_MIXED || (prec & ~(0xff | H3D_ACC_FP64)))
+    throw h3d::Error(H3D_ERR_INVALID, "unknown precision");
+  if (p == H3D_PREC_MIXED) {
+    a = 6;
+    b = 7;
+    return;
+  }
+  a = p + ((prec & H3D_ACC_FP64) && p != 0 ? 3 : 0);
+  b = -1;
+}
+}  // namespace
+
+int h3d_get_tuning(h3d_handle* h, int prec, int* v) {
+  return guarded([&] {
+    if (!h || !v) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    int a, b;
+    tuning_slots(prec, a, b);
+    const h3d::Engine& e = h->e;
+    v[0] = e.tune_p1[a];
+    v[1] = e.tune_p2[a];
+    v[2] = b >= 0 ? e.tune_p1[b] : 0;
+    v[3] = b >= 0 ? e.tune_p2[b] : 0;
+    v[4] = b >= 0 ? e.tune_k[a] : 0;
+    v[5] = b >= 0 ? e.tune_k[b] : 0;
+  });
+}
+
+int h3d_set_tuning_state(h3d_handle* h, int prec, const int* v) {
+  return guarded([&] {
+    if (!h || !v) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    int a, b;
+    tuning_slots(prec, a, b);
+    const int p = prec & 0xff;
+    const int es = p == H3D_PREC_MIXED ? 4 : h3d::elem_size(p);
+    auto ok = [&](int w, int lo) { return (w == 16 || w == 8 || w == 4 || w == 2) && w >= lo; };
+    if (!ok(v[0], es) || !ok(v[1], es)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
+    if (b >= 0) {
+      if (v[0] != 16 || v[2] != 16)
+        throw h3d::Error(H3D_ERR_INVALID, "the mixed phase 1 takes 16-byte loads");
+      if (!ok(v[3], 2)) throw h3d::Error(H3D_ERR_INVALID, "bad load width");
+      for (int q : {4, 5})
+        if (v[q] != 0 && v[q] != 1 && v[q] != 4)
+          throw h3d::Error(H3D_ERR_INVALID, "rows per lane must be 0, 1 or 4");
+    }
+    h3d::Engine& e = h->e;
+    e.tune_p1[a] = v[0];
+    e.tune_p2[a] = v[1];
+    if (b >= 0) {
+      e.tune_p1[b] = v[2];
+      e.tune_p2[b] = v[3];
+      e.tune_k[a] = v[4];
+      e.tune_k[b] = v[5];
+    }
+  });
+}
+
 int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* out) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");

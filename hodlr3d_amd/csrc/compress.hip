@@ -83,7 +83,7 @@ struct PItem {
 // block X, row tiles of 32 staged in LDS, each thread owning up to EMAX of
 // the r(r+1)/2 entries (l <= l'); the tile sums are accumulated with Kahan
 // compensation, so long columns add no rounding beyond a 32-term sum.
-template <int EMAX>
+template <int EMAX, class S>
 __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
                                                    void* const* __restrict__ base64,
                                                    const int32_t* __restrict__ list, int nlist,
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
   const int r = d.r;
   const int rows = side ? d.m : d.n;
   const int ld = side ? d.ldv : d.ldu;
-  const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu);
+  const S* X = (const S*)base64[d.chunk] + (side ? d.xv : d.xu);
   const int E = r * (r + 1) / 2;
   int el[EMAX], er[EMAX];
   double acc[EMAX], cmp[EMAX];  // sums with Kahan compensation over the row tiles
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
     __syncthreads();
     for (int idx = threadIdx.x; idx < TR * r; idx += 256) {
       const int l = idx / TR, i = idx - l * TR;
-      tile[l * (TR + 1) + i] = i < nr ? X[(int64_t)l * ld + r0 + i] : 0.0;
+      tile[l * (TR + 1) + i] = i < nr ? (double)X[(int64_t)l * ld + r0 + i] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -357,6 +357,7 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
 // Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
 // and stored in the column's class (k < r32: fp32, else fp16).  Raw blocks
 // copy their columns unscaled into fp32.
+template <class S>
 __global__ __launch_bounds__(256) void project_kernel(
     const PItem* __restrict__ items, const CDesc* __restrict__ ds, const PDesc* __restrict__ ps,
     void* const* __restrict__ base64, const double* __restrict__ ws,
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(256) void project_kernel(
   const int r = d.r;
   const int ld = side ? d.ldv : d.ldu;  // the fp64 copy's stride (and the classes' V stride)
   const int ld32 = side ? ld : p.ldu_o[0], ld16 = side ? ld : p.ldu_o[1];
-  const double* X = (const double*)base64[d.chunk] + (side ? d.xv : d.xu) + row;
+  const S* X = (const S*)base64[d.chunk] + (side ? d.xv : d.xu) + row;
   const double* W = ws + d.w;
   const double* T = W + (side ? (int64_t)r * r : 0);
   const double* lam = W + 2 * (int64_t)r * r;
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(256) void project_kernel(
                                   (side ? p.v_out[1] : p.u_out[1]) + row
                             : nullptr;
   if (p.raw) {
-    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld32] = (float)X[(int64_t)k * ld];
+    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld32] = (float)X[(int64_t)k * ld];  // S -> fp32
     return;
   }
   const int keep = p.r32 + p.r16;
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(256) void project_kernel(
 #pragma unroll
     for (int q = 0; q < KT; ++q) acc[q] = 0.0;
     for (int l = 0; l < r; ++l) {
-      const double x = X[(int64_t)l * ld];
+      const double x = (double)X[(int64_t)l * ld];
 #pragma unroll
       for (int q = 0; q < KT; ++q)
         if (k0 + q < keep) acc[q] += x * T[l + (int64_t)(k0 + q) * r];
@@ -509,8 +510,11 @@ void release_mixed(Engine& e) {
 void build_mixed(Engine& e, double tol) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "compress before assemble");
   if (!(tol > 0.0) || !std::isfinite(tol)) throw Error(H3D_ERR_INVALID, "tol must be > 0");
-  if (e.fac_bytes[0] == 0)
-    throw Error(H3D_ERR_STATE, "the recompressed copy is built from the fp64 factors");
+  // source: the fp64 ACA copy, else the fp32 one (same element offsets; its
+  // rounding, 2^-24 of each entry, is far below any useful budget)
+  const int src = e.fac_bytes[0] ? 0 : (e.fac_bytes[1] ? 1 : -1);
+  if (src < 0)
+    throw Error(H3D_ERR_STATE, "the recompressed copy is built from the fp64 or the fp32 factors");
   auto t0 = std::chrono::steady_clock::now();
   release_mixed(e);
   MixedCopy& mx = e.mix;
@@ -564,9 +568,9 @@ void build_mixed(Engine& e, double tol) {
     while (cur < (int)e.chunks.size()) chunk_first[++cur] = (int32_t)cd.size();
   }
   const int64_t nb = (int64_t)cd.size();
-  // base pointers of the fp64 chunks (whole chunk: U region then V region)
+  // base pointers of the source chunks (whole chunk: U region then V region)
   std::vector<void*> b64(e.chunks.size());
-  for (size_t c = 0; c < e.chunks.size(); ++c) b64[c] = e.chunks[c].data[0];
+  for (size_t c = 0; c < e.chunks.size(); ++c) b64[c] = e.chunks[c].data[src];
   void** d_b64 = upload(b64, st);
 
   // per block workspace size (doubles) and batches bounded by a budget
@@ -629,11 +633,14 @@ void build_mixed(Engine& e, double tol) {
         const int cnt = (int)std::min<int64_t>(1000000, (int64_t)g.size() - s);
         const unsigned grid = 2u * (unsigned)cnt;
         if (which == 0)
-          gram_kernel<4><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+          (src ? gram_kernel<4, float> : gram_kernel<4, double>)<<<grid, 256, lds, st>>>(
+              d_cd, d_b64, d_list + off + s, cnt, d_ws);
         else if (which == 1)
-          gram_kernel<16><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+          (src ? gram_kernel<16, float> : gram_kernel<16, double>)<<<grid, 256, lds, st>>>(
+              d_cd, d_b64, d_list + off + s, cnt, d_ws);
         else
-          gram_kernel<40><<<grid, 256, lds, st>>>(d_cd, d_b64, d_list + off + s, cnt, d_ws);
+          (src ? gram_kernel<40, float> : gram_kernel<40, double>)<<<grid, 256, lds, st>>>(
+              d_cd, d_b64, d_list + off + s, cnt, d_ws);
         H3D_LAUNCH_CHECK();
       }
       off += (int64_t)g.size();
@@ -1030,7 +1037,7 @@ void build_mixed(Engine& e, double tol) {
       PItem* d_it = upload(items, st);
       for (int64_t s = 0; s < (int64_t)items.size(); s += 1 << 30) {
         const unsigned g = (unsigned)std::min<int64_t>(1 << 30, (int64_t)items.size() - s);
-        project_kernel<<<g, 256, 0, st>>>(d_it + s, d_cd, d_pd, d_b64, d_ws, mx.cls[0].d_ubase,
+        (src ? project_kernel<float> : project_kernel<double>)<<<g, 256, 0, st>>>(d_it + s, d_cd, d_pd, d_b64, d_ws, mx.cls[0].d_ubase,
                                           mx.cls[0].d_vbase, mx.cls[1].d_ubase, mx.cls[1].d_vbase);
         H3D_LAUNCH_CHECK();
       }

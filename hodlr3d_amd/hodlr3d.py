@@ -268,6 +268,17 @@ class HODLR3D:
         `prec`; for 'mixed' phase 1 takes 16 only (H3DError otherwise)."""
         nat.check(nat.lib().h3d_set_tuning(self._h, _prec(prec), int(p1_bytes), int(p2_bytes)))
 
+    def get_tuning(self, prec):
+        """The complete tuning state of `prec` (see h3d_get_tuning): six ints."""
+        v = (ctypes.c_int * 6)()
+        nat.check(nat.lib().h3d_get_tuning(self._h, _prec(prec), v))
+        return list(v)
+
+    def set_tuning_state(self, prec, v):
+        """Pin a tuning state read back by ``get_tuning`` (h3d_set_tuning_state)."""
+        a = (ctypes.c_int * 6)(*[int(t) for t in v])
+        nat.check(nat.lib().h3d_set_tuning_state(self._h, _prec(prec), a))
+
     def set_rhs_block(self, width):
         """Group width (16 or 32; 0: one at a time) of products with several
         right-hand sides on the ACA factor copies (matrix-core path)."""

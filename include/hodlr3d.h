@@ -148,12 +148,14 @@ int h3d_release_precision(h3d_handle* h, int prec);
 
 /* Builds the recompressed mixed-precision copy (no reference counterpart; an
  * extension of the reference's HODLR3D::assemble, src/HODLR3D.cpp:17-19) from
- * the fp64 factors: every block B = U V^T is recompressed through the
+ * the fp64 factors, else from the fp32 copy (same blocks; its 2^-24 rounding
+ * per entry stays far below any useful budget): every block B = U V^T is
+ * recompressed through the
  * eigendecomposition of its Gram matrices (B = U' V'^T, V' orthonormal), and
  * its columns are dropped, stored in fp16 or in fp32 so that the expected
  * added relative error ||A_mixed - A||_F / ||A||_F stays within `tol` (one
  * budget over all blocks, spent on the columns with the least error per stored
- * element).  Needs the fp64 copy resident; stats may be NULL.  A matvec with
+ * element).  Needs the fp64 or the fp32 copy resident; stats may be NULL.  A matvec with
  * H3D_PREC_MIXED builds the copy with tol = 5 * 10^-TOL_POW if absent. */
 int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* stats);
 int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* stats);
@@ -203,6 +205,16 @@ int h3d_autotune_rhs(h3d_handle* h, int prec, int nrhs, int reps, double* ms, in
  * bytes only (its items span 64 lanes x 4 vectors of 16 bytes of rows) and
  * phase 2 at least 4. */
 int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
+
+/* The complete tuning state of mode `prec` (the autotuner's choice or what was
+ * pinned): v[0], v[1] = phase-1 / phase-2 load bytes per lane (for
+ * H3D_PREC_MIXED those of its fp32 class), v[2], v[3] = those of the mixed
+ * copy's fp16 class, v[4], v[5] = the mixed phase-1 kernel's load vectors of
+ * rows per lane per class (4 or 1; 0: chosen per item); 0 where a slot does
+ * not apply.  h3d_set_tuning_state pins a state read back earlier, so that a
+ * profiled run can time exactly the kernel variants an autotuned run used. */
+int h3d_get_tuning(h3d_handle* h, int prec, int* v);
+int h3d_set_tuning_state(h3d_handle* h, int prec, const int* v);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
  * stream (enable != 0).  ms[9] receives the accumulated milliseconds of
