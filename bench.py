@@ -5,15 +5,25 @@ precision, HBM GB/s of the factor-streaming kernels, and the refined GMRES solve
 Prints ONE JSON line (see the "metric"/"value" keys).  A step is one matvec of
 the mixed-precision operator on a vector resident in HBM.
 
-  python bench.py [--steps K] [--warmup W] [--config C3|C2|C1] [--no-cpu] [--no-solve]
+  python bench.py [--steps K] [--warmup W] [--config C3|C2|C1] [--no-cpu]
+                  [--no-solve] [--no-spec] [--pin-tuning JSON]
 
 Workloads (synthetic, seeded):
   C3  N = 1,048,576 distinct cells of the cell-centred 128^3 lattice of
-      [-1,1]^3, jitter U(+-h/4), leaf1d 4 (nLevels 5), ACA tol 1e-7, K = 1/r;
-      solve A = I + h^3/(4 pi) K to 1e-10 with GMRES(30) in fp32 inside fp64
-      iterative refinement, b ~ N(0,1)
+      [-1,1]^3, jitter U(+-h/4), leaf1d 4 (nLevels 5), K = 1/r, ACA tol 1e-8;
+      solve A = I + h^3/(4 pi) K to 1e-10 with GMRES(30) on the mixed copy
+      inside fp64 iterative refinement, b ~ N(0,1).  The same points at the
+      precision configuration's own ACA tolerance, 1e-7, are measured first
+      ("spec_tol_1e-7"): there the fp64 ACA operator alone exceeds the 1e-6
+      bar on some N(0,1) right-hand sides, so the headline runs at 1e-8.
   C2  N = 262,144 iid uniform in [-1,1]^3, leaf1d 4 (nLevels 4), tol 1e-8
   C1  16^3 lattice of the reference driver, leaf1d 4 (nLevels 2), tol 1e-10
+
+Errors are against the direct fp64 sum (h3d_direct): the largest over eight
+N(0,1) right-hand sides of 32,768 sampled rows each, and over all N rows for
+the headline right-hand side.  The fp64 factors are timed first, then
+converted to fp32 (they do not fit beside the other copies at tol 1e-8), and
+the recompressed mixed copy is built from the fp32 copy.
 
 After the headline (and with --only-multi instead of it) the products with
 several right-hand sides are measured on N = 262,144 iid uniform points,
@@ -25,7 +35,6 @@ import argparse
 import json
 import math
 import os
-import platform
 import sys
 import time
 
@@ -46,8 +55,9 @@ NEAR_VALU_PER_PAIR = {"near32_kernel<double, InvR32>": 13, "near32_kernel<float,
 NEAR_KERNEL = {"mixed": "near32_kernel<double, InvR32>", "fp32": "near32_kernel<float, InvR32>",
                "bf16": "near32_kernel<float, InvR32>", "fp32/acc64": "near_nr_kernel",
                "bf16/acc64": "near_nr_kernel", "fp64": "near64_kernel<InvR>"}
-NUM_ERR_ROWS = 2048
 REL_ERR_BAR = 1e-6
+DRAWS = 8            # right-hand sides of the error check
+DRAW_ROWS = 32768    # sampled rows of each
 
 
 MODE_DESC = {
@@ -66,12 +76,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(name):
+def workload(name, tol_pow=None):
     from hodlr3d_amd import configs
     if name == "C3":
-        return dict(name="C3", pts=configs.jittered(2 ** 20, seed=0), leaf1d=4, tol_pow=7,
-                    desc="N=1048576 jittered 128^3-lattice subset in [-1,1]^3, leaf1d 4 "
-                         "(nLevels 5), K=1/r, ACA tol 1e-7",
+        t = 8 if tol_pow is None else tol_pow
+        return dict(name="C3", pts=configs.jittered(2 ** 20, seed=0), leaf1d=4, tol_pow=t,
+                    desc=f"N=1048576 jittered 128^3-lattice subset in [-1,1]^3, leaf1d 4 "
+                         f"(nLevels 5), K=1/r, ACA tol 1e-{t}",
                     alpha=configs.second_kind_scale(128))
     if name == "C2":
         return dict(name="C2", pts=configs.uniform(262144, seed=0), leaf1d=4, tol_pow=8,
@@ -119,39 +130,37 @@ def near_metrics(st, res, mode):
     per = NEAR_VALU_PER_PAIR[k]
     evals = st["near_pairs"] / (ms * 1e-3)
     peak = VALU_LANE_OPS / per
+    f64 = res["fp64"]["kernel_ms"]["near"]
     return {"kernel": k, "pairs": st["near_pairs"], "ms": round(ms, 4),
             "evals_per_s": round(evals / 1e9, 1), "unit": "G pair evaluations/s",
             "valu_per_pair": per, "peak": round(peak / 1e9, 1), "frac": round(evals / peak, 4),
-            "fp64_mode": {"kernel": NEAR_KERNEL["fp64"], "ms": round(res["fp64"]["kernel_ms"]["near"], 4),
-                          "evals_per_s": round(st["near_pairs"] / (res["fp64"]["kernel_ms"]["near"] * 1e-3) / 1e9, 1),
-                          "frac": round(st["near_pairs"] / (res["fp64"]["kernel_ms"]["near"] * 1e-3)
-                                        / (VALU_LANE_OPS / NEAR_VALU_PER_PAIR[NEAR_KERNEL["fp64"]]), 4)}}
+            "fp64_mode": {"kernel": NEAR_KERNEL["fp64"], "ms": round(f64, 4),
+                          "evals_per_s": round(st["near_pairs"] / (f64 * 1e-3) / 1e9, 1),
+                          "frac": round(st["near_pairs"] / (f64 * 1e-3)
+                                        / (VALU_LANE_OPS / NEAR_VALU_PER_PAIR[NEAR_KERNEL["fp64"]]),
+                                        4)}}
 
 
-def best_width(tuned):
-    return min(tuned, key=tuned.get)
-
-
-def mixed_kernels(H, km, tuned):
+def mixed_kernels(H, km):
     """(name, ms, algorithmic bytes) of the four far-field kernels of the mixed
     copy: phase 1 / phase 2 of its fp32 class (fp64 x and t) and of its fp16
     class (fp32 x and t).  Phase 1 (the kernel alone, event-timed) reads the
     class's V entries and x once per level and writes t; phase 2 reads the U
     entries and t and writes (fp32 class) or updates (fp16 class) y once per
-    level."""
+    level.  Template arguments follow the tuning the products used."""
     ms_ = H.mixed_stats()
     st = H.geometry_stats()
+    tun = H.get_tuning("mixed")
     lvn = st["nlevels"] * st["N"]
     out = []
-    for c, (es, sx, name, tp) in enumerate(((4, 8, "fp32", "float, double"),
-                                           (2, 4, "fp16", "_Float16, float"))):
+    for c, (es, sx, tp) in enumerate(((4, 8, "float, double"), (2, 4, "_Float16, float"))):
         rank = ms_["rank_fp32"] if c == 0 else ms_["rank_fp16"]
         p1 = es * ms_["v_elems"][c] + sx * lvn + sx * rank
         p2 = es * ms_["u_elems"][c] + sx * rank + 8 * lvn * (1 if c == 0 else 2)
         t1 = km["phase1_kernel"] if c == 0 else km["phase1_kernel_second"]
         t2 = km["phase2"] - km["phase2_second"] if c == 0 else km["phase2_second"]
-        w1 = best_width(tuned["phase1" if c == 0 else "phase1_fp16"]) // es
-        w2 = best_width(tuned["phase2" if c == 0 else "phase2_fp16"]) // es
+        w1 = tun[2 * c] // es
+        w2 = tun[2 * c + 1] // es
         out.append((f"mphase1_kernel<{tp}, {w1}>", t1, p1))
         out.append((f"phase2_kernel<{tp}, {w2}>", t2, p2))
     return out
@@ -170,7 +179,41 @@ def alg_bytes(H, elem, acc64=False):
     return p1, p2
 
 
-def cpu_baseline(wl, q, budget_s):
+class Draws:
+    """Right-hand sides iid N(0,1) with their direct fp64 sums on sampled rows;
+    the direct sums depend only on the kernel, so every mode shares them."""
+
+    def __init__(self, torch, H, n, rows, seed):
+        self.items = []
+        for d in range(n):
+            rng = np.random.default_rng(seed + d)
+            q = torch.from_numpy(rng.standard_normal(H.N)).cuda()
+            r = np.sort(rng.choice(H.N, min(rows, H.N), replace=False))
+            self.items.append((q, r, H.direct(q, r).cpu().numpy()))
+
+    def errors(self, H, prec):
+        out = []
+        for q, r, yd in self.items:
+            y = H.matvec(q, prec=prec).cpu().numpy()
+            out.append(float(np.linalg.norm(y[r] - yd) / np.linalg.norm(yd)))
+        return out
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "omp_threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None}
+
+
+def cpu_sample(wl, q, budget_s):
     """Reference-semantics CPU matvec (entry regeneration + BLAS-2) on a bounded
     sample of target boxes per level, scaled to the full problem."""
     from oracle import oracle as O
@@ -199,12 +242,47 @@ def cpu_baseline(wl, q, budget_s):
     tn = o.time_near(q, mask)
     total_ms += tn * 1e3 * nleaf / kl
     parts.append(f"near:{kl}/{nleaf} leaves")
-    return {"value": round(total_ms, 1), "unit": "ms per matvec (extrapolated)",
+    return {"value": round(total_ms, 1), "unit": "ms per matvec (extrapolated from a sample)",
             "cores": O.max_threads(), "kind": "port",
-            "sample": "reference-semantics far field (entry regeneration + BLAS-2) timed on "
-                      "target boxes sampled per level (" + ", ".join(parts) +
+            "sample": f"{wl['desc']}: reference-semantics far field (entry regeneration + "
+                      "BLAS-2) timed on target boxes sampled per level (" + ", ".join(parts) +
                       "), each level scaled by boxes/sampled; ACA of the sample not counted",
             "wall_s": round(time.perf_counter() - t0, 1)}
+
+
+def full_cpu_vs_gpu(torch, H3, name):
+    """The oracle's whole assembly and reference-form matvec (entries
+    regenerated) on the CPU beside the GPU's fp64 assembly and matvec, for a
+    configuration small enough to run in full."""
+    from oracle import oracle as O
+    wl = workload(name)
+    x = wl["pts"]
+    q = np.random.default_rng(5).standard_normal(len(x))
+    t0 = time.perf_counter()
+    H = H3.HODLR3D(wl["leaf1d"], 1.0, wl["tol_pow"], 7, x)
+    H.assemble()
+    torch.cuda.synchronize()
+    g_asm = time.perf_counter() - t0
+    qd = torch.from_numpy(q).cuda()
+    yg = H.matvec(qd).cpu().numpy()
+    g_ms = timed(torch, lambda: H.matvec(qd), 5, 1)
+    H.close()
+    t0 = time.perf_counter()
+    o = O.Oracle(x, wl["leaf1d"], tol_pow=wl["tol_pow"])
+    o.assemble()
+    c_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    yo = o.matvec(q, mode=0)
+    c_ms = (time.perf_counter() - t0) * 1e3
+    ass, mvt = o.timers()
+    out = {"workload": wl["desc"], "cpu_threads": O.max_threads(),
+           "cpu_assemble_s": round(c_asm, 2), "cpu_matvec_ms": round(c_ms, 1),
+           "cpu_ref_timers_s": {"assTime": round(ass, 3), "matVecTime": round(mvt, 3)},
+           "gpu_assemble_s": round(g_asm, 3), "gpu_fp64_matvec_ms": round(g_ms, 3),
+           "matvec_speedup": round(c_ms / g_ms, 1),
+           "gpu_vs_cpu_rel_diff": float(np.linalg.norm(yg - yo) / np.linalg.norm(yo))}
+    log(f"[bench] full CPU vs GPU {name}: {out}")
+    return out
 
 
 def multi_rhs(torch, H3, nrhs, steps, warmup):
@@ -298,6 +376,35 @@ def batch_instances(torch, H3, count, nmin, nmax, steps, warmup):
     return out
 
 
+def spec_tolerance_run(torch, H3, args):
+    """The precision configuration at its own ACA tolerance 1e-7: fp64 and
+    mixed products, timed, and their errors over the same eight draws."""
+    wl = workload("C3", tol_pow=7)
+    t0 = time.perf_counter()
+    H = H3.HODLR3D(wl["leaf1d"], 1.0, 7, 7, wl["pts"])
+    H.assemble()
+    t_asm = time.perf_counter() - t0
+    draws = Draws(torch, H, DRAWS, DRAW_ROWS, 400)
+    q = draws.items[0][0]
+    out = {"workload": wl["desc"], "assemble_s": round(t_asm, 2),
+           "fp64_GB": round(H.stats()["factor_elems"] * 8 / 1e9, 1)}
+    for mode in ("fp64", "mixed"):
+        if mode == "mixed":
+            H.compress(args.mixed_tol)
+        H.autotune(mode, reps=2)
+        errs = draws.errors(H, mode)
+        ms = timed(torch, lambda: H.matvec(q, prec=mode), args.steps, args.warmup)
+        out[mode] = {"ms": round(ms, 4), "rel_err_max": max(errs),
+                     "rel_err_draws": [float("%.3e" % e) for e in errs],
+                     "draws_over_bar": sum(e > REL_ERR_BAR for e in errs)}
+    out["mixed_tol"] = args.mixed_tol
+    out["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    H.close()
+    torch.cuda.empty_cache()
+    log(f"[bench] spec tol 1e-7: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,11 +419,17 @@ def main():
     # budget keeps a margin under the 1e-6 bar on ||A x||.
     ap.add_argument("--mixed-tol", type=float, default=4e-7,
                     help="added relative error budget (vs ||A||_F) of the recompressed mixed copy")
+    ap.add_argument("--pin-tuning", default=None, metavar="JSON",
+                    help='pin the tuning of modes instead of autotuning them, e.g. '
+                         '\'{"mixed": [16,16,16,16,4,4]}\' (the "tuning" of an earlier line), '
+                         'so that a profiled run times the same kernel variants')
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (for scaling "
                          "HBM counters of a profiler run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-spec", action="store_true",
+                    help="skip the run at the precision configuration's ACA tolerance 1e-7")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--multi-rhs", type=int, default=32, metavar="NRHS",
                     help="also time products with NRHS right-hand sides (0: skip)")
@@ -333,6 +446,7 @@ def main():
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
         sys.exit(2)
+    pins = json.loads(args.pin_tuning) if args.pin_tuning else {}
 
     import torch
     import hodlr3d_amd as H3
@@ -358,10 +472,14 @@ def main():
                           "dtype": "f32", "data": "synthetic",
                           "config": {"workload": mr["workload"]}, "multi_rhs": mr}), flush=True)
         return
+
+    spec = None
+    if args.config == "C3" and not args.no_spec:
+        spec = spec_tolerance_run(torch, H3, args)
+
     wl = workload(args.config)
     N = len(wl["pts"])
     log(f"[bench] {wl['name']}: {wl['desc']}")
-
     t0 = time.perf_counter()
     H = H3.HODLR3D(wl["leaf1d"], 1.0, wl["tol_pow"], 7, wl["pts"])
     t_create = time.perf_counter() - t0
@@ -371,28 +489,32 @@ def main():
     st = H.stats()
     log(f"[bench] create {t_create:.2f}s assemble {t_asm:.2f}s stats {st}")
 
-    rng = np.random.default_rng(1)
-    q_np = rng.standard_normal(N)
+    draws = Draws(torch, H, DRAWS, DRAW_ROWS, 400)
+    # the headline right-hand side, checked on every row
+    q_np = np.random.default_rng(1).standard_normal(N)
     q = torch.from_numpy(q_np).to(dev)
-    rows = np.sort(np.random.default_rng(2).choice(N, size=min(N, NUM_ERR_ROWS), replace=False))
-    y_direct = H.direct(q, rows).cpu().numpy()
+    y_direct = H.direct(q, np.arange(N)).cpu().numpy()
     ynorm = np.linalg.norm(y_direct)
-
-    def rel_err(y):
-        return float(np.linalg.norm(y.cpu().numpy()[rows] - y_direct) / ynorm)
 
     res = {}
     ref = {}
 
     def measure(prec):
         out = {}
-        out["autotune"] = H.autotune(prec, reps=3)
-        log(f"[bench] autotune {prec}: {out['autotune']}")
+        if prec in pins:
+            H.set_tuning_state(prec, pins[prec])
+            out["autotune"] = "pinned"
+        else:
+            out["autotune"] = H.autotune(prec, reps=3)
+        out["tuning"] = H.get_tuning(prec)
+        log(f"[bench] tuning {prec}: {out['autotune']} -> {out['tuning']}")
         y = H.matvec(q, prec=prec)
-        out["rel_err"] = rel_err(y)
+        out["rel_err"] = float(np.linalg.norm(y.cpu().numpy() - y_direct) / ynorm)
+        out["rel_err_draws"] = draws.errors(H, prec)
+        out["rel_err_max"] = max([out["rel_err"]] + out["rel_err_draws"])
         if prec == "fp64":
             ref["y64"] = y.clone()
-        elif "y64" in ref and H.stats()["factor_bytes"][0] > 0:
+        elif "y64" in ref:
             # difference from the fp64 product of the same ACA operator (all N rows)
             out["vs_fp64"] = float(torch.linalg.norm(y - ref["y64"]) / torch.linalg.norm(ref["y64"]))
         # whole matvecs without per-kernel events (the near field then overlaps
@@ -407,7 +529,8 @@ def main():
         out["kernel_ms"] = {k: kt[k] / c for k in ("phase1", "phase2", "near", "combine", "total",
                                                    "phase1_second", "phase2_second",
                                                    "phase1_kernel", "phase1_kernel_second")}
-        log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e}, kernels "
+        log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e} (max over draws "
+            f"{out['rel_err_max']:.3e}), kernels "
             f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }")
         return out
 
@@ -422,29 +545,32 @@ def main():
         del src, dst
 
     res["fp64"] = measure("fp64")
-    modes = [m for m in args.modes.split(",") if m]
+    p1_64, p2_64 = alg_bytes(H, 8)
+    fp64_far_gbs = (p1_64 + p2_64) / ((res["fp64"]["kernel_ms"]["phase1"] +
+                                       res["fp64"]["kernel_ms"]["phase2"]) * 1e-3) / 1e9
+    modes = [m for m in args.modes.split(",") if m and m != "fp64"]
+    # fp64 -> fp32 chunk by chunk: at tol 1e-8 the fp64 copy (252 GB) does not
+    # fit beside the others; the mixed copy is built from the fp32 one
+    H.move_precision("fp32")
     mixed_info = None
     if "mixed" in modes:
-        # built from the fp64 factors, so before they are converted
         modes = ["mixed"] + [m for m in modes if m != "mixed"]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         mixed_info = H.compress(args.mixed_tol)
         mixed_info["wall_s"] = round(time.perf_counter() - t0, 2)
+        mixed_info["source"] = "fp32 ACA copy"
         log(f"[bench] mixed copy: {mixed_info}")
     for mode in modes:
-        base = mode.split("/")[0]
-        if base == "fp32" and H.stats()["factor_bytes"][1] == 0:
-            # fp64 -> fp32 chunk by chunk (both copies do not fit at N = 2^20)
-            H.move_precision("fp32")
         res[mode] = measure(mode)
-    # the headline mode: the fastest reduced-precision mode within the error bar
-    passing = [k for k in modes if res[k]["rel_err"] <= REL_ERR_BAR]
+    # the headline mode: the fastest reduced-precision mode within the error
+    # bar on every right-hand side checked
+    passing = [k for k in modes if res[k]["rel_err_max"] <= REL_ERR_BAR]
     mixed = min(passing, key=lambda k: res[k]["ms"]) if passing else "fp64"
     m = res[mixed]
     km = m["kernel_ms"]
     if mixed == "mixed":
-        kern = mixed_kernels(H, km, m["autotune"])
+        kern = mixed_kernels(H, km)
     else:
         elem = {"fp64": 8, "fp32": 4, "bf16": 2}[mixed.split("/")[0]]
         p1b, p2b = alg_bytes(H, elem, acc64="acc64" in mixed)
@@ -452,14 +578,9 @@ def main():
     dom, dom_ms, dom_bytes = max(kern, key=lambda k: k[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     far_gbs = sum(k[2] for k in kern) / (sum(k[1] for k in kern) * 1e-3) / 1e9
-    p1_64, p2_64 = alg_bytes(H, 8)
-    fp64_far_gbs = (p1_64 + p2_64) / ((res["fp64"]["kernel_ms"]["phase1"] +
-                                       res["fp64"]["kernel_ms"]["phase2"]) * 1e-3) / 1e9
 
     solve = None
     if not args.no_solve:
-        if H.stats()["factor_bytes"][1] == 0:
-            H.move_precision("fp32")  # the inner solver runs on fp32 factors
         H.set_operator(alpha=wl["alpha"], diag=1.0)
         b = torch.from_numpy(np.random.default_rng(3).standard_normal(N)).to(dev)
         torch.cuda.synchronize()
@@ -479,11 +600,6 @@ def main():
         log(f"[bench] solve {solve}")
         H.set_operator(alpha=1.0, diag=0.0)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(wl, q_np, args.cpu_budget)
-        log(f"[bench] cpu baseline {cpu}")
-
     line = {
         "metric": METRIC,
         "value": round(m["ms"], 4),
@@ -501,16 +617,25 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl["desc"], "N": N, "nlevels": st["nlevels"],
                    "mixed": MODE_DESC.get(mixed, mixed), "err_bar": REL_ERR_BAR,
+                   "err_check": f"max over the headline right-hand side on all {N} rows and "
+                                f"{DRAWS} more N(0,1) right-hand sides on {DRAW_ROWS} sampled "
+                                "rows each, against the direct fp64 sum",
                    "mixed_tol": args.mixed_tol if mixed == "mixed" else None,
+                   "aca_tol_note": "ACA tol 1e-8: at the precision configuration's 1e-7 the "
+                                   "fp64 ACA operator alone exceeds 1e-6 on some right-hand "
+                                   "sides (spec_tol_1e-7)",
                    "rhs": "iid N(0,1)", "parallelism": "single GPU"},
         "fp64_ms": round(res["fp64"]["ms"], 4),
         "speedup_vs_fp64": round(res["fp64"]["ms"] / m["ms"], 3),
         "hbm_gbs": round(far_gbs, 1),
         "fp64_hbm_gbs": round(fp64_far_gbs, 1),
         "rel_err": {k: v["rel_err"] for k, v in res.items()},
+        "rel_err_max": {k: v["rel_err_max"] for k, v in res.items()},
+        "rel_err_draws": {k: [float("%.3e" % e) for e in v["rel_err_draws"]]
+                          for k, v in res.items()},
         "rel_diff_vs_fp64": {k: v["vs_fp64"] for k, v in res.items() if "vs_fp64" in v},
         "matvec_ms": {k: round(v["ms"], 4) for k, v in res.items()},
-        "err_rows": len(rows),
+        "tuning": {k: v["tuning"] for k, v in res.items()},
         "solve_ms": solve["ms"] if solve else None,
         "solve": solve,
         "assemble_s": round(t_asm, 2),
@@ -530,12 +655,19 @@ def main():
                                      for k in kern]},
         "mixed_copy": mixed_info,
         "near_field": near_metrics(st, res, mixed),
-        "cpu_baseline": cpu,
-        "host": {"cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()},
+        "spec_tol_1e-7": spec,
+        "host": host_cpu(),
     }
+    H.close()  # frees the headline factors first
+    torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_sample(wl, q_np, args.cpu_budget)
+        log(f"[bench] cpu sample {cpu}")
+        cpu["full_runs"] = {c: full_cpu_vs_gpu(torch, H3, c) for c in ("C1", "C2")}
+    line["cpu_baseline"] = cpu
     if args.multi_rhs > 0:
-        H.close()  # frees the headline factors first
-        torch.cuda.empty_cache()
         line["multi_rhs"] = multi_rhs(torch, H3, args.multi_rhs, min(args.steps, 5), 1)
     print(json.dumps(line), flush=True)
 
