@@ -19,6 +19,7 @@ H3D_ERR_NOT_CONVERGED = 6
 H3D_ERR_INTERNAL = 7
 H3D_ERR_IO = 8
 
+QCHOICE_PLUGIN = 100
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 2
@@ -46,7 +47,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("N", _i64), ("nlevels", _i32), ("max_rank", _i32), ("n_blocks", _i64),
                 ("rank_sum", _i64), ("factor_elems", _i64), ("find_memory", _d),
                 ("near_pairs", _i64), ("t_tree_ms", _d), ("t_assemble_ms", _d),
-                ("factor_bytes", _i64 * 3), ("u_elems", _i64), ("v_elems", _i64)]
+                ("factor_bytes", _i64 * 3), ("u_elems", _i64), ("v_elems", _i64),
+                ("ref_ass_time", _d), ("ref_matvec_time", _d)]
 
 
 class MixedStats(ctypes.Structure):
@@ -86,6 +88,8 @@ SIGNATURES = {
     "h3d_autotune": (_i32, [_p, _i32, _i32, _p]),
     "h3d_set_tuning": (_i32, [_p, _i32, _i32, _i32]),
     "h3d_get_tuning": (_i32, [_p, _i32, _p]),
+    "h3d_get_matrix": (_i32, [_p, _p, _i64, _p, _i64, _p, _i32]),
+    "h3d_plugin_name": (ctypes.c_char_p, []),
     "h3d_set_tuning_state": (_i32, [_p, _i32, _p]),
     "h3d_set_rhs_block": (_i32, [_p, _i32]),
     "h3d_autotune_rhs": (_i32, [_p, _i32, _i32, _i32, _p, _p]),

@@ -142,9 +142,7 @@ __device__ void canon_reduce(double (&a)[NV][256 / T], double* lds, double (&out
 template <class KF>
 __device__ inline double kentry(const KernelDev& kd, double xa, double ya, double za, double xb,
                                 double yb, double zb) {
-  double dx = xa - xb, dy = ya - yb, dz = za - zb;
-  double R2 = dx * dx + dy * dy + dz * dz;
-  return kd.scale * KF::f64(kd.q, R2);
+  return kd.scale * kpair<KF>(kd.q, xa, ya, za, xb, yb, zb);
 }
 
 __device__ inline bool bit_get(const uint32_t* m, int s) { return (m[s >> 5] >> (s & 31)) & 1u; }
@@ -638,10 +636,13 @@ void launch_pass(Engine& e, Pass& P, AcaWorkspace& ws, hipStream_t st) {
     P.off[cls] = off;
     if (sub.empty()) continue;
     if (smax > 160 * 1024) throw Error(H3D_ERR_INVALID, "ACA block too large for LDS");
-    const void* kfun = cls == 0 ? (fast ? (const void*)aca_kernel<64, InvR>
-                                        : (const void*)aca_kernel<64, AnyQ>)
-                                : (fast ? (const void*)aca_kernel<256, InvR>
-                                        : (const void*)aca_kernel<256, AnyQ>);
+    const bool plug = e.kern.q == kQPlugin;
+    const void* kfun = cls == 0 ? (fast   ? (const void*)aca_kernel<64, InvR>
+                                   : plug ? (const void*)aca_kernel<64, Plugin>
+                                          : (const void*)aca_kernel<64, AnyQ>)
+                                : (fast   ? (const void*)aca_kernel<256, InvR>
+                                   : plug ? (const void*)aca_kernel<256, Plugin>
+                                          : (const void*)aca_kernel<256, AnyQ>);
     if (smax > 64 * 1024)
       H3D_HIP(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smax));
     AcaDesc* d_ds = ws.D + off;

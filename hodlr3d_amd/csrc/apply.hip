@@ -678,7 +678,7 @@ namespace {
 // every lane works.  A wave's LDS operations complete in order and no other
 // wave uses its slice, so the staging needs no barrier, only to stay in
 // program order (wave barriers for the compiler).
-template <class KF>
+template <class KF, bool SWAP = false>
 __global__ __launch_bounds__(256) void near64_kernel(
     const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
@@ -718,9 +718,10 @@ __global__ __launch_bounds__(256) void near64_kernel(
         __builtin_amdgcn_wave_barrier();  // before this chunk's reads
         const int self = ip - cs - s0;
         auto pair = [&](const double2& a, const double2& b, int u) {
-          const double dx = xi - a.x, dy = yi - a.y, dz = zi - b.x;
-          const double R2 = dx * dx + dy * dy + dz * dz;
-          const double v = kd.scale * KF::f64(kd.q, R2) * b.y;
+          // entry (target, source); SWAP: (source, target), the transpose's
+          const double k = SWAP ? kpair<KF>(kd.q, a.x, a.y, b.x, xi, yi, zi)
+                                : kpair<KF>(kd.q, xi, yi, zi, a.x, a.y, b.x);
+          const double v = kd.scale * k * b.y;
           if constexpr (self_box) acc += u == self ? 0.0 : v;
           else acc += v;
         };
@@ -971,9 +972,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
   double s = 0.0, c = 0.0;
   for (int64_t q = threadIdx.x; q < N; q += 256) {
     if (q == ip) continue;
-    const double dx = xi - px[q], dy = yi - py[q], dz = zi - pz[q];
-    const double R2 = dx * dx + dy * dy + dz * dz;
-    const double v = kd.scale * KF::f64(kd.q, R2) * b[perm[q]];
+    const double v = kd.scale * kpair<KF>(kd.q, xi, yi, zi, px[q], py[q], pz[q]) * b[perm[q]];
     const double t = s + v;
     c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
     s = t;
@@ -996,7 +995,59 @@ __global__ __launch_bounds__(256) void direct_kernel(
   }
 }
 
+// One thread per entry of the column-major block: K(i, j) for rows i and
+// columns j in the caller's order; i == j takes the diagonal value.
+template <class KF>
+__global__ __launch_bounds__(256) void entries_kernel(
+    const int64_t* __restrict__ rows, int64_t nr, const int64_t* __restrict__ cols, int64_t n,
+    int64_t base, const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const int32_t* __restrict__ iperm, KernelDev kd,
+    const double* __restrict__ diag_vec, double* __restrict__ out) {
+  const int64_t idx = base + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n) return;
+  const int64_t c = idx / nr, r = idx - c * nr;
+  const int64_t i = rows[r], j = cols[c];
+  double v;
+  if (i == j) {
+    v = diag_vec ? diag_vec[i] : kd.diag;
+  } else {
+    const int64_t a = iperm[i], b = iperm[j];
+    v = kd.scale * kpair<KF>(kd.q, px[a], py[a], pz[a], px[b], py[b], pz[b]);
+  }
+  out[idx] = v;
+}
+
 }  // namespace
+
+void get_entries(Engine& e, const int64_t* h_rows, int64_t nrows, const int64_t* h_cols,
+                 int64_t ncols, double* d_out) {
+  if (nrows <= 0 || ncols <= 0) return;
+  for (int64_t r = 0; r < nrows; ++r)
+    if (h_rows[r] < 0 || h_rows[r] >= e.N) throw Error(H3D_ERR_INVALID, "row out of range");
+  for (int64_t c = 0; c < ncols; ++c)
+    if (h_cols[c] < 0 || h_cols[c] >= e.N) throw Error(H3D_ERR_INVALID, "column out of range");
+  hipStream_t st = e.stream;
+  DevBuf<int64_t> d_rows(nrows), d_cols(ncols);
+  H3D_HIP(hipMemcpyAsync(d_rows.p, h_rows, sizeof(int64_t) * nrows, hipMemcpyHostToDevice, st));
+  H3D_HIP(hipMemcpyAsync(d_cols.p, h_cols, sizeof(int64_t) * ncols, hipMemcpyHostToDevice, st));
+  const int64_t n = nrows * ncols;
+  const int64_t per = int64_t(1) << 30;  // entries per launch (2^22 workgroups)
+  for (int64_t base = 0; base < n; base += per) {
+    const unsigned g = (unsigned)((std::min(per, n - base) + 255) / 256);
+    const bool invr = e.kern.q == 7 || e.kern.q == 16;
+    if (invr)
+      entries_kernel<InvR><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
+                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    else if (e.kern.q == kQPlugin)
+      entries_kernel<Plugin><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px,
+                                                e.d_py, e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    else
+      entries_kernel<AnyQ><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
+                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    H3D_LAUNCH_CHECK();
+  }
+  H3D_HIP(hipStreamSynchronize(st));
+}
 
 void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out) {
   if (nrows <= 0) return;
@@ -1011,6 +1062,10 @@ void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nr
       direct_kernel<InvR><<<g, 256, 0, st>>>(d_rows + off, nrows - off, e.d_px, e.d_py, e.d_pz,
                                              e.d_perm, e.d_iperm, e.N, e.kern, e.alpha, e.d_diag,
                                              d_b, d_out + off);
+    else if (e.kern.q == kQPlugin)
+      direct_kernel<Plugin><<<g, 256, 0, st>>>(d_rows + off, nrows - off, e.d_px, e.d_py,
+                                               e.d_pz, e.d_perm, e.d_iperm, e.N, e.kern, e.alpha,
+                                               e.d_diag, d_b, d_out + off);
     else
       direct_kernel<AnyQ><<<g, 256, 0, st>>>(d_rows + off, nrows - off, e.d_px, e.d_py, e.d_pz,
                                              e.d_perm, e.d_iperm, e.N, e.kern, e.alpha, e.d_diag,
@@ -1620,7 +1675,7 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
   check_mode(e, mode, prec, acc64);
   ensure_copy(e, prec);
   ensure_scratch(e, 1);
-  if (block_width(e, nrhs, prec, transpose) > 1) {
+  if (block_width(e, nrhs, prec, transpose, acc64) > 1) {
     matvec_block(e, d_b, d_y, nrhs, prec, acc64);
     return;
   }
@@ -1646,7 +1701,17 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
     // fp64: the exact entries
     const bool invr = kd.q == 7 || kd.q == 16;
-    if (mixed && invr)
+    // the plugin: its entries in fp64 for every mode (from the fp64 x)
+    if (kd.q == kQPlugin && transpose)
+      near64_kernel<Plugin, true><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
+                                                     e.d_bstart + lo, e.d_bcount + lo, e.d_px,
+                                                     e.d_py, e.d_pz, e.d_perm, kd, e.d_xp,
+                                                     e.d_ynear);
+    else if (kd.q == kQPlugin)
+      near64_kernel<Plugin><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
+                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
+                                               kd, e.d_xp, e.d_ynear);
+    else if (mixed && invr)
       near32_kernel<double, InvR32><<<g, 256, 0, ns>>>(
           e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
           e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
@@ -1749,6 +1814,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       H3D_HIP(hipEventElapsedTime(&ms[7], e.ev[1], e.ev[8]));  // first copy's phase-1 kernel
       H3D_HIP(hipEventElapsedTime(&ms[8], e.ev[6], e.ev[9]));  // second copy's phase-1 kernel
       for (int q = 0; q < 9; ++q) e.ktime[q] += ms[q];
+      e.ref_blas2_ms += ms[0] + ms[1];  // far field: the BLAS-2 applies
+      e.ref_regen_ms += ms[2];          // near field: entries regenerated in the product
       e.kcount++;
     }
   }

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "kernels.hpp"
 
 struct h3d_handle {
   h3d::Engine e;
@@ -185,7 +186,8 @@ int h3d_create(const double* xyz, int64_t N, int leaf1d, double L, int tol_pow,
       throw h3d::Error(H3D_ERR_INVALID, "nLevels must be in [1,7] (got " + std::to_string(nL) + ")");
     h3d_kernel k{7, 1.0, 0.0, nullptr};
     if (kernel) k = *kernel;
-    if (k.qchoice < 0 || k.qchoice > 16) throw h3d::Error(H3D_ERR_INVALID, "qchoice out of range");
+    if ((k.qchoice < 0 || k.qchoice > 16) && k.qchoice != h3d::kQPlugin)
+      throw h3d::Error(H3D_ERR_INVALID, "qchoice out of range");
     auto* h = new h3d_handle();
     try {
       h3d::Engine& e = h->e;
@@ -240,6 +242,9 @@ int h3d_assemble(h3d_handle* h) {
 int h3d_save(h3d_handle* h, const char* path) {
   return guarded([&] {
     if (!h || !path) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (h->e.kern.q == h3d::kQPlugin)
+      throw h3d::Error(H3D_ERR_INVALID, "plugin kernels are not saved (a file cannot name the "
+                                        "plugin built into the library that reads it)");
     h3d::save_engine(h->e, path);
   });
 }
@@ -479,7 +484,7 @@ static int do_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int pr
       return;
     }
     h3d::ensure_scratch(e, 1);
-    const int G = h3d::block_width(e, nrhs, prec & 0xff, transpose);
+    const int G = h3d::block_width(e, nrhs, prec & 0xff, transpose, (prec & H3D_ACC_FP64) != 0);
     if (G > 1) {  // stage whole groups of columns
       const int Wg = e.rhs_block == 16 ? 16 : 32;
       if (e.io_w < Wg) {
@@ -620,6 +625,30 @@ int h3d_direct(h3d_handle* h, const double* b, const int64_t* rows, int64_t nrow
   });
 }
 
+int h3d_get_matrix(h3d_handle* h, const int64_t* rows, int64_t nrows, const int64_t* cols,
+                   int64_t ncols, double* out, int where) {
+  return guarded([&] {
+    if (!h || (nrows > 0 && ncols > 0 && (!rows || !cols || !out)))
+      throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    if (nrows < 0 || ncols < 0) throw h3d::Error(H3D_ERR_INVALID, "negative size");
+    if (where != H3D_HOST && where != H3D_DEVICE) throw h3d::Error(H3D_ERR_INVALID, "bad where");
+    if (nrows == 0 || ncols == 0) return;
+    if (nrows > INT64_MAX / ncols) throw h3d::Error(H3D_ERR_INVALID, "block too large");
+    h3d::Engine& e = h->e;
+    if (where == H3D_DEVICE) {
+      h3d::get_entries(e, rows, nrows, cols, ncols, out);
+      return;
+    }
+    h3d::DevBuf<double> d(nrows * ncols);
+    h3d::get_entries(e, rows, nrows, cols, ncols, d.p);
+    H3D_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * nrows * ncols, hipMemcpyDeviceToHost,
+                           e.stream));
+    H3D_HIP(hipStreamSynchronize(e.stream));
+  });
+}
+
+const char* h3d_plugin_name(void) { return h3d_user_kernel::name; }
+
 int h3d_export_perm(h3d_handle* h, int32_t* perm) {
   return guarded([&] {
     if (!h || !perm) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
@@ -722,6 +751,8 @@ int h3d_get_stats(h3d_handle* h, h3d_stats* s) {
     for (int p = 0; p < 3; ++p) s->factor_bytes[p] = e.fac_bytes[p];
     s->u_elems = e.u_elems;
     s->v_elems = e.v_elems;
+    s->ref_ass_time = e.ref_regen_ms * 1e-3;
+    s->ref_matvec_time = e.ref_blas2_ms * 1e-3;
   });
 }
 

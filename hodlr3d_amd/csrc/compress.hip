@@ -434,8 +434,7 @@ __global__ __launch_bounds__(256) void near_fro2_kernel(
       const int cs = bstart[nn], m = bcount[nn];
       for (int jp = cs; jp < cs + m; ++jp) {
         if (jp == ip) continue;
-        const double dx = px[ip] - px[jp], dy = py[ip] - py[jp], dz = pz[ip] - pz[jp];
-        const double v = kd.scale * KF::f64(kd.q, dx * dx + dy * dy + dz * dz);
+        const double v = kd.scale * kpair<KF>(kd.q, px[ip], py[ip], pz[ip], px[jp], py[jp], pz[jp]);
         acc += v * v;
       }
     }
@@ -731,6 +730,10 @@ void build_mixed(Engine& e, double tol) {
         near_fro2_kernel<InvR><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[nL].d_nbr, e.d_bstart + lo,
                                                   e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.kern,
                                                   d_out);
+      else if (e.kern.q == kQPlugin)
+        near_fro2_kernel<Plugin><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[nL].d_nbr,
+                                                    e.d_bstart + lo, e.d_bcount + lo, e.d_px,
+                                                    e.d_py, e.d_pz, e.kern, d_out);
       else
         near_fro2_kernel<AnyQ><<<g, 256, 0, st>>>(e.d_near, nn, e.lev[nL].d_nbr, e.d_bstart + lo,
                                                   e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.kern,

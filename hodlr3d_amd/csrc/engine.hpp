@@ -346,6 +346,8 @@ struct Engine {
   hipEvent_t ev[10] = {};
   double ktime[9] = {};
   int kcount = 0;
+  // reference-semantics timers (never reset), fed by the per-kernel timing
+  double ref_regen_ms = 0, ref_blas2_ms = 0;
 
   // stats
   int max_rank = 0;
@@ -399,7 +401,7 @@ void save_engine(Engine& e, const char* path);
 void load_engine(Engine& e, const char* path, int64_t ws_budget);
 // multi.hip: group width for nrhs right-hand sides (1: one at a time), and
 // the grouped product (forward, ACA factor copies)
-int block_width(const Engine& e, int nrhs, int prec, bool transpose);
+int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64);
 void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64);
 // times the grouped product of nrhs columns for every group width and
 // phase-1 item size, keeps the fastest; ms[8] (width-major), best[2]
@@ -415,6 +417,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec
 // matvec itself allocates nothing and can be captured into a graph.
 void prepare_mode(Engine& e, int mode);
 void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out);
+// Kernel entries K(rows[r], cols[c]) (original indices, host arrays) into the
+// column-major nrows x ncols d_out (kernel::getMatrix, src/kernel.cpp:37-50).
+void get_entries(Engine& e, const int64_t* h_rows, int64_t nrows, const int64_t* h_cols,
+                 int64_t ncols, double* d_out);
 // Times every load width of phase 1 and phase 2 for one precision mode on the
 // assembled problem and keeps the fastest; ms[8] = phase 1 at 16/8/4/2 bytes,
 // then phase 2 at 16/8/4/2 bytes (negative when the width does not apply).
@@ -449,6 +455,17 @@ void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
   p = nullptr;
 }
+// Owner of a device buffer for the duration of one call: freed on every exit,
+// exceptions included (hipFree waits for the device first).
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  explicit DevBuf(int64_t n) : p(dalloc<T>(n)) {}
+  ~DevBuf() { dfree(p); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
 // Device copy of a host vector (at least one element), queued on `st`.
 template <class T>
 T* upload(const std::vector<T>& v, hipStream_t st) {

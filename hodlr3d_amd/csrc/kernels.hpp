@@ -9,7 +9,17 @@
 
 #include <hip/hip_runtime.h>
 
+// The user kernel plugin (h3d_user_kernel), chosen at build time.
+#ifdef H3D_KERNEL_PLUGIN
+#include H3D_KERNEL_PLUGIN
+#else
+#include "../../include/h3d_plugin_plummer.hpp"
+#endif
+
 namespace h3d {
+
+// Qchoice of the plugin kernel (H3D_QCHOICE_PLUGIN in include/hodlr3d.h).
+constexpr int kQPlugin = 100;
 
 static __device__ __noinline__ double phi64_any(int q, double R2) {
   double R = sqrt(R2);
@@ -38,12 +48,36 @@ static __device__ __noinline__ double phi64_any(int q, double R2) {
 
 // Kernel functors.  InvR covers Qchoice 7 and 16 (1/r, the hot case) inline;
 // AnyQ calls the out-of-line catalogue so the fast kernels keep few registers.
+// Catalogue functors are radial (f64 of r^2); Plugin evaluates the user's
+// entry from the two points (kPoints).
 struct InvR {
+  static constexpr bool kPoints = false;
   __device__ static inline double f64(int, double R2) { return 1 / sqrt(R2); }
 };
 struct AnyQ {
+  static constexpr bool kPoints = false;
   __device__ static inline double f64(int q, double R2) { return phi64_any(q, R2); }
 };
+struct Plugin {
+  static constexpr bool kPoints = true;
+  __device__ static inline double pts(double xa, double ya, double za, double xb, double yb,
+                                      double zb) {
+    return h3d_user_kernel::entry(h3d_point{xa, ya, za}, h3d_point{xb, yb, zb});
+  }
+};
+
+// Unscaled entry between two distinct points a and b (phi(r_ab) for the
+// catalogue, with r^2 = (dx*dx + dy*dy) + dz*dz as the reference forms it).
+template <class KF>
+__device__ inline double kpair(int q, double xa, double ya, double za, double xb, double yb,
+                               double zb) {
+  if constexpr (KF::kPoints) {
+    return KF::pts(xa, ya, za, xb, yb, zb);
+  } else {
+    const double dx = xa - xb, dy = ya - yb, dz = za - zb;
+    return KF::f64(q, dx * dx + dy * dy + dz * dz);
+  }
+}
 
 static __device__ __noinline__ float phi32_any(int q, float R2);
 
@@ -95,6 +129,7 @@ struct AnyQ32 {
 // 1/r in fp64 from the fp32 reciprocal square root refined by one Newton step,
 // y1 = y0 (3 - r^2 y0^2) / 2 (relative error ~1e-7 -> ~2e-14).
 struct InvRNewton {
+  static constexpr bool kPoints = false;
   __device__ static inline double f64(int, double R2) {
     const double y0 = (double)__builtin_amdgcn_rsqf(fmaxf((float)R2, 1.17549435e-38f));
     return y0 * (1.5 - 0.5 * R2 * y0 * y0);

@@ -420,9 +420,7 @@ __global__ __launch_bounds__(256) void near_w64_kernel(
     if (nn < 0) continue;
     const int cs = bstart[nn], m = bcount[nn];
     for (int s = cs; s < cs + m; ++s) {
-      const double dx = xi - px[s], dy = yi - py[s], dz = zi - pz[s];
-      const double R2 = dx * dx + dy * dy + dz * dz;
-      const double k = s == ip ? 0.0 : KF::f64(kd.q, R2);
+      const double k = s == ip ? 0.0 : kpair<KF>(kd.q, xi, yi, zi, px[s], py[s], pz[s]);
       const double* xs = xr + (int64_t)s * W;
 #pragma unroll
       for (int j = 0; j < W; ++j) acc[j] += k * xs[j];
@@ -643,7 +641,11 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
                                                       e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
                                                       e.d_leafc, kd, xr, e.d_ynw);
     } else {
-      if (invr && prec != H3D_PREC_FP64)
+      if (kd.q == kQPlugin)
+        near_w64_kernel<Plugin, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_px, e.d_py, e.d_pz,
+                                                      kd, xr, e.d_ynw);
+      else if (invr && prec != H3D_PREC_FP64)
         near_w64_kernel<InvRNewton, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
                                                           e.d_bcount + lo, e.d_px, e.d_py,
                                                           e.d_pz, kd, xr, e.d_ynw);
@@ -708,8 +710,10 @@ void release_group_tables(Engine& e) {
   e.n_gitems = e.n_rcols = e.g_np = 0;
 }
 
-int block_width(const Engine& e, int nrhs, int prec, bool transpose) {
+int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64) {
   if (transpose || prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
+  // the grouped fp32-sum near field works on fp32 offsets of radial kernels
+  if (e.kern.q == kQPlugin && prec != H3D_PREC_FP64 && !acc64) return 1;
   return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
 }
 

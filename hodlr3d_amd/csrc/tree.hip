@@ -21,6 +21,7 @@
 #include <cmath>
 
 #include "engine.hpp"
+#include "kernels.hpp"
 
 namespace h3d {
 
@@ -289,7 +290,16 @@ void build_tree(Engine& e, const double* h_xyz) {
   dfree(d_key2);
   dfree(d_val);
   dfree(d_cnt);
-  if (flag) throw Error(H3D_ERR_NONFINITE, "duplicate points (K(i,j) = 1/0)");
+  // coincident points are fine for kernels finite at r = 0: Qchoice 0-6 and
+  // 11 (src/kernel.cpp:76-143 evaluate phi(0)), a plugin whose entry of a
+  // point with itself is finite, or a zero kernel
+  if (flag) {
+    const int q = e.kern.q;
+    const h3d_point o{0.0, 0.0, 0.0};
+    const bool finite0 = e.kern.scale == 0.0 || (q >= 0 && q <= 6) || q == 11 ||
+                         (q == kQPlugin && std::isfinite(h3d_user_kernel::entry(o, o)));
+    if (!finite0) throw Error(H3D_ERR_NONFINITE, "duplicate points (K(i,j) infinite at r = 0)");
+  }
 
   // neighbour slots and interaction lists per level
   e.lev.assign(nL + 1, Level());

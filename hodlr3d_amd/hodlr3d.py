@@ -9,7 +9,12 @@ Names and argument meaning follow SAFRAN-LAB/HODLR3D:
   * ``classGMRES().gmres(H, b, maxIterations, threshold)`` -- include/gmres.hpp:13-27,
     src/gmres.cpp:40-94, returning what the reference writes to its reference
     arguments: ``(x, error, noOfIterations, e)``;
-  * ``findMemory()`` / ``getMaxRank()`` / ``assTime`` -- src/HODLR3DTree.cpp:1757-1818.
+  * ``H.A`` (the tree): ``findMemory()``, ``getMaxRank()``, ``assTime``,
+    ``matVecTime`` -- src/HODLR3DTree.cpp:1757-1818, include/HODLR3DTree.hpp:46;
+  * ``H.mykernel``: ``getMatrixEntry(i, j)``, ``getRow``, ``getCol``,
+    ``getMatrix`` -- include/kernel.hpp:19-33, src/kernel.cpp:7-62;
+  * ``userkernel(H3.QCHOICE_PLUGIN)`` selects the kernel plugin built into the
+    library (include/h3d_kernel_plugin.hpp).
 Every call goes through libhodlr3d.so (the C ABI of include/hodlr3d.h) and runs
 on the GPU; invalid input raises ``H3DError`` instead of undefined behaviour.
 """
@@ -80,6 +85,67 @@ class userkernel:
         self.scale = float(scale)
         self.diag = float(diag)
         self.diag_vec = None if diag_vec is None else np.ascontiguousarray(diag_vec, np.float64)
+
+
+QCHOICE_PLUGIN = nat.QCHOICE_PLUGIN
+
+
+def plugin_name():
+    """Name of the kernel plugin compiled into libhodlr3d.so."""
+    return nat.lib().h3d_plugin_name().decode()
+
+
+class _BoundKernel:
+    """Kernel entries of a handle's points, evaluated on the GPU
+    (h3d_get_matrix): the reference's kernel::getRow/getCol/getMatrix
+    (src/kernel.cpp:7-62) with original indices."""
+
+    def __init__(self, H):
+        self._H = H
+        self.Qchoice = H.kernel.Qchoice
+
+    def getMatrix(self, row_indices, col_indices):
+        r = np.ascontiguousarray(row_indices, dtype=np.int64).ravel()
+        c = np.ascontiguousarray(col_indices, dtype=np.int64).ravel()
+        out = np.empty((len(c), len(r)))  # column-major nrows x ncols
+        nat.check(nat.lib().h3d_get_matrix(self._H._h, r.ctypes.data_as(ctypes.c_void_p), len(r),
+                                           c.ctypes.data_as(ctypes.c_void_p), len(c),
+                                           out.ctypes.data_as(ctypes.c_void_p), nat.HOST))
+        return out.T
+
+    def getMatrixEntry(self, i, j):
+        return float(self.getMatrix([i], [j])[0, 0])
+
+    def getRow(self, j, col_indices):
+        return self.getMatrix([j], col_indices)[0]
+
+    def getCol(self, k, row_indices=None):
+        rows = np.arange(self._H.N) if row_indices is None else row_indices
+        return self.getMatrix(rows, [k])[:, 0]
+
+
+class _Tree:
+    """The reference's HODLR3DTree members the drivers read (H->A->...)."""
+
+    def __init__(self, H):
+        self._H = H
+
+    @property
+    def assTime(self):
+        """Seconds of kernel-entry regeneration inside products (the on-the-fly
+        near field), accumulated while per-kernel timing is on."""
+        return self._H.stats()["ref_ass_time"]
+
+    @property
+    def matVecTime(self):
+        """Seconds of the low-rank BLAS-2 applies (far field), accumulated the same way."""
+        return self._H.stats()["ref_matvec_time"]
+
+    def findMemory(self):
+        return self._H.stats()["find_memory"]
+
+    def getMaxRank(self):
+        return self._H.stats()["max_rank"]
 
 
 def _is_torch_cuda(a):
@@ -182,8 +248,12 @@ class HODLR3D:
         return self.stats()["max_rank"]
 
     @property
-    def assTime(self):
-        return self.stats()["t_assemble_ms"] * 1e-3
+    def A(self):
+        return _Tree(self)
+
+    @property
+    def mykernel(self):
+        return _BoundKernel(self)
 
     # -- engine API
     def _bind_stream(self):
@@ -381,7 +451,8 @@ class HODLR3D:
                 "find_memory": s.find_memory, "near_pairs": s.near_pairs,
                 "t_tree_ms": s.t_tree_ms, "t_assemble_ms": s.t_assemble_ms,
                 "factor_bytes": list(s.factor_bytes), "u_elems": s.u_elems,
-                "v_elems": s.v_elems}
+                "v_elems": s.v_elems, "ref_ass_time": s.ref_ass_time,
+                "ref_matvec_time": s.ref_matvec_time}
 
     def geometry_stats(self):
         """Sizes that set the algorithmic bytes of a matvec."""

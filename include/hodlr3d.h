@@ -43,6 +43,10 @@ extern "C" {
  * fp32/bf16-stored factors. */
 #define H3D_ACC_FP64 0x100
 
+/* Qchoice of the user kernel plugin compiled into the library
+ * (include/h3d_kernel_plugin.hpp). */
+#define H3D_QCHOICE_PLUGIN 100
+
 /* ---- pointer locations ---- */
 #define H3D_HOST 0
 #define H3D_DEVICE 1
@@ -55,9 +59,14 @@ typedef struct h3d_handle h3d_handle;
  * reference's Qchoice numbering 0..16; scale = 1, diag = 0 reproduces the
  * reference for Qchoice 0..15.  For Qchoice 16 (IE_CUBE_3D,
  * src/kernel.cpp:65-73) pass scale = diag = 0 to get h3 = 1/N and
- * Kii = 1 + the self-cell integral (src/Integral3D.cpp:9-43). */
+ * Kii = 1 + the self-cell integral (src/Integral3D.cpp:9-43).  With
+ * qchoice = H3D_QCHOICE_PLUGIN the entry is scale * h3d_user_kernel::entry of
+ * the two points, a device functor built into the library
+ * (include/h3d_kernel_plugin.hpp): the reference's user-defined
+ * getMatrixEntry.  Coincident points are accepted for kernels finite at
+ * r = 0 (Qchoice 0-6, 11, a plugin finite there) and refused otherwise. */
 typedef struct h3d_kernel {
-  int qchoice;
+  int qchoice;             /* 0..16, or H3D_QCHOICE_PLUGIN */
   double scale;
   double diag;
   const double* diag_vec; /* optional per-point diagonal (host, original order) */
@@ -85,6 +94,15 @@ typedef struct h3d_stats {
   int64_t factor_bytes[3]; /* resident factor bytes per precision (0 if absent) */
   int64_t u_elems;         /* sum over blocks of r*n (U entries, read by phase 2) */
   int64_t v_elems;         /* sum over blocks of r*m (V entries, read by phase 1) */
+  /* Timers with the reference's meaning (HODLR3DTree::assTime / matVecTime,
+   * include/HODLR3DTree.hpp:46, accumulated in src/HODLR3DTree.cpp:1660-1755
+   * and never reset), in seconds, summed over every product made while
+   * per-kernel timing is on (h3d_kernel_timing): ref_ass_time = regeneration
+   * of kernel entries inside products (here the on-the-fly near field; the far
+   * field stores its factors), ref_matvec_time = the low-rank BLAS-2 applies
+   * (the far-field kernels). */
+  double ref_ass_time;
+  double ref_matvec_time;
 } h3d_stats;
 
 typedef struct h3d_mixed_stats {
@@ -250,6 +268,19 @@ int h3d_solve_ir(h3d_handle* h, const double* b, double* x, int restart, int max
  * drivers form exact products from kernel columns (examples/testHODLR3D.cpp:139-143). */
 int h3d_direct(h3d_handle* h, const double* b, const int64_t* rows, int64_t nrows, double* out,
                int where);
+
+/* Kernel entries: out[r + c * nrows] = K(rows[r], cols[c]) for original
+ * indices (host arrays), the diagonal value where they are equal; out follows
+ * `where`.  Replaces kernel::getMatrix(row_indices, col_indices)
+ * (src/kernel.cpp:37-50); kernel::getRow(j, cols) (:7-15) and getCol(k, rows)
+ * / getCol(k) (:17-35) are its one-row and one-column cases.  Entries of the
+ * kernel only: the apply-time alpha of h3d_set_operator is not applied. */
+int h3d_get_matrix(h3d_handle* h, const int64_t* rows, int64_t nrows, const int64_t* cols,
+                   int64_t ncols, double* out, int where);
+
+/* Name of the kernel plugin compiled into the library (Qchoice
+ * H3D_QCHOICE_PLUGIN). */
+const char* h3d_plugin_name(void);
 
 /* Structure export for bit-exact checks (the reference keeps these in
  * HODLR3DBox, include/HODLR3DTree.hpp:11-24). */

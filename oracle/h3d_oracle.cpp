@@ -42,6 +42,14 @@
 #include <vector>
 #include <omp.h>
 
+// The user kernel plugin (h3d_user_kernel, Qchoice 100), the same header the
+// library is built with.
+#ifdef H3D_KERNEL_PLUGIN
+#include H3D_KERNEL_PLUGIN
+#else
+#include "../include/h3d_plugin_plummer.hpp"
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------- kernels --
@@ -81,6 +89,7 @@ inline double phi(int q, double R2) {
 inline double entry(const KernelSpec& k, double xi, double yi, double zi, double xj, double yj,
                     double zj, int64_t i, int64_t j) {
   if (i == j) return k.diag_vec ? k.diag_vec[i] : k.diag;
+  if (k.q == 100) return k.scale * h3d_user_kernel::entry({xi, yi, zi}, {xj, yj, zj});
   double dx = xi - xj, dy = yi - yj, dz = zi - zj;
   double R2 = dx * dx + dy * dy + dz * dz;
   return k.scale * phi(k.q, R2);

@@ -34,3 +34,21 @@ def test_cpp_mirror_builds_and_reports_errors(tmp_path):
                            "-L", lib, "-lhodlr3d", f"-Wl,-rpath,{lib}", "-o", str(exe)])
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 3 and "nLevels" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_reference_driver_compiles_against_the_mirror(tmp_path):
+    """tests/cpp/ref_driver.cpp makes the reference driver's calls (H->A->assTime,
+    H->A->findMemory(sum), H->mykernel->getCol(k), MatVecProduct on an
+    Eigen-shaped vector); without a GPU it must fail with the library's error."""
+    exe = tmp_path / "drv"
+    lib = os.path.join(ROOT, "hodlr3d_amd")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", os.path.join(ROOT, "tests", "cpp",
+                                                                    "ref_driver.cpp"),
+                           "-I", os.path.join(ROOT, "include"), "-L", lib, "-lhodlr3d",
+                           f"-Wl,-rpath,{lib}", "-o", str(exe)])
+    r = subprocess.run([str(exe), "4", "2", "1.0", "7", "7", "1", str(tmp_path / "o.bin")],
+                       capture_output=True, text=True)
+    import torch
+    if not torch.cuda.is_available():
+        assert r.returncode == 1 and "hodlr3d:" in r.stdout

@@ -1,0 +1,166 @@
+"""The reference's interface end to end (GPU): the C++ mirror driven as the
+reference's driver drives HODLR3D (examples/testHODLR3D.cpp:103-154), the user
+kernel plugin (getMatrixEntry, include/kernel.hpp:19-22), kernel entry gathers
+(kernel::getRow/getCol/getMatrix, src/kernel.cpp:7-62), coincident points for
+kernels finite at r = 0, and the reference-semantics timers -- each checked
+against the CPU oracle on the same points."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import lattice, uniform
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "ref_driver")
+
+
+def run_driver(tmp_path, cube, leaf, tol, q, seed=3):
+    if not os.path.exists(DRIVER):
+        import __graft_entry__ as g
+        g.build_ref_driver()
+    out = tmp_path / f"drv_{cube}_{q}.bin"
+    print(f"[driver] cubeRootN {cube} leaf1d {leaf} TOL_POW {tol} Qchoice {q}", flush=True)
+    r = subprocess.run([DRIVER, str(cube), str(leaf), "1.0", str(tol), str(q), str(seed), str(out)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout.strip(), flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = np.fromfile(out, dtype=np.float64)
+    N, nL, err, mem, mr, ass, mvt, ns = d[:8]
+    ones = d[8:8 + int(ns)].astype(np.int64)
+    Hb = d[8 + int(ns):]
+    assert len(Hb) == int(N)
+    return dict(N=int(N), nLevels=int(nL), err=err, mem=mem, maxrank=int(mr), assTime=ass,
+                matVecTime=mvt, ones=ones, Hb=Hb)
+
+
+@pytest.mark.parametrize("cube,leaf,tol,q", [(16, 4, 10, 7), (12, 3, 9, 3),
+                                             (12, 3, 9, 100)])
+def test_reference_driver_through_the_cpp_mirror(tmp_path, cube, leaf, tol, q):
+    res = run_driver(tmp_path, cube, leaf, tol, q)
+    x = lattice(cube)
+    o = O.Oracle(x, leaf, tol_pow=tol, qchoice=q)
+    o.assemble()
+    b = np.zeros(len(x))
+    b[res["ones"]] = 1.0
+    yo = o.matvec(b)
+    assert res["N"] == len(x) and res["nLevels"] == o.nLevels
+    assert np.linalg.norm(res["Hb"] - yo) / np.linalg.norm(yo) < 1e-12
+    # the driver's own check: H b against the sum of exact kernel columns
+    assert res["err"] < 50 * 10.0 ** (-tol)
+    assert res["mem"] == o.find_memory() and res["maxrank"] == o.max_rank()
+    assert res["assTime"] > 0 and res["matVecTime"] > 0
+
+
+@pytest.fixture(scope="module")
+def plugin_pair():
+    import hodlr3d_amd as H3
+    x = uniform(20000, 71)
+    h = H3.HODLR3D(4, 1.0, 9, H3.QCHOICE_PLUGIN, x)
+    h.assemble()
+    o = O.Oracle(x, 4, tol_pow=9, qchoice=100)
+    o.assemble()
+    yield x, h, o
+    h.close()
+
+
+def test_plugin_is_built_in():
+    import hodlr3d_amd as H3
+    assert "plummer" in H3.plugin_name()
+
+
+def test_plugin_ranks_pivots_and_product_match_the_oracle(plugin_pair):
+    x, h, o = plugin_pair
+    for j in range(1, h.nLevels + 1):
+        rg, _ = h.ranks(j)
+        ro, _ = o.ranks(j)
+        assert np.array_equal(rg, ro), j
+        for b in np.random.default_rng(j).choice(len(rg), min(len(rg), 100), replace=False):
+            if rg[b]:
+                Ig, Jg, _, _ = h.block(j, b, rg[b])
+                Io, Jo, _, _, _, _ = o.block(j, b, ro[b])
+                assert np.array_equal(Ig, Io) and np.array_equal(Jg, Jo)
+    q = np.random.default_rng(72).standard_normal(len(x))
+    yg, yo = h.matvec(q), o.matvec(q)
+    assert np.linalg.norm(yg - yo) / np.linalg.norm(yo) < 1e-12
+    rows = np.random.default_rng(73).choice(len(x), 64, replace=False)
+    yd = h.direct(q, rows)
+    assert np.allclose(yd, O.direct_rows(x, q, rows, qchoice=100), rtol=1e-13, atol=0)
+    assert np.linalg.norm(yg[rows] - yd) / np.linalg.norm(yd) < 1e-7
+    for prec, tol in (("fp32/acc64", 1e-6), ("mixed", 5e-6)):
+        assert np.linalg.norm(h.matvec(q, prec=prec) - yg) / np.linalg.norm(yg) < tol, prec
+
+
+def test_plugin_transpose_is_the_adjoint(plugin_pair):
+    x, h, o = plugin_pair
+    rng = np.random.default_rng(74)
+    a, b = rng.standard_normal(len(x)), rng.standard_normal(len(x))
+    lhs = b @ h.matvec(a)
+    rhs = a @ h.matvec_transpose(b)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(a) * np.linalg.norm(b) * 10
+
+
+def test_entries_match_the_oracle(plugin_pair):
+    import hodlr3d_amd as H3
+    x, h, o = plugin_pair
+    rng = np.random.default_rng(75)
+    rows = rng.choice(len(x), 40, replace=False)
+    cols = np.concatenate([rng.choice(len(x), 30, replace=False), rows[:3]])  # with i == j
+    for hh, q in ((h, 100), (None, 7)):
+        if hh is None:
+            hh = H3.HODLR3D(4, 1.0, 9, 7, x)
+        M = hh.mykernel.getMatrix(rows, cols)
+        ref = np.array([[O.entry(q, x[i], x[j], i, j) for j in cols] for i in rows])
+        assert np.array_equal(M, ref), q
+        assert np.array_equal(hh.mykernel.getRow(int(rows[0]), cols), ref[0])
+        col = hh.mykernel.getCol(int(cols[1]))
+        assert col.shape == (len(x),) and col[cols[1]] == 0.0
+        assert hh.mykernel.getMatrixEntry(int(rows[1]), int(cols[2])) == ref[1, 2]
+        if q == 7:
+            hh.close()
+
+
+def test_coincident_points_for_kernels_finite_at_zero():
+    import hodlr3d_amd as H3
+    from hodlr3d_amd import _native as nat
+    x = uniform(6000, 76)
+    x = np.concatenate([x, x[:500]])  # 500 points twice
+    q = np.random.default_rng(77).standard_normal(len(x))
+    for qc in (3, 5, 100):
+        h = H3.HODLR3D(4, 1.0, 9, qc, x)
+        h.assemble()
+        o = O.Oracle(x, 4, tol_pow=9, qchoice=qc)
+        o.assemble()
+        for j in range(1, h.nLevels + 1):
+            assert np.array_equal(h.ranks(j)[0], o.ranks(j)[0]), (qc, j)
+        yg, yo = h.matvec(q), o.matvec(q)
+        assert np.linalg.norm(yg - yo) / np.linalg.norm(yo) < 1e-12, qc
+        h.close()
+    with pytest.raises(nat.H3DError) as ei:
+        H3.HODLR3D(4, 1.0, 9, 7, x)
+    assert ei.value.code == nat.H3D_ERR_NONFINITE
+
+
+def test_reference_timers_accumulate():
+    import hodlr3d_amd as H3
+    x = uniform(20000, 78)
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    q = np.random.default_rng(79).standard_normal(len(x))
+    assert h.A.assTime == 0 and h.A.matVecTime == 0
+    h.kernel_timing(True)
+    y1 = h.MatVecProduct(q)
+    a1, m1 = h.A.assTime, h.A.matVecTime
+    assert a1 > 0 and m1 > 0
+    y2 = h.MatVecProduct(q)
+    assert h.A.assTime > a1 and h.A.matVecTime > m1  # never reset
+    h.kernel_timing(False)
+    h.MatVecProduct(q)
+    assert h.A.matVecTime == h.stats()["ref_matvec_time"]
+    assert np.array_equal(y1, y2)
+    assert h.A.findMemory() == h.findMemory() and h.A.getMaxRank() == h.getMaxRank()
+    h.close()

@@ -56,3 +56,18 @@ def test_ie_cube_self_term():
     exact_unit = 3 * math.log((math.sqrt(3) + 1) / (math.sqrt(3) - 1)) / 2 - math.pi / 4
     a = h / 2
     assert (val - 1) == pytest.approx(8 * a * a * exact_unit / (4 * math.pi), rel=2e-2)
+
+
+def test_plugin_entry_is_the_plummer_kernel():
+    """Qchoice 100 is the plugin built in (include/h3d_plugin_plummer.hpp):
+    1/sqrt(r^2 + eps^2), eps = 1/64, finite for coincident points."""
+    eps = 1.0 / 64.0
+    for r in R:
+        a = np.array([0.1, -0.2, 0.3])
+        b = a + r * np.array([0.6, 0.0, 0.8])
+        d = a - b
+        ref = 1.0 / math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps * eps)
+        assert O.entry(100, a, b) == ref
+    a = np.array([0.3, 0.3, 0.3])
+    assert O.entry(100, a, a, 0, 1) == 64.0  # coincident points, distinct indices
+    assert O.entry(100, a, a, 2, 2, diag=5.0) == 5.0  # i == j: the diagonal
