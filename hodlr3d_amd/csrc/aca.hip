@@ -9,7 +9,8 @@
 //     (maxAbsVector, ACA.cpp:10-23);
 //   * residual row/column: kernel entry minus sum_l u_l[i] v_l[s], l outer,
 //     no FMA (ACA.cpp:181-193); u = column / pivot;
-//   * reject when ||u|| or ||v|| < 1e-16 (ACA.cpp:196-200);
+//   * reject when ||u|| or ||v|| < 1e-16 (ACA.cpp:196-200), or when the
+//     pivot is exactly zero (there the reference divides by zero);
 //   * normS^2 = normS^2 + (|u||v|)^2 + sum_{l<=r} 2|u_l.u v_l.v| including the
 //     new pair (ACA.cpp:206-212); stop when |u||v| <= tol normS or r = min(n,m);
 //   * next row = argmax of the raw residual column over the remaining rows.
@@ -345,6 +346,10 @@ __global__ __launch_bounds__(T) void aca_kernel(const AcaDesc* __restrict__ desc
     Arg g = block_argmax<T>(best, amx);
     jj = g.idx;
     piv = g.v;
+    // A residual row that vanishes on every remaining column (a block with
+    // coincident points, rank reached exactly): the reference would divide by
+    // this zero pivot (NaN factors); stop as its small-norm test does.
+    if (piv == 0.0) break;
     for (int l = t; l < r; l += T) vjl[l] = V[(int64_t)l * m + jj];
     __syncthreads();
     const double xj = cx[jj], yj = cy[jj], zj = cz[jj];
@@ -1042,6 +1047,64 @@ void finish_assembly(Engine& e) {
   }
   upload_chunk_table(e);
   e.assembled = true;
+}
+
+// Kernel entries for h3d_get_matrix, in this translation unit so that they
+// are formed without contraction, exactly as the ACA and the oracle form them.
+namespace {
+
+// One thread per entry of the column-major block: K(i, j) for rows i and
+// columns j in the caller's order; i == j takes the diagonal value.
+template <class KF>
+__global__ __launch_bounds__(256) void entries_kernel(
+    const int64_t* __restrict__ rows, int64_t nr, const int64_t* __restrict__ cols, int64_t n,
+    int64_t base, const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ pz, const int32_t* __restrict__ iperm, KernelDev kd,
+    const double* __restrict__ diag_vec, double* __restrict__ out) {
+  const int64_t idx = base + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n) return;
+  const int64_t c = idx / nr, r = idx - c * nr;
+  const int64_t i = rows[r], j = cols[c];
+  double v;
+  if (i == j) {
+    v = diag_vec ? diag_vec[i] : kd.diag;
+  } else {
+    const int64_t a = iperm[i], b = iperm[j];
+    v = kd.scale * kpair<KF>(kd.q, px[a], py[a], pz[a], px[b], py[b], pz[b]);
+  }
+  out[idx] = v;
+}
+
+}  // namespace
+
+void get_entries(Engine& e, const int64_t* h_rows, int64_t nrows, const int64_t* h_cols,
+                 int64_t ncols, double* d_out) {
+  if (nrows <= 0 || ncols <= 0) return;
+  for (int64_t r = 0; r < nrows; ++r)
+    if (h_rows[r] < 0 || h_rows[r] >= e.N) throw Error(H3D_ERR_INVALID, "row out of range");
+  for (int64_t c = 0; c < ncols; ++c)
+    if (h_cols[c] < 0 || h_cols[c] >= e.N) throw Error(H3D_ERR_INVALID, "column out of range");
+  hipStream_t st = e.stream;
+  DevBuf<int64_t> d_rows(nrows), d_cols(ncols);
+  H3D_HIP(hipMemcpyAsync(d_rows.p, h_rows, sizeof(int64_t) * nrows, hipMemcpyHostToDevice, st));
+  H3D_HIP(hipMemcpyAsync(d_cols.p, h_cols, sizeof(int64_t) * ncols, hipMemcpyHostToDevice, st));
+  const int64_t n = nrows * ncols;
+  const int64_t per = int64_t(1) << 30;  // entries per launch (2^22 workgroups)
+  for (int64_t base = 0; base < n; base += per) {
+    const unsigned g = (unsigned)((std::min(per, n - base) + 255) / 256);
+    const bool invr = e.kern.q == 7 || e.kern.q == 16;
+    if (invr)
+      entries_kernel<InvR><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
+                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    else if (e.kern.q == kQPlugin)
+      entries_kernel<Plugin><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px,
+                                                e.d_py, e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    else
+      entries_kernel<AnyQ><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
+                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
+    H3D_LAUNCH_CHECK();
+  }
+  H3D_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace h3d

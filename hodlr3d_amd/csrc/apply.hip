@@ -995,59 +995,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
   }
 }
 
-// One thread per entry of the column-major block: K(i, j) for rows i and
-// columns j in the caller's order; i == j takes the diagonal value.
-template <class KF>
-__global__ __launch_bounds__(256) void entries_kernel(
-    const int64_t* __restrict__ rows, int64_t nr, const int64_t* __restrict__ cols, int64_t n,
-    int64_t base, const double* __restrict__ px, const double* __restrict__ py,
-    const double* __restrict__ pz, const int32_t* __restrict__ iperm, KernelDev kd,
-    const double* __restrict__ diag_vec, double* __restrict__ out) {
-  const int64_t idx = base + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n) return;
-  const int64_t c = idx / nr, r = idx - c * nr;
-  const int64_t i = rows[r], j = cols[c];
-  double v;
-  if (i == j) {
-    v = diag_vec ? diag_vec[i] : kd.diag;
-  } else {
-    const int64_t a = iperm[i], b = iperm[j];
-    v = kd.scale * kpair<KF>(kd.q, px[a], py[a], pz[a], px[b], py[b], pz[b]);
-  }
-  out[idx] = v;
-}
-
 }  // namespace
-
-void get_entries(Engine& e, const int64_t* h_rows, int64_t nrows, const int64_t* h_cols,
-                 int64_t ncols, double* d_out) {
-  if (nrows <= 0 || ncols <= 0) return;
-  for (int64_t r = 0; r < nrows; ++r)
-    if (h_rows[r] < 0 || h_rows[r] >= e.N) throw Error(H3D_ERR_INVALID, "row out of range");
-  for (int64_t c = 0; c < ncols; ++c)
-    if (h_cols[c] < 0 || h_cols[c] >= e.N) throw Error(H3D_ERR_INVALID, "column out of range");
-  hipStream_t st = e.stream;
-  DevBuf<int64_t> d_rows(nrows), d_cols(ncols);
-  H3D_HIP(hipMemcpyAsync(d_rows.p, h_rows, sizeof(int64_t) * nrows, hipMemcpyHostToDevice, st));
-  H3D_HIP(hipMemcpyAsync(d_cols.p, h_cols, sizeof(int64_t) * ncols, hipMemcpyHostToDevice, st));
-  const int64_t n = nrows * ncols;
-  const int64_t per = int64_t(1) << 30;  // entries per launch (2^22 workgroups)
-  for (int64_t base = 0; base < n; base += per) {
-    const unsigned g = (unsigned)((std::min(per, n - base) + 255) / 256);
-    const bool invr = e.kern.q == 7 || e.kern.q == 16;
-    if (invr)
-      entries_kernel<InvR><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
-                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
-    else if (e.kern.q == kQPlugin)
-      entries_kernel<Plugin><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px,
-                                                e.d_py, e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
-    else
-      entries_kernel<AnyQ><<<g, 256, 0, st>>>(d_rows.p, nrows, d_cols.p, n, base, e.d_px, e.d_py,
-                                              e.d_pz, e.d_iperm, e.kern, e.d_diag, d_out);
-    H3D_LAUNCH_CHECK();
-  }
-  H3D_HIP(hipStreamSynchronize(st));
-}
 
 void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nrows, double* d_out) {
   if (nrows <= 0) return;

@@ -479,11 +479,14 @@ static int do_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int pr
     if (!e.assembled) throw h3d::Error(H3D_ERR_STATE, "matvec before assemble");
     if (nrhs == 0) return;
     const int64_t N = e.N;
+    // Build what the mode needs (a factor copy, the mixed copy, scratch sized
+    // for it) before x is staged: doing it inside the product would free and
+    // reallocate the staging buffer that already holds x.
+    h3d::prepare_mode(e, prec);
     if (where == H3D_DEVICE) {
       h3d::matvec_device(e, b, y, nrhs, prec, transpose);
       return;
     }
-    h3d::ensure_scratch(e, 1);
     const int G = h3d::block_width(e, nrhs, prec & 0xff, transpose, (prec & H3D_ACC_FP64) != 0);
     if (G > 1) {  // stage whole groups of columns
       const int Wg = e.rhs_block == 16 ? 16 : 32;

@@ -74,6 +74,9 @@ __device__ inline double kpair(int q, double xa, double ya, double za, double xb
   if constexpr (KF::kPoints) {
     return KF::pts(xa, ya, za, xb, yb, zb);
   } else {
+    // formed without contraction in every translation unit, so the entries
+    // equal the reference's (and the ACA's) bit for bit
+#pragma clang fp contract(off)
     const double dx = xa - xb, dy = ya - yb, dz = za - zb;
     return KF::f64(q, dx * dx + dy * dy + dz * dz);
   }

@@ -495,6 +495,10 @@ void aca_block(const Handle& h, int rs, int n, int cs, int m, Block& B) {
       for (int s = 0; s < m; ++s) row[s] -= ul * vl[s];
     }
     jj = argmax_rem(row, crem);
+    // an exactly zero pivot (residual row zero on every remaining column, e.g.
+    // coincident points): the reference divides by it and its factors turn
+    // NaN; stop here, as the small-norm test below does (ACA.cpp:196-200)
+    if (row[jj] == 0.0) break;
     for (int s = 0; s < n; ++s) col[s] = K(s, jj);
     for (int l = 0; l < r; ++l) {
       double vl = V[(size_t)l * m + jj];

@@ -174,3 +174,21 @@ def test_refined_solve_with_the_mixed_inner_operator():
     r = b - h.matvec(sol, prec="fp64")
     assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(b), info
     h.close()
+
+
+def test_first_mixed_product_of_host_vectors_builds_the_copy_first():
+    """A host-vector product in the mixed mode on a handle without the mixed
+    copy builds the copy (and the larger scratch it needs) before x is staged;
+    building it inside the product used to reallocate the staging buffer that
+    held x.  Regression test: the first product equals the later ones."""
+    import hodlr3d_amd as H3
+    x = np.random.default_rng(81).uniform(-1, 1, (20000, 3))
+    q = np.random.default_rng(82).standard_normal(len(x))
+    h = H3.HODLR3D(4, 1.0, 9, 7, x)
+    h.assemble()
+    y64 = h.matvec(q)
+    ym1 = h.matvec(q, prec="mixed")
+    ym2 = h.matvec(q, prec="mixed")
+    assert np.array_equal(ym1, ym2)
+    assert np.linalg.norm(ym1 - y64) / np.linalg.norm(y64) < 1e-6
+    h.close()
