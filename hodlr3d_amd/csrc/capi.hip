@@ -64,7 +64,7 @@ void Engine::free_all() {
   dfree(d_ylev);
   dfree(d_ynear);
   if (side) (void)hipStreamSynchronize(side);
-  for (hipEvent_t* x : {&ev_fork, &ev_join})
+  for (hipEvent_t* x : {&ev_fork, &ev_join, &ev_switch})
     if (*x) (void)hipEventDestroy(*x), *x = nullptr;
   if (side) (void)hipStreamDestroy(side);
   side = nullptr;
@@ -827,10 +827,11 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       if (branch[i] < 0) branch[i] = next++ % nb;
     }
     std::vector<hipStream_t> saved(count);
-    std::vector<bool> timing(count);
+    std::vector<bool> timing(count), overlap(count);
     for (int i = 0; i < count; ++i) {
       saved[i] = hs[i]->e.stream;
       timing[i] = hs[i]->e.timing;
+      overlap[i] = hs[i]->e.overlap_near;
     }
     // captured products keep every kernel on their branch (the near field in
     // line, not forked to the handle's second stream)
@@ -839,7 +840,7 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
       for (int i = 0; i < count; ++i) {
         hs[i]->e.stream = saved[i];
         hs[i]->e.timing = timing[i];
-        hs[i]->e.overlap_near = true;
+        hs[i]->e.overlap_near = overlap[i];
       }
     };
     auto* g = new h3d_graph;
@@ -893,17 +894,30 @@ void h3d_graph_destroy(h3d_graph* g) {
   delete g;
 }
 
+namespace {
+// Moves a handle to stream `to` behind everything already queued on its
+// current stream: all of a handle's products share its scratch, so a product
+// on the new stream must not start before the old stream's ones have ended.
+void switch_stream(h3d::Engine& e, hipStream_t to) {
+  if (to == e.stream) return;
+  if (!e.ev_switch) H3D_HIP(hipEventCreateWithFlags(&e.ev_switch, hipEventDisableTiming));
+  H3D_HIP(hipEventRecord(e.ev_switch, e.stream));
+  H3D_HIP(hipStreamWaitEvent(to, e.ev_switch, 0));
+  e.stream = to;
+}
+}  // namespace
+
 int h3d_set_stream(h3d_handle* h, void* stream) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    h->e.stream = (hipStream_t)stream;
+    switch_stream(h->e, (hipStream_t)stream);
   });
 }
 
 int h3d_use_own_stream(h3d_handle* h) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    h->e.stream = h->e.own_stream;
+    switch_stream(h->e, h->e.own_stream);
   });
 }
 

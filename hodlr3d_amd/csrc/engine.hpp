@@ -309,6 +309,7 @@ struct Engine {
   double* d_ynear = nullptr;  // near field by position: N
   hipStream_t side = nullptr;                              // the near field's stream
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;         // its fork / join events
+  hipEvent_t ev_switch = nullptr;  // orders a new stream after the old (h3d_set_stream)
   bool overlap_near = true;  // near field on `side` (off while a graph is captured)
   double* d_io = nullptr;     // staging for host vectors
   int64_t t_cap = 0, part_cap = 0;  // allocated entries of d_t / d_part per right-hand side

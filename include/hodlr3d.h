@@ -334,7 +334,10 @@ int h3d_graph_launch(h3d_graph* g, void* stream);
 void h3d_graph_destroy(h3d_graph* g);
 
 /* Run subsequent calls on a caller-owned hipStream_t (passed as void*; NULL is
- * the legacy default stream); h3d_use_own_stream returns to the handle's own. */
+ * the legacy default stream); h3d_use_own_stream returns to the handle's own.
+ * Either switch makes the new stream wait for the work already queued on the
+ * old one (an event, also valid under stream capture), since all products of
+ * a handle share its scratch. */
 int h3d_set_stream(h3d_handle* h, void* stream);
 int h3d_use_own_stream(h3d_handle* h);
 int h3d_synchronize(h3d_handle* h);

@@ -682,6 +682,10 @@ void near_apply(const Handle& h, const double* qp, double* pot, const uint8_t* l
 void matvec(Handle& h, const double* q, double* out, bool use_uv, const uint8_t* leafmask) {
   const int nL = h.nLevels;
   const int64_t N = h.N;
+  if (use_uv && !h.keep_uv) {  // the ACA vectors were not kept: no vector form
+    for (int64_t s = 0; s < N; ++s) out[s] = std::numeric_limits<double>::quiet_NaN();
+    return;
+  }
   std::vector<double> qp(N), total(N, 0.0), pot(N);
   for (int64_t s = 0; s < N; ++s) qp[s] = q[h.perm[s]];  // assignCharges (:1647-1658)
   for (int j = 1; j <= nL; ++j) {

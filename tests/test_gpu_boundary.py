@@ -143,11 +143,18 @@ def test_coincident_points_for_kernels_finite_at_zero():
     for qc in (3, 6, 100):  # kernels of sqrt and division only: bit-exact entries
         h = H3.HODLR3D(4, 1.0, 9, qc, x)
         h.assemble()
-        o = O.Oracle(x, 4, tol_pow=9, qchoice=qc)
+        o = O.Oracle(x, 4, tol_pow=9, qchoice=qc, keep_uv=True)
         o.assemble()
         for j in range(1, h.nLevels + 1):
-            assert np.array_equal(h.ranks(j)[0], o.ranks(j)[0]), (qc, j)
-        yg, yo = h.matvec(q), o.matvec(q)
+            rg, ro = h.ranks(j)[0], o.ranks(j)[0]
+            assert np.array_equal(rg, ro), (qc, j)
+            for b in np.random.default_rng(j).choice(len(rg), min(len(rg), 200), replace=False):
+                if rg[b]:
+                    assert np.array_equal(h.block(j, b, rg[b])[0], o.block(j, b, ro[b])[0])
+        # the blocks may end on tiny pivots here, where the reference's CUR form
+        # (K(:,J) R^-1 L^-1 K(I,:)) amplifies rounding: compare with the
+        # oracle's product from the same ACA vectors the GPU applies
+        yg, yo = h.matvec(q), o.matvec(q, mode=1)
         assert np.all(np.isfinite(yg)), qc
         assert np.linalg.norm(yg - yo) / np.linalg.norm(yo) < 1e-12, qc
         h.close()

@@ -44,3 +44,29 @@ def test_nonfinite_rhs_propagates_without_fault():
     y = h.matvec(q)
     assert np.isnan(y).any()
     h.close()
+
+
+def test_torch_and_numpy_products_interleave_in_order():
+    """Device-tensor products run on torch's stream and host-array products on
+    the handle's own; both share the handle's scratch, so each switch of stream
+    must wait for the other stream's work.  Interleaved with no explicit
+    synchronisation, every product equals the one made alone."""
+    import torch
+    import hodlr3d_amd as H3
+    x = np.random.default_rng(101).uniform(-1, 1, (60000, 3))
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    qs = [np.random.default_rng(102 + i).standard_normal(len(x)) for i in range(6)]
+    ref = [h.matvec(q) for q in qs]
+    torch.cuda.synchronize()
+    outs = []
+    for i, q in enumerate(qs):
+        if i % 2:
+            outs.append(h.matvec(torch.from_numpy(q).cuda()))
+        else:
+            outs.append(h.matvec(q))
+    torch.cuda.synchronize()
+    for i, (o, r) in enumerate(zip(outs, ref)):
+        o = o.cpu().numpy() if hasattr(o, "cpu") else o
+        assert np.array_equal(o, r), i
+    h.close()
