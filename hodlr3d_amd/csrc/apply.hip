@@ -14,6 +14,7 @@
 // Every factor element is read exactly once per matvec.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "engine.hpp"
@@ -1767,6 +1768,116 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       e.kcount++;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free product (h3d_matvec_noalloc).  Every buffer a product
+// uses beyond its input and output -- x by position (fp64 and fp32), t, the
+// phase-1 partial sums, the per-level y and the near field -- is carved from
+// this array instead of the handle's own scratch.  Each GPU the library runs
+// on holds one copy until the process ends; every call on that GPU shares it
+// and it keeps the last call's contents, so calls that overlap on one GPU race.
+namespace {
+
+constexpr int64_t kNoAllocBytes = int64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current GPU's copy of `scratch` (looked up once
+// per device; host code never uses the array's own name as an address).
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  H3D_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(H3D_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    H3D_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(H3D_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return (unsigned char*)g_scratch_ptr[dev];
+}
+
+// Clears the 64 entries past N of x by position (fp64 and fp32): vector loads
+// of a box's last rows may read them (the handle's own buffers are cleared
+// once at allocation; the shared array keeps what the last call left).
+__global__ void clear_x_tails(double* __restrict__ xp, float* __restrict__ xf, int64_t N) {
+  const int i = threadIdx.x;
+  if (i < 64) {
+    xp[N + i] = 0.0;
+    xf[N + i] = 0.0f;
+  }
+}
+
+}  // namespace
+
+int64_t noalloc_max_bytes() { return kNoAllocBytes; }
+
+void matvec_noalloc(Engine& e, const double* d_b, double* d_y, int mode) {
+  if (!e.assembled) throw Error(H3D_ERR_STATE, "matvec before assemble");
+  int prec;
+  bool acc64;
+  check_mode(e, mode, prec, acc64);
+  if (prec == kPrecMixed ? !e.mix.ready : e.fac_bytes[prec] == 0)
+    throw Error(H3D_ERR_STATE, "the factor copy of this precision is not resident; build it "
+                               "first (h3d_prepare_precision / h3d_compress)");
+  // the sizes ensure_scratch gives the handle's own buffers for one vector
+  int64_t T = e.T, np = e.plan.npartial;
+  if (e.mix.ready) {
+    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T) + 2);
+    for (const FactorClass& c : e.mix.cls)
+      np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
+  }
+  T = std::max<int64_t>(T, 1);
+  np = std::max<int64_t>(np, 1);
+  const int64_t N = e.N;
+  auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+  const int64_t o_xf = up(8 * (N + 64)), o_t = o_xf + up(4 * (N + 64)), o_part = o_t + up(8 * T),
+                o_ylev = o_part + up(8 * np), o_yn = o_ylev + up(8 * N * e.nL),
+                need = o_yn + up(8 * N);
+  if (need > kNoAllocBytes)
+    throw Error(H3D_ERR_INVALID, "this product needs " + std::to_string(need) +
+                                     " bytes of scratch, more than the " +
+                                     std::to_string(kNoAllocBytes) + " of h3d_matvec_noalloc");
+  unsigned char* base = noalloc_scratch();
+  // Swap the carved buffers in for this call (the caps match them, so the
+  // product's own scratch check allocates nothing), with the near field in
+  // line (no second stream) and no event timing; restored on every exit.
+  struct Saved {
+    Engine& e;
+    double *xp, *t, *part, *ylev, *yn;
+    float* xf;
+    int max_nrhs;
+    int64_t t_cap, part_cap;
+    bool overlap, timing;
+    explicit Saved(Engine& en)
+        : e(en), xp(en.d_xp), t(en.d_t), part(en.d_part), ylev(en.d_ylev), yn(en.d_ynear),
+          xf(en.d_xf), max_nrhs(en.max_nrhs), t_cap(en.t_cap), part_cap(en.part_cap),
+          overlap(en.overlap_near), timing(en.timing) {}
+    ~Saved() {
+      e.d_xp = xp, e.d_t = t, e.d_part = part, e.d_ylev = ylev, e.d_ynear = yn, e.d_xf = xf;
+      e.max_nrhs = max_nrhs, e.t_cap = t_cap, e.part_cap = part_cap;
+      e.overlap_near = overlap, e.timing = timing;
+    }
+  } saved(e);
+  e.d_xp = (double*)base;
+  e.d_xf = (float*)(base + o_xf);
+  e.d_t = (double*)(base + o_t);
+  e.d_part = (double*)(base + o_part);
+  e.d_ylev = (double*)(base + o_ylev);
+  e.d_ynear = (double*)(base + o_yn);
+  e.max_nrhs = 1;
+  e.t_cap = T;
+  e.part_cap = np;
+  e.overlap_near = false;
+  e.timing = false;
+  clear_x_tails<<<1, 64, 0, e.stream>>>(e.d_xp, e.d_xf, N);
+  H3D_LAUNCH_CHECK();
+  matvec_device(e, d_b, d_y, 1, mode, false);
 }
 
 }  // namespace h3d

@@ -526,6 +526,13 @@ int h3d_matvec_transpose(h3d_handle* h, const double* b, double* y, int nrhs, in
   return do_matvec(h, b, y, nrhs, prec, where, true);
 }
 
+int h3d_matvec_noalloc(h3d_handle* h, const double* b, double* y, int prec) {
+  return guarded([&] {
+    if (!h || !b || !y) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    h3d::matvec_noalloc(h->e, b, y, prec);
+  });
+}
+
 int h3d_set_diag(h3d_handle* h, const double* diag_vec, int where) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");

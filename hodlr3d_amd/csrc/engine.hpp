@@ -414,6 +414,10 @@ void prepare_precision(Engine& e, int prec, bool move = false);
 void release_precision(Engine& e, int prec);
 void upload_chunk_table(Engine& e);
 void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool transpose);
+// One product with every scratch buffer carved from a static per-GPU array
+// (no allocation); the precision's copy must be resident.
+void matvec_noalloc(Engine& e, const double* d_b, double* d_y, int mode);
+int64_t noalloc_max_bytes();
 // Builds whatever a matvec in `mode` needs (factor copy, scratch) so that the
 // matvec itself allocates nothing and can be captured into a graph.
 void prepare_mode(Engine& e, int mode);

@@ -182,6 +182,22 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* stats);
  * column-major right-hand sides of length N. */
 int h3d_matvec(h3d_handle* h, const double* b, double* y, int nrhs, int prec, int where);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the product of h3d_matvec for one right-hand side, b and
+ * y in device memory, whose scratch (x by position, t, partial sums, per-level
+ * y, near field) is carved from a static device array of 64 MiB that each GPU
+ * holds until the process ends, not from the handle's buffers; it calls no
+ * hipMalloc or hipFree.  A product needing more than 64 MiB of scratch
+ * (N * (20 + 8 nLevels) bytes, plus 8 bytes per low-rank column -- twice that
+ * for the mixed copy -- and the partial sums: up to about 10^5 points)
+ * returns H3D_ERR_INVALID, and a precision whose factor copy is
+ * not resident H3D_ERR_STATE (build it first with h3d_prepare_precision or
+ * h3d_compress).  Results equal h3d_matvec's bit for bit.  Every call on one
+ * GPU shares its array: calls that overlap on one GPU (two streams, two
+ * threads) race, so serialise them. */
+int h3d_matvec_noalloc(h3d_handle* h, const double* b, double* y, int prec);
+
 /* y = A^T b with the same stored factors (no reference counterpart; the
  * adjoint of MatVecProduct). */
 int h3d_matvec_transpose(h3d_handle* h, const double* b, double* y, int nrhs, int prec,
