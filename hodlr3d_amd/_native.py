@@ -89,6 +89,7 @@ SIGNATURES = {
     "h3d_set_tuning": (_i32, [_p, _i32, _i32, _i32]),
     "h3d_get_tuning": (_i32, [_p, _i32, _p]),
     "h3d_matvec_noalloc": (_i32, [_p, _p, _p, _i32]),
+    "h3d_debug_fail_alloc": (_i32, [_i64]),
     "h3d_get_matrix": (_i32, [_p, _p, _i64, _p, _i64, _p, _i32]),
     "h3d_plugin_name": (ctypes.c_char_p, []),
     "h3d_set_tuning_state": (_i32, [_p, _i32, _p]),

@@ -800,7 +800,7 @@ void assemble_batch(Engine& e, int j, int64_t k0, int64_t k1, AcaWorkspace (&ws)
     for (int p = 0; p < 3; ++p) {
       if (!(e.store_mask & (1 << p))) continue;
       const size_t bytes = (size_t)(ch.nU + ch.nV + kChunkTail) * elem_size(p);
-      H3D_HIP(hipMalloc(&ch.data[p], bytes));
+      dmalloc(&ch.data[p], bytes);
       H3D_HIP(hipMemsetAsync(ch.data[p], 0, bytes, st));
       e.fac_bytes[p] += (ch.nU + ch.nV) * elem_size(p);
     }

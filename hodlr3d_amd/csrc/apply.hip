@@ -1025,23 +1025,37 @@ void direct_rows(Engine& e, const double* d_b, const int64_t* h_rows, int64_t nr
   dfree(d_rows);
 }
 
+// The per-precision tables of chunk base pointers, rebuilt as a whole: the new
+// tables are made first and replace the old ones only once all exist, so a
+// failed allocation leaves the previous tables in place.
 void upload_chunk_table(Engine& e) {
   const int64_t nc = (int64_t)e.chunks.size();
+  void** nu[3] = {nullptr, nullptr, nullptr};
+  void** nv[3] = {nullptr, nullptr, nullptr};
+  try {
+    for (int p = 0; p < 3; ++p) {
+      bool have = nc > 0;
+      for (const Chunk& c : e.chunks) have = have && c.data[p] != nullptr;
+      if (!have) continue;
+      std::vector<void*> u(nc), v(nc);
+      for (int64_t i = 0; i < nc; ++i) {
+        u[i] = e.chunks[i].data[p];
+        v[i] = (char*)e.chunks[i].data[p] + e.chunks[i].nU * elem_size(p);
+      }
+      nu[p] = upload(u, e.stream);
+      nv[p] = upload(v, e.stream);
+    }
+    H3D_HIP(hipStreamSynchronize(e.stream));
+  } catch (...) {
+    for (int p = 0; p < 3; ++p) dfree(nu[p]), dfree(nv[p]);
+    throw;
+  }
   for (int p = 0; p < 3; ++p) {
     dfree(e.d_ubase[p]);
     dfree(e.d_vbase[p]);
-    bool have = nc > 0;
-    for (const Chunk& c : e.chunks) have = have && c.data[p] != nullptr;
-    if (!have) continue;
-    std::vector<void*> u(nc), v(nc);
-    for (int64_t i = 0; i < nc; ++i) {
-      u[i] = e.chunks[i].data[p];
-      v[i] = (char*)e.chunks[i].data[p] + e.chunks[i].nU * elem_size(p);
-    }
-    e.d_ubase[p] = upload(u, e.stream);
-    e.d_vbase[p] = upload(v, e.stream);
+    e.d_ubase[p] = nu[p];
+    e.d_vbase[p] = nv[p];
   }
-  H3D_HIP(hipStreamSynchronize(e.stream));
 }
 
 void Plan::free_device() {
@@ -1276,33 +1290,60 @@ void prepare_precision(Engine& e, int prec, bool move) {
   if (src < 0) throw Error(H3D_ERR_STATE, "no factor copy to convert from");
   hipStream_t st = e.stream;
   int64_t bytes = 0;
-  for (Chunk& c : e.chunks) {
-    const int64_t n = c.nU + c.nV + kChunkTail;
-    H3D_HIP(hipMalloc(&c.data[prec], (size_t)n * elem_size(prec)));
-    if (n > 0) {
-      if (prec == H3D_PREC_FP32)
-        convert_kernel<float><<<2048, 256, 0, st>>>(c.data[src], src, n, (float*)c.data[prec]);
-      else if (prec == H3D_PREC_BF16)
-        convert_kernel<uint16_t><<<2048, 256, 0, st>>>(c.data[src], src, n, (uint16_t*)c.data[prec]);
-      else
-        convert_kernel<double><<<2048, 256, 0, st>>>(c.data[src], src, n, (double*)c.data[prec]);
-      H3D_LAUNCH_CHECK();
+  size_t done = 0;  // chunks converted so far
+  try {
+    for (Chunk& c : e.chunks) {
+      const int64_t n = c.nU + c.nV + kChunkTail;
+      dmalloc(&c.data[prec], (size_t)n * elem_size(prec));
+      if (n > 0) {
+        if (prec == H3D_PREC_FP32)
+          convert_kernel<float><<<2048, 256, 0, st>>>(c.data[src], src, n, (float*)c.data[prec]);
+        else if (prec == H3D_PREC_BF16)
+          convert_kernel<uint16_t><<<2048, 256, 0, st>>>(c.data[src], src, n,
+                                                         (uint16_t*)c.data[prec]);
+        else
+          convert_kernel<double><<<2048, 256, 0, st>>>(c.data[src], src, n, (double*)c.data[prec]);
+        H3D_LAUNCH_CHECK();
+      }
+      bytes += (c.nU + c.nV) * elem_size(prec);
+      if (move) {  // free the other copies of this chunk right away (peak = one chunk)
+        H3D_HIP(hipStreamSynchronize(st));
+        for (int p = 0; p < 3; ++p)
+          if (p != prec && c.data[p]) {
+            (void)hipFree(c.data[p]);
+            c.data[p] = nullptr;
+          }
+      }
+      ++done;
     }
-    bytes += (c.nU + c.nV) * elem_size(prec);
-    if (move) {  // free the other copies of this chunk right away (peak = one chunk)
-      H3D_HIP(hipStreamSynchronize(st));
+    H3D_HIP(hipStreamSynchronize(st));
+    upload_chunk_table(e);  // replaces the tables only once all new ones exist
+    if (move)
+      for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
+    e.fac_bytes[prec] = std::max<int64_t>(bytes, 1);
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    if (!move || done == 0) {
+      // nothing of the other copies was freed and the tables are the old ones:
+      // drop this call's new copy
+      for (Chunk& c : e.chunks)
+        if (c.data[prec]) (void)hipFree(c.data[prec]), c.data[prec] = nullptr;
+      throw;
+    }
+    // a move that stopped part way: its first chunks exist only in the new
+    // precision, the others only in the old ones, so no copy is whole.  Drop
+    // every factor (the handle then answers H3D_ERR_STATE until assembled
+    // again) rather than leave a table pointing at freed memory.
+    for (Chunk& c : e.chunks)
       for (int p = 0; p < 3; ++p)
-        if (p != prec && c.data[p]) {
-          (void)hipFree(c.data[p]);
-          c.data[p] = nullptr;
-        }
-    }
-  }
-  H3D_HIP(hipStreamSynchronize(st));
-  if (move)
+        if (c.data[p]) (void)hipFree(c.data[p]), c.data[p] = nullptr;
     for (int p = 0; p < 3; ++p) e.fac_bytes[p] = 0;
-  e.fac_bytes[prec] = std::max<int64_t>(bytes, 1);
-  upload_chunk_table(e);
+    e.assembled = false;
+    e.factors_lost = true;
+    for (int p = 0; p < 3; ++p) dfree(e.d_ubase[p]), dfree(e.d_vbase[p]);
+    throw Error(H3D_ERR_OOM, "a move of the factor copy ran out of device memory part way; the "
+                             "factors are lost (create the handle again)");
+  }
 }
 
 void release_precision(Engine& e, int prec) {

@@ -15,6 +15,8 @@ struct h3d_handle {
 
 namespace h3d {
 
+std::atomic<int64_t> g_alloc_countdown{-1};
+
 Engine::~Engine() { free_all(); }
 
 void Engine::free_all() {
@@ -233,6 +235,8 @@ int h3d_assemble(h3d_handle* h) {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
     h3d::Engine& e = h->e;
     if (e.assembled) throw h3d::Error(H3D_ERR_STATE, "already assembled");
+    if (e.factors_lost)
+      throw h3d::Error(H3D_ERR_STATE, "factors lost in a failed move: create the handle again");
     h3d::assemble_all(e);
     h3d::build_plan(e);
     h3d::ensure_scratch(e, 1);
@@ -658,6 +662,11 @@ int h3d_get_matrix(h3d_handle* h, const int64_t* rows, int64_t nrows, const int6
 }
 
 const char* h3d_plugin_name(void) { return h3d_user_kernel::name; }
+
+int h3d_debug_fail_alloc(int64_t after) {
+  h3d::g_alloc_countdown.store(after < 0 ? -1 : after);
+  return H3D_OK;
+}
 
 int h3d_export_perm(h3d_handle* h, int32_t* perm) {
   return guarded([&] {

@@ -1006,7 +1006,7 @@ void build_mixed(Engine& e, double tol) {
     int64_t total = 0;
     for (int64_t n : fc.nU) total += span(n);
     for (int64_t n : fc.nV) total += span(n);
-    H3D_HIP(hipMalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256)));
+    dmalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256));
     H3D_HIP(hipMemsetAsync(fc.arena, 0, (size_t)std::max<int64_t>(total, 256), st));
     int64_t off = 0;
     auto carve = [&](void*& p, int64_t n) {

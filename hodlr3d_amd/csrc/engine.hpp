@@ -23,6 +23,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -277,6 +278,7 @@ struct Engine {
   std::vector<Level> lev;
 
   bool assembled = false;
+  bool factors_lost = false;  // a move ran out of memory part way (no copy is whole)
   int64_t nblk_all = 0;
   int32_t T = 0;                 // total t length
   std::vector<Chunk> chunks;
@@ -448,11 +450,23 @@ int solve_ir(Engine& e, const double* d_b, double* d_x, int restart, int max_inn
              double inner_tol, int inner_prec, double tol, int max_outer, bool transpose,
              h3d_solve_info* info);
 
+// Every device allocation of the library goes through dmalloc, so that a test
+// can make one of them fail (h3d_debug_fail_alloc): the countdown counts the
+// allocations still allowed; the one found at zero fails and disarms it.
+extern std::atomic<int64_t> g_alloc_countdown;  // < 0: no failure armed
+inline void dmalloc(void** p, size_t bytes) {
+  int64_t c = g_alloc_countdown.load();
+  while (c >= 0 && !g_alloc_countdown.compare_exchange_weak(c, c - 1)) {
+  }
+  if (c == 0) throw Error(H3D_ERR_OOM, "device allocation failed (injected by h3d_debug_fail_alloc)");
+  H3D_HIP(hipMalloc(p, bytes));
+}
+
 template <class T>
 T* dalloc(int64_t n) {
   T* p = nullptr;
   if (n <= 0) return nullptr;
-  H3D_HIP(hipMalloc((void**)&p, sizeof(T) * (size_t)n));
+  dmalloc((void**)&p, sizeof(T) * (size_t)n);
   return p;
 }
 template <class T>

@@ -122,7 +122,9 @@ typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
   int inner_iterations;    /* total inner GMRES iterations */
   int outer_prec;          /* precision mode of the outer residual operator */
-  double rel_residual;     /* final ||b - A x|| / ||b|| with the fp64 operator */
+  double rel_residual;     /* final ||b - A x|| / ||b|| with the outer operator, outer_prec:
+                              the fp64 factors if resident, else fp64 arithmetic over the
+                              fp32 factors (H3D_PREC_FP32 | H3D_ACC_FP64) */
   double history[64];      /* relative residual after each refinement step */
 } h3d_solve_info;
 
@@ -157,10 +159,14 @@ int h3d_load(const char* path, const h3d_options* opt, h3d_handle** out);
 
 /* Builds a copy of the factors in `prec` (from the fp64 copy, else the fp32
  * one), or frees one (H3D_PREC_MIXED too); the last fp64/fp32 copy cannot be
- * freed. */
+ * freed.  On H3D_ERR_OOM nothing of the new copy is kept and the others are
+ * unchanged. */
 int h3d_prepare_precision(h3d_handle* h, int prec);
 /* Like h3d_prepare_precision, but frees every other copy chunk by chunk, so the
- * peak is one chunk above the larger of the two copies. */
+ * peak is one chunk above the larger of the two copies.  If memory runs out
+ * after the first chunk has moved, no copy is whole any more: the factors are
+ * dropped, the call returns H3D_ERR_OOM and the handle answers H3D_ERR_STATE
+ * (create it again). */
 int h3d_move_precision(h3d_handle* h, int prec);
 int h3d_release_precision(h3d_handle* h, int prec);
 
@@ -297,6 +303,11 @@ int h3d_get_matrix(h3d_handle* h, const int64_t* rows, int64_t nrows, const int6
 /* Name of the kernel plugin compiled into the library (Qchoice
  * H3D_QCHOICE_PLUGIN). */
 const char* h3d_plugin_name(void);
+
+/* Fault injection for tests (no reference counterpart): after `after` more
+ * device allocations of the library (process-wide), the next one fails with
+ * H3D_ERR_OOM and the hook disarms itself; a negative value disarms it. */
+int h3d_debug_fail_alloc(int64_t after);
 
 /* Structure export for bit-exact checks (the reference keeps these in
  * HODLR3DBox, include/HODLR3DTree.hpp:11-24). */

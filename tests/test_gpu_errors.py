@@ -70,3 +70,58 @@ def test_torch_and_numpy_products_interleave_in_order():
         o = o.cpu().numpy() if hasattr(o, "cpu") else o
         assert np.array_equal(o, r), i
     h.close()
+
+
+def test_allocation_failures_leave_a_consistent_handle():
+    """An injected device allocation failure (h3d_debug_fail_alloc) part way
+    through a copy: prepare_precision keeps nothing of the new copy and the
+    old one still serves products; a move that already freed some of the old
+    chunks drops the factors and refuses products instead of reading freed
+    memory."""
+    import hodlr3d_amd as H3
+    from hodlr3d_amd import _native as nat
+    x = uniform(40000, 111)
+    q = np.random.default_rng(112).standard_normal(len(x))
+    h = H3.HODLR3D(4, 1.0, 8, 7, x)
+    h.assemble()
+    y = h.matvec(q)
+    nchunks_guess = 3
+    for after in (0, nchunks_guess):
+        nat.lib().h3d_debug_fail_alloc(after)
+        try:
+            with pytest.raises(H3.H3DError) as e:
+                h.prepare_precision("bf16")
+            assert e.value.code == nat.H3D_ERR_OOM
+        finally:
+            nat.lib().h3d_debug_fail_alloc(-1)
+        assert h.stats()["factor_bytes"][2] == 0
+        assert np.array_equal(h.matvec(q), y)
+    h.prepare_precision("bf16")  # a retry without the failure builds it whole
+    assert h.stats()["factor_bytes"][2] > 0
+    assert np.linalg.norm(h.matvec(q, prec="bf16") - y) / np.linalg.norm(y) < 3e-2
+    # a move failing after its first chunk
+    nat.lib().h3d_debug_fail_alloc(1)
+    try:
+        with pytest.raises(H3.H3DError) as e:
+            h.move_precision("fp32")
+        assert e.value.code == nat.H3D_ERR_OOM
+    finally:
+        nat.lib().h3d_debug_fail_alloc(-1)
+    with pytest.raises(H3.H3DError) as e:
+        h.matvec(q)
+    assert e.value.code == nat.H3D_ERR_STATE
+    with pytest.raises(H3.H3DError) as e:
+        h.assemble()
+    assert e.value.code == nat.H3D_ERR_STATE
+    h.close()
+    # a failed move before any chunk moved keeps the old copy whole
+    g = H3.HODLR3D(4, 1.0, 8, 7, x)
+    g.assemble()
+    nat.lib().h3d_debug_fail_alloc(0)
+    try:
+        with pytest.raises(H3.H3DError):
+            g.move_precision("fp32")
+    finally:
+        nat.lib().h3d_debug_fail_alloc(-1)
+    assert np.array_equal(g.matvec(q), y)
+    g.close()
