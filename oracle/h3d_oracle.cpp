@@ -950,8 +950,8 @@ int orc_aca_positions(void* p, int rs, int n, int cs, int m, int32_t* I, int32_t
     I[a] = B.I[a];
     J[a] = B.J[a];
   }
-  if (U) std::memcpy(U, B.U.data(), sizeof(double) * B.U.size());
-  if (V) std::memcpy(V, B.V.data(), sizeof(double) * B.V.size());
+  if (U && !B.U.empty()) std::memcpy(U, B.U.data(), sizeof(double) * B.U.size());
+  if (V && !B.V.empty()) std::memcpy(V, B.V.data(), sizeof(double) * B.V.size());
   return B.rank;
 }
 
