@@ -25,6 +25,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 #include "engine.hpp"
 #include "kernels.hpp"
@@ -77,6 +79,23 @@ struct PItem {
   int32_t row0;
   int32_t pad;
 };
+
+// Runs f(chunk, i0, i1) over [0, n) split into kHostChunks fixed chunks, on up
+// to hardware_concurrency host threads.  The chunks do not depend on the thread
+// count, so per-chunk partial results combined in chunk order are the same on
+// every machine.
+constexpr int kHostChunks = 64;
+template <class F>
+void host_chunks(int64_t n, F f) {
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto run = [&](int t) {
+    for (int c = t; c < kHostChunks; c += nt) f(c, n * c / kHostChunks, n * (c + 1) / kHostChunks);
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(run, t);
+  run(0);
+  for (auto& x : th) x.join();
+}
 
 // ------------------------------------------------------------------ Gram
 // One workgroup per (block, side): G = X^T X for the n x r (or m x r) column
@@ -163,31 +182,45 @@ __device__ inline void wg_sync() { __syncthreads(); }
 // In-place right-looking Cholesky of the r x r matrix A (lower triangle used);
 // sets *flag when a pivot falls below kCholFail times the original diagonal.
 // Called by every thread of the workgroup.
+// Element (i, j) of an r x r work matrix with stride ld: row-major in LDS
+// (odd ld: a column read by consecutive lanes hits distinct banks), column-
+// major in global scratch (a column is contiguous, so the column reads and
+// rotations coalesce).
+template <bool CM>
+__device__ inline int ix(int i, int j, int ld) {
+  return CM ? j * ld + i : i * ld + j;
+}
+
+template <bool CM>
 __device__ inline void chol_inplace(double* A, int r, int ld, double* dg, int* flag, int tid, int nt) {
-  for (int i = tid; i < r; i += nt) dg[i] = A[i * ld + i];
+  for (int i = tid; i < r; i += nt) dg[i] = A[ix<CM>(i, i, ld)];
   wg_sync();
   for (int k = 0; k < r; ++k) {
-    const double dkk = A[k * ld + k];
+    const double dkk = A[ix<CM>(k, k, ld)];
     if (!(dkk > kCholFail * dg[k])) {  // also catches NaN
       if (tid == 0) *flag = 1;
       break;
     }
     const double lkk = sqrt(dkk);
     wg_sync();
-    for (int i = k + 1 + tid; i < r; i += nt) A[i * ld + k] /= lkk;
-    if (tid == 0) A[k * ld + k] = lkk;
+    for (int i = k + 1 + tid; i < r; i += nt) A[ix<CM>(i, k, ld)] /= lkk;
+    if (tid == 0) A[ix<CM>(k, k, ld)] = lkk;
     wg_sync();
     const int m = r - k - 1;
     for (int idx = tid; idx < m * m; idx += nt) {
-      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
-      if (j <= i) A[i * ld + j] -= A[i * ld + k] * A[j * ld + k];
+      // consecutive threads take consecutive i (a column) in the column-major layout
+      const int i = k + 1 + (CM ? idx % m : idx / m), j = k + 1 + (CM ? idx / m : idx % m);
+      if (j <= i) A[ix<CM>(i, j, ld)] -= A[ix<CM>(i, k, ld)] * A[ix<CM>(j, k, ld)];
     }
     wg_sync();
   }
   wg_sync();
 }
 
-template <bool GLOBAL>
+// VEC = false (the first pass, which needs the singular values only): J is not
+// accumulated and no projections are formed; M's rotations, and so the values,
+// are the same bit for bit as with VEC = true.
+template <bool GLOBAL, bool VEC>
 __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
                                                   const int32_t* __restrict__ list, int nlist,
                                                   double* __restrict__ ws,
@@ -199,7 +232,9 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   const CDesc d = ds[list[item]];
   const int r = d.r;
   const int rr = r + (r & 1);
-  const int ld = r | 1;  // odd row stride of L, A, M: a column read hits distinct banks
+  // LDS: odd row stride (distinct banks); global: column-major with the same stride
+  const int ld = r | 1;
+  auto at = [&](int i, int j) { return ix<GLOBAL>(i, j, ld); };
   double* L = GLOBAL ? gscratch + 3 * gstride * item : sm;
   double* A = L + r * ld;
   double* M = A + r * ld;
@@ -216,26 +251,27 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
 
   for (int idx = tid; idx < r * r; idx += nt) {
     const int i = idx / r, j = idx % r;
-    L[i * ld + j] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
-    A[i * ld + j] = W[idx];                   // Gu
+    L[at(i, j)] = W[(int64_t)r * r + idx];  // Gv (symmetric: row/column major alike)
+    A[at(i, j)] = W[idx];                   // Gu
   }
   if (tid == 0) flag = 0;
   wg_sync();
-  chol_inplace(L, r, ld, dg, &flag, tid, nt);
-  if (!flag) chol_inplace(A, r, ld, dg, &flag, tid, nt);
+  chol_inplace<GLOBAL>(L, r, ld, dg, &flag, tid, nt);
+  if (!flag) chol_inplace<GLOBAL>(A, r, ld, dg, &flag, tid, nt);
   if (flag) {
     if (tid == 0) W[2 * (int64_t)r * r + r] = 1.0;  // raw
     return;
   }
   // M = Lu^T Lv: M[i][j] = sum_{k >= max(i,j)} Lu[k][i] Lv[k][j]
   for (int idx = tid; idx < r * r; idx += nt) {
-    const int i = idx / r, j = idx % r;
+    const int i = GLOBAL ? idx % r : idx / r, j = GLOBAL ? idx / r : idx % r;
     double s = 0.0;
-    for (int k = max(i, j); k < r; ++k) s += A[k * ld + i] * L[k * ld + j];
-    M[i * ld + j] = s;
+    for (int k = max(i, j); k < r; ++k) s += A[at(k, i)] * L[at(k, j)];
+    M[at(i, j)] = s;
   }
   wg_sync();
-  for (int idx = tid; idx < r * r; idx += nt) A[(idx / r) * ld + idx % r] = (idx / r == idx % r) ? 1.0 : 0.0;  // J
+  if (VEC)
+    for (int idx = tid; idx < r * r; idx += nt) A[at(idx / r, idx % r)] = (idx / r == idx % r) ? 1.0 : 0.0;  // J
   wg_sync();
   // one-sided Jacobi on M's columns, round-robin pairing (index 0 fixed).  A
   // pair's column dot products are split over tpp lanes of one wave (row i to
@@ -265,7 +301,7 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
         double a = 0.0, b = 0.0, g = 0.0;
         if (act && q < r)
           for (int i = sub; i < r; i += tpp) {
-            const double mp = M[i * ld + p], mq = M[i * ld + q];
+            const double mp = M[at(i, p)], mq = M[at(i, q)];
             a += mp * mp;
             b += mq * mq;
             g += mp * mq;
@@ -297,12 +333,14 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
         const int p = pp[t], q = pq[t];
         if (q < 0 || sn[t] == 0.0) continue;
         const double c = cs[t], s = sn[t];
-        const double a = M[i * ld + p], b = M[i * ld + q];
-        M[i * ld + p] = c * a - s * b;
-        M[i * ld + q] = s * a + c * b;
-        const double ja = A[i * ld + p], jb = A[i * ld + q];
-        A[i * ld + p] = c * ja - s * jb;
-        A[i * ld + q] = s * ja + c * jb;
+        const double a = M[at(i, p)], b = M[at(i, q)];
+        M[at(i, p)] = c * a - s * b;
+        M[at(i, q)] = s * a + c * b;
+        if (VEC) {
+          const double ja = A[at(i, p)], jb = A[at(i, q)];
+          A[at(i, p)] = c * ja - s * jb;
+          A[at(i, q)] = s * ja + c * jb;
+        }
       }
       wg_sync();
     }
@@ -313,7 +351,7 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   // squared singular values = squared column norms of M J, sorted descending
   for (int k = tid; k < r; k += nt) {
     double s = 0.0;
-    for (int i = 0; i < r; ++i) s += M[i * ld + k] * M[i * ld + k];
+    for (int i = 0; i < r; ++i) s += M[at(i, k)] * M[at(i, k)];
     lam[k] = s;
   }
   wg_sync();
@@ -333,19 +371,20 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   double* out_lam = W + 2 * (int64_t)r * r;
   for (int k = tid; k < r; k += nt) {
     const int c = perm[k];
+    out_lam[k] = lam[c];
+    if (!VEC) continue;
     double* tu = W + (int64_t)k * r;
     double* tv = W + (int64_t)r * r + (int64_t)k * r;
     for (int i = 0; i < r; ++i) {
       double s = 0.0;
-      for (int j = 0; j <= i; ++j) s += L[i * ld + j] * A[j * ld + c];
+      for (int j = 0; j <= i; ++j) s += L[at(i, j)] * A[at(j, c)];
       tu[i] = s;
     }
     for (int i = r - 1; i >= 0; --i) {
-      double s = A[i * ld + c];
-      for (int j = i + 1; j < r; ++j) s -= L[j * ld + i] * tv[j];
-      tv[i] = s / L[i * ld + i];
+      double s = A[at(i, c)];
+      for (int j = i + 1; j < r; ++j) s -= L[at(j, i)] * tv[j];
+      tv[i] = s / L[at(i, i)];
     }
-    out_lam[k] = lam[c];
   }
   if (tid == 0) {
     W[2 * (int64_t)r * r + r] = 0.0;
@@ -524,7 +563,10 @@ void build_mixed(Engine& e, double tol) {
   int dev = 0, lds_max = 65536;
   H3D_HIP(hipGetDevice(&dev));
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  lds_max = std::min(lds_max, 65536);
+  // gfx950 lets one workgroup take all 160 KB of a CU's LDS (with the
+  // attribute set below): ranks up to ~80 keep their matrices there
+  // (less a margin for the kernel's own static LDS)
+  lds_max = std::min(lds_max, 163840) - 1024;
   // LDS of the eigen step: matrices (unless global) + rotation data
   auto eig_lds = [](int r, bool global) {
     const int rr = r + (r & 1);
@@ -592,7 +634,11 @@ void build_mixed(Engine& e, double tol) {
   int64_t ws_cap = 0;
   CDesc* d_cd = nullptr;
   int32_t* d_list = nullptr;
-  auto run_eig = [&](int64_t i0, int64_t i1) {
+  for (const void* f : {(const void*)eig_kernel<false, true>, (const void*)eig_kernel<false, false>})
+    H3D_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  // vectors: J and the projections as well (the second pass); the first pass
+  // needs the singular values only
+  auto run_eig = [&](int64_t i0, int64_t i1, bool vectors) {
     int64_t need = 0;
     for (int64_t i = i0; i < i1; ++i) {
       cd[i].w = need;
@@ -647,30 +693,45 @@ void build_mixed(Engine& e, double tol) {
     launch_g(g4, 0);
     launch_g(g16, 1);
     launch_g(g40, 2);
-    // eig: small ranks one wave per block, larger ranks four waves, ranks
-    // beyond rmax with their matrices in global scratch
-    std::vector<int32_t> grp[3];
+    // eig: small ranks one wave per block; larger ranks four waves, launched in
+    // classes of LDS footprint (each launch sizes its LDS by its largest rank,
+    // and that sets how many workgroups share a CU); ranks beyond rmax with
+    // their matrices in global scratch
+    constexpr int kEigClasses = 6;  // 0: one wave; 1-4: LDS up to the caps; 5: global
+    const size_t caps[4] = {40 << 10, 64 << 10, 96 << 10, (size_t)lds_max};
+    std::vector<int32_t> grp[kEigClasses];
     for (int32_t li : eg) {
       const int r = cd[i0 + li].r;
-      grp[r <= 24 ? 0 : (r <= rmax ? 1 : 2)].push_back(li);
+      int q = 5;
+      if (r <= 24) q = 0;
+      else if (r <= rmax)
+        for (int c = 0; c < 4; ++c)
+          if (eig_lds(r, false) <= caps[c]) {
+            q = 1 + c;
+            break;
+          }
+      grp[q].push_back(li);
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < kEigClasses; ++q) {
       std::vector<int32_t>& g = grp[q];
       if (g.empty()) continue;
+      const bool glob = q == kEigClasses - 1;
       int rm = 1;
       for (int32_t li : g) rm = std::max(rm, cd[i0 + li].r);
       int32_t* d_g = upload(g, st);
       const int64_t gstride = (int64_t)rm * (rm | 1);
-      const int64_t per = q == 2 ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (3 * gstride)))
-                                 : 1000000;
-      double* gs = q == 2 ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
+      const int64_t per = glob ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (3 * gstride)))
+                               : 1000000;
+      double* gs = glob ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
       for (int64_t s = 0; s < (int64_t)g.size(); s += per) {
         const int cnt = (int)std::min<int64_t>(per, (int64_t)g.size() - s);
-        if (q < 2)
-          eig_kernel<false><<<cnt, q ? 256 : 64, eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws,
+        if (!glob)
+          (vectors ? eig_kernel<false, true> : eig_kernel<false, false>)<<<cnt, q ? 256 : 64,
+                                                                         eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws,
                                                                            nullptr, 0);
         else
-          eig_kernel<true><<<cnt, 256, eig_lds(rm, true), st>>>(d_cd, d_g + s, cnt, d_ws, gs,
+          (vectors ? eig_kernel<true, true> : eig_kernel<true, false>)<<<cnt, 256, eig_lds(rm, true),
+                                                                       st>>>(d_cd, d_g + s, cnt, d_ws, gs,
                                                                 gstride);
         H3D_LAUNCH_CHECK();
       }
@@ -686,7 +747,7 @@ void build_mixed(Engine& e, double tol) {
     while (i0 < nb) {
       int64_t i1 = i0, need = 0;
       while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
-      run_eig(i0, i1);
+      run_eig(i0, i1, false);
       const int cnt = (int)(i1 - i0);
       std::vector<int64_t> off(cnt);
       for (int64_t i = i0; i < i1; ++i) off[i - i0] = lam_off[i] - lam_off[i0];
@@ -717,9 +778,15 @@ void build_mixed(Engine& e, double tol) {
   // ||A||_F^2: far blocks (exact for the recompressed ones) + near field
   {
     mx.fro2_far = 0;
-    for (int64_t i = 0; i < nb; ++i)
-      if (!raw[i])
-        for (int k = 0; k < cd[i].r; ++k) mx.fro2_far += lam_all[lam_off[i] + k];
+    std::vector<double> part(kHostChunks, 0.0);
+    host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
+      double acc = 0.0;
+      for (int64_t i = i0; i < i1; ++i)
+        if (!raw[i])
+          for (int k = 0; k < cd[i].r; ++k) acc += lam_all[lam_off[i] + k];
+      part[c] = acc;
+    });
+    for (double v : part) mx.fro2_far += v;
     const int nn = (int)e.nitems.size();
     mx.fro2_near = 0;
     if (nn) {
@@ -767,14 +834,28 @@ void build_mixed(Engine& e, double tol) {
     // Dp: columns that stay fp32
     std::vector<double> Dd(kBins + 1, 0.0), Dh(kBins + 1, 0.0);
     double Dp = 0.0;
-    for (int64_t i = 0; i < nb; ++i) {
-      if (raw[i]) continue;
-      const int nm = cd[i].n + cd[i].m;
-      for (int k = 0; k < cd[i].r; ++k) {
-        const double l = lam_all[lam_off[i] + k];
-        if (l > cap_fp16) Dp += l;
-        else if (l > cap_drop) Dh[bin_of(l, nm)] += l;
-        else Dd[bin_of(l, nm)] += l;
+    {  // per-chunk histograms, added in chunk order
+      std::vector<std::vector<double>> cd_(kHostChunks), ch_(kHostChunks);
+      std::vector<double> cp(kHostChunks, 0.0);
+      host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
+        std::vector<double>& hd = cd_[c];
+        std::vector<double>& hh = ch_[c];
+        hd.assign(kBins + 1, 0.0);
+        hh.assign(kBins + 1, 0.0);
+        for (int64_t i = i0; i < i1; ++i) {
+          if (raw[i]) continue;
+          const int nm = cd[i].n + cd[i].m;
+          for (int k = 0; k < cd[i].r; ++k) {
+            const double l = lam_all[lam_off[i] + k];
+            if (l > cap_fp16) cp[c] += l;
+            else if (l > cap_drop) hh[bin_of(l, nm)] += l;
+            else hd[bin_of(l, nm)] += l;
+          }
+        }
+      });
+      for (int c = 0; c < kHostChunks; ++c) {
+        for (int b = 0; b <= kBins; ++b) Dd[b] += cd_[c][b], Dh[b] += ch_[c][b];
+        Dp += cp[c];
       }
     }
     std::vector<double> Pd(kBins + 1, 0.0), Ph(kBins + 1, 0.0);  // prefix sums
@@ -816,27 +897,35 @@ void build_mixed(Engine& e, double tol) {
   mx.dropped = 0;
   mx.raw_blocks = 0;
   mx.raw_rank = 0;
+  host_chunks(nb, [&](int, int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      const int r = cd[i].r;
+      if (raw[i]) {
+        r32[i] = r;
+        r16[i] = 0;
+        continue;
+      }
+      const int nm = cd[i].n + cd[i].m;
+      int keep = 0, k32 = 0;
+      for (int k = 0; k < r; ++k) {
+        const double l = lam_all[lam_off[i] + k];
+        const int b = bin_of(l, nm);
+        keep += (b >= bd || l > cap_drop);
+        k32 += (b >= bd + delta || l > cap_fp16);
+      }
+      r32[i] = k32;
+      r16[i] = keep - k32;
+    }
+  });
   for (int64_t i = 0; i < nb; ++i) {
     const int r = cd[i].r;
     mx.aca_rank_sum += r;
     if (raw[i]) {
-      r32[i] = r;
-      r16[i] = 0;
       mx.raw_blocks++;
       mx.raw_rank += r;
       continue;
     }
-    const int nm = cd[i].n + cd[i].m;
-    int keep = 0, k32 = 0;
-    for (int k = 0; k < r; ++k) {
-      const double l = lam_all[lam_off[i] + k];
-      const int b = bin_of(l, nm);
-      keep += (b >= bd || l > cap_drop);
-      k32 += (b >= bd + delta || l > cap_fp16);
-    }
-    r32[i] = k32;
-    r16[i] = keep - k32;
-    mx.dropped += r - keep;
+    mx.dropped += r - (r32[i] + r16[i]);
   }
 
   // ---------------- class layouts
@@ -1025,7 +1114,7 @@ void build_mixed(Engine& e, double tol) {
     while (i0 < nb) {
       int64_t i1 = i0, need = 0;
       while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
-      run_eig(i0, i1);
+      run_eig(i0, i1, true);
       std::vector<PDesc> sub(pd.begin() + i0, pd.begin() + i1);
       // blocks of rank > kMaxEigRank have no eigen step: raw in both passes
       PDesc* d_pd = upload(sub, st);
