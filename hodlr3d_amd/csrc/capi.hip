@@ -73,14 +73,15 @@ void Engine::free_all() {
   dfree(d_io);
   dfree(d_xw);
   dfree(d_tw);
-  dfree(d_partw);
   dfree(d_ylevw);
   dfree(d_ynw);
   dfree(d_iow);
   dfree(d_rcols);
   dfree(d_gitems);
-  dfree(d_gblk);
+  dfree(d_gcolv);
+  dfree(d_gcolo);
   n_rcols = n_gitems = g_np = 0;
+  n_p1[0] = n_p1[1] = n_p1[2] = 0;
   w_cap = io_w = 0;
   for (auto& ev : this->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -59,20 +59,17 @@ struct P1Item {
   int32_t pad;
 };
 
-// Phase-1 item of the multi-RHS path: rows [s0, s1) of one source box (at
-// position src) against the blocks by_src[q0 .. q1), which all have it as source.
-struct GItem {
-  int32_t q0, q1;
-  int32_t src;
-  int32_t s0, s1;
-  int32_t pad;
-};
-
-// Block descriptor of the multi-RHS phase 1 (one scalar load): V offset and
-// chunk, rank, rows, t index of column 0, first partial slot (-1: none).
-struct GDesc {
-  int64_t voff;
-  int32_t chunk, r, m, tb, pb, pad;
+// Phase-1 item of the multi-RHS path: rows [s0, s0 + nrows) (at most 256) of
+// one source box against the columns [col0, col0 + ncols) of the column table
+// (at most 256, all with that source box).  The table lists the V columns of
+// every block of rank >= 1, grouped by source box (by-source block order).
+struct GTItem {
+  int32_t col0, ncols;
+  int32_t a0;     // position of the range's first row
+  int32_t nrows;
+  int32_t s0;     // first row of the range within the source box
+  int32_t range;  // s0 / 256: selects the partial-sum slot
+  int32_t pad0, pad1;
 };
 
 // Partial-sum reduction: nsplit row ranges x nc columns at part[pbase + q*nc + l]
@@ -318,21 +315,22 @@ struct Engine {
   // products with several right-hand sides: groups of up to rhs_block (16 or
   // 32; 0 = one at a time) applied together (multi.hip); scratch for W per entry
   int rhs_block = 32;
-  int g_cols = 1024;                // columns of one multi-RHS phase-1 item
+  int g_cols = 2048;                // columns of one short-source phase-1 item (1024 or 2048)
   int w_cap = 0;                    // W of the allocated group scratch
   int64_t w_t = 0, w_np = 0;        // t and partial entries it holds per RHS
-  double* d_xw = nullptr;           // x by position, quad and row layouts
-  double* d_tw = nullptr;           // t, W per column
-  double* d_partw = nullptr;        // phase-1 partial sums, W per entry
+  double* d_xw = nullptr;           // x by position, W per point
+  double* d_tw = nullptr;           // t (T + 1 slots) then the phase-1 partial sums, W per slot
   double* d_ylevw = nullptr;        // per-level y by position, W per point
   double* d_ynw = nullptr;          // near field by position, W per point
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
   void* d_rcols = nullptr;          // per column of a row-split block: int4 reduce entry
   int64_t n_rcols = 0;
-  void* d_gitems = nullptr;         // phase-1 items grouped by source box (GItem)
+  void* d_gitems = nullptr;         // phase-1 items (GTItem), by class of 64, 128, 256 rows
   int64_t n_gitems = 0;
-  void* d_gblk = nullptr;           // GDesc of the blocks of rank >= 1, by source box
+  int64_t n_p1[3] = {0, 0, 0};      // items of each class
+  int64_t* d_gcolv = nullptr;       // column table: element offset in its V chunk | chunk << 48
+  int2* d_gcolo = nullptr;          // column table: {output slot of range 0, slot step per range}
   int64_t g_np = 0;                 // partial entries per RHS
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b

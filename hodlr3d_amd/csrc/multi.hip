@@ -3,11 +3,12 @@
 // of them on the matrix cores (the caller side of MatVecProduct,
 // src/HODLR3D.cpp:21-27, called once per column in the reference).
 //
-//   gather    x by position in two layouts: quads (x[p..p+3] of one RHS
-//             contiguous, the MFMA B operand of phase 1) and rows (the W
-//             values of one point contiguous, read by the near field)
-//   phase 1   T_b = V_b^T X_s   per block, 16x16x4 MFMA: A = 16 columns x 4
-//             rows of V (one 4-row vector per lane), B = 4 rows x 16 RHS
+//   gather    x by position, the W values of one point contiguous (read by
+//             phase 1, staged per row range, and by the near field)
+//   phase 1   T_b = V_b^T X_s   per source box s: the V columns of every block
+//             with that source concatenated, 32-column tiles x 16-row steps,
+//             16x16x4 MFMA: A = 16 columns x 4 rows of V (one 4-row vector
+//             per lane), B = 4 rows x 16 RHS of x (from LDS)
 //   phase 2   Y_k = U_k T_k     per <= 64 rows of a target box, 16x16x4 MFMA:
 //             A = 16 row quads x 4 columns of U, B = 4 columns x 16 RHS of T
 //   near      y_N = K_near X    lanes own target rows, W sums each, source
@@ -15,10 +16,12 @@
 //             own buffer, on the handle's second stream under phases 1 and 2
 //   combine   y[:, j] = alpha * (sum_levels + y_N) + d * b[:, j] (an LDS transpose)
 //
-// The work lists are the single-RHS plan's (phase-1 items, partial sums and
-// phase-2 items), so the order of the block loop is the same; fp64 factors
-// and the fp64-accumulation modes use the fp64 MFMA, fp32/bf16 factors with
-// fp32 accumulation the fp32 MFMA.
+// The streaming loops of phases 1 and 2 issue the same loads on every pass
+// and store nothing: on gfx950 loads and stores share one in-order counter,
+// so a store or a skipped load inside the loop made the compiler wait for the
+// prefetched loads (and the stores) before the next MFMAs.  fp64 factors and
+// the fp64-accumulation modes use the fp64 MFMA, fp32/bf16 factors with fp32
+// accumulation the fp32 MFMA.
 #include <algorithm>
 #include <type_traits>
 
@@ -30,6 +33,12 @@ namespace h3d {
 namespace {
 
 constexpr int kGRows = 256;     // rows of one phase-1 item (x staged in LDS)
+constexpr int kTileCols = 32;   // columns of one phase-1 tile (two MFMA tiles)
+constexpr int64_t kColMask = (int64_t(1) << 48) - 1;
+// chunk bases a phase-1 workgroup keeps in LDS (a handle with more factor
+// chunks takes its products one column at a time)
+constexpr int kLdsChunks = 1024;
+static_assert(kLdsChunks % 256 == 0, "staged by 256 threads");
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -57,23 +66,32 @@ struct Mfma<double> {
   __device__ static constexpr int row(int q, int a) { return 4 * a + q; }
 };
 
-// Four consecutive stored elements (4-byte aligned) converted to A.
-template <class S, class A>
+// Four consecutive stored elements (4-byte aligned) converted to A, with the
+// nontemporal hint (NT: the load's lines are not kept in the cache) or
+// without (a line that the next instruction reads again stays there).
+// (global address space named: a pointer read from LDS would otherwise give
+// flat loads, which count on both wait counters)
+template <bool NT, class T>
+__device__ inline T vload(const void* p) {
+  if constexpr (NT) return ntload<T>(p);
+  else return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class S, class A, bool NT = true>
 __device__ inline void load4(const S* p, A (&o)[4]) {
   if constexpr (sizeof(S) == 8) {
-    const u32x4a v0 = ntload<u32x4a>(p), v1 = ntload<u32x4a>(p + 2);
+    const u32x4a v0 = vload<NT, u32x4a>(p), v1 = vload<NT, u32x4a>(p + 2);
     o[0] = (A)__hiloint2double((int)v0.y, (int)v0.x);
     o[1] = (A)__hiloint2double((int)v0.w, (int)v0.z);
     o[2] = (A)__hiloint2double((int)v1.y, (int)v1.x);
     o[3] = (A)__hiloint2double((int)v1.w, (int)v1.z);
   } else if constexpr (sizeof(S) == 4) {
-    const u32x4a v = ntload<u32x4a>(p);
+    const u32x4a v = vload<NT, u32x4a>(p);
     o[0] = (A)__uint_as_float(v.x);
     o[1] = (A)__uint_as_float(v.y);
     o[2] = (A)__uint_as_float(v.z);
     o[3] = (A)__uint_as_float(v.w);
   } else {  // bf16: element 0 in the low half of each word
-    const u32x2a v = ntload<u32x2a>(p);
+    const u32x2a v = vload<NT, u32x2a>(p);
     o[0] = (A)__uint_as_float(v.x << 16);
     o[1] = (A)__uint_as_float(v.x & 0xffff0000u);
     o[2] = (A)__uint_as_float(v.y << 16);
@@ -81,7 +99,7 @@ __device__ inline void load4(const S* p, A (&o)[4]) {
   }
 }
 
-// Four consecutive x values of one RHS in the quad layout (16 or 32 bytes).
+// Four consecutive x values of one RHS in the LDS quad layout (16 or 32 bytes).
 template <class A>
 __device__ inline void load_quad(const A* p, A (&o)[4]) {
   if constexpr (sizeof(A) == 8) {
@@ -99,134 +117,155 @@ __device__ inline void load_quad(const A* p, A (&o)[4]) {
   }
 }
 
-// Quad layout: x(p, j) at xq[((p / 4) W + j) 4 + p % 4], Npad positions (the
-// rows past N are zero); row layout: x(p, j) at xr[p W + j].  Columns j >= cnt
-// are zero.
+// x(p, j) at xr[p W + j]; columns j >= cnt are zero.
 template <class A, int W>
 __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __restrict__ perm,
-                                int64_t N, int64_t Npad, int cnt, A* __restrict__ xq,
-                                A* __restrict__ xr) {
+                                int64_t N, int cnt, A* __restrict__ xr) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= Npad * W) return;
-  const int j = (int)((id >> 2) % W);
-  const int64_t p = 4 * (id / (4 * W)) + (id & 3);
-  const A v = (p < N && j < cnt) ? (A)b[(int64_t)j * N + perm[p]] : (A)0;
-  xq[id] = v;
-  xr[p * W + j] = v;
+  if (id >= N * W) return;
+  const int j = (int)(id % W);
+  xr[id] = j < cnt ? (A)b[(int64_t)j * N + perm[id / W]] : (A)0;
 }
 
-// Phase 1, grouped by source box: one workgroup per item = a row range
-// [s0, s1) (at most kGRows rows) of one source box and a slice of the blocks
-// with that source (at most Engine::g_cols columns, tuned).  x of the range (as quads,
-// rows outside the range zeroed) is staged once in LDS and shared by the four
-// waves, which take the slice's blocks in turn: T_b(c, j) = sum over the range
-// of V_b(c, p) x(p, j), 32 columns (two 16-column tiles) per pass, UF 16-row
-// steps per unit of work (gdesc describes the blocks of rank >= 1 by source box).
-// In step s lane (c = l % 16, q = l / 16) takes rows
-// P + 16 s + 4q .. + 3 of column c.  The factor loads of the next unit (next
-// steps, next pass or next block) are issued before the MFMAs of the current
-// one (two register buffers).  A block whose source has more than kGRows rows
-// writes partial sums (slot pbw[b] + range * r + c), summed in range order
-// afterwards.
-template <class S, class A, int W>
-__global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ items, int nitems,
-                                                  const GDesc* __restrict__ gdesc,
-                                                  void* const* __restrict__ vbase,
-                                                  const A* __restrict__ xq, A* __restrict__ tw,
-                                                  A* __restrict__ partw) {
+// Phase 1: one workgroup per item = a row range of at most NC x 64 rows of one
+// source box (the whole box if it has at most 256 rows, else one 256-row range
+// of it) x at most LC columns of the box's column list (a multiple of 32: the
+// table pads every source).  x of the range (zero past it) and the item's
+// column table go to LDS once; then every wave streams 32-column tiles
+// k = wave, wave + 4, .., each a complete unit of NC chunks of 4 steps of 16
+// rows: lane (c = l % 16, q = l / 16) holds rows 16 s + 4q .. + 3 of column c
+// of each 16-column half, x of the same rows comes from LDS (16x16x4 MFMA:
+// A = 16 columns x 4 rows of V, B = 4 rows x 16 RHS).  A tile's sums are
+// stored (W per column) at its slot out0 + range * step: a t index, a partial
+// slot (sources of more than 256 rows, after the T + 1 t slots), or the spare
+// t slot T (padding).
+//
+// Every pass issues the same loads and stores and no branch skips one (rows
+// past the range are moved to row 0 and meet zero x; tiles past the wave's
+// last are clamped to it and their sums never stored), the next chunk's loads
+// are issued before the current chunk's MFMAs (two register buffers; the
+// scheduling barrier keeps each batch whole), and the first tile (NC > 1) or
+// first pair of tiles (NC = 1) is peeled off so that the loop is entered with
+// the same operations in flight as it loops.  The compiler's waits then count
+// exactly the loads each chunk needs: on gfx950 loads and stores share one
+// in-order counter, and where two paths with different operations in flight
+// meet, the compiler must assume the fewest, which made a wave wait for its
+// prefetched loads and its stores.
+template <class S, class A, int W, int NC, int LC>
+__global__ __launch_bounds__(256) void p1r_kernel(const GTItem* __restrict__ items, int nitems,
+                                                  const int64_t* __restrict__ colv,
+                                                  const int2* __restrict__ colo,
+                                                  void* const* __restrict__ vbase, int nchunk,
+                                                  const A* __restrict__ xr, A* __restrict__ tw) {
   using M = Mfma<A>;
   using Acc = typename M::acc;
   constexpr int H = W / 16;
   constexpr int CT = 2;
-  constexpr int UF = sizeof(A) == 8 ? 2 : 4;
-  constexpr int QMAX = 4 * ((kGRows + 15) / 16);  // quads of the steps of one range
-  __shared__ A xl[QMAX * W * 4];
+  constexpr int ROWS = 64 * NC;
+  constexpr int PER = LC / 256;  // table entries staged per thread
+  constexpr int VE = 16 / sizeof(A);
+  constexpr int XV = ROWS * W / VE;  // 16-byte vectors of x
+  constexpr int XPER = XV / 256;
+  static_assert(LC % 256 == 0 && XV % 256 == 0, "whole staging passes");
+  static_assert(NC == 1 || NC % 2 == 0, "chunks alternate between two load buffers");
+  typedef A vec __attribute__((ext_vector_type(VE)));
+  extern __shared__ __align__(16) unsigned char p1r_lds[];  // ROWS W values of x
+  A* xl = (A*)p1r_lds;
+  __shared__ int64_t lcv[LC];
+  __shared__ int32_t lslot[LC];
+  __shared__ const void* cbase[kLdsChunks];
   const int item = blockIdx.x;
   if (item >= nitems) return;  // whole workgroup
-  const GItem it = items[item];
-  const int a0 = it.src + it.s0;
-  const int nrows = it.s1 - it.s0;
-  const int nsteps = (nrows + 15) >> 4;
+  const GTItem it = items[item];
+  const int tid = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int lc = lane & 15, kq = lane >> 4;
+
+  // 1. staging: every load issued before any is used, at clamped indices; the
+  // chunk bases are read from LDS only (a select between an LDS and a global
+  // address gives flat loads, which wait for everything in flight)
   {
-    // x of range row i (position a0 + i) at xl[((i / 4) W + j) 4 + i % 4]; zero past the range
-    const int ne = 16 * nsteps * W;
-    for (int e = threadIdx.x; e < ne; e += 256) {
-      const int i = (e / (4 * W)) * 4 + (e & 3), j = (e >> 2) % W;
-      const int p = a0 + i;
-      xl[e] = i < nrows ? xq[((int64_t)(p >> 2) * W + j) * 4 + (p & 3)] : (A)0;
+    int64_t tcv[PER];
+    int32_t tsl[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int c = tid + 256 * q;
+      const int cc = it.col0 + (c < it.ncols ? c : it.ncols - 1);
+      tcv[q] = colv[cc];
+      const int2 o = colo[cc];
+      tsl[q] = o.x + it.range * o.y;
+    }
+    const void* cb[kLdsChunks / 256];
+#pragma unroll
+    for (int q = 0; q < kLdsChunks / 256; ++q) {
+      const int c = tid + 256 * q;
+      cb[q] = vbase[c < nchunk ? c : nchunk - 1];
+    }
+    const int nvalid = it.nrows * W / VE;
+    const A* src = xr + (int64_t)it.a0 * W;
+    vec tmp[XPER];
+#pragma unroll
+    for (int q = 0; q < XPER; ++q) {
+      const int v = tid + 256 * q;
+      tmp[q] = *(const vec*)(src + (int64_t)(v < nvalid ? v : 0) * VE);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      lcv[tid + 256 * q] = tcv[q];
+      lslot[tid + 256 * q] = tsl[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kLdsChunks / 256; ++q)
+      if (tid + 256 * q < nchunk) cbase[tid + 256 * q] = cb[q];
+#pragma unroll
+    for (int q = 0; q < XPER; ++q) {
+      const int v = tid + 256 * q;
+#pragma unroll
+      for (int u = 0; u < VE; ++u) {
+        const int e = v * VE + u, i = e / W, j = e % W;
+        xl[((i >> 2) * W + j) * 4 + (i & 3)] = v < nvalid ? tmp[q][u] : (A)0;
+      }
     }
   }
   __syncthreads();
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int lc = lane & 15, kq = lane >> 4;
-  const int range = it.s0 / kGRows;
 
-  // unit of work: block gdesc[q], column pass cb, steps [s0, s0 + UF)
-  struct Unit {
-    int q, r, cb, s0;
-    const S* V;
-    int64_t ldv;
-    A* out;
-  };
-  // the wave's next block descriptor is loaded one block ahead
-  GDesc pre = it.q0 + wv < it.q1 ? gdesc[it.q0 + wv] : GDesc{};
-  auto open_block = [&](Unit& u) -> bool {  // first unit of block q (rank >= 1)
-    if (u.q >= it.q1) return false;
-    const GDesc d = pre;
-    if (u.q + kWavesPerBlock < it.q1) pre = gdesc[u.q + kWavesPerBlock];
-    u.r = d.r;
-    u.cb = 0;
-    u.s0 = 0;
-    u.V = (const S*)vbase[d.chunk] + d.voff;
-    u.ldv = d.m + (d.m & 1);
-    u.out = d.pb < 0 ? tw + (int64_t)d.tb * W
-                     : partw + ((int64_t)d.pb + (int64_t)range * d.r) * W;
-    return true;
-  };
-  auto advance = [&](Unit& u) -> bool {
-    u.s0 += UF;
-    if (u.s0 < nsteps) return true;
-    u.s0 = 0;
-    u.cb += 16 * CT;
-    if (u.cb < u.r) return true;
-    u.q += kWavesPerBlock;
-    return open_block(u);
-  };
-  // Branch-free loads: one 4-row vector per lane and tile at an in-bounds
-  // address (columns clamped to r - 1; a vector starting inside the range ends
-  // at most 3 rows past it, in the column padding, the next column or the
-  // chunk's zero tail); every factor value read is finite.
-  // The loaded vectors are consumed only by the MFMAs (compute), so the wait
-  // for a unit's loads falls after the previous unit's MFMAs.
-  auto load = [&](const Unit& u, A (&av)[UF][CT][4]) {
+  // 2. the wave's tiles (a wave without one has nothing in flight here)
+  const int ntiles = it.ncols / kTileCols;
+  const int ntw = ntiles > wv ? (ntiles - wv + 3) / 4 : 0;
+  if (ntw == 0) return;
+  auto load = [&](int i, int c, A (&av)[4][CT][4]) {  // chunk c of the wave's tile i (clamped)
+    const int k = wv + 4 * (i < ntw ? i : ntw - 1);
+    const S* vp[CT];
 #pragma unroll
-    for (int st = 0; st < UF; ++st) {
-      const int i = 16 * (u.s0 + st) + 4 * kq;
-      const int il = i < nrows ? i : 0;
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int c = u.cb + 16 * ct + lc;
-        const int cl = c < u.r ? c : u.r - 1;
-        load4<S, A>(u.V + cl * u.ldv + it.s0 + il, av[st][ct]);
-      }
+    for (int ct = 0; ct < CT; ++ct) {
+      const int64_t cv = lcv[k * kTileCols + 16 * ct + lc];
+      vp[ct] = (const S*)cbase[(int)(cv >> 48)] + (cv & kColMask) + it.s0;
     }
-  };
-  // Both column tiles are multiplied even when the second lies past the rank
-  // (its loads are clamped, its output rows never stored): a wave-uniform
-  // branch between the last MFMA and the accumulator reads of the store let
-  // the compiler place those reads 2 wait states after the MFMA issue (it
-  // needs 10 for the fp32 16x16x4 product), so the store took stale sums.
-  auto compute = [&](const Unit& u, const A (&av)[UF][CT][4], Acc (&acc)[CT][H]) {
 #pragma unroll
-    for (int st = 0; st < UF; ++st) {
-      if (u.s0 + st >= nsteps) break;
+    for (int st = 0; st < 4; ++st) {
+      const int r = 64 * c + 16 * st + 4 * kq;
+      const int rl = r < it.nrows ? r : 0;
+      // cached loads: a column's 128-byte line is read by two instructions
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) load4<S, A, false>(vp[ct] + rl, av[st][ct]);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the batch stays ahead of the MFMAs
+  };
+  Acc acc[CT][H];
+  auto clear = [&]() {
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
+  };
+  auto compute = [&](int c, const A (&av)[4][CT][4]) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
       A xb[H][4];
 #pragma unroll
       for (int h = 0; h < H; ++h)
-        load_quad<A>(xl + ((4 * (u.s0 + st) + kq) * W + 16 * h + lc) * 4, xb[h]);
-      // no masks: rows past the range meet zero x (staged so), and the
-      // output rows of clamped columns (c >= r) are never stored
+        load_quad<A>(xl + ((4 * (4 * c + st) + kq) * W + 16 * h + lc) * 4, xb[h]);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -235,50 +274,62 @@ __global__ __launch_bounds__(256) void p1g_kernel(const GItem* __restrict__ item
           for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
     }
   };
-  auto store = [&](const Unit& u, Acc (&acc)[CT][H]) {
-    const bool two = u.cb + 16 < u.r;
+  auto store = [&](int i) {
+    const int k = wv + 4 * i;
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      if (ct == 0 || two) {
+    for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int c = u.cb + 16 * ct + M::row(kq, a);
-          if (c < u.r) {
+      for (int a = 0; a < 4; ++a) {
+        A* dst = tw + (int64_t)lslot[k * kTileCols + 16 * ct + M::row(kq, a)] * W;
 #pragma unroll
-            for (int h = 0; h < H; ++h) u.out[(int64_t)c * W + 16 * h + lc] = acc[ct][h][a];
-          }
-        }
+        for (int h = 0; h < H; ++h) dst[16 * h + lc] = acc[ct][h][a];
       }
-#pragma unroll
-      for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
-    }
   };
-
-  Unit cur;
-  cur.q = it.q0 + wv;
-  if (!open_block(cur)) return;
-  A av0[UF][CT][4], av1[UF][CT][4];
-  Acc acc[CT][H];
+  A av0[4][CT][4], av1[4][CT][4];
+  if constexpr (NC == 1) {
+    // tile 0, then pairs (1, 2), (3, 4), .. with no exit inside a pair (an
+    // exit there reads, to the compiler, as a second way back to the loop
+    // head with the other buffer's loads in flight), then the odd tile left
+    load(0, 0, av0);
+    load(1, 0, av1);
+    clear();
+    compute(0, av0);
+    store(0);
+    const int npairs = (ntw - 1) / 2;
+    for (int p = 0; p < npairs; ++p) {
+      const int i = 2 * p + 1;
+      load(i + 1, 0, av0);
+      clear();
+      compute(0, av1);
+      store(i);
+      load(i + 2, 0, av1);
+      clear();
+      compute(0, av0);
+      store(i + 1);
+    }
+    if ((ntw - 1) & 1) {  // tile ntw - 1, loaded into av1
+      clear();
+      compute(0, av1);
+      store(ntw - 1);
+    }
+  } else {
+    // a tile's chunks 0, 1, .., NC - 1 in av0, av1, av0, ..; the last one
+    // prefetches chunk 0 of the next tile
+    auto tile = [&](int i) {
+      clear();
 #pragma unroll
-  for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-    for (int h = 0; h < H; ++h) acc[ct][h] = Acc{};
-  load(cur, av0);
-  for (;;) {
-    Unit nxt = cur;
-    bool more = advance(nxt);
-    if (more) load(nxt, av1);
-    compute(cur, av0, acc);
-    if (!more || nxt.q != cur.q || nxt.cb != cur.cb) store(cur, acc);
-    if (!more) break;
-    cur = nxt;
-    nxt = cur;
-    more = advance(nxt);
-    if (more) load(nxt, av0);
-    compute(cur, av1, acc);
-    if (!more || nxt.q != cur.q || nxt.cb != cur.cb) store(cur, acc);
-    if (!more) break;
-    cur = nxt;
+      for (int c = 0; c < NC; c += 2) {
+        load(i, c + 1, av1);
+        compute(c, av0);
+        if (c + 2 < NC) load(i, c + 2, av0);  // compile-time branch
+        else load(i + 1, 0, av0);
+        compute(c + 1, av1);
+      }
+      store(i);
+    };
+    load(0, 0, av0);
+    tile(0);
+    for (int i = 1; i < ntw; ++i) tile(i);
   }
 }
 
@@ -350,6 +401,7 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
         for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
         load4<S, A>(U + (int64_t)cl * it.ld + il, av[st]);
       }
+      __builtin_amdgcn_sched_barrier(0);  // the batch stays ahead of the MFMAs
     };
     Acc acc[4][H];
 #pragma unroll
@@ -359,9 +411,10 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
     auto compute = [&](int c0, const A (&av)[UF][4], const A (&tb)[UF][H]) {
 #pragma unroll
       for (int st = 0; st < UF; ++st) {
-        if (c0 + 4 * st >= it.R) break;  // wave-uniform
-        // only t is masked (clamped columns); rows past the box give output
-        // rows that are never stored
+        // every step runs (a skipped one would leave its loads unwaited on
+        // one path, and the compiler then waits for everything in flight at
+        // the loop head): only t is masked (clamped columns give zero terms);
+        // rows past the box give output rows that are never stored
         const bool cok = c0 + 4 * st + kq < it.R;
         A bm[H];
 #pragma unroll
@@ -374,12 +427,17 @@ __global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ ite
     };
     constexpr int STEP = 4 * UF;
     A av0[UF][4], av1[UF][4], tb0[UF][H], tb1[UF][H];
-    if (it.R > 0) load(0, av0, tb0);
+    // loads on every pass (columns clamped, so past the end they reread the
+    // last column): the wait before a step's MFMAs then counts only its own
+    // loads, not the prefetched ones
+    // two steps per pass and no exit between them (an exit there reads, to
+    // the compiler, as a second way back to the loop head with loads in
+    // flight); a pass past R adds zeros
+    load(0, av0, tb0);
     for (int c0 = 0; c0 < it.R; c0 += 2 * STEP) {
-      if (c0 + STEP < it.R) load(c0 + STEP, av1, tb1);
+      load(c0 + STEP, av1, tb1);
       compute(c0, av0, tb0);
-      if (c0 + STEP >= it.R) break;
-      if (c0 + 2 * STEP < it.R) load(c0 + 2 * STEP, av0, tb0);
+      load(c0 + 2 * STEP, av0, tb0);
       compute(c0 + STEP, av1, tb1);
     }
 #pragma unroll
@@ -512,19 +570,33 @@ __global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict
   }
 }
 
-inline int64_t padded_points(int64_t N) { return ((N + 3) & ~int64_t(3)) + 16; }
-
-// Phase-1 items grouped by source box, the partial-sum slot of every block
-// whose source has more than kGRows rows (-1 otherwise), and one reduce entry
-// per column of those blocks {first slot, ranges, slot stride, t index}.
+// The phase-1 column table: the V columns of every block of rank >= 1,
+// grouped by source box in by-source block order, each source padded to a
+// multiple of 32 columns with copies of its last column.  Entry: element
+// offset in the V chunk | chunk << 48, and the output slot {out0, step} (slot
+// out0 + range * step): the t index and 0; for a source of more than kGRows
+// rows, partial slot T + 1 + p (the partial sums follow the t slots) and the
+// block's rank; for padding, the spare t slot T.  Items: every source of at
+// most kGRows rows whole, every larger one in ranges of kGRows rows, each
+// range in the class of 64 NC rows that holds it (NC = 1, 2, 4), in slices of
+// g_cols / NC columns; class 1 items first, then 2, then 4.  One reduce entry
+// per column of a split source {first partial (from T + 1), ranges, slot
+// stride, t index}.
 void build_group_tables(Engine& e) {
-  std::vector<GItem> items;
-  std::vector<int32_t> gblk;  // blocks of rank >= 1, grouped by source box
-  std::vector<int32_t> pbw(std::max<int64_t>(e.nblk_all, 1), -1);
+  std::vector<GTItem> items[3];
+  std::vector<int64_t> colv;
+  std::vector<int2> colo;
   std::vector<int4> cols;
   std::vector<int32_t> btoff(std::max<int64_t>(e.nblk_all, 1), 0);
-  if (e.nblk_all)
+  std::vector<BlockDev> blk(std::max<int64_t>(e.nblk_all, 1));
+  if (e.nblk_all) {
     H3D_HIP(hipMemcpy(btoff.data(), e.d_btoff, sizeof(int32_t) * e.nblk_all, hipMemcpyDeviceToHost));
+    H3D_HIP(hipMemcpy(blk.data(), e.d_blk, sizeof(BlockDev) * e.nblk_all, hipMemcpyDeviceToHost));
+  }
+  colv.reserve(e.rank_sum + e.rank_sum / 16);
+  colo.reserve(e.rank_sum + e.rank_sum / 16);
+  const int gc = e.g_cols == 1024 ? 1024 : 2048;
+  const int64_t pbase = (int64_t)e.T + 1;  // first partial slot
   int64_t np = 0;
   for (int j = 1; j <= e.nL; ++j) {
     const Level& lv = e.lev[j];
@@ -537,47 +609,55 @@ void build_group_tables(Engine& e) {
       const int64_t qb = lv.bbase + first[k], qe = lv.bbase + first[k + 1];
       if (m == 0 || qb == qe) continue;
       const int nsplit = (m + kGRows - 1) / kGRows;
-      if (nsplit > 1)
-        for (int64_t q = qb; q < qe; ++q) {
-          const int32_t b = e.by_src[q];
-          const int r = lv.rank[b - lv.bbase];
-          pbw[b] = (int32_t)np;
-          for (int c = 0; c < r; ++c) cols.push_back(make_int4((int)np + c, nsplit, r, btoff[b] + c));
-          np += (int64_t)nsplit * r;
-          if (np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
-        }
-      const int64_t g0 = (int64_t)gblk.size();
-      for (int64_t q = qb; q < qe; ++q)
-        if (lv.rank[e.by_src[q] - lv.bbase] > 0) gblk.push_back(e.by_src[q]);
-      const int64_t g1 = (int64_t)gblk.size();
-      for (int s0 = 0; s0 < m; s0 += kGRows) {
-        int64_t q = g0;
-        while (q < g1) {
-          GItem it{(int32_t)q, 0, e.bstart[lo + k], s0, std::min(m, s0 + kGRows), 0};
-          int ncol = 0;
-          while (q < g1) {
-            const int r = lv.rank[gblk[q] - lv.bbase];
-            if (ncol > 0 && ncol + r > e.g_cols) break;
-            ncol += r;
-            ++q;
+      const int64_t ldv = m + (m & 1);
+      const int64_t c0 = (int64_t)colv.size();
+      for (int64_t q = qb; q < qe; ++q) {
+        const int32_t b = e.by_src[q];
+        const int r = lv.rank[b - lv.bbase];
+        if (r == 0) continue;
+        const BlockDev& d = blk[b];
+        if (d.chunk >= (1 << 15) || d.voff + r * ldv > kColMask)
+          throw Error(H3D_ERR_INVALID, "problem too large");
+        for (int c = 0; c < r; ++c) {
+          colv.push_back(((int64_t)d.chunk << 48) | (d.voff + c * ldv));
+          if (nsplit > 1) {
+            colo.push_back(make_int2((int)(pbase + np + c), r));
+            cols.push_back(make_int4((int)(np + c), nsplit, r, btoff[b] + c));
+          } else {
+            colo.push_back(make_int2(btoff[b] + c, 0));
           }
-          it.q1 = (int32_t)q;
-          items.push_back(it);
         }
+        if (nsplit > 1) {
+          np += (int64_t)nsplit * r;
+          if (pbase + np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+        }
+      }
+      if ((int64_t)colv.size() == c0) continue;
+      while ((colv.size() - c0) % kTileCols) {
+        colv.push_back(colv.back());
+        colo.push_back(make_int2((int)e.T, 0));
+      }
+      const int64_t c1 = (int64_t)colv.size();
+      if (c1 > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+      for (int s0 = 0; s0 < m; s0 += kGRows) {
+        const int nr = std::min(kGRows, m - s0);
+        const int cls = nr <= 64 ? 0 : (nr <= 128 ? 1 : 2);
+        const int lcols = gc >> cls;
+        for (int64_t cc = c0; cc < c1; cc += lcols)
+          items[cls].push_back(GTItem{(int32_t)cc, (int32_t)std::min<int64_t>(lcols, c1 - cc),
+                                      e.bstart[lo + k] + s0, nr, s0, s0 / kGRows, 0, 0});
       }
     }
   }
-  std::vector<BlockDev> blk(std::max<int64_t>(e.nblk_all, 1));
-  if (e.nblk_all)
-    H3D_HIP(hipMemcpy(blk.data(), e.d_blk, sizeof(BlockDev) * e.nblk_all, hipMemcpyDeviceToHost));
-  std::vector<GDesc> desc(gblk.size());
-  for (size_t q = 0; q < gblk.size(); ++q) {
-    const int32_t b = gblk[q];
-    desc[q] = GDesc{blk[b].voff, blk[b].chunk, blk[b].r, blk[b].m, btoff[b], pbw[b], 0};
+  std::vector<GTItem> all;
+  for (int c = 0; c < 3; ++c) {
+    e.n_p1[c] = (int64_t)items[c].size();
+    all.insert(all.end(), items[c].begin(), items[c].end());
   }
-  e.n_gitems = (int64_t)items.size();
-  e.d_gitems = upload(items, e.stream);
-  e.d_gblk = upload(desc, e.stream);
+  e.n_gitems = (int64_t)all.size();
+  e.d_gitems = upload(all, e.stream);
+  e.d_gcolv = upload(colv, e.stream);
+  e.d_gcolo = upload(colo, e.stream);
   e.n_rcols = (int64_t)cols.size();
   e.d_rcols = upload(cols, e.stream);
   e.g_np = np;
@@ -586,18 +666,17 @@ void build_group_tables(Engine& e) {
 
 void ensure_block_scratch(Engine& e, int W) {
   if (!e.d_gitems) build_group_tables(e);
-  const int64_t T = std::max<int64_t>(e.T, 1), np = std::max<int64_t>(e.g_np, 1);
+  // t slots 0..T (T: spare for padded columns), then the partial sums
+  const int64_t T = e.T + 1, np = std::max<int64_t>(e.g_np, 1);
   if (e.w_cap >= W && e.w_t >= T && e.w_np >= np) return;
   W = std::max(W, e.w_cap);
   dfree(e.d_xw);
   dfree(e.d_tw);
-  dfree(e.d_partw);
   dfree(e.d_ylevw);
   dfree(e.d_ynw);
   e.w_cap = 0;
-  e.d_xw = dalloc<double>(2 * padded_points(e.N) * W);
-  e.d_tw = dalloc<double>(T * W);
-  e.d_partw = dalloc<double>(np * W);
+  e.d_xw = dalloc<double>(e.N * W);
+  e.d_tw = dalloc<double>((T + np) * W);
   e.d_ylevw = dalloc<double>(e.N * e.nL * W);
   e.d_ynw = dalloc<double>(e.N * W);
   e.w_cap = W;
@@ -605,16 +684,29 @@ void ensure_block_scratch(Engine& e, int W) {
   e.w_np = np;
 }
 
+template <class S, class A, int W, int NC>
+void launch_p1(Engine& e, int prec, int64_t first, int64_t count, const A* xr, A* tw,
+               hipStream_t st) {
+  if (count == 0) return;
+  auto f = e.g_cols == 1024 ? p1r_kernel<S, A, W, NC, 1024 / NC> : p1r_kernel<S, A, W, NC, 2048 / NC>;
+  const size_t lds = sizeof(A) * 64 * NC * W;  // x; the tables are static
+  const size_t stat = (sizeof(int64_t) + sizeof(int32_t)) * (e.g_cols == 1024 ? 1024 : 2048) / NC +
+                      sizeof(void*) * kLdsChunks;
+  if (lds + stat > 65536)
+    H3D_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  f<<<(unsigned)count, 256, lds, st>>>((const GTItem*)e.d_gitems + first, (int)count, e.d_gcolv,
+                                       e.d_gcolo, e.d_vbase[prec], (int)e.chunks.size(), xr, tw);
+  H3D_LAUNCH_CHECK();
+}
+
 template <class S, class A, int W>
 void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int cnt) {
   hipStream_t st = e.stream;
-  const int64_t N = e.N, Npad = padded_points(N);
-  A* xq = (A*)e.d_xw;
-  A* xr = xq + Npad * W;
+  const int64_t N = e.N;
+  A* xr = (A*)e.d_xw;
   A* tw = (A*)e.d_tw;
-  A* partw = (A*)e.d_partw;
-  gather_w_kernel<A, W><<<(unsigned)((Npad * W + 255) / 256), 256, 0, st>>>(b, e.d_perm, N, Npad,
-                                                                           cnt, xq, xr);
+  A* partw = tw + (e.T + 1) * W;  // partial sums after the t slots
+  gather_w_kernel<A, W><<<(unsigned)((N * W + 255) / 256), 256, 0, st>>>(b, e.d_perm, N, cnt, xr);
   H3D_LAUNCH_CHECK();
   {
     const int64_t n = N * e.nL * W;
@@ -667,12 +759,9 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     H3D_HIP(hipEventRecord(e.ev_join, e.side));
   }
   const Plan& p = e.plan;
-  if (e.n_gitems) {
-    p1g_kernel<S, A, W><<<(unsigned)e.n_gitems, 256, 0, st>>>(
-        (const GItem*)e.d_gitems, (int)e.n_gitems, (const GDesc*)e.d_gblk, e.d_vbase[prec], xq,
-        tw, partw);
-    H3D_LAUNCH_CHECK();
-  }
+  launch_p1<S, A, W, 1>(e, prec, 0, e.n_p1[0], xr, tw, st);
+  launch_p1<S, A, W, 2>(e, prec, e.n_p1[0], e.n_p1[1], xr, tw, st);
+  launch_p1<S, A, W, 4>(e, prec, e.n_p1[0] + e.n_p1[1], e.n_p1[2], xr, tw, st);
   if (e.n_rcols) {
     reduce_w_kernel<A, W><<<(unsigned)((e.n_rcols * W + 255) / 256), 256, 0, st>>>(
         (const int4*)e.d_rcols, e.n_rcols, partw, tw);
@@ -705,13 +794,16 @@ void run_group_w(Engine& e, int prec, bool acc64, const double* b, double* y, in
 
 void release_group_tables(Engine& e) {
   dfree(e.d_gitems);
-  dfree(e.d_gblk);
+  dfree(e.d_gcolv);
+  dfree(e.d_gcolo);
   dfree(e.d_rcols);
   e.n_gitems = e.n_rcols = e.g_np = 0;
+  e.n_p1[0] = e.n_p1[1] = e.n_p1[2] = 0;
 }
 
 int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64) {
   if (transpose || prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
+  if ((int64_t)e.chunks.size() > kLdsChunks) return 1;  // phase 1 keeps the chunk bases in LDS
   // the grouped fp32-sum near field works on fp32 offsets of radial kernels
   if (e.kern.q == kQPlugin && prec != H3D_PREC_FP64 && !acc64) return 1;
   return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
@@ -736,6 +828,8 @@ void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best
     throw Error(H3D_ERR_INVALID, "grouped products take fp64, fp32 or bf16 factor copies");
   if (nrhs < 2) throw Error(H3D_ERR_INVALID, "nrhs must be >= 2");
   if (reps < 1) throw Error(H3D_ERR_INVALID, "reps must be >= 1");
+  if ((int64_t)e.chunks.size() > kLdsChunks)
+    throw Error(H3D_ERR_STATE, "more than 1024 factor chunks: products run one column at a time");
   const bool acc64 = prec == H3D_PREC_FP64 || (mode & H3D_ACC_FP64);
   prepare_mode(e, mode);
   const int64_t N = e.N;
@@ -743,15 +837,15 @@ void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best
   double* Y = dalloc<double>(N * nrhs);
   hipEvent_t ev[2] = {nullptr, nullptr};
   const int saved_w = e.rhs_block, saved_c = e.g_cols;
-  const int widths[2] = {16, 32}, cols[4] = {256, 512, 1024, 2048};
+  const int widths[2] = {16, 32}, cols[2] = {1024, 2048};
   int bw = saved_w > 0 ? saved_w : 32, bc = saved_c;
   try {
     H3D_HIP(hipMemsetAsync(B, 0, sizeof(double) * N * nrhs, e.stream));
     for (auto& x : ev) H3D_HIP(hipEventCreate(&x));
     double bt = 1e300;
     for (int wi = 0; wi < 2; ++wi)
-      for (int ci = 0; ci < 4; ++ci) {
-        if (ms) ms[wi * 4 + ci] = -1.0;
+      for (int ci = 0; ci < 2; ++ci) {
+        if (ms) ms[wi * 2 + ci] = -1.0;
         if (wi == 1 && nrhs <= 16) continue;  // groups of <= 16 run at width 16 anyway
         e.rhs_block = widths[wi];
         if (e.g_cols != cols[ci]) {
@@ -767,7 +861,7 @@ void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best
         float t = 0;
         H3D_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
         const double avg = t / reps;
-        if (ms) ms[wi * 4 + ci] = avg;
+        if (ms) ms[wi * 2 + ci] = avg;
         if (avg < bt) {
           bt = avg;
           bw = widths[wi];

@@ -357,11 +357,11 @@ class HODLR3D:
     def autotune_rhs(self, prec="fp32", nrhs=32, reps=3):
         """Pick the group width and phase-1 item size of grouped products of
         `nrhs` columns by timing them; returns the choice and every timing."""
-        ms = (ctypes.c_double * 8)()
+        ms = (ctypes.c_double * 4)()
         best = (ctypes.c_int * 2)()
         nat.check(nat.lib().h3d_autotune_rhs(self._h, _prec(prec), int(nrhs), int(reps), ms, best))
-        times = {(w, c): ms[i * 4 + j] for i, w in enumerate((16, 32))
-                 for j, c in enumerate((256, 512, 1024, 2048)) if ms[i * 4 + j] >= 0}
+        times = {(w, c): ms[i * 2 + j] for i, w in enumerate((16, 32))
+                 for j, c in enumerate((1024, 2048)) if ms[i * 2 + j] >= 0}
         return {"width": best[0], "columns": best[1], "ms": times}
 
     def kernel_timing(self, enable=True):

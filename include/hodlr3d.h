@@ -234,10 +234,11 @@ int h3d_set_rhs_block(h3d_handle* h, int width);
 
 /* Autotuner of the grouped path: times the product of `nrhs` (>= 2) columns
  * in mode `prec` (an ACA copy) for group widths 16 and 32 and phase-1 items
- * of 256, 512, 1024 and 2048 columns, `reps` products each, and keeps the
- * fastest (the results do not depend on the choice).  ms[8] (may be NULL)
- * receives the averages, width-major, -1 where a width does not apply;
- * best[2] (may be NULL) the chosen width and item size. */
+ * of 1024 and 2048 columns on sources of at most 64 points (the leaves of a
+ * uniform tree), `reps` products each, and keeps the fastest (the
+ * results do not depend on the choice).  ms[4] (may be NULL) receives the
+ * averages, width-major, -1 where a width does not apply; best[2] (may be
+ * NULL) the chosen width and item size. */
 int h3d_autotune_rhs(h3d_handle* h, int prec, int nrhs, int reps, double* ms, int* best);
 
 /* Pins the per-lane load widths (bytes, >= the element size) for one mode.

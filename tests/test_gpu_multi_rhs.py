@@ -139,8 +139,8 @@ def test_autotune_rhs_keeps_the_product(handle):
     for p in ("fp32", "fp64", "bf16"):
         assert np.array_equal(refs[16][p], refs[32][p]), p
     res = h.autotune_rhs("fp32", nrhs=24, reps=1)
-    assert res["width"] in (16, 32) and res["columns"] in (256, 512, 1024, 2048)
-    assert len(res["ms"]) == 8 and all(v > 0 for v in res["ms"].values())
+    assert res["width"] in (16, 32) and res["columns"] in (1024, 2048)
+    assert len(res["ms"]) == 4 and all(v > 0 for v in res["ms"].values())
     assert np.array_equal(h.matvec(Q, prec="fp32"), refs[32]["fp32"])
     for w in (16, 32):
         h.set_rhs_block(w)
@@ -149,12 +149,12 @@ def test_autotune_rhs_keeps_the_product(handle):
     h.set_rhs_block(32)
     ref = h.matvec(Q, prec="fp32")
     res = h.autotune_rhs("fp32", nrhs=24, reps=1)
-    assert res["width"] in (16, 32) and res["columns"] in (256, 512, 1024, 2048)
-    assert len(res["ms"]) == 8 and all(v > 0 for v in res["ms"].values())
+    assert res["width"] in (16, 32) and res["columns"] in (1024, 2048)
+    assert len(res["ms"]) == 4 and all(v > 0 for v in res["ms"].values())
     assert np.array_equal(h.matvec(Q, prec="fp32"), ref)
-    for w, c in ((16, 256), (32, 2048)):
+    for w in (16, 32):
         h.set_rhs_block(w)
-        assert np.array_equal(h.matvec(Q, prec="fp32"), ref), (w, c)
+        assert np.array_equal(h.matvec(Q, prec="fp32"), ref), w
     h.set_rhs_block(32)
 
 
