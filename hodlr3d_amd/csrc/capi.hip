@@ -78,6 +78,8 @@ void Engine::free_all() {
   dfree(d_iow);
   dfree(d_rcols);
   dfree(d_gitems);
+  dfree(d_gp2);
+  n_gp2 = 0;
   dfree(d_gcolv);
   dfree(d_gcolo);
   n_rcols = n_gitems = g_np = 0;

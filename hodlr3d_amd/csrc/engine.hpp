@@ -329,6 +329,8 @@ struct Engine {
   void* d_gitems = nullptr;         // phase-1 items (GTItem), by class of 64, 128, 256 rows
   int64_t n_gitems = 0;
   int64_t n_p1[3] = {0, 0, 0};      // items of each class
+  void* d_gp2 = nullptr;            // phase-2 items (P2Item) of 128 rows
+  int64_t n_gp2 = 0;
   int64_t* d_gcolv = nullptr;       // column table: element offset in its V chunk | chunk << 48
   int2* d_gcolo = nullptr;          // column table: {output slot of range 0, slot step per range}
   int64_t g_np = 0;                 // partial entries per RHS

@@ -8,7 +8,8 @@
 //   phase 1   T_b = V_b^T X_s   per source box s: the V columns of every block
 //             with that source concatenated, 32-column tiles x 16-row steps,
 //             16x16x4 MFMA: A = 16 columns x 4 rows of V (one 4-row vector
-//             per lane), B = 4 rows x 16 RHS of x (from LDS)
+//             per lane), B = 4 rows x 16 RHS of x (from LDS); sources of more
+//             than 256 rows in 256-row ranges summed afterwards (reduce)
 //   phase 2   Y_k = U_k T_k     per <= 64 rows of a target box, 16x16x4 MFMA:
 //             A = 16 row quads x 4 columns of U, B = 4 columns x 16 RHS of T
 //   near      y_N = K_near X    lanes own target rows, W sums each, source
@@ -16,12 +17,13 @@
 //             own buffer, on the handle's second stream under phases 1 and 2
 //   combine   y[:, j] = alpha * (sum_levels + y_N) + d * b[:, j] (an LDS transpose)
 //
-// The streaming loops of phases 1 and 2 issue the same loads on every pass
-// and store nothing: on gfx950 loads and stores share one in-order counter,
-// so a store or a skipped load inside the loop made the compiler wait for the
-// prefetched loads (and the stores) before the next MFMAs.  fp64 factors and
-// the fp64-accumulation modes use the fp64 MFMA, fp32/bf16 factors with fp32
-// accumulation the fp32 MFMA.
+// The streaming loops of phases 1 and 2 issue the same loads (and, in phase
+// 1, the same stores) on every pass, with no branch that skips one: on gfx950
+// loads and stores share one in-order counter, and where two paths with
+// different operations in flight meet, the compiler must assume the fewest,
+// which made a wave wait for its prefetched loads (and stores) before the
+// next MFMAs.  fp64 factors and the fp64-accumulation modes use the fp64
+// MFMA, fp32/bf16 factors with fp32 accumulation the fp32 MFMA.
 #include <algorithm>
 #include <type_traits>
 
@@ -39,6 +41,12 @@ constexpr int64_t kColMask = (int64_t(1) << 48) - 1;
 // chunks takes its products one column at a time)
 constexpr int kLdsChunks = 1024;
 static_assert(kLdsChunks % 256 == 0, "staged by 256 threads");
+
+// Waves per SIMD the streaming kernels are compiled for: with fp32 sums four
+// (at most 128 registers, no spills), which keeps four 8 KB load batches in
+// flight per SIMD, about what HBM needs; fp64 sums need more registers.
+template <class A>
+constexpr int kWavesFor = sizeof(A) == 4 ? 4 : 1;
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -152,7 +160,7 @@ __global__ void gather_w_kernel(const double* __restrict__ b, const int32_t* __r
 // meet, the compiler must assume the fewest, which made a wave wait for its
 // prefetched loads and its stores.
 template <class S, class A, int W, int NC, int LC>
-__global__ __launch_bounds__(256) void p1r_kernel(const GTItem* __restrict__ items, int nitems,
+__global__ __launch_bounds__(256, kWavesFor<A>) void p1r_kernel(const GTItem* __restrict__ items, int nitems,
                                                   const int64_t* __restrict__ colv,
                                                   const int2* __restrict__ colo,
                                                   void* const* __restrict__ vbase, int nchunk,
@@ -365,7 +373,7 @@ __global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
 // of the current one.  The u-th MFMA of a step produces rows i + 4g' + u (g'
 // the MFMA output row).
 template <class S, class A, int W>
-__global__ __launch_bounds__(256) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
+__global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
                                                   void* const* __restrict__ ubase,
                                                   const A* __restrict__ tw,
                                                   double* __restrict__ ylevw, int64_t N) {
@@ -661,6 +669,35 @@ void build_group_tables(Engine& e) {
   e.n_rcols = (int64_t)cols.size();
   e.d_rcols = upload(cols, e.stream);
   e.g_np = np;
+  // phase 2: every target box in items of 128 rows, the largest first
+  std::vector<P2Item> p2;
+  for (int j = 1; j <= e.nL; ++j) {
+    const Level& lv = e.lev[j];
+    const int64_t lo = level_offset(j);
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      const int R = lv.Rt[k];
+      if (n <= 0 || R <= 0) continue;
+      for (int i0 = 0; i0 < n; i0 += 128) {
+        P2Item it{};
+        it.uoff = lv.uoff[k];
+        it.chunk = lv.chunk[k];
+        it.ld = aca_ldu(n);
+        it.i0 = i0;
+        it.nr = std::min(128, n - i0);
+        it.R = R;
+        it.toff = lv.toff[k];
+        it.yoff = e.bstart[lo + k];
+        it.level = j;
+        p2.push_back(it);
+      }
+    }
+  }
+  std::stable_sort(p2.begin(), p2.end(), [](const P2Item& a, const P2Item& b) {
+    return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
+  });
+  e.n_gp2 = (int64_t)p2.size();
+  e.d_gp2 = upload(p2, e.stream);
   H3D_HIP(hipStreamSynchronize(e.stream));
 }
 
@@ -758,7 +795,6 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     launch_near(e.side);
     H3D_HIP(hipEventRecord(e.ev_join, e.side));
   }
-  const Plan& p = e.plan;
   launch_p1<S, A, W, 1>(e, prec, 0, e.n_p1[0], xr, tw, st);
   launch_p1<S, A, W, 2>(e, prec, e.n_p1[0], e.n_p1[1], xr, tw, st);
   launch_p1<S, A, W, 4>(e, prec, e.n_p1[0] + e.n_p1[1], e.n_p1[2], xr, tw, st);
@@ -767,9 +803,9 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
         (const int4*)e.d_rcols, e.n_rcols, partw, tw);
     H3D_LAUNCH_CHECK();
   }
-  if (!p.p2.empty()) {
-    p2w_kernel<S, A, W><<<wave_grid(p.p2.size()), 256, 0, st>>>(p.d_p2, (int)p.p2.size(),
-                                                                e.d_ubase[prec], tw, e.d_ylevw, N);
+  if (e.n_gp2) {
+    p2w_kernel<S, A, W><<<wave_grid(e.n_gp2), 256, 0, st>>>((const P2Item*)e.d_gp2, (int)e.n_gp2,
+                                                           e.d_ubase[prec], tw, e.d_ylevw, N);
     H3D_LAUNCH_CHECK();
   }
   if (nn && !overlap) launch_near(st);
@@ -794,6 +830,8 @@ void run_group_w(Engine& e, int prec, bool acc64, const double* b, double* y, in
 
 void release_group_tables(Engine& e) {
   dfree(e.d_gitems);
+  dfree(e.d_gp2);
+  e.n_gp2 = 0;
   dfree(e.d_gcolv);
   dfree(e.d_gcolo);
   dfree(e.d_rcols);
