@@ -117,6 +117,16 @@ SIGNATURES = {
                                 _i32, _i32, ctypes.POINTER(_p)]),
     "h3d_graph_launch": (_i32, [_p, _p]),
     "h3d_graph_destroy": (None, [_p]),
+    "h3d_batch_create": (_i32, [ctypes.POINTER(_p), _i32, _i32, ctypes.POINTER(_p)]),
+    "h3d_batch_layout": (_i32, [_p, _p]),
+    "h3d_batch_matvec": (_i32, [_p, _p, _p]),
+    "h3d_batch_set_stream": (_i32, [_p, _p]),
+    "h3d_batch_synchronize": (_i32, [_p]),
+    "h3d_batch_autotune": (_i32, [_p, _i32, _p]),
+    "h3d_batch_set_tuning": (_i32, [_p, _i32, _i32]),
+    "h3d_batch_get_tuning": (_i32, [_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "h3d_batch_graph_create": (_i32, [_p, _p, _p, ctypes.POINTER(_p)]),
+    "h3d_batch_destroy": (None, [_p]),
     "h3d_set_stream": (_i32, [_p, _p]),
     "h3d_use_own_stream": (_i32, [_p]),
     "h3d_synchronize": (_i32, [_p]),

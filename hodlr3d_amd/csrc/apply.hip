@@ -1423,6 +1423,48 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   H3D_LAUNCH_CHECK();
 }
 
+// Phase-1 launch over an item table with a load width of `bytes` per lane.
+template <class S, class A>
+void p1_raw(const P1Item* it, int n1, void* const* vbase, const BlockDev* blk, const A* x, A* t,
+            A* part, const double* scale, int bytes, hipStream_t st) {
+  if (!n1) return;
+  const unsigned g = wave_grid(n1);
+  constexpr int es = (int)sizeof(S);
+  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
+    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t, part, scale);
+  else if (bytes == 8 && 8 / es >= 1)
+    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
+                                                                      part, scale);
+  else if (bytes == 4 && 4 / es >= 1)
+    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
+                                                                      part, scale);
+  else
+    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
+                                                                      part, scale);
+  H3D_LAUNCH_CHECK();
+}
+
+// Phase-2 launch over an item table with a load width of `bytes` per lane.
+template <class S, class A>
+void p2_raw(const P2Item* it, int n2, void* const* ubase, const A* t, double* ylev, int64_t N,
+            int acc, int bytes, hipStream_t st) {
+  if (!n2) return;
+  const unsigned g = wave_grid(n2);
+  constexpr int es = (int)sizeof(S);
+  if (bytes >= 16)
+    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N, acc);
+  else if (bytes == 8)
+    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
+                                                                      acc);
+  else if (bytes == 4)
+    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
+                                                                      acc);
+  else
+    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
+                                                                      acc);
+  H3D_LAUNCH_CHECK();
+}
+
 // Phase-1 / phase-2 launch with a load width of `bytes` per lane.
 template <class S, class A>
 void launch_p1(Engine& e, const FactorView& f, int bytes, const A* x, A* t, A* part) {
@@ -1430,24 +1472,8 @@ void launch_p1(Engine& e, const FactorView& f, int bytes, const A* x, A* t, A* p
     launch_mp1<S, A>(e, f, bytes, e.tune_k[f.key], x, t, part);
     return;
   }
-  const int n1 = (int)f.plan->p1.size();
-  if (!n1) return;
-  const unsigned g = wave_grid(n1);
-  hipStream_t st = e.stream;
-  constexpr int es = (int)sizeof(S);
-  const P1Item* it = f.plan->d_p1;
-  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
-    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t, part, f.scale);
-  else if (bytes == 8 && 8 / es >= 1)
-    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
-                                                                      part, f.scale);
-  else if (bytes == 4 && 4 / es >= 1)
-    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
-                                                                      part, f.scale);
-  else
-    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, f.vbase, f.blk, x, t,
-                                                                      part, f.scale);
-  H3D_LAUNCH_CHECK();
+  p1_raw<S, A>(f.plan->d_p1, (int)f.plan->p1.size(), f.vbase, f.blk, x, t, part, f.scale, bytes,
+               e.stream);
 }
 
 template <class A>
@@ -1474,25 +1500,7 @@ void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
 
 template <class S, class A>
 void launch_p2(Engine& e, const FactorView& f, int bytes, const A* t) {
-  const int n2 = f.n2;
-  if (!n2) return;
-  const unsigned g = wave_grid(n2);
-  hipStream_t st = e.stream;
-  constexpr int es = (int)sizeof(S);
-  const P2Item* it = f.p2;
-  const int acc = f.accumulate;
-  if (bytes >= 16)
-    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev, e.N, acc);
-  else if (bytes == 8)
-    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
-                                                                      e.N, acc);
-  else if (bytes == 4)
-    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
-                                                                      e.N, acc);
-  else
-    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n2, f.ubase, t, e.d_ylev,
-                                                                      e.N, acc);
-  H3D_LAUNCH_CHECK();
+  p2_raw<S, A>(f.p2, f.n2, f.ubase, t, e.d_ylev, e.N, f.accumulate, bytes, e.stream);
 }
 
 // `mark` (if set) is recorded right after the phase-1 kernel itself.
@@ -1619,6 +1627,117 @@ void ensure_copy(Engine& e, int prec) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Launchers over raw tables of the ACA-copy product (the single product and
+// the instance batch share them).  x, t and the partial sums are fp64 with
+// fp64 accumulation (xp), else fp32 (xf, and t / part read as floats).
+
+NearTables near_tables(Engine& e) {
+  NearTables nt;
+  const int64_t lo = level_offset(e.nL);
+  nt.items = e.d_near;
+  nt.n = (int)e.nitems.size();
+  nt.nbr = e.lev[e.nL].d_nbr;
+  nt.bstart = e.d_bstart + lo;
+  nt.bcount = e.d_bcount + lo;
+  nt.px = e.d_px, nt.py = e.d_py, nt.pz = e.d_pz;
+  nt.lx = e.d_lx, nt.ly = e.d_ly, nt.lz = e.d_lz;
+  nt.leafc = e.d_leafc;
+  nt.perm = e.d_perm;
+  nt.kd = e.kern;
+  nt.xp = e.d_xp;
+  nt.xf = e.d_xf;
+  nt.ynear = e.d_ynear;
+  return nt;
+}
+
+void far_gather(const double* b, const int32_t* perm, int64_t N, double* xp, float* xf,
+                hipStream_t st) {
+  if (N <= 0) return;
+  gather_kernel<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(b, perm, N, xp, xf);
+  H3D_LAUNCH_CHECK();
+}
+
+void far_zero(double* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  zero_kernel<<<(unsigned)std::min<int64_t>(8192, (n + 255) / 256), 256, 0, st>>>(p, n);
+  H3D_LAUNCH_CHECK();
+}
+
+void far_phase1(int prec, bool acc64, int bytes, const P1Item* items, int n, void* const* vbase,
+                const BlockDev* blk, const double* xp, const float* xf, double* t, double* part,
+                hipStream_t st) {
+  if (acc64) {
+    if (prec == H3D_PREC_FP64) p1_raw<double, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
+    else if (prec == H3D_PREC_FP32) p1_raw<float, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
+    else p1_raw<uint16_t, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
+  } else if (prec == H3D_PREC_FP32) {
+    p1_raw<float, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st);
+  } else {
+    p1_raw<uint16_t, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st);
+  }
+}
+
+void far_reduce(bool acc64, const P1Reduce* rs, int n, const double* part, double* t,
+                hipStream_t st) {
+  if (n <= 0) return;
+  if (acc64)
+    reduce_kernel<double><<<wave_grid(n), 256, 0, st>>>(rs, n, part, t, nullptr);
+  else
+    reduce_kernel<float><<<wave_grid(n), 256, 0, st>>>(rs, n, (const float*)part, (float*)t, nullptr);
+  H3D_LAUNCH_CHECK();
+}
+
+void far_phase2(int prec, bool acc64, int bytes, const P2Item* items, int n, void* const* ubase,
+                const double* t, double* ylev, int64_t N, hipStream_t st) {
+  if (acc64) {
+    if (prec == H3D_PREC_FP64) p2_raw<double, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
+    else if (prec == H3D_PREC_FP32) p2_raw<float, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
+    else p2_raw<uint16_t, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
+  } else if (prec == H3D_PREC_FP32) {
+    p2_raw<float, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st);
+  } else {
+    p2_raw<uint16_t, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st);
+  }
+}
+
+// Near field of an ACA-copy product (every mode but the mixed copy): fp64
+// factors take the exact entries, fp32/bf16 factors with fp64 accumulation the
+// Newton-refined 1/r (or the exact entries of other kernels), fp32 sums the
+// fp32 entries; a plugin kernel's entries are fp64 in every mode, and the
+// transpose of a plugin evaluates entry(x_j, x_i).
+void far_near(const NearTables& nt, int prec, bool acc64, bool transpose, hipStream_t st) {
+  const int nn = nt.n;
+  if (nn <= 0) return;
+  const KernelDev kd = nt.kd;
+  const unsigned g = wave_grid(nn);
+  const bool invr = kd.q == 7 || kd.q == 16;
+  if (kd.q == kQPlugin && transpose)
+    near64_kernel<Plugin, true><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount,
+                                                   nt.px, nt.py, nt.pz, nt.perm, kd, nt.xp, nt.ynear);
+  else if (kd.q == kQPlugin)
+    near64_kernel<Plugin><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount, nt.px,
+                                             nt.py, nt.pz, nt.perm, kd, nt.xp, nt.ynear);
+  else if (acc64 && prec != H3D_PREC_FP64 && invr)
+    near_nr_kernel<<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount, nt.px, nt.py,
+                                      nt.pz, kd, nt.xp, nt.ynear);
+  else if (acc64 && invr)
+    near64_kernel<InvR><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount, nt.px,
+                                           nt.py, nt.pz, nt.perm, kd, nt.xp, nt.ynear);
+  else if (acc64)
+    near64_kernel<AnyQ><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount, nt.px,
+                                           nt.py, nt.pz, nt.perm, kd, nt.xp, nt.ynear);
+  else if (invr)
+    near32_kernel<float, InvR32><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount,
+                                                    nt.lx, nt.ly, nt.lz, nt.leafc, nt.perm, kd,
+                                                    nt.xf, nt.ynear);
+  else
+    near32_kernel<float, AnyQ32><<<g, 256, 0, st>>>(nt.items, nn, nt.nbr, nt.bstart, nt.bcount,
+                                                    nt.lx, nt.ly, nt.lz, nt.leafc, nt.perm, kd,
+                                                    nt.xf, nt.ynear);
+  H3D_LAUNCH_CHECK();
+}
+
 void autotune(Engine& e, int mode, int reps, double* ms) {
   if (!e.assembled) throw Error(H3D_ERR_STATE, "autotune before assemble");
   int prec;
@@ -1687,48 +1806,22 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     const KernelDev kd = e.kern;
     const unsigned g = wave_grid(nn);
     // mixed: fp32 entries summed in fp64 (their error, ~2e-8 of ||A x|| at
-    // 2^20 points, sits below the copy's fp32 rounding floor); fp32/bf16
-    // factors with fp64 accumulation: Newton-refined 1/r (fp64 accuracy);
-    // fp64: the exact entries
+    // 2^20 points, sits below the copy's fp32 rounding floor); the ACA copies
+    // as far_near chooses
     const bool invr = kd.q == 7 || kd.q == 16;
-    // the plugin: its entries in fp64 for every mode (from the fp64 x)
-    if (kd.q == kQPlugin && transpose)
-      near64_kernel<Plugin, true><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr,
-                                                     e.d_bstart + lo, e.d_bcount + lo, e.d_px,
-                                                     e.d_py, e.d_pz, e.d_perm, kd, e.d_xp,
-                                                     e.d_ynear);
-    else if (kd.q == kQPlugin)
-      near64_kernel<Plugin><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                               e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
-                                               kd, e.d_xp, e.d_ynear);
-    else if (mixed && invr)
-      near32_kernel<double, InvR32><<<g, 256, 0, ns>>>(
-          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
-    else if (mixed)
-      near32_kernel<double, AnyQ32><<<g, 256, 0, ns>>>(
-          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
-    else if (acc64 && prec != H3D_PREC_FP64 && (kd.q == 7 || kd.q == 16))
-      near_nr_kernel<<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                        e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd, e.d_xp, e.d_ynear);
-    else if (acc64 && (kd.q == 7 || kd.q == 16))
-      near64_kernel<InvR><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                             e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
-                                             kd, e.d_xp, e.d_ynear);
-    else if (acc64)
-      near64_kernel<AnyQ><<<g, 256, 0, ns>>>(e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo,
-                                             e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, e.d_perm,
-                                             kd, e.d_xp, e.d_ynear);
-    else if (invr)
-      near32_kernel<float, InvR32><<<g, 256, 0, ns>>>(
-          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, e.d_ynear);
-    else
-      near32_kernel<float, AnyQ32><<<g, 256, 0, ns>>>(
-          e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
-          e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xf, e.d_ynear);
-    H3D_LAUNCH_CHECK();
+    if (mixed && kd.q != kQPlugin) {
+      if (invr)
+        near32_kernel<double, InvR32><<<g, 256, 0, ns>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
+      else
+        near32_kernel<double, AnyQ32><<<g, 256, 0, ns>>>(
+            e.d_near, nn, e.lev[e.nL].d_nbr, e.d_bstart + lo, e.d_bcount + lo, e.d_lx, e.d_ly,
+            e.d_lz, e.d_leafc, e.d_perm, kd, e.d_xp, e.d_ynear);
+      H3D_LAUNCH_CHECK();
+      return;
+    }
+    far_near(near_tables(e), prec, acc64, transpose, ns);
   };
   for (int r = 0; r < nrhs; ++r) {
     const double* b = d_b + (int64_t)r * N;

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
 
@@ -911,6 +912,140 @@ void h3d_graph_destroy(h3d_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+}
+
+struct h3d_batch {
+  h3d::Batch b;
+};
+
+int h3d_batch_create(h3d_handle* const* hs, int count, int mode, h3d_batch** out) {
+  return guarded([&] {
+    if (!hs || !out || count < 1) throw h3d::Error(H3D_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    std::vector<h3d::Engine*> es(count);
+    for (int i = 0; i < count; ++i) {
+      if (!hs[i]) throw h3d::Error(H3D_ERR_INVALID, "null handle");
+      es[i] = &hs[i]->e;
+    }
+    auto* bt = new h3d_batch;
+    try {
+      h3d::build_batch(bt->b, es.data(), count, mode);
+    } catch (...) {
+      delete bt;
+      throw;
+    }
+    *out = bt;
+  });
+}
+
+int h3d_batch_layout(h3d_batch* bt, int64_t* offsets) {
+  return guarded([&] {
+    if (!bt || !offsets) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    std::memcpy(offsets, bt->b.off.data(), sizeof(int64_t) * bt->b.off.size());
+  });
+}
+
+int h3d_batch_matvec(h3d_batch* bt, const double* d_b, double* d_y) {
+  return guarded([&] {
+    if (!bt || !d_b || !d_y) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    h3d::batch_matvec(bt->b, d_b, d_y);
+  });
+}
+
+int h3d_batch_set_stream(h3d_batch* bt, void* stream) {
+  return guarded([&] {
+    if (!bt) throw h3d::Error(H3D_ERR_INVALID, "null batch");
+    hipStream_t to = (hipStream_t)stream;  // NULL: the legacy default stream
+    if (to == bt->b.stream) return;
+    // the batch's scratch is shared by its products: the new stream waits for
+    // what the old one holds
+    H3D_HIP(hipEventRecord(bt->b.ev_fork, bt->b.stream));
+    H3D_HIP(hipStreamWaitEvent(to, bt->b.ev_fork, 0));
+    bt->b.stream = to;
+  });
+}
+
+int h3d_batch_synchronize(h3d_batch* bt) {
+  return guarded([&] {
+    if (!bt) throw h3d::Error(H3D_ERR_INVALID, "null batch");
+    H3D_HIP(hipStreamSynchronize(bt->b.stream));
+  });
+}
+
+int h3d_batch_autotune(h3d_batch* bt, int reps, double* ms) {
+  return guarded([&] {
+    if (!bt) throw h3d::Error(H3D_ERR_INVALID, "null batch");
+    h3d::batch_autotune(bt->b, reps, ms);
+    H3D_HIP(hipStreamSynchronize(bt->b.stream));
+  });
+}
+
+int h3d_batch_set_tuning(h3d_batch* bt, int p1_bytes, int p2_bytes) {
+  return guarded([&] {
+    if (!bt) throw h3d::Error(H3D_ERR_INVALID, "null batch");
+    const int es = h3d::elem_size(bt->b.prec);
+    for (int w : {p1_bytes, p2_bytes})
+      if ((w != 2 && w != 4 && w != 8 && w != 16) || w < es)
+        throw h3d::Error(H3D_ERR_INVALID, "load width must be 2, 4, 8 or 16 bytes and hold an element");
+    bt->b.tune_p1 = p1_bytes;
+    bt->b.tune_p2 = p2_bytes;
+  });
+}
+
+int h3d_batch_get_tuning(h3d_batch* bt, int* p1_bytes, int* p2_bytes) {
+  return guarded([&] {
+    if (!bt || !p1_bytes || !p2_bytes) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    *p1_bytes = bt->b.tune_p1;
+    *p2_bytes = bt->b.tune_p2;
+  });
+}
+
+int h3d_batch_graph_create(h3d_batch* bt, const double* d_b, double* d_y, h3d_graph** out) {
+  return guarded([&] {
+    if (!bt || !d_b || !d_y || !out) throw h3d::Error(H3D_ERR_INVALID, "null pointer");
+    *out = nullptr;
+    H3D_HIP(hipStreamSynchronize(bt->b.stream));
+    // the capture runs on a stream of its own, the near field on a branch
+    // forked from it (streams and events made for this capture only)
+    hipStream_t cap = nullptr, br = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto release = [&] {
+      for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+      if (br) (void)hipStreamDestroy(br);
+      if (cap) (void)hipStreamDestroy(cap);
+    };
+    auto* g = new h3d_graph;
+    try {
+      H3D_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      H3D_HIP(hipStreamCreateWithFlags(&br, hipStreamNonBlocking));
+      for (auto& x : ev) H3D_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+      H3D_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      try {
+        h3d::batch_product(bt->b, d_b, d_y, cap, br, ev[0], ev[1]);
+      } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(cap, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+      }
+      H3D_HIP(hipStreamEndCapture(cap, &g->graph));
+      H3D_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+      release();
+      throw;
+    }
+    release();
+    *out = g;
+  });
+}
+
+void h3d_batch_destroy(h3d_batch* bt) {
+  if (!bt) return;
+  delete bt;
 }
 
 namespace {

@@ -393,6 +393,42 @@ inline int elem_size(int prec) { return prec == 0 ? 8 : (prec == 1 ? 4 : 2); }
 // Precision code of the recompressed mixed copy (fp32 + fp16 columns, fp64 accumulation).
 constexpr int kPrecMixed = 3;
 
+// The near field's inputs: items over the leaves, each leaf's 26 neighbour
+// slots (-1: none) and position segment, the points (fp64 by position, fp32
+// offsets from the leaf centre) and x by position.
+struct NearTables {
+  const NearItem* items = nullptr;
+  int n = 0;
+  const int32_t* nbr = nullptr;
+  const int32_t* bstart = nullptr;
+  const int32_t* bcount = nullptr;
+  const double *px = nullptr, *py = nullptr, *pz = nullptr;
+  const float *lx = nullptr, *ly = nullptr, *lz = nullptr;
+  const double* leafc = nullptr;
+  const int32_t* perm = nullptr;
+  KernelDev kd;
+  const double* xp = nullptr;
+  const float* xf = nullptr;
+  double* ynear = nullptr;
+};
+NearTables near_tables(Engine& e);
+
+// Launchers of the ACA-copy product over raw tables (apply.hip), shared by
+// the single product and the instance batch (batch.hip).  With fp64
+// accumulation x, t and the partial sums are fp64 (xp); otherwise x is xf and
+// t / part hold floats.
+void far_gather(const double* b, const int32_t* perm, int64_t N, double* xp, float* xf,
+                hipStream_t st);
+void far_zero(double* p, int64_t n, hipStream_t st);
+void far_phase1(int prec, bool acc64, int bytes, const P1Item* items, int n, void* const* vbase,
+                const BlockDev* blk, const double* xp, const float* xf, double* t, double* part,
+                hipStream_t st);
+void far_reduce(bool acc64, const P1Reduce* rs, int n, const double* part, double* t,
+                hipStream_t st);
+void far_phase2(int prec, bool acc64, int bytes, const P2Item* items, int n, void* const* ubase,
+                const double* t, double* ylev, int64_t N, hipStream_t st);
+void far_near(const NearTables& nt, int prec, bool acc64, bool transpose, hipStream_t st);
+
 // tree.hip
 void build_tree(Engine& e, const double* h_xyz);
 // aca.hip

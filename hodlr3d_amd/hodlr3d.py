@@ -547,6 +547,111 @@ class MatvecGraph:
             pass
 
 
+class Batch:
+    """One product of each of many assembled handles in one launch per step for
+    all of them (h3d_batch_create: the instances' work tables merged, factors
+    read in place).  Vectors are packed CUDA float64 tensors: instance i owns
+    ``b[off[i]:off[i+1]]`` in its own point order (``offsets``; ``pack`` and
+    ``unpack`` convert lists of per-instance tensors).  Products run on torch's
+    current stream.  Keep the handles and their ``prec`` copy alive while the
+    batch is used."""
+
+    def __init__(self, handles, prec="fp32"):
+        if not handles:
+            raise ValueError("a batch needs at least one handle")
+        self.handles = list(handles)
+        n = len(self.handles)
+        hs = (ctypes.c_void_p * n)(*[h._h.value for h in self.handles])
+        bt = ctypes.c_void_p()
+        nat.check(nat.lib().h3d_batch_create(hs, n, _prec(prec), ctypes.byref(bt)))
+        self._b = bt
+        off = np.zeros(n + 1, dtype=np.int64)
+        nat.check(nat.lib().h3d_batch_layout(self._b, off.ctypes.data_as(ctypes.c_void_p)))
+        self.offsets = off
+        self.N = int(off[-1])
+        self._graphs = []
+
+    def _bind(self):
+        import torch
+        nat.check(nat.lib().h3d_batch_set_stream(
+            self._b, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def _check(self, t):
+        import torch
+        if not (_is_torch_cuda(t) and t.dtype == torch.float64 and t.is_contiguous()
+                and t.numel() == self.N):
+            raise ValueError("packed vectors are contiguous CUDA float64 tensors of the batch's N")
+
+    def pack(self, parts):
+        import torch
+        if len(parts) != len(self.handles):
+            raise ValueError("one vector per instance")
+        return torch.cat([p.reshape(-1).to(torch.float64) for p in parts]).contiguous()
+
+    def unpack(self, y):
+        o = self.offsets
+        return [y[o[i]:o[i + 1]] for i in range(len(self.handles))]
+
+    def matvec(self, b, out=None):
+        """y = A_i b_i for every instance i (packed b -> packed y)."""
+        import torch
+        self._check(b)
+        y = torch.empty_like(b) if out is None else out
+        self._check(y)
+        self._bind()
+        nat.check(nat.lib().h3d_batch_matvec(self._b, ctypes.c_void_p(b.data_ptr()),
+                                             ctypes.c_void_p(y.data_ptr())))
+        return y
+
+    def autotune(self, reps=3):
+        """Fastest load width per phase over the merged tables; ms per width."""
+        self._bind()
+        ms = (ctypes.c_double * 8)(*([-1.0] * 8))
+        nat.check(nat.lib().h3d_batch_autotune(self._b, int(reps), ms))
+        w = [16, 8, 4, 2]
+        return {"phase1": {w[i]: ms[i] for i in range(4) if ms[i] >= 0},
+                "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0}}
+
+    def get_tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.lib().h3d_batch_get_tuning(self._b, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_tuning(self, p1_bytes, p2_bytes):
+        nat.check(nat.lib().h3d_batch_set_tuning(self._b, int(p1_bytes), int(p2_bytes)))
+
+    def graph(self, b, y):
+        """Captures one product b -> y (both kept alive here) into a HIP graph;
+        returns a callable that replays it on torch's current stream."""
+        import torch
+        self._check(b)
+        self._check(y)
+        self._bind()
+        g = ctypes.c_void_p()
+        nat.check(nat.lib().h3d_batch_graph_create(self._b, ctypes.c_void_p(b.data_ptr()),
+                                                   ctypes.c_void_p(y.data_ptr()), ctypes.byref(g)))
+        self._graphs.append((g, b, y))
+
+        def launch():
+            nat.check(nat.lib().h3d_graph_launch(
+                g, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return launch
+
+    def close(self):
+        for g, _, _ in self._graphs:
+            nat.lib().h3d_graph_destroy(g)
+        self._graphs = []
+        if getattr(self, "_b", None):
+            nat.lib().h3d_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class classGMRES:
     """src/gmres.cpp:40-94 on the GPU: unrestarted GMRES, x0 = 0, MGS + Givens."""
 

@@ -361,6 +361,46 @@ int h3d_graph_create(h3d_handle* const* hs, int count, const double* const* b, d
 int h3d_graph_launch(h3d_graph* g, void* stream);
 void h3d_graph_destroy(h3d_graph* g);
 
+/* Instance batch (no reference counterpart; the caller side of MatVecProduct,
+ * src/HODLR3D.cpp:21-27, for many independent problems): one product of each
+ * of `count` assembled handles in one launch per step for all of them.  The
+ * work tables of the instances (far-field items, row-split reductions,
+ * near-field items, chunk base pointers into each handle's own factors) are
+ * merged at creation, so a product is gather -> phase 1 -> reduce -> phase 2
+ * -> combine over every instance at once, with the near field of all leaves on
+ * a second stream.  Each instance's result equals its own h3d_matvec in the
+ * same mode bit for bit when both use the same load widths.
+ *   mode: H3D_PREC_FP64, H3D_PREC_FP32 or H3D_PREC_BF16, optionally
+ *         | H3D_ACC_FP64 (the ACA copies; the copy is made where missing).
+ * All handles share the kernel (Qchoice and scale); each keeps its own operator
+ * (alpha, diagonal) as set at creation.  A handle may appear once.
+ * Vectors are packed device arrays of float64: instance i owns entries
+ * [off[i], off[i+1]) of b and y in its own point order (h3d_batch_layout
+ * writes the count + 1 offsets).  The batch reads the handles' factors: it
+ * stays valid while every handle lives and keeps the copy of `mode` (no
+ * release, move or destroy in between).  Products run on the batch's own
+ * stream until h3d_batch_set_stream names another (NULL: the legacy default
+ * stream; the new stream first waits for the old one's work); products and
+ * graph replays of one batch share its scratch, so the caller keeps them in
+ * order.
+ * h3d_batch_autotune times the load widths (2..16 bytes per lane) of both
+ * phases over the merged tables and keeps the fastest (ms[8] as
+ * h3d_autotune's); h3d_batch_set_tuning / get_tuning pin or read them.
+ * h3d_batch_graph_create captures one product b -> y (device pointers baked
+ * in) into a graph replayed by h3d_graph_launch and freed by
+ * h3d_graph_destroy. */
+typedef struct h3d_batch h3d_batch;
+int h3d_batch_create(h3d_handle* const* hs, int count, int mode, h3d_batch** out);
+int h3d_batch_layout(h3d_batch* bt, int64_t* offsets);
+int h3d_batch_matvec(h3d_batch* bt, const double* d_b, double* d_y);
+int h3d_batch_set_stream(h3d_batch* bt, void* stream);
+int h3d_batch_synchronize(h3d_batch* bt);
+int h3d_batch_autotune(h3d_batch* bt, int reps, double* ms);
+int h3d_batch_set_tuning(h3d_batch* bt, int p1_bytes, int p2_bytes);
+int h3d_batch_get_tuning(h3d_batch* bt, int* p1_bytes, int* p2_bytes);
+int h3d_batch_graph_create(h3d_batch* bt, const double* d_b, double* d_y, h3d_graph** out);
+void h3d_batch_destroy(h3d_batch* bt);
+
 /* Run subsequent calls on a caller-owned hipStream_t (passed as void*; NULL is
  * the legacy default stream); h3d_use_own_stream returns to the handle's own.
  * Either switch makes the new stream wait for the work already queued on the
