@@ -373,6 +373,9 @@ def batch_instances(torch, H3, count, nmin, nmax, steps, warmup):
     torch.cuda.synchronize()
     batch_same = all(torch.equal(yi, ri) for yi, ri in zip(bt.unpack(yb), ref))
     tune = bt.autotune(reps=3)
+    w1, w2 = bt.get_tuning()
+    chosen = {"phase1": {"bytes": w1 & 0xff, "persistent": bool(w1 & H3.PERSISTENT)},
+              "phase2": {"bytes": w2 & 0xff, "persistent": bool(w2 & H3.PERSISTENT)}}
     merged_ms = timed(torch, lambda: bt.matvec(bp, out=yb), steps, warmup)
     yg = torch.empty_like(bp)
     launch = bt.graph(bp, yg)
@@ -405,18 +408,13 @@ def batch_instances(torch, H3, count, nmin, nmax, steps, warmup):
            "speedup": round(eager_ms / best, 2),
            "factor_GB": round(fbytes / 1e9, 2),
            "factor_GB_per_s": round(fbytes / (best * 1e-3) / 1e9, 1),
-           "merged_tuning": bt_tuning(tune),
+           "merged_tuning": chosen, "merged_autotune_ms": tune,
            "graph_equals_eager": same, "merged_equals_eager": batch_same,
            "merged_graph_equals_merged": graph_same,
            "rel_err_vs_direct": err, "assemble_s": round(t_asm, 2),
            "batch_create_s": round(t_batch, 2)}
     log(f"[bench] batch {out}")
     return out
-
-
-def bt_tuning(ms):
-    """Chosen load width per phase from the batch autotuner's timings."""
-    return {k: min(v, key=v.get) for k, v in ms.items()}
 
 
 def spec_tolerance_run(torch, H3, args):

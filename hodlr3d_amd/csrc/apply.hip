@@ -143,17 +143,13 @@ __device__ inline void load_factor(const S* p, A (&o)[NV]) {
 // the item's range see x = 0, so loads may run into padding or the next column.
 // With `scale` (indexed like t) the finished t_k is multiplied by scale_k.
 template <class S, class A, int VEC>
-__global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ items, int nitems,
-                                                     void* const* __restrict__ vbase,
-                                                     const BlockDev* __restrict__ blk,
-                                                     const A* __restrict__ x, A* __restrict__ t,
-                                                     A* __restrict__ part,
-                                                     const double* __restrict__ scale) {
+__device__ __forceinline__ void phase1_item(const P1Item& it, void* const* __restrict__ vbase,
+                                            const BlockDev* __restrict__ blk,
+                                            const A* __restrict__ x, A* __restrict__ t,
+                                            A* __restrict__ part,
+                                            const double* __restrict__ scale) {
   constexpr int KMAX = 8 / VEC > 0 ? 8 / VEC : 1;  // rows per lane <= 8 (items have <= 512 rows)
-  const int w = wave_id();
-  if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
-  const P1Item it = items[w];
   const S* V = (const S*)vbase[it.chunk];
   for (int b = it.b0; b < it.b1; ++b) {
     const BlockDev bd = blk[b];
@@ -247,6 +243,36 @@ __global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ 
       }
     }
   }
+}
+
+template <class S, class A, int VEC>
+__global__ __launch_bounds__(256) void phase1_kernel(const P1Item* __restrict__ items, int nitems,
+                                                     void* const* __restrict__ vbase,
+                                                     const BlockDev* __restrict__ blk,
+                                                     const A* __restrict__ x, A* __restrict__ t,
+                                                     A* __restrict__ part,
+                                                     const double* __restrict__ scale) {
+  const int w = wave_id();
+  if (w >= nitems) return;
+  phase1_item<S, A, VEC>(items[w], vbase, blk, x, t, part, scale);
+}
+
+// The persistent form: a grid of as many waves as stay resident, each taking
+// groups of items from the queue (an atomic counter cleared before the
+// launch) until none is left; every wave exits once the counter has passed
+// nitems.
+template <class S, class A, int VEC>
+__global__ __launch_bounds__(256) void phase1_queue_kernel(const P1Item* __restrict__ items,
+                                                           int nitems,
+                                                           void* const* __restrict__ vbase,
+                                                           const BlockDev* __restrict__ blk,
+                                                           const A* __restrict__ x,
+                                                           A* __restrict__ t, A* __restrict__ part,
+                                                           const double* __restrict__ scale,
+                                                           unsigned* __restrict__ queue) {
+  // four items per take: phase 1 has many small items (runs of blocks)
+  queue_loop<4>(queue, nitems,
+                [&](int w) { phase1_item<S, A, VEC>(items[w], vbase, blk, x, t, part, scale); });
 }
 
 // Sum of the nsplit partials part[base + q * nc] (q = 0, 1, ...) in q order,
@@ -462,19 +488,14 @@ __global__ void tgather_kernel(const int32_t* __restrict__ map, int n, const A* 
 // into G groups that walk different columns, so one wave instruction reads G
 // consecutive columns of the box (contiguous when the item spans the box).
 // accumulate != 0 adds to y_level (a second factor copy) instead of storing.
-template <class S, class A, int VR>
-__global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ items, int nitems,
-                                                     void* const* __restrict__ ubase,
-                                                     const A* __restrict__ t,
-                                                     double* __restrict__ ylev, int64_t N,
-                                                     int accumulate) {
-  __shared__ A red[kWavesPerBlock][64 * VR];
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int w = wave_id();
+// QUEUE: the persistent form's item body (valid always; the wave's own LDS
+// slice is ordered by wave barriers instead of a workgroup barrier).
+template <class S, class A, int VR, bool QUEUE>
+__device__ __forceinline__ void phase2_item(const P2Item& it, bool valid, A* __restrict__ red,
+                                            void* const* __restrict__ ubase,
+                                            const A* __restrict__ t, double* __restrict__ ylev,
+                                            int64_t N, int accumulate) {
   const int lane = threadIdx.x & 63;
-  const bool valid = w < nitems;
-  P2Item it;
-  if (valid) it = items[w];
   int G = 1, rcn = 1;
   A acc[VR];
 #pragma unroll
@@ -579,21 +600,55 @@ __global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ 
       }
     }
   }
+  if constexpr (QUEUE) __builtin_amdgcn_wave_barrier();  // after the last item's reads
   if (G > 1) {
 #pragma unroll
-    for (int q = 0; q < VR; ++q) red[wv][lane * VR + q] = acc[q];
+    for (int q = 0; q < VR; ++q) red[lane * VR + q] = acc[q];
   }
-  __syncthreads();
+  if constexpr (QUEUE) __builtin_amdgcn_wave_barrier();  // a wave's LDS operations stay in order
+  else __syncthreads();
   if (!valid || multi || lane >= rcn) return;
 #pragma unroll
   for (int q = 0; q < VR; ++q) {
     const int row = lane * VR + q;
     if (row >= it.nr) break;
     A s = acc[q];
-    for (int g = 1; g < G; ++g) s += red[wv][(g * rcn + lane) * VR + q];
+    for (int g = 1; g < G; ++g) s += red[(g * rcn + lane) * VR + q];
     double* yp = ylev + (int64_t)(it.level - 1) * N + it.yoff + it.i0 + row;
     *yp = accumulate ? *yp + (double)s : (double)s;
   }
+}
+
+template <class S, class A, int VR>
+__global__ __launch_bounds__(256) void phase2_kernel(const P2Item* __restrict__ items, int nitems,
+                                                     void* const* __restrict__ ubase,
+                                                     const A* __restrict__ t,
+                                                     double* __restrict__ ylev, int64_t N,
+                                                     int accumulate) {
+  __shared__ A red[kWavesPerBlock][64 * VR];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = wave_id();
+  const bool valid = w < nitems;
+  P2Item it;
+  if (valid) it = items[w];
+  phase2_item<S, A, VR, false>(it, valid, red[wv], ubase, t, ylev, N, accumulate);
+}
+
+// The persistent form of phase 2 (as phase1_queue_kernel).
+template <class S, class A, int VR>
+__global__ __launch_bounds__(256) void phase2_queue_kernel(const P2Item* __restrict__ items,
+                                                           int nitems,
+                                                           void* const* __restrict__ ubase,
+                                                           const A* __restrict__ t,
+                                                           double* __restrict__ ylev, int64_t N,
+                                                           int accumulate,
+                                                           unsigned* __restrict__ queue) {
+  __shared__ A red[kWavesPerBlock][64 * VR];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // one item per take: phase 2 has fewer, larger items (largest first)
+  queue_loop<1>(queue, nitems, [&](int w) {
+    phase2_item<S, A, VR, true>(items[w], true, red[wv], ubase, t, ylev, N, accumulate);
+  });
 }
 
 
@@ -1423,45 +1478,85 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   H3D_LAUNCH_CHECK();
 }
 
-// Phase-1 launch over an item table with a load width of `bytes` per lane.
+// Workgroups of a persistent kernel: as many as stay resident on the device
+// at once (256 threads each), at most enough for `items` waves.
+template <class K>
+unsigned resident_grid(K kernel, int64_t items) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, unsigned>> known;
+  unsigned full = 0;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& k : known)
+      if (k.first == (const void*)kernel) full = k.second;
+    if (!full) {
+      int per_cu = 0, cus = 0, dev = 0;
+      H3D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+      H3D_HIP(hipGetDevice(&dev));
+      H3D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      full = (unsigned)std::max(1, per_cu * cus);
+      known.emplace_back((const void*)kernel, full);
+    }
+  }
+  return std::min(full, wave_grid(items));
+}
+
+// Phase-1 launch over an item table with a load width of `bytes` per lane;
+// with `queue`, the persistent kernel (its counter cleared first).
+template <class S, class A, int NV>
+void p1_launch(const P1Item* it, int n1, void* const* vbase, const BlockDev* blk, const A* x,
+               A* t, A* part, const double* scale, unsigned* queue, hipStream_t st) {
+  if (queue) {
+    queue_reset_kernel<<<1, 64, 0, st>>>(queue, 1);
+    auto k = phase1_queue_kernel<S, A, NV>;
+    k<<<resident_grid(k, n1), 256, 0, st>>>(it, n1, vbase, blk, x, t, part, scale, queue);
+  } else {
+    phase1_kernel<S, A, NV><<<wave_grid(n1), 256, 0, st>>>(it, n1, vbase, blk, x, t, part, scale);
+  }
+}
+
 template <class S, class A>
 void p1_raw(const P1Item* it, int n1, void* const* vbase, const BlockDev* blk, const A* x, A* t,
-            A* part, const double* scale, int bytes, hipStream_t st) {
+            A* part, const double* scale, int bytes, hipStream_t st, unsigned* queue = nullptr) {
   if (!n1) return;
-  const unsigned g = wave_grid(n1);
   constexpr int es = (int)sizeof(S);
   if (bytes >= 16 || 16 / es == 1 && bytes >= es)
-    phase1_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t, part, scale);
+    p1_launch<S, A, 16 / es>(it, n1, vbase, blk, x, t, part, scale, queue, st);
   else if (bytes == 8 && 8 / es >= 1)
-    phase1_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
-                                                                      part, scale);
+    p1_launch<S, A, (8 / es > 0 ? 8 / es : 1)>(it, n1, vbase, blk, x, t, part, scale, queue, st);
   else if (bytes == 4 && 4 / es >= 1)
-    phase1_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
-                                                                      part, scale);
+    p1_launch<S, A, (4 / es > 0 ? 4 / es : 1)>(it, n1, vbase, blk, x, t, part, scale, queue, st);
   else
-    phase1_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n1, vbase, blk, x, t,
-                                                                      part, scale);
+    p1_launch<S, A, (2 / es > 0 ? 2 / es : 1)>(it, n1, vbase, blk, x, t, part, scale, queue, st);
   H3D_LAUNCH_CHECK();
+}
+
+template <class S, class A, int NV>
+void p2_launch(const P2Item* it, int n2, void* const* ubase, const A* t, double* ylev, int64_t N,
+               int acc, unsigned* queue, hipStream_t st) {
+  if (queue) {
+    queue_reset_kernel<<<1, 64, 0, st>>>(queue, 1);
+    auto k = phase2_queue_kernel<S, A, NV>;
+    k<<<resident_grid(k, n2), 256, 0, st>>>(it, n2, ubase, t, ylev, N, acc, queue);
+  } else {
+    phase2_kernel<S, A, NV><<<wave_grid(n2), 256, 0, st>>>(it, n2, ubase, t, ylev, N, acc);
+  }
 }
 
 // Phase-2 launch over an item table with a load width of `bytes` per lane.
 template <class S, class A>
 void p2_raw(const P2Item* it, int n2, void* const* ubase, const A* t, double* ylev, int64_t N,
-            int acc, int bytes, hipStream_t st) {
+            int acc, int bytes, hipStream_t st, unsigned* queue = nullptr) {
   if (!n2) return;
-  const unsigned g = wave_grid(n2);
   constexpr int es = (int)sizeof(S);
   if (bytes >= 16)
-    phase2_kernel<S, A, 16 / es><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N, acc);
+    p2_launch<S, A, 16 / es>(it, n2, ubase, t, ylev, N, acc, queue, st);
   else if (bytes == 8)
-    phase2_kernel<S, A, (8 / es > 0 ? 8 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
-                                                                      acc);
+    p2_launch<S, A, (8 / es > 0 ? 8 / es : 1)>(it, n2, ubase, t, ylev, N, acc, queue, st);
   else if (bytes == 4)
-    phase2_kernel<S, A, (4 / es > 0 ? 4 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
-                                                                      acc);
+    p2_launch<S, A, (4 / es > 0 ? 4 / es : 1)>(it, n2, ubase, t, ylev, N, acc, queue, st);
   else
-    phase2_kernel<S, A, (2 / es > 0 ? 2 / es : 1)><<<g, 256, 0, st>>>(it, n2, ubase, t, ylev, N,
-                                                                      acc);
+    p2_launch<S, A, (2 / es > 0 ? 2 / es : 1)>(it, n2, ubase, t, ylev, N, acc, queue, st);
   H3D_LAUNCH_CHECK();
 }
 
@@ -1666,15 +1761,20 @@ void far_zero(double* p, int64_t n, hipStream_t st) {
 
 void far_phase1(int prec, bool acc64, int bytes, const P1Item* items, int n, void* const* vbase,
                 const BlockDev* blk, const double* xp, const float* xf, double* t, double* part,
-                hipStream_t st) {
+                hipStream_t st, unsigned* queue) {
   if (acc64) {
-    if (prec == H3D_PREC_FP64) p1_raw<double, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
-    else if (prec == H3D_PREC_FP32) p1_raw<float, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
-    else p1_raw<uint16_t, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st);
+    if (prec == H3D_PREC_FP64)
+      p1_raw<double, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st, queue);
+    else if (prec == H3D_PREC_FP32)
+      p1_raw<float, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st, queue);
+    else
+      p1_raw<uint16_t, double>(items, n, vbase, blk, xp, t, part, nullptr, bytes, st, queue);
   } else if (prec == H3D_PREC_FP32) {
-    p1_raw<float, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st);
+    p1_raw<float, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st,
+                         queue);
   } else {
-    p1_raw<uint16_t, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st);
+    p1_raw<uint16_t, float>(items, n, vbase, blk, xf, (float*)t, (float*)part, nullptr, bytes, st,
+                            queue);
   }
 }
 
@@ -1689,15 +1789,18 @@ void far_reduce(bool acc64, const P1Reduce* rs, int n, const double* part, doubl
 }
 
 void far_phase2(int prec, bool acc64, int bytes, const P2Item* items, int n, void* const* ubase,
-                const double* t, double* ylev, int64_t N, hipStream_t st) {
+                const double* t, double* ylev, int64_t N, hipStream_t st, unsigned* queue) {
   if (acc64) {
-    if (prec == H3D_PREC_FP64) p2_raw<double, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
-    else if (prec == H3D_PREC_FP32) p2_raw<float, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
-    else p2_raw<uint16_t, double>(items, n, ubase, t, ylev, N, 0, bytes, st);
+    if (prec == H3D_PREC_FP64)
+      p2_raw<double, double>(items, n, ubase, t, ylev, N, 0, bytes, st, queue);
+    else if (prec == H3D_PREC_FP32)
+      p2_raw<float, double>(items, n, ubase, t, ylev, N, 0, bytes, st, queue);
+    else
+      p2_raw<uint16_t, double>(items, n, ubase, t, ylev, N, 0, bytes, st, queue);
   } else if (prec == H3D_PREC_FP32) {
-    p2_raw<float, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st);
+    p2_raw<float, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st, queue);
   } else {
-    p2_raw<uint16_t, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st);
+    p2_raw<uint16_t, float>(items, n, ubase, (const float*)t, ylev, N, 0, bytes, st, queue);
   }
 }
 

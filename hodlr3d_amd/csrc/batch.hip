@@ -72,7 +72,7 @@ void Batch::free_all() {
                    (void**)&d_lstart, (void**)&d_lcount, (void**)&d_px, (void**)&d_py,
                    (void**)&d_pz, (void**)&d_lx, (void**)&d_ly, (void**)&d_lz, (void**)&d_leafc,
                    (void**)&d_xp, (void**)&d_xf, (void**)&d_t, (void**)&d_part, (void**)&d_ylev,
-                   (void**)&d_ynear})
+                   (void**)&d_ynear, (void**)&d_queue})
     dfree(*p);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
@@ -269,6 +269,7 @@ void build_batch(Batch& bt, Engine* const* es, int count, int mode) {
   bt.d_part = dalloc<double>(std::max<int64_t>(bt.npart, 1));
   bt.d_ylev = dalloc<double>(std::max<int64_t>(bt.N * bt.nL, 1));
   bt.d_ynear = dalloc<double>(std::max<int64_t>(bt.N, 1));
+  bt.d_queue = dalloc<unsigned>(2);
   for (int i = 0; i < C; ++i) bt.factor_bytes += es[i]->fac_bytes[prec];
   H3D_HIP(hipStreamSynchronize(st));
 }
@@ -304,10 +305,12 @@ void batch_product(Batch& bt, const double* b, double* y, hipStream_t st, hipStr
     far_near(nt, bt.prec, acc64, false, side);
     H3D_HIP(hipEventRecord(join, side));
   }
-  far_phase1(bt.prec, acc64, bt.tune_p1, bt.d_p1, bt.n1, bt.d_vbase, bt.d_blk, bt.d_xp, bt.d_xf,
-             bt.d_t, bt.d_part, st);
+  far_phase1(bt.prec, acc64, bt.tune_p1 & 0xff, bt.d_p1, bt.n1, bt.d_vbase, bt.d_blk, bt.d_xp,
+             bt.d_xf, bt.d_t, bt.d_part, st,
+             (bt.tune_p1 & H3D_PERSISTENT) ? bt.d_queue : nullptr);
   far_reduce(acc64, bt.d_p1r, bt.n1r, bt.d_part, bt.d_t, st);
-  far_phase2(bt.prec, acc64, bt.tune_p2, bt.d_p2, bt.n2, bt.d_ubase, bt.d_t, bt.d_ylev, bt.N, st);
+  far_phase2(bt.prec, acc64, bt.tune_p2 & 0xff, bt.d_p2, bt.n2, bt.d_ubase, bt.d_t, bt.d_ylev,
+             bt.N, st, (bt.tune_p2 & H3D_PERSISTENT) ? bt.d_queue + 1 : nullptr);
   if (overlap) H3D_HIP(hipStreamWaitEvent(st, join, 0));
   else far_near(nt, bt.prec, acc64, false, st);
   bcombine_kernel<<<(unsigned)((bt.N + 255) / 256), 256, 0, st>>>(
@@ -331,32 +334,38 @@ void batch_autotune(Batch& bt, int reps, double* ms) {
     H3D_HIP(hipEventCreate(&z));
     // timing does not depend on the values: x is whatever the last product left
     for (int which = 1; which <= 2; ++which) {
+      auto run = [&](int w, bool persist) {
+        if (which == 1)
+          far_phase1(bt.prec, bt.acc64, w, bt.d_p1, bt.n1, bt.d_vbase, bt.d_blk, bt.d_xp, bt.d_xf,
+                     bt.d_t, bt.d_part, st, persist ? bt.d_queue : nullptr);
+        else
+          far_phase2(bt.prec, bt.acc64, w, bt.d_p2, bt.n2, bt.d_ubase, bt.d_t, bt.d_ylev, bt.N, st,
+                     persist ? bt.d_queue + 1 : nullptr);
+      };
+      auto time = [&](int w, bool persist) {
+        run(w, persist);  // untimed
+        H3D_HIP(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r) run(w, persist);
+        H3D_HIP(hipEventRecord(z, st));
+        H3D_HIP(hipEventSynchronize(z));
+        float t = 0;
+        H3D_HIP(hipEventElapsedTime(&t, a, z));
+        return (double)t / reps;
+      };
       double best = 1e300;
       int bw = 16;
       for (int q = 0; q < 4; ++q) {
         const int w = widths[q];
         double v = -1;
         if (w >= es) {
-          auto run = [&] {
-            if (which == 1)
-              far_phase1(bt.prec, bt.acc64, w, bt.d_p1, bt.n1, bt.d_vbase, bt.d_blk, bt.d_xp,
-                         bt.d_xf, bt.d_t, bt.d_part, st);
-            else
-              far_phase2(bt.prec, bt.acc64, w, bt.d_p2, bt.n2, bt.d_ubase, bt.d_t, bt.d_ylev,
-                         bt.N, st);
-          };
-          run();  // untimed
-          H3D_HIP(hipEventRecord(a, st));
-          for (int r = 0; r < reps; ++r) run();
-          H3D_HIP(hipEventRecord(z, st));
-          H3D_HIP(hipEventSynchronize(z));
-          float t = 0;
-          H3D_HIP(hipEventElapsedTime(&t, a, z));
-          v = t / reps;
+          v = time(w, false);
           if (v < best) best = v, bw = w;
         }
         if (ms) ms[(which - 1) * 4 + q] = v;
       }
+      const double vp = time(bw, true);
+      if (ms) ms[8 + which - 1] = vp;
+      if (vp < best) bw |= H3D_PERSISTENT;
       (which == 1 ? bt.tune_p1 : bt.tune_p2) = bw;
     }
   } catch (...) {

@@ -45,7 +45,9 @@ struct Batch {
   double* d_ynear = nullptr;
   hipStream_t own = nullptr, stream = nullptr, side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int tune_p1 = 16, tune_p2 = 16;  // load width in bytes per lane
+  // load width in bytes per lane, | H3D_PERSISTENT for the persistent launch
+  int tune_p1 = 16, tune_p2 = 16;
+  unsigned* d_queue = nullptr;     // the persistent launches' item counters (phase 1, phase 2)
 
   ~Batch();
   void free_all();
@@ -60,8 +62,9 @@ void build_batch(Batch& bt, Engine* const* es, int count, int mode);
 void batch_product(Batch& bt, const double* b, double* y, hipStream_t st, hipStream_t side,
                    hipEvent_t fork, hipEvent_t join);
 void batch_matvec(Batch& bt, const double* b, double* y);
-// Times every load width of both phases over the merged tables and keeps the
-// fastest; ms[8] as h3d_autotune's.
+// Times every load width of both phases over the merged tables, then the
+// persistent launch at the fastest width, and keeps the fastest; ms[10]: as
+// h3d_autotune's, then phase 1 and phase 2 persistent.
 void batch_autotune(Batch& bt, int reps, double* ms);
 
 }  // namespace h3d

@@ -984,9 +984,12 @@ int h3d_batch_set_tuning(h3d_batch* bt, int p1_bytes, int p2_bytes) {
   return guarded([&] {
     if (!bt) throw h3d::Error(H3D_ERR_INVALID, "null batch");
     const int es = h3d::elem_size(bt->b.prec);
-    for (int w : {p1_bytes, p2_bytes})
+    for (int v : {p1_bytes, p2_bytes}) {
+      const int w = v & ~H3D_PERSISTENT;
       if ((w != 2 && w != 4 && w != 8 && w != 16) || w < es)
-        throw h3d::Error(H3D_ERR_INVALID, "load width must be 2, 4, 8 or 16 bytes and hold an element");
+        throw h3d::Error(H3D_ERR_INVALID,
+                         "load width must be 2, 4, 8 or 16 bytes (| H3D_PERSISTENT) and hold an element");
+    }
     bt->b.tune_p1 = p1_bytes;
     bt->b.tune_p2 = p2_bytes;
   });

@@ -420,13 +420,16 @@ NearTables near_tables(Engine& e);
 void far_gather(const double* b, const int32_t* perm, int64_t N, double* xp, float* xf,
                 hipStream_t st);
 void far_zero(double* p, int64_t n, hipStream_t st);
+// queue: persistent launch (a resident grid taking items from this counter;
+// null: one wave per item)
 void far_phase1(int prec, bool acc64, int bytes, const P1Item* items, int n, void* const* vbase,
                 const BlockDev* blk, const double* xp, const float* xf, double* t, double* part,
-                hipStream_t st);
+                hipStream_t st, unsigned* queue = nullptr);
 void far_reduce(bool acc64, const P1Reduce* rs, int n, const double* part, double* t,
                 hipStream_t st);
 void far_phase2(int prec, bool acc64, int bytes, const P2Item* items, int n, void* const* ubase,
-                const double* t, double* ylev, int64_t N, hipStream_t st);
+                const double* t, double* ylev, int64_t N, hipStream_t st,
+                unsigned* queue = nullptr);
 void far_near(const NearTables& nt, int prec, bool acc64, bool transpose, hipStream_t st);
 
 // tree.hip

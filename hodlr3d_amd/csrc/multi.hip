@@ -279,19 +279,35 @@ __global__ __launch_bounds__(256, kWavesFor<A>) void p1r_kernel(const GTItem* __
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-          for (int h = 0; h < H; ++h) acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
+          for (int h = 0; h < H; ++h) {
+            // fp32: the transposed tile (A = x, B = V), so that a lane holds
+            // four consecutive RHS of one column and stores them as one
+            // 16-byte vector (four stores per tile instead of sixteen)
+            if constexpr (sizeof(A) == 4) acc[ct][h] = M::mac(xb[h][e], av[st][ct][e], acc[ct][h]);
+            else acc[ct][h] = M::mac(av[st][ct][e], xb[h][e], acc[ct][h]);
+          }
     }
   };
   auto store = [&](int i) {
     const int k = wv + 4 * i;
+    if constexpr (sizeof(A) == 4) {
+      // element a of acc[ct][h]: RHS 16 h + 4 kq + a of the tile's column 16 ct + lc
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
+      for (int ct = 0; ct < CT; ++ct) {
+        A* dst = tw + (int64_t)lslot[k * kTileCols + 16 * ct + lc] * W + 4 * kq;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        A* dst = tw + (int64_t)lslot[k * kTileCols + 16 * ct + M::row(kq, a)] * W;
-#pragma unroll
-        for (int h = 0; h < H; ++h) dst[16 * h + lc] = acc[ct][h][a];
+        for (int h = 0; h < H; ++h) *(Acc*)(dst + 16 * h) = acc[ct][h];
       }
+    } else {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          A* dst = tw + (int64_t)lslot[k * kTileCols + 16 * ct + M::row(kq, a)] * W;
+#pragma unroll
+          for (int h = 0; h < H; ++h) dst[16 * h + lc] = acc[ct][h][a];
+        }
+    }
   };
   A av0[4][CT][4], av1[4][CT][4];
   if constexpr (NC == 1) {

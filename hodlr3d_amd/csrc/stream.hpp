@@ -29,6 +29,39 @@ __device__ inline T ntload(const void* p) {
   return __builtin_nontemporal_load((const __attribute__((address_space(1))) T*)p);
 }
 
+// A persistent kernel's queue: an atomic counter of items taken.  A wave
+// takes GRAB consecutive items per add (one add by lane 0: with many small
+// items, one counter hit once per item by every wave of the device would
+// serialize), and asks for its next group before working on the current one,
+// so the add's latency is hidden.  queue_take returns lane 0's raw result;
+// queue_first makes it wave-uniform where it is needed.
+template <int GRAB>
+__device__ inline unsigned queue_take(unsigned* queue) {
+  unsigned v = 0;
+  if ((threadIdx.x & 63) == 0) v = atomicAdd(queue, (unsigned)GRAB);
+  return v;
+}
+__device__ inline int queue_first(unsigned v) { return (int)__builtin_amdgcn_readfirstlane(v); }
+
+// Runs body(w) for every item w < nitems that the calling wave takes from the
+// queue; every wave returns once the counter has passed nitems.
+template <int GRAB, class F>
+__device__ inline void queue_loop(unsigned* queue, int nitems, F&& body) {
+  int w0 = queue_first(queue_take<GRAB>(queue));
+  while (w0 < nitems) {
+    const unsigned next = queue_take<GRAB>(queue);
+    const int w1 = min(w0 + GRAB, nitems);
+    for (int w = w0; w < w1; ++w) body(w);
+    w0 = queue_first(next);
+  }
+}
+
+// Clears n queue counters (a kernel, not a memset node, so that it may be
+// captured into a graph).
+__global__ void queue_reset_kernel(unsigned* __restrict__ q, int n) {
+  if ((int)threadIdx.x < n) q[threadIdx.x] = 0u;
+}
+
 // Zero fill (used instead of a memset node, so that a matvec may be captured
 // into a HIP graph).
 __global__ void zero_kernel(double* __restrict__ p, int64_t n) {

@@ -547,6 +547,9 @@ class MatvecGraph:
             pass
 
 
+PERSISTENT = 0x100  # h3d_batch_set_tuning flag: the persistent (work-queue) launch
+
+
 class Batch:
     """One product of each of many assembled handles in one launch per step for
     all of them (h3d_batch_create: the instances' work tables merged, factors
@@ -604,15 +607,18 @@ class Batch:
         return y
 
     def autotune(self, reps=3):
-        """Fastest load width per phase over the merged tables; ms per width."""
+        """Fastest load width per phase over the merged tables, then the
+        persistent launch at that width; ms per width and 'persistent' ms."""
         self._bind()
-        ms = (ctypes.c_double * 8)(*([-1.0] * 8))
+        ms = (ctypes.c_double * 10)(*([-1.0] * 10))
         nat.check(nat.lib().h3d_batch_autotune(self._b, int(reps), ms))
         w = [16, 8, 4, 2]
         return {"phase1": {w[i]: ms[i] for i in range(4) if ms[i] >= 0},
-                "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0}}
+                "phase2": {w[i]: ms[4 + i] for i in range(4) if ms[4 + i] >= 0},
+                "phase1_persistent": ms[8], "phase2_persistent": ms[9]}
 
     def get_tuning(self):
+        """(phase-1, phase-2) load bytes per lane, | PERSISTENT for the persistent launch."""
         a, b = ctypes.c_int(), ctypes.c_int()
         nat.check(nat.lib().h3d_batch_get_tuning(self._b, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value

@@ -383,12 +383,19 @@ void h3d_graph_destroy(h3d_graph* g);
  * stream; the new stream first waits for the old one's work); products and
  * graph replays of one batch share its scratch, so the caller keeps them in
  * order.
+ * Each far-field phase launches either one wave per item or persistently
+ * (H3D_PERSISTENT): a grid of as many workgroups as stay resident, whose
+ * waves take items from a work queue (an atomic counter, cleared by a kernel
+ * before the launch) until every item of every instance is taken.
  * h3d_batch_autotune times the load widths (2..16 bytes per lane) of both
- * phases over the merged tables and keeps the fastest (ms[8] as
- * h3d_autotune's); h3d_batch_set_tuning / get_tuning pin or read them.
+ * phases over the merged tables, then the persistent launch at the fastest
+ * width, and keeps the fastest (ms[10]: ms[0..7] as h3d_autotune's, ms[8],
+ * ms[9] phase 1 / phase 2 persistent); h3d_batch_set_tuning / get_tuning pin
+ * or read the choice (load bytes | H3D_PERSISTENT).
  * h3d_batch_graph_create captures one product b -> y (device pointers baked
  * in) into a graph replayed by h3d_graph_launch and freed by
  * h3d_graph_destroy. */
+#define H3D_PERSISTENT 0x100
 typedef struct h3d_batch h3d_batch;
 int h3d_batch_create(h3d_handle* const* hs, int count, int mode, h3d_batch** out);
 int h3d_batch_layout(h3d_batch* bt, int64_t* offsets);

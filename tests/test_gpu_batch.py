@@ -104,6 +104,34 @@ def test_graph_replays_equal_eager_batch_products(instances):
     bt.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_persistent_launch_equals_one_wave_per_item(instances, prec):
+    """The work-queue launch takes the same items in another order: every
+    item's sums are its own, so the product is the same bit for bit, on
+    repeated launches (the queue counters are cleared before each) and in a
+    graph replay."""
+    import torch
+    import hodlr3d_amd as H3
+    hs = instances
+    bt = H3.Batch(hs, prec=prec)
+    b = packed_rhs(bt, 9)
+    ref = bt.matvec(b)
+    for p1, p2 in ((H3.PERSISTENT, 0), (0, H3.PERSISTENT), (H3.PERSISTENT, H3.PERSISTENT)):
+        bt.set_tuning(16 | p1, 16 | p2)
+        assert bt.get_tuning() == (16 | p1, 16 | p2)
+        for rep in range(3):
+            y = bt.matvec(b)
+            torch.cuda.synchronize()
+            assert torch.equal(y, ref), (prec, p1, p2, rep)
+    yg = torch.empty_like(b)
+    launch = bt.graph(b, yg)
+    for rep in range(3):
+        launch()
+        torch.cuda.synchronize()
+        assert torch.equal(yg, ref), (prec, rep)
+    bt.close()
+
+
 def test_autotune_keeps_the_product(instances):
     import torch
     import hodlr3d_amd as H3
@@ -111,9 +139,10 @@ def test_autotune_keeps_the_product(instances):
     bt = H3.Batch(hs, prec="fp32")
     ms = bt.autotune(reps=2)
     assert set(ms["phase1"]) == {16, 8, 4} and set(ms["phase2"]) == {16, 8, 4}
+    assert ms["phase1_persistent"] > 0 and ms["phase2_persistent"] > 0
     w1, w2 = bt.get_tuning()
     for h in hs:
-        h.set_tuning("fp32", w1, w2)
+        h.set_tuning("fp32", w1 & ~H3.PERSISTENT, w2 & ~H3.PERSISTENT)
     b = packed_rhs(bt, 7)
     y = bt.matvec(b)
     for h, bi, yi in zip(hs, bt.unpack(b), bt.unpack(y)):
