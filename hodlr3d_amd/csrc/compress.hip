@@ -392,6 +392,203 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
   }
 }
 
+// The same eigen step for ranks up to G (16 or 32) with every matrix in
+// registers: one wave holds 64 / G blocks, one per group of G lanes, and lane
+// c of a group holds column c of the group's matrices (rows beyond r, and the
+// columns c >= r, are zero).  The Choleskys broadcast the pivot column k with
+// shuffles inside the group; M = Lu^T Lv reads Lu from the group's LDS slice,
+// which then holds Lv for the projections of the second pass.  The Jacobi
+// rounds pair column c with column c ^ k for k = 1 .. G - 1, which for a power
+// of two G meets every pair once per sweep; both lanes of a pair form the same
+// dot products from the same values in the same order, so they agree on the
+// rotation, and each updates its own column.  A sweep with no rotation leaves
+// the columns unchanged, so a group whose block has converged sweeps on
+// without effect while the other groups of its wave finish.  VEC = false
+// computes the values only; its rotations of M are those of VEC = true bit for
+// bit.  The workgroup is one wave.
+template <int G, bool VEC>
+__global__ __launch_bounds__(64, 4) void eig_wave_kernel(const CDesc* __restrict__ ds,
+                                                      const int32_t* __restrict__ list,
+                                                      int nlist, double* __restrict__ ws) {
+  constexpr int NG = 64 / G;
+  __shared__ double slice[NG][G * G];  // per group, column-major: Lu, then Lv
+  const int lane = threadIdx.x;
+  const int grp = lane / G, c = lane % G, gb = grp * G;
+  const int item = blockIdx.x * NG + grp;
+  const bool valid = item < nlist;
+  CDesc d{};
+  if (valid) d = ds[list[item]];
+  const int r = valid ? d.r : 0;
+  double* W = ws + d.w;
+  double* L = slice[grp];
+
+  // column c of the symmetric r x r matrix at W + off (zero outside it;
+  // loads at clamped in-bounds indices, then masked)
+  const int cc = c < r ? c : (r > 0 ? r - 1 : 0);
+  auto load_sym = [&](int64_t off, double (&a)[G]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int ic = i < r ? i : (r > 0 ? r - 1 : 0);
+      const double x = W[off + (int64_t)cc * r + ic];
+      a[i] = (i < r && c < r) ? x : 0.0;
+    }
+  };
+  // in-place Cholesky (lower triangle, column c in lane c); true when a pivot
+  // falls below kCholFail times its original diagonal (the group agrees)
+  auto chol = [&](double (&a)[G]) __attribute__((always_inline)) {
+    double dg = 0.0;
+#pragma unroll
+    for (int i = 0; i < G; ++i) dg = i == c ? a[i] : dg;
+    bool fail = false;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const double dkk = __shfl(a[k], gb + k);
+      const double dgk = __shfl(dg, gb + k);
+      if (k < r && !(dkk > kCholFail * dgk)) fail = true;
+      const bool act = k < r && !fail;
+      const double lkk = act ? sqrt(dkk) : 1.0;
+      double lk[G];
+      double lck = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < G; ++i) {
+        lk[i] = __shfl(a[i], gb + k) / lkk;
+        lck = i == c ? lk[i] : lck;
+      }
+      if (act) {
+        if (c == k) {
+          a[k] = lkk;
+#pragma unroll
+          for (int i = k + 1; i < G; ++i) a[i] = lk[i];
+        } else if (c > k) {
+#pragma unroll
+          for (int i = k + 1; i < G; ++i)
+            if (i >= c) a[i] -= lk[i] * lck;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) a[i] = i < c ? 0.0 : a[i];
+    return fail;
+  };
+
+  bool raw;
+  {
+    double u[G];
+    load_sym(0, u);              // Gu
+    raw = chol(u);               // Lu, to the group's slice
+#pragma unroll
+    for (int i = 0; i < G; ++i) L[i + c * G] = u[i];
+  }
+  double v[G], m[G];
+  load_sym((int64_t)r * r, v);  // Gv
+  raw = chol(v) || raw;          // Lv
+  __syncthreads();
+  // M[i][c] = sum_k Lu[k][i] Lv[k][c] (k increasing; the zeros of the
+  // triangles add nothing)
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) s += L[k + i * G] * v[k];
+    m[i] = raw ? 0.0 : s;
+  }
+  __syncthreads();
+  double j[VEC ? G : 1];
+  if constexpr (VEC) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) L[i + c * G] = v[i];  // Lv, for the projections
+#pragma unroll
+    for (int i = 0; i < G; ++i) j[i] = i == c ? 1.0 : 0.0;
+  }
+  // one-sided Jacobi on the columns of M
+  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+    bool rot = false;
+    for (int k = 1; k < G; ++k) {
+      const int p = c ^ k;
+      const bool lower = c < p;
+      double so = 0.0, sp = 0.0, g = 0.0;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const double y = __shfl_xor(m[i], k);
+        so += m[i] * m[i];
+        sp += y * y;
+        g += m[i] * y;
+      }
+      const double a = lower ? so : sp, b = lower ? sp : so;
+      double cs = 1.0, sn = 0.0;
+      if ((c > p ? c : p) < r && g != 0.0 && fabs(g) > 1e-15 * sqrt(a * b)) {
+        const double th = (b - a) / (2.0 * g);
+        const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+        cs = 1.0 / sqrt(1.0 + tt * tt);
+        sn = tt * cs;
+        rot = true;
+      }
+      // (x_lo, x_hi) <- (c x_lo - s x_hi, s x_lo + c x_hi); the shuffle reads
+      // the partner's element before either lane updates it
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const double y = __shfl_xor(m[i], k);
+        m[i] = lower ? cs * m[i] - sn * y : sn * y + cs * m[i];
+      }
+      if constexpr (VEC) {
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          const double y = __shfl_xor(j[i], k);
+          j[i] = lower ? cs * j[i] - sn * y : sn * y + cs * j[i];
+        }
+      }
+    }
+    if (!__any(rot)) break;
+  }
+  // squared singular values = squared column norms of M J, sorted descending
+  double lam = 0.0;
+#pragma unroll
+  for (int i = 0; i < G; ++i) lam += m[i] * m[i];
+  int rank = 0;
+  double tr = 0.0;
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const double lq = __shfl(lam, gb + q);
+    if (q < r) {
+      rank += (lq > lam) || (lq == lam && q < c);
+      tr += lq;
+    }
+  }
+  if (!valid) return;
+  double* out_lam = W + 2 * (int64_t)r * r;
+  if (raw) {
+    if (c == 0) out_lam[r] = 1.0;
+    return;
+  }
+  if (c < r) out_lam[rank] = lam;
+  if (c == 0) {
+    out_lam[r] = 0.0;
+    out_lam[r + 1] = 4.0 * r * 2.220446049250313e-16 * tr;
+  }
+  if constexpr (VEC) {
+    __builtin_amdgcn_wave_barrier();
+    // Tu[:, rank] = Lv J[:, c], Tv[:, rank] = Lv^-T J[:, c] (column-major, ld r)
+    double* tu = W + (int64_t)rank * r;
+    double* tv = W + (int64_t)r * r + (int64_t)rank * r;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q <= i; ++q) s += L[i + q * G] * j[q];
+      if (c < r && i < r) tu[i] = s;
+    }
+    double t[G];
+#pragma unroll
+    for (int i = G - 1; i >= 0; --i) {
+      double s = j[i];
+#pragma unroll
+      for (int q = i + 1; q < G; ++q) s -= L[q + i * G] * t[q];
+      t[i] = i < r ? s / L[i + i * G] : 0.0;
+      if (c < r && i < r) tv[i] = t[i];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ project
 // Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
 // and stored in the column's class (k < r32: fp32, else fp16).  Raw blocks
@@ -693,21 +890,23 @@ void build_mixed(Engine& e, double tol) {
     launch_g(g4, 0);
     launch_g(g16, 1);
     launch_g(g40, 2);
-    // eig: small ranks one wave per block; larger ranks four waves, launched in
-    // classes of LDS footprint (each launch sizes its LDS by its largest rank,
-    // and that sets how many workgroups share a CU); ranks beyond rmax with
-    // their matrices in global scratch
-    constexpr int kEigClasses = 6;  // 0: one wave; 1-4: LDS up to the caps; 5: global
+    // eig: ranks up to 16 / 32 in registers (eig_wave_kernel, 4 / 2 blocks per
+    // wave); larger ranks four waves per block, launched in classes of LDS
+    // footprint (each launch sizes its LDS by its largest rank, and that sets
+    // how many workgroups share a CU); ranks beyond rmax with their matrices
+    // in global scratch
+    constexpr int kEigClasses = 7;  // 0, 1: registers; 2-5: LDS up to the caps; 6: global
     const size_t caps[4] = {40 << 10, 64 << 10, 96 << 10, (size_t)lds_max};
     std::vector<int32_t> grp[kEigClasses];
     for (int32_t li : eg) {
       const int r = cd[i0 + li].r;
-      int q = 5;
-      if (r <= 24) q = 0;
+      int q = 6;
+      if (r <= 16) q = 0;
+      else if (r <= 32) q = 1;
       else if (r <= rmax)
         for (int c = 0; c < 4; ++c)
           if (eig_lds(r, false) <= caps[c]) {
-            q = 1 + c;
+            q = 2 + c;
             break;
           }
       grp[q].push_back(li);
@@ -725,8 +924,14 @@ void build_mixed(Engine& e, double tol) {
       double* gs = glob ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
       for (int64_t s = 0; s < (int64_t)g.size(); s += per) {
         const int cnt = (int)std::min<int64_t>(per, (int64_t)g.size() - s);
-        if (!glob)
-          (vectors ? eig_kernel<false, true> : eig_kernel<false, false>)<<<cnt, q ? 256 : 64,
+        if (q == 0)
+          (vectors ? eig_wave_kernel<16, true> : eig_wave_kernel<16, false>)<<<(cnt + 3) / 4, 64, 0, st>>>(
+              d_cd, d_g + s, cnt, d_ws);
+        else if (q == 1)
+          (vectors ? eig_wave_kernel<32, true> : eig_wave_kernel<32, false>)<<<(cnt + 1) / 2, 64, 0, st>>>(
+              d_cd, d_g + s, cnt, d_ws);
+        else if (!glob)
+          (vectors ? eig_kernel<false, true> : eig_kernel<false, false>)<<<cnt, 256,
                                                                          eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws,
                                                                            nullptr, 0);
         else
