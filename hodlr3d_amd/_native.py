@@ -56,7 +56,8 @@ class MixedStats(ctypes.Structure):
                 ("err_fp32", _d), ("fro_far", _d), ("fro_near", _d),
                 ("aca_rank_sum", _i64), ("rank_fp32", _i64), ("rank_fp16", _i64),
                 ("dropped", _i64), ("raw_blocks", _i64), ("raw_rank", _i64), ("bytes", _i64),
-                ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d)]
+                ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d),
+                ("single_jacobi", _i32)]
 
 
 class Config(ctypes.Structure):

@@ -442,6 +442,7 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
     s->v_elems[0] = m.cls[0].v_elems;
     s->v_elems[1] = m.cls[1].v_elems;
     s->build_ms = m.ms;
+    s->single_jacobi = m.single_jacobi ? 1 : 0;
   });
 }
 

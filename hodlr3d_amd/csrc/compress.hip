@@ -58,6 +58,8 @@ struct CDesc {
   int64_t xu;     // fp64 element offset of U_b's first column in its chunk
   int64_t xv;     // fp64 element offset of V_b in its chunk
   int64_t w;      // workspace offset: Gu/Tu (r*r), Gv/Tv (r*r), lam (r), status, floor
+  int64_t j;      // offset of the block's stored J (sorted columns, r*r), -1: none
+  int64_t lamo;   // offset of the block's lam in the batch's lam array (second pass)
   int32_t n, m, r, ldu, ldv, chunk;
 };
 
@@ -102,16 +104,17 @@ void host_chunks(int64_t n, F f) {
 // block X, row tiles of 32 staged in LDS, each thread owning up to EMAX of
 // the r(r+1)/2 entries (l <= l'); the tile sums are accumulated with Kahan
 // compensation, so long columns add no rounding beyond a 32-term sum.
+// sides = 2: both (workgroup 2 i + side); sides = 1: the V side only.
 template <int EMAX, class S>
 __global__ __launch_bounds__(256) void gram_kernel(const CDesc* __restrict__ ds,
                                                    void* const* __restrict__ base64,
                                                    const int32_t* __restrict__ list, int nlist,
-                                                   double* __restrict__ ws) {
+                                                   double* __restrict__ ws, int sides) {
   constexpr int TR = 32;
   extern __shared__ double tile[];  // r x (TR + 1)
-  const int item = blockIdx.x >> 1;
+  const int item = sides == 2 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
   if (item >= nlist) return;
-  const int side = blockIdx.x & 1;
+  const int side = sides == 2 ? (int)(blockIdx.x & 1) : 1;
   const CDesc d = ds[list[item]];
   const int r = d.r;
   const int rows = side ? d.m : d.n;
@@ -219,13 +222,14 @@ __device__ inline void chol_inplace(double* A, int r, int ld, double* dg, int* f
 
 // VEC = false (the first pass, which needs the singular values only): J is not
 // accumulated and no projections are formed; M's rotations, and so the values,
-// are the same bit for bit as with VEC = true.
-template <bool GLOBAL, bool VEC>
+// are the same bit for bit as with VEC = true.  STOREJ (with VEC): the sorted
+// columns of J go to jbuf + d.j instead of the projections, for finish_kernel.
+template <bool GLOBAL, bool VEC, bool STOREJ = false>
 __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
                                                   const int32_t* __restrict__ list, int nlist,
                                                   double* __restrict__ ws,
                                                   double* __restrict__ gscratch,
-                                                  int64_t gstride) {
+                                                  int64_t gstride, double* __restrict__ jbuf) {
   extern __shared__ double sm[];
   const int item = blockIdx.x;
   if (item >= nlist) return;
@@ -373,6 +377,11 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
     const int c = perm[k];
     out_lam[k] = lam[c];
     if (!VEC) continue;
+    if (STOREJ) {
+      double* jo = jbuf + d.j + (int64_t)k * r;
+      for (int i = 0; i < r; ++i) jo[i] = A[at(i, c)];
+      continue;
+    }
     double* tu = W + (int64_t)k * r;
     double* tv = W + (int64_t)r * r + (int64_t)k * r;
     for (int i = 0; i < r; ++i) {
@@ -406,10 +415,11 @@ __global__ __launch_bounds__(256) void eig_kernel(const CDesc* __restrict__ ds,
 // without effect while the other groups of its wave finish.  VEC = false
 // computes the values only; its rotations of M are those of VEC = true bit for
 // bit.  The workgroup is one wave.
-template <int G, bool VEC>
+template <int G, bool VEC, bool STOREJ = false>
 __global__ __launch_bounds__(64, 4) void eig_wave_kernel(const CDesc* __restrict__ ds,
                                                       const int32_t* __restrict__ list,
-                                                      int nlist, double* __restrict__ ws) {
+                                                      int nlist, double* __restrict__ ws,
+                                                      double* __restrict__ jbuf) {
   constexpr int NG = 64 / G;
   __shared__ double slice[NG][G * G];  // per group, column-major: Lu, then Lv
   const int lane = threadIdx.x;
@@ -565,7 +575,12 @@ __global__ __launch_bounds__(64, 4) void eig_wave_kernel(const CDesc* __restrict
     out_lam[r] = 0.0;
     out_lam[r + 1] = 4.0 * r * 2.220446049250313e-16 * tr;
   }
-  if constexpr (VEC) {
+  if constexpr (VEC && STOREJ) {
+    double* jo = jbuf + d.j + (int64_t)rank * r;
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+      if (c < r && i < r) jo[i] = j[i];
+  } else if constexpr (VEC) {
     __builtin_amdgcn_wave_barrier();
     // Tu[:, rank] = Lv J[:, c], Tv[:, rank] = Lv^-T J[:, c] (column-major, ld r)
     double* tu = W + (int64_t)rank * r;
@@ -585,6 +600,61 @@ __global__ __launch_bounds__(64, 4) void eig_wave_kernel(const CDesc* __restrict
       for (int q = i + 1; q < G; ++q) s -= L[q + i * G] * t[q];
       t[i] = i < r ? s / L[i + i * G] : 0.0;
       if (c < r && i < r) tv[i] = t[i];
+    }
+  }
+}
+
+// Second pass of the single-Jacobi build: Tu[:, k] = Lv J[:, k] and
+// Tv[:, k] = Lv^-T J[:, k] from the stored sorted columns of J and the
+// Cholesky factor of Gv (recomputed from the same Gram matrix, so the same
+// bits as in the first pass), plus the block's lam and floor for the
+// projection.  Lv in LDS, or in global scratch (gstride doubles per item).
+template <bool GLOBAL>
+__global__ __launch_bounds__(256) void finish_kernel(const CDesc* __restrict__ ds,
+                                                     const int32_t* __restrict__ list, int nlist,
+                                                     double* __restrict__ ws,
+                                                     const double* __restrict__ jbuf,
+                                                     const double* __restrict__ lam_in,
+                                                     const double* __restrict__ floor_in,
+                                                     double* __restrict__ gscratch,
+                                                     int64_t gstride) {
+  extern __shared__ double sm[];
+  const int item = blockIdx.x;
+  if (item >= nlist) return;
+  const int li = list[item];
+  const CDesc d = ds[li];
+  const int r = d.r;
+  const int ld = r | 1;
+  auto at = [&](int i, int j) { return ix<GLOBAL>(i, j, ld); };
+  double* L = GLOBAL ? gscratch + gstride * item : sm;
+  double* dg = GLOBAL ? sm : L + r * ld;  // r
+  __shared__ int flag;
+  double* W = ws + d.w;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int idx = tid; idx < r * r; idx += nt) L[at(idx / r, idx % r)] = W[(int64_t)r * r + idx];
+  if (tid == 0) flag = 0;
+  wg_sync();
+  chol_inplace<GLOBAL>(L, r, ld, dg, &flag, tid, nt);
+  double* out_lam = W + 2 * (int64_t)r * r;
+  for (int k = tid; k < r; k += nt) out_lam[k] = lam_in[d.lamo + k];
+  if (tid == 0) {
+    out_lam[r] = 0.0;
+    out_lam[r + 1] = floor_in[li];
+  }
+  const double* J = jbuf + d.j;
+  for (int k = tid; k < r; k += nt) {
+    const double* jk = J + (int64_t)k * r;
+    double* tu = W + (int64_t)k * r;
+    double* tv = W + (int64_t)r * r + (int64_t)k * r;
+    for (int i = 0; i < r; ++i) {
+      double s = 0.0;
+      for (int q = 0; q <= i; ++q) s += L[at(i, q)] * jk[q];
+      tu[i] = s;
+    }
+    for (int i = r - 1; i >= 0; --i) {
+      double s = jk[i];
+      for (int q = i + 1; q < r; ++q) s -= L[at(q, i)] * tv[q];
+      tv[i] = s / L[at(i, i)];
     }
   }
 }
@@ -825,20 +895,25 @@ void build_mixed(Engine& e, double tol) {
   std::vector<uint8_t> raw(nb, 0);
   std::vector<double> lam_floor(nb, 0.0);
 
-  // Runs gram + eig on blocks [i0, i1) (ws offsets assigned here); pass 1
-  // copies lam/status out, pass 2 keeps them on the device for projection.
+  // Runs gram + the eigen step on blocks [i0, i1) (ws offsets assigned here)
+  // in one of four modes: kValues (the singular values only), kVectors (and
+  // the projections Tu, Tv), kStoreJ (values, and the sorted columns of J to
+  // d_j), kFinish (Tu, Tv from the stored J, with lam and floor from the
+  // first pass).  `sel` (optional) picks the blocks that need the step.
+  enum { kValues, kVectors, kStoreJ, kFinish };
   double* d_ws = nullptr;
   int64_t ws_cap = 0;
   CDesc* d_cd = nullptr;
   int32_t* d_list = nullptr;
-  for (const void* f : {(const void*)eig_kernel<false, true>, (const void*)eig_kernel<false, false>})
+  double* d_j = nullptr;  // stored J of every block (single-Jacobi build), or null
+  for (const void* f : {(const void*)eig_kernel<false, true>, (const void*)eig_kernel<false, false>,
+                        (const void*)eig_kernel<false, true, true>, (const void*)finish_kernel<false>})
     H3D_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-  // vectors: J and the projections as well (the second pass); the first pass
-  // needs the singular values only
-  auto run_eig = [&](int64_t i0, int64_t i1, bool vectors) {
+  auto run_eig = [&](int64_t i0, int64_t i1, int mode, const std::vector<uint8_t>* sel) {
     int64_t need = 0;
     for (int64_t i = i0; i < i1; ++i) {
       cd[i].w = need;
+      cd[i].lamo = lam_off[i] - lam_off[i0];
       need += wsize(cd[i].r);
     }
     if (need > ws_cap) {
@@ -854,7 +929,7 @@ void build_mixed(Engine& e, double tol) {
     for (int64_t i = i0; i < i1; ++i) {
       const int r = cd[i].r;
       const int32_t li = (int32_t)(i - i0);
-      if (r > kMaxEigRank) continue;
+      if (r > kMaxEigRank || (sel && !(*sel)[i])) continue;
       const int E = r * (r + 1) / 2;
       (E <= 4 * 256 ? g4 : (E <= 16 * 256 ? g16 : g40)).push_back(li);
       eg.push_back(li);
@@ -865,6 +940,7 @@ void build_mixed(Engine& e, double tol) {
     all.insert(all.end(), g40.begin(), g40.end());
     dfree(d_list);
     d_list = upload(all, st);
+    const int sides = mode == kFinish ? 1 : 2;  // the finish needs Gv only
     int64_t off = 0;
     auto launch_g = [&](std::vector<int32_t>& g, int which) {
       if (g.empty()) return;
@@ -873,16 +949,16 @@ void build_mixed(Engine& e, double tol) {
       const size_t lds = sizeof(double) * 33 * (size_t)rm;
       for (int64_t s = 0; s < (int64_t)g.size(); s += 1000000) {
         const int cnt = (int)std::min<int64_t>(1000000, (int64_t)g.size() - s);
-        const unsigned grid = 2u * (unsigned)cnt;
+        const unsigned grid = (unsigned)sides * (unsigned)cnt;
         if (which == 0)
           (src ? gram_kernel<4, float> : gram_kernel<4, double>)<<<grid, 256, lds, st>>>(
-              d_cd, d_b64, d_list + off + s, cnt, d_ws);
+              d_cd, d_b64, d_list + off + s, cnt, d_ws, sides);
         else if (which == 1)
           (src ? gram_kernel<16, float> : gram_kernel<16, double>)<<<grid, 256, lds, st>>>(
-              d_cd, d_b64, d_list + off + s, cnt, d_ws);
+              d_cd, d_b64, d_list + off + s, cnt, d_ws, sides);
         else
           (src ? gram_kernel<40, float> : gram_kernel<40, double>)<<<grid, 256, lds, st>>>(
-              d_cd, d_b64, d_list + off + s, cnt, d_ws);
+              d_cd, d_b64, d_list + off + s, cnt, d_ws, sides);
         H3D_LAUNCH_CHECK();
       }
       off += (int64_t)g.size();
@@ -890,18 +966,32 @@ void build_mixed(Engine& e, double tol) {
     launch_g(g4, 0);
     launch_g(g16, 1);
     launch_g(g40, 2);
+    // the first pass's lam and floor of these blocks, for the finish
+    double* d_lam = nullptr;
+    double* d_flo = nullptr;
+    if (mode == kFinish) {
+      std::vector<double> lam(lam_all.begin() + lam_off[i0], lam_all.begin() + lam_off[i1]);
+      std::vector<double> flo(lam_floor.begin() + i0, lam_floor.begin() + i1);
+      d_lam = upload(lam, st);
+      d_flo = upload(flo, st);
+    }
     // eig: ranks up to 16 / 32 in registers (eig_wave_kernel, 4 / 2 blocks per
     // wave); larger ranks four waves per block, launched in classes of LDS
     // footprint (each launch sizes its LDS by its largest rank, and that sets
     // how many workgroups share a CU); ranks beyond rmax with their matrices
-    // in global scratch
+    // in global scratch.  The finish: one wave per block up to rank 32, four
+    // above, Lv in LDS (global scratch where it does not fit).
     constexpr int kEigClasses = 7;  // 0, 1: registers; 2-5: LDS up to the caps; 6: global
     const size_t caps[4] = {40 << 10, 64 << 10, 96 << 10, (size_t)lds_max};
+    auto finish_lds = [](int r, bool global) {
+      return (size_t)((global ? 0 : r * (r | 1)) + r) * sizeof(double);
+    };
     std::vector<int32_t> grp[kEigClasses];
     for (int32_t li : eg) {
       const int r = cd[i0 + li].r;
       int q = 6;
-      if (r <= 16) q = 0;
+      if (mode == kFinish) q = r <= 32 ? 0 : (finish_lds(r, false) <= (size_t)lds_max ? 1 : 6);
+      else if (r <= 16) q = 0;
       else if (r <= 32) q = 1;
       else if (r <= rmax)
         for (int c = 0; c < 4; ++c)
@@ -911,6 +1001,7 @@ void build_mixed(Engine& e, double tol) {
           }
       grp[q].push_back(li);
     }
+    const bool vec = mode != kValues, store = mode == kStoreJ;
     for (int q = 0; q < kEigClasses; ++q) {
       std::vector<int32_t>& g = grp[q];
       if (g.empty()) continue;
@@ -918,33 +1009,92 @@ void build_mixed(Engine& e, double tol) {
       int rm = 1;
       for (int32_t li : g) rm = std::max(rm, cd[i0 + li].r);
       int32_t* d_g = upload(g, st);
-      const int64_t gstride = (int64_t)rm * (rm | 1);
-      const int64_t per = glob ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (3 * gstride)))
+      const int64_t gstride = mode == kFinish ? (int64_t)rm * (rm | 1) : (int64_t)rm * (rm | 1);
+      const int64_t mats = mode == kFinish ? 1 : 3;  // work matrices per item in global scratch
+      const int64_t per = glob ? std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t(1) << 28) / (mats * gstride)))
                                : 1000000;
-      double* gs = glob ? dalloc<double>(3 * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
+      double* gs = glob ? dalloc<double>(mats * gstride * std::min<int64_t>(per, (int64_t)g.size())) : nullptr;
       for (int64_t s = 0; s < (int64_t)g.size(); s += per) {
         const int cnt = (int)std::min<int64_t>(per, (int64_t)g.size() - s);
-        if (q == 0)
-          (vectors ? eig_wave_kernel<16, true> : eig_wave_kernel<16, false>)<<<(cnt + 3) / 4, 64, 0, st>>>(
-              d_cd, d_g + s, cnt, d_ws);
-        else if (q == 1)
-          (vectors ? eig_wave_kernel<32, true> : eig_wave_kernel<32, false>)<<<(cnt + 1) / 2, 64, 0, st>>>(
-              d_cd, d_g + s, cnt, d_ws);
-        else if (!glob)
-          (vectors ? eig_kernel<false, true> : eig_kernel<false, false>)<<<cnt, 256,
-                                                                         eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws,
-                                                                           nullptr, 0);
-        else
-          (vectors ? eig_kernel<true, true> : eig_kernel<true, false>)<<<cnt, 256, eig_lds(rm, true),
-                                                                       st>>>(d_cd, d_g + s, cnt, d_ws, gs,
-                                                                gstride);
+        if (mode == kFinish) {
+          const int nt = q == 0 ? 64 : 256;
+          if (glob)
+            finish_kernel<true><<<cnt, nt, finish_lds(rm, true), st>>>(d_cd, d_g + s, cnt, d_ws, d_j,
+                                                                        d_lam, d_flo, gs, gstride);
+          else
+            finish_kernel<false><<<cnt, nt, finish_lds(rm, false), st>>>(
+                d_cd, d_g + s, cnt, d_ws, d_j, d_lam, d_flo, nullptr, 0);
+        } else if (q == 0) {
+          auto k = store ? eig_wave_kernel<16, true, true>
+                         : (vec ? eig_wave_kernel<16, true> : eig_wave_kernel<16, false>);
+          k<<<(cnt + 3) / 4, 64, 0, st>>>(d_cd, d_g + s, cnt, d_ws, d_j);
+        } else if (q == 1) {
+          auto k = store ? eig_wave_kernel<32, true, true>
+                         : (vec ? eig_wave_kernel<32, true> : eig_wave_kernel<32, false>);
+          k<<<(cnt + 1) / 2, 64, 0, st>>>(d_cd, d_g + s, cnt, d_ws, d_j);
+        } else if (!glob) {
+          auto k = store ? eig_kernel<false, true, true>
+                         : (vec ? eig_kernel<false, true> : eig_kernel<false, false>);
+          k<<<cnt, 256, eig_lds(rm, false), st>>>(d_cd, d_g + s, cnt, d_ws, nullptr, 0, d_j);
+        } else {
+          auto k = store ? eig_kernel<true, true, true>
+                         : (vec ? eig_kernel<true, true> : eig_kernel<true, false>);
+          k<<<cnt, 256, eig_lds(rm, true), st>>>(d_cd, d_g + s, cnt, d_ws, gs, gstride, d_j);
+        }
         H3D_LAUNCH_CHECK();
       }
       H3D_HIP(hipStreamSynchronize(st));
       dfree(gs);
       dfree(d_g);
     }
+    dfree(d_lam);
+    dfree(d_flo);
   };
+
+  // scratch freed on every exit, exceptions included
+  struct Guard {
+    double*& ws;
+    CDesc*& cd;
+    int32_t*& list;
+    double*& j;
+    void**& b64;
+    ~Guard() {
+      dfree(ws);
+      dfree(cd);
+      dfree(list);
+      dfree(j);
+      dfree(b64);
+    }
+  } guard{d_ws, d_cd, d_list, d_j, d_b64};
+
+  // Single Jacobi: the first pass keeps every block's sorted J (r^2 doubles
+  // each) so that the second pass forms the projections without repeating the
+  // sweeps.  Only where it fits beside the copy being built; otherwise (and if
+  // the copy's own allocation later needs the room) the second pass runs the
+  // identical sweeps again with vectors.
+  {
+    int64_t jneed = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+      cd[i].j = cd[i].r <= kMaxEigRank ? jneed : -1;
+      if (cd[i].r <= kMaxEigRank) jneed += (int64_t)cd[i].r * cd[i].r;
+    }
+    size_t fb = 0, tb = 0;
+    H3D_HIP(hipMemGetInfo(&fb, &tb));
+    // room left for the batch workspace and the copy's classes (a third of
+    // the source copy's bytes, well above what a useful budget keeps)
+    int64_t src_bytes = 0;
+    for (const Chunk& ch : e.chunks) src_bytes += (ch.nU + ch.nV) * elem_size(src);
+    const int64_t reserve = 8 * ws_budget + src_bytes / 3 + (int64_t(2) << 30);
+    if (jneed > 0 && (int64_t)fb - 8 * jneed > reserve) {
+      try {
+        d_j = dalloc<double>(jneed);
+      } catch (const Error& er) {
+        if (er.code != H3D_ERR_OOM) throw;
+        d_j = nullptr;
+      }
+    }
+    mx.single_jacobi = d_j != nullptr;
+  }
 
   // ---------------- pass 1: eigenvalues of every block
   {
@@ -952,7 +1102,7 @@ void build_mixed(Engine& e, double tol) {
     while (i0 < nb) {
       int64_t i1 = i0, need = 0;
       while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
-      run_eig(i0, i1, false);
+      run_eig(i0, i1, d_j ? kStoreJ : kValues, nullptr);
       const int cnt = (int)(i1 - i0);
       std::vector<int64_t> off(cnt);
       for (int64_t i = i0; i < i1; ++i) off[i - i0] = lam_off[i] - lam_off[i0];
@@ -1300,7 +1450,15 @@ void build_mixed(Engine& e, double tol) {
     int64_t total = 0;
     for (int64_t n : fc.nU) total += span(n);
     for (int64_t n : fc.nV) total += span(n);
-    dmalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256));
+    try {
+      dmalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256));
+    } catch (const Error& er) {
+      if (er.code != H3D_ERR_OOM || !d_j) throw;
+      // the stored J gives way: the second pass repeats the sweeps instead
+      dfree(d_j);
+      mx.single_jacobi = false;
+      dmalloc(&fc.arena, (size_t)std::max<int64_t>(total, 256));
+    }
     H3D_HIP(hipMemsetAsync(fc.arena, 0, (size_t)std::max<int64_t>(total, 256), st));
     int64_t off = 0;
     auto carve = [&](void*& p, int64_t n) {
@@ -1313,13 +1471,17 @@ void build_mixed(Engine& e, double tol) {
     fc.d_ubase = upload(fc.udata, st);
     fc.d_vbase = upload(fc.vdata, st);
   }
-  // ---------------- pass 2: eigen again, then project into the classes
+  // ---------------- pass 2: the projections (from the stored J, or from the
+  // sweeps again), then project into the classes; blocks that keep no column
+  // and raw blocks need no eigen step
+  std::vector<uint8_t> work(nb, 0);
+  for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && (pd[i].r32 + pd[i].r16) > 0;
   {
     int64_t i0 = 0;
     while (i0 < nb) {
       int64_t i1 = i0, need = 0;
       while (i1 < nb && (i1 == i0 || need + wsize(cd[i1].r) <= ws_budget)) need += wsize(cd[i1++].r);
-      run_eig(i0, i1, true);
+      run_eig(i0, i1, d_j ? kFinish : kVectors, &work);
       std::vector<PDesc> sub(pd.begin() + i0, pd.begin() + i1);
       // blocks of rank > kMaxEigRank have no eigen step: raw in both passes
       PDesc* d_pd = upload(sub, st);

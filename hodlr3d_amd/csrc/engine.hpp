@@ -241,6 +241,7 @@ struct MixedCopy {
   // statistics of the last build
   double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0;
   double theta_drop = 0, ms = 0;
+  bool single_jacobi = false;        // the build kept J between its passes (one Jacobi per block)
   int64_t aca_rank_sum = 0, dropped = 0, raw_blocks = 0, raw_rank = 0;
   void free_all();
 };

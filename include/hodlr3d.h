@@ -116,6 +116,8 @@ typedef struct h3d_mixed_stats {
   int64_t bytes;            /* resident bytes of the copy */
   int64_t u_elems[2], v_elems[2];        /* per class (0: fp32, 1: fp16) */
   double build_ms;
+  int single_jacobi;        /* 1: the build kept each block's Jacobi rotations between its
+                               two passes (one Jacobi per block); 0: it ran them twice */
 } h3d_mixed_stats;
 
 typedef struct h3d_solve_info {

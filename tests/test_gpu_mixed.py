@@ -192,3 +192,28 @@ def test_first_mixed_product_of_host_vectors_builds_the_copy_first():
     assert np.array_equal(ym1, ym2)
     assert np.linalg.norm(ym1 - y64) / np.linalg.norm(y64) < 1e-6
     h.close()
+
+
+def test_two_pass_build_matches_the_single_jacobi_build(built):
+    """Where the stored rotations do not fit (forced here: the allocation of
+    the stored J fails), the build runs the Jacobi twice; both builds drop and
+    keep the same columns (the first passes are identical) and their products
+    agree to rounding."""
+    import hodlr3d_amd as H3
+    from hodlr3d_amd import _native as nat
+    name, x, h, tol, st = built
+    assert st["single_jacobi"] == 1
+    q = np.random.default_rng(41).standard_normal(len(x))
+    y1 = h.matvec(q, prec="mixed")
+    nat.lib().h3d_debug_fail_alloc(1)  # the build's second allocation is the stored J
+    try:
+        st2 = h.compress(tol)
+    finally:
+        nat.lib().h3d_debug_fail_alloc(-1)
+    assert st2["single_jacobi"] == 0
+    for k in ("rank_fp32", "rank_fp16", "dropped", "raw_blocks", "bytes"):
+        assert st2[k] == st[k], k
+    assert st2["predicted_rel_err"] == st["predicted_rel_err"]
+    y2 = h.matvec(q, prec="mixed")
+    assert np.linalg.norm(y2 - y1) / np.linalg.norm(y1) < 1e-12
+    h.compress(tol)  # the single-Jacobi copy again for the tests after this one
