@@ -305,6 +305,13 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
     y = h.matvec(Q, prec="fp32")
     ms = timed(torch, lambda: h.matvec(Q, prec="fp32"), steps, warmup)
     diff = float((torch.linalg.norm(y - y1, dim=0) / torch.linalg.norm(y1, dim=0)).max())
+    # the transposed product of the same group (A^T Q, the backward of A Q)
+    yt = h.matvec(Q, prec="fp32", transpose=True)
+    t_ms = timed(torch, lambda: h.matvec(Q, prec="fp32", transpose=True), steps, warmup)
+    h.set_rhs_block(0)
+    yt1 = h.matvec(Q[:, :2].contiguous(), prec="fp32", transpose=True)
+    h.set_rhs_block(tuned["width"])
+    t_diff = float((torch.linalg.norm(yt[:, :2] - yt1, dim=0) / torch.linalg.norm(yt1, dim=0)).max())
     rows = np.sort(np.random.default_rng(13).choice(N, size=1024, replace=False))
     errs = []
     for j in (0, nrhs - 1):
@@ -322,6 +329,8 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
            "factor_GB": round(fbytes / 1e9, 2),
            "factor_GB_per_s": round(fbytes * math.ceil(nrhs / 32) / (ms * 1e-3) / 1e9, 1),
            "rel_diff_vs_one_at_a_time": diff, "rel_err_vs_direct": max(errs),
+           "transpose_ms": round(t_ms, 4),
+           "transpose_rel_diff_vs_one_at_a_time": t_diff,
            "err_rows": len(rows), "assemble_s": round(t_asm, 2)}
     log(f"[bench] multi-rhs {out}")
     return out

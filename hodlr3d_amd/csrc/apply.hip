@@ -1888,7 +1888,7 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
   ensure_copy(e, prec);
   ensure_scratch(e, 1);
   if (block_width(e, nrhs, prec, transpose, acc64) > 1) {
-    matvec_block(e, d_b, d_y, nrhs, prec, acc64);
+    matvec_block(e, d_b, d_y, nrhs, prec, acc64, transpose);
     return;
   }
   hipStream_t st = e.stream;

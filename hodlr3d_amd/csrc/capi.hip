@@ -77,14 +77,7 @@ void Engine::free_all() {
   dfree(d_ylevw);
   dfree(d_ynw);
   dfree(d_iow);
-  dfree(d_rcols);
-  dfree(d_gitems);
-  dfree(d_gp2);
-  n_gp2 = 0;
-  dfree(d_gcolv);
-  dfree(d_gcolo);
-  n_rcols = n_gitems = g_np = 0;
-  n_p1[0] = n_p1[1] = n_p1[2] = 0;
+  for (GroupTables& g : grp) g.free_all();
   w_cap = io_w = 0;
   for (auto& ev : this->ev)
     if (ev) (void)hipEventDestroy(ev);

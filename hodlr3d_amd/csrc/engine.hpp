@@ -246,6 +246,30 @@ struct MixedCopy {
   void free_all();
 };
 
+// Work tables of the grouped products (several right-hand sides per factor
+// read, multi.hip) in one direction.  Phase 1 takes a column table: forward,
+// the V columns of every block grouped by source box; transposed, the U
+// columns of every target box.  Phase 2: forward, items over the rows of
+// each target box (its U is one matrix); transposed, items over the rows of
+// each source box with a column table (the V columns of the blocks with that
+// source lie in different chunks) and the t index of each column.
+struct GroupTables {
+  bool built = false;
+  void* d_rcols = nullptr;          // per column of a row-split source: int4 reduce entry
+  int64_t n_rcols = 0;
+  void* d_gitems = nullptr;         // phase-1 items (GTItem), by class of 64, 128, 256 rows
+  int64_t n_gitems = 0;
+  int64_t n_p1[3] = {0, 0, 0};      // items of each class
+  void* d_gp2 = nullptr;            // phase-2 items (P2Item)
+  int64_t n_gp2 = 0;
+  int64_t* d_gcolv = nullptr;       // phase-1 column table: element offset | chunk << 48
+  int2* d_gcolo = nullptr;          // phase-1 column table: {output slot of range 0, step per range}
+  int64_t g_np = 0;                 // partial entries per RHS
+  int64_t* d_p2col = nullptr;       // transposed phase 2: per column, element offset | chunk << 48
+  int32_t* d_p2t = nullptr;         // transposed phase 2: per column, its t index
+  void free_all();  // multi.hip
+};
+
 struct Engine {
   int64_t N = 0;
   int nL = 0;
@@ -325,16 +349,7 @@ struct Engine {
   double* d_ynw = nullptr;          // near field by position, W per point
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
-  void* d_rcols = nullptr;          // per column of a row-split block: int4 reduce entry
-  int64_t n_rcols = 0;
-  void* d_gitems = nullptr;         // phase-1 items (GTItem), by class of 64, 128, 256 rows
-  int64_t n_gitems = 0;
-  int64_t n_p1[3] = {0, 0, 0};      // items of each class
-  void* d_gp2 = nullptr;            // phase-2 items (P2Item) of 128 rows
-  int64_t n_gp2 = 0;
-  int64_t* d_gcolv = nullptr;       // column table: element offset in its V chunk | chunk << 48
-  int2* d_gcolo = nullptr;          // column table: {output slot of range 0, slot step per range}
-  int64_t g_np = 0;                 // partial entries per RHS
+  GroupTables grp[2];               // grouped products: forward, transposed
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
@@ -445,7 +460,8 @@ void load_engine(Engine& e, const char* path, int64_t ws_budget);
 // multi.hip: group width for nrhs right-hand sides (1: one at a time), and
 // the grouped product (forward, ACA factor copies)
 int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64);
-void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64);
+void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64,
+                  bool transpose);
 // times the grouped product of nrhs columns for every group width and
 // phase-1 item size, keeps the fastest; ms[8] (width-major), best[2]
 void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best);

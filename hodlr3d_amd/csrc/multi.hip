@@ -387,12 +387,18 @@ __global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
 // (g = l % 16, q = l / 16) takes rows i + 4g .. i + 4g + 3 of column c + q,
 // UF 4-column steps per unit, the next unit's loads issued before the MFMAs
 // of the current one.  The u-th MFMA of a step produces rows i + 4g' + u (g'
-// the MFMA output row).
-template <class S, class A, int W>
+// the MFMA output row).  CT (the transposed product, y_s = sum_b V_b t_b over
+// the blocks with source s): column c of the item is column table entry
+// it.uoff + c (element offset | chunk << 48 in `ubase`'s chunks) with its t
+// index in p2t; otherwise column c of U at ubase[it.chunk] + it.uoff + c ld and
+// t index it.toff + c.
+template <class S, class A, int W, bool CT = false>
 __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
                                                   void* const* __restrict__ ubase,
                                                   const A* __restrict__ tw,
-                                                  double* __restrict__ ylevw, int64_t N) {
+                                                  double* __restrict__ ylevw, int64_t N,
+                                                  const int64_t* __restrict__ p2col,
+                                                  const int32_t* __restrict__ p2t) {
   using M = Mfma<A>;
   using Acc = typename M::acc;
   constexpr int H = W / 16;
@@ -402,7 +408,7 @@ __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __
   const int lane = threadIdx.x & 63;
   const int lc = lane & 15, kq = lane >> 4;
   const P2Item it = items[w];
-  const S* U = (const S*)ubase[it.chunk] + it.uoff;
+  const S* U = CT ? nullptr : (const S*)ubase[it.chunk] + it.uoff;
   double* yb = ylevw + ((int64_t)(it.level - 1) * N + it.yoff) * W;
   for (int r0 = 0; r0 < it.nr; r0 += 64) {
     const int lim = it.i0 + min(it.nr, r0 + 64);  // rows < lim (box-relative)
@@ -421,9 +427,17 @@ __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __
       for (int st = 0; st < UF; ++st) {
         const int c = c0 + 4 * st + kq;
         const int cl = c < it.R ? c : it.R - 1;
+        if constexpr (CT) {
+          const int64_t cv = p2col[it.uoff + cl];
+          const int64_t tslot = p2t[it.uoff + cl];
 #pragma unroll
-        for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
-        load4<S, A>(U + (int64_t)cl * it.ld + il, av[st]);
+          for (int h = 0; h < H; ++h) tb[st][h] = tw[tslot * W + 16 * h + lc];
+          load4<S, A>((const S*)ubase[(int)(cv >> 48)] + (cv & kColMask) + il, av[st]);
+        } else {
+#pragma unroll
+          for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
+          load4<S, A>(U + (int64_t)cl * it.ld + il, av[st]);
+        }
       }
       __builtin_amdgcn_sched_barrier(0);  // the batch stays ahead of the MFMAs
     };
@@ -594,34 +608,79 @@ __global__ __launch_bounds__(256) void combine_w_kernel(const double* __restrict
   }
 }
 
-// The phase-1 column table: the V columns of every block of rank >= 1,
-// grouped by source box in by-source block order, each source padded to a
-// multiple of 32 columns with copies of its last column.  Entry: element
-// offset in the V chunk | chunk << 48, and the output slot {out0, step} (slot
-// out0 + range * step): the t index and 0; for a source of more than kGRows
-// rows, partial slot T + 1 + p (the partial sums follow the t slots) and the
-// block's rank; for padding, the spare t slot T.  Items: every source of at
-// most kGRows rows whole, every larger one in ranges of kGRows rows, each
-// range in the class of 64 NC rows that holds it (NC = 1, 2, 4), in slices of
-// g_cols / NC columns; class 1 items first, then 2, then 4.  One reduce entry
-// per column of a split source {first partial (from T + 1), ranges, slot
-// stride, t index}.
-void build_group_tables(Engine& e) {
-  std::vector<GTItem> items[3];
+// Phase-1 items of one group of columns [c0, c1) (a multiple of kTileCols)
+// whose rows are the m points at positions a0 ..: every group of at most
+// kGRows rows whole, a larger one in ranges of kGRows rows, each range in the
+// class of 64 NC rows that holds it (NC = 1, 2, 4), in slices of g_cols / NC
+// columns.
+void p1_items(std::vector<GTItem> (&items)[3], int64_t c0, int64_t c1, int m, int32_t a0, int gc) {
+  for (int s0 = 0; s0 < m; s0 += kGRows) {
+    const int nr = std::min(kGRows, m - s0);
+    const int cls = nr <= 64 ? 0 : (nr <= 128 ? 1 : 2);
+    const int lcols = gc >> cls;
+    for (int64_t cc = c0; cc < c1; cc += lcols)
+      items[cls].push_back(GTItem{(int32_t)cc, (int32_t)std::min<int64_t>(lcols, c1 - cc), a0 + s0,
+                                  nr, s0, s0 / kGRows, 0, 0});
+  }
+}
+
+// Appends one column to the phase-1 table: its factor column (element offset
+// | chunk << 48) and output slot {out0, step}: t index t (and step 0) for a
+// group of at most kGRows rows, else partial slot pbase + np + c with step
+// ncols (the partial sums of range q of column c at pbase + np + q ncols + c)
+// and a reduce entry {np + c, nsplit, ncols, t}.
+struct P1Table {
   std::vector<int64_t> colv;
   std::vector<int2> colo;
   std::vector<int4> cols;
+  int64_t np = 0;
+  void add(int32_t chunk, int64_t off, int nsplit, int64_t pbase, int c, int ncols, int64_t t) {
+    if (chunk >= (1 << 15) || off > kColMask) throw Error(H3D_ERR_INVALID, "problem too large");
+    colv.push_back(((int64_t)chunk << 48) | off);
+    if (nsplit > 1) {
+      colo.push_back(make_int2((int)(pbase + np + c), ncols));
+      cols.push_back(make_int4((int)(np + c), nsplit, ncols, (int)t));
+    } else {
+      colo.push_back(make_int2((int)t, 0));
+    }
+  }
+  // pads the group begun at c0 to a multiple of kTileCols with copies of its
+  // last column writing the spare t slot T
+  void pad(int64_t c0, int32_t spare) {
+    while ((colv.size() - c0) % kTileCols) {
+      colv.push_back(colv.back());
+      colo.push_back(make_int2(spare, 0));
+    }
+    if ((int64_t)colv.size() > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+  }
+};
+
+// Phase 1 forward: the V columns of every block of rank >= 1, grouped by
+// source box in by-source block order (x of the source staged once per item).
+// Phase 2 forward: every target box in items of 128 rows (its U is one
+// column-major matrix).  Transposed: phase 1 over the U columns of every
+// target box (U_k^T x_k, the box's rows staged once), phase 2 over the rows
+// of every source box, columns from a table: the V columns of the blocks with
+// that source (in different chunks) and their t indices.  Items of phase 2
+// largest first in both directions.
+void build_group_tables(Engine& e, int dir) {
+  GroupTables& g = e.grp[dir];
+  g.free_all();
+  std::vector<GTItem> items[3];
   std::vector<int32_t> btoff(std::max<int64_t>(e.nblk_all, 1), 0);
   std::vector<BlockDev> blk(std::max<int64_t>(e.nblk_all, 1));
   if (e.nblk_all) {
     H3D_HIP(hipMemcpy(btoff.data(), e.d_btoff, sizeof(int32_t) * e.nblk_all, hipMemcpyDeviceToHost));
     H3D_HIP(hipMemcpy(blk.data(), e.d_blk, sizeof(BlockDev) * e.nblk_all, hipMemcpyDeviceToHost));
   }
-  colv.reserve(e.rank_sum + e.rank_sum / 16);
-  colo.reserve(e.rank_sum + e.rank_sum / 16);
+  P1Table tab;
+  tab.colv.reserve(e.rank_sum + e.rank_sum / 16);
+  tab.colo.reserve(e.rank_sum + e.rank_sum / 16);
   const int gc = e.g_cols == 1024 ? 1024 : 2048;
   const int64_t pbase = (int64_t)e.T + 1;  // first partial slot
-  int64_t np = 0;
+  std::vector<P2Item> p2;
+  std::vector<int64_t> p2col;
+  std::vector<int32_t> p2t;
   for (int j = 1; j <= e.nL; ++j) {
     const Level& lv = e.lev[j];
     const int64_t lo = level_offset(j);
@@ -629,98 +688,102 @@ void build_group_tables(Engine& e) {
     for (int32_t k : lv.ilidx) ++first[k + 1];
     for (int64_t k = 0; k < lv.nboxes; ++k) first[k + 1] += first[k];
     for (int64_t k = 0; k < lv.nboxes; ++k) {
-      const int m = e.bcount[lo + k];
-      const int64_t qb = lv.bbase + first[k], qe = lv.bbase + first[k + 1];
-      if (m == 0 || qb == qe) continue;
-      const int nsplit = (m + kGRows - 1) / kGRows;
-      const int64_t ldv = m + (m & 1);
-      const int64_t c0 = (int64_t)colv.size();
-      for (int64_t q = qb; q < qe; ++q) {
-        const int32_t b = e.by_src[q];
-        const int r = lv.rank[b - lv.bbase];
-        if (r == 0) continue;
-        const BlockDev& d = blk[b];
-        if (d.chunk >= (1 << 15) || d.voff + r * ldv > kColMask)
-          throw Error(H3D_ERR_INVALID, "problem too large");
-        for (int c = 0; c < r; ++c) {
-          colv.push_back(((int64_t)d.chunk << 48) | (d.voff + c * ldv));
-          if (nsplit > 1) {
-            colo.push_back(make_int2((int)(pbase + np + c), r));
-            cols.push_back(make_int4((int)(np + c), nsplit, r, btoff[b] + c));
-          } else {
-            colo.push_back(make_int2(btoff[b] + c, 0));
+      const int n = e.bcount[lo + k];  // rows of box k (as target or as source)
+      if (n == 0) continue;
+      const int nsplit = (n + kGRows - 1) / kGRows;
+      const int64_t c0 = (int64_t)tab.colv.size();
+      if (dir == 0) {  // V columns of the blocks with source k
+        const int64_t ldv = n + (n & 1);
+        for (int64_t q = lv.bbase + first[k]; q < lv.bbase + first[k + 1]; ++q) {
+          const int32_t b = e.by_src[q];
+          const int r = lv.rank[b - lv.bbase];
+          for (int c = 0; c < r; ++c)
+            tab.add(blk[b].chunk, blk[b].voff + c * ldv, nsplit, pbase, c, r, btoff[b] + c);
+          if (nsplit > 1 && r) tab.np += (int64_t)nsplit * r;
+        }
+      } else {  // U columns of target box k
+        const int R = lv.Rt[k];
+        const int64_t ld = aca_ldu(n);
+        for (int c = 0; c < R; ++c)
+          tab.add(lv.chunk[k], lv.uoff[k] + c * ld, nsplit, pbase, c, R, lv.toff[k] + c);
+        if (nsplit > 1 && R) tab.np += (int64_t)nsplit * R;
+      }
+      if (pbase + tab.np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+      if ((int64_t)tab.colv.size() == c0) continue;
+      tab.pad(c0, e.T);
+      p1_items(items, c0, (int64_t)tab.colv.size(), n, e.bstart[lo + k], gc);
+    }
+    // phase 2
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      if (n <= 0) continue;
+      P2Item it{};
+      if (dir == 0) {
+        const int R = lv.Rt[k];
+        if (R <= 0) continue;
+        it.uoff = lv.uoff[k];
+        it.chunk = lv.chunk[k];
+        it.ld = aca_ldu(n);
+        it.R = R;
+        it.toff = lv.toff[k];
+      } else {  // the columns of V_s: blocks with source k
+        const int64_t ldv = n + (n & 1);
+        const int64_t c0 = (int64_t)p2col.size();
+        for (int64_t q = lv.bbase + first[k]; q < lv.bbase + first[k + 1]; ++q) {
+          const int32_t b = e.by_src[q];
+          const int r = lv.rank[b - lv.bbase];
+          for (int c = 0; c < r; ++c) {
+            p2col.push_back(((int64_t)blk[b].chunk << 48) | (blk[b].voff + c * ldv));
+            p2t.push_back(btoff[b] + c);
           }
         }
-        if (nsplit > 1) {
-          np += (int64_t)nsplit * r;
-          if (pbase + np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
-        }
+        const int64_t R = (int64_t)p2col.size() - c0;
+        if (R <= 0) continue;
+        if (R > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+        it.uoff = c0;
+        it.R = (int32_t)R;
       }
-      if ((int64_t)colv.size() == c0) continue;
-      while ((colv.size() - c0) % kTileCols) {
-        colv.push_back(colv.back());
-        colo.push_back(make_int2((int)e.T, 0));
-      }
-      const int64_t c1 = (int64_t)colv.size();
-      if (c1 > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
-      for (int s0 = 0; s0 < m; s0 += kGRows) {
-        const int nr = std::min(kGRows, m - s0);
-        const int cls = nr <= 64 ? 0 : (nr <= 128 ? 1 : 2);
-        const int lcols = gc >> cls;
-        for (int64_t cc = c0; cc < c1; cc += lcols)
-          items[cls].push_back(GTItem{(int32_t)cc, (int32_t)std::min<int64_t>(lcols, c1 - cc),
-                                      e.bstart[lo + k] + s0, nr, s0, s0 / kGRows, 0, 0});
+      it.yoff = e.bstart[lo + k];
+      it.level = j;
+      for (int i0 = 0; i0 < n; i0 += 128) {
+        it.i0 = i0;
+        it.nr = std::min(128, n - i0);
+        p2.push_back(it);
       }
     }
   }
   std::vector<GTItem> all;
   for (int c = 0; c < 3; ++c) {
-    e.n_p1[c] = (int64_t)items[c].size();
+    g.n_p1[c] = (int64_t)items[c].size();
     all.insert(all.end(), items[c].begin(), items[c].end());
-  }
-  e.n_gitems = (int64_t)all.size();
-  e.d_gitems = upload(all, e.stream);
-  e.d_gcolv = upload(colv, e.stream);
-  e.d_gcolo = upload(colo, e.stream);
-  e.n_rcols = (int64_t)cols.size();
-  e.d_rcols = upload(cols, e.stream);
-  e.g_np = np;
-  // phase 2: every target box in items of 128 rows, the largest first
-  std::vector<P2Item> p2;
-  for (int j = 1; j <= e.nL; ++j) {
-    const Level& lv = e.lev[j];
-    const int64_t lo = level_offset(j);
-    for (int64_t k = 0; k < lv.nboxes; ++k) {
-      const int n = e.bcount[lo + k];
-      const int R = lv.Rt[k];
-      if (n <= 0 || R <= 0) continue;
-      for (int i0 = 0; i0 < n; i0 += 128) {
-        P2Item it{};
-        it.uoff = lv.uoff[k];
-        it.chunk = lv.chunk[k];
-        it.ld = aca_ldu(n);
-        it.i0 = i0;
-        it.nr = std::min(128, n - i0);
-        it.R = R;
-        it.toff = lv.toff[k];
-        it.yoff = e.bstart[lo + k];
-        it.level = j;
-        p2.push_back(it);
-      }
-    }
   }
   std::stable_sort(p2.begin(), p2.end(), [](const P2Item& a, const P2Item& b) {
     return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
   });
-  e.n_gp2 = (int64_t)p2.size();
-  e.d_gp2 = upload(p2, e.stream);
+  g.n_gitems = (int64_t)all.size();
+  g.d_gitems = upload(all, e.stream);
+  g.d_gcolv = upload(tab.colv, e.stream);
+  g.d_gcolo = upload(tab.colo, e.stream);
+  g.n_rcols = (int64_t)tab.cols.size();
+  g.d_rcols = upload(tab.cols, e.stream);
+  g.g_np = tab.np;
+  g.n_gp2 = (int64_t)p2.size();
+  g.d_gp2 = upload(p2, e.stream);
+  if (dir == 1) {
+    g.d_p2col = upload(p2col, e.stream);
+    g.d_p2t = upload(p2t, e.stream);
+  }
   H3D_HIP(hipStreamSynchronize(e.stream));
+  g.built = true;
 }
 
-void ensure_block_scratch(Engine& e, int W) {
-  if (!e.d_gitems) build_group_tables(e);
-  // t slots 0..T (T: spare for padded columns), then the partial sums
-  const int64_t T = e.T + 1, np = std::max<int64_t>(e.g_np, 1);
+void ensure_block_scratch(Engine& e, int W, int dir) {
+  if (!e.grp[dir].built) build_group_tables(e, dir);
+  // t slots 0..T (T: spare for padded columns), then the partial sums (the
+  // larger count of the directions built)
+  int64_t np = 1;
+  for (const GroupTables& g : e.grp) np = std::max(np, g.g_np);
+  const int64_t T = e.T + 1;
   if (e.w_cap >= W && e.w_t >= T && e.w_np >= np) return;
   W = std::max(W, e.w_cap);
   dfree(e.d_xw);
@@ -738,8 +801,8 @@ void ensure_block_scratch(Engine& e, int W) {
 }
 
 template <class S, class A, int W, int NC>
-void launch_p1(Engine& e, int prec, int64_t first, int64_t count, const A* xr, A* tw,
-               hipStream_t st) {
+void launch_p1(Engine& e, const GroupTables& g, void* const* base, int64_t first, int64_t count,
+               const A* xr, A* tw, hipStream_t st) {
   if (count == 0) return;
   auto f = e.g_cols == 1024 ? p1r_kernel<S, A, W, NC, 1024 / NC> : p1r_kernel<S, A, W, NC, 2048 / NC>;
   const size_t lds = sizeof(A) * 64 * NC * W;  // x; the tables are static
@@ -747,15 +810,19 @@ void launch_p1(Engine& e, int prec, int64_t first, int64_t count, const A* xr, A
                       sizeof(void*) * kLdsChunks;
   if (lds + stat > 65536)
     H3D_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  f<<<(unsigned)count, 256, lds, st>>>((const GTItem*)e.d_gitems + first, (int)count, e.d_gcolv,
-                                       e.d_gcolo, e.d_vbase[prec], (int)e.chunks.size(), xr, tw);
+  f<<<(unsigned)count, 256, lds, st>>>((const GTItem*)g.d_gitems + first, (int)count, g.d_gcolv,
+                                       g.d_gcolo, base, (int)e.chunks.size(), xr, tw);
   H3D_LAUNCH_CHECK();
 }
 
 template <class S, class A, int W>
-void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int cnt) {
+void run_group(Engine& e, int prec, bool acc64, bool transpose, const double* b, double* y, int cnt) {
   hipStream_t st = e.stream;
   const int64_t N = e.N;
+  const GroupTables& g = e.grp[transpose ? 1 : 0];
+  // forward: phase 1 reads V, phase 2 U; transposed the other way round
+  void* const* p1base = transpose ? e.d_ubase[prec] : e.d_vbase[prec];
+  void* const* p2base = transpose ? e.d_vbase[prec] : e.d_ubase[prec];
   A* xr = (A*)e.d_xw;
   A* tw = (A*)e.d_tw;
   A* partw = tw + (e.T + 1) * W;  // partial sums after the t slots
@@ -767,41 +834,43 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     H3D_LAUNCH_CHECK();
   }
   // near field into its own buffer (W per point) on the handle's second
-  // stream, overlapping phases 1 and 2 (in line while a graph is captured)
+  // stream, overlapping phases 1 and 2 (in line while a graph is captured);
+  // the grouped transposes take symmetric kernels only, whose near field is
+  // its own transpose
   const int nn = (int)e.nitems.size();
   const bool overlap = nn && e.overlap_near && e.side;
   auto launch_near = [&](hipStream_t ns) {
     const int64_t lo = level_offset(e.nL);
     const KernelDev kd = e.kern;
     const bool invr = kd.q == 7 || kd.q == 16;
-    const unsigned g = wave_grid(nn);
+    const unsigned gn = wave_grid(nn);
     const int32_t* nbr = e.lev[e.nL].d_nbr;
     if constexpr (std::is_same<A, float>::value) {
       if (invr)
-        near_w32_kernel<InvR32, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                      e.d_leafc, kd, xr, e.d_ynw);
+        near_w32_kernel<InvR32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                       e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                       e.d_leafc, kd, xr, e.d_ynw);
       else
-        near_w32_kernel<AnyQ32, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                      e.d_leafc, kd, xr, e.d_ynw);
+        near_w32_kernel<AnyQ32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                       e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                       e.d_leafc, kd, xr, e.d_ynw);
     } else {
       if (kd.q == kQPlugin)
-        near_w64_kernel<Plugin, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                      e.d_bcount + lo, e.d_px, e.d_py, e.d_pz,
-                                                      kd, xr, e.d_ynw);
+        near_w64_kernel<Plugin, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                       e.d_bcount + lo, e.d_px, e.d_py, e.d_pz,
+                                                       kd, xr, e.d_ynw);
       else if (invr && prec != H3D_PREC_FP64)
-        near_w64_kernel<InvRNewton, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                          e.d_bcount + lo, e.d_px, e.d_py,
-                                                          e.d_pz, kd, xr, e.d_ynw);
+        near_w64_kernel<InvRNewton, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                           e.d_bcount + lo, e.d_px, e.d_py,
+                                                           e.d_pz, kd, xr, e.d_ynw);
       else if (invr)
-        near_w64_kernel<InvR, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
-                                                    xr, e.d_ynw);
+        near_w64_kernel<InvR, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                     e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                     xr, e.d_ynw);
       else
-        near_w64_kernel<AnyQ, W><<<g, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                    e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
-                                                    xr, e.d_ynw);
+        near_w64_kernel<AnyQ, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                     e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                     xr, e.d_ynw);
     }
     H3D_LAUNCH_CHECK();
   };
@@ -811,17 +880,21 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
     launch_near(e.side);
     H3D_HIP(hipEventRecord(e.ev_join, e.side));
   }
-  launch_p1<S, A, W, 1>(e, prec, 0, e.n_p1[0], xr, tw, st);
-  launch_p1<S, A, W, 2>(e, prec, e.n_p1[0], e.n_p1[1], xr, tw, st);
-  launch_p1<S, A, W, 4>(e, prec, e.n_p1[0] + e.n_p1[1], e.n_p1[2], xr, tw, st);
-  if (e.n_rcols) {
-    reduce_w_kernel<A, W><<<(unsigned)((e.n_rcols * W + 255) / 256), 256, 0, st>>>(
-        (const int4*)e.d_rcols, e.n_rcols, partw, tw);
+  launch_p1<S, A, W, 1>(e, g, p1base, 0, g.n_p1[0], xr, tw, st);
+  launch_p1<S, A, W, 2>(e, g, p1base, g.n_p1[0], g.n_p1[1], xr, tw, st);
+  launch_p1<S, A, W, 4>(e, g, p1base, g.n_p1[0] + g.n_p1[1], g.n_p1[2], xr, tw, st);
+  if (g.n_rcols) {
+    reduce_w_kernel<A, W><<<(unsigned)((g.n_rcols * W + 255) / 256), 256, 0, st>>>(
+        (const int4*)g.d_rcols, g.n_rcols, partw, tw);
     H3D_LAUNCH_CHECK();
   }
-  if (e.n_gp2) {
-    p2w_kernel<S, A, W><<<wave_grid(e.n_gp2), 256, 0, st>>>((const P2Item*)e.d_gp2, (int)e.n_gp2,
-                                                           e.d_ubase[prec], tw, e.d_ylevw, N);
+  if (g.n_gp2) {
+    if (transpose)
+      p2w_kernel<S, A, W, true><<<wave_grid(g.n_gp2), 256, 0, st>>>(
+          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, g.d_p2col, g.d_p2t);
+    else
+      p2w_kernel<S, A, W><<<wave_grid(g.n_gp2), 256, 0, st>>>(
+          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, nullptr, nullptr);
     H3D_LAUNCH_CHECK();
   }
   if (nn && !overlap) launch_near(st);
@@ -834,45 +907,53 @@ void run_group(Engine& e, int prec, bool acc64, const double* b, double* y, int 
 }
 
 template <int W>
-void run_group_w(Engine& e, int prec, bool acc64, const double* b, double* y, int cnt) {
-  if (prec == H3D_PREC_FP64) run_group<double, double, W>(e, prec, acc64, b, y, cnt);
-  else if (prec == H3D_PREC_FP32 && acc64) run_group<float, double, W>(e, prec, acc64, b, y, cnt);
-  else if (prec == H3D_PREC_FP32) run_group<float, float, W>(e, prec, acc64, b, y, cnt);
-  else if (acc64) run_group<uint16_t, double, W>(e, prec, acc64, b, y, cnt);
-  else run_group<uint16_t, float, W>(e, prec, acc64, b, y, cnt);
+void run_group_w(Engine& e, int prec, bool acc64, bool tr, const double* b, double* y, int cnt) {
+  if (prec == H3D_PREC_FP64) run_group<double, double, W>(e, prec, acc64, tr, b, y, cnt);
+  else if (prec == H3D_PREC_FP32 && acc64) run_group<float, double, W>(e, prec, acc64, tr, b, y, cnt);
+  else if (prec == H3D_PREC_FP32) run_group<float, float, W>(e, prec, acc64, tr, b, y, cnt);
+  else if (acc64) run_group<uint16_t, double, W>(e, prec, acc64, tr, b, y, cnt);
+  else run_group<uint16_t, float, W>(e, prec, acc64, tr, b, y, cnt);
 }
 
 }  // namespace
 
+void GroupTables::free_all() {
+  dfree(d_rcols);
+  dfree(d_gitems);
+  dfree(d_gp2);
+  dfree(d_gcolv);
+  dfree(d_gcolo);
+  dfree(d_p2col);
+  dfree(d_p2t);
+  n_rcols = n_gitems = n_gp2 = g_np = 0;
+  n_p1[0] = n_p1[1] = n_p1[2] = 0;
+  built = false;
+}
+
 void release_group_tables(Engine& e) {
-  dfree(e.d_gitems);
-  dfree(e.d_gp2);
-  e.n_gp2 = 0;
-  dfree(e.d_gcolv);
-  dfree(e.d_gcolo);
-  dfree(e.d_rcols);
-  e.n_gitems = e.n_rcols = e.g_np = 0;
-  e.n_p1[0] = e.n_p1[1] = e.n_p1[2] = 0;
+  for (GroupTables& g : e.grp) g.free_all();
 }
 
 int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64) {
-  if (transpose || prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
+  if (prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
   if ((int64_t)e.chunks.size() > kLdsChunks) return 1;  // phase 1 keeps the chunk bases in LDS
-  // the grouped fp32-sum near field works on fp32 offsets of radial kernels
-  if (e.kern.q == kQPlugin && prec != H3D_PREC_FP64 && !acc64) return 1;
+  // a plugin kernel: the grouped fp32-sum near field works on fp32 offsets of
+  // radial kernels, and a plugin's transpose needs entry(x_j, x_i)
+  if (e.kern.q == kQPlugin && (transpose || (prec != H3D_PREC_FP64 && !acc64))) return 1;
   return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
 }
 
-void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64) {
+void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64,
+                  bool transpose) {
   const int Wmax = e.rhs_block == 16 ? 16 : 32;
-  ensure_block_scratch(e, nrhs <= 16 ? 16 : Wmax);
+  ensure_block_scratch(e, nrhs <= 16 ? 16 : Wmax, transpose ? 1 : 0);
   const int64_t N = e.N;
   for (int r0 = 0; r0 < nrhs; r0 += Wmax) {
     const int cnt = std::min(Wmax, nrhs - r0);
     const double* b = d_b + (int64_t)r0 * N;
     double* y = d_y + (int64_t)r0 * N;
-    if (cnt <= 16) run_group_w<16>(e, prec, acc64, b, y, cnt);
-    else run_group_w<32>(e, prec, acc64, b, y, cnt);
+    if (cnt <= 16) run_group_w<16>(e, prec, acc64, transpose, b, y, cnt);
+    else run_group_w<32>(e, prec, acc64, transpose, b, y, cnt);
   }
 }
 
@@ -907,9 +988,9 @@ void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best
           release_group_tables(e);
           e.g_cols = cols[ci];
         }
-        matvec_block(e, B, Y, nrhs, prec, acc64);  // warm-up (builds the tables)
+        matvec_block(e, B, Y, nrhs, prec, acc64, false);  // warm-up (builds the tables)
         H3D_HIP(hipEventRecord(ev[0], e.stream));
-        for (int q = 0; q < reps; ++q) matvec_block(e, B, Y, nrhs, prec, acc64);
+        for (int q = 0; q < reps; ++q) matvec_block(e, B, Y, nrhs, prec, acc64, false);
         H3D_HIP(hipEventRecord(ev[1], e.stream));
         H3D_HIP(hipEventSynchronize(ev[1]));
         float t = 0;

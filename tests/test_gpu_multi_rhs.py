@@ -87,12 +87,41 @@ def test_nonfinite_column_stays_in_its_column(handle):
     assert _relcols(Y[:, good], _single(h, Q[:, good], "fp64")) <= 1e-13
 
 
-def test_transpose_and_mixed_stay_per_column(handle):
+@pytest.mark.parametrize("prec", list(MODES))
+def test_grouped_transposes_equal_single_transposes(handle, prec):
+    """A^T with several right-hand sides: phase 1 takes U_k^T x_k per target
+    box, phase 2 sums V_b t_b over the blocks of each source box (columns from
+    a table); every column equals the single transposed product within the
+    mode's rounding, for widths 16 and 32 and ragged counts."""
+    x, h = handle
+    tol = MODES[prec]
+    Q = np.random.default_rng(9).standard_normal((len(x), 40))
+    ref = np.stack([h.matvec(Q[:, j], prec=prec, transpose=True) for j in range(40)], axis=1)
+    for width, counts in ((32, (2, 17, 40)), (16, (16, 40))):
+        h.set_rhs_block(width)
+        for n in counts:
+            print(f"[{prec} transposed] width {width} nrhs={n}", flush=True)
+            Y = h.matvec(Q[:, :n], prec=prec, transpose=True)
+            assert _relcols(Y, ref[:, :n]) <= tol, (prec, width, n)
+    h.set_rhs_block(32)
+
+
+def test_grouped_transpose_is_the_adjoint(handle):
+    """<A^T X, Z> = <X, A Z> column by column, both sides grouped (fp64)."""
+    x, h = handle
+    rng = np.random.default_rng(10)
+    X = rng.standard_normal((len(x), 20))
+    Z = rng.standard_normal((len(x), 20))
+    AtX = h.matvec(X, transpose=True)
+    AZ = h.matvec(Z)
+    lhs = np.einsum("ij,ij->j", AtX, Z)
+    rhs = np.einsum("ij,ij->j", X, AZ)
+    assert np.max(np.abs(lhs - rhs) / np.abs(rhs)) <= 1e-12
+
+
+def test_mixed_stays_per_column(handle):
     x, h = handle
     Q = np.random.default_rng(5).standard_normal((len(x), 3))
-    YT = h.matvec(Q, transpose=True)
-    for j in range(3):
-        assert np.array_equal(YT[:, j], h.matvec(Q[:, j], transpose=True))
     h.compress(1e-6)
     YM = h.matvec(Q, prec="mixed")
     for j in range(3):
@@ -168,8 +197,10 @@ def test_grouped_products_repeat_bit_for_bit(handle, prec):
     Q = np.random.default_rng(13).standard_normal((len(x), 24))
     for w in (16, 32):
         h.set_rhs_block(w)
-        for cnt in (3, 16, 24):
-            first = h.matvec(Q[:, :cnt], prec=prec)
-            for _ in range(3):
-                assert np.array_equal(h.matvec(Q[:, :cnt], prec=prec), first), (prec, w, cnt)
+        for tr in (False, True):
+            for cnt in (3, 16, 24):
+                first = h.matvec(Q[:, :cnt], prec=prec, transpose=tr)
+                for _ in range(3):
+                    assert np.array_equal(h.matvec(Q[:, :cnt], prec=prec, transpose=tr), first), \
+                        (prec, w, tr, cnt)
     h.set_rhs_block(32)
