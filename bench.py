@@ -489,9 +489,9 @@ def main():
                     help="only the many-instance graph measurement (its own JSON line)")
     ap.add_argument("--batch", type=int, default=64,
                     help="instances of the many-instance measurement (0: skip)")
-    ap.add_argument("--batch-n", default="8192,65536",
-                    help="range of N of --only-batch (log-uniform; the spec's 200000 upper end "
-                         "needs ~270 GB of fp32 factors for 64 instances)")
+    ap.add_argument("--batch-n", default="8192,200000",
+                    help="range of N of the many-instance measurement (log-uniform; the batching "
+                         "configuration's range: 242 GB of fp32 factors for the 64 seeded draws)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
