@@ -573,6 +573,113 @@ __global__ __launch_bounds__(256) void near_w32_kernel(
   for (int j = 0; j < W; ++j) yp[j] = kd.scale * (double)acc[j];
 }
 
+// Near field with fp32 entries and sums on the matrix cores: Y(t, j) = sum_s
+// K(t, s) X(s, j) for the <= 64 targets t of an item.  The sources of each
+// neighbour leaf are staged 64 at a time in the wave's LDS slice (coordinates
+// as fp32 offsets from their leaf centre, and their W values); each step takes
+// 4 of them: lane (c = l % 16, q = l / 16) forms K(target 16 mt + c, source
+// s0 + q) for the four 16-target tiles mt on the VALU (the A operands), reads
+// X(source s0 + q, RHS 16 h + c) from LDS (the B operands), and the 16x16x4
+// MFMA accumulates D(target, RHS); lane l then holds targets 16 mt + 4 q + a.
+// Every tile is multiplied on every step (rows past the item are not stored),
+// sources past the chunk and the self pair meet a zero entry.  Each sum runs
+// over the sources in the same order whatever the group width.
+template <class KF, int W>
+__global__ __launch_bounds__(256) void near_wm_kernel(
+    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
+    const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
+    const double* __restrict__ leafc, KernelDev kd, const float* __restrict__ xr,
+    double* __restrict__ yNw) {
+  using M = Mfma<float>;
+  using Acc = M::acc;
+  constexpr int H = W / 16;
+  // row stride of the staged values: 16 banks apart from one source to the
+  // next, so that the two 16-lane groups of a half-wave read distinct banks
+  constexpr int XS = W % 32 == 0 ? W + 16 : W;
+  __shared__ float4 scoord[kWavesPerBlock][64];
+  __shared__ __align__(16) float sval[kWavesPerBlock][64 * XS];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int w = wave_id();
+  if (w >= nitems) return;
+  const int lane = threadIdx.x & 63, lc = lane & 15, kq = lane >> 4;
+  const NearItem it = items[w];
+  const int rs = bstart[it.leaf];
+  // the lane's targets (clamped to the item's rows; the extra rows are not stored)
+  float tx[4], ty[4], tz[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const int t = min(16 * mt + lc, it.nr - 1);
+    const int ip = rs + it.i0 + t;
+    tx[mt] = lx[ip], ty[mt] = ly[ip], tz[mt] = lz[ip];
+  }
+  const double cxk = leafc[3 * (int64_t)it.leaf], cyk = leafc[3 * (int64_t)it.leaf + 1],
+               czk = leafc[3 * (int64_t)it.leaf + 2];
+  Acc acc[4][H];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int h = 0; h < H; ++h) acc[mt][h] = Acc{};
+  for (int sl = 0; sl <= 26; ++sl) {
+    const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
+    if (nn < 0) continue;
+    const int cs = bstart[nn], m = bcount[nn];
+    const float dcx = (float)(cxk - leafc[3 * (int64_t)nn]);
+    const float dcy = (float)(cyk - leafc[3 * (int64_t)nn + 1]);
+    const float dcz = (float)(czk - leafc[3 * (int64_t)nn + 2]);
+    float ax[4], ay[4], az[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) ax[mt] = tx[mt] + dcx, ay[mt] = ty[mt] + dcy, az[mt] = tz[mt] + dcz;
+    const bool own = nn == it.leaf;
+    for (int c0 = 0; c0 < m; c0 += 64) {
+      const int cnt = min(64, m - c0);
+      {  // stage the chunk: lane i takes source c0 + i (clamped)
+        const int jp = cs + c0 + (lane < cnt ? lane : cnt - 1);
+        const float4 pc = make_float4(lx[jp], ly[jp], lz[jp], 0.0f);
+        const float4* src = (const float4*)(xr + (int64_t)jp * W);
+        float4 v[W / 4];
+#pragma unroll
+        for (int u = 0; u < W / 4; ++u) v[u] = src[u];
+        __builtin_amdgcn_wave_barrier();  // after the previous chunk's reads
+        scoord[wv][lane] = pc;
+#pragma unroll
+        for (int u = 0; u < W / 4; ++u) *(float4*)&sval[wv][lane * XS + 4 * u] = v[u];
+        __builtin_amdgcn_wave_barrier();  // before this chunk's reads
+      }
+      for (int s0 = 0; s0 < cnt; s0 += 4) {
+        const int j = s0 + kq;
+        const bool valid = j < cnt;
+        const int jc = valid ? j : cnt - 1;
+        const float4 p = scoord[wv][jc];
+        float xb[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) xb[h] = sval[wv][jc * XS + 16 * h + lc];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const float dx = ax[mt] - p.x, dy = ay[mt] - p.y, dz = az[mt] - p.z;
+          const float R2 = dx * dx + dy * dy + dz * dz;
+          float kv = KF::f(kd.q, R2);
+          const bool self = own && c0 + j == it.i0 + 16 * mt + lc;
+          kv = (valid && !self) ? kv : 0.0f;
+#pragma unroll
+          for (int h = 0; h < H; ++h) acc[mt][h] = M::mac(kv, xb[h], acc[mt][h]);
+        }
+      }
+    }
+  }
+  double* yb = yNw + (int64_t)(rs + it.i0) * W;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int t = 16 * mt + M::row(kq, a);
+      if (t < it.nr) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) yb[(int64_t)t * W + 16 * h + lc] = kd.scale * (double)acc[mt][h][a];
+      }
+    }
+}
+
 // y[j N + i] = alpha * (sum over levels of y_level(iperm[i], j) + y_N(iperm[i], j)) + d_i b[j N + i]
 // for j < cnt: 64 points per block, transposed through LDS so that both the
 // level sums (W per point) and the outputs (64 points per RHS) are contiguous.
@@ -847,13 +954,13 @@ void run_group(Engine& e, int prec, bool acc64, bool transpose, const double* b,
     const int32_t* nbr = e.lev[e.nL].d_nbr;
     if constexpr (std::is_same<A, float>::value) {
       if (invr)
-        near_w32_kernel<InvR32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                       e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                       e.d_leafc, kd, xr, e.d_ynw);
+        near_wm_kernel<InvR32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, e.d_ynw);
       else
-        near_w32_kernel<AnyQ32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
-                                                       e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
-                                                       e.d_leafc, kd, xr, e.d_ynw);
+        near_wm_kernel<AnyQ32, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                      e.d_bcount + lo, e.d_lx, e.d_ly, e.d_lz,
+                                                      e.d_leafc, kd, xr, e.d_ynw);
     } else {
       if (kd.q == kQPlugin)
         near_w64_kernel<Plugin, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
