@@ -12,8 +12,9 @@
 //             than 256 rows in 256-row ranges summed afterwards (reduce)
 //   phase 2   Y_k = U_k T_k     per <= 64 rows of a target box, 16x16x4 MFMA:
 //             A = 16 row quads x 4 columns of U, B = 4 columns x 16 RHS of T
-//   near      y_N = K_near X    lanes own target rows, W sums each, source
-//             points and their W values through wave-uniform (scalar) loads;
+//   near      y_N = K_near X    fp32 sums: entries of 4 sources x 64 targets
+//             per step on the VALU, multiplied on the matrix cores (sources
+//             staged in LDS); fp64 sums: lanes own target rows, W sums each;
 //             own buffer, on the handle's second stream under phases 1 and 2
 //   combine   y[:, j] = alpha * (sum_levels + y_N) + d * b[:, j] (an LDS transpose)
 //
@@ -526,51 +527,6 @@ __global__ __launch_bounds__(256) void near_w64_kernel(
   double* yp = yNw + (int64_t)ip * W;
 #pragma unroll
   for (int j = 0; j < W; ++j) yp[j] = kd.scale * acc[j];
-}
-
-// Near field with fp32 entries and sums: coordinates as fp32 offsets from the
-// leaf centre plus the fp32 difference of the two centres (as the single-RHS
-// fp32 near field).
-template <class KF, int W>
-__global__ __launch_bounds__(256) void near_w32_kernel(
-    const NearItem* __restrict__ items, int nitems, const int32_t* __restrict__ nbr,
-    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bcount,
-    const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
-    const double* __restrict__ leafc, KernelDev kd, const float* __restrict__ xr,
-    double* __restrict__ yNw) {
-  const int w = wave_id();
-  if (w >= nitems) return;
-  const int lane = threadIdx.x & 63;
-  const NearItem it = items[w];
-  const int rs = bstart[it.leaf];
-  const bool act = lane < it.nr;
-  const int ip = rs + it.i0 + (act ? lane : 0);
-  const float xi = lx[ip], yi = ly[ip], zi = lz[ip];
-  const double cxk = leafc[3 * (int64_t)it.leaf], cyk = leafc[3 * (int64_t)it.leaf + 1],
-               czk = leafc[3 * (int64_t)it.leaf + 2];
-  float acc[W];
-#pragma unroll
-  for (int j = 0; j < W; ++j) acc[j] = 0.0f;
-  for (int sl = 0; sl <= 26; ++sl) {
-    const int nn = sl < 26 ? nbr[(int64_t)it.leaf * 26 + sl] : it.leaf;
-    if (nn < 0) continue;
-    const int cs = bstart[nn], m = bcount[nn];
-    const float ax = xi + (float)(cxk - leafc[3 * (int64_t)nn]);
-    const float ay = yi + (float)(cyk - leafc[3 * (int64_t)nn + 1]);
-    const float az = zi + (float)(czk - leafc[3 * (int64_t)nn + 2]);
-    for (int s = cs; s < cs + m; ++s) {
-      const float dx = ax - lx[s], dy = ay - ly[s], dz = az - lz[s];
-      const float R2 = dx * dx + dy * dy + dz * dz;
-      const float k = s == ip ? 0.0f : KF::f(kd.q, R2);
-      const float* xs = xr + (int64_t)s * W;
-#pragma unroll
-      for (int j = 0; j < W; ++j) acc[j] += k * xs[j];
-    }
-  }
-  if (!act) return;
-  double* yp = yNw + (int64_t)ip * W;
-#pragma unroll
-  for (int j = 0; j < W; ++j) yp[j] = kd.scale * (double)acc[j];
 }
 
 // Near field with fp32 entries and sums on the matrix cores: Y(t, j) = sum_s
