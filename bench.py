@@ -314,10 +314,21 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
     t_diff = float((torch.linalg.norm(yt[:, :2] - yt1, dim=0) / torch.linalg.norm(yt1, dim=0)).max())
     rows = np.sort(np.random.default_rng(13).choice(N, size=1024, replace=False))
     errs = []
+    directs = {}
     for j in (0, nrhs - 1):
         d = h.direct(Q[:, j], rows).cpu().numpy()
+        directs[j] = d
         errs.append(float(np.linalg.norm(y[:, j].cpu().numpy()[rows] - d) / np.linalg.norm(d)))
     fbytes = st["factor_elems"] * 4
+    # the recompressed copy (budget 4e-7) in groups and one column at a time
+    mst = h.compress(4e-7)
+    ym = h.matvec(Q, prec="mixed")
+    mixed_ms = timed(torch, lambda: h.matvec(Q, prec="mixed"), steps, warmup)
+    h.set_rhs_block(0)
+    mixed_single_ms = timed(torch, lambda: h.matvec(Q, prec="mixed"), 1, 0)
+    h.set_rhs_block(tuned["width"])
+    merr = max(float(np.linalg.norm(ym[:, j].cpu().numpy()[rows] - d) / np.linalg.norm(d))
+               for j, d in directs.items())
     h.close()
     out = {"workload": "N=262144 iid uniform in [-1,1]^3, leaf1d 4 (nLevels 4), K=1/r, "
                        "ACA tol 1e-8, fp32 factors, fp32 accumulation",
@@ -331,6 +342,9 @@ def multi_rhs(torch, H3, nrhs, steps, warmup):
            "rel_diff_vs_one_at_a_time": diff, "rel_err_vs_direct": max(errs),
            "transpose_ms": round(t_ms, 4),
            "transpose_rel_diff_vs_one_at_a_time": t_diff,
+           "mixed": {"ms": round(mixed_ms, 4), "one_at_a_time_ms": round(mixed_single_ms, 3),
+                     "bytes_GB": round(mst["bytes"] / 1e9, 2), "budget": 4e-7,
+                     "rel_err_vs_direct": merr},
            "err_rows": len(rows), "assemble_s": round(t_asm, 2)}
     log(f"[bench] multi-rhs {out}")
     return out

@@ -78,7 +78,8 @@ void Engine::free_all() {
   dfree(d_ynw);
   dfree(d_iow);
   for (GroupTables& g : grp) g.free_all();
-  w_cap = io_w = 0;
+  release_mixed_groups(*this);
+  w_cap = io_w = tw_w = 0;
   for (auto& ev : this->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (own_stream) (void)hipStreamDestroy(own_stream);

@@ -809,6 +809,7 @@ void MixedCopy::free_all() {
 
 void release_mixed(Engine& e) {
   H3D_HIP(hipStreamSynchronize(e.stream));
+  release_mixed_groups(e);  // its tables point into the classes
   e.mix.free_all();
 }
 

@@ -350,6 +350,12 @@ struct Engine {
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
   GroupTables grp[2];               // grouped products: forward, transposed
+  GroupTables grpm[2];              // grouped products on the mixed copy's classes (forward)
+  int tw_w = 0;                     // W of the allocated d_tw
+  float* d_xw32 = nullptr;          // x by position in fp32, W per point (the fp16 class)
+  int xw32_w = 0;
+  void* d_mtw[2] = {nullptr, nullptr};  // the mixed classes' t and partial sums, W per slot
+  int64_t mtw_cap[2] = {0, 0};      // their allocated bytes
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
@@ -462,6 +468,8 @@ void load_engine(Engine& e, const char* path, int64_t ws_budget);
 int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64);
 void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64,
                   bool transpose);
+// Frees the grouped tables and scratch of the mixed copy (it is rebuilt or released).
+void release_mixed_groups(Engine& e);
 // times the grouped product of nrhs columns for every group width and
 // phase-1 item size, keeps the fastest; ms[8] (width-major), best[2]
 void autotune_rhs(Engine& e, int mode, int nrhs, int reps, double* ms, int* best);

@@ -27,6 +27,7 @@
 // MFMA, fp32/bf16 factors with fp32 accumulation the fp32 MFMA.
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "engine.hpp"
 #include "kernels.hpp"
@@ -99,6 +100,12 @@ __device__ inline void load4(const S* p, A (&o)[4]) {
     o[1] = (A)__uint_as_float(v.y);
     o[2] = (A)__uint_as_float(v.z);
     o[3] = (A)__uint_as_float(v.w);
+  } else if constexpr (std::is_same<S, _Float16>::value) {  // IEEE half
+    const u32x2a v = vload<NT, u32x2a>(p);
+    o[0] = (A)(float)__builtin_bit_cast(_Float16, (uint16_t)(v.x & 0xffffu));
+    o[1] = (A)(float)__builtin_bit_cast(_Float16, (uint16_t)(v.x >> 16));
+    o[2] = (A)(float)__builtin_bit_cast(_Float16, (uint16_t)(v.y & 0xffffu));
+    o[3] = (A)(float)__builtin_bit_cast(_Float16, (uint16_t)(v.y >> 16));
   } else {  // bf16: element 0 in the low half of each word
     const u32x2a v = vload<NT, u32x2a>(p);
     o[0] = (A)__uint_as_float(v.x << 16);
@@ -392,14 +399,17 @@ __global__ void reduce_w_kernel(const int4* __restrict__ cols, int64_t ncols,
 // the blocks with source s): column c of the item is column table entry
 // it.uoff + c (element offset | chunk << 48 in `ubase`'s chunks) with its t
 // index in p2t; otherwise column c of U at ubase[it.chunk] + it.uoff + c ld and
-// t index it.toff + c.
+// t index it.toff + c.  With `scale` (the mixed copy's classes) t of slot q is
+// multiplied by scale[q] as it is loaded; `accumulate` adds to the level sums
+// (the second class) instead of storing them.
 template <class S, class A, int W, bool CT = false>
 __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __restrict__ items, int nitems,
                                                   void* const* __restrict__ ubase,
                                                   const A* __restrict__ tw,
                                                   double* __restrict__ ylevw, int64_t N,
                                                   const int64_t* __restrict__ p2col,
-                                                  const int32_t* __restrict__ p2t) {
+                                                  const int32_t* __restrict__ p2t,
+                                                  const double* __restrict__ scale, int accumulate) {
   using M = Mfma<A>;
   using Acc = typename M::acc;
   constexpr int H = W / 16;
@@ -435,8 +445,9 @@ __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __
           for (int h = 0; h < H; ++h) tb[st][h] = tw[tslot * W + 16 * h + lc];
           load4<S, A>((const S*)ubase[(int)(cv >> 48)] + (cv & kColMask) + il, av[st]);
         } else {
+          const A sc = scale ? (A)scale[it.toff + cl] : (A)1;
 #pragma unroll
-          for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc];
+          for (int h = 0; h < H; ++h) tb[st][h] = tw[(int64_t)(it.toff + cl) * W + 16 * h + lc] * sc;
           load4<S, A>(U + (int64_t)cl * it.ld + il, av[st]);
         }
       }
@@ -486,7 +497,10 @@ __global__ __launch_bounds__(256, kWavesFor<A>) void p2w_kernel(const P2Item* __
         const int row = it.i0 + r0 + 4 * M::row(kq, a) + u;
         if (row < lim) {
 #pragma unroll
-          for (int h = 0; h < H; ++h) yb[(int64_t)row * W + 16 * h + lc] = (double)acc[u][h][a];
+          for (int h = 0; h < H; ++h) {
+            double* yp = yb + (int64_t)row * W + 16 * h + lc;
+            *yp = accumulate ? *yp + (double)acc[u][h][a] : (double)acc[u][h][a];
+          }
         }
       }
   }
@@ -840,32 +854,148 @@ void build_group_tables(Engine& e, int dir) {
   g.built = true;
 }
 
+// Tables of class c of the mixed copy (forward): phase 1 over the class's V
+// columns grouped by source box (V_s, one matrix per source in its V chunk),
+// writing t in the class's target order; phase 2 over each target box's
+// class U.  The columns' factors c_k are applied in phase 2 (`scale`).
+void build_mixed_group_tables(Engine& e, int c) {
+  GroupTables& g = e.grpm[c];
+  g.free_all();
+  const FactorClass& fc = e.mix.cls[c];
+  std::vector<int32_t> btoff(std::max<int64_t>(e.nblk_all, 1), 0);
+  std::vector<BlockDev> blk(std::max<int64_t>(e.nblk_all, 1));
+  if (e.nblk_all) {
+    H3D_HIP(hipMemcpy(btoff.data(), fc.d_btoff, sizeof(int32_t) * e.nblk_all, hipMemcpyDeviceToHost));
+    H3D_HIP(hipMemcpy(blk.data(), fc.d_blk, sizeof(BlockDev) * e.nblk_all, hipMemcpyDeviceToHost));
+  }
+  std::vector<GTItem> items[3];
+  P1Table tab;
+  tab.colv.reserve(fc.rank_sum + fc.rank_sum / 16);
+  tab.colo.reserve(fc.rank_sum + fc.rank_sum / 16);
+  const int gc = e.g_cols == 1024 ? 1024 : 2048;
+  const int64_t pbase = (int64_t)fc.T + 1;
+  std::vector<P2Item> p2;
+  for (int j = 1; j <= e.nL; ++j) {
+    const Level& lv = e.lev[j];
+    const LevelFactors& lf = fc.lf[j];
+    const int64_t lo = level_offset(j);
+    std::vector<int64_t> first(lv.nboxes + 1, 0);
+    for (int32_t k : lv.ilidx) ++first[k + 1];
+    for (int64_t k = 0; k < lv.nboxes; ++k) first[k + 1] += first[k];
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      if (n == 0) continue;
+      const int nsplit = (n + kGRows - 1) / kGRows;
+      const int64_t c0 = (int64_t)tab.colv.size();
+      const int64_t ldv = n + (n & 1);
+      for (int64_t q = lv.bbase + first[k]; q < lv.bbase + first[k + 1]; ++q) {
+        const int32_t b = e.by_src[q];
+        const int r = lf.rank[b - lv.bbase];
+        for (int cc = 0; cc < r; ++cc)
+          tab.add(blk[b].chunk, blk[b].voff + cc * ldv, nsplit, pbase, cc, r, btoff[b] + cc);
+        if (nsplit > 1 && r) tab.np += (int64_t)nsplit * r;
+      }
+      if (pbase + tab.np > INT32_MAX) throw Error(H3D_ERR_INVALID, "problem too large");
+      if ((int64_t)tab.colv.size() == c0) continue;
+      tab.pad(c0, fc.T);
+      p1_items(items, c0, (int64_t)tab.colv.size(), n, e.bstart[lo + k], gc);
+    }
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      const int R = lf.Rt[k];
+      if (n <= 0 || R <= 0) continue;
+      P2Item it{};
+      it.uoff = lf.uoff[k];
+      it.chunk = lv.chunk[k];
+      it.ld = class_ldu(n, fc.elem);
+      it.R = R;
+      it.toff = lf.toff[k];
+      it.yoff = e.bstart[lo + k];
+      it.level = j;
+      for (int i0 = 0; i0 < n; i0 += 128) {
+        it.i0 = i0;
+        it.nr = std::min(128, n - i0);
+        p2.push_back(it);
+      }
+    }
+  }
+  std::vector<GTItem> all;
+  for (int q = 0; q < 3; ++q) {
+    g.n_p1[q] = (int64_t)items[q].size();
+    all.insert(all.end(), items[q].begin(), items[q].end());
+  }
+  std::stable_sort(p2.begin(), p2.end(), [](const P2Item& a, const P2Item& b) {
+    return (int64_t)a.R * a.nr > (int64_t)b.R * b.nr;
+  });
+  g.n_gitems = (int64_t)all.size();
+  g.d_gitems = upload(all, e.stream);
+  g.d_gcolv = upload(tab.colv, e.stream);
+  g.d_gcolo = upload(tab.colo, e.stream);
+  g.n_rcols = (int64_t)tab.cols.size();
+  g.d_rcols = upload(tab.cols, e.stream);
+  g.g_np = tab.np;
+  g.n_gp2 = (int64_t)p2.size();
+  g.d_gp2 = upload(p2, e.stream);
+  H3D_HIP(hipStreamSynchronize(e.stream));
+  g.built = true;
+}
+
+// x (fp64), per-level y and near field of W right-hand sides per point.
+void ensure_w_bufs(Engine& e, int W) {
+  if (e.w_cap >= W) return;
+  dfree(e.d_xw);
+  dfree(e.d_ylevw);
+  dfree(e.d_ynw);
+  e.w_cap = 0;
+  e.d_xw = dalloc<double>(e.N * W);
+  e.d_ylevw = dalloc<double>(e.N * e.nL * W);
+  e.d_ynw = dalloc<double>(e.N * W);
+  e.w_cap = W;
+}
+
 void ensure_block_scratch(Engine& e, int W, int dir) {
   if (!e.grp[dir].built) build_group_tables(e, dir);
+  ensure_w_bufs(e, W);
+  W = e.w_cap;
   // t slots 0..T (T: spare for padded columns), then the partial sums (the
   // larger count of the directions built)
   int64_t np = 1;
   for (const GroupTables& g : e.grp) np = std::max(np, g.g_np);
   const int64_t T = e.T + 1;
-  if (e.w_cap >= W && e.w_t >= T && e.w_np >= np) return;
-  W = std::max(W, e.w_cap);
-  dfree(e.d_xw);
+  if (e.tw_w >= W && e.w_t >= T && e.w_np >= np) return;
   dfree(e.d_tw);
-  dfree(e.d_ylevw);
-  dfree(e.d_ynw);
-  e.w_cap = 0;
-  e.d_xw = dalloc<double>(e.N * W);
+  e.tw_w = 0;
   e.d_tw = dalloc<double>((T + np) * W);
-  e.d_ylevw = dalloc<double>(e.N * e.nL * W);
-  e.d_ynw = dalloc<double>(e.N * W);
-  e.w_cap = W;
+  e.tw_w = W;
   e.w_t = T;
   e.w_np = np;
 }
 
+void ensure_mixed_scratch(Engine& e, int W) {
+  for (int c = 0; c < 2; ++c)
+    if (!e.grpm[c].built) build_mixed_group_tables(e, c);
+  ensure_w_bufs(e, W);
+  W = e.w_cap;
+  if (e.xw32_w < W) {
+    dfree(e.d_xw32);
+    e.xw32_w = 0;
+    e.d_xw32 = dalloc<float>(e.N * W);
+    e.xw32_w = W;
+  }
+  for (int c = 0; c < 2; ++c) {
+    const int64_t need = ((int64_t)e.mix.cls[c].T + 1 + std::max<int64_t>(e.grpm[c].g_np, 1)) * W *
+                         (c == 0 ? (int64_t)sizeof(double) : (int64_t)sizeof(float));
+    if (e.mtw_cap[c] >= need) continue;
+    dfree(e.d_mtw[c]);
+    e.mtw_cap[c] = 0;
+    dmalloc(&e.d_mtw[c], (size_t)need);
+    e.mtw_cap[c] = need;
+  }
+}
+
 template <class S, class A, int W, int NC>
-void launch_p1(Engine& e, const GroupTables& g, void* const* base, int64_t first, int64_t count,
-               const A* xr, A* tw, hipStream_t st) {
+void launch_p1(Engine& e, const GroupTables& g, void* const* base, int nchunk, int64_t first,
+               int64_t count, const A* xr, A* tw, hipStream_t st) {
   if (count == 0) return;
   auto f = e.g_cols == 1024 ? p1r_kernel<S, A, W, NC, 1024 / NC> : p1r_kernel<S, A, W, NC, 2048 / NC>;
   const size_t lds = sizeof(A) * 64 * NC * W;  // x; the tables are static
@@ -874,7 +1004,7 @@ void launch_p1(Engine& e, const GroupTables& g, void* const* base, int64_t first
   if (lds + stat > 65536)
     H3D_HIP(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   f<<<(unsigned)count, 256, lds, st>>>((const GTItem*)g.d_gitems + first, (int)count, g.d_gcolv,
-                                       g.d_gcolo, base, (int)e.chunks.size(), xr, tw);
+                                       g.d_gcolo, base, nchunk, xr, tw);
   H3D_LAUNCH_CHECK();
 }
 
@@ -943,9 +1073,10 @@ void run_group(Engine& e, int prec, bool acc64, bool transpose, const double* b,
     launch_near(e.side);
     H3D_HIP(hipEventRecord(e.ev_join, e.side));
   }
-  launch_p1<S, A, W, 1>(e, g, p1base, 0, g.n_p1[0], xr, tw, st);
-  launch_p1<S, A, W, 2>(e, g, p1base, g.n_p1[0], g.n_p1[1], xr, tw, st);
-  launch_p1<S, A, W, 4>(e, g, p1base, g.n_p1[0] + g.n_p1[1], g.n_p1[2], xr, tw, st);
+  const int nch = (int)e.chunks.size();
+  launch_p1<S, A, W, 1>(e, g, p1base, nch, 0, g.n_p1[0], xr, tw, st);
+  launch_p1<S, A, W, 2>(e, g, p1base, nch, g.n_p1[0], g.n_p1[1], xr, tw, st);
+  launch_p1<S, A, W, 4>(e, g, p1base, nch, g.n_p1[0] + g.n_p1[1], g.n_p1[2], xr, tw, st);
   if (g.n_rcols) {
     reduce_w_kernel<A, W><<<(unsigned)((g.n_rcols * W + 255) / 256), 256, 0, st>>>(
         (const int4*)g.d_rcols, g.n_rcols, partw, tw);
@@ -954,10 +1085,12 @@ void run_group(Engine& e, int prec, bool acc64, bool transpose, const double* b,
   if (g.n_gp2) {
     if (transpose)
       p2w_kernel<S, A, W, true><<<wave_grid(g.n_gp2), 256, 0, st>>>(
-          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, g.d_p2col, g.d_p2t);
+          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, g.d_p2col, g.d_p2t,
+          nullptr, 0);
     else
       p2w_kernel<S, A, W><<<wave_grid(g.n_gp2), 256, 0, st>>>(
-          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, nullptr, nullptr);
+          (const P2Item*)g.d_gp2, (int)g.n_gp2, p2base, tw, e.d_ylevw, N, nullptr, nullptr,
+          nullptr, 0);
     H3D_LAUNCH_CHECK();
   }
   if (nn && !overlap) launch_near(st);
@@ -978,6 +1111,84 @@ void run_group_w(Engine& e, int prec, bool acc64, bool tr, const double* b, doub
   else run_group<uint16_t, float, W>(e, prec, acc64, tr, b, y, cnt);
 }
 
+// One group of W right-hand sides on the mixed copy: class 0 (fp32 factors,
+// fp64 sums and x), class 1 (fp16 factors, fp32 sums and x) each through
+// phase 1, the row-split reduce and phase 2 (scaled t; class 1 adds to the
+// level sums); the near field as the single product forms it (fp64 sums),
+// beside them on the second stream.
+template <int W>
+void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
+  hipStream_t st = e.stream;
+  const int64_t N = e.N;
+  double* x64 = e.d_xw;
+  float* x32 = e.d_xw32;
+  gather_w_kernel<double, W><<<(unsigned)((N * W + 255) / 256), 256, 0, st>>>(b, e.d_perm, N, cnt, x64);
+  H3D_LAUNCH_CHECK();
+  gather_w_kernel<float, W><<<(unsigned)((N * W + 255) / 256), 256, 0, st>>>(b, e.d_perm, N, cnt, x32);
+  H3D_LAUNCH_CHECK();
+  {
+    const int64_t n = N * e.nL * W;
+    zero_kernel<<<(unsigned)std::min<int64_t>(8192, (n + 255) / 256), 256, 0, st>>>(e.d_ylevw, n);
+    H3D_LAUNCH_CHECK();
+  }
+  const int nn = (int)e.nitems.size();
+  const bool overlap = nn && e.overlap_near && e.side;
+  auto launch_near = [&](hipStream_t ns) {
+    const int64_t lo = level_offset(e.nL);
+    const KernelDev kd = e.kern;
+    const unsigned gn = wave_grid(nn);
+    const int32_t* nbr = e.lev[e.nL].d_nbr;
+    if (kd.q == kQPlugin)
+      near_w64_kernel<Plugin, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                     e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                     x64, e.d_ynw);
+    else if (kd.q == 7 || kd.q == 16)
+      near_w64_kernel<InvRNewton, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                         e.d_bcount + lo, e.d_px, e.d_py, e.d_pz,
+                                                         kd, x64, e.d_ynw);
+    else
+      near_w64_kernel<AnyQ, W><<<gn, 256, 0, ns>>>(e.d_near, nn, nbr, e.d_bstart + lo,
+                                                   e.d_bcount + lo, e.d_px, e.d_py, e.d_pz, kd,
+                                                   x64, e.d_ynw);
+    H3D_LAUNCH_CHECK();
+  };
+  if (overlap) {
+    H3D_HIP(hipEventRecord(e.ev_fork, st));
+    H3D_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));
+    launch_near(e.side);
+    H3D_HIP(hipEventRecord(e.ev_join, e.side));
+  }
+  auto run_class = [&](auto SA, int c, const auto* x, auto* tw) {
+    using S = typename decltype(SA)::first_type;
+    using A = typename decltype(SA)::second_type;
+    const FactorClass& fc = e.mix.cls[c];
+    const GroupTables& g = e.grpm[c];
+    const int nch = (int)fc.vdata.size();
+    launch_p1<S, A, W, 1>(e, g, fc.d_vbase, nch, 0, g.n_p1[0], x, tw, st);
+    launch_p1<S, A, W, 2>(e, g, fc.d_vbase, nch, g.n_p1[0], g.n_p1[1], x, tw, st);
+    launch_p1<S, A, W, 4>(e, g, fc.d_vbase, nch, g.n_p1[0] + g.n_p1[1], g.n_p1[2], x, tw, st);
+    if (g.n_rcols) {
+      reduce_w_kernel<A, W><<<(unsigned)((g.n_rcols * W + 255) / 256), 256, 0, st>>>(
+          (const int4*)g.d_rcols, g.n_rcols, tw + ((int64_t)fc.T + 1) * W, tw);
+      H3D_LAUNCH_CHECK();
+    }
+    if (g.n_gp2) {
+      p2w_kernel<S, A, W><<<wave_grid(g.n_gp2), 256, 0, st>>>(
+          (const P2Item*)g.d_gp2, (int)g.n_gp2, fc.d_ubase, tw, e.d_ylevw, N, nullptr, nullptr,
+          e.mix.d_scale + fc.tbase, c);
+      H3D_LAUNCH_CHECK();
+    }
+  };
+  run_class(std::pair<float, double>{}, 0, (const double*)x64, (double*)e.d_mtw[0]);
+  run_class(std::pair<_Float16, float>{}, 1, (const float*)x32, (float*)e.d_mtw[1]);
+  if (nn && !overlap) launch_near(st);
+  if (overlap) H3D_HIP(hipStreamWaitEvent(st, e.ev_join, 0));
+  combine_w_kernel<W><<<(unsigned)((N + 63) / 64), 256, 0, st>>>(
+      e.d_ylevw, e.nL, N, nn ? e.d_ynw : nullptr, e.d_iperm, b, cnt, e.alpha, e.kern.diag,
+      e.d_diag, y);
+  H3D_LAUNCH_CHECK();
+}
+
 }  // namespace
 
 void GroupTables::free_all() {
@@ -995,20 +1206,46 @@ void GroupTables::free_all() {
 
 void release_group_tables(Engine& e) {
   for (GroupTables& g : e.grp) g.free_all();
+  for (GroupTables& g : e.grpm) g.free_all();
+}
+
+void release_mixed_groups(Engine& e) {
+  for (GroupTables& g : e.grpm) g.free_all();
+  for (int c = 0; c < 2; ++c) {
+    dfree(e.d_mtw[c]);
+    e.mtw_cap[c] = 0;
+  }
+  dfree(e.d_xw32);
+  e.xw32_w = 0;
 }
 
 int block_width(const Engine& e, int nrhs, int prec, bool transpose, bool acc64) {
-  if (prec == kPrecMixed || e.rhs_block <= 0 || nrhs < 2) return 1;
+  if (e.rhs_block <= 0 || nrhs < 2) return 1;
+  if (prec == kPrecMixed && transpose) return 1;  // the mixed copy's transposes: one at a time
   if ((int64_t)e.chunks.size() > kLdsChunks) return 1;  // phase 1 keeps the chunk bases in LDS
   // a plugin kernel: the grouped fp32-sum near field works on fp32 offsets of
-  // radial kernels, and a plugin's transpose needs entry(x_j, x_i)
-  if (e.kern.q == kQPlugin && (transpose || (prec != H3D_PREC_FP64 && !acc64))) return 1;
+  // radial kernels, and a plugin's transpose needs entry(x_j, x_i) (the mixed
+  // copy's grouped near field sums in fp64 with fp64 coordinates)
+  if (e.kern.q == kQPlugin && prec != kPrecMixed &&
+      (transpose || (prec != H3D_PREC_FP64 && !acc64)))
+    return 1;
   return nrhs <= 16 || e.rhs_block == 16 ? 16 : 32;
 }
 
 void matvec_block(Engine& e, const double* d_b, double* d_y, int nrhs, int prec, bool acc64,
                   bool transpose) {
   const int Wmax = e.rhs_block == 16 ? 16 : 32;
+  if (prec == kPrecMixed) {
+    ensure_mixed_scratch(e, nrhs <= 16 ? 16 : Wmax);
+    for (int r0 = 0; r0 < nrhs; r0 += Wmax) {
+      const int cnt = std::min(Wmax, nrhs - r0);
+      const double* b = d_b + (int64_t)r0 * e.N;
+      double* y = d_y + (int64_t)r0 * e.N;
+      if (cnt <= 16) run_group_mixed<16>(e, b, y, cnt);
+      else run_group_mixed<32>(e, b, y, cnt);
+    }
+    return;
+  }
   ensure_block_scratch(e, nrhs <= 16 ? 16 : Wmax, transpose ? 1 : 0);
   const int64_t N = e.N;
   for (int r0 = 0; r0 < nrhs; r0 += Wmax) {

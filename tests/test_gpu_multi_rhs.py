@@ -119,13 +119,40 @@ def test_grouped_transpose_is_the_adjoint(handle):
     assert np.max(np.abs(lhs - rhs) / np.abs(rhs)) <= 1e-12
 
 
-def test_mixed_stays_per_column(handle):
+def test_grouped_mixed_products(handle):
+    """The recompressed copy in groups: each class through the grouped phases
+    (fp32 factors with fp64 sums, fp16 factors with fp32 sums, scaled t), the
+    near field with fp64 coordinates and sums.  Columns agree with the single
+    mixed products (whose near field forms fp32 entries: ~6e-8 of y apart) and
+    stay within the copy's budget of the direct sum; repeated calls give the
+    same bits, widths 16 and 32 the same columns."""
+    x, h = handle
+    N = len(x)
+    Q = np.random.default_rng(14).standard_normal((N, 20))
+    tol = 1e-6
+    h.compress(tol)
+    ref = np.stack([h.matvec(Q[:, j], prec="mixed") for j in range(20)], axis=1)
+    h.set_rhs_block(32)
+    Y = h.matvec(Q, prec="mixed")
+    assert _relcols(Y, ref) <= 2e-7
+    assert np.array_equal(h.matvec(Q, prec="mixed"), Y)
+    h.set_rhs_block(16)
+    assert np.array_equal(h.matvec(Q, prec="mixed"), Y)
+    h.set_rhs_block(32)
+    rows = np.arange(0, N, 61)
+    for j in (0, 19):
+        d = h.direct(Q[:, j], rows)
+        assert np.linalg.norm(Y[rows, j] - d) <= 3 * tol * np.linalg.norm(d), j
+    h.release_precision("mixed")
+
+
+def test_mixed_transposes_stay_per_column(handle):
     x, h = handle
     Q = np.random.default_rng(5).standard_normal((len(x), 3))
     h.compress(1e-6)
-    YM = h.matvec(Q, prec="mixed")
+    YM = h.matvec(Q, prec="mixed", transpose=True)
     for j in range(3):
-        assert np.array_equal(YM[:, j], h.matvec(Q[:, j], prec="mixed"))
+        assert np.array_equal(YM[:, j], h.matvec(Q[:, j], prec="mixed", transpose=True))
     h.release_precision("mixed")
 
 
