@@ -57,7 +57,8 @@ class MixedStats(ctypes.Structure):
                 ("aca_rank_sum", _i64), ("rank_fp32", _i64), ("rank_fp16", _i64),
                 ("dropped", _i64), ("raw_blocks", _i64), ("raw_rank", _i64), ("bytes", _i64),
                 ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d),
-                ("single_jacobi", _i32)]
+                ("single_jacobi", _i32), ("formats", _i32), ("err_fp8", _d),
+                ("rank_fp8", _i64), ("u_elems_fp8", _i64), ("v_elems_fp8", _i64)]
 
 
 class Config(ctypes.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "h3d_autotune_rhs": (_i32, [_p, _i32, _i32, _i32, _p, _p]),
     "h3d_compress": (_i32, [_p, _d, ctypes.POINTER(MixedStats)]),
     "h3d_get_mixed_stats": (_i32, [_p, ctypes.POINTER(MixedStats)]),
+    "h3d_set_mixed_formats": (_i32, [_p, _i32]),
     "h3d_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_matvec_transpose": (_i32, [_p, _p, _p, _i32, _i32, _i32]),
     "h3d_set_diag": (_i32, [_p, _p, _i32]),

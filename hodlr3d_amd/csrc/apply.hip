@@ -115,6 +115,22 @@ __device__ inline void raw_cvt(const Raw<S, NV>& v, A (&o)[NV]) {
     } else {
       o[0] = (A)v;
     }
+  } else if constexpr (sizeof(S) == 1) {  // fp8: two values per conversion
+    uint32_t w[B >= 4 ? B / 4 : 1];
+    if constexpr (B == 16) {
+      w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
+    } else if constexpr (B == 8) {
+      w[0] = v.x, w[1] = v.y;
+    } else {
+      w[0] = (uint32_t)v;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q += 2) {
+      float a, b;
+      if (q & 2) f8x2_to_f<true>(w[q / 4], a, b);
+      else f8x2_to_f<false>(w[q / 4], a, b);
+      o[q] = (A)a, o[q + 1] = (A)b;
+    }
   } else {
     uint32_t w[B >= 4 ? B / 4 : 1];
     if constexpr (B == 16) {
@@ -307,11 +323,33 @@ __global__ void reduce_kernel(const P1Reduce* __restrict__ rs, int n, const A* _
   }
 }
 
+// VEC consecutive x values (fp32 or fp64; VEC * sizeof(A) a multiple of 16
+// bytes) by 16-byte loads from a 4-byte aligned address, kept in the cache
+// (x is read again by every item of the box).
+template <class A, int VEC>
+__device__ inline void load_xvec(const A* p, A (&o)[VEC]) {
+  constexpr int NW = VEC * (int)sizeof(A) / 16;
+  static_assert(NW * 16 == VEC * (int)sizeof(A), "whole 16-byte vectors");
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const u32x4a v = *(const __attribute__((address_space(1))) u32x4a*)((const char*)p + 16 * w);
+    if constexpr (sizeof(A) == 4) {
+      o[4 * w] = __uint_as_float(v.x);
+      o[4 * w + 1] = __uint_as_float(v.y);
+      o[4 * w + 2] = __uint_as_float(v.z);
+      o[4 * w + 3] = __uint_as_float(v.w);
+    } else {
+      o[2 * w] = __hiloint2double((int)v.y, (int)v.x);
+      o[2 * w + 1] = __hiloint2double((int)v.w, (int)v.z);
+    }
+  }
+}
+
 // ---------------------------------------------------------- mixed phase 1
 // t = V^T x for one class of the recompressed copy, grouped by source box: an
 // item is a run of columns of V_s (the columns of every block with source s,
-// concatenated) over at most 64 x 4 load vectors of rows (1024 fp32 or 2048
-// fp16 rows with 16-byte loads), so x is loaded once per item.  lpc lanes
+// concatenated) over at most 64 x KMAX load vectors of rows (1024 fp32, 2048
+// fp16 or 2048 fp8 rows with 16-byte loads), so x is loaded once per item.  lpc lanes
 // share a column, each holding K <= kmax load vectors of VEC rows (x in
 // registers), and reduce with an xor butterfly.  t is written in source order
 // (t_src[tsb + c], contiguous stores); a gather puts it in target order.  A
@@ -323,7 +361,8 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
                                                       const A* __restrict__ x,
                                                       A* __restrict__ tsrc, A* __restrict__ part,
                                                       int kmax) {
-  constexpr int KMAX = VEC >= 2 ? 4 : 8;  // 64 lanes x KMAX x VEC >= the item's rows
+  // 64 lanes x KMAX x VEC >= the item's rows (mix_item_rows)
+  constexpr int KMAX = VEC >= 16 ? 2 : (VEC >= 2 ? 4 : 8);
   const int w = wave_id();
   if (w >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -339,22 +378,25 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
   const int cpp = 64 / lpc, slot = lane / lpc, sub = lane % lpc;
   A xv[KMAX][VEC];
   const A* xs = x + it.xoff + it.s0;
-  // All loads first, at indices inside the item's rows, then a 0/1 factor for
-  // the entries outside (a select of the loaded value would be compiled into
-  // a branch around each load, each followed by a wait for it).
+  // x of the lane's rows, NX vectors (one for K == 1): all loads first, as
+  // 16-byte vectors; a vector starts inside the item's rows (lanes past them
+  // take the last one), so it ends less than VEC entries past them, in the
+  // next box's x or in the zero tail of at least 64 entries after x.  Then a
+  // 0/1 factor for the entries outside (a select of the loaded value would be
+  // compiled into a branch around each load, each followed by a wait for it).
+  const int slast = (rows - 1) / VEC * VEC;
+  auto load_x = [&](auto NXC) {
+    constexpr int NX = decltype(NXC)::value;
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int s = (sub + k * lpc) * VEC;
+    for (int k = 0; k < NX; ++k) load_xvec<A, VEC>(xs + min((sub + k * lpc) * VEC, slast), xv[k]);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int q = 0; q < VEC; ++q) xv[k][q] = xs[min(s + q, rows - 1)];
-  }
-  __builtin_amdgcn_sched_barrier(0);
+    for (int k = 0; k < NX; ++k) {
+      const int s = (sub + k * lpc) * VEC;
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int s = (sub + k * lpc) * VEC;
-#pragma unroll
-    for (int q = 0; q < VEC; ++q) xv[k][q] *= (k < K && s + q < rows) ? (A)1 : (A)0;
-  }
+      for (int q = 0; q < VEC; ++q) xv[k][q] *= (k < K && s + q < rows) ? (A)1 : (A)0;
+    }
+  };
   const S* base = (const S*)vbase[it.vchunk] + it.voff + it.s0;
   // The columns are taken in units (UP column passes with one load vector per
   // lane when K == 1, else one pass with KMAX vectors per lane), two units'
@@ -380,6 +422,7 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
   auto run = [&](auto ONE) {
     constexpr bool one = decltype(ONE)::value;
     constexpr int NL = one ? UP : KMAX;  // load vectors per lane and unit
+    load_x(std::integral_constant<int, one ? 1 : KMAX>{});
     int roff[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
@@ -607,15 +650,25 @@ __device__ __forceinline__ void phase2_item(const P2Item& it, bool valid, A* __r
   }
   if constexpr (QUEUE) __builtin_amdgcn_wave_barrier();  // a wave's LDS operations stay in order
   else __syncthreads();
-  if (!valid || multi || lane >= rcn) return;
+  if (!valid || multi) return;
+  double* y0 = ylev + (int64_t)(it.level - 1) * N + it.yoff + it.i0;
+  if (G > 1) {
+    // row o (held by lane o / VR of every group, entry o % VR) summed over
+    // the groups in group order by lane o % 64: every lane reads G
+    // consecutive-address values and the stores are contiguous
+    for (int o = lane; o < it.nr; o += 64) {
+      A s = red[o];
+      for (int g = 1; g < G; ++g) s += red[g * rcn * VR + o];
+      y0[o] = accumulate ? y0[o] + (double)s : (double)s;
+    }
+    return;
+  }
+  if (lane >= rcn) return;
 #pragma unroll
   for (int q = 0; q < VR; ++q) {
     const int row = lane * VR + q;
     if (row >= it.nr) break;
-    A s = acc[q];
-    for (int g = 1; g < G; ++g) s += red[(g * rcn + lane) * VR + q];
-    double* yp = ylev + (int64_t)(it.level - 1) * N + it.yoff + it.i0 + row;
-    *yp = accumulate ? *yp + (double)s : (double)s;
+    y0[row] = accumulate ? y0[row] + (double)acc[q] : (double)acc[q];
   }
 }
 
@@ -707,7 +760,7 @@ __global__ __launch_bounds__(256) void tphase2_kernel(const TP2Item* __restrict_
     const int b = by_src[it.sfirst + q];
     const BlockDev bd = blk[b];
     if (bd.r == 0) continue;
-    const int64_t ldv = bd.m + (bd.m & 1);
+    const int64_t ldv = class_ldv(bd.m, (int)sizeof(S));
     const S* col = (const S*)vbase[bd.chunk] + bd.voff + s;
     const A* tt = t + btoff[b];
     for (int l = 0; l < bd.r; ++l) acc += (A)Load<S>::f(col + l * ldv) * tt[l];
@@ -1296,8 +1349,8 @@ void build_plan(Engine& e) {
 void ensure_scratch(Engine& e, int nrhs) {
   int64_t T = e.T, np = e.plan.npartial;
   if (e.mix.ready) {
-    // target- and source-order t of both classes (class_t0 .. class_tsrc)
-    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T) + 2);
+    // target- and source-order t of every class (mix_t_layout)
+    T = std::max<int64_t>(T, mix_t_doubles(e.mix));
     for (const FactorClass& c : e.mix.cls)
       np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
   }
@@ -1441,26 +1494,25 @@ FactorView aca_view(Engine& e, int prec, bool acc64) {
           prec + (acc64 && prec != 0 ? 3 : 0), nullptr, nullptr};
 }
 
-// Class c of the mixed copy: class 0 (fp32 storage) accumulates in fp64 with t
-// at d_t[0, T0); class 1 (fp16 storage, a small part of the operator) works
-// in fp32 with its t as floats right after class 0's.
-// Scratch t layout of the mixed copy (in doubles): class 0 target order [0, T0),
-// class 1 target order (T1 floats) from T0 rounded up to even (16-byte aligned,
-// for the gather's vector stores), then the source-order copies: class 0's
-// T0 doubles from T0e + T1, class 1's T1 floats after them; at most
-// 2 (T0 + T1) + 1 doubles in all.
-inline int64_t class_t1_off(const Engine& e) { return (e.mix.cls[0].T + 1) & ~int64_t(1); }
+// Class c of the mixed copy: class 0 (fp32 storage) accumulates in fp64;
+// classes 1 and 2 (fp16 and fp8 storage, a small part of the operator) work
+// in fp32 with their t as floats.
+// Scratch t layout of the mixed copy (bytes from d_t, every part 16-byte
+// aligned for the gathers' vector stores): each class's target-order t, then
+// each class's source-order t (phase-1 output).
 inline double* class_t0(Engine& e) { return e.d_t; }
-inline float* class_t1(Engine& e) { return (float*)(e.d_t + class_t1_off(e)); }
+inline float* class_tf(Engine& e, int c) {
+  return (float*)((char*)e.d_t + mix_t_layout(e.mix).t[c]);
+}
 inline void* class_tsrc(Engine& e, int c) {
-  const int64_t T0 = e.mix.cls[0].T, T1 = e.mix.cls[1].T, o = class_t1_off(e) + T1;
-  return c == 0 ? (void*)(e.d_t + o) : (void*)(e.d_t + o + T0);
+  return (void*)((char*)e.d_t + mix_t_layout(e.mix).src[c]);
 }
 
+// Classes 1 and 2 share one tuning slot (their phases run with the same settings).
 FactorView class_view(Engine& e, int c) {
   const FactorClass& fc = e.mix.cls[c];
   return {&fc.plan, fc.plan.d_p2, (int)fc.plan.p2.size(), fc.d_blk, fc.d_btoff, fc.d_ubase,
-          fc.d_vbase, e.mix.d_scale + fc.tbase, c, 6 + c, &fc, class_tsrc(e, c)};
+          fc.d_vbase, e.mix.d_scale + fc.tbase, c, 6 + std::min(c, 1), &fc, class_tsrc(e, c)};
 }
 
 // Source-grouped phase 1 of a mixed class with a load width of `bytes` per lane.
@@ -1852,8 +1904,9 @@ void autotune(Engine& e, int mode, int reps, double* ms) {
   // timing does not depend on the values: x is whatever the last matvec left
   if (prec == kPrecMixed) {
     tune_mode<float, double>(e, class_view(e, 0), reps, e.d_xp, class_t0(e), e.d_part, ms);
-    tune_mode<_Float16, float>(e, class_view(e, 1), reps, e.d_xf, class_t1(e), (float*)e.d_part,
-                               ms ? ms + 8 : nullptr);
+    // the fp16 class's settings serve the fp8 class too (one slot)
+    tune_mode<_Float16, float>(e, class_view(e, 1), reps, e.d_xf, class_tf(e, 1),
+                               (float*)e.d_part, ms ? ms + 8 : nullptr);
   } else if (acc64) {
     const FactorView f = aca_view(e, prec, true);
     if (prec == H3D_PREC_FP64) tune_mode<double, double>(e, f, reps, e.d_xp, e.d_t, e.d_part, ms);
@@ -1945,18 +1998,24 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     }
     // event slots: 1 phase 1 start, 8 after the first copy's phase-1 kernel,
     // 6 after its phase 1 (reduce, gather), 9 after the second copy's phase-1
-    // kernel, 2 phase 2 start, 7 after the first copy's phase 2, 3 near start,
-    // 4 combine start, 5 end
+    // kernel, 10 after its phase 1, 12 after the third copy's phase-1 kernel,
+    // 2 phase 2 start, 7 after the first copy's phase 2, 11 after the
+    // second's, 3 near start, 4 combine start, 5 end (the copies: the mixed
+    // copy's classes; one copy in the other modes)
     for (int which = 1; which <= 2; ++which) {
       if (e.timing) H3D_HIP(hipEventRecord(e.ev[which], st));
       hipEvent_t m1 = (e.timing && which == 1) ? e.ev[8] : nullptr;
       hipEvent_t m2 = (e.timing && which == 1) ? e.ev[9] : nullptr;
+      hipEvent_t m3 = (e.timing && which == 1) ? e.ev[12] : nullptr;
       if (mixed) {
         run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_t0(e),
                                  e.d_part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
-        run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_t1(e),
+        run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_tf(e, 1),
                                    (float*)e.d_part, m2);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
+        run_phase<f8e4m3, float>(e, class_view(e, 2), which, transpose, e.d_xf, class_tf(e, 2),
+                                 (float*)e.d_part, m3);
       } else if (acc64) {
         const FactorView f = aca_view(e, prec, true);
         if (prec == H3D_PREC_FP64)
@@ -1967,6 +2026,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
           run_phase<uint16_t, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         if (m2) H3D_HIP(hipEventRecord(m2, st));
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
+        if (m3) H3D_HIP(hipEventRecord(m3, st));
       } else {
         const FactorView f = aca_view(e, prec, false);
         float* t = (float*)e.d_t;
@@ -1977,6 +2038,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
           run_phase<uint16_t, float>(e, f, which, transpose, e.d_xf, t, part, m1);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         if (m2) H3D_HIP(hipEventRecord(m2, st));
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
+        if (m3) H3D_HIP(hipEventRecord(m3, st));
       }
     }
     if (e.timing) H3D_HIP(hipEventRecord(e.ev[3], st));
@@ -1989,17 +2052,20 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     if (e.timing) {
       H3D_HIP(hipEventRecord(e.ev[5], st));
       H3D_HIP(hipEventSynchronize(e.ev[5]));
-      float ms[9];
+      float ms[12];
       H3D_HIP(hipEventElapsedTime(&ms[0], e.ev[1], e.ev[2]));  // phase 1 (+ partial reduce)
       H3D_HIP(hipEventElapsedTime(&ms[1], e.ev[2], e.ev[3]));  // phase 2
       H3D_HIP(hipEventElapsedTime(&ms[2], e.ev[3], e.ev[4]));  // near field
       H3D_HIP(hipEventElapsedTime(&ms[3], e.ev[4], e.ev[5]));  // combine
       H3D_HIP(hipEventElapsedTime(&ms[4], e.ev[0], e.ev[5]));  // whole matvec
-      H3D_HIP(hipEventElapsedTime(&ms[5], e.ev[6], e.ev[2]));  // phase 1 of the second copy
-      H3D_HIP(hipEventElapsedTime(&ms[6], e.ev[7], e.ev[3]));  // phase 2 of the second copy
-      H3D_HIP(hipEventElapsedTime(&ms[7], e.ev[1], e.ev[8]));  // first copy's phase-1 kernel
-      H3D_HIP(hipEventElapsedTime(&ms[8], e.ev[6], e.ev[9]));  // second copy's phase-1 kernel
-      for (int q = 0; q < 9; ++q) e.ktime[q] += ms[q];
+      H3D_HIP(hipEventElapsedTime(&ms[5], e.ev[6], e.ev[10]));  // phase 1 of the second copy
+      H3D_HIP(hipEventElapsedTime(&ms[6], e.ev[7], e.ev[11]));  // phase 2 of the second copy
+      H3D_HIP(hipEventElapsedTime(&ms[7], e.ev[1], e.ev[8]));   // first copy's phase-1 kernel
+      H3D_HIP(hipEventElapsedTime(&ms[8], e.ev[6], e.ev[9]));   // second copy's phase-1 kernel
+      H3D_HIP(hipEventElapsedTime(&ms[9], e.ev[10], e.ev[2]));  // phase 1 of the third copy
+      H3D_HIP(hipEventElapsedTime(&ms[10], e.ev[11], e.ev[3]));  // phase 2 of the third copy
+      H3D_HIP(hipEventElapsedTime(&ms[11], e.ev[10], e.ev[12]));  // third copy's phase-1 kernel
+      for (int q = 0; q < 12; ++q) e.ktime[q] += ms[q];
       e.ref_blas2_ms += ms[0] + ms[1];  // far field: the BLAS-2 applies
       e.ref_regen_ms += ms[2];          // near field: entries regenerated in the product
       e.kcount++;
@@ -2065,7 +2131,7 @@ void matvec_noalloc(Engine& e, const double* d_b, double* d_y, int mode) {
   // the sizes ensure_scratch gives the handle's own buffers for one vector
   int64_t T = e.T, np = e.plan.npartial;
   if (e.mix.ready) {
-    T = std::max<int64_t>(T, 2 * ((int64_t)e.mix.cls[0].T + e.mix.cls[1].T) + 2);
+    T = std::max<int64_t>(T, mix_t_doubles(e.mix));
     for (const FactorClass& c : e.mix.cls)
       np = std::max<int64_t>(np, std::max(c.plan.npartial, c.mpartial));
   }

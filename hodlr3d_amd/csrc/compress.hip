@@ -43,13 +43,17 @@ constexpr int kMaxEigRank = 142;          // larger blocks stay raw
 // fp16, 24 for fp32); the constants carry a 15% margin.
 constexpr double kFp16Err = 3.4e-4;
 constexpr double kFp32Err = 4.2e-8;
+constexpr double kFp8Err = 4.3e-2;         // e4m3: p = 4
+// fp8 columns are stored 8x larger than the other classes' (rms entry 8, not
+// 1): the format's normal range, [2^-6, 448], then holds entries from 1/500
+// to 56 times a column's rms (larger ones are clamped to 448)
+constexpr double kFp8Gain = 8.0;
+constexpr int kClasses = 3;
+constexpr int kClassElem[kClasses] = {4, 2, 1};
 constexpr double kCholFail = 1e-12;       // pivot / original diagonal below this: raw block
 constexpr int kMaxSweeps = 24;
 constexpr int kBinsPerOctave = 8;
 constexpr double kColShare = 1.0 / 4096;  // largest share of the error budget one column may take
-// rows of a mixed phase-1 item: 64 lanes x 4 vectors of 16 bytes (the phase-1
-// kernel's reach with 16-byte loads), 1024 fp32 or 2048 fp16 rows
-constexpr int kMixItemRowBytes = 64 * 4 * 16;
 constexpr int64_t kMixItemBytes = 131072;           // factor bytes of a mixed phase-1 item
 constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (whole source boxes)
 
@@ -65,12 +69,15 @@ struct CDesc {
 
 // Per-block output description (pass 2).
 struct PDesc {
-  int64_t u_out[2];  // class element offset of the block's first U column (in its U chunk)
-  int64_t v_out[2];  // class element offset of the block's V (in its V chunk)
-  int32_t vch[2];    // V chunk of the block per class
-  int32_t r32, r16;  // columns kept per class (prefix of the sorted columns)
+  int64_t u_out[3];  // class element offset of the block's first U column (in its U chunk)
+  int64_t v_out[3];  // class element offset of the block's V (in its V chunk)
+  int32_t vch[3];    // V chunk of the block per class
+  int32_t r32, r16, r8;  // columns kept per class (fp32, fp16, fp8: consecutive runs of the
+                         // sorted columns)
   int32_t raw;
-  int32_t ldu_o[2];  // U column stride per class (the V side keeps the fp64 copy's)
+  int32_t ldu_o[3];  // U column stride per class
+  int32_t ldv_o[3];  // V column stride per class
+  int32_t pad;
   double su, sv;     // power-of-two scales of the stored columns
 };
 
@@ -660,15 +667,19 @@ __global__ __launch_bounds__(256) void finish_kernel(const CDesc* __restrict__ d
 }
 
 // ------------------------------------------------------------------ project
+// Per class, the base pointers of its U chunks and V chunks.
+struct ClassBases {
+  void* const* u[kClasses];
+  void* const* v[kClasses];
+};
+
 // Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
-// and stored in the column's class (k < r32: fp32, else fp16).  Raw blocks
-// copy their columns unscaled into fp32.
+// and stored in the column's class (k < r32: fp32, then r16 columns in fp16,
+// then r8 in fp8).  Raw blocks copy their columns unscaled into fp32.
 template <class S>
 __global__ __launch_bounds__(256) void project_kernel(
     const PItem* __restrict__ items, const CDesc* __restrict__ ds, const PDesc* __restrict__ ps,
-    void* const* __restrict__ base64, const double* __restrict__ ws,
-    void* const* __restrict__ ubase32, void* const* __restrict__ vbase32,
-    void* const* __restrict__ ubase16, void* const* __restrict__ vbase16) {
+    void* const* __restrict__ base64, const double* __restrict__ ws, ClassBases cb) {
   constexpr int KT = 8;
   const PItem it = items[blockIdx.x];
   const CDesc d = ds[it.blk];
@@ -678,23 +689,27 @@ __global__ __launch_bounds__(256) void project_kernel(
   const int row = it.row0 + threadIdx.x;
   if (row >= rows) return;
   const int r = d.r;
-  const int ld = side ? d.ldv : d.ldu;  // the fp64 copy's stride (and the classes' V stride)
-  const int ld32 = side ? ld : p.ldu_o[0], ld16 = side ? ld : p.ldu_o[1];
+  const int ld = side ? d.ldv : d.ldu;  // the source copy's stride
+  const int* ldo = side ? p.ldv_o : p.ldu_o;
   const S* X = (const S*)base64[d.chunk] + (side ? d.xv : d.xu) + row;
   const double* W = ws + d.w;
   const double* T = W + (side ? (int64_t)r * r : 0);
   const double* lam = W + 2 * (int64_t)r * r;
   const double floor_ = lam[r + 1];
-  float* o32 = (float*)(side ? vbase32[p.vch[0]] : ubase32[d.chunk]) +
-               (side ? p.v_out[0] : p.u_out[0]) + row;
-  _Float16* o16 = p.r16 > 0 ? (_Float16*)(side ? vbase16[p.vch[1]] : ubase16[d.chunk]) +
-                                  (side ? p.v_out[1] : p.u_out[1]) + row
-                            : nullptr;
+  auto out = [&](int c) -> char* {
+    const int nk = c == 0 ? p.r32 : (c == 1 ? p.r16 : p.r8);
+    if (nk == 0) return nullptr;
+    return (char*)(side ? cb.v[c][p.vch[c]] : cb.u[c][d.chunk]) +
+           ((side ? p.v_out[c] : p.u_out[c]) + row) * kClassElem[c];
+  };
+  float* o32 = (float*)out(0);
+  _Float16* o16 = (_Float16*)out(1);
+  uint8_t* o8 = (uint8_t*)out(2);
   if (p.raw) {
-    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ld32] = (float)X[(int64_t)k * ld];  // S -> fp32
+    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ldo[0]] = (float)X[(int64_t)k * ld];  // S -> fp32
     return;
   }
-  const int keep = p.r32 + p.r16;
+  const int keep = p.r32 + p.r16 + p.r8;
   const double sc = side ? p.sv : p.su;
   for (int k0 = 0; k0 < keep; k0 += KT) {
     double acc[KT];
@@ -712,8 +727,16 @@ __global__ __launch_bounds__(256) void project_kernel(
       if (k >= keep) break;
       // U side: unit-norm column times su; V side: V' columns are orthonormal
       const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(fmax(lam[k], floor_)));
-      if (k < p.r32) o32[(int64_t)k * ld32] = (float)v;
-      else o16[(int64_t)(k - p.r32) * ld16] = (_Float16)v;
+      if (k < p.r32) {
+        o32[(int64_t)k * ldo[0]] = (float)v;
+      } else if (k < p.r32 + p.r16) {
+        o16[(int64_t)(k - p.r32) * ldo[1]] = (_Float16)v;
+      } else {
+        float f = (float)(v * kFp8Gain);
+        if (fabsf(f) > 448.0f) f = copysignf(448.0f, f);  // NaN passes
+        o8[(int64_t)(k - p.r32 - p.r16) * ldo[2]] =
+            (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(f, 0.0f, 0, false) & 0xff);
+      }
     }
   }
 }
@@ -1169,69 +1192,101 @@ void build_mixed(Engine& e, double tol) {
       for (double v : h) mx.fro2_near += v;
     }
   }
-  // thresholds: bins of log2(lam / (n + m)) at 1/8 octave
+  // Each kept column goes to one tier t of (drop, fp8, fp16, fp32) with
+  // bytes b_t per stored entry and squared relative error a_t (a_drop = 1).
+  // Spending the budget with the least bytes is a Lagrangian choice: a column
+  // of error per element lam / (n + m) takes the tier minimising
+  // b_t + mu a_t lam / (n + m), so the tiers are ranges of that ratio with
+  // fixed ratios between their thresholds, th = (b_t' - b_t) / (a_t - a_t')
+  // for consecutive tiers t < t' (the formats turned off are skipped); one
+  // search for the lowest threshold spends the budget.
   const int kBinLo = -1100 * kBinsPerOctave, kBins = 2200 * kBinsPerOctave;
   auto bin_of = [&](double lam, int nm) {
     if (!(lam > 0)) return 0;
     const double v = std::log2(lam / nm) * kBinsPerOctave;
     return (int)std::min<double>(kBins - 1, std::max<double>(0, std::floor(v) - kBinLo));
   };
-  const int delta = (int)std::lround(std::log2(1.0 / (kFp16Err * kFp16Err)) * kBinsPerOctave);
+  const double ta[4] = {1.0, kFp8Err * kFp8Err, kFp16Err * kFp16Err, kFp32Err * kFp32Err};
+  const double tb[4] = {0.0, 1.0, 2.0, 4.0};
+  int av[4], nav = 0;  // available tiers, increasing
+  av[nav++] = 0;
+  if (mx.formats & kMixFp8) av[nav++] = 1;
+  if (mx.formats & kMixFp16) av[nav++] = 2;
+  av[nav++] = 3;
+  // bin offsets of the thresholds above the lowest one
+  int toff[4] = {0, 0, 0, 0};
+  {
+    const double th0 = (tb[av[1]] - tb[av[0]]) / (ta[av[0]] - ta[av[1]]);
+    for (int i = 1; i + 1 < nav; ++i) {
+      const double th = (tb[av[i + 1]] - tb[av[i]]) / (ta[av[i]] - ta[av[i + 1]]);
+      toff[i] = (int)std::lround(std::log2(th / th0) * kBinsPerOctave);
+    }
+  }
   // No single column may carry more than kColShare of the budget (dropped, or
-  // as fp16 rounding): the added error is then a sum over thousands of
-  // independent terms and the error for one right-hand side stays close to
-  // its expectation.  Columns above a cap get the next higher precision.
+  // as rounding): the added error is then a sum over thousands of independent
+  // terms and the error for one right-hand side stays close to its
+  // expectation.  A column above a tier's cap takes the next available tier.
   const double budget = tol * tol * (mx.fro2_far + mx.fro2_near);
-  const double cap_drop = kColShare * budget;
-  const double cap_fp16 = kColShare * budget / (kFp16Err * kFp16Err);
+  const double cap = kColShare * budget;
+  auto min_tier = [&](double l) {  // index into av
+    int i = 0;
+    while (i + 1 < nav && ta[av[i]] * l > cap) ++i;
+    return i;
+  };
+  // index into av of the tier of a column in bin b with the lowest threshold at bin bd
+  auto bin_tier = [&](int b, int bd) {
+    int i = 0;
+    while (i + 1 < nav && b >= bd + toff[i]) ++i;
+    return i;
+  };
   int bd = 0;
   {
-    // Dd: droppable columns; Dh: columns that may be fp16 but not dropped;
-    // Dp: columns that stay fp32
-    std::vector<double> Dd(kBins + 1, 0.0), Dh(kBins + 1, 0.0);
-    double Dp = 0.0;
+    // H[m]: per bin, lam of the columns whose lowest allowed tier is av[m]
+    std::vector<double> H[4];
+    for (int m = 0; m < nav; ++m) H[m].assign(kBins + 1, 0.0);
     {  // per-chunk histograms, added in chunk order
-      std::vector<std::vector<double>> cd_(kHostChunks), ch_(kHostChunks);
-      std::vector<double> cp(kHostChunks, 0.0);
+      std::vector<std::vector<double>> ch_(kHostChunks * 4);
       host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
-        std::vector<double>& hd = cd_[c];
-        std::vector<double>& hh = ch_[c];
-        hd.assign(kBins + 1, 0.0);
-        hh.assign(kBins + 1, 0.0);
+        for (int m = 0; m < nav; ++m) ch_[c * 4 + m].assign(kBins + 1, 0.0);
         for (int64_t i = i0; i < i1; ++i) {
           if (raw[i]) continue;
           const int nm = cd[i].n + cd[i].m;
           for (int k = 0; k < cd[i].r; ++k) {
             const double l = lam_all[lam_off[i] + k];
-            if (l > cap_fp16) cp[c] += l;
-            else if (l > cap_drop) hh[bin_of(l, nm)] += l;
-            else hd[bin_of(l, nm)] += l;
+            ch_[c * 4 + min_tier(l)][bin_of(l, nm)] += l;
           }
         }
       });
-      for (int c = 0; c < kHostChunks; ++c) {
-        for (int b = 0; b <= kBins; ++b) Dd[b] += cd_[c][b], Dh[b] += ch_[c][b];
-        Dp += cp[c];
+      for (int c = 0; c < kHostChunks; ++c)
+        for (int m = 0; m < nav; ++m)
+          for (int b = 0; b <= kBins; ++b) H[m][b] += ch_[c * 4 + m][b];
+    }
+    std::vector<double> P[4];  // prefix sums
+    for (int m = 0; m < nav; ++m) {
+      P[m].assign(kBins + 1, 0.0);
+      for (int b = 0; b < kBins; ++b) P[m][b + 1] = P[m][b] + H[m][b];
+    }
+    // squared error per tier (index into av) with the lowest threshold at bd
+    auto parts = [&](int bdv, double* out) {
+      for (int i = 0; i < nav; ++i) out[i] = 0.0;
+      for (int m = 0; m < nav; ++m) {
+        int lo = 0;
+        for (int i = 0; i < nav; ++i) {
+          const int hi = i + 1 < nav ? std::min(kBins, std::max(lo, bdv + toff[i])) : kBins;
+          const int t = std::max(m, i);
+          out[t] += ta[av[t]] * (P[m][hi] - P[m][lo]);
+          lo = hi;
+        }
       }
-    }
-    std::vector<double> Pd(kBins + 1, 0.0), Ph(kBins + 1, 0.0);  // prefix sums
-    for (int b = 0; b < kBins; ++b) {
-      Pd[b + 1] = Pd[b] + Dd[b];
-      Ph[b + 1] = Ph[b] + Dh[b];
-    }
-    const double e16 = kFp16Err * kFp16Err, e32 = kFp32Err * kFp32Err;
-    auto parts = [&](int b, double* drop, double* f16, double* f32) {
-      const int h = std::min(kBins, b + delta);
-      *drop = Pd[b];
-      *f16 = e16 * (Pd[h] - Pd[b] + Ph[h]);
-      *f32 = e32 * (Pd[kBins] - Pd[h] + Ph[kBins] - Ph[h] + Dp);
     };
-    auto err = [&](int b) {
-      double a, c, d;
-      parts(b, &a, &c, &d);
-      return a + c + d;
+    auto err = [&](int bdv) {
+      double q[4];
+      parts(bdv, q);
+      double sum = 0.0;
+      for (int i = 0; i < nav; ++i) sum += q[i];
+      return sum;
     };
-    // largest b with err(b) <= budget (err is non-decreasing in b)
+    // largest bd with err(bd) <= budget (err is non-decreasing in bd)
     int lo = 0, hi = kBins;
     if (err(0) > budget) {
       bd = 0;
@@ -1244,11 +1299,18 @@ void build_mixed(Engine& e, double tol) {
       bd = lo;
     }
     mx.theta_drop = std::pow(2.0, (double)(bd + kBinLo) / kBinsPerOctave);
-    parts(bd, &mx.err2_drop, &mx.err2_fp16, &mx.err2_fp32);
+    double q[4];
+    parts(bd, q);
+    double by[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nav; ++i) by[av[i]] = q[i];
+    mx.err2_drop = by[0];
+    mx.err2_fp8 = by[1];
+    mx.err2_fp16 = by[2];
+    mx.err2_fp32 = by[3];
   }
-  // per block class ranks: the class only rises with lam, so each class is a
-  // prefix of the block's sorted columns
-  std::vector<int32_t> r32(nb), r16(nb);
+  // per block class ranks: the tier only rises with lam, so each class is a
+  // prefix of the block's sorted columns (fp32, then fp16, then fp8)
+  std::vector<int32_t> r32(nb), r16(nb), r8(nb);
   mx.aca_rank_sum = 0;
   mx.dropped = 0;
   mx.raw_blocks = 0;
@@ -1256,21 +1318,21 @@ void build_mixed(Engine& e, double tol) {
   host_chunks(nb, [&](int, int64_t i0, int64_t i1) {
     for (int64_t i = i0; i < i1; ++i) {
       const int r = cd[i].r;
+      r8[i] = 0;
       if (raw[i]) {
         r32[i] = r;
         r16[i] = 0;
         continue;
       }
       const int nm = cd[i].n + cd[i].m;
-      int keep = 0, k32 = 0;
+      int cnt[4] = {0, 0, 0, 0};
       for (int k = 0; k < r; ++k) {
         const double l = lam_all[lam_off[i] + k];
-        const int b = bin_of(l, nm);
-        keep += (b >= bd || l > cap_drop);
-        k32 += (b >= bd + delta || l > cap_fp16);
+        cnt[av[std::max(min_tier(l), bin_tier(bin_of(l, nm), bd))]]++;
       }
-      r32[i] = k32;
-      r16[i] = keep - k32;
+      r32[i] = cnt[3];
+      r16[i] = cnt[2];
+      r8[i] = cnt[1];
     }
   });
   for (int64_t i = 0; i < nb; ++i) {
@@ -1281,7 +1343,7 @@ void build_mixed(Engine& e, double tol) {
       mx.raw_rank += r;
       continue;
     }
-    mx.dropped += r - (r32[i] + r16[i]);
+    mx.dropped += r - (r32[i] + r16[i] + r8[i]);
   }
 
   // ---------------- class layouts
@@ -1291,9 +1353,9 @@ void build_mixed(Engine& e, double tol) {
   // chunks of whole source boxes.
   std::vector<int64_t> ci(e.nblk_all, -1);  // global block id -> index in cd
   for (int64_t i = 0; i < nb; ++i) ci[e.lev[refs[i].j].bbase + refs[i].b] = i;
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < kClasses; ++c) {
     FactorClass& fc = mx.cls[c];
-    fc.elem = c == 0 ? 4 : 2;
+    fc.elem = kClassElem[c];
     fc.lf.assign(nL + 1, LevelFactors());
     fc.nU.assign(e.chunks.size(), 0);
     fc.udata.assign(e.chunks.size(), nullptr);
@@ -1301,16 +1363,16 @@ void build_mixed(Engine& e, double tol) {
     fc.tmap.clear();
   }
   std::vector<PDesc> pd(nb);
-  int32_t tsrc_cur[2] = {0, 0};
-  std::vector<int32_t> vch[2];
-  for (int c = 0; c < 2; ++c) vch[c].assign(e.nblk_all, 0);
+  int32_t tsrc_cur[kClasses] = {0, 0, 0};
+  std::vector<int32_t> vch[kClasses];
+  for (int c = 0; c < kClasses; ++c) vch[c].assign(e.nblk_all, 0);
   {
-    int32_t T[2] = {0, 0};
+    int32_t T[kClasses] = {0, 0, 0};
     for (int j = 1; j <= nL; ++j) {
       const Level& lv = e.lev[j];
       const int64_t lo = level_offset(j);
       const int64_t nblk = (int64_t)lv.ilidx.size();
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < kClasses; ++c) {
         LevelFactors& f = mx.cls[c].lf[j];
         f.rank.assign(nblk, 0);
         f.col.assign(nblk, 0);
@@ -1322,13 +1384,13 @@ void build_mixed(Engine& e, double tol) {
       for (int64_t k = 0; k < lv.nboxes; ++k) {
         const int n = e.bcount[lo + k];
         const int ch = lv.chunk[k];
-        int32_t R[2] = {0, 0};
+        int32_t R[kClasses] = {0, 0, 0};
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
           const int64_t i = ci[lv.bbase + b];
           if (i < 0) continue;
           const int m = e.bcount[lo + lv.ilidx[b]];
-          const int rc[2] = {r32[i], r16[i]};
-          for (int c = 0; c < 2; ++c) {
+          const int rc[kClasses] = {r32[i], r16[i], r8[i]};
+          for (int c = 0; c < kClasses; ++c) {
             FactorClass& fc = mx.cls[c];
             LevelFactors& f = fc.lf[j];
             f.rank[b] = rc[c];
@@ -1340,11 +1402,13 @@ void build_mixed(Engine& e, double tol) {
           }
           pd[i].r32 = rc[0];
           pd[i].r16 = rc[1];
+          pd[i].r8 = rc[2];
+          for (int c = 0; c < kClasses; ++c) pd[i].ldv_o[c] = class_ldv(m, kClassElem[c]);
           pd[i].raw = raw[i];
           pd[i].su = raw[i] ? 1.0 : pow2_near_sqrt(n);
           pd[i].sv = raw[i] ? 1.0 : pow2_near_sqrt(m);
         }
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < kClasses; ++c) {
           FactorClass& fc = mx.cls[c];
           LevelFactors& f = fc.lf[j];
           f.Rt[k] = R[c];
@@ -1356,7 +1420,7 @@ void build_mixed(Engine& e, double tol) {
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
           const int64_t i = ci[lv.bbase + b];
           if (i < 0) continue;
-          for (int c = 0; c < 2; ++c) {
+          for (int c = 0; c < kClasses; ++c) {
             const LevelFactors& f = mx.cls[c].lf[j];
             const int ldu = class_ldu(n, mx.cls[c].elem);
             pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * ldu;
@@ -1370,14 +1434,14 @@ void build_mixed(Engine& e, double tol) {
       for (int32_t ki : lv.ilidx) nsrc[ki]++;
       for (int64_t k = 0; k < lv.nboxes; ++k)
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) tgt[b] = (int32_t)k;
-      for (int c = 0; c < 2; ++c) mx.cls[c].tmap.resize(T[c], 0);
+      for (int c = 0; c < kClasses; ++c) mx.cls[c].tmap.resize(T[c], 0);
       int64_t pos = lv.bbase;
       for (int64_t sb = 0; sb < lv.nboxes; ++sb) {
         const int m = e.bcount[lo + sb];
-        const int ld = m + (m & 1);
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < kClasses; ++c) {
           FactorClass& fc = mx.cls[c];
           LevelFactors& f = fc.lf[j];
+          const int ld = class_ldv(m, fc.elem);
           int64_t C = 0;
           for (int32_t q = 0; q < nsrc[sb]; ++q) C += f.rank[e.by_src[pos + q] - lv.bbase];
           if (fc.nV.back() > 0 && fc.nV.back() + C * ld > kVChunkElems) fc.nV.push_back(0);
@@ -1405,7 +1469,7 @@ void build_mixed(Engine& e, double tol) {
           // phase-1 items: rows in ranges of <= kMixRows, columns in runs of
           // about kMixItemBytes of factors
           if (m == 0 || C == 0) continue;
-          const int kMixRows = kMixItemRowBytes / fc.elem;
+          const int kMixRows = mix_item_rows(fc.elem);
           const int nsplit = (m + kMixRows - 1) / kMixRows;
           const int rows0 = std::min(m, kMixRows);
           const int ncol = (int)std::max<int64_t>(
@@ -1437,14 +1501,16 @@ void build_mixed(Engine& e, double tol) {
         pos += nsrc[sb];
       }
     }
-    mx.cls[0].T = T[0];
-    mx.cls[1].T = T[1];
-    mx.cls[0].tbase = 0;
-    mx.cls[1].tbase = T[0];
+    int32_t tb = 0;
+    for (int c = 0; c < kClasses; ++c) {
+      mx.cls[c].T = T[c];
+      mx.cls[c].tbase = tb;
+      tb += T[c];
+    }
   }
   // allocate and zero the class chunks: one allocation per class (a few large
   // allocations rather than one per chunk), each chunk at a 256-byte boundary
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < kClasses; ++c) {
     FactorClass& fc = mx.cls[c];
     fc.vdata.assign(fc.nV.size(), nullptr);
     auto span = [&](int64_t n) { return ((n + kChunkTail) * fc.elem + 255) / 256 * 256; };
@@ -1476,7 +1542,7 @@ void build_mixed(Engine& e, double tol) {
   // sweeps again), then project into the classes; blocks that keep no column
   // and raw blocks need no eigen step
   std::vector<uint8_t> work(nb, 0);
-  for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && (pd[i].r32 + pd[i].r16) > 0;
+  for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && (pd[i].r32 + pd[i].r16 + pd[i].r8) > 0;
   {
     int64_t i0 = 0;
     while (i0 < nb) {
@@ -1488,17 +1554,22 @@ void build_mixed(Engine& e, double tol) {
       PDesc* d_pd = upload(sub, st);
       std::vector<PItem> items;
       for (int64_t i = i0; i < i1; ++i) {
-        if (pd[i].r32 + pd[i].r16 == 0) continue;
+        if (pd[i].r32 + pd[i].r16 + pd[i].r8 == 0) continue;
         for (int side = 0; side < 2; ++side) {
           const int rows = side ? cd[i].m : cd[i].n;
           for (int r0 = 0; r0 < rows; r0 += 256) items.push_back({(int32_t)(i - i0), side, r0, 0});
         }
       }
       PItem* d_it = upload(items, st);
+      ClassBases cb{};
+      for (int c = 0; c < kClasses; ++c) {
+        cb.u[c] = mx.cls[c].d_ubase;
+        cb.v[c] = mx.cls[c].d_vbase;
+      }
       for (int64_t s = 0; s < (int64_t)items.size(); s += 1 << 30) {
         const unsigned g = (unsigned)std::min<int64_t>(1 << 30, (int64_t)items.size() - s);
-        (src ? project_kernel<float> : project_kernel<double>)<<<g, 256, 0, st>>>(d_it + s, d_cd, d_pd, d_b64, d_ws, mx.cls[0].d_ubase,
-                                          mx.cls[0].d_vbase, mx.cls[1].d_ubase, mx.cls[1].d_vbase);
+        (src ? project_kernel<float> : project_kernel<double>)<<<g, 256, 0, st>>>(
+            d_it + s, d_cd, d_pd, d_b64, d_ws, cb);
         H3D_LAUNCH_CHECK();
       }
       H3D_HIP(hipStreamSynchronize(st));
@@ -1512,12 +1583,13 @@ void build_mixed(Engine& e, double tol) {
   dfree(d_list);
   dfree(d_b64);
   // per column factor c_k = nu_k nv_k / (su sv) = sqrt(lam_k) / (su sv) in t
-  // order (raw columns: 1), block tables and plans of both classes
+  // order (fp8 columns: / kFp8Gain^2; raw columns: 1), block tables and plans
+  // of every class
   {
-    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T, 1.0);
-    std::vector<BlockDev> blk[2];
-    std::vector<int32_t> btoff[2];
-    for (int c = 0; c < 2; ++c) {
+    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T + mx.cls[2].T, 1.0);
+    std::vector<BlockDev> blk[kClasses];
+    std::vector<int32_t> btoff[kClasses];
+    for (int c = 0; c < kClasses; ++c) {
       blk[c].assign(e.nblk_all, BlockDev{});
       btoff[c].assign(e.nblk_all, 0);
     }
@@ -1527,7 +1599,7 @@ void build_mixed(Engine& e, double tol) {
       for (int64_t k = 0; k < lv.nboxes; ++k)
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
           const int32_t ki = lv.ilidx[b];
-          for (int c = 0; c < 2; ++c) {
+          for (int c = 0; c < kClasses; ++c) {
             const FactorClass& fc = mx.cls[c];
             const LevelFactors& f = fc.lf[j];
             BlockDev d{};
@@ -1541,34 +1613,37 @@ void build_mixed(Engine& e, double tol) {
             btoff[c][lv.bbase + b] = f.toff[k] + f.col[b];
           }
           const int64_t i = ci[lv.bbase + b];
-          if (i < 0) continue;
-          if (!raw[i]) {
-            const double inv = 1.0 / (pd[i].su * pd[i].sv);
-            const LevelFactors& f0 = mx.cls[0].lf[j];
-            const LevelFactors& f1 = mx.cls[1].lf[j];
-            const double fl = lam_floor[i];
-            for (int kk = 0; kk < r32[i]; ++kk)
-              scale[(size_t)f0.toff[k] + f0.col[b] + kk] =
-                  std::sqrt(std::max(lam_all[lam_off[i] + kk], fl)) * inv;
-            for (int kk = 0; kk < r16[i]; ++kk)
-              scale[(size_t)mx.cls[1].tbase + f1.toff[k] + f1.col[b] + kk] =
-                  std::sqrt(std::max(lam_all[lam_off[i] + r32[i] + kk], fl)) * inv;
+          if (i < 0 || raw[i]) continue;
+          const double inv = 1.0 / (pd[i].su * pd[i].sv);
+          const double fl = lam_floor[i];
+          const int rc[kClasses] = {r32[i], r16[i], r8[i]};
+          int k0 = 0;  // first sorted column of the class
+          for (int c = 0; c < kClasses; ++c) {
+            const LevelFactors& f = mx.cls[c].lf[j];
+            const double g = c == 2 ? inv / (kFp8Gain * kFp8Gain) : inv;
+            for (int kk = 0; kk < rc[c]; ++kk)
+              scale[(size_t)mx.cls[c].tbase + f.toff[k] + f.col[b] + kk] =
+                  std::sqrt(std::max(lam_all[lam_off[i] + k0 + kk], fl)) * g;
+            k0 += rc[c];
           }
         }
     }
     mx.d_scale = upload(scale, st);
-    // class 1 sums in fp32: its factors converted once (the value the gather
-    // would convert to), held until the stream synchronizes
-    std::vector<float> scale32(mx.cls[1].T);
-    for (int32_t q = 0; q < mx.cls[1].T; ++q) scale32[q] = (float)scale[(size_t)mx.cls[1].tbase + q];
-    mx.cls[1].d_scale32 = upload(scale32, st);
-    for (int c = 0; c < 2; ++c) {
+    // classes 1 and 2 sum in fp32: their factors converted once (the value
+    // the gather would convert to), held until the stream synchronizes
+    for (int c = 1; c < kClasses; ++c) {
+      std::vector<float> scale32(mx.cls[c].T);
+      for (int32_t q = 0; q < mx.cls[c].T; ++q)
+        scale32[q] = (float)scale[(size_t)mx.cls[c].tbase + q];
+      mx.cls[c].d_scale32 = upload(scale32, st);
+    }
+    for (int c = 0; c < kClasses; ++c) {
       FactorClass& fc = mx.cls[c];
       fc.d_blk = upload(blk[c], st);
       fc.d_btoff = upload(btoff[c], st);
       std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
       for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
-      build_plan_for(e, lfs, fc.plan, c == 0 ? 256 : 512, fc.elem);  // fp32 / fp16 storage
+      build_plan_for(e, lfs, fc.plan, 1024 / fc.elem, fc.elem);  // 1 KB of rows per phase-2 item
       fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
       fc.plan.p1r.clear();
       fc.d_mp1 = upload(fc.mp1, st);
@@ -1577,6 +1652,7 @@ void build_mixed(Engine& e, double tol) {
     }
     H3D_HIP(hipStreamSynchronize(st));
   }
+  mx.built_formats = mx.formats;
   mx.ready = true;
   mx.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -198,13 +198,13 @@ struct Plan {
   void free_device();
 };
 
-// One storage class of the recompressed copy (fp32 or fp16 columns).  Columns
+// One storage class of the recompressed copy (fp32, fp16 or fp8 columns).  Columns
 // are stored scaled: the class's U and V columns hold u_k * su / nu_k and
 // v_k * sv / nv_k (su, sv powers of two near sqrt(n), sqrt(m)), and the
 // factor c_k = nu_k nv_k / (su sv) multiplies t_k after phase 1.  Chunk ids
 // and the block/box structure are those of the ACA copy.
 struct FactorClass {
-  int elem = 4;                      // bytes per stored element (4: fp32, 2: fp16)
+  int elem = 4;                      // bytes per stored element (4: fp32, 2: fp16, 1: fp8)
   std::vector<LevelFactors> lf;      // [level]; voff is relative to the block's V chunk
   std::vector<void*> udata, vdata;   // U chunks (the ACA copy's chunk ids), V chunks
   void* arena = nullptr;             // one allocation holding every chunk of the class
@@ -223,7 +223,7 @@ struct FactorClass {
   MixP1Item* d_mp1 = nullptr;
   MixReduce* d_mp1r = nullptr;
   int32_t* d_tmap = nullptr;
-  float* d_scale32 = nullptr;        // class 1: c_k in target order as fp32 (its sums' type)
+  float* d_scale32 = nullptr;        // classes 1, 2: c_k in target order as fp32 (their sums' type)
   int32_t mpartial = 0;              // partial sums of the phase-1 items
   int32_t T = 0;                     // t entries of this class
   int32_t tbase = 0;                 // offset of this class in the shared t / scale arrays
@@ -231,20 +231,53 @@ struct FactorClass {
   void free_all();
 };
 
+// Rows of a mixed phase-1 item for entries of `elem` bytes: the kernel's reach
+// with 16-byte loads, 64 lanes x 4 vectors (x 2 for 1-byte entries, whose
+// vectors of 16 rows would otherwise hold 64 x values per lane in registers).
+inline int mix_item_rows(int elem) { return elem == 1 ? 64 * 2 * 16 : 64 * 4 * 16 / elem; }
+
+// Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats).
+constexpr int kMixFp16 = 1, kMixFp8 = 2;
+
 // Recompressed mixed-precision copy: SVD-truncated factors of every block with
-// columns in fp32 or fp16 chosen against a global Frobenius error budget.
+// columns in fp32, fp16 or fp8 chosen against a global Frobenius error budget.
 struct MixedCopy {
   bool ready = false;
   double tol = 0;                    // requested relative Frobenius error
-  FactorClass cls[2];                // 0: fp32 columns, 1: fp16 columns
-  double* d_scale = nullptr;         // c_k per t entry (class 0 then class 1)
+  int formats = kMixFp16 | kMixFp8;  // formats the next build may use
+  int built_formats = 0;             // those the copy in place could use
+  FactorClass cls[3];                // 0: fp32 columns, 1: fp16 columns, 2: fp8 (e4m3) columns
+  double* d_scale = nullptr;         // c_k per t entry (class 0, then 1, then 2)
   // statistics of the last build
-  double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0;
+  double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0, err2_fp8 = 0;
   double theta_drop = 0, ms = 0;
   bool single_jacobi = false;        // the build kept J between its passes (one Jacobi per block)
   int64_t aca_rank_sum = 0, dropped = 0, raw_blocks = 0, raw_rank = 0;
   void free_all();
 };
+
+// Scratch t layout of the mixed copy's product (bytes from the t buffer,
+// every part 16-byte aligned for vector stores): each class's target-order t
+// (class 0 in doubles, classes 1 and 2 in floats), then each class's
+// source-order t (phase-1 output).
+struct MixT {
+  int64_t t[3], src[3], end;
+};
+inline MixT mix_t_layout(const MixedCopy& m) {
+  MixT L{};
+  int64_t o = 0;
+  auto put = [&](int64_t bytes) {
+    const int64_t at = o;
+    o = (o + bytes + 15) / 16 * 16;
+    return at;
+  };
+  for (int c = 0; c < 3; ++c) L.t[c] = put((int64_t)m.cls[c].T * (c == 0 ? 8 : 4));
+  for (int c = 0; c < 3; ++c) L.src[c] = put((int64_t)m.cls[c].T * (c == 0 ? 8 : 4));
+  L.end = o;
+  return L;
+}
+// doubles of t scratch the mixed copy's product needs
+inline int64_t mix_t_doubles(const MixedCopy& m) { return mix_t_layout(m).end / 8 + 2; }
 
 // Work tables of the grouped products (several right-hand sides per factor
 // read, multi.hip) in one direction.  Phase 1 takes a column table: forward,
@@ -350,12 +383,12 @@ struct Engine {
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
   GroupTables grp[2];               // grouped products: forward, transposed
-  GroupTables grpm[2];              // grouped products on the mixed copy's classes (forward)
+  GroupTables grpm[3];              // grouped products on the mixed copy's classes (forward)
   int tw_w = 0;                     // W of the allocated d_tw
   float* d_xw32 = nullptr;          // x by position in fp32, W per point (the fp16 class)
   int xw32_w = 0;
-  void* d_mtw[2] = {nullptr, nullptr};  // the mixed classes' t and partial sums, W per slot
-  int64_t mtw_cap[2] = {0, 0};      // their allocated bytes
+  void* d_mtw[3] = {nullptr, nullptr, nullptr};  // the mixed classes' t and partial sums, W per slot
+  int64_t mtw_cap[3] = {0, 0, 0};   // their allocated bytes
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
@@ -368,8 +401,8 @@ struct Engine {
   int tune_k[8] = {4, 4, 4, 4, 4, 4, 4, 4};
   // per-kernel event timing of matvecs
   bool timing = false;
-  hipEvent_t ev[10] = {};
-  double ktime[9] = {};
+  hipEvent_t ev[13] = {};
+  double ktime[12] = {};
   int kcount = 0;
   // reference-semantics timers (never reset), fed by the per-kernel timing
   double ref_regen_ms = 0, ref_blas2_ms = 0;
@@ -403,7 +436,13 @@ inline int64_t aca_ustart(int64_t off, int n) { return n > 256 ? (off + 31) / 32
 // up to even): one item reads the box's columns as one region.
 inline int class_ldu(int n, int elem) {
   const int line = 128 / elem;
-  return n * elem > 1024 ? (n + line - 1) / line * line : n + (n & 1);
+  return n * elem > 1024 ? (n + line - 1) / line * line : (elem == 1 ? (n + 3) & ~3 : n + (n & 1));
+}
+// Column stride of V for a source box of m rows (the ACA copies and the
+// recompressed classes; elem bytes per entry): even, a multiple of 4 for 1-byte
+// entries, so that every column starts 4-byte aligned for vector loads.
+__host__ __device__ inline int class_ldv(int m, int elem) {
+  return elem == 1 ? (m + 3) & ~3 : m + (m & 1);
 }
 // Element offset (in its chunk) of the U of such a box placed at `off` or after.
 inline int64_t class_ustart(int64_t off, int n, int elem) {

@@ -100,6 +100,12 @@ __device__ inline void load4(const S* p, A (&o)[4]) {
     o[1] = (A)__uint_as_float(v.y);
     o[2] = (A)__uint_as_float(v.z);
     o[3] = (A)__uint_as_float(v.w);
+  } else if constexpr (sizeof(S) == 1) {  // fp8 (e4m3)
+    const uint32_t v = vload<NT, uint32_t>(p);
+    float a, b, c, d;
+    f8x2_to_f<false>(v, a, b);
+    f8x2_to_f<true>(v, c, d);
+    o[0] = (A)a, o[1] = (A)b, o[2] = (A)c, o[3] = (A)d;
   } else if constexpr (std::is_same<S, _Float16>::value) {  // IEEE half
     const u32x2a v = vload<NT, u32x2a>(p);
     o[0] = (A)(float)__builtin_bit_cast(_Float16, (uint16_t)(v.x & 0xffffu));
@@ -887,7 +893,7 @@ void build_mixed_group_tables(Engine& e, int c) {
       if (n == 0) continue;
       const int nsplit = (n + kGRows - 1) / kGRows;
       const int64_t c0 = (int64_t)tab.colv.size();
-      const int64_t ldv = n + (n & 1);
+      const int64_t ldv = class_ldv(n, fc.elem);
       for (int64_t q = lv.bbase + first[k]; q < lv.bbase + first[k + 1]; ++q) {
         const int32_t b = e.by_src[q];
         const int r = lf.rank[b - lv.bbase];
@@ -972,7 +978,7 @@ void ensure_block_scratch(Engine& e, int W, int dir) {
 }
 
 void ensure_mixed_scratch(Engine& e, int W) {
-  for (int c = 0; c < 2; ++c)
+  for (int c = 0; c < 3; ++c)
     if (!e.grpm[c].built) build_mixed_group_tables(e, c);
   ensure_w_bufs(e, W);
   W = e.w_cap;
@@ -982,7 +988,7 @@ void ensure_mixed_scratch(Engine& e, int W) {
     e.d_xw32 = dalloc<float>(e.N * W);
     e.xw32_w = W;
   }
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 3; ++c) {
     const int64_t need = ((int64_t)e.mix.cls[c].T + 1 + std::max<int64_t>(e.grpm[c].g_np, 1)) * W *
                          (c == 0 ? (int64_t)sizeof(double) : (int64_t)sizeof(float));
     if (e.mtw_cap[c] >= need) continue;
@@ -1112,10 +1118,10 @@ void run_group_w(Engine& e, int prec, bool acc64, bool tr, const double* b, doub
 }
 
 // One group of W right-hand sides on the mixed copy: class 0 (fp32 factors,
-// fp64 sums and x), class 1 (fp16 factors, fp32 sums and x) each through
-// phase 1, the row-split reduce and phase 2 (scaled t; class 1 adds to the
-// level sums); the near field as the single product forms it (fp64 sums),
-// beside them on the second stream.
+// fp64 sums and x), classes 1 and 2 (fp16 and fp8 factors, fp32 sums and x)
+// each through phase 1, the row-split reduce and phase 2 (scaled t; classes
+// 1 and 2 add to the level sums); the near field as the single product forms
+// it (fp64 sums), beside them on the second stream.
 template <int W>
 void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
   hipStream_t st = e.stream;
@@ -1181,6 +1187,7 @@ void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
   };
   run_class(std::pair<float, double>{}, 0, (const double*)x64, (double*)e.d_mtw[0]);
   run_class(std::pair<_Float16, float>{}, 1, (const float*)x32, (float*)e.d_mtw[1]);
+  run_class(std::pair<f8e4m3, float>{}, 2, (const float*)x32, (float*)e.d_mtw[2]);
   if (nn && !overlap) launch_near(st);
   if (overlap) H3D_HIP(hipStreamWaitEvent(st, e.ev_join, 0));
   combine_w_kernel<W><<<(unsigned)((N + 63) / 64), 256, 0, st>>>(
@@ -1211,7 +1218,7 @@ void release_group_tables(Engine& e) {
 
 void release_mixed_groups(Engine& e) {
   for (GroupTables& g : e.grpm) g.free_all();
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < 3; ++c) {
     dfree(e.d_mtw[c]);
     e.mtw_cap[c] = 0;
   }

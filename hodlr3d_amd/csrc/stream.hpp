@@ -74,6 +74,20 @@ __global__ void zero_kernel(double* __restrict__ p, int64_t n) {
 typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u32x2a __attribute__((ext_vector_type(2), aligned(4)));
 
+// A stored 8-bit float: OCP e4m3 (1 sign, 4 exponent bits of bias 7, 3
+// mantissa bits; largest finite 448), gfx950's fp8 conversion format.
+struct f8e4m3 {
+  uint8_t bits;
+};
+
+// Two fp8 values of a 32-bit word (HI: bytes 2-3) as floats.
+template <bool HI>
+__device__ inline void f8x2_to_f(uint32_t w, float& a, float& b) {
+  const auto v = __builtin_amdgcn_cvt_pk_f32_fp8((int)w, HI);
+  a = v[0];
+  b = v[1];
+}
+
 template <class S>
 struct Load;
 template <>
@@ -87,6 +101,12 @@ struct Load<float> {
 template <>
 struct Load<uint16_t> {
   __device__ static inline float f(const uint16_t* p) { return bf16_to_f(ntload<uint16_t>(p)); }
+};
+template <>
+struct Load<f8e4m3> {
+  __device__ static inline float f(const f8e4m3* p) {
+    return __builtin_amdgcn_cvt_f32_fp8((int)ntload<uint8_t>(p), 0);
+  }
 };
 template <>
 struct Load<_Float16> {

@@ -316,11 +316,25 @@ class HODLR3D:
             out["phase2_fp16"] = {w[i]: ms[12 + i] for i in range(4) if ms[12 + i] >= 0}
         return out
 
-    def compress(self, tol=None):
+    def set_mixed_formats(self, formats=("fp16", "fp8")):
+        """Formats the next build of the mixed copy may store columns in besides
+        fp32: any of 'fp16' and 'fp8' (h3d_set_mixed_formats)."""
+        bits = {"fp16": MIXED_FP16, "fp8": MIXED_FP8}
+        mask = 0
+        for f in formats:
+            if f not in bits:
+                raise nat.H3DError(nat.H3D_ERR_INVALID, f"unknown mixed-copy format {f!r}")
+            mask |= bits[f]
+        nat.check(nat.lib().h3d_set_mixed_formats(self._h, mask))
+
+    def compress(self, tol=None, formats=None):
         """Build the recompressed mixed-precision copy ('mixed') with an expected
-        added relative error of `tol` (default 5 * 10^-TOL_POW); returns its stats."""
+        added relative error of `tol` (default 5 * 10^-TOL_POW); returns its stats.
+        `formats` (optional) sets the storage formats first (set_mixed_formats)."""
         if tol is None:
             tol = 5.0 * 10.0 ** (-self.TOL_POW)
+        if formats is not None:
+            self.set_mixed_formats(formats)
         s = nat.MixedStats()
         nat.check(nat.lib().h3d_compress(self._h, float(tol), ctypes.byref(s)))
         return self.mixed_stats()
@@ -366,12 +380,14 @@ class HODLR3D:
 
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
-        ms = (ctypes.c_double * 9)()
+        ms = (ctypes.c_double * 12)()
         cnt = ctypes.c_int()
         nat.check(nat.lib().h3d_kernel_timing(self._h, int(enable), ms, ctypes.byref(cnt)))
         return {"phase1": ms[0], "phase2": ms[1], "near": ms[2], "combine": ms[3],
                 "total": ms[4], "phase1_second": ms[5], "phase2_second": ms[6],
-                "phase1_kernel": ms[7], "phase1_kernel_second": ms[8], "count": cnt.value}
+                "phase1_kernel": ms[7], "phase1_kernel_second": ms[8],
+                "phase1_third": ms[9], "phase2_third": ms[10], "phase1_kernel_third": ms[11],
+                "count": cnt.value}
 
     def set_diag(self, c):
         if _is_torch_cuda(c):
@@ -548,6 +564,8 @@ class MatvecGraph:
 
 
 PERSISTENT = 0x100  # h3d_batch_set_tuning flag: the persistent (work-queue) launch
+MIXED_FP16 = 1      # h3d_set_mixed_formats bits
+MIXED_FP8 = 2
 
 
 class Batch:

@@ -118,7 +118,15 @@ typedef struct h3d_mixed_stats {
   double build_ms;
   int single_jacobi;        /* 1: the build kept each block's Jacobi rotations between its
                                two passes (one Jacobi per block); 0: it ran them twice */
+  int formats;              /* formats the copy could use (H3D_MIXED_*) */
+  double err_fp8;           /* part of predicted_rel_err from fp8 rounding */
+  int64_t rank_fp8;         /* columns in fp8 (e4m3) */
+  int64_t u_elems_fp8, v_elems_fp8;
 } h3d_mixed_stats;
+
+/* Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats). */
+#define H3D_MIXED_FP16 1
+#define H3D_MIXED_FP8 2
 
 typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
@@ -178,12 +186,21 @@ int h3d_release_precision(h3d_handle* h, int prec);
  * per entry stays far below any useful budget): every block B = U V^T is
  * recompressed through the
  * eigendecomposition of its Gram matrices (B = U' V'^T, V' orthonormal), and
- * its columns are dropped, stored in fp16 or in fp32 so that the expected
+ * its columns are dropped, stored in fp8, fp16 or fp32 so that the expected
  * added relative error ||A_mixed - A||_F / ||A||_F stays within `tol` (one
  * budget over all blocks, spent on the columns with the least error per stored
  * element).  Needs the fp64 or the fp32 copy resident; stats may be NULL.  A matvec with
  * H3D_PREC_MIXED builds the copy with tol = 5 * 10^-TOL_POW if absent. */
 int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* stats);
+/* Formats the next build of the recompressed copy may store columns in besides
+ * fp32 (no reference counterpart): H3D_MIXED_FP16 | H3D_MIXED_FP8 (the
+ * default), one of them, or 0 (fp32 columns only).  Each format is a tier of
+ * the budget's choice: a column of the least error per stored element is
+ * dropped, the next ones stored in fp8 (e4m3, 1 byte), fp16 (2 bytes) or fp32,
+ * each tier over a fixed range of that error measure (the ranges minimise the
+ * bytes for the budget); a format turned off is skipped.  The copy in place
+ * is kept until the next h3d_compress. */
+int h3d_set_mixed_formats(h3d_handle* h, int formats);
 int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* stats);
 
 /* Replaces HODLR3D::MatVecProduct (src/HODLR3D.cpp:21-27): y = A b for nrhs
@@ -260,12 +277,14 @@ int h3d_get_tuning(h3d_handle* h, int prec, int* v);
 int h3d_set_tuning_state(h3d_handle* h, int prec, const int* v);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
- * stream (enable != 0).  ms[9] receives the accumulated milliseconds of
+ * stream (enable != 0).  ms[12] receives the accumulated milliseconds of
  * phase 1 (V^T x, with its partial-sum reduction and, for H3D_PREC_MIXED,
  * the gather into target order), phase 2 (U t), near field, combine and whole
  * matvecs since the last read; then the parts of phase 1 and phase 2 spent on
  * the second factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); then
- * the phase-1 kernel alone of the first and of the second copy.  *count gets
+ * the phase-1 kernel alone of the first and of the second copy; then the
+ * parts of phase 1 and phase 2 spent on the third copy (the fp8 class) and its
+ * phase-1 kernel alone.  *count gets
  * the number of matvecs; all are then reset.  Grouped products of several
  * right-hand sides (h3d_set_rhs_block) are not timed.  While timing is on,
  * every kernel of a matvec runs in line on the handle's stream; otherwise the

@@ -102,7 +102,7 @@ int main(int argc, char** argv) {
   int tun[6];
   EXPECT(h3d_get_tuning(h, H3D_PREC_MIXED, tun) == H3D_OK);
   EXPECT(h3d_set_tuning_state(h, H3D_PREC_MIXED, tun) == H3D_OK);
-  double kms[9];
+  double kms[12];
   int cnt = 0;
   EXPECT(h3d_kernel_timing(h, 1, nullptr, nullptr) == H3D_OK);
   EXPECT(h3d_matvec(h, b.data(), y2.data(), 1, H3D_PREC_FP64, H3D_HOST) == H3D_OK);

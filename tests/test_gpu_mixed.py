@@ -1,4 +1,4 @@
-"""Recompressed mixed-precision copy (fp32 + fp16 columns, fp64 accumulation):
+"""Recompressed mixed-precision copy (fp32 + fp16 + fp8 columns, fp64 accumulation):
 error against the direct FP64 product within the requested budget, adjoint,
 determinism, every load width, and refined solves on it (GPU).
 
@@ -39,9 +39,11 @@ def _rows(N, k=2048, seed=3):
 def test_stats_are_consistent(built):
     name, x, h, tol, st = built
     assert st["tol"] == tol
-    assert st["rank_fp32"] + st["rank_fp16"] + st["dropped"] == st["aca_rank_sum"]
+    assert st["rank_fp32"] + st["rank_fp16"] + st["rank_fp8"] + st["dropped"] == st["aca_rank_sum"]
+    assert st["formats"] == 3 and st["rank_fp8"] > 0 and st["rank_fp16"] > 0
     assert 0 <= st["predicted_rel_err"] <= tol * (1 + 1e-9)
-    parts = np.sqrt(st["err_drop"] ** 2 + st["err_fp16"] ** 2 + st["err_fp32"] ** 2)
+    parts = np.sqrt(st["err_drop"] ** 2 + st["err_fp8"] ** 2 + st["err_fp16"] ** 2 +
+                    st["err_fp32"] ** 2)
     assert abs(parts - st["predicted_rel_err"]) <= 1e-9 * tol
     full = h.stats()
     assert st["aca_rank_sum"] == full["rank_sum"]
@@ -117,8 +119,8 @@ def test_overlapped_near_field_equals_the_in_line_product(built):
 
 def test_every_load_width_gives_the_same_product(built):
     """Load widths and rows per lane change the summation order; the fp16
-    class (about 1e-3 of the operator) sums in fp32, so results agree to
-    ~1e-10 rather than to fp64 rounding."""
+    and fp8 classes (a small part of the operator) sum in fp32, so results
+    agree to ~1e-10 rather than to fp64 rounding."""
     name, x, h, tol, st = built
     q = np.random.default_rng(8).standard_normal(len(x))
     h.set_tuning("mixed", 16, 16)
@@ -148,7 +150,7 @@ def test_tiny_budget_keeps_every_column_in_fp32():
     h = H3.HODLR3D(4, 1.0, 8, 7, x)
     h.assemble()
     st = h.compress(1e-15)
-    assert st["dropped"] == 0 and st["rank_fp16"] == 0
+    assert st["dropped"] == 0 and st["rank_fp16"] == 0 and st["rank_fp8"] == 0
     assert st["err_drop"] == 0 and st["err_fp16"] == 0 and st["err_fp32"] < 1e-7
     q = np.random.default_rng(9).standard_normal(len(x))
     y64 = h.matvec(q, prec="fp64")
@@ -211,9 +213,44 @@ def test_two_pass_build_matches_the_single_jacobi_build(built):
     finally:
         nat.lib().h3d_debug_fail_alloc(-1)
     assert st2["single_jacobi"] == 0
-    for k in ("rank_fp32", "rank_fp16", "dropped", "raw_blocks", "bytes"):
+    for k in ("rank_fp32", "rank_fp16", "rank_fp8", "dropped", "raw_blocks", "bytes"):
         assert st2[k] == st[k], k
     assert st2["predicted_rel_err"] == st["predicted_rel_err"]
     y2 = h.matvec(q, prec="mixed")
     assert np.linalg.norm(y2 - y1) / np.linalg.norm(y1) < 1e-12
     h.compress(tol)  # the single-Jacobi copy again for the tests after this one
+
+
+def test_storage_formats_are_tiers_of_one_budget(built):
+    """fp16 and fp8 are tiers the budget may use: with one turned off its
+    columns go to a neighbouring tier, every copy meets the budget, and the
+    copy with both tiers stores the fewest bytes.  Turning both off leaves
+    fp32 columns only."""
+    import hodlr3d_amd as H3
+    name, x, h, tol, st = built
+    q = np.random.default_rng(43).standard_normal(len(x))
+    y64 = h.matvec(q, prec="fp64")
+    y_all = h.matvec(q, prec="mixed")
+    nbytes = {}
+    for fm, mask in ((("fp16",), 1), (("fp8",), 2), ((), 0), (("fp16", "fp8"), 3)):
+        s = h.compress(tol, formats=fm)
+        assert s["formats"] == mask
+        if "fp8" not in fm:
+            assert s["rank_fp8"] == 0 and s["err_fp8"] == 0
+        if "fp16" not in fm:
+            assert s["rank_fp16"] == 0 and s["err_fp16"] == 0
+        assert s["rank_fp32"] + s["rank_fp16"] + s["rank_fp8"] + s["dropped"] == s["aca_rank_sum"]
+        assert s["predicted_rel_err"] <= tol * (1 + 1e-9)
+        ym = h.matvec(q, prec="mixed")
+        dm = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
+        print(f"[{name}] formats {fm}: {s['bytes'] / 1e6:.1f} MB, ranks fp32/fp16/fp8 "
+              f"{s['rank_fp32']}/{s['rank_fp16']}/{s['rank_fp8']}, dropped {s['dropped']}, "
+              f"vs fp64 {dm:.2e}", flush=True)
+        assert dm <= 2.0 * tol
+        nbytes[fm] = s["bytes"]
+    assert nbytes[("fp16", "fp8")] <= 1.001 * min(nbytes[("fp16",)], nbytes[("fp8",)])
+    assert max(nbytes[("fp16",)], nbytes[("fp8",)]) <= nbytes[()]
+    # the last build used both formats again: the same copy as the fixture's
+    assert np.array_equal(h.matvec(q, prec="mixed"), y_all)
+    with pytest.raises(H3.H3DError):
+        h.set_mixed_formats(("bf16",))
