@@ -67,7 +67,7 @@ MODE_DESC = {
     "bf16": "bf16 factors, fp32 accumulation, fp32 on-the-fly near field",
     "bf16/acc64": "bf16 factors, fp64 accumulation, fp64 on-the-fly near field",
     "mixed": "recompressed factors (eigendecomposition of each block's Gram matrices), "
-             "columns dropped / fp8 (e4m3) / fp16 / fp32 under one global error budget, "
+             "columns dropped / fp8 (e4m3) / fp16 / fp24 / fp32 under one global error budget, "
              "fp64 accumulation, fp32 near-field entries summed in fp64",
 }
 
@@ -143,8 +143,8 @@ def near_metrics(st, res, mode):
 
 def mixed_kernels(H, km):
     """(name, ms, algorithmic bytes) of the far-field kernels of the mixed copy:
-    phase 1 / phase 2 of its fp32 class (fp64 x and t), of its fp16 class and
-    of its fp8 class (fp32 x and t).  Phase 1 (the kernel alone, event-timed)
+    phase 1 / phase 2 of its fp32 and fp24 classes (fp64 x and t) and of its
+    fp16 and fp8 classes (fp32 x and t).  Phase 1 (the kernel alone, event-timed)
     reads the class's V entries and x once per level and writes t; phase 2
     reads the U entries and t and writes (fp32 class) or updates (the others)
     y once per level.  Template arguments follow the tuning the products used
@@ -155,20 +155,23 @@ def mixed_kernels(H, km):
     lvn = st["nlevels"] * st["N"]
     classes = (
         (4, 8, "float, double", ms_["rank_fp32"], ms_["v_elems"][0], ms_["u_elems"][0],
-         km["phase1_kernel"], km["phase2"] - km["phase2_second"] - km["phase2_third"], 0),
+         km["phase1_kernel"],
+         km["phase2"] - km["phase2_second"] - km["phase2_third"] - km["phase2_fourth"], 0),
         (2, 4, "_Float16, float", ms_["rank_fp16"], ms_["v_elems"][1], ms_["u_elems"][1],
          km["phase1_kernel_second"], km["phase2_second"], 1),
         (1, 4, "f8e4m3, float", ms_["rank_fp8"], ms_["v_elems_fp8"], ms_["u_elems_fp8"],
          km["phase1_kernel_third"], km["phase2_third"], 1),
+        (3, 8, "f24, double", ms_["rank_fp24"], ms_["v_elems_fp24"], ms_["u_elems_fp24"],
+         km["phase1_kernel_fourth"], km["phase2_fourth"], 0),
     )
     out = []
     for es, sx, tp, rank, ve, ue, t1, t2, slot in classes:
         if rank == 0:
             continue
         p1 = es * ve + sx * lvn + sx * rank
-        p2 = es * ue + sx * rank + 8 * lvn * (1 if slot == 0 else 2)
-        w1 = tun[2 * slot] // es
-        w2 = tun[2 * slot + 1] // es
+        p2 = es * ue + sx * rank + 8 * lvn * (1 if es == 4 else 2)
+        w1 = tun[2 * slot] // es if es != 3 else 4
+        w2 = tun[2 * slot + 1] // es if es != 3 else 4
         out.append((f"mphase1_kernel<{tp}, {w1}>", t1, p1))
         out.append((f"phase2_kernel<{tp}, {w2}>", t2, p2))
     return out
@@ -500,7 +503,7 @@ def main():
                          "HBM counters of a profiler run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-formats-compare", action="store_true",
-                    help="skip the mixed copy without the fp8 tier")
+                    help="skip the mixed copy with the fp16 tier only")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-spec", action="store_true",
                     help="skip the run at the precision configuration's ACA tolerance 1e-7")
@@ -606,7 +609,8 @@ def main():
                                                    "phase1_second", "phase2_second",
                                                    "phase1_kernel", "phase1_kernel_second",
                                                    "phase1_third", "phase2_third",
-                                                   "phase1_kernel_third")}
+                                                   "phase1_kernel_third", "phase1_fourth",
+                                                   "phase2_fourth", "phase1_kernel_fourth")}
         log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e} (max over draws "
             f"{out['rel_err_max']:.3e}), kernels "
             f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }")
@@ -678,8 +682,8 @@ def main():
         log(f"[bench] solve {solve}")
         H.set_operator(alpha=1.0, diag=0.0)
 
-    # the same budget without the fp8 tier (columns dropped / fp16 / fp32): what
-    # the fp8 tier changes
+    # the same budget with the fp16 tier only (columns dropped / fp16 / fp32):
+    # what the fp8 and fp24 tiers change
     formats_compare = None
     if mixed_info and not args.no_formats_compare:
         torch.cuda.synchronize()
@@ -692,8 +696,8 @@ def main():
                            "predicted_rel_err": s16["predicted_rel_err"],
                            "rank_fp32": s16["rank_fp32"], "rank_fp16": s16["rank_fp16"],
                            "dropped": s16["dropped"],
-                           "speedup_of_fp8_tier": round(r16["ms"] / m["ms"], 3)}
-        log(f"[bench] mixed copy without the fp8 tier: {formats_compare}")
+                           "speedup_of_fp8_fp24_tiers": round(r16["ms"] / m["ms"], 3)}
+        log(f"[bench] mixed copy with the fp16 tier only: {formats_compare}")
 
     line = {
         "metric": METRIC,
@@ -708,7 +712,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "fp32/acc64": "f32 storage, f64 accumulation",
                   "bf16": "bf16", "bf16/acc64": "bf16 storage, f64 accumulation",
-                  "mixed": "f32+f16+f8 storage, f64 accumulation"}.get(mixed, mixed),
+                  "mixed": "f32+f24+f16+f8 storage, f64 accumulation"}.get(mixed, mixed),
         "data": "synthetic",
         "config": {"workload": wl["desc"], "N": N, "nlevels": st["nlevels"],
                    "mixed": MODE_DESC.get(mixed, mixed), "err_bar": REL_ERR_BAR,
@@ -749,7 +753,7 @@ def main():
                                       "GB/s": round(k[2] / (k[1] * 1e-3) / 1e9, 1) if k[1] > 0 else None}
                                      for k in kern]},
         "mixed_copy": mixed_info,
-        "mixed_fp16_tiers_only": formats_compare,
+        "mixed_fp16_tier_only": formats_compare,
         "near_field": near_metrics(st, res, mixed),
         "spec_tol_1e-7": spec,
         "host": host_cpu(),

@@ -58,7 +58,9 @@ class MixedStats(ctypes.Structure):
                 ("dropped", _i64), ("raw_blocks", _i64), ("raw_rank", _i64), ("bytes", _i64),
                 ("u_elems", _i64 * 2), ("v_elems", _i64 * 2), ("build_ms", _d),
                 ("single_jacobi", _i32), ("formats", _i32), ("err_fp8", _d),
-                ("rank_fp8", _i64), ("u_elems_fp8", _i64), ("v_elems_fp8", _i64)]
+                ("rank_fp8", _i64), ("u_elems_fp8", _i64), ("v_elems_fp8", _i64),
+                ("err_fp24", _d), ("rank_fp24", _i64), ("u_elems_fp24", _i64),
+                ("v_elems_fp24", _i64)]
 
 
 class Config(ctypes.Structure):

@@ -90,6 +90,24 @@ template <class S, int NV>
 struct RawT {
   using T = typename RawOf<NV * (int)sizeof(S)>::T;
 };
+// packed fp24 entries: four (12 bytes, an 8-byte and a 4-byte load) or
+// eight (24 bytes: 16 + 8)
+struct Raw12 {
+  u32x2a a;
+  uint32_t b;
+};
+struct Raw24 {
+  u32x4a a;
+  u32x2a b;
+};
+template <>
+struct RawT<f24, 4> {
+  using T = Raw12;
+};
+template <>
+struct RawT<f24, 8> {
+  using T = Raw24;
+};
 template <>
 struct RawT<double, 2> {
   using T = f64x2a;
@@ -103,7 +121,20 @@ using Raw = typename RawT<S, NV>::T;
 
 template <class S, int NV>
 __device__ inline Raw<S, NV> raw_load(const S* p) {
-  return ntload<Raw<S, NV>>(p);
+  if constexpr (sizeof(S) == 3) {
+    static_assert(NV == 4 || NV == 8, "fp24 entries are loaded four or eight at a time");
+    Raw<S, NV> r;
+    if constexpr (NV == 4) {
+      r.a = ntload<u32x2a>(p);
+      r.b = ntload<uint32_t>((const char*)p + 8);
+    } else {
+      r.a = ntload<u32x4a>(p);
+      r.b = ntload<u32x2a>((const char*)p + 16);
+    }
+    return r;
+  } else {
+    return ntload<Raw<S, NV>>(p);
+  }
 }
 
 template <class S, class A, int NV>
@@ -115,6 +146,16 @@ __device__ inline void raw_cvt(const Raw<S, NV>& v, A (&o)[NV]) {
     } else {
       o[0] = (A)v;
     }
+  } else if constexpr (sizeof(S) == 3) {  // fp24: packed values, four per 12 bytes
+    uint32_t u[8];
+    if constexpr (NV == 4) {
+      f24x4_bits(v.a.x, v.a.y, v.b, *(uint32_t(*)[4])u);
+    } else {
+      f24x4_bits(v.a.x, v.a.y, v.a.z, *(uint32_t(*)[4])u);
+      f24x4_bits(v.a.w, v.b.x, v.b.y, *(uint32_t(*)[4])(u + 4));
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) o[q] = (A)__uint_as_float(u[q]);
   } else if constexpr (sizeof(S) == 1) {  // fp8: two values per conversion
     uint32_t w[B >= 4 ? B / 4 : 1];
     if constexpr (B == 16) {
@@ -361,7 +402,8 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
                                                       const A* __restrict__ x,
                                                       A* __restrict__ tsrc, A* __restrict__ part,
                                                       int kmax) {
-  // 64 lanes x KMAX x VEC >= the item's rows (mix_item_rows)
+  // 64 lanes x KMAX x VEC >= the item's rows (mix_item_rows); two vectors for
+  // 16 fp8 entries per lane (x of their rows stays in registers)
   constexpr int KMAX = VEC >= 16 ? 2 : (VEC >= 2 ? 4 : 8);
   const int w = wave_id();
   if (w >= nitems) return;
@@ -1500,19 +1542,22 @@ FactorView aca_view(Engine& e, int prec, bool acc64) {
 // Scratch t layout of the mixed copy (bytes from d_t, every part 16-byte
 // aligned for the gathers' vector stores): each class's target-order t, then
 // each class's source-order t (phase-1 output).
-inline double* class_t0(Engine& e) { return e.d_t; }
-inline float* class_tf(Engine& e, int c) {
+inline double* class_td(Engine& e, int c) {  // classes 0 and 3 (fp64 sums)
+  return (double*)((char*)e.d_t + mix_t_layout(e.mix).t[c]);
+}
+inline float* class_tf(Engine& e, int c) {  // classes 1 and 2 (fp32 sums)
   return (float*)((char*)e.d_t + mix_t_layout(e.mix).t[c]);
 }
 inline void* class_tsrc(Engine& e, int c) {
   return (void*)((char*)e.d_t + mix_t_layout(e.mix).src[c]);
 }
 
-// Classes 1 and 2 share one tuning slot (their phases run with the same settings).
+// Classes 0 and 3 share one tuning slot, classes 1 and 2 another (fp64 and
+// fp32 sums).  Class 0's phase 2 stores y, the others add to it.
 FactorView class_view(Engine& e, int c) {
   const FactorClass& fc = e.mix.cls[c];
   return {&fc.plan, fc.plan.d_p2, (int)fc.plan.p2.size(), fc.d_blk, fc.d_btoff, fc.d_ubase,
-          fc.d_vbase, e.mix.d_scale + fc.tbase, c, 6 + std::min(c, 1), &fc, class_tsrc(e, c)};
+          fc.d_vbase, e.mix.d_scale + fc.tbase, c, class_acc64(c) ? 6 : 7, &fc, class_tsrc(e, c)};
 }
 
 // Source-grouped phase 1 of a mixed class with a load width of `bytes` per lane.
@@ -1521,12 +1566,12 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   const FactorClass& fc = *f.fc;
   const int n1 = (int)fc.mp1.size();
   if (!n1) return;
-  // the items span up to 64 lanes x 4 vectors x 16 bytes of rows, which only
-  // the 16-byte loads cover
+  // the items span the rows of 64 lanes x KMAX vectors of 16 bytes (12 for
+  // fp24: four entries), which only the 16-byte loads cover
   if (bytes != 16) throw Error(H3D_ERR_INVALID, "the mixed phase 1 takes 16-byte loads");
   constexpr int es = (int)sizeof(S);
-  mphase1_kernel<S, A, 16 / es><<<wave_grid(n1), 256, 0, e.stream>>>(fc.d_mp1, n1, f.vbase, x,
-                                                                     (A*)f.tsrc, part, kmax);
+  mphase1_kernel<S, A, es == 3 ? 4 : 16 / es><<<wave_grid(n1), 256, 0, e.stream>>>(
+      fc.d_mp1, n1, f.vbase, x, (A*)f.tsrc, part, kmax);
   H3D_LAUNCH_CHECK();
 }
 
@@ -1572,7 +1617,9 @@ void p1_raw(const P1Item* it, int n1, void* const* vbase, const BlockDev* blk, c
             A* part, const double* scale, int bytes, hipStream_t st, unsigned* queue = nullptr) {
   if (!n1) return;
   constexpr int es = (int)sizeof(S);
-  if (bytes >= 16 || 16 / es == 1 && bytes >= es)
+  if constexpr (es == 3) {  // fp24 is a mixed class only (source-grouped phase 1)
+    throw Error(H3D_ERR_INVALID, "no block-walking phase 1 for fp24 entries");
+  } else if (bytes >= 16 || 16 / es == 1 && bytes >= es)
     p1_launch<S, A, 16 / es>(it, n1, vbase, blk, x, t, part, scale, queue, st);
   else if (bytes == 8 && 8 / es >= 1)
     p1_launch<S, A, (8 / es > 0 ? 8 / es : 1)>(it, n1, vbase, blk, x, t, part, scale, queue, st);
@@ -1601,7 +1648,9 @@ void p2_raw(const P2Item* it, int n2, void* const* ubase, const A* t, double* yl
             int acc, int bytes, hipStream_t st, unsigned* queue = nullptr) {
   if (!n2) return;
   constexpr int es = (int)sizeof(S);
-  if (bytes >= 16)
+  if constexpr (es == 3)  // fp24: four entries (12 bytes) per lane whatever the width
+    p2_launch<S, A, 4>(it, n2, ubase, t, ylev, N, acc, queue, st);
+  else if (bytes >= 16)
     p2_launch<S, A, 16 / es>(it, n2, ubase, t, ylev, N, acc, queue, st);
   else if (bytes == 8)
     p2_launch<S, A, (8 / es > 0 ? 8 / es : 1)>(it, n2, ubase, t, ylev, N, acc, queue, st);
@@ -1903,7 +1952,7 @@ void autotune(Engine& e, int mode, int reps, double* ms) {
   ensure_scratch(e, 1);
   // timing does not depend on the values: x is whatever the last matvec left
   if (prec == kPrecMixed) {
-    tune_mode<float, double>(e, class_view(e, 0), reps, e.d_xp, class_t0(e), e.d_part, ms);
+    tune_mode<float, double>(e, class_view(e, 0), reps, e.d_xp, class_td(e, 0), e.d_part, ms);
     // the fp16 class's settings serve the fp8 class too (one slot)
     tune_mode<_Float16, float>(e, class_view(e, 1), reps, e.d_xf, class_tf(e, 1),
                                (float*)e.d_part, ms ? ms + 8 : nullptr);
@@ -1997,19 +2046,25 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       H3D_HIP(hipEventRecord(e.ev_join, e.side));
     }
     // event slots: 1 phase 1 start, 8 after the first copy's phase-1 kernel,
-    // 6 after its phase 1 (reduce, gather), 9 after the second copy's phase-1
+    // 13 after its phase 1 (reduce, gather), 15 after the fourth copy's
+    // phase-1 kernel, 6 after its phase 1, 9 after the second copy's phase-1
     // kernel, 10 after its phase 1, 12 after the third copy's phase-1 kernel,
-    // 2 phase 2 start, 7 after the first copy's phase 2, 11 after the
-    // second's, 3 near start, 4 combine start, 5 end (the copies: the mixed
-    // copy's classes; one copy in the other modes)
+    // 2 phase 2 start, 14 after the first copy's phase 2, 7 after the
+    // fourth's, 11 after the second's, 3 near start, 4 combine start, 5 end
+    // (the copies: the mixed copy's classes 0, 1, 2, 3 = fp32, fp16, fp8,
+    // fp24, run in the order 0, 3, 1, 2; one copy in the other modes)
     for (int which = 1; which <= 2; ++which) {
       if (e.timing) H3D_HIP(hipEventRecord(e.ev[which], st));
       hipEvent_t m1 = (e.timing && which == 1) ? e.ev[8] : nullptr;
       hipEvent_t m2 = (e.timing && which == 1) ? e.ev[9] : nullptr;
       hipEvent_t m3 = (e.timing && which == 1) ? e.ev[12] : nullptr;
+      hipEvent_t m4 = (e.timing && which == 1) ? e.ev[15] : nullptr;
       if (mixed) {
-        run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_t0(e),
+        run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_td(e, 0),
                                  e.d_part, m1);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[12 + which], st));
+        run_phase<f24, double>(e, class_view(e, 3), which, transpose, e.d_xp, class_td(e, 3),
+                               e.d_part, m4);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_tf(e, 1),
                                    (float*)e.d_part, m2);
@@ -2024,6 +2079,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
           run_phase<float, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
         else
           run_phase<uint16_t, double>(e, f, which, transpose, e.d_xp, e.d_t, e.d_part, m1);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[12 + which], st));
+        if (m4) H3D_HIP(hipEventRecord(m4, st));
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         if (m2) H3D_HIP(hipEventRecord(m2, st));
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
@@ -2036,6 +2093,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
           run_phase<float, float>(e, f, which, transpose, e.d_xf, t, part, m1);
         else
           run_phase<uint16_t, float>(e, f, which, transpose, e.d_xf, t, part, m1);
+        if (e.timing) H3D_HIP(hipEventRecord(e.ev[12 + which], st));
+        if (m4) H3D_HIP(hipEventRecord(m4, st));
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[5 + which], st));
         if (m2) H3D_HIP(hipEventRecord(m2, st));
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
@@ -2052,7 +2111,7 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     if (e.timing) {
       H3D_HIP(hipEventRecord(e.ev[5], st));
       H3D_HIP(hipEventSynchronize(e.ev[5]));
-      float ms[12];
+      float ms[15];
       H3D_HIP(hipEventElapsedTime(&ms[0], e.ev[1], e.ev[2]));  // phase 1 (+ partial reduce)
       H3D_HIP(hipEventElapsedTime(&ms[1], e.ev[2], e.ev[3]));  // phase 2
       H3D_HIP(hipEventElapsedTime(&ms[2], e.ev[3], e.ev[4]));  // near field
@@ -2065,7 +2124,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       H3D_HIP(hipEventElapsedTime(&ms[9], e.ev[10], e.ev[2]));  // phase 1 of the third copy
       H3D_HIP(hipEventElapsedTime(&ms[10], e.ev[11], e.ev[3]));  // phase 2 of the third copy
       H3D_HIP(hipEventElapsedTime(&ms[11], e.ev[10], e.ev[12]));  // third copy's phase-1 kernel
-      for (int q = 0; q < 12; ++q) e.ktime[q] += ms[q];
+      H3D_HIP(hipEventElapsedTime(&ms[12], e.ev[13], e.ev[6]));  // phase 1 of the fourth copy
+      H3D_HIP(hipEventElapsedTime(&ms[13], e.ev[14], e.ev[7]));  // phase 2 of the fourth copy
+      H3D_HIP(hipEventElapsedTime(&ms[14], e.ev[13], e.ev[15]));  // fourth copy's phase-1 kernel
+      for (int q = 0; q < 15; ++q) e.ktime[q] += ms[q];
       e.ref_blas2_ms += ms[0] + ms[1];  // far field: the BLAS-2 applies
       e.ref_regen_ms += ms[2];          // near field: entries regenerated in the product
       e.kcount++;

@@ -419,7 +419,8 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
     s->tol = m.tol;
     const double a2 = m.fro2_far + m.fro2_near;
     s->predicted_rel_err =
-        a2 > 0 ? std::sqrt((m.err2_drop + m.err2_fp8 + m.err2_fp16 + m.err2_fp32) / a2) : 0;
+        a2 > 0 ? std::sqrt((m.err2_drop + m.err2_fp8 + m.err2_fp16 + m.err2_fp24 + m.err2_fp32) / a2)
+               : 0;
     s->err_drop = a2 > 0 ? std::sqrt(m.err2_drop / a2) : 0;
     s->err_fp16 = a2 > 0 ? std::sqrt(m.err2_fp16 / a2) : 0;
     s->err_fp32 = a2 > 0 ? std::sqrt(m.err2_fp32 / a2) : 0;
@@ -431,7 +432,7 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
     s->dropped = m.dropped;
     s->raw_blocks = m.raw_blocks;
     s->raw_rank = m.raw_rank;
-    s->bytes = m.cls[0].bytes + m.cls[1].bytes + m.cls[2].bytes;
+    s->bytes = m.cls[0].bytes + m.cls[1].bytes + m.cls[2].bytes + m.cls[3].bytes;
     s->u_elems[0] = m.cls[0].u_elems;
     s->u_elems[1] = m.cls[1].u_elems;
     s->v_elems[0] = m.cls[0].v_elems;
@@ -443,16 +444,21 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
     s->rank_fp8 = m.cls[2].rank_sum;
     s->u_elems_fp8 = m.cls[2].u_elems;
     s->v_elems_fp8 = m.cls[2].v_elems;
+    s->err_fp24 = a2 > 0 ? std::sqrt(m.err2_fp24 / a2) : 0;
+    s->rank_fp24 = m.cls[3].rank_sum;
+    s->u_elems_fp24 = m.cls[3].u_elems;
+    s->v_elems_fp24 = m.cls[3].v_elems;
   });
 }
 
-static_assert(H3D_MIXED_FP16 == h3d::kMixFp16 && H3D_MIXED_FP8 == h3d::kMixFp8,
+static_assert(H3D_MIXED_FP16 == h3d::kMixFp16 && H3D_MIXED_FP8 == h3d::kMixFp8 &&
+                  H3D_MIXED_FP24 == h3d::kMixFp24,
               "mixed-copy format bits");
 
 int h3d_set_mixed_formats(h3d_handle* h, int formats) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    if (formats & ~(H3D_MIXED_FP16 | H3D_MIXED_FP8))
+    if (formats & ~(H3D_MIXED_FP16 | H3D_MIXED_FP8 | H3D_MIXED_FP24))
       throw h3d::Error(H3D_ERR_INVALID, "unknown mixed-copy format");
     h->e.mix.formats = formats;
   });
@@ -470,9 +476,9 @@ int h3d_kernel_timing(h3d_handle* h, int enable, double* ms, int* count) {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
     h3d::Engine& e = h->e;
     if (ms)
-      for (int q = 0; q < 12; ++q) ms[q] = e.ktime[q];
+      for (int q = 0; q < 15; ++q) ms[q] = e.ktime[q];
     if (count) *count = e.kcount;
-    for (int q = 0; q < 12; ++q) e.ktime[q] = 0;
+    for (int q = 0; q < 15; ++q) e.ktime[q] = 0;
     e.kcount = 0;
     e.timing = enable != 0;
   });

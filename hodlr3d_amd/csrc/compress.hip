@@ -23,6 +23,7 @@
 // once to write the copy) so that only O(rank) numbers per block are kept
 // between the passes; both runs are deterministic and give the same bits.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <thread>
@@ -44,12 +45,15 @@ constexpr int kMaxEigRank = 142;          // larger blocks stay raw
 constexpr double kFp16Err = 3.4e-4;
 constexpr double kFp32Err = 4.2e-8;
 constexpr double kFp8Err = 4.3e-2;         // e4m3: p = 4
+constexpr double kFp24Err = 1.05e-5;       // fp32 rounded to 16 significant bits: p = 16
 // fp8 columns are stored 8x larger than the other classes' (rms entry 8, not
 // 1): the format's normal range, [2^-6, 448], then holds entries from 1/500
 // to 56 times a column's rms (larger ones are clamped to 448)
 constexpr double kFp8Gain = 8.0;
-constexpr int kClasses = 3;
-constexpr int kClassElem[kClasses] = {4, 2, 1};
+constexpr int kClasses = kMixClasses;
+constexpr int kClassElem[kClasses] = {4, 2, 1, 3};  // fp32, fp16, fp8, fp24
+// a block's kept columns (sorted, largest first) by class: fp32, fp24, fp16, fp8
+constexpr int kColumnOrder[kClasses] = {0, 3, 1, 2};
 constexpr double kCholFail = 1e-12;       // pivot / original diagonal below this: raw block
 constexpr int kMaxSweeps = 24;
 constexpr int kBinsPerOctave = 8;
@@ -69,16 +73,15 @@ struct CDesc {
 
 // Per-block output description (pass 2).
 struct PDesc {
-  int64_t u_out[3];  // class element offset of the block's first U column (in its U chunk)
-  int64_t v_out[3];  // class element offset of the block's V (in its V chunk)
-  int32_t vch[3];    // V chunk of the block per class
-  int32_t r32, r16, r8;  // columns kept per class (fp32, fp16, fp8: consecutive runs of the
-                         // sorted columns)
+  int64_t u_out[kClasses];  // class element offset of the block's first U column (in its U chunk)
+  int64_t v_out[kClasses];  // class element offset of the block's V (in its V chunk)
+  int32_t vch[kClasses];    // V chunk of the block per class
+  int32_t rc[kClasses];     // columns kept per class (consecutive runs of the sorted
+                            // columns, in kColumnOrder)
   int32_t raw;
-  int32_t ldu_o[3];  // U column stride per class
-  int32_t ldv_o[3];  // V column stride per class
-  int32_t pad;
-  double su, sv;     // power-of-two scales of the stored columns
+  int32_t ldu_o[kClasses];  // U column stride per class
+  int32_t ldv_o[kClasses];  // V column stride per class
+  double su, sv;            // power-of-two scales of the stored columns
 };
 
 // One project work item: rows [row0, row0 + 256) of one side of one block.
@@ -674,8 +677,9 @@ struct ClassBases {
 };
 
 // Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
-// and stored in the column's class (k < r32: fp32, then r16 columns in fp16,
-// then r8 in fp8).  Raw blocks copy their columns unscaled into fp32.
+// and stored in the column's class (the sorted columns: rc[0] in fp32, then
+// rc[3] in fp24, rc[1] in fp16, rc[2] in fp8).  Raw blocks copy their columns
+// unscaled into fp32.
 template <class S>
 __global__ __launch_bounds__(256) void project_kernel(
     const PItem* __restrict__ items, const CDesc* __restrict__ ds, const PDesc* __restrict__ ps,
@@ -696,20 +700,21 @@ __global__ __launch_bounds__(256) void project_kernel(
   const double* T = W + (side ? (int64_t)r * r : 0);
   const double* lam = W + 2 * (int64_t)r * r;
   const double floor_ = lam[r + 1];
-  auto out = [&](int c) -> char* {
-    const int nk = c == 0 ? p.r32 : (c == 1 ? p.r16 : p.r8);
-    if (nk == 0) return nullptr;
-    return (char*)(side ? cb.v[c][p.vch[c]] : cb.u[c][d.chunk]) +
+  auto out = [&](int c) -> uint8_t* {
+    if (p.rc[c] == 0) return nullptr;
+    return (uint8_t*)(side ? cb.v[c][p.vch[c]] : cb.u[c][d.chunk]) +
            ((side ? p.v_out[c] : p.u_out[c]) + row) * kClassElem[c];
   };
   float* o32 = (float*)out(0);
   _Float16* o16 = (_Float16*)out(1);
-  uint8_t* o8 = (uint8_t*)out(2);
+  uint8_t* o8 = out(2);
+  uint8_t* o24 = out(3);
   if (p.raw) {
-    for (int k = 0; k < p.r32; ++k) o32[(int64_t)k * ldo[0]] = (float)X[(int64_t)k * ld];  // S -> fp32
+    for (int k = 0; k < p.rc[0]; ++k) o32[(int64_t)k * ldo[0]] = (float)X[(int64_t)k * ld];  // S -> fp32
     return;
   }
-  const int keep = p.r32 + p.r16 + p.r8;
+  // first sorted column of each class
+  const int k24 = p.rc[0], k16 = k24 + p.rc[3], k8 = k16 + p.rc[1], keep = k8 + p.rc[2];
   const double sc = side ? p.sv : p.su;
   for (int k0 = 0; k0 < keep; k0 += KT) {
     double acc[KT];
@@ -727,14 +732,21 @@ __global__ __launch_bounds__(256) void project_kernel(
       if (k >= keep) break;
       // U side: unit-norm column times su; V side: V' columns are orthonormal
       const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(fmax(lam[k], floor_)));
-      if (k < p.r32) {
+      if (k < k24) {
         o32[(int64_t)k * ldo[0]] = (float)v;
-      } else if (k < p.r32 + p.r16) {
-        o16[(int64_t)(k - p.r32) * ldo[1]] = (_Float16)v;
+      } else if (k < k16) {
+        // fp32 rounded to 16 significant bits (half away from zero), top 3 bytes kept
+        const uint32_t u = __float_as_uint((float)v) + 0x80u;
+        uint8_t* o = o24 + (int64_t)(k - k24) * ldo[3] * 3;
+        o[0] = (uint8_t)(u >> 8);
+        o[1] = (uint8_t)(u >> 16);
+        o[2] = (uint8_t)(u >> 24);
+      } else if (k < k8) {
+        o16[(int64_t)(k - k16) * ldo[1]] = (_Float16)v;
       } else {
         float f = (float)(v * kFp8Gain);
         if (fabsf(f) > 448.0f) f = copysignf(448.0f, f);  // NaN passes
-        o8[(int64_t)(k - p.r32 - p.r16) * ldo[2]] =
+        o8[(int64_t)(k - k8) * ldo[2]] =
             (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(f, 0.0f, 0, false) & 0xff);
       }
     }
@@ -1192,7 +1204,7 @@ void build_mixed(Engine& e, double tol) {
       for (double v : h) mx.fro2_near += v;
     }
   }
-  // Each kept column goes to one tier t of (drop, fp8, fp16, fp32) with
+  // Each kept column goes to one tier t of (drop, fp8, fp16, fp24, fp32) with
   // bytes b_t per stored entry and squared relative error a_t (a_drop = 1).
   // Spending the budget with the least bytes is a Lagrangian choice: a column
   // of error per element lam / (n + m) takes the tier minimising
@@ -1206,15 +1218,19 @@ void build_mixed(Engine& e, double tol) {
     const double v = std::log2(lam / nm) * kBinsPerOctave;
     return (int)std::min<double>(kBins - 1, std::max<double>(0, std::floor(v) - kBinLo));
   };
-  const double ta[4] = {1.0, kFp8Err * kFp8Err, kFp16Err * kFp16Err, kFp32Err * kFp32Err};
-  const double tb[4] = {0.0, 1.0, 2.0, 4.0};
-  int av[4], nav = 0;  // available tiers, increasing
+  constexpr int kTiers = 5;
+  const double ta[kTiers] = {1.0, kFp8Err * kFp8Err, kFp16Err * kFp16Err, kFp24Err * kFp24Err,
+                             kFp32Err * kFp32Err};
+  const double tb[kTiers] = {0.0, 1.0, 2.0, 3.0, 4.0};
+  const int tier_class[kTiers] = {-1, 2, 1, 3, 0};
+  int av[kTiers], nav = 0;  // available tiers, increasing
   av[nav++] = 0;
   if (mx.formats & kMixFp8) av[nav++] = 1;
   if (mx.formats & kMixFp16) av[nav++] = 2;
-  av[nav++] = 3;
+  if (mx.formats & kMixFp24) av[nav++] = 3;
+  av[nav++] = 4;
   // bin offsets of the thresholds above the lowest one
-  int toff[4] = {0, 0, 0, 0};
+  int toff[kTiers] = {0, 0, 0, 0, 0};
   {
     const double th0 = (tb[av[1]] - tb[av[0]]) / (ta[av[0]] - ta[av[1]]);
     for (int i = 1; i + 1 < nav; ++i) {
@@ -1242,26 +1258,26 @@ void build_mixed(Engine& e, double tol) {
   int bd = 0;
   {
     // H[m]: per bin, lam of the columns whose lowest allowed tier is av[m]
-    std::vector<double> H[4];
+    std::vector<double> H[kTiers];
     for (int m = 0; m < nav; ++m) H[m].assign(kBins + 1, 0.0);
     {  // per-chunk histograms, added in chunk order
-      std::vector<std::vector<double>> ch_(kHostChunks * 4);
+      std::vector<std::vector<double>> ch_(kHostChunks * kTiers);
       host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
-        for (int m = 0; m < nav; ++m) ch_[c * 4 + m].assign(kBins + 1, 0.0);
+        for (int m = 0; m < nav; ++m) ch_[c * kTiers + m].assign(kBins + 1, 0.0);
         for (int64_t i = i0; i < i1; ++i) {
           if (raw[i]) continue;
           const int nm = cd[i].n + cd[i].m;
           for (int k = 0; k < cd[i].r; ++k) {
             const double l = lam_all[lam_off[i] + k];
-            ch_[c * 4 + min_tier(l)][bin_of(l, nm)] += l;
+            ch_[c * kTiers + min_tier(l)][bin_of(l, nm)] += l;
           }
         }
       });
       for (int c = 0; c < kHostChunks; ++c)
         for (int m = 0; m < nav; ++m)
-          for (int b = 0; b <= kBins; ++b) H[m][b] += ch_[c * 4 + m][b];
+          for (int b = 0; b <= kBins; ++b) H[m][b] += ch_[c * kTiers + m][b];
     }
-    std::vector<double> P[4];  // prefix sums
+    std::vector<double> P[kTiers];  // prefix sums
     for (int m = 0; m < nav; ++m) {
       P[m].assign(kBins + 1, 0.0);
       for (int b = 0; b < kBins; ++b) P[m][b + 1] = P[m][b] + H[m][b];
@@ -1280,7 +1296,7 @@ void build_mixed(Engine& e, double tol) {
       }
     };
     auto err = [&](int bdv) {
-      double q[4];
+      double q[kTiers];
       parts(bdv, q);
       double sum = 0.0;
       for (int i = 0; i < nav; ++i) sum += q[i];
@@ -1299,18 +1315,20 @@ void build_mixed(Engine& e, double tol) {
       bd = lo;
     }
     mx.theta_drop = std::pow(2.0, (double)(bd + kBinLo) / kBinsPerOctave);
-    double q[4];
+    double q[kTiers];
     parts(bd, q);
-    double by[4] = {0, 0, 0, 0};
+    double by[kTiers] = {0, 0, 0, 0, 0};
     for (int i = 0; i < nav; ++i) by[av[i]] = q[i];
     mx.err2_drop = by[0];
     mx.err2_fp8 = by[1];
     mx.err2_fp16 = by[2];
-    mx.err2_fp32 = by[3];
+    mx.err2_fp24 = by[3];
+    mx.err2_fp32 = by[4];
   }
   // per block class ranks: the tier only rises with lam, so each class is a
-  // prefix of the block's sorted columns (fp32, then fp16, then fp8)
-  std::vector<int32_t> r32(nb), r16(nb), r8(nb);
+  // run of the block's sorted columns (fp32, then fp24, fp16, fp8)
+  std::vector<std::array<int32_t, kClasses>> rk(nb);
+  auto kept = [&](int64_t i) { return rk[i][0] + rk[i][1] + rk[i][2] + rk[i][3]; };
   mx.aca_rank_sum = 0;
   mx.dropped = 0;
   mx.raw_blocks = 0;
@@ -1318,21 +1336,17 @@ void build_mixed(Engine& e, double tol) {
   host_chunks(nb, [&](int, int64_t i0, int64_t i1) {
     for (int64_t i = i0; i < i1; ++i) {
       const int r = cd[i].r;
-      r8[i] = 0;
+      rk[i].fill(0);
       if (raw[i]) {
-        r32[i] = r;
-        r16[i] = 0;
+        rk[i][0] = r;
         continue;
       }
       const int nm = cd[i].n + cd[i].m;
-      int cnt[4] = {0, 0, 0, 0};
       for (int k = 0; k < r; ++k) {
         const double l = lam_all[lam_off[i] + k];
-        cnt[av[std::max(min_tier(l), bin_tier(bin_of(l, nm), bd))]]++;
+        const int t = av[std::max(min_tier(l), bin_tier(bin_of(l, nm), bd))];
+        if (t > 0) rk[i][tier_class[t]]++;
       }
-      r32[i] = cnt[3];
-      r16[i] = cnt[2];
-      r8[i] = cnt[1];
     }
   });
   for (int64_t i = 0; i < nb; ++i) {
@@ -1343,7 +1357,7 @@ void build_mixed(Engine& e, double tol) {
       mx.raw_rank += r;
       continue;
     }
-    mx.dropped += r - (r32[i] + r16[i] + r8[i]);
+    mx.dropped += r - kept(i);
   }
 
   // ---------------- class layouts
@@ -1389,7 +1403,7 @@ void build_mixed(Engine& e, double tol) {
           const int64_t i = ci[lv.bbase + b];
           if (i < 0) continue;
           const int m = e.bcount[lo + lv.ilidx[b]];
-          const int rc[kClasses] = {r32[i], r16[i], r8[i]};
+          const std::array<int32_t, kClasses>& rc = rk[i];
           for (int c = 0; c < kClasses; ++c) {
             FactorClass& fc = mx.cls[c];
             LevelFactors& f = fc.lf[j];
@@ -1400,10 +1414,10 @@ void build_mixed(Engine& e, double tol) {
             fc.u_elems += (int64_t)rc[c] * n;
             fc.v_elems += (int64_t)rc[c] * m;
           }
-          pd[i].r32 = rc[0];
-          pd[i].r16 = rc[1];
-          pd[i].r8 = rc[2];
-          for (int c = 0; c < kClasses; ++c) pd[i].ldv_o[c] = class_ldv(m, kClassElem[c]);
+          for (int c = 0; c < kClasses; ++c) {
+            pd[i].rc[c] = rc[c];
+            pd[i].ldv_o[c] = class_ldv(m, kClassElem[c]);
+          }
           pd[i].raw = raw[i];
           pd[i].su = raw[i] ? 1.0 : pow2_near_sqrt(n);
           pd[i].sv = raw[i] ? 1.0 : pow2_near_sqrt(m);
@@ -1542,7 +1556,7 @@ void build_mixed(Engine& e, double tol) {
   // sweeps again), then project into the classes; blocks that keep no column
   // and raw blocks need no eigen step
   std::vector<uint8_t> work(nb, 0);
-  for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && (pd[i].r32 + pd[i].r16 + pd[i].r8) > 0;
+  for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && kept(i) > 0;
   {
     int64_t i0 = 0;
     while (i0 < nb) {
@@ -1554,7 +1568,7 @@ void build_mixed(Engine& e, double tol) {
       PDesc* d_pd = upload(sub, st);
       std::vector<PItem> items;
       for (int64_t i = i0; i < i1; ++i) {
-        if (pd[i].r32 + pd[i].r16 + pd[i].r8 == 0) continue;
+        if (kept(i) == 0) continue;
         for (int side = 0; side < 2; ++side) {
           const int rows = side ? cd[i].m : cd[i].n;
           for (int r0 = 0; r0 < rows; r0 += 256) items.push_back({(int32_t)(i - i0), side, r0, 0});
@@ -1586,7 +1600,7 @@ void build_mixed(Engine& e, double tol) {
   // order (fp8 columns: / kFp8Gain^2; raw columns: 1), block tables and plans
   // of every class
   {
-    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T + mx.cls[2].T, 1.0);
+    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T + mx.cls[2].T + mx.cls[3].T, 1.0);
     std::vector<BlockDev> blk[kClasses];
     std::vector<int32_t> btoff[kClasses];
     for (int c = 0; c < kClasses; ++c) {
@@ -1616,15 +1630,14 @@ void build_mixed(Engine& e, double tol) {
           if (i < 0 || raw[i]) continue;
           const double inv = 1.0 / (pd[i].su * pd[i].sv);
           const double fl = lam_floor[i];
-          const int rc[kClasses] = {r32[i], r16[i], r8[i]};
           int k0 = 0;  // first sorted column of the class
-          for (int c = 0; c < kClasses; ++c) {
+          for (int c : kColumnOrder) {
             const LevelFactors& f = mx.cls[c].lf[j];
             const double g = c == 2 ? inv / (kFp8Gain * kFp8Gain) : inv;
-            for (int kk = 0; kk < rc[c]; ++kk)
+            for (int kk = 0; kk < rk[i][c]; ++kk)
               scale[(size_t)mx.cls[c].tbase + f.toff[k] + f.col[b] + kk] =
                   std::sqrt(std::max(lam_all[lam_off[i] + k0 + kk], fl)) * g;
-            k0 += rc[c];
+            k0 += rk[i][c];
           }
         }
     }
@@ -1632,6 +1645,7 @@ void build_mixed(Engine& e, double tol) {
     // classes 1 and 2 sum in fp32: their factors converted once (the value
     // the gather would convert to), held until the stream synchronizes
     for (int c = 1; c < kClasses; ++c) {
+      if (class_acc64(c)) continue;
       std::vector<float> scale32(mx.cls[c].T);
       for (int32_t q = 0; q < mx.cls[c].T; ++q)
         scale32[q] = (float)scale[(size_t)mx.cls[c].tbase + q];
@@ -1643,7 +1657,9 @@ void build_mixed(Engine& e, double tol) {
       fc.d_btoff = upload(btoff[c], st);
       std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
       for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
-      build_plan_for(e, lfs, fc.plan, 1024 / fc.elem, fc.elem);  // 1 KB of rows per phase-2 item
+      // about 1 KB of rows per phase-2 item (a multiple of 4 rows for fp24, so
+      // that its 4-entry vectors start 4-byte aligned)
+      build_plan_for(e, lfs, fc.plan, fc.elem == 3 ? 256 : 1024 / fc.elem, fc.elem);
       fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
       fc.plan.p1r.clear();
       fc.d_mp1 = upload(fc.mp1, st);

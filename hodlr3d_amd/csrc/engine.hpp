@@ -198,13 +198,13 @@ struct Plan {
   void free_device();
 };
 
-// One storage class of the recompressed copy (fp32, fp16 or fp8 columns).  Columns
+// One storage class of the recompressed copy (fp32, fp16, fp8 or fp24 columns).  Columns
 // are stored scaled: the class's U and V columns hold u_k * su / nu_k and
 // v_k * sv / nv_k (su, sv powers of two near sqrt(n), sqrt(m)), and the
 // factor c_k = nu_k nv_k / (su sv) multiplies t_k after phase 1.  Chunk ids
 // and the block/box structure are those of the ACA copy.
 struct FactorClass {
-  int elem = 4;                      // bytes per stored element (4: fp32, 2: fp16, 1: fp8)
+  int elem = 4;                      // bytes per stored element (4: fp32, 3: fp24, 2: fp16, 1: fp8)
   std::vector<LevelFactors> lf;      // [level]; voff is relative to the block's V chunk
   std::vector<void*> udata, vdata;   // U chunks (the ACA copy's chunk ids), V chunks
   void* arena = nullptr;             // one allocation holding every chunk of the class
@@ -234,22 +234,30 @@ struct FactorClass {
 // Rows of a mixed phase-1 item for entries of `elem` bytes: the kernel's reach
 // with 16-byte loads, 64 lanes x 4 vectors (x 2 for 1-byte entries, whose
 // vectors of 16 rows would otherwise hold 64 x values per lane in registers).
-inline int mix_item_rows(int elem) { return elem == 1 ? 64 * 2 * 16 : 64 * 4 * 16 / elem; }
+inline int mix_item_rows(int elem) {
+  return elem == 1 ? 64 * 2 * 16 : (elem == 3 ? 64 * 4 * 4 : 64 * 4 * 16 / elem);
+}
 
 // Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats).
-constexpr int kMixFp16 = 1, kMixFp8 = 2;
+constexpr int kMixFp16 = 1, kMixFp8 = 2, kMixFp24 = 4;
 
 // Recompressed mixed-precision copy: SVD-truncated factors of every block with
-// columns in fp32, fp16 or fp8 chosen against a global Frobenius error budget.
+// columns in fp32, fp24, fp16 or fp8 chosen against a global Frobenius error
+// budget.  Classes 0 and 3 (fp32, fp24) sum in fp64, classes 1 and 2 (fp16,
+// fp8) in fp32.
+constexpr int kMixClasses = 4;
+inline bool class_acc64(int c) { return c == 0 || c == 3; }
 struct MixedCopy {
   bool ready = false;
   double tol = 0;                    // requested relative Frobenius error
-  int formats = kMixFp16 | kMixFp8;  // formats the next build may use
+  int formats = kMixFp16 | kMixFp8 | kMixFp24;  // formats the next build may use
   int built_formats = 0;             // those the copy in place could use
-  FactorClass cls[3];                // 0: fp32 columns, 1: fp16 columns, 2: fp8 (e4m3) columns
-  double* d_scale = nullptr;         // c_k per t entry (class 0, then 1, then 2)
+  // 0: fp32 columns, 1: fp16, 2: fp8 (e4m3), 3: fp24 (fp32 without its last 8 bits)
+  FactorClass cls[kMixClasses];
+  double* d_scale = nullptr;         // c_k per t entry (class 0, 1, 2, then 3)
   // statistics of the last build
-  double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0, err2_fp8 = 0;
+  double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0, err2_fp8 = 0,
+         err2_fp24 = 0;
   double theta_drop = 0, ms = 0;
   bool single_jacobi = false;        // the build kept J between its passes (one Jacobi per block)
   int64_t aca_rank_sum = 0, dropped = 0, raw_blocks = 0, raw_rank = 0;
@@ -258,10 +266,10 @@ struct MixedCopy {
 
 // Scratch t layout of the mixed copy's product (bytes from the t buffer,
 // every part 16-byte aligned for vector stores): each class's target-order t
-// (class 0 in doubles, classes 1 and 2 in floats), then each class's
+// (classes 0 and 3 in doubles, 1 and 2 in floats), then each class's
 // source-order t (phase-1 output).
 struct MixT {
-  int64_t t[3], src[3], end;
+  int64_t t[kMixClasses], src[kMixClasses], end;
 };
 inline MixT mix_t_layout(const MixedCopy& m) {
   MixT L{};
@@ -271,8 +279,8 @@ inline MixT mix_t_layout(const MixedCopy& m) {
     o = (o + bytes + 15) / 16 * 16;
     return at;
   };
-  for (int c = 0; c < 3; ++c) L.t[c] = put((int64_t)m.cls[c].T * (c == 0 ? 8 : 4));
-  for (int c = 0; c < 3; ++c) L.src[c] = put((int64_t)m.cls[c].T * (c == 0 ? 8 : 4));
+  for (int c = 0; c < kMixClasses; ++c) L.t[c] = put((int64_t)m.cls[c].T * (class_acc64(c) ? 8 : 4));
+  for (int c = 0; c < kMixClasses; ++c) L.src[c] = put((int64_t)m.cls[c].T * (class_acc64(c) ? 8 : 4));
   L.end = o;
   return L;
 }
@@ -383,12 +391,12 @@ struct Engine {
   double* d_iow = nullptr;          // host staging of one group (2 N W)
   int io_w = 0;
   GroupTables grp[2];               // grouped products: forward, transposed
-  GroupTables grpm[3];              // grouped products on the mixed copy's classes (forward)
+  GroupTables grpm[kMixClasses];    // grouped products on the mixed copy's classes (forward)
   int tw_w = 0;                     // W of the allocated d_tw
   float* d_xw32 = nullptr;          // x by position in fp32, W per point (the fp16 class)
   int xw32_w = 0;
-  void* d_mtw[3] = {nullptr, nullptr, nullptr};  // the mixed classes' t and partial sums, W per slot
-  int64_t mtw_cap[3] = {0, 0, 0};   // their allocated bytes
+  void* d_mtw[kMixClasses] = {};    // the mixed classes' t and partial sums, W per slot
+  int64_t mtw_cap[kMixClasses] = {};  // their allocated bytes
 
   // apply-time operator: y = alpha * (off-diagonal part) + diag * b
   double alpha = 1.0;
@@ -401,8 +409,8 @@ struct Engine {
   int tune_k[8] = {4, 4, 4, 4, 4, 4, 4, 4};
   // per-kernel event timing of matvecs
   bool timing = false;
-  hipEvent_t ev[13] = {};
-  double ktime[12] = {};
+  hipEvent_t ev[16] = {};
+  double ktime[15] = {};
   int kcount = 0;
   // reference-semantics timers (never reset), fed by the per-kernel timing
   double ref_regen_ms = 0, ref_blas2_ms = 0;
@@ -435,18 +443,20 @@ inline int64_t aca_ustart(int64_t off, int n) { return n > 256 ? (off + 31) / 32
 // column then touches 8 lines, not 9.  Shorter columns are packed (n rounded
 // up to even): one item reads the box's columns as one region.
 inline int class_ldu(int n, int elem) {
-  const int line = 128 / elem;
-  return n * elem > 1024 ? (n + line - 1) / line * line : (elem == 1 ? (n + 3) & ~3 : n + (n & 1));
+  // 3-byte entries: 128 entries (3 lines) per step for long columns
+  const int line = elem == 3 ? 128 : 128 / elem;
+  return n * elem > 1024 ? (n + line - 1) / line * line
+                         : (elem == 1 || elem == 3 ? (n + 3) & ~3 : n + (n & 1));
 }
 // Column stride of V for a source box of m rows (the ACA copies and the
-// recompressed classes; elem bytes per entry): even, a multiple of 4 for 1-byte
-// entries, so that every column starts 4-byte aligned for vector loads.
+// recompressed classes; elem bytes per entry): even, a multiple of 4 for 1- and
+// 3-byte entries, so that every column starts 4-byte aligned for vector loads.
 __host__ __device__ inline int class_ldv(int m, int elem) {
-  return elem == 1 ? (m + 3) & ~3 : m + (m & 1);
+  return elem == 1 || elem == 3 ? (m + 3) & ~3 : m + (m & 1);
 }
 // Element offset (in its chunk) of the U of such a box placed at `off` or after.
 inline int64_t class_ustart(int64_t off, int n, int elem) {
-  const int line = 128 / elem;
+  const int line = elem == 3 ? 128 : 128 / elem;
   return n * elem > 1024 ? (off + line - 1) / line * line : off;
 }
 constexpr int64_t kChunkTail = 64;  // zero elements after every factor chunk

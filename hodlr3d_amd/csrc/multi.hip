@@ -100,6 +100,13 @@ __device__ inline void load4(const S* p, A (&o)[4]) {
     o[1] = (A)__uint_as_float(v.y);
     o[2] = (A)__uint_as_float(v.z);
     o[3] = (A)__uint_as_float(v.w);
+  } else if constexpr (sizeof(S) == 3) {  // fp24: 12 bytes, four packed values
+    const u32x2a v = vload<NT, u32x2a>(p);
+    const uint32_t w = vload<NT, uint32_t>((const char*)p + 8);
+    uint32_t u[4];
+    f24x4_bits(v.x, v.y, w, u);
+    o[0] = (A)__uint_as_float(u[0]), o[1] = (A)__uint_as_float(u[1]);
+    o[2] = (A)__uint_as_float(u[2]), o[3] = (A)__uint_as_float(u[3]);
   } else if constexpr (sizeof(S) == 1) {  // fp8 (e4m3)
     const uint32_t v = vload<NT, uint32_t>(p);
     float a, b, c, d;
@@ -978,7 +985,7 @@ void ensure_block_scratch(Engine& e, int W, int dir) {
 }
 
 void ensure_mixed_scratch(Engine& e, int W) {
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < kMixClasses; ++c)
     if (!e.grpm[c].built) build_mixed_group_tables(e, c);
   ensure_w_bufs(e, W);
   W = e.w_cap;
@@ -988,9 +995,9 @@ void ensure_mixed_scratch(Engine& e, int W) {
     e.d_xw32 = dalloc<float>(e.N * W);
     e.xw32_w = W;
   }
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kMixClasses; ++c) {
     const int64_t need = ((int64_t)e.mix.cls[c].T + 1 + std::max<int64_t>(e.grpm[c].g_np, 1)) * W *
-                         (c == 0 ? (int64_t)sizeof(double) : (int64_t)sizeof(float));
+                         (class_acc64(c) ? (int64_t)sizeof(double) : (int64_t)sizeof(float));
     if (e.mtw_cap[c] >= need) continue;
     dfree(e.d_mtw[c]);
     e.mtw_cap[c] = 0;
@@ -1117,11 +1124,12 @@ void run_group_w(Engine& e, int prec, bool acc64, bool tr, const double* b, doub
   else run_group<uint16_t, float, W>(e, prec, acc64, tr, b, y, cnt);
 }
 
-// One group of W right-hand sides on the mixed copy: class 0 (fp32 factors,
-// fp64 sums and x), classes 1 and 2 (fp16 and fp8 factors, fp32 sums and x)
-// each through phase 1, the row-split reduce and phase 2 (scaled t; classes
-// 1 and 2 add to the level sums); the near field as the single product forms
-// it (fp64 sums), beside them on the second stream.
+// One group of W right-hand sides on the mixed copy: classes 0 and 3 (fp32
+// and fp24 factors, fp64 sums and x), classes 1 and 2 (fp16 and fp8 factors,
+// fp32 sums and x) each through phase 1, the row-split reduce and phase 2
+// (scaled t; every class after the first adds to the level sums); the near
+// field as the single product forms it (fp64 sums), beside them on the second
+// stream.
 template <int W>
 void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
   hipStream_t st = e.stream;
@@ -1186,6 +1194,7 @@ void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
     }
   };
   run_class(std::pair<float, double>{}, 0, (const double*)x64, (double*)e.d_mtw[0]);
+  run_class(std::pair<f24, double>{}, 3, (const double*)x64, (double*)e.d_mtw[3]);
   run_class(std::pair<_Float16, float>{}, 1, (const float*)x32, (float*)e.d_mtw[1]);
   run_class(std::pair<f8e4m3, float>{}, 2, (const float*)x32, (float*)e.d_mtw[2]);
   if (nn && !overlap) launch_near(st);
@@ -1218,7 +1227,7 @@ void release_group_tables(Engine& e) {
 
 void release_mixed_groups(Engine& e) {
   for (GroupTables& g : e.grpm) g.free_all();
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kMixClasses; ++c) {
     dfree(e.d_mtw[c]);
     e.mtw_cap[c] = 0;
   }

@@ -80,6 +80,21 @@ struct f8e4m3 {
   uint8_t bits;
 };
 
+// A stored 24-bit float: the upper three bytes of an fp32 (sign, 8 exponent
+// bits, 15 mantissa bits), rounded to nearest; entries are packed, 3 bytes each.
+struct f24 {
+  uint8_t b[3];
+};
+static_assert(sizeof(f24) == 3, "packed 3-byte entries");
+
+// Four packed fp24 values (12 bytes) as fp32 bit patterns.
+__device__ inline void f24x4_bits(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t (&o)[4]) {
+  o[0] = w0 << 8;
+  o[1] = ((w0 >> 24) << 8) | (w1 << 16);
+  o[2] = ((w1 >> 16) << 8) | (w2 << 24);
+  o[3] = w2 & 0xffffff00u;
+}
+
 // Two fp8 values of a 32-bit word (HI: bytes 2-3) as floats.
 template <bool HI>
 __device__ inline void f8x2_to_f(uint32_t w, float& a, float& b) {
@@ -101,6 +116,15 @@ struct Load<float> {
 template <>
 struct Load<uint16_t> {
   __device__ static inline float f(const uint16_t* p) { return bf16_to_f(ntload<uint16_t>(p)); }
+};
+template <>
+struct Load<f24> {
+  __device__ static inline float f(const f24* p) {
+    const uint8_t* b = (const uint8_t*)p;
+    const uint32_t u = ((uint32_t)ntload<uint8_t>(b) << 8) | ((uint32_t)ntload<uint8_t>(b + 1) << 16) |
+                       ((uint32_t)ntload<uint8_t>(b + 2) << 24);
+    return __uint_as_float(u);
+  }
 };
 template <>
 struct Load<f8e4m3> {

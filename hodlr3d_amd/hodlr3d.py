@@ -316,10 +316,10 @@ class HODLR3D:
             out["phase2_fp16"] = {w[i]: ms[12 + i] for i in range(4) if ms[12 + i] >= 0}
         return out
 
-    def set_mixed_formats(self, formats=("fp16", "fp8")):
+    def set_mixed_formats(self, formats=("fp16", "fp8", "fp24")):
         """Formats the next build of the mixed copy may store columns in besides
-        fp32: any of 'fp16' and 'fp8' (h3d_set_mixed_formats)."""
-        bits = {"fp16": MIXED_FP16, "fp8": MIXED_FP8}
+        fp32: any of 'fp16', 'fp8' and 'fp24' (h3d_set_mixed_formats)."""
+        bits = {"fp16": MIXED_FP16, "fp8": MIXED_FP8, "fp24": MIXED_FP24}
         mask = 0
         for f in formats:
             if f not in bits:
@@ -380,13 +380,14 @@ class HODLR3D:
 
     def kernel_timing(self, enable=True):
         """Returns accumulated per-kernel ms since the last call, then (re)arms timing."""
-        ms = (ctypes.c_double * 12)()
+        ms = (ctypes.c_double * 15)()
         cnt = ctypes.c_int()
         nat.check(nat.lib().h3d_kernel_timing(self._h, int(enable), ms, ctypes.byref(cnt)))
         return {"phase1": ms[0], "phase2": ms[1], "near": ms[2], "combine": ms[3],
                 "total": ms[4], "phase1_second": ms[5], "phase2_second": ms[6],
                 "phase1_kernel": ms[7], "phase1_kernel_second": ms[8],
                 "phase1_third": ms[9], "phase2_third": ms[10], "phase1_kernel_third": ms[11],
+                "phase1_fourth": ms[12], "phase2_fourth": ms[13], "phase1_kernel_fourth": ms[14],
                 "count": cnt.value}
 
     def set_diag(self, c):
@@ -566,6 +567,7 @@ class MatvecGraph:
 PERSISTENT = 0x100  # h3d_batch_set_tuning flag: the persistent (work-queue) launch
 MIXED_FP16 = 1      # h3d_set_mixed_formats bits
 MIXED_FP8 = 2
+MIXED_FP24 = 4
 
 
 class Batch:

@@ -122,11 +122,15 @@ typedef struct h3d_mixed_stats {
   double err_fp8;           /* part of predicted_rel_err from fp8 rounding */
   int64_t rank_fp8;         /* columns in fp8 (e4m3) */
   int64_t u_elems_fp8, v_elems_fp8;
+  double err_fp24;          /* part of predicted_rel_err from fp24 rounding */
+  int64_t rank_fp24;        /* columns in fp24 (an fp32 without its last 8 bits) */
+  int64_t u_elems_fp24, v_elems_fp24;
 } h3d_mixed_stats;
 
 /* Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats). */
 #define H3D_MIXED_FP16 1
 #define H3D_MIXED_FP8 2
+#define H3D_MIXED_FP24 4
 
 typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
@@ -186,20 +190,21 @@ int h3d_release_precision(h3d_handle* h, int prec);
  * per entry stays far below any useful budget): every block B = U V^T is
  * recompressed through the
  * eigendecomposition of its Gram matrices (B = U' V'^T, V' orthonormal), and
- * its columns are dropped, stored in fp8, fp16 or fp32 so that the expected
+ * its columns are dropped, stored in fp8, fp16, fp24 or fp32 so that the expected
  * added relative error ||A_mixed - A||_F / ||A||_F stays within `tol` (one
  * budget over all blocks, spent on the columns with the least error per stored
  * element).  Needs the fp64 or the fp32 copy resident; stats may be NULL.  A matvec with
  * H3D_PREC_MIXED builds the copy with tol = 5 * 10^-TOL_POW if absent. */
 int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* stats);
 /* Formats the next build of the recompressed copy may store columns in besides
- * fp32 (no reference counterpart): H3D_MIXED_FP16 | H3D_MIXED_FP8 (the
- * default), one of them, or 0 (fp32 columns only).  Each format is a tier of
- * the budget's choice: a column of the least error per stored element is
- * dropped, the next ones stored in fp8 (e4m3, 1 byte), fp16 (2 bytes) or fp32,
- * each tier over a fixed range of that error measure (the ranges minimise the
- * bytes for the budget); a format turned off is skipped.  The copy in place
- * is kept until the next h3d_compress. */
+ * fp32 (no reference counterpart): any of H3D_MIXED_FP16, H3D_MIXED_FP8 and
+ * H3D_MIXED_FP24 (all three by default), or 0 (fp32 columns only).  Each
+ * format is a tier of the budget's choice: a column of the least error per
+ * stored element is dropped, the next ones stored in fp8 (e4m3, 1 byte), fp16
+ * (2 bytes), fp24 (3 bytes: fp32 rounded to 16 significant bits) or fp32, each
+ * tier over a fixed range of that error measure (the ranges minimise the bytes
+ * for the budget); a format turned off is skipped.  The copy in place is kept
+ * until the next h3d_compress. */
 int h3d_set_mixed_formats(h3d_handle* h, int formats);
 int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* stats);
 
@@ -277,14 +282,15 @@ int h3d_get_tuning(h3d_handle* h, int prec, int* v);
 int h3d_set_tuning_state(h3d_handle* h, int prec, const int* v);
 
 /* Per-kernel timing of subsequent matvecs with HIP events on the handle's
- * stream (enable != 0).  ms[12] receives the accumulated milliseconds of
+ * stream (enable != 0).  ms[15] receives the accumulated milliseconds of
  * phase 1 (V^T x, with its partial-sum reduction and, for H3D_PREC_MIXED,
  * the gather into target order), phase 2 (U t), near field, combine and whole
  * matvecs since the last read; then the parts of phase 1 and phase 2 spent on
  * the second factor copy (the fp16 class of H3D_PREC_MIXED; 0 otherwise); then
  * the phase-1 kernel alone of the first and of the second copy; then the
  * parts of phase 1 and phase 2 spent on the third copy (the fp8 class) and its
- * phase-1 kernel alone.  *count gets
+ * phase-1 kernel alone; then the same three for the fourth copy (the fp24
+ * class).  *count gets
  * the number of matvecs; all are then reset.  Grouped products of several
  * right-hand sides (h3d_set_rhs_block) are not timed.  While timing is on,
  * every kernel of a matvec runs in line on the handle's stream; otherwise the

@@ -1,4 +1,4 @@
-"""Recompressed mixed-precision copy (fp32 + fp16 + fp8 columns, fp64 accumulation):
+"""Recompressed mixed-precision copy (fp32 + fp24 + fp16 + fp8 columns, fp64 accumulation):
 error against the direct FP64 product within the requested budget, adjoint,
 determinism, every load width, and refined solves on it (GPU).
 
@@ -39,11 +39,12 @@ def _rows(N, k=2048, seed=3):
 def test_stats_are_consistent(built):
     name, x, h, tol, st = built
     assert st["tol"] == tol
-    assert st["rank_fp32"] + st["rank_fp16"] + st["rank_fp8"] + st["dropped"] == st["aca_rank_sum"]
-    assert st["formats"] == 3 and st["rank_fp8"] > 0 and st["rank_fp16"] > 0
+    assert (st["rank_fp32"] + st["rank_fp24"] + st["rank_fp16"] + st["rank_fp8"] + st["dropped"]
+            == st["aca_rank_sum"])
+    assert st["formats"] == 7 and st["rank_fp8"] > 0 and st["rank_fp16"] > 0 and st["rank_fp24"] > 0
     assert 0 <= st["predicted_rel_err"] <= tol * (1 + 1e-9)
     parts = np.sqrt(st["err_drop"] ** 2 + st["err_fp8"] ** 2 + st["err_fp16"] ** 2 +
-                    st["err_fp32"] ** 2)
+                    st["err_fp24"] ** 2 + st["err_fp32"] ** 2)
     assert abs(parts - st["predicted_rel_err"]) <= 1e-9 * tol
     full = h.stats()
     assert st["aca_rank_sum"] == full["rank_sum"]
@@ -151,6 +152,7 @@ def test_tiny_budget_keeps_every_column_in_fp32():
     h.assemble()
     st = h.compress(1e-15)
     assert st["dropped"] == 0 and st["rank_fp16"] == 0 and st["rank_fp8"] == 0
+    assert st["rank_fp24"] == 0
     assert st["err_drop"] == 0 and st["err_fp16"] == 0 and st["err_fp32"] < 1e-7
     q = np.random.default_rng(9).standard_normal(len(x))
     y64 = h.matvec(q, prec="fp64")
@@ -213,7 +215,7 @@ def test_two_pass_build_matches_the_single_jacobi_build(built):
     finally:
         nat.lib().h3d_debug_fail_alloc(-1)
     assert st2["single_jacobi"] == 0
-    for k in ("rank_fp32", "rank_fp16", "rank_fp8", "dropped", "raw_blocks", "bytes"):
+    for k in ("rank_fp32", "rank_fp24", "rank_fp16", "rank_fp8", "dropped", "raw_blocks", "bytes"):
         assert st2[k] == st[k], k
     assert st2["predicted_rel_err"] == st["predicted_rel_err"]
     y2 = h.matvec(q, prec="mixed")
@@ -222,9 +224,9 @@ def test_two_pass_build_matches_the_single_jacobi_build(built):
 
 
 def test_storage_formats_are_tiers_of_one_budget(built):
-    """fp16 and fp8 are tiers the budget may use: with one turned off its
-    columns go to a neighbouring tier, every copy meets the budget, and the
-    copy with both tiers stores the fewest bytes.  Turning both off leaves
+    """fp8, fp16 and fp24 are tiers the budget may use: with one turned off
+    its columns go to a neighbouring tier, every copy meets the budget, and
+    the copy with every tier stores the fewest bytes.  Turning all off leaves
     fp32 columns only."""
     import hodlr3d_amd as H3
     name, x, h, tol, st = built
@@ -232,25 +234,28 @@ def test_storage_formats_are_tiers_of_one_budget(built):
     y64 = h.matvec(q, prec="fp64")
     y_all = h.matvec(q, prec="mixed")
     nbytes = {}
-    for fm, mask in ((("fp16",), 1), (("fp8",), 2), ((), 0), (("fp16", "fp8"), 3)):
+    ALL = ("fp16", "fp8", "fp24")
+    for fm, mask in ((("fp16",), 1), (("fp8",), 2), (("fp24",), 4), ((), 0), (("fp16", "fp8"), 3),
+                     (ALL, 7)):
         s = h.compress(tol, formats=fm)
         assert s["formats"] == mask
-        if "fp8" not in fm:
-            assert s["rank_fp8"] == 0 and s["err_fp8"] == 0
-        if "fp16" not in fm:
-            assert s["rank_fp16"] == 0 and s["err_fp16"] == 0
-        assert s["rank_fp32"] + s["rank_fp16"] + s["rank_fp8"] + s["dropped"] == s["aca_rank_sum"]
+        for f in ALL:
+            if f not in fm:
+                assert s["rank_" + f] == 0 and s["err_" + f] == 0, f
+        assert (s["rank_fp32"] + s["rank_fp24"] + s["rank_fp16"] + s["rank_fp8"] + s["dropped"]
+                == s["aca_rank_sum"])
         assert s["predicted_rel_err"] <= tol * (1 + 1e-9)
         ym = h.matvec(q, prec="mixed")
         dm = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
-        print(f"[{name}] formats {fm}: {s['bytes'] / 1e6:.1f} MB, ranks fp32/fp16/fp8 "
-              f"{s['rank_fp32']}/{s['rank_fp16']}/{s['rank_fp8']}, dropped {s['dropped']}, "
-              f"vs fp64 {dm:.2e}", flush=True)
+        print(f"[{name}] formats {fm}: {s['bytes'] / 1e6:.1f} MB, ranks fp32/fp24/fp16/fp8 "
+              f"{s['rank_fp32']}/{s['rank_fp24']}/{s['rank_fp16']}/{s['rank_fp8']}, dropped "
+              f"{s['dropped']}, vs fp64 {dm:.2e}", flush=True)
         assert dm <= 2.0 * tol
         nbytes[fm] = s["bytes"]
+    assert nbytes[ALL] <= 1.001 * min(v for k, v in nbytes.items() if k != ALL)
     assert nbytes[("fp16", "fp8")] <= 1.001 * min(nbytes[("fp16",)], nbytes[("fp8",)])
-    assert max(nbytes[("fp16",)], nbytes[("fp8",)]) <= nbytes[()]
-    # the last build used both formats again: the same copy as the fixture's
+    assert max(nbytes[("fp16",)], nbytes[("fp8",)], nbytes[("fp24",)]) <= nbytes[()]
+    # the last build used every format again: the same copy as the fixture's
     assert np.array_equal(h.matvec(q, prec="mixed"), y_all)
     with pytest.raises(H3.H3DError):
         h.set_mixed_formats(("bf16",))
