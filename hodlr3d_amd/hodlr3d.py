@@ -32,7 +32,7 @@ PRECISIONS = {"fp64": nat.PREC_FP64, "fp32": nat.PREC_FP32, "bf16": nat.PREC_BF1
 
 def _prec(p):
     """'fp64' | 'fp32' | 'bf16', optionally with '/acc64' (fp64 accumulation), or
-    'mixed' (the recompressed fp32/fp16 copy built by ``compress``)."""
+    'mixed' (the recompressed fp32/fp24/fp16/fp8 copy built by ``compress``)."""
     if isinstance(p, str):
         base, _, acc = p.partition("/")
         v = PRECISIONS[base]

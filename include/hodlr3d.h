@@ -246,7 +246,9 @@ int h3d_set_operator(h3d_handle* h, double alpha, double diag);
  * per-lane load width (16/8/4/2 bytes) on the assembled problem, `reps`
  * launches each, and keeps the fastest for later matvecs.  ms (may be NULL)
  * receives phase-1 then phase-2 averages per width, -1 where a width does not
- * apply: 8 values, or 16 for H3D_PREC_MIXED (fp32 class, then fp16 class). */
+ * apply: 8 values, or 16 for H3D_PREC_MIXED (fp32 class, then fp16 class; the
+ * fp24 class runs with the fp32 class's settings, the fp8 class with the fp16
+ * class's, and fp24 always loads four entries per lane in both phases). */
 int h3d_autotune(h3d_handle* h, int prec, int reps, double* ms);
 
 /* Products with several right-hand sides (h3d_matvec with nrhs >= 2 on an
@@ -273,10 +275,10 @@ int h3d_set_tuning(h3d_handle* h, int prec, int p1_bytes, int p2_bytes);
 
 /* The complete tuning state of mode `prec` (the autotuner's choice or what was
  * pinned): v[0], v[1] = phase-1 / phase-2 load bytes per lane (for
- * H3D_PREC_MIXED those of its fp32 class), v[2], v[3] = those of the mixed
- * copy's fp16 class, v[4], v[5] = the mixed phase-1 kernel's load vectors of
- * rows per lane per class (4 or 1; 0: chosen per item); 0 where a slot does
- * not apply.  h3d_set_tuning_state pins a state read back earlier, so that a
+ * H3D_PREC_MIXED those of its fp32 and fp24 classes), v[2], v[3] = those of the
+ * mixed copy's fp16 and fp8 classes, v[4], v[5] = the mixed phase-1 kernel's
+ * load vectors of rows per lane for each of the two groups (4 or 1; 0: chosen
+ * per item); 0 where a slot does not apply.  h3d_set_tuning_state pins a state read back earlier, so that a
  * profiled run can time exactly the kernel variants an autotuned run used. */
 int h3d_get_tuning(h3d_handle* h, int prec, int* v);
 int h3d_set_tuning_state(h3d_handle* h, int prec, const int* v);
