@@ -1221,7 +1221,7 @@ void Plan::free_device() {
 // (64 lanes x one 16-byte vector: 128 fp64, 256 fp32 or 512 fp16/bf16 rows), so
 // that a wave instruction reads 1 KB of one column when the item spans that
 // many rows; larger items first (less tail at the end of a launch).
-static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int p2rows,
+static void p2_items(const Engine& e, const std::vector<const LevelFactors*>& lfs, int p2rows,
                      int class_elem, std::vector<P2Item>& out) {
   out.clear();
   for (int j = 1; j <= e.nL; ++j) {
@@ -1252,10 +1252,9 @@ static void p2_items(Engine& e, const std::vector<const LevelFactors*>& lfs, int
   });
 }
 
-void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
-                    int p2rows, int class_elem) {
+void build_plan_host(const Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
+                     int p2rows, int class_elem) {
   const int nL = e.nL;
-  hipStream_t st = e.stream;
   out.p1.clear();
   out.p1r.clear();
   out.p2.clear();
@@ -1357,6 +1356,10 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
   }
   p2_items(e, lfs, p2rows, class_elem, out.p2);
   out.npartial = std::max(pslot, tslot);
+}
+
+void upload_plan(Engine& e, Plan& out) {
+  hipStream_t st = e.stream;
   out.free_device();
   out.d_p1 = upload(out.p1, st);
   out.d_p1r = upload(out.p1r, st);
@@ -1365,6 +1368,12 @@ void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan
   out.d_tp1r = upload(out.tp1r, st);
   out.d_tp2 = upload(out.tp2, st);
   H3D_HIP(hipStreamSynchronize(st));
+}
+
+void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lfs, Plan& out,
+                    int p2rows, int class_elem) {
+  build_plan_host(e, lfs, out, p2rows, class_elem);
+  upload_plan(e, out);
 }
 
 void build_plan(Engine& e) {

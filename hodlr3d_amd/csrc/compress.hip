@@ -26,6 +26,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -799,6 +800,25 @@ __global__ void pack_lam_kernel(const CDesc* __restrict__ ds, int nblk,
   flo[i] = W[d.r + 1];
 }
 
+// Runs f(c) for every class on a host thread of its own; the first exception
+// of any of them is rethrown after all have ended.
+template <class F>
+void run_per_class(F&& f) {
+  std::exception_ptr err[kClasses];
+  std::vector<std::thread> th;
+  for (int c = 0; c < kClasses; ++c)
+    th.emplace_back([&, c] {
+      try {
+        f(c);
+      } catch (...) {
+        err[c] = std::current_exception();
+      }
+    });
+  for (auto& t : th) t.join();
+  for (auto& x : err)
+    if (x) std::rethrow_exception(x);
+}
+
 inline double pow2_near_sqrt(int n) {
   return std::ldexp(1.0, (int)std::lround(0.5 * std::log2((double)std::max(n, 1))));
 }
@@ -1377,149 +1397,153 @@ void build_mixed(Engine& e, double tol) {
     fc.tmap.clear();
   }
   std::vector<PDesc> pd(nb);
-  int32_t tsrc_cur[kClasses] = {0, 0, 0};
   std::vector<int32_t> vch[kClasses];
   for (int c = 0; c < kClasses; ++c) vch[c].assign(e.nblk_all, 0);
-  {
-    int32_t T[kClasses] = {0, 0, 0};
+  // per level: the target box of each block and the blocks per source box
+  std::vector<std::vector<int32_t>> nsrc_l(nL + 1), tgt_l(nL + 1);
+  for (int j = 1; j <= nL; ++j) {
+    const Level& lv = e.lev[j];
+    const int64_t lo = level_offset(j);
+    nsrc_l[j].assign(lv.nboxes, 0);
+    tgt_l[j].assign(lv.ilidx.size(), 0);
+    for (int32_t ki : lv.ilidx) nsrc_l[j][ki]++;
+    for (int64_t k = 0; k < lv.nboxes; ++k) {
+      const int n = e.bcount[lo + k];
+      for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+        tgt_l[j][b] = (int32_t)k;
+        const int64_t i = ci[lv.bbase + b];
+        if (i < 0) continue;
+        const int m = e.bcount[lo + lv.ilidx[b]];
+        for (int c = 0; c < kClasses; ++c) {
+          pd[i].rc[c] = rk[i][c];
+          pd[i].ldv_o[c] = class_ldv(m, kClassElem[c]);
+        }
+        pd[i].raw = raw[i];
+        pd[i].su = raw[i] ? 1.0 : pow2_near_sqrt(n);
+        pd[i].sv = raw[i] ? 1.0 : pow2_near_sqrt(m);
+      }
+    }
+  }
+  // each class on a host thread of its own (its layout depends only on its
+  // own ranks; the threads write disjoint fields)
+  auto layout_class = [&](int c) {
+    FactorClass& fc = mx.cls[c];
+    int32_t T = 0, tsrc_cur = 0;
     for (int j = 1; j <= nL; ++j) {
       const Level& lv = e.lev[j];
       const int64_t lo = level_offset(j);
       const int64_t nblk = (int64_t)lv.ilidx.size();
-      for (int c = 0; c < kClasses; ++c) {
-        LevelFactors& f = mx.cls[c].lf[j];
-        f.rank.assign(nblk, 0);
-        f.col.assign(nblk, 0);
-        f.voff.assign(nblk, 0);
-        f.Rt.assign(lv.nboxes, 0);
-        f.toff.assign(lv.nboxes, 0);
-        f.uoff.assign(lv.nboxes, 0);
-      }
+      LevelFactors& f = fc.lf[j];
+      f.rank.assign(nblk, 0);
+      f.col.assign(nblk, 0);
+      f.voff.assign(nblk, 0);
+      f.Rt.assign(lv.nboxes, 0);
+      f.toff.assign(lv.nboxes, 0);
+      f.uoff.assign(lv.nboxes, 0);
       for (int64_t k = 0; k < lv.nboxes; ++k) {
         const int n = e.bcount[lo + k];
         const int ch = lv.chunk[k];
-        int32_t R[kClasses] = {0, 0, 0};
+        int32_t R = 0;
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
           const int64_t i = ci[lv.bbase + b];
           if (i < 0) continue;
           const int m = e.bcount[lo + lv.ilidx[b]];
-          const std::array<int32_t, kClasses>& rc = rk[i];
-          for (int c = 0; c < kClasses; ++c) {
-            FactorClass& fc = mx.cls[c];
-            LevelFactors& f = fc.lf[j];
-            f.rank[b] = rc[c];
-            f.col[b] = R[c];
-            R[c] += rc[c];
-            fc.rank_sum += rc[c];
-            fc.u_elems += (int64_t)rc[c] * n;
-            fc.v_elems += (int64_t)rc[c] * m;
-          }
-          for (int c = 0; c < kClasses; ++c) {
-            pd[i].rc[c] = rc[c];
-            pd[i].ldv_o[c] = class_ldv(m, kClassElem[c]);
-          }
-          pd[i].raw = raw[i];
-          pd[i].su = raw[i] ? 1.0 : pow2_near_sqrt(n);
-          pd[i].sv = raw[i] ? 1.0 : pow2_near_sqrt(m);
+          const int r = rk[i][c];
+          f.rank[b] = r;
+          f.col[b] = R;
+          R += r;
+          fc.rank_sum += r;
+          fc.u_elems += (int64_t)r * n;
+          fc.v_elems += (int64_t)r * m;
         }
-        for (int c = 0; c < kClasses; ++c) {
-          FactorClass& fc = mx.cls[c];
-          LevelFactors& f = fc.lf[j];
-          f.Rt[k] = R[c];
-          f.uoff[k] = class_ustart(fc.nU[ch], n, fc.elem);
-          fc.nU[ch] = f.uoff[k] + (int64_t)class_ldu(n, fc.elem) * R[c];
-          f.toff[k] = T[c];
-          T[c] += R[c];
-        }
+        const int ldu = class_ldu(n, fc.elem);
+        f.Rt[k] = R;
+        f.uoff[k] = class_ustart(fc.nU[ch], n, fc.elem);
+        fc.nU[ch] = f.uoff[k] + (int64_t)ldu * R;
+        f.toff[k] = T;
+        T += R;
         for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
           const int64_t i = ci[lv.bbase + b];
           if (i < 0) continue;
-          for (int c = 0; c < kClasses; ++c) {
-            const LevelFactors& f = mx.cls[c].lf[j];
-            const int ldu = class_ldu(n, mx.cls[c].elem);
-            pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * ldu;
-            pd[i].ldu_o[c] = ldu;
-          }
+          pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * ldu;
+          pd[i].ldu_o[c] = ldu;
         }
       }
       // V by source box; the by-source list of level j occupies
       // e.by_src[bbase, bbase + nblk), grouped by source box in order
-      std::vector<int32_t> nsrc(lv.nboxes, 0), tgt(nblk, 0);
-      for (int32_t ki : lv.ilidx) nsrc[ki]++;
-      for (int64_t k = 0; k < lv.nboxes; ++k)
-        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) tgt[b] = (int32_t)k;
-      for (int c = 0; c < kClasses; ++c) mx.cls[c].tmap.resize(T[c], 0);
+      const std::vector<int32_t>& nsrc = nsrc_l[j];
+      const std::vector<int32_t>& tgt = tgt_l[j];
+      fc.tmap.resize(T, 0);
       int64_t pos = lv.bbase;
       for (int64_t sb = 0; sb < lv.nboxes; ++sb) {
         const int m = e.bcount[lo + sb];
-        for (int c = 0; c < kClasses; ++c) {
-          FactorClass& fc = mx.cls[c];
-          LevelFactors& f = fc.lf[j];
-          const int ld = class_ldv(m, fc.elem);
-          int64_t C = 0;
-          for (int32_t q = 0; q < nsrc[sb]; ++q) C += f.rank[e.by_src[pos + q] - lv.bbase];
-          if (fc.nV.back() > 0 && fc.nV.back() + C * ld > kVChunkElems) fc.nV.push_back(0);
-          const int32_t vc = (int32_t)fc.nV.size() - 1;
-          const int64_t vs = fc.nV.back();  // V_s column 0
-          const int32_t tsb = tsrc_cur[c];  // source-order t of V_s column 0
-          tsrc_cur[c] += (int32_t)C;
-          int32_t col = 0;
-          for (int32_t q = 0; q < nsrc[sb]; ++q) {
-            const int32_t gid = e.by_src[pos + q];
-            const int32_t b = gid - (int32_t)lv.bbase;
-            const int r = f.rank[b];
-            f.voff[b] = vs + (int64_t)col * ld;
-            vch[c][gid] = vc;
-            const int32_t tt = f.toff[tgt[b]] + f.col[b];
-            for (int l = 0; l < r; ++l) fc.tmap[tt + l] = tsb + col + l;
-            const int64_t i = ci[gid];
-            if (i >= 0) {
-              pd[i].v_out[c] = f.voff[b];
-              pd[i].vch[c] = vc;
-            }
-            col += r;
+        const int ld = class_ldv(m, fc.elem);
+        int64_t C = 0;
+        for (int32_t q = 0; q < nsrc[sb]; ++q) C += f.rank[e.by_src[pos + q] - lv.bbase];
+        if (fc.nV.back() > 0 && fc.nV.back() + C * ld > kVChunkElems) fc.nV.push_back(0);
+        const int32_t vc = (int32_t)fc.nV.size() - 1;
+        const int64_t vs = fc.nV.back();  // V_s column 0
+        const int32_t tsb = tsrc_cur;     // source-order t of V_s column 0
+        tsrc_cur += (int32_t)C;
+        int32_t col = 0;
+        for (int32_t q = 0; q < nsrc[sb]; ++q) {
+          const int32_t gid = e.by_src[pos + q];
+          const int32_t b = gid - (int32_t)lv.bbase;
+          const int r = f.rank[b];
+          f.voff[b] = vs + (int64_t)col * ld;
+          vch[c][gid] = vc;
+          const int32_t tt = f.toff[tgt[b]] + f.col[b];
+          for (int l = 0; l < r; ++l) fc.tmap[tt + l] = tsb + col + l;
+          const int64_t i = ci[gid];
+          if (i >= 0) {
+            pd[i].v_out[c] = f.voff[b];
+            pd[i].vch[c] = vc;
           }
-          fc.nV.back() += C * ld;
-          // phase-1 items: rows in ranges of <= kMixRows, columns in runs of
-          // about kMixItemBytes of factors
-          if (m == 0 || C == 0) continue;
-          const int kMixRows = mix_item_rows(fc.elem);
-          const int nsplit = (m + kMixRows - 1) / kMixRows;
-          const int rows0 = std::min(m, kMixRows);
-          const int ncol = (int)std::max<int64_t>(
-              16, std::min<int64_t>(4096, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
-          for (int32_t c0 = 0; c0 < C; c0 += ncol) {
-            const int32_t c1 = (int32_t)std::min<int64_t>(C, (int64_t)c0 + ncol);
-            int32_t pbase = -1;
-            if (nsplit > 1) {
-              pbase = fc.mpartial;
-              fc.mpartial += nsplit * (c1 - c0);
-              fc.mp1r.push_back({pbase, nsplit, c1 - c0, tsb + c0});
-            }
-            for (int q = 0; q < nsplit; ++q) {
-              MixP1Item it{};
-              it.voff = vs;
-              it.vchunk = vc;
-              it.ld = ld;
-              it.s0 = q * kMixRows;
-              it.s1 = std::min(m, (q + 1) * kMixRows);
-              it.c0 = c0;
-              it.c1 = c1;
-              it.tsb = tsb;
-              it.xoff = e.bstart[lo + sb];
-              it.pslot = nsplit > 1 ? pbase + q * (c1 - c0) : -1;
-              fc.mp1.push_back(it);
-            }
+          col += r;
+        }
+        fc.nV.back() += C * ld;
+        pos += nsrc[sb];
+        // phase-1 items: rows in ranges of <= kMixRows, columns in runs of
+        // about kMixItemBytes of factors
+        if (m == 0 || C == 0) continue;
+        const int kMixRows = mix_item_rows(fc.elem);
+        const int nsplit = (m + kMixRows - 1) / kMixRows;
+        const int rows0 = std::min(m, kMixRows);
+        const int ncol = (int)std::max<int64_t>(
+            16, std::min<int64_t>(4096, kMixItemBytes / ((int64_t)rows0 * fc.elem)));
+        for (int32_t c0 = 0; c0 < C; c0 += ncol) {
+          const int32_t c1 = (int32_t)std::min<int64_t>(C, (int64_t)c0 + ncol);
+          int32_t pbase = -1;
+          if (nsplit > 1) {
+            pbase = fc.mpartial;
+            fc.mpartial += nsplit * (c1 - c0);
+            fc.mp1r.push_back({pbase, nsplit, c1 - c0, tsb + c0});
+          }
+          for (int q = 0; q < nsplit; ++q) {
+            MixP1Item it{};
+            it.voff = vs;
+            it.vchunk = vc;
+            it.ld = ld;
+            it.s0 = q * kMixRows;
+            it.s1 = std::min(m, (q + 1) * kMixRows);
+            it.c0 = c0;
+            it.c1 = c1;
+            it.tsb = tsb;
+            it.xoff = e.bstart[lo + sb];
+            it.pslot = nsplit > 1 ? pbase + q * (c1 - c0) : -1;
+            fc.mp1.push_back(it);
           }
         }
-        pos += nsrc[sb];
       }
     }
+    fc.T = T;
+  };
+  run_per_class(layout_class);
+  {
     int32_t tb = 0;
     for (int c = 0; c < kClasses; ++c) {
-      mx.cls[c].T = T[c];
       mx.cls[c].tbase = tb;
-      tb += T[c];
+      tb += mx.cls[c].T;
     }
   }
   // allocate and zero the class chunks: one allocation per class (a few large
@@ -1552,6 +1576,89 @@ void build_mixed(Engine& e, double tol) {
     fc.d_ubase = upload(fc.udata, st);
     fc.d_vbase = upload(fc.vdata, st);
   }
+  // per column factor c_k = nu_k nv_k / (su sv) = sqrt(lam_k) / (su sv) in t
+  // order (fp8 columns: / kFp8Gain^2; raw columns: 1), block tables and the
+  // classes' work lists: formed on a host thread while the GPU runs pass 2
+  // (they need the layouts only), uploaded after it
+  std::vector<double> scale;
+  std::vector<float> scale32[kClasses];
+  std::vector<BlockDev> blk[kClasses];
+  std::vector<int32_t> btoff[kClasses];
+  std::exception_ptr tables_err;
+  auto tables = [&] {
+    scale.assign((size_t)mx.cls[0].T + mx.cls[1].T + mx.cls[2].T + mx.cls[3].T, 1.0);
+    for (int c = 0; c < kClasses; ++c) {
+      blk[c].assign(e.nblk_all, BlockDev{});
+      btoff[c].assign(e.nblk_all, 0);
+    }
+    for (int j = 1; j <= nL; ++j) {
+      const Level& lv = e.lev[j];
+      const int64_t lo = level_offset(j);
+      for (int64_t k = 0; k < lv.nboxes; ++k)
+        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
+          const int32_t ki = lv.ilidx[b];
+          for (int c = 0; c < kClasses; ++c) {
+            const FactorClass& fc = mx.cls[c];
+            const LevelFactors& f = fc.lf[j];
+            BlockDev d{};
+            d.voff = f.voff[b];
+            d.col = f.col[b];
+            d.r = f.rank[b];
+            d.m = e.bcount[lo + ki];
+            d.src = e.bstart[lo + ki];
+            d.chunk = vch[c][lv.bbase + b];
+            blk[c][lv.bbase + b] = d;
+            btoff[c][lv.bbase + b] = f.toff[k] + f.col[b];
+          }
+          const int64_t i = ci[lv.bbase + b];
+          if (i < 0 || raw[i]) continue;
+          const double inv = 1.0 / (pd[i].su * pd[i].sv);
+          const double fl = lam_floor[i];
+          int k0 = 0;  // first sorted column of the class
+          for (int c : kColumnOrder) {
+            const LevelFactors& f = mx.cls[c].lf[j];
+            const double g = c == 2 ? inv / (kFp8Gain * kFp8Gain) : inv;
+            for (int kk = 0; kk < rk[i][c]; ++kk)
+              scale[(size_t)mx.cls[c].tbase + f.toff[k] + f.col[b] + kk] =
+                  std::sqrt(std::max(lam_all[lam_off[i] + k0 + kk], fl)) * g;
+            k0 += rk[i][c];
+          }
+        }
+    }
+    // classes 1 and 2 sum in fp32: their factors converted once (the value
+    // the gather would convert to)
+    for (int c = 1; c < kClasses; ++c) {
+      if (class_acc64(c)) continue;
+      scale32[c].resize(mx.cls[c].T);
+      for (int32_t q = 0; q < mx.cls[c].T; ++q)
+        scale32[c][q] = (float)scale[(size_t)mx.cls[c].tbase + q];
+    }
+    for (int c = 0; c < kClasses; ++c) {
+      FactorClass& fc = mx.cls[c];
+      std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
+      for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
+      // about 1 KB of rows per phase-2 item (a multiple of 4 rows for fp24, so
+      // that its 4-entry vectors start 4-byte aligned); phase 1 runs on the
+      // source-grouped items, so the plan keeps no phase-1 list
+      build_plan_host(e, lfs, fc.plan, fc.elem == 3 ? 256 : 1024 / fc.elem, fc.elem);
+      fc.plan.p1.clear();
+      fc.plan.p1r.clear();
+    }
+  };
+  std::thread tables_thread([&] {
+    try {
+      tables();
+    } catch (...) {
+      tables_err = std::current_exception();
+    }
+  });
+  // joined on every exit from here (pass 2 may throw)
+  struct Joiner {
+    std::thread& t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } joiner{tables_thread};
   // ---------------- pass 2: the projections (from the stored J, or from the
   // sweeps again), then project into the classes; blocks that keep no column
   // and raw blocks need no eigen step
@@ -1596,72 +1703,17 @@ void build_mixed(Engine& e, double tol) {
   dfree(d_cd);
   dfree(d_list);
   dfree(d_b64);
-  // per column factor c_k = nu_k nv_k / (su sv) = sqrt(lam_k) / (su sv) in t
-  // order (fp8 columns: / kFp8Gain^2; raw columns: 1), block tables and plans
-  // of every class
+  tables_thread.join();
+  if (tables_err) std::rethrow_exception(tables_err);
   {
-    std::vector<double> scale((size_t)mx.cls[0].T + mx.cls[1].T + mx.cls[2].T + mx.cls[3].T, 1.0);
-    std::vector<BlockDev> blk[kClasses];
-    std::vector<int32_t> btoff[kClasses];
-    for (int c = 0; c < kClasses; ++c) {
-      blk[c].assign(e.nblk_all, BlockDev{});
-      btoff[c].assign(e.nblk_all, 0);
-    }
-    for (int j = 1; j <= nL; ++j) {
-      const Level& lv = e.lev[j];
-      const int64_t lo = level_offset(j);
-      for (int64_t k = 0; k < lv.nboxes; ++k)
-        for (int32_t b = lv.iloff[k]; b < lv.iloff[k + 1]; ++b) {
-          const int32_t ki = lv.ilidx[b];
-          for (int c = 0; c < kClasses; ++c) {
-            const FactorClass& fc = mx.cls[c];
-            const LevelFactors& f = fc.lf[j];
-            BlockDev d{};
-            d.voff = f.voff[b];
-            d.col = f.col[b];
-            d.r = f.rank[b];
-            d.m = e.bcount[lo + ki];
-            d.src = e.bstart[lo + ki];
-            d.chunk = vch[c][lv.bbase + b];
-            blk[c][lv.bbase + b] = d;
-            btoff[c][lv.bbase + b] = f.toff[k] + f.col[b];
-          }
-          const int64_t i = ci[lv.bbase + b];
-          if (i < 0 || raw[i]) continue;
-          const double inv = 1.0 / (pd[i].su * pd[i].sv);
-          const double fl = lam_floor[i];
-          int k0 = 0;  // first sorted column of the class
-          for (int c : kColumnOrder) {
-            const LevelFactors& f = mx.cls[c].lf[j];
-            const double g = c == 2 ? inv / (kFp8Gain * kFp8Gain) : inv;
-            for (int kk = 0; kk < rk[i][c]; ++kk)
-              scale[(size_t)mx.cls[c].tbase + f.toff[k] + f.col[b] + kk] =
-                  std::sqrt(std::max(lam_all[lam_off[i] + k0 + kk], fl)) * g;
-            k0 += rk[i][c];
-          }
-        }
-    }
     mx.d_scale = upload(scale, st);
-    // classes 1 and 2 sum in fp32: their factors converted once (the value
-    // the gather would convert to), held until the stream synchronizes
-    for (int c = 1; c < kClasses; ++c) {
-      if (class_acc64(c)) continue;
-      std::vector<float> scale32(mx.cls[c].T);
-      for (int32_t q = 0; q < mx.cls[c].T; ++q)
-        scale32[q] = (float)scale[(size_t)mx.cls[c].tbase + q];
-      mx.cls[c].d_scale32 = upload(scale32, st);
-    }
+    for (int c = 1; c < kClasses; ++c)
+      if (!class_acc64(c)) mx.cls[c].d_scale32 = upload(scale32[c], st);
     for (int c = 0; c < kClasses; ++c) {
       FactorClass& fc = mx.cls[c];
       fc.d_blk = upload(blk[c], st);
       fc.d_btoff = upload(btoff[c], st);
-      std::vector<const LevelFactors*> lfs(nL + 1, nullptr);
-      for (int j = 1; j <= nL; ++j) lfs[j] = &fc.lf[j];
-      // about 1 KB of rows per phase-2 item (a multiple of 4 rows for fp24, so
-      // that its 4-entry vectors start 4-byte aligned)
-      build_plan_for(e, lfs, fc.plan, fc.elem == 3 ? 256 : 1024 / fc.elem, fc.elem);
-      fc.plan.p1.clear();  // phase 1 runs on the source-grouped items
-      fc.plan.p1r.clear();
+      upload_plan(e, fc.plan);
       fc.d_mp1 = upload(fc.mp1, st);
       fc.d_mp1r = upload(fc.mp1r, st);
       fc.d_tmap = upload(fc.tmap, st);

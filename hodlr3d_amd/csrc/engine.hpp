@@ -556,6 +556,11 @@ void release_mixed(Engine& e);
 // bytes per entry of a mixed class (U stride class_ldu)
 void build_plan_for(Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out, int p2rows,
                     int class_elem = 0);
+// The same in two steps: the work lists on the host (reads e only; safe on a
+// host thread while the stream works), then their device copies.
+void build_plan_host(const Engine& e, const std::vector<const LevelFactors*>& lf, Plan& out,
+                     int p2rows, int class_elem);
+void upload_plan(Engine& e, Plan& out);
 // krylov.hip
 int gmres_reference(Engine& e, const double* d_b, double* d_x, int maxit, double thr,
                     double* err, int* its, double* hist);
