@@ -386,13 +386,41 @@ __device__ inline void load_xvec(const A* p, A (&o)[VEC]) {
   }
 }
 
+// One DPP move of a 32- or 64-bit value (CTRL: the instruction's dpp_ctrl).
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over aligned groups of g lanes (g a power of two <= 64, wave-uniform);
+// every lane of a group gets the group's sum.  Within a row of 16 lanes the
+// partners come by DPP (quad permutes for 1 and 2 lanes apart, then the
+// half-row and row mirrors, whose partners lie in the other half), so that
+// the sum takes no LDS round trip; across rows by swizzles.
+template <class A>
+__device__ inline A group_sum(A a, int g) {
+  if (g >= 2) a += dpp_move<0xB1>(a);   // quad_perm [1,0,3,2]
+  if (g >= 4) a += dpp_move<0x4E>(a);   // quad_perm [2,3,0,1]
+  if (g >= 8) a += dpp_move<0x141>(a);  // row_half_mirror
+  if (g >= 16) a += dpp_move<0x140>(a); // row_mirror
+  if (g >= 32) a += __shfl_xor(a, 16);
+  if (g >= 64) a += __shfl_xor(a, 32);
+  return a;
+}
+
 // ---------------------------------------------------------- mixed phase 1
 // t = V^T x for one class of the recompressed copy, grouped by source box: an
 // item is a run of columns of V_s (the columns of every block with source s,
 // concatenated) over at most 64 x KMAX load vectors of rows (1024 fp32, 2048
 // fp16 or 2048 fp8 rows with 16-byte loads), so x is loaded once per item.  lpc lanes
 // share a column, each holding K <= kmax load vectors of VEC rows (x in
-// registers), and reduce with an xor butterfly.  t is written in source order
+// registers), and reduce with group_sum.  t is written in source order
 // (t_src[tsb + c], contiguous stores); a gather puts it in target order.  A
 // load starts inside the rows, so it ends less than VEC elements past them
 // (next column, padding or the chunk's zero tail).
@@ -453,7 +481,7 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
   const int nu = (it.c1 - it.c0 + cols - 1) / cols;
   if (nu <= 0) return;  // no columns (the first prefetch would read column c0 - 1)
   auto finish = [&](int c, A a) {
-    for (int d = lpc >> 1; d >= 1; d >>= 1) a += __shfl_xor(a, d);
+    a = group_sum(a, lpc);
     if (c < it.c1 && sub == 0) {
       if (it.pslot < 0) tsrc[it.tsb + c] = a;
       else part[(int64_t)it.pslot + (c - it.c0)] = a;
