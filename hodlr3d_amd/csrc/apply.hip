@@ -37,6 +37,34 @@ __device__ inline A wave_sum_xor(A v) {
 }
 
 
+// One DPP move of a 32- or 64-bit value (CTRL: the instruction's dpp_ctrl).
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over aligned groups of g lanes (g a power of two <= 64, wave-uniform);
+// every lane of a group gets the group's sum.  Within a row of 16 lanes the
+// partners come by DPP (quad permutes for 1 and 2 lanes apart, then the
+// half-row and row mirrors, whose partners lie in the other half), so that
+// the sum takes no LDS round trip; across rows by swizzles.
+template <class A>
+__device__ inline A group_sum(A a, int g) {
+  if (g >= 2) a += dpp_move<0xB1>(a);   // quad_perm [1,0,3,2]
+  if (g >= 4) a += dpp_move<0x4E>(a);   // quad_perm [2,3,0,1]
+  if (g >= 8) a += dpp_move<0x141>(a);  // row_half_mirror
+  if (g >= 16) a += dpp_move<0x140>(a); // row_mirror
+  if (g >= 32) a += __shfl_xor(a, 16);
+  if (g >= 64) a += __shfl_xor(a, 32);
+  return a;
+}
+
 __global__ void gather_kernel(const double* __restrict__ b, const int32_t* __restrict__ perm,
                               int64_t N, double* __restrict__ xp, float* __restrict__ xf) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,7 +224,7 @@ __device__ inline void load_factor(const S* p, A (&o)[NV]) {
 // For each block V_b (column-major, ldv = m + (m & 1)) lpc lanes share a
 // column: each lane holds up to 8 rows of x in registers (loaded once per
 // block) and reads VEC consecutive rows of the column per load; the lanes'
-// partial sums are combined with an xor butterfly inside the group.  Rows past
+// partial sums are combined inside the group (group_sum).  Rows past
 // the item's range see x = 0, so loads may run into padding or the next column.
 // With `scale` (indexed like t) the finished t_k is multiplied by scale_k.
 template <class S, class A, int VEC>
@@ -226,7 +254,7 @@ __device__ __forceinline__ void phase1_item(const P1Item& it, void* const* __res
     const A* xs = x + bd.src + it.s0;
     const S* base = V + bd.voff + it.s0;
     auto finish = [&](int l, A acc) {
-      for (int d = lpc >> 1; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+      acc = group_sum(acc, lpc);
       if (l < bd.r && sub == 0) {
         if (it.pslot < 0)
           t[it.toff + bd.col + l] = scale ? acc * (A)scale[it.toff + bd.col + l] : acc;
@@ -384,34 +412,6 @@ __device__ inline void load_xvec(const A* p, A (&o)[VEC]) {
       o[2 * w + 1] = __hiloint2double((int)v.w, (int)v.z);
     }
   }
-}
-
-// One DPP move of a 32- or 64-bit value (CTRL: the instruction's dpp_ctrl).
-template <int CTRL>
-__device__ inline float dpp_move(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-template <int CTRL>
-__device__ inline double dpp_move(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-// Sum over aligned groups of g lanes (g a power of two <= 64, wave-uniform);
-// every lane of a group gets the group's sum.  Within a row of 16 lanes the
-// partners come by DPP (quad permutes for 1 and 2 lanes apart, then the
-// half-row and row mirrors, whose partners lie in the other half), so that
-// the sum takes no LDS round trip; across rows by swizzles.
-template <class A>
-__device__ inline A group_sum(A a, int g) {
-  if (g >= 2) a += dpp_move<0xB1>(a);   // quad_perm [1,0,3,2]
-  if (g >= 4) a += dpp_move<0x4E>(a);   // quad_perm [2,3,0,1]
-  if (g >= 8) a += dpp_move<0x141>(a);  // row_half_mirror
-  if (g >= 16) a += dpp_move<0x140>(a); // row_mirror
-  if (g >= 32) a += __shfl_xor(a, 16);
-  if (g >= 64) a += __shfl_xor(a, 32);
-  return a;
 }
 
 // ---------------------------------------------------------- mixed phase 1
