@@ -28,6 +28,7 @@ namespace {
 constexpr int kP1Rows = 512;          // rows of a phase-1 item (64 lanes x 8 rows)
 constexpr int64_t kP1Elems = 8192;    // factor entries per multi-block phase-1 item
 constexpr int kP2Rows = 128;          // rows of one phase-2 item
+constexpr int64_t kTItemBytes = 131072;  // factor bytes of a transposed phase-1 item
 
 template <class A>
 __device__ inline A wave_sum_xor(A v) {
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
                                                       int nitems, void* const* __restrict__ vbase,
                                                       const A* __restrict__ x,
                                                       A* __restrict__ tsrc, A* __restrict__ part,
-                                                      int kmax) {
+                                                      int kmax, const double* __restrict__ scale) {
   // 64 lanes x KMAX x VEC >= the item's rows (mix_item_rows); two vectors for
   // 16 fp8 entries per lane (x of their rows stays in registers)
   constexpr int KMAX = VEC >= 16 ? 2 : (VEC >= 2 ? 4 : 8);
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
   auto finish = [&](int c, A a) {
     a = group_sum(a, lpc);
     if (c < it.c1 && sub == 0) {
-      if (it.pslot < 0) tsrc[it.tsb + c] = a;
+      if (it.pslot < 0) tsrc[it.tsb + c] = scale ? a * (A)scale[it.tsb + c] : a;
       else part[(int64_t)it.pslot + (c - it.c0)] = a;
     }
   };
@@ -550,16 +551,28 @@ __global__ __launch_bounds__(256) void mphase1_kernel(const MixP1Item* __restric
     run(std::false_type{});
 }
 
-// Row-split partials of the mixed phase 1 summed in range order into t_src.
+// Row-split partials of the mixed phase 1 summed in range order into t_src
+// (with `scale`, each times its factor, as the phase-1 kernel stores).
 template <class A>
 __global__ void mreduce_kernel(const MixReduce* __restrict__ rs, int n, const A* __restrict__ part,
-                               A* __restrict__ tsrc) {
+                               A* __restrict__ tsrc, const double* __restrict__ scale) {
   const int w = wave_id();
   if (w >= n) return;
   const int lane = threadIdx.x & 63;
   const MixReduce r = rs[w];
-  for (int l = lane; l < r.nc; l += 64)
-    tsrc[r.tsb + l] = split_sum(part, (int64_t)r.pbase + l, r.nsplit, r.nc);
+  for (int l = lane; l < r.nc; l += 64) {
+    const A v = split_sum(part, (int64_t)r.pbase + l, r.nsplit, r.nc);
+    tsrc[r.tsb + l] = scale ? v * (A)scale[r.tsb + l] : v;
+  }
+}
+
+// t_src[map[i]] = t[i]: target-order t into source order (the transposed
+// phase 2 of a recompressed class reads each source box's V_s against it).
+template <class A>
+__global__ void tscatter_kernel(const int32_t* __restrict__ map, int n, const A* __restrict__ t,
+                                A* __restrict__ tsrc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tsrc[map[i]] = t[i];
 }
 
 // t[i] = scale[i] * t_src[map[i]]: source-order phase-1 results into target
@@ -776,38 +789,6 @@ __global__ __launch_bounds__(256) void phase2_queue_kernel(const P2Item* __restr
 
 
 // --------------------------------------------------------- transpose phases
-// t[toff + c] = sum_i U[c*ld + i] x[yoff + i] for the item's columns: lanes
-// over rows, one wave reduction per column.
-template <class S, class A>
-__global__ __launch_bounds__(256) void tphase1_kernel(const TP1Item* __restrict__ items, int nitems,
-                                                      void* const* __restrict__ ubase,
-                                                      const A* __restrict__ x, A* __restrict__ t,
-                                                      A* __restrict__ part,
-                                                      const double* __restrict__ scale) {
-  const int w = wave_id();
-  if (w >= nitems) return;
-  const int lane = threadIdx.x & 63;
-  const TP1Item it = items[w];
-  const S* U = (const S*)ubase[it.chunk] + it.uoff;
-  const int64_t ld = it.ld;
-  A mine = 0;
-  for (int s = it.s0; s < it.s1; s += 64) {
-    const bool rv = s + lane < it.s1;
-    const A xv = rv ? x[it.yoff + s + lane] : (A)0;
-    const int rr = rv ? s + lane : it.s0;
-    for (int cc = 0; cc < it.nc; ++cc) {
-      A p = (A)Load<S>::f(U + (int64_t)(it.c0 + cc) * ld + rr) * xv;
-      p = wave_sum_xor(p);
-      if (lane == cc) mine += p;
-    }
-  }
-  if (lane >= it.nc) return;
-  if (it.pslot < 0)
-    t[it.toff + it.c0 + lane] = scale ? mine * (A)scale[it.toff + it.c0 + lane] : mine;
-  else
-    part[(int64_t)it.pslot + lane] = mine;
-}
-
 // y[s] = sum over blocks b with this source box of sum_l V_b[s, l] t[btoff_b + l]:
 // lanes over rows, every block's row s read as r contiguous values.
 template <class S, class A>
@@ -1350,25 +1331,40 @@ void build_plan_host(const Engine& e, const std::vector<const LevelFactors*>& lf
           }
           flush(lv.iloff[k + 1]);
         }
-        // transpose phase 1
-        const int nsplit = (n + kP1Rows - 1) / kP1Rows;
-        for (int c0 = 0; c0 < R; c0 += 64) {
-          const int nc = std::min(64, R - c0);
-          if (nsplit > 1) out.tp1r.push_back({tslot, nsplit, lf.toff[k] + c0, nc});
-          for (int q = 0; q < nsplit; ++q) {
-            TP1Item it{};
-            it.uoff = lf.uoff[k];
-            it.chunk = lv.chunk[k];
-            it.ld = class_elem ? class_ldu(n, class_elem) : aca_ldu(n);
-            it.c0 = c0;
-            it.nc = nc;
-            it.s0 = q * kP1Rows;
-            it.s1 = std::min(n, (q + 1) * kP1Rows);
-            it.yoff = e.bstart[lo + k];
-            it.toff = lf.toff[k];
-            it.pslot = nsplit > 1 ? tslot : -1;
-            if (nsplit > 1) tslot += nc;
-            out.tp1.push_back(it);
+        // transpose phase 1: the box's U columns in runs (about kTItemBytes
+        // of factors), rows in ranges of the mixed phase-1 kernel's reach
+        // (that of the fp64 copy for the ACA copies, whose plan serves every
+        // precision)
+        {
+          const int es = class_elem ? class_elem : 8;
+          const int trows = mix_item_rows(es);
+          const int nsplit = (n + trows - 1) / trows;
+          const int rows0 = std::min(n, trows);
+          const int ncol = (int)std::max<int64_t>(
+              16, std::min<int64_t>(4096, kTItemBytes / ((int64_t)rows0 * es)));
+          const int ld = class_elem ? class_ldu(n, class_elem) : aca_ldu(n);
+          for (int c0 = 0; c0 < R; c0 += ncol) {
+            const int c1 = std::min(R, c0 + ncol);
+            int32_t pbase = -1;
+            if (nsplit > 1) {
+              pbase = tslot;
+              tslot += nsplit * (c1 - c0);
+              out.tp1r.push_back({pbase, nsplit, c1 - c0, lf.toff[k] + c0});
+            }
+            for (int q = 0; q < nsplit; ++q) {
+              MixP1Item it{};
+              it.voff = lf.uoff[k];
+              it.vchunk = lv.chunk[k];
+              it.ld = ld;
+              it.s0 = q * trows;
+              it.s1 = std::min(n, (q + 1) * trows);
+              it.c0 = c0;
+              it.c1 = c1;
+              it.tsb = lf.toff[k];
+              it.xoff = e.bstart[lo + k];
+              it.pslot = nsplit > 1 ? pbase + q * (c1 - c0) : -1;
+              out.tp1.push_back(it);
+            }
           }
         }
       }
@@ -1608,7 +1604,7 @@ void launch_mp1(Engine& e, const FactorView& f, int bytes, int kmax, const A* x,
   if (bytes != 16) throw Error(H3D_ERR_INVALID, "the mixed phase 1 takes 16-byte loads");
   constexpr int es = (int)sizeof(S);
   mphase1_kernel<S, A, es == 3 ? 4 : 16 / es><<<wave_grid(n1), 256, 0, e.stream>>>(
-      fc.d_mp1, n1, f.vbase, x, (A*)f.tsrc, part, kmax);
+      fc.d_mp1, n1, f.vbase, x, (A*)f.tsrc, part, kmax, nullptr);
   H3D_LAUNCH_CHECK();
 }
 
@@ -1715,7 +1711,7 @@ void launch_p1_reduce(Engine& e, const FactorView& f, const A* part, A* t) {
     const FactorClass& fc = *f.fc;
     if (!fc.mp1r.empty())
       mreduce_kernel<A><<<wave_grid(fc.mp1r.size()), 256, 0, e.stream>>>(
-          fc.d_mp1r, (int)fc.mp1r.size(), part, (A*)f.tsrc);
+          fc.d_mp1r, (int)fc.mp1r.size(), part, (A*)f.tsrc, nullptr);
     H3D_LAUNCH_CHECK();
     if (fc.T > 0)
       tgather_kernel<A><<<(fc.T / 4 + 256) / 256, 256, 0, e.stream>>>(
@@ -1804,23 +1800,45 @@ void tune_mode(Engine& e, const FactorView& f, int reps, const A* x, A* t, A* pa
   }
 }
 
+// Transposed phase 1, t = U^T x over the U columns of every target box: the
+// mixed phase-1 kernel on the plan's items (U chunks for V chunks, the box's
+// rows for a source's), each t written at its own index (times the copy's
+// factor for a recompressed class), row-split partials summed after it.
 template <class S, class A>
 void run_tp1(Engine& e, const FactorView& f, const A* x, A* t, A* part, hipEvent_t mark) {
   const Plan& p = *f.plan;
   const int n1 = (int)p.tp1.size();
   hipStream_t st = e.stream;
+  constexpr int es = (int)sizeof(S);
   if (n1)
-    tphase1_kernel<S, A><<<wave_grid(n1), 256, 0, st>>>(p.d_tp1, n1, f.ubase, x, t, part, f.scale);
+    mphase1_kernel<S, A, es == 3 ? 4 : 16 / es><<<wave_grid(n1), 256, 0, st>>>(
+        p.d_tp1, n1, f.ubase, x, t, part, 0, f.scale);
   H3D_LAUNCH_CHECK();
   if (mark) H3D_HIP(hipEventRecord(mark, st));
   if (!p.tp1r.empty())
-    reduce_kernel<A><<<wave_grid(p.tp1r.size()), 256, 0, st>>>(p.d_tp1r, (int)p.tp1r.size(), part,
-                                                               t, f.scale);
+    mreduce_kernel<A><<<wave_grid(p.tp1r.size()), 256, 0, st>>>(p.d_tp1r, (int)p.tp1r.size(), part,
+                                                                t, f.scale);
   H3D_LAUNCH_CHECK();
 }
 
+// Transposed phase 2, y_s = sum over the blocks with source s of V_b t_b.  A
+// recompressed class keeps each source box's V columns together (V_s), so t
+// goes to source order and the phase-2 kernel reads V_s as it reads a target
+// box's U; the ACA copies' V lie per block, read row by row.
 template <class S, class A>
 void run_tp2(Engine& e, const FactorView& f, const A* t) {
+  if (f.fc) {
+    const FactorClass& fc = *f.fc;
+    const int n2 = (int)fc.tp2v.size();
+    if (!n2) return;
+    A* tsrc = (A*)class_tsrc(e, (int)(f.fc - e.mix.cls));
+    if (fc.T > 0)
+      tscatter_kernel<A><<<(fc.T + 255) / 256, 256, 0, e.stream>>>(fc.d_tmap, fc.T, t, tsrc);
+    H3D_LAUNCH_CHECK();
+    p2_raw<S, A>(fc.d_tp2v, n2, f.vbase, tsrc, e.d_ylev, e.N, f.accumulate, e.tune_p2[f.key],
+                 e.stream);
+    return;
+  }
   const Plan& p = *f.plan;
   const int n2 = (int)p.tp2.size();
   if (n2)

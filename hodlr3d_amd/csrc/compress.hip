@@ -846,6 +846,8 @@ void FactorClass::free_all() {
   dfree(d_mp1r);
   dfree(d_tmap);
   dfree(d_scale32);
+  dfree(d_tp2v);
+  tp2v.clear();
   plan.free_device();
   plan = Plan();
   mp1.clear();
@@ -1503,6 +1505,24 @@ void build_mixed(Engine& e, double tol) {
         }
         fc.nV.back() += C * ld;
         pos += nsrc[sb];
+        // transposed phase 2 over V_s: rows in pieces of about 1 KB (a
+        // multiple of 4 rows for fp24), t in source order
+        if (m > 0 && C > 0) {
+          const int p2r = fc.elem == 3 ? 256 : 1024 / fc.elem;
+          for (int i0 = 0; i0 < m; i0 += p2r) {
+            P2Item it{};
+            it.uoff = vs;
+            it.chunk = vc;
+            it.ld = ld;
+            it.i0 = i0;
+            it.nr = std::min(p2r, m - i0);
+            it.R = (int32_t)C;
+            it.toff = tsb;
+            it.yoff = e.bstart[lo + sb];
+            it.level = j;
+            fc.tp2v.push_back(it);
+          }
+        }
         // phase-1 items: rows in ranges of <= kMixRows, columns in runs of
         // about kMixItemBytes of factors
         if (m == 0 || C == 0) continue;
@@ -1716,6 +1736,7 @@ void build_mixed(Engine& e, double tol) {
       upload_plan(e, fc.plan);
       fc.d_mp1 = upload(fc.mp1, st);
       fc.d_mp1r = upload(fc.mp1r, st);
+      fc.d_tp2v = upload(fc.tp2v, st);
       fc.d_tmap = upload(fc.tmap, st);
     }
     H3D_HIP(hipStreamSynchronize(st));

@@ -91,20 +91,6 @@ struct P2Item {
   int32_t level;  // 1..nLevels
 };
 
-// Transpose phase 1 item: columns [c0, c0+nc) of a target box's U against
-// rows [s0, s1) of x.
-struct TP1Item {
-  int64_t uoff;
-  int32_t chunk;
-  int32_t ld;
-  int32_t c0, nc;
-  int32_t s0, s1;
-  int32_t yoff;
-  int32_t toff;
-  int32_t pslot;
-  int32_t pad;
-};
-
 // Transpose phase 2 item: rows [i0, i0+nr) of one source box gather
 // V_b t_b from every block whose source is that box.
 struct TP2Item {
@@ -185,15 +171,17 @@ struct Plan {
   std::vector<P1Item> p1;
   std::vector<P1Reduce> p1r;
   std::vector<P2Item> p2;
-  std::vector<TP1Item> tp1;
-  std::vector<P1Reduce> tp1r;
+  // transpose phase 1 over the U columns of each target box, in the form of
+  // the mixed phase 1 (column runs of a box's U, rows in ranges)
+  std::vector<MixP1Item> tp1;
+  std::vector<MixReduce> tp1r;
   std::vector<TP2Item> tp2;
   int32_t npartial = 0;
   P1Item* d_p1 = nullptr;
   P1Reduce* d_p1r = nullptr;
   P2Item* d_p2 = nullptr;
-  TP1Item* d_tp1 = nullptr;
-  P1Reduce* d_tp1r = nullptr;
+  MixP1Item* d_tp1 = nullptr;
+  MixReduce* d_tp1r = nullptr;
   TP2Item* d_tp2 = nullptr;
   void free_device();
 };
@@ -222,6 +210,9 @@ struct FactorClass {
   std::vector<int32_t> tmap;
   MixP1Item* d_mp1 = nullptr;
   MixReduce* d_mp1r = nullptr;
+  // transpose phase 2 over each source box's V_s (t in source order)
+  std::vector<P2Item> tp2v;
+  P2Item* d_tp2v = nullptr;
   int32_t* d_tmap = nullptr;
   float* d_scale32 = nullptr;        // classes 1, 2: c_k in target order as fp32 (their sums' type)
   int32_t mpartial = 0;              // partial sums of the phase-1 items
