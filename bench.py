@@ -611,9 +611,19 @@ def main():
                                                    "phase1_third", "phase2_third",
                                                    "phase1_kernel_third", "phase1_fourth",
                                                    "phase2_fourth", "phase1_kernel_fourth")}
+        if prec in ("fp64", "mixed"):
+            # the transposed product (the adjoint's apply): the kernel is
+            # symmetric, the ACA blocks (k, s) and (s, k) are separate
+            # approximations, so A^T q differs from A q by about the ACA error
+            yt = H.matvec(q, prec=prec, transpose=True)
+            out["transpose_ms"] = timed(torch, lambda: H.matvec(q, prec=prec, transpose=True),
+                                        args.steps, 1)
+            out["transpose_rel_diff_vs_forward"] = float(torch.linalg.norm(yt - y) /
+                                                         torch.linalg.norm(y))
         log(f"[bench] {prec}: {ms:.3f} ms/matvec, rel err {out['rel_err']:.3e} (max over draws "
             f"{out['rel_err_max']:.3e}), kernels "
-            f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }")
+            f"{ {k: round(v, 3) for k, v in out['kernel_ms'].items()} }"
+            + (f", transposed {out['transpose_ms']:.3f} ms" if "transpose_ms" in out else ""))
         return out
 
     if args.copy_check > 0:
@@ -725,6 +735,9 @@ def main():
                                    "sides (spec_tol_1e-7)",
                    "rhs": "iid N(0,1)", "parallelism": "single GPU"},
         "fp64_ms": round(res["fp64"]["ms"], 4),
+        "transpose_ms": {k: round(res[k]["transpose_ms"], 4) for k in res if "transpose_ms" in res[k]},
+        "transpose_rel_diff_vs_forward": {k: res[k]["transpose_rel_diff_vs_forward"] for k in res
+                                          if "transpose_ms" in res[k]},
         "speedup_vs_fp64": round(res["fp64"]["ms"] / m["ms"], 3),
         "hbm_gbs": round(far_gbs, 1),
         "fp64_hbm_gbs": round(fp64_far_gbs, 1),
