@@ -505,6 +505,9 @@ def main():
     ap.add_argument("--no-formats-compare", action="store_true",
                     help="skip the mixed copy with the fp16 tier only")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-transpose", action="store_true",
+                    help="skip the transposed products (a profiled run then holds forward "
+                         "dispatches only)")
     ap.add_argument("--no-spec", action="store_true",
                     help="skip the run at the precision configuration's ACA tolerance 1e-7")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
@@ -611,7 +614,7 @@ def main():
                                                    "phase1_third", "phase2_third",
                                                    "phase1_kernel_third", "phase1_fourth",
                                                    "phase2_fourth", "phase1_kernel_fourth")}
-        if prec in ("fp64", "mixed"):
+        if prec in ("fp64", "mixed") and not args.no_transpose:
             # the transposed product (the adjoint's apply): the kernel is
             # symmetric, the ACA blocks (k, s) and (s, k) are separate
             # approximations, so A^T q differs from A q by about the ACA error

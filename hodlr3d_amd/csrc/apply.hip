@@ -1445,7 +1445,7 @@ void ensure_scratch(Engine& e, int nrhs) {
   H3D_HIP(hipMemsetAsync(e.d_xp, 0, sizeof(double) * (e.N * nrhs + 64), e.stream));
   H3D_HIP(hipMemsetAsync(e.d_xf, 0, sizeof(float) * (e.N * nrhs + 64), e.stream));
   e.d_t = dalloc<double>(T * nrhs);
-  e.d_part = dalloc<double>(np * nrhs);
+  e.d_part = dalloc<double>(np * std::max(nrhs, 2));
   e.d_ylev = dalloc<double>(e.N * e.nL * nrhs);
   e.d_io = dalloc<double>(2 * e.N * nrhs);
   dfree(e.d_ynear);
@@ -1454,6 +1454,10 @@ void ensure_scratch(Engine& e, int nrhs) {
     H3D_HIP(hipStreamCreateWithFlags(&e.side, hipStreamNonBlocking));
     H3D_HIP(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
     H3D_HIP(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+  }
+  if (!e.side2) {
+    H3D_HIP(hipStreamCreateWithFlags(&e.side2, hipStreamNonBlocking));
+    H3D_HIP(hipEventCreateWithFlags(&e.ev_join2, hipEventDisableTiming));
   }
   e.max_nrhs = nrhs;
   e.t_cap = T;
@@ -1848,6 +1852,14 @@ void run_tp2(Engine& e, const FactorView& f, const A* t) {
   H3D_LAUNCH_CHECK();
 }
 
+// Launches within its scope go to stream `s` (the launch helpers use e.stream).
+struct OnStream {
+  Engine& e;
+  hipStream_t saved;
+  OnStream(Engine& en, hipStream_t s) : e(en), saved(en.stream) { e.stream = s; }
+  ~OnStream() { e.stream = saved; }
+};
+
 // Phase 1 (which = 1) or phase 2 (which = 2) of one copy, forward or transpose.
 template <class S, class A>
 void run_phase(Engine& e, const FactorView& f, int which, bool transpose, const A* x, A* t, A* part,
@@ -2061,6 +2073,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
   // of products is captured, it runs in line.
   const int nn = (int)e.nitems.size();
   const bool overlap = nn && !e.timing && e.overlap_near;
+  // The mixed copy's classes on two streams (without per-kernel timing, as
+  // the near field): each class's phase 2 needs only its own t, and the
+  // phase-2 kernels, which add into y, keep the in-line order.
+  const bool split = mixed && !e.timing && e.overlap_near && e.side2;
   auto launch_near = [&](hipStream_t ns) {
     const int64_t lo = level_offset(e.nL);
     const KernelDev kd = e.kern;
@@ -2094,8 +2110,8 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
       zero_kernel<<<(unsigned)std::min<int64_t>(4096, (n + 255) / 256), 256, 0, st>>>(e.d_ylev, n);
       H3D_LAUNCH_CHECK();
     }
+    if (overlap || split) H3D_HIP(hipEventRecord(e.ev_fork, st));
     if (overlap) {
-      H3D_HIP(hipEventRecord(e.ev_fork, st));
       H3D_HIP(hipStreamWaitEvent(e.side, e.ev_fork, 0));
       launch_near(e.side);
       H3D_HIP(hipEventRecord(e.ev_join, e.side));
@@ -2108,7 +2124,33 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
     // fourth's, 11 after the second's, 3 near start, 4 combine start, 5 end
     // (the copies: the mixed copy's classes 0, 1, 2, 3 = fp32, fp16, fp8,
     // fp24, run in the order 0, 3, 1, 2; one copy in the other modes)
-    for (int which = 1; which <= 2; ++which) {
+    if (split) {
+      // Classes 0 and 3 (fp64 sums), phase 1 then phase 2, on side2 while
+      // classes 1 and 2 run phase 1 here; their phase 2 waits for side2's,
+      // so y takes the classes in the in-line order 0, 3, 1, 2 (the same
+      // sums to the bit).  Each stream has its own region of partial sums.
+      H3D_HIP(hipStreamWaitEvent(e.side2, e.ev_fork, 0));
+      {
+        OnStream on(e, e.side2);
+        double* part2 = e.d_part + e.part_cap;
+        for (int which = 1; which <= 2; ++which) {
+          run_phase<float, double>(e, class_view(e, 0), which, transpose, e.d_xp, class_td(e, 0),
+                                   part2);
+          run_phase<f24, double>(e, class_view(e, 3), which, transpose, e.d_xp, class_td(e, 3),
+                                 part2);
+        }
+        H3D_HIP(hipEventRecord(e.ev_join2, e.side2));
+      }
+      float* part = (float*)e.d_part;
+      for (int which = 1; which <= 2; ++which) {
+        if (which == 2) H3D_HIP(hipStreamWaitEvent(st, e.ev_join2, 0));
+        run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_tf(e, 1),
+                                   part);
+        run_phase<f8e4m3, float>(e, class_view(e, 2), which, transpose, e.d_xf, class_tf(e, 2),
+                                 part);
+      }
+    }
+    for (int which = 1; which <= 2 && !split; ++which) {
       if (e.timing) H3D_HIP(hipEventRecord(e.ev[which], st));
       hipEvent_t m1 = (e.timing && which == 1) ? e.ev[8] : nullptr;
       hipEvent_t m2 = (e.timing && which == 1) ? e.ev[9] : nullptr;

@@ -67,10 +67,12 @@ void Engine::free_all() {
   dfree(d_ylev);
   dfree(d_ynear);
   if (side) (void)hipStreamSynchronize(side);
-  for (hipEvent_t* x : {&ev_fork, &ev_join, &ev_switch})
+  if (side2) (void)hipStreamSynchronize(side2);
+  for (hipEvent_t* x : {&ev_fork, &ev_join, &ev_join2, &ev_switch})
     if (*x) (void)hipEventDestroy(*x), *x = nullptr;
   if (side) (void)hipStreamDestroy(side);
-  side = nullptr;
+  if (side2) (void)hipStreamDestroy(side2);
+  side = side2 = nullptr;
   dfree(d_io);
   dfree(d_xw);
   dfree(d_tw);

@@ -365,10 +365,16 @@ struct Engine {
   double* d_ynear = nullptr;  // near field by position: N
   hipStream_t side = nullptr;                              // the near field's stream
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;         // its fork / join events
+  hipStream_t side2 = nullptr;      // the mixed copy's fp64-sum classes (forked at ev_fork)
+  hipEvent_t ev_join2 = nullptr;    // its join event
   hipEvent_t ev_switch = nullptr;  // orders a new stream after the old (h3d_set_stream)
-  bool overlap_near = true;  // near field on `side` (off while a graph is captured)
+  // products fork work to `side` (near field) and `side2` (half of the mixed
+  // copy's classes); off while a graph is captured and in h3d_matvec_noalloc
+  bool overlap_near = true;
   double* d_io = nullptr;     // staging for host vectors
-  int64_t t_cap = 0, part_cap = 0;  // allocated entries of d_t / d_part per right-hand side
+  // allocated entries of d_t / d_part per right-hand side (d_part holds
+  // max(max_nrhs, 2) regions of part_cap: the second is side2's)
+  int64_t t_cap = 0, part_cap = 0;
   // products with several right-hand sides: groups of up to rhs_block (16 or
   // 32; 0 = one at a time) applied together (multi.hip); scratch for W per entry
   int rhs_block = 32;

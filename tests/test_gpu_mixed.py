@@ -99,23 +99,25 @@ def test_rebuild_is_bit_identical(built):
     assert np.array_equal(y1, y2)
 
 
-def test_overlapped_near_field_equals_the_in_line_product(built):
-    """Without per-kernel timing the near field runs on a second stream,
-    concurrently with the far field, into a buffer of its own; with timing
-    every kernel runs in line on one stream.  The two schedules give the same
-    bits (another mode's product in between leaves its own values in the
-    shared buffers)."""
+@pytest.mark.parametrize("transpose", [False, True])
+def test_overlapped_near_field_equals_the_in_line_product(built, transpose):
+    """Without per-kernel timing the near field runs on a second stream and
+    the fp32 and fp24 classes on a third, concurrently with the fp16 and fp8
+    classes' phase 1, each stream with buffers of its own; with timing every
+    kernel runs in line on one stream.  The two schedules give the same bits,
+    forward and transposed (another mode's product in between leaves its own
+    values in the shared buffers)."""
     name, x, h, tol, st = built
     q = np.random.default_rng(11).standard_normal(len(x))
     h.kernel_timing(False)
-    y_conc = h.matvec(q, prec="mixed")
+    y_conc = h.matvec(q, prec="mixed", transpose=transpose)
     h.matvec(q[::-1].copy(), prec="fp64")
     h.kernel_timing(True)
-    y_line = h.matvec(q, prec="mixed")
+    y_line = h.matvec(q, prec="mixed", transpose=transpose)
     h.kernel_timing(False)
     assert np.array_equal(y_conc, y_line)
     h.matvec(q[::-1].copy(), prec="fp32")
-    assert np.array_equal(h.matvec(q, prec="mixed"), y_conc)
+    assert np.array_equal(h.matvec(q, prec="mixed", transpose=transpose), y_conc)
 
 
 def test_every_load_width_gives_the_same_product(built):
