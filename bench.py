@@ -67,7 +67,8 @@ MODE_DESC = {
     "bf16": "bf16 factors, fp32 accumulation, fp32 on-the-fly near field",
     "bf16/acc64": "bf16 factors, fp64 accumulation, fp64 on-the-fly near field",
     "mixed": "recompressed factors (eigendecomposition of each block's Gram matrices), "
-             "columns dropped / fp8 (e4m3) / fp16 / fp24 / fp32 under one global error budget, "
+             "columns dropped / int8 (a scale per column side) / fp16 / fp24 / fp32 under one "
+             "global error budget, "
              "fp64 accumulation, fp32 near-field entries summed in fp64",
 }
 
@@ -144,11 +145,11 @@ def near_metrics(st, res, mode):
 def mixed_kernels(H, km):
     """(name, ms, algorithmic bytes) of the far-field kernels of the mixed copy:
     phase 1 / phase 2 of its fp32 and fp24 classes (fp64 x and t) and of its
-    fp16 and fp8 classes (fp32 x and t).  Phase 1 (the kernel alone, event-timed)
+    fp16 and 8-bit classes (fp32 x and t).  Phase 1 (the kernel alone, event-timed)
     reads the class's V entries and x once per level and writes t; phase 2
     reads the U entries and t and writes (fp32 class) or updates (the others)
     y once per level.  Template arguments follow the tuning the products used
-    (the fp8 class shares the fp16 class's)."""
+    (the 8-bit class, int8 or e4m3, shares the fp16 class's)."""
     ms_ = H.mixed_stats()
     st = H.geometry_stats()
     tun = H.get_tuning("mixed")
@@ -159,7 +160,8 @@ def mixed_kernels(H, km):
          km["phase2"] - km["phase2_second"] - km["phase2_third"] - km["phase2_fourth"], 0),
         (2, 4, "_Float16, float", ms_["rank_fp16"], ms_["v_elems"][1], ms_["u_elems"][1],
          km["phase1_kernel_second"], km["phase2_second"], 1),
-        (1, 4, "f8e4m3, float", ms_["rank_fp8"], ms_["v_elems_fp8"], ms_["u_elems_fp8"],
+        (1, 4, "i8s, float" if ms_["formats"] & 8 else "f8e4m3, float", ms_["rank_fp8"],
+         ms_["v_elems_fp8"], ms_["u_elems_fp8"],
          km["phase1_kernel_third"], km["phase2_third"], 1),
         (3, 8, "f24, double", ms_["rank_fp24"], ms_["v_elems_fp24"], ms_["u_elems_fp24"],
          km["phase1_kernel_fourth"], km["phase2_fourth"], 0),
@@ -696,9 +698,21 @@ def main():
         H.set_operator(alpha=1.0, diag=0.0)
 
     # the same budget with the fp16 tier only (columns dropped / fp16 / fp32):
-    # what the fp8 and fp24 tiers change
-    formats_compare = None
+    # what the 8-bit and fp24 tiers change; and with the 8-bit tier in e4m3
+    # instead of int8
+    formats_compare = e4m3_compare = None
     if mixed_info and not args.no_formats_compare:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s8 = H.compress(args.mixed_tol, formats=("fp16", "fp8", "fp24"))
+        t_b = time.perf_counter() - t0
+        r8 = measure("mixed")
+        e4m3_compare = {"ms": round(r8["ms"], 4), "rel_err_max": r8["rel_err_max"],
+                        "bytes": s8["bytes"], "build_s": round(t_b, 2),
+                        "predicted_rel_err": s8["predicted_rel_err"],
+                        "rank_fp8": s8["rank_fp8"], "rank_fp16": s8["rank_fp16"],
+                        "speedup_of_int8": round(r8["ms"] / m["ms"], 3)}
+        log(f"[bench] mixed copy with the 8-bit tier in e4m3: {e4m3_compare}")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s16 = H.compress(args.mixed_tol, formats=("fp16",))
@@ -725,7 +739,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "fp32/acc64": "f32 storage, f64 accumulation",
                   "bf16": "bf16", "bf16/acc64": "bf16 storage, f64 accumulation",
-                  "mixed": "f32+f24+f16+f8 storage, f64 accumulation"}.get(mixed, mixed),
+                  "mixed": "f32+f24+f16+i8 storage, f64 accumulation"}.get(mixed, mixed),
         "data": "synthetic",
         "config": {"workload": wl["desc"], "N": N, "nlevels": st["nlevels"],
                    "mixed": MODE_DESC.get(mixed, mixed), "err_bar": REL_ERR_BAR,
@@ -770,6 +784,7 @@ def main():
                                      for k in kern]},
         "mixed_copy": mixed_info,
         "mixed_fp16_tier_only": formats_compare,
+        "mixed_e4m3": e4m3_compare,
         "near_field": near_metrics(st, res, mixed),
         "spec_tol_1e-7": spec,
         "host": host_cpu(),

@@ -60,7 +60,7 @@ class MixedStats(ctypes.Structure):
                 ("single_jacobi", _i32), ("formats", _i32), ("err_fp8", _d),
                 ("rank_fp8", _i64), ("u_elems_fp8", _i64), ("v_elems_fp8", _i64),
                 ("err_fp24", _d), ("rank_fp24", _i64), ("u_elems_fp24", _i64),
-                ("v_elems_fp24", _i64)]
+                ("v_elems_fp24", _i64), ("err_fp8_model", _d)]
 
 
 class Config(ctypes.Structure):

@@ -185,7 +185,7 @@ __device__ inline void raw_cvt(const Raw<S, NV>& v, A (&o)[NV]) {
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) o[q] = (A)__uint_as_float(u[q]);
-  } else if constexpr (sizeof(S) == 1) {  // fp8: two values per conversion
+  } else if constexpr (sizeof(S) == 1) {  // fp8: two values per conversion; int8: one
     uint32_t w[B >= 4 ? B / 4 : 1];
     if constexpr (B == 16) {
       w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
@@ -193,6 +193,12 @@ __device__ inline void raw_cvt(const Raw<S, NV>& v, A (&o)[NV]) {
       w[0] = v.x, w[1] = v.y;
     } else {
       w[0] = (uint32_t)v;
+    }
+    if constexpr (std::is_same<S, i8s>::value) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+        o[q] = (A)i8_to_f(w[q / 4], q % 4);
+      return;
     }
 #pragma unroll
     for (int q = 0; q < NV; q += 2) {
@@ -1852,6 +1858,14 @@ void run_tp2(Engine& e, const FactorView& f, const A* t) {
   H3D_LAUNCH_CHECK();
 }
 
+// Class 2 of the mixed copy holds e4m3 entries, or int8 ones (kMixInt8):
+// f(storage tag).
+template <class F>
+void with_class2(const Engine& e, F f) {
+  if (e.mix.int8) f(i8s{});
+  else f(f8e4m3{});
+}
+
 // Launches within its scope go to stream `s` (the launch helpers use e.stream).
 struct OnStream {
   Engine& e;
@@ -2146,8 +2160,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
         if (which == 2) H3D_HIP(hipStreamWaitEvent(st, e.ev_join2, 0));
         run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_tf(e, 1),
                                    part);
-        run_phase<f8e4m3, float>(e, class_view(e, 2), which, transpose, e.d_xf, class_tf(e, 2),
-                                 part);
+        with_class2(e, [&](auto s8) {
+          run_phase<decltype(s8), float>(e, class_view(e, 2), which, transpose, e.d_xf,
+                                         class_tf(e, 2), part);
+        });
       }
     }
     for (int which = 1; which <= 2 && !split; ++which) {
@@ -2166,8 +2182,10 @@ void matvec_device(Engine& e, const double* d_b, double* d_y, int nrhs, int mode
         run_phase<_Float16, float>(e, class_view(e, 1), which, transpose, e.d_xf, class_tf(e, 1),
                                    (float*)e.d_part, m2);
         if (e.timing) H3D_HIP(hipEventRecord(e.ev[9 + which], st));
-        run_phase<f8e4m3, float>(e, class_view(e, 2), which, transpose, e.d_xf, class_tf(e, 2),
-                                 (float*)e.d_part, m3);
+        with_class2(e, [&](auto s8) {
+          run_phase<decltype(s8), float>(e, class_view(e, 2), which, transpose, e.d_xf,
+                                         class_tf(e, 2), (float*)e.d_part, m3);
+        });
       } else if (acc64) {
         const FactorView f = aca_view(e, prec, true);
         if (prec == H3D_PREC_FP64)

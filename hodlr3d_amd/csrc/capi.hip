@@ -450,17 +450,18 @@ int h3d_get_mixed_stats(h3d_handle* h, h3d_mixed_stats* s) {
     s->rank_fp24 = m.cls[3].rank_sum;
     s->u_elems_fp24 = m.cls[3].u_elems;
     s->v_elems_fp24 = m.cls[3].v_elems;
+    s->err_fp8_model = a2 > 0 ? std::sqrt(m.err2_fp8_model / a2) : 0;
   });
 }
 
 static_assert(H3D_MIXED_FP16 == h3d::kMixFp16 && H3D_MIXED_FP8 == h3d::kMixFp8 &&
-                  H3D_MIXED_FP24 == h3d::kMixFp24,
+                  H3D_MIXED_FP24 == h3d::kMixFp24 && H3D_MIXED_INT8 == h3d::kMixInt8,
               "mixed-copy format bits");
 
 int h3d_set_mixed_formats(h3d_handle* h, int formats) {
   return guarded([&] {
     if (!h) throw h3d::Error(H3D_ERR_INVALID, "null handle");
-    if (formats & ~(H3D_MIXED_FP16 | H3D_MIXED_FP8 | H3D_MIXED_FP24))
+    if (formats & ~(H3D_MIXED_FP16 | H3D_MIXED_FP8 | H3D_MIXED_FP24 | H3D_MIXED_INT8))
       throw h3d::Error(H3D_ERR_INVALID, "unknown mixed-copy format");
     h->e.mix.formats = formats;
   });

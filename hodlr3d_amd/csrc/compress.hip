@@ -26,6 +26,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <exception>
 #include <thread>
 #include <vector>
@@ -58,6 +59,12 @@ constexpr int kColumnOrder[kClasses] = {0, 3, 1, 2};
 constexpr double kCholFail = 1e-12;       // pivot / original diagonal below this: raw block
 constexpr int kMaxSweeps = 24;
 constexpr int kBinsPerOctave = 8;
+// int8 columns (kMixInt8): rho^2 = (max / rms)^2 of a stored column side,
+// modelled as kI8Rho2 sqrt(rows).  Exact SVDs of 1/r blocks between
+// well-separated jittered boxes of 64-2197 points give 1.10-1.19 sqrt(rows)
+// for the columns of this tier; the constant carries a 15% margin.
+constexpr double kI8Rho2 = 1.37;
+constexpr int kSizeBuckets = 64;  // quarter octaves of sqrt(n) + sqrt(m): up to 2^16
 constexpr double kColShare = 1.0 / 4096;  // largest share of the error budget one column may take
 constexpr int64_t kMixItemBytes = 131072;           // factor bytes of a mixed phase-1 item
 constexpr int64_t kVChunkElems = int64_t(1) << 28;  // elements of a V chunk (whole source boxes)
@@ -82,6 +89,7 @@ struct PDesc {
   int32_t raw;
   int32_t ldu_o[kClasses];  // U column stride per class
   int32_t ldv_o[kClasses];  // V column stride per class
+  int32_t t8;               // class 2's t index of the block's first column (int8 maxima)
   double su, sv;            // power-of-two scales of the stored columns
 };
 
@@ -679,12 +687,17 @@ struct ClassBases {
 
 // Thread per row: out_k = sum_l X[row, l] T[l, k] for the kept columns, scaled
 // and stored in the column's class (the sorted columns: rc[0] in fp32, then
-// rc[3] in fp24, rc[1] in fp16, rc[2] in fp8).  Raw blocks copy their columns
-// unscaled into fp32.
-template <class S>
+// rc[3] in fp24, rc[1] in fp16, rc[2] in fp8 or int8).  Raw blocks copy their
+// columns unscaled into fp32.  MODE 0 writes class 2 in e4m3; MODE 1 only
+// takes the largest |entry| of each class-2 column side (a wave maximum, then
+// an atomic maximum of its fp32 bits, which order as the values do for x >=
+// 0) into mx8[side * T2 + t8 + column]; MODE 2 writes class 2 in int8,
+// round(127 v / max).  Both int8 passes compute the same values.
+template <class S, int MODE>
 __global__ __launch_bounds__(256) void project_kernel(
     const PItem* __restrict__ items, const CDesc* __restrict__ ds, const PDesc* __restrict__ ps,
-    void* const* __restrict__ base64, const double* __restrict__ ws, ClassBases cb) {
+    void* const* __restrict__ base64, const double* __restrict__ ws, ClassBases cb,
+    uint32_t* __restrict__ mx8, int64_t T2) {
   constexpr int KT = 8;
   const PItem it = items[blockIdx.x];
   const CDesc d = ds[it.blk];
@@ -692,6 +705,43 @@ __global__ __launch_bounds__(256) void project_kernel(
   const int side = it.side;
   const int rows = side ? d.m : d.n;
   const int row = it.row0 + threadIdx.x;
+  if constexpr (MODE == 1) {
+    // every lane of the wave takes part in the maxima (lanes past the rows add 0)
+    if (p.raw || p.rc[2] == 0) return;
+    const bool act = row < rows;
+    const int r = d.r;
+    const int ld = side ? d.ldv : d.ldu;
+    const S* X = (const S*)base64[d.chunk] + (side ? d.xv : d.xu) + (act ? row : 0);
+    const double* W = ws + d.w;
+    const double* T = W + (side ? (int64_t)r * r : 0);
+    const double* lam = W + 2 * (int64_t)r * r;
+    const double floor_ = lam[r + 1];
+    const int k8 = p.rc[0] + p.rc[3] + p.rc[1], keep = k8 + p.rc[2];
+    const double sc = side ? p.sv : p.su;
+    uint32_t* out = mx8 + (side ? T2 : 0) + p.t8 - k8;
+    for (int k0 = k8; k0 < keep; k0 += KT) {
+      double acc[KT];
+#pragma unroll
+      for (int q = 0; q < KT; ++q) acc[q] = 0.0;
+      if (act)
+        for (int l = 0; l < r; ++l) {
+          const double x = (double)X[(int64_t)l * ld];
+#pragma unroll
+          for (int q = 0; q < KT; ++q)
+            if (k0 + q < keep) acc[q] += x * T[l + (int64_t)(k0 + q) * r];
+        }
+#pragma unroll
+      for (int q = 0; q < KT; ++q) {
+        const int k = k0 + q;
+        if (k >= keep) break;
+        const double v = side ? acc[q] * sc : acc[q] * (sc / sqrt(fmax(lam[k], floor_)));
+        float a = act ? fabsf((float)v) : 0.0f;
+        for (int o = 32; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o));
+        if ((threadIdx.x & 63) == 0 && a > 0.0f) atomicMax(out + k, __float_as_uint(a));
+      }
+    }
+    return;
+  }
   if (row >= rows) return;
   const int r = d.r;
   const int ld = side ? d.ldv : d.ldu;  // the source copy's stride
@@ -744,6 +794,10 @@ __global__ __launch_bounds__(256) void project_kernel(
         o[2] = (uint8_t)(u >> 24);
       } else if (k < k8) {
         o16[(int64_t)(k - k16) * ldo[1]] = (_Float16)v;
+      } else if constexpr (MODE == 2) {
+        const float m = __uint_as_float(mx8[(side ? T2 : 0) + p.t8 + (k - k8)]);
+        const float f = m > 0.0f ? rintf((float)v * (127.0f / m)) : 0.0f;
+        o8[(int64_t)(k - k8) * ldo[2]] = (uint8_t)(int8_t)fminf(127.0f, fmaxf(-127.0f, f));
       } else {
         float f = (float)(v * kFp8Gain);
         if (fabsf(f) > 448.0f) f = copysignf(448.0f, f);  // NaN passes
@@ -1241,23 +1295,45 @@ void build_mixed(Engine& e, double tol) {
     return (int)std::min<double>(kBins - 1, std::max<double>(0, std::floor(v) - kBinLo));
   };
   constexpr int kTiers = 5;
-  const double ta[kTiers] = {1.0, kFp8Err * kFp8Err, kFp16Err * kFp16Err, kFp24Err * kFp24Err,
-                             kFp32Err * kFp32Err};
+  const bool int8 = (mx.formats & kMixInt8) != 0;
+  // The 8-bit tier's error: e4m3's fixed relative error, or for int8 (each
+  // column scaled by its largest entry) rho^2 / (12 127^2) per side, rho =
+  // max / rms of the column, modelled as kI8Rho2 sqrt(rows) per side; the
+  // model depends on the block's box sizes, so the blocks fall into size
+  // buckets (a quarter octave of sqrt(n) + sqrt(m), the error taken at the
+  // bucket's upper edge) with thresholds of their own.
+  const int nbk = int8 ? kSizeBuckets : 1;
+  auto bucket_of = [&](int n, int m) {
+    if (!int8) return 0;
+    const double sz = std::sqrt((double)n) + std::sqrt((double)m);
+    return std::min(kSizeBuckets - 1, std::max(0, (int)std::floor(std::log2(sz) * 4.0)));
+  };
+  std::vector<std::array<double, kTiers>> ta(nbk);
+  for (int q = 0; q < nbk; ++q) {
+    const double a8 = int8 ? kI8Rho2 * std::exp2((q + 1) / 4.0) / (12.0 * 127.0 * 127.0)
+                           : kFp8Err * kFp8Err;
+    ta[q] = {1.0, a8, kFp16Err * kFp16Err, kFp24Err * kFp24Err, kFp32Err * kFp32Err};
+  }
   const double tb[kTiers] = {0.0, 1.0, 2.0, 3.0, 4.0};
   const int tier_class[kTiers] = {-1, 2, 1, 3, 0};
   int av[kTiers], nav = 0;  // available tiers, increasing
   av[nav++] = 0;
-  if (mx.formats & kMixFp8) av[nav++] = 1;
+  if (mx.formats & (kMixFp8 | kMixInt8)) av[nav++] = 1;
   if (mx.formats & kMixFp16) av[nav++] = 2;
   if (mx.formats & kMixFp24) av[nav++] = 3;
   av[nav++] = 4;
-  // bin offsets of the thresholds above the lowest one
-  int toff[kTiers] = {0, 0, 0, 0, 0};
+  // per bucket, bin offsets of the thresholds above the lowest one (the
+  // lowest, drop against the first stored tier, is the same for every bucket
+  // to within 1 - a_t)
+  std::vector<std::array<int, kTiers>> toff(nbk);
   {
-    const double th0 = (tb[av[1]] - tb[av[0]]) / (ta[av[0]] - ta[av[1]]);
-    for (int i = 1; i + 1 < nav; ++i) {
-      const double th = (tb[av[i + 1]] - tb[av[i]]) / (ta[av[i]] - ta[av[i + 1]]);
-      toff[i] = (int)std::lround(std::log2(th / th0) * kBinsPerOctave);
+    const double th0 = (tb[av[1]] - tb[av[0]]) / (ta[0][av[0]] - ta[0][av[1]]);
+    for (int q = 0; q < nbk; ++q) {
+      toff[q].fill(0);
+      for (int i = 1; i + 1 < nav; ++i) {
+        const double th = (tb[av[i + 1]] - tb[av[i]]) / (ta[q][av[i]] - ta[q][av[i + 1]]);
+        toff[q][i] = (int)std::lround(std::log2(th / th0) * kBinsPerOctave);
+      }
     }
   }
   // No single column may carry more than kColShare of the budget (dropped, or
@@ -1266,56 +1342,94 @@ void build_mixed(Engine& e, double tol) {
   // expectation.  A column above a tier's cap takes the next available tier.
   const double budget = tol * tol * (mx.fro2_far + mx.fro2_near);
   const double cap = kColShare * budget;
-  auto min_tier = [&](double l) {  // index into av
+  auto min_tier = [&](int q, double l) {  // index into av
     int i = 0;
-    while (i + 1 < nav && ta[av[i]] * l > cap) ++i;
+    while (i + 1 < nav && ta[q][av[i]] * l > cap) ++i;
     return i;
   };
-  // index into av of the tier of a column in bin b with the lowest threshold at bin bd
-  auto bin_tier = [&](int b, int bd) {
+  // index into av of the tier of a column of bucket q in bin b with the lowest threshold at bin bd
+  auto bin_tier = [&](int q, int b, int bd) {
     int i = 0;
-    while (i + 1 < nav && b >= bd + toff[i]) ++i;
+    while (i + 1 < nav && b >= bd + toff[q][i]) ++i;
     return i;
   };
   int bd = 0;
   {
-    // H[m]: per bin, lam of the columns whose lowest allowed tier is av[m]
-    std::vector<double> H[kTiers];
-    for (int m = 0; m < nav; ++m) H[m].assign(kBins + 1, 0.0);
-    {  // per-chunk histograms, added in chunk order
-      std::vector<std::vector<double>> ch_(kHostChunks * kTiers);
+    // the bins and buckets the columns occupy (a first pass; per chunk,
+    // combined in order)
+    int blo = kBins, bhi = 0, qlo = nbk, qhi = 0;
+    {
+      std::vector<std::array<int, 4>> cr(kHostChunks, {kBins, 0, nbk, 0});
       host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
-        for (int m = 0; m < nav; ++m) ch_[c * kTiers + m].assign(kBins + 1, 0.0);
+        for (int64_t i = i0; i < i1; ++i) {
+          if (raw[i] || cd[i].r == 0) continue;
+          const int nm = cd[i].n + cd[i].m;
+          const int q = bucket_of(cd[i].n, cd[i].m);
+          cr[c][2] = std::min(cr[c][2], q);
+          cr[c][3] = std::max(cr[c][3], q);
+          for (int k = 0; k < cd[i].r; ++k) {
+            const int b = bin_of(lam_all[lam_off[i] + k], nm);
+            cr[c][0] = std::min(cr[c][0], b);
+            cr[c][1] = std::max(cr[c][1], b);
+          }
+        }
+      });
+      for (int c = 0; c < kHostChunks; ++c) {
+        blo = std::min(blo, cr[c][0]), bhi = std::max(bhi, cr[c][1]);
+        qlo = std::min(qlo, cr[c][2]), qhi = std::max(qhi, cr[c][3]);
+      }
+      if (blo > bhi) blo = bhi = 0;
+      if (qlo > qhi) qlo = qhi = 0;
+    }
+    const int nbin = bhi - blo + 1;
+    // H[hix(q, m, b)]: per bucket and bin, lam of the columns whose lowest
+    // allowed tier is av[m] (buckets qlo..qhi, bins blo..bhi)
+    const int64_t hsz = (int64_t)(qhi - qlo + 1) * nav * (nbin + 1);
+    auto hix = [&](int q, int m, int b) {
+      return ((int64_t)(q - qlo) * nav + m) * (nbin + 1) + (b - blo);
+    };
+    std::vector<double> H(hsz, 0.0);
+    {  // per-chunk histograms, added in chunk order
+      std::vector<std::vector<double>> ch_(kHostChunks);
+      host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
+        ch_[c].assign(hsz, 0.0);
         for (int64_t i = i0; i < i1; ++i) {
           if (raw[i]) continue;
           const int nm = cd[i].n + cd[i].m;
+          const int q = bucket_of(cd[i].n, cd[i].m);
           for (int k = 0; k < cd[i].r; ++k) {
             const double l = lam_all[lam_off[i] + k];
-            ch_[c * kTiers + min_tier(l)][bin_of(l, nm)] += l;
+            ch_[c][hix(q, min_tier(q, l), bin_of(l, nm))] += l;
           }
         }
       });
       for (int c = 0; c < kHostChunks; ++c)
-        for (int m = 0; m < nav; ++m)
-          for (int b = 0; b <= kBins; ++b) H[m][b] += ch_[c * kTiers + m][b];
+        for (int64_t x = 0; x < hsz; ++x) H[x] += ch_[c][x];
     }
-    std::vector<double> P[kTiers];  // prefix sums
-    for (int m = 0; m < nav; ++m) {
-      P[m].assign(kBins + 1, 0.0);
-      for (int b = 0; b < kBins; ++b) P[m][b + 1] = P[m][b] + H[m][b];
-    }
+    // prefix sums per (bucket, lowest tier): P[hix(q, m, blo + j)] = sum of bins < blo + j
+    std::vector<double> P(hsz, 0.0);
+    for (int q = qlo; q <= qhi; ++q)
+      for (int m = 0; m < nav; ++m) {
+        const int64_t o = hix(q, m, blo);
+        for (int j = 0; j < nbin; ++j) P[o + j + 1] = P[o + j] + H[o + j];
+      }
+    auto pref = [&](int q, int m, int b) {  // sum over bins < b (any b)
+      const int j = std::min(nbin, std::max(0, b - blo));
+      return P[hix(q, m, blo) + j];
+    };
     // squared error per tier (index into av) with the lowest threshold at bd
     auto parts = [&](int bdv, double* out) {
       for (int i = 0; i < nav; ++i) out[i] = 0.0;
-      for (int m = 0; m < nav; ++m) {
-        int lo = 0;
-        for (int i = 0; i < nav; ++i) {
-          const int hi = i + 1 < nav ? std::min(kBins, std::max(lo, bdv + toff[i])) : kBins;
-          const int t = std::max(m, i);
-          out[t] += ta[av[t]] * (P[m][hi] - P[m][lo]);
-          lo = hi;
+      for (int q = qlo; q <= qhi; ++q)
+        for (int m = 0; m < nav; ++m) {
+          int lo = 0;
+          for (int i = 0; i < nav; ++i) {
+            const int hi = i + 1 < nav ? std::min(kBins, std::max(lo, bdv + toff[q][i])) : kBins;
+            const int t = std::max(m, i);
+            out[t] += ta[q][av[t]] * (pref(q, m, hi) - pref(q, m, lo));
+            lo = hi;
+          }
         }
-      }
     };
     auto err = [&](int bdv) {
       double q[kTiers];
@@ -1364,9 +1478,10 @@ void build_mixed(Engine& e, double tol) {
         continue;
       }
       const int nm = cd[i].n + cd[i].m;
+      const int q = bucket_of(cd[i].n, cd[i].m);
       for (int k = 0; k < r; ++k) {
         const double l = lam_all[lam_off[i] + k];
-        const int t = av[std::max(min_tier(l), bin_tier(bin_of(l, nm), bd))];
+        const int t = av[std::max(min_tier(q, l), bin_tier(q, bin_of(l, nm), bd))];
         if (t > 0) rk[i][tier_class[t]]++;
       }
     }
@@ -1469,6 +1584,7 @@ void build_mixed(Engine& e, double tol) {
           if (i < 0) continue;
           pd[i].u_out[c] = f.uoff[k] + (int64_t)f.col[b] * ldu;
           pd[i].ldu_o[c] = ldu;
+          if (c == 2) pd[i].t8 = f.toff[k] + f.col[b];
         }
       }
       // V by source box; the by-source list of level j occupies
@@ -1637,7 +1753,9 @@ void build_mixed(Engine& e, double tol) {
           int k0 = 0;  // first sorted column of the class
           for (int c : kColumnOrder) {
             const LevelFactors& f = mx.cls[c].lf[j];
-            const double g = c == 2 ? inv / (kFp8Gain * kFp8Gain) : inv;
+            // e4m3 columns carry kFp8Gain per side; int8 columns their
+            // maxima / 127, applied after the projection
+            const double g = c == 2 && !int8 ? inv / (kFp8Gain * kFp8Gain) : inv;
             for (int kk = 0; kk < rk[i][c]; ++kk)
               scale[(size_t)mx.cls[c].tbase + f.toff[k] + f.col[b] + kk] =
                   std::sqrt(std::max(lam_all[lam_off[i] + k0 + kk], fl)) * g;
@@ -1684,6 +1802,17 @@ void build_mixed(Engine& e, double tol) {
   // and raw blocks need no eigen step
   std::vector<uint8_t> work(nb, 0);
   for (int64_t i = 0; i < nb; ++i) work[i] = !raw[i] && kept(i) > 0;
+  // int8: each class-2 column side's largest |entry| (fp32 bits), U sides then V sides
+  const int64_t T2 = int8 ? mx.cls[2].T : 0;
+  uint32_t* d_mx8 = nullptr;
+  struct Mx8Free {
+    uint32_t*& p;
+    ~Mx8Free() { dfree(p); }
+  } mx8_free{d_mx8};
+  if (T2 > 0) {
+    d_mx8 = dalloc<uint32_t>(2 * T2);
+    H3D_HIP(hipMemsetAsync(d_mx8, 0, sizeof(uint32_t) * 2 * T2, st));
+  }
   {
     int64_t i0 = 0;
     while (i0 < nb) {
@@ -1709,8 +1838,16 @@ void build_mixed(Engine& e, double tol) {
       }
       for (int64_t s = 0; s < (int64_t)items.size(); s += 1 << 30) {
         const unsigned g = (unsigned)std::min<int64_t>(1 << 30, (int64_t)items.size() - s);
-        (src ? project_kernel<float> : project_kernel<double>)<<<g, 256, 0, st>>>(
-            d_it + s, d_cd, d_pd, d_b64, d_ws, cb);
+        if (T2 > 0) {
+          (src ? project_kernel<float, 1> : project_kernel<double, 1>)<<<g, 256, 0, st>>>(
+              d_it + s, d_cd, d_pd, d_b64, d_ws, cb, d_mx8, T2);
+          H3D_LAUNCH_CHECK();
+          (src ? project_kernel<float, 2> : project_kernel<double, 2>)<<<g, 256, 0, st>>>(
+              d_it + s, d_cd, d_pd, d_b64, d_ws, cb, d_mx8, T2);
+        } else {
+          (src ? project_kernel<float, 0> : project_kernel<double, 0>)<<<g, 256, 0, st>>>(
+              d_it + s, d_cd, d_pd, d_b64, d_ws, cb, nullptr, 0);
+        }
         H3D_LAUNCH_CHECK();
       }
       H3D_HIP(hipStreamSynchronize(st));
@@ -1725,6 +1862,41 @@ void build_mixed(Engine& e, double tol) {
   dfree(d_b64);
   tables_thread.join();
   if (tables_err) std::rethrow_exception(tables_err);
+  mx.err2_fp8_model = mx.err2_fp8;
+  if (T2 > 0) {
+    // int8 factors: each column's factor times (max_u / 127) (max_v / 127),
+    // and the realized rounding error of the class, sum over its columns of
+    // lam (rho_u^2 + rho_v^2) / (12 127^2) with rho^2 = max^2 rows / s^2
+    std::vector<uint32_t> hm(2 * T2);
+    H3D_HIP(hipMemcpyAsync(hm.data(), d_mx8, sizeof(uint32_t) * 2 * T2, hipMemcpyDeviceToHost, st));
+    H3D_HIP(hipStreamSynchronize(st));
+    auto mxf = [&](int64_t q) {
+      float f;
+      std::memcpy(&f, &hm[q], 4);
+      return (double)f;
+    };
+    const int64_t tb2 = mx.cls[2].tbase;
+    for (int64_t q = 0; q < T2; ++q) {
+      scale[tb2 + q] *= (mxf(q) / 127.0) * (mxf(T2 + q) / 127.0);
+      scale32[2][q] = (float)scale[tb2 + q];
+    }
+    std::vector<double> ce(kHostChunks, 0.0);
+    host_chunks(nb, [&](int c, int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        if (raw[i] || rk[i][2] == 0) continue;
+        const int k8 = rk[i][0] + rk[i][3] + rk[i][1];
+        const double su2 = pd[i].su * pd[i].su, sv2 = pd[i].sv * pd[i].sv;
+        for (int k = 0; k < rk[i][2]; ++k) {
+          const int64_t q = pd[i].t8 + k;
+          const double l = std::max(lam_all[lam_off[i] + k8 + k], lam_floor[i]);
+          const double mu = mxf(q), mv = mxf(T2 + q);
+          ce[c] += l * (mu * mu * cd[i].n / su2 + mv * mv * cd[i].m / sv2) / (12.0 * 127.0 * 127.0);
+        }
+      }
+    });
+    mx.err2_fp8 = 0.0;
+    for (double v : ce) mx.err2_fp8 += v;
+  }
   {
     mx.d_scale = upload(scale, st);
     for (int c = 1; c < kClasses; ++c)
@@ -1742,6 +1914,7 @@ void build_mixed(Engine& e, double tol) {
     H3D_HIP(hipStreamSynchronize(st));
   }
   mx.built_formats = mx.formats;
+  mx.int8 = int8;
   mx.ready = true;
   mx.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -230,7 +230,9 @@ inline int mix_item_rows(int elem) {
 }
 
 // Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats).
-constexpr int kMixFp16 = 1, kMixFp8 = 2, kMixFp24 = 4;
+// kMixInt8: the 8-bit tier stores int8 with a scale per column side (its
+// largest entry) instead of e4m3
+constexpr int kMixFp16 = 1, kMixFp8 = 2, kMixFp24 = 4, kMixInt8 = 8;
 
 // Recompressed mixed-precision copy: SVD-truncated factors of every block with
 // columns in fp32, fp24, fp16 or fp8 chosen against a global Frobenius error
@@ -241,14 +243,15 @@ inline bool class_acc64(int c) { return c == 0 || c == 3; }
 struct MixedCopy {
   bool ready = false;
   double tol = 0;                    // requested relative Frobenius error
-  int formats = kMixFp16 | kMixFp8 | kMixFp24;  // formats the next build may use
+  int formats = kMixFp16 | kMixFp8 | kMixFp24 | kMixInt8;  // formats the next build may use
   int built_formats = 0;             // those the copy in place could use
-  // 0: fp32 columns, 1: fp16, 2: fp8 (e4m3), 3: fp24 (fp32 without its last 8 bits)
+  bool int8 = false;                 // class 2 holds int8 columns (built with kMixInt8)
+  // 0: fp32 columns, 1: fp16, 2: fp8 (e4m3, or int8), 3: fp24 (fp32 without its last 8 bits)
   FactorClass cls[kMixClasses];
   double* d_scale = nullptr;         // c_k per t entry (class 0, 1, 2, then 3)
   // statistics of the last build
   double fro2_far = 0, fro2_near = 0, err2_drop = 0, err2_fp16 = 0, err2_fp32 = 0, err2_fp8 = 0,
-         err2_fp24 = 0;
+         err2_fp24 = 0, err2_fp8_model = 0;  // int8: err2_fp8 realized, the model's plan
   double theta_drop = 0, ms = 0;
   bool single_jacobi = false;        // the build kept J between its passes (one Jacobi per block)
   int64_t aca_rank_sum = 0, dropped = 0, raw_blocks = 0, raw_rank = 0;

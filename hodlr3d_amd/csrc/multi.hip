@@ -107,6 +107,11 @@ __device__ inline void load4(const S* p, A (&o)[4]) {
     f24x4_bits(v.x, v.y, w, u);
     o[0] = (A)__uint_as_float(u[0]), o[1] = (A)__uint_as_float(u[1]);
     o[2] = (A)__uint_as_float(u[2]), o[3] = (A)__uint_as_float(u[3]);
+  } else if constexpr (std::is_same<S, i8s>::value) {  // int8 (column-scaled)
+    const uint32_t v = vload<NT, uint32_t>(p);
+    float f[4];
+    i8x4_to_f(v, f);
+    o[0] = (A)f[0], o[1] = (A)f[1], o[2] = (A)f[2], o[3] = (A)f[3];
   } else if constexpr (sizeof(S) == 1) {  // fp8 (e4m3)
     const uint32_t v = vload<NT, uint32_t>(p);
     float a, b, c, d;
@@ -1196,7 +1201,8 @@ void run_group_mixed(Engine& e, const double* b, double* y, int cnt) {
   run_class(std::pair<float, double>{}, 0, (const double*)x64, (double*)e.d_mtw[0]);
   run_class(std::pair<f24, double>{}, 3, (const double*)x64, (double*)e.d_mtw[3]);
   run_class(std::pair<_Float16, float>{}, 1, (const float*)x32, (float*)e.d_mtw[1]);
-  run_class(std::pair<f8e4m3, float>{}, 2, (const float*)x32, (float*)e.d_mtw[2]);
+  if (e.mix.int8) run_class(std::pair<i8s, float>{}, 2, (const float*)x32, (float*)e.d_mtw[2]);
+  else run_class(std::pair<f8e4m3, float>{}, 2, (const float*)x32, (float*)e.d_mtw[2]);
   if (nn && !overlap) launch_near(st);
   if (overlap) H3D_HIP(hipStreamWaitEvent(st, e.ev_join, 0));
   combine_w_kernel<W><<<(unsigned)((N + 63) / 64), 256, 0, st>>>(

@@ -80,6 +80,27 @@ struct f8e4m3 {
   uint8_t bits;
 };
 
+// A stored int8 entry of a column scaled by its largest entry (the scale is
+// carried by the column's factor).
+struct i8s {
+  int8_t v;
+};
+
+// Int8 entry q of a 32-bit word (byte q) as a float: the byte flipped to
+// offset binary (v + 128, an unsigned byte), converted, 128 subtracted (all
+// exact).  A sign-extended byte converted as a signed integer was compiled for
+// gfx950 into an unsigned conversion of the sign-extended byte (wrong for
+// v < 0), so the conversion stays unsigned.
+__device__ inline float i8_to_f(uint32_t w, int q) {
+  return (float)(((w ^ 0x80808080u) >> (8 * q)) & 0xffu) - 128.0f;
+}
+
+// Four int8 entries of a 32-bit word (byte 0 first) as floats.
+__device__ inline void i8x4_to_f(uint32_t w, float (&o)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = i8_to_f(w, q);
+}
+
 // A stored 24-bit float: the upper three bytes of an fp32 (sign, 8 exponent
 // bits, 15 mantissa bits), rounded to nearest; entries are packed, 3 bytes each.
 struct f24 {
@@ -131,6 +152,10 @@ struct Load<f8e4m3> {
   __device__ static inline float f(const f8e4m3* p) {
     return __builtin_amdgcn_cvt_f32_fp8((int)ntload<uint8_t>(p), 0);
   }
+};
+template <>
+struct Load<i8s> {
+  __device__ static inline float f(const i8s* p) { return i8_to_f(ntload<uint8_t>(p), 0); }
 };
 template <>
 struct Load<_Float16> {

@@ -316,10 +316,11 @@ class HODLR3D:
             out["phase2_fp16"] = {w[i]: ms[12 + i] for i in range(4) if ms[12 + i] >= 0}
         return out
 
-    def set_mixed_formats(self, formats=("fp16", "fp8", "fp24")):
+    def set_mixed_formats(self, formats=("fp16", "fp8", "fp24", "int8")):
         """Formats the next build of the mixed copy may store columns in besides
-        fp32: any of 'fp16', 'fp8' and 'fp24' (h3d_set_mixed_formats)."""
-        bits = {"fp16": MIXED_FP16, "fp8": MIXED_FP8, "fp24": MIXED_FP24}
+        fp32: any of 'fp16', 'fp8', 'fp24' and 'int8' (the 8-bit tier in int8
+        with a scale per column side instead of e4m3) (h3d_set_mixed_formats)."""
+        bits = {"fp16": MIXED_FP16, "fp8": MIXED_FP8, "fp24": MIXED_FP24, "int8": MIXED_INT8}
         mask = 0
         for f in formats:
             if f not in bits:
@@ -568,6 +569,7 @@ PERSISTENT = 0x100  # h3d_batch_set_tuning flag: the persistent (work-queue) lau
 MIXED_FP16 = 1      # h3d_set_mixed_formats bits
 MIXED_FP8 = 2
 MIXED_FP24 = 4
+MIXED_INT8 = 8
 
 
 class Batch:

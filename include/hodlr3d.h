@@ -125,12 +125,18 @@ typedef struct h3d_mixed_stats {
   double err_fp24;          /* part of predicted_rel_err from fp24 rounding */
   int64_t rank_fp24;        /* columns in fp24 (an fp32 without its last 8 bits) */
   int64_t u_elems_fp24, v_elems_fp24;
+  double err_fp8_model;     /* with H3D_MIXED_INT8: the selection's model of err_fp8 (err_fp8
+                               is then the realized value, from each column's largest entry) */
 } h3d_mixed_stats;
 
 /* Storage formats of the recompressed copy besides fp32 (h3d_set_mixed_formats). */
 #define H3D_MIXED_FP16 1
 #define H3D_MIXED_FP8 2
 #define H3D_MIXED_FP24 4
+/* The 8-bit tier in int8 with a scale per column side (its largest entry)
+ * instead of e4m3: an error of rho / (127 sqrt 12) per side, rho = max / rms
+ * of the column, modelled from the box size when the tiers are chosen. */
+#define H3D_MIXED_INT8 8
 
 typedef struct h3d_solve_info {
   int outer_iterations;    /* refinement steps */
@@ -198,7 +204,8 @@ int h3d_release_precision(h3d_handle* h, int prec);
 int h3d_compress(h3d_handle* h, double tol, h3d_mixed_stats* stats);
 /* Formats the next build of the recompressed copy may store columns in besides
  * fp32 (no reference counterpart): any of H3D_MIXED_FP16, H3D_MIXED_FP8 and
- * H3D_MIXED_FP24 (all three by default), or 0 (fp32 columns only).  Each
+ * H3D_MIXED_FP24, or 0 (fp32 columns only), plus H3D_MIXED_INT8 (the 8-bit
+ * tier in int8; it enables that tier); all four by default.  Each
  * format is a tier of the budget's choice: a column of the least error per
  * stored element is dropped, the next ones stored in fp8 (e4m3, 1 byte), fp16
  * (2 bytes), fp24 (3 bytes: fp32 rounded to 16 significant bits) or fp32, each
